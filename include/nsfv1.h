@@ -1,0 +1,172 @@
+/*
+ * nsfv1.h -- C ABI of the MI355X-native FV1 incompressible Navier-Stokes
+ * assembly (defect + Jacobian), the drop-in for the element loop of UG4's
+ * NavierStokesFV1.
+ *
+ * The reference boundary is a per-element callback set registered per
+ * reference-object id (incompressible/fv1/navier_stokes_fv1.cpp:1553-1579:
+ * prep_elem_loop, prep_elem, add_jac_A_elem, add_jac_M_elem, add_def_A_elem,
+ * add_def_M_elem, add_rhs_elem) driven by ugcore's element loop.  A
+ * per-element callback cannot feed a GPU, so this boundary is the whole-mesh
+ * assembly that the ugcore loop performs with those callbacks: one call
+ * assembles every element's contributions into a global CSR Jacobian and
+ * defect.  The configuration calls mirror the Lua/registry surface
+ * (incompressible/fv1/register_fv1.cpp:167-184, register_navier_stokes.cpp:
+ * 104-126, incompressible/incompressible_navier_stokes_plugin.cpp:244-266)
+ * with the same string keys, defaults and error cases.
+ *
+ * Conventions
+ *   - DoF of (node v, function c) = v*(dim+1) + c; c = 0..dim-1 velocity,
+ *     c = dim pressure (the _P_ = dim of navier_stokes_fv1.h:614).
+ *   - CSR pattern = node adjacency closure (x) (dim+1)^2, columns sorted.
+ *   - Array arguments of set_mesh are HOST pointers (copied); solution,
+ *     Jacobian and defect arguments of assemble are DEVICE pointers and the
+ *     call is asynchronous on the given hipStream_t (NULL = default stream).
+ *   - Every call returns NSFV1_OK (0) or a negative NSFV1_ERR_* code; the
+ *     message is available from nsfv1_last_error().
+ *   - Handles are independent; one handle must not be used from two host
+ *     threads at once.
+ */
+#ifndef NSFV1_H
+#define NSFV1_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nsfv1_handle nsfv1_handle;
+
+/* element types (ugcore ReferenceObjectID subset registered by
+   navier_stokes_fv1.cpp:1515-1551 for the 2D and 3D domains) */
+enum {
+    NSFV1_TRIANGLE = 0,
+    NSFV1_QUADRILATERAL = 1,
+    NSFV1_TETRAHEDRON = 2,
+    NSFV1_HEXAHEDRON = 3
+};
+
+/* assembled parts (bit mask) */
+enum { NSFV1_PART_A = 1, NSFV1_PART_M = 2, NSFV1_PART_RHS = 4 };
+
+/* Jacobian value precision */
+enum { NSFV1_F32 = 32, NSFV1_F64 = 64 };
+
+/* status codes; each mirrors a UG_THROW of the reference path */
+#define NSFV1_OK                   0
+#define NSFV1_ERR_ARGUMENT        -1  /* invalid argument / NULL handle */
+#define NSFV1_ERR_FUNCTION_COUNT  -2  /* navier_stokes_fv1.cpp:66-68 */
+#define NSFV1_ERR_TRIAL_SPACE     -3  /* navier_stokes_fv1.cpp:96-103 */
+#define NSFV1_ERR_NO_STABILIZATION -4 /* navier_stokes_fv1.cpp:147-148 */
+#define NSFV1_ERR_NO_UPWIND       -5  /* navier_stokes_fv1.cpp:156-157 */
+#define NSFV1_ERR_NO_VISCOSITY    -6  /* navier_stokes_fv1.cpp:169-171 */
+#define NSFV1_ERR_HANGING_NODES   -7  /* navier_stokes_fv1.cpp:1528, :92-93 */
+#define NSFV1_ERR_TIME_SERIES     -8  /* navier_stokes_fv1.cpp:272-274, 621-623 */
+#define NSFV1_ERR_CUT_SIDE        -9  /* upwind.cpp:354, 422, 550 */
+#define NSFV1_ERR_UNKNOWN_NAME   -10  /* upwind_interface.cpp:60-61, stabilization.cpp:55-56, 97-99 */
+#define NSFV1_ERR_NOT_IMPLEMENTED -11 /* a reference option this build does not provide */
+#define NSFV1_ERR_NO_MESH        -12  /* assemble before set_mesh */
+#define NSFV1_ERR_DEVICE         -13  /* HIP runtime error */
+#define NSFV1_ERR_DEGENERATE     -14  /* zero Jacobian determinant in an element */
+
+/* Thread-local message of the last failing call. */
+const char* nsfv1_last_error(void);
+
+/* NavierStokesFV1(functions, subsets): replaces the constructors at
+   incompressible/fv1/navier_stokes_fv1.cpp:45-59 and init() :62-87.
+   `functions` is a comma separated list; it must name exactly dim+1
+   functions.  Defaults: density 1.0, exact-Jacobian factor 0, no Stokes,
+   Laplace or Peclet blend, no stabilisation, no upwind, no viscosity. */
+int nsfv1_create(int dim, const char* functions, const char* subsets, nsfv1_handle** out);
+int nsfv1_destroy(nsfv1_handle* h);
+
+/* prepare_setting: navier_stokes_fv1.cpp:89-107.  lfe_type 0 = Lagrange;
+   each of the dim+1 spaces must be Lagrange of order 1. */
+int nsfv1_prepare_setting(nsfv1_handle* h, const int* lfe_type, const int* order, int n, int non_regular_grid);
+
+/* disc_type(): navier_stokes_fv1.h:241 -- writes "fv1". */
+int nsfv1_disc_type(nsfv1_handle* h, char* buf, size_t n);
+
+/* NavierStokesBase setters: navier_stokes_base.h:170-196,
+   navier_stokes_base.cpp:72-77 (constant user data only). */
+int nsfv1_set_kinematic_viscosity(nsfv1_handle* h, double nu);
+int nsfv1_set_source(nsfv1_handle* h, const double* f, int n);
+int nsfv1_set_exact_jacobian(nsfv1_handle* h, double factor);
+
+/* IncompressibleNavierStokesBase setters:
+   incompressible_navier_stokes_base.h:170-227 */
+int nsfv1_set_density(nsfv1_handle* h, double rho);
+int nsfv1_set_stokes(nsfv1_handle* h, int on);
+int nsfv1_set_laplace(nsfv1_handle* h, int on);
+int nsfv1_set_peclet_blend(nsfv1_handle* h, int on);
+int nsfv1_set_grad_div(nsfv1_handle* h, double factor);
+
+/* NavierStokesFV1::set_upwind(name): navier_stokes_fv1.h:213-215 with the
+   factory upwind_interface.cpp:43-62 (keys trimmed, case-insensitive: no,
+   full, skewed, linearprofileskewed|lps; positive|pos and regular|reg are
+   recognised but not provided by this build). */
+int nsfv1_set_upwind(nsfv1_handle* h, const char* name);
+
+/* set_stabilization(name[, diffLength]): navier_stokes_fv1.h:189-200 with
+   the factories stabilization.cpp:46-57 (fields, flow) and :86-100 (raw,
+   fivepoint, cor).  diff_length may be NULL (keeps RAW, stabilization.h:321-325). */
+int nsfv1_set_stabilization(nsfv1_handle* h, const char* name, const char* diff_length);
+
+/* set_stabilization(NavierStokesFV1WithoutStabilization): the object form
+   of register_fv1.cpp:172 with the class of stabilization.h:585-647. */
+int nsfv1_set_stabilization_without(nsfv1_handle* h);
+
+/* INavierStokesFV1Stabilization::set_upwind(name) on the current
+   stabilisation object: stabilization.h:86-90 (object-sharing semantics of
+   navier_stokes_fv1.h:190-215 otherwise apply). */
+int nsfv1_set_stabilization_upwind(nsfv1_handle* h, const char* name);
+
+/* set_pac_upwind: navier_stokes_fv1.h:217-226 */
+int nsfv1_set_pac_upwind(nsfv1_handle* h, int on);
+
+/* Mesh: node coordinates [nv][dim] and connectivity [ne][corners] with UG4
+   corner order; one element type per handle.  Host pointers.  Builds the
+   device copies, the CSR pattern and the node->element incidence used by the
+   row-owner gather. */
+int nsfv1_set_mesh(nsfv1_handle* h, int elem_type, const double* xyz, int64_t nv,
+                   const int32_t* conn, int64_t ne);
+
+int nsfv1_num_dofs(nsfv1_handle* h, int64_t* ndof);
+int nsfv1_nnz(nsfv1_handle* h, int64_t* nnz);
+
+/* Copies the CSR pattern to HOST arrays rowptr[ndof+1], colidx[nnz]. */
+int nsfv1_get_pattern(nsfv1_handle* h, int64_t* rowptr, int32_t* colidx);
+
+/* Device pointers of the pattern owned by the handle. */
+int nsfv1_pattern_device(nsfv1_handle* h, const int64_t** rowptr, const int32_t** colidx);
+
+/* Whole-mesh assembly; replaces ugcore's element loop over
+   add_jac_A_elem/add_def_A_elem (navier_stokes_fv1.cpp:250-778),
+   add_jac_M_elem/add_def_M_elem (:781-838) and add_rhs_elem (:841-869):
+     J = s_a*J_A + s_m*J_M,   d = s_a*d_A + s_m*d_M + s_f*f
+   over the parts selected in `parts`.  u, u_old: device [nv][dim+1] FP64;
+   u_old != NULL selects the time-dependent stabilisation with time step dt
+   (the two-point local time series of navier_stokes_fv1.cpp:268-280).
+   jvals: device [nnz] of jprec (NSFV1_F32 or NSFV1_F64), or NULL;
+   defect: device [ndof] FP64, or NULL.  Both are overwritten.
+   stream: hipStream_t or NULL. */
+int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double dt,
+                   double s_a, double s_m, double s_f, int parts,
+                   void* jvals, int jprec, double* defect, void* stream);
+
+/* Synchronises `stream` and reports (then clears) errors raised inside the
+   kernels since the last call: NSFV1_ERR_CUT_SIDE when a skewed/LPS upwind
+   ray found no side (upwind.cpp:354, 422, 550), NSFV1_ERR_DEGENERATE for an
+   element with a zero Jacobian determinant. */
+int nsfv1_sync_errors(nsfv1_handle* h, void* stream);
+
+/* Launch shape of the assembly (threads per workgroup, LDS budget in bytes);
+   0 keeps the current value.  Used by the autotuner. */
+int nsfv1_set_launch(nsfv1_handle* h, int threads, int lds_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,253 @@
+"""ctypes wrapper of the CPU oracle (``oracle/fv1_oracle.c``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and ``bench.py``'s ``cpu_baseline`` leg as the checker / CPU baseline.  The
+product package ``plugin_navierstokes_amd`` never imports it.
+
+The C library restates the reference's FV1 assembly
+(``incompressible/fv1/navier_stokes_fv1.cpp:250-892``, ``upwind.cpp:52-575``,
+``incompressible/fv1/stabilization.cpp:102-242,805-850``,
+``incompressible/fv1/diffusion_length.h:48-199``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "lib", "libfv1oracle.so")
+
+TRI, QUAD, TET, HEX = 0, 1, 2, 3
+UPW = {"no": 0, "full": 1, "skewed": 2, "lps": 3, "linearprofileskewed": 3}
+STAB_FIELDS, STAB_WITHOUT = 0, 1
+DIFFLEN = {"raw": 0, "fivepoint": 1, "cor": 2}
+NSH = {TRI: 3, QUAD: 4, TET: 4, HEX: 8}
+NIP = {TRI: 3, QUAD: 4, TET: 6, HEX: 12}
+DIM = {TRI: 2, QUAD: 2, TET: 3, HEX: 3}
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc)."""
+    src = os.path.join(_HERE, "fv1_oracle.c")
+    if force or not os.path.exists(_LIB_PATH) or os.path.getmtime(_LIB_PATH) < os.path.getmtime(src):
+        subprocess.check_call(["make", "-s", "-C", _HERE, "-B" if force else "lib/libfv1oracle.so"])
+    return _LIB_PATH
+
+
+class _Params(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("conv_upwind", C.c_int), ("stab_type", C.c_int),
+        ("stab_upwind", C.c_int), ("pac", C.c_int), ("difflen", C.c_int),
+        ("stokes", C.c_int), ("laplace", C.c_int), ("peclet", C.c_int),
+        ("exact", C.c_double), ("nu", C.c_double), ("rho", C.c_double),
+        ("has_source", C.c_int), ("f", C.c_double * 3),
+    ]
+
+
+class _Geo(C.Structure):
+    _fields_ = [
+        ("et", C.c_int), ("dim", C.c_int), ("nsh", C.c_int), ("nip", C.c_int),
+        ("co", (C.c_double * 3) * 8), ("frm", C.c_int * 12), ("to", C.c_int * 12),
+        ("ipl", (C.c_double * 3) * 12), ("ipg", (C.c_double * 3) * 12), ("n", (C.c_double * 3) * 12),
+        ("phi", (C.c_double * 8) * 12), ("g", ((C.c_double * 3) * 8) * 12),
+        ("c0", (C.c_double * 3) * 12), ("c2", (C.c_double * 3) * 12), ("vol", C.c_double * 8),
+    ]
+
+
+@dataclass
+class Params:
+    """Oracle configuration, the same switches as the reference disc object."""
+    dim: int = 2
+    upwind: str | None = "full"          # convective upwind (m_spConvUpwind)
+    stab: str | None = "fields"          # "fields" | "without" | None
+    stab_upwind: str | None = None       # defaults to ``upwind`` (shared object)
+    pac: bool = False
+    difflen: str = "raw"
+    stokes: bool = False
+    laplace: bool = False
+    peclet: bool = False
+    exact: float = 0.0
+    nu: float = 0.01
+    rho: float = 1.0
+    source: tuple | None = None
+
+    def c(self) -> _Params:
+        p = _Params()
+        p.dim = self.dim
+        p.conv_upwind = -1 if (self.upwind is None or self.pac) else UPW[self.upwind]
+        p.stab_type = {None: -1, "fields": STAB_FIELDS, "without": STAB_WITHOUT}[self.stab]
+        su = self.stab_upwind if self.stab_upwind is not None else self.upwind
+        p.stab_upwind = -1 if su is None else UPW[su]
+        p.pac = int(self.pac)
+        p.difflen = DIFFLEN[self.difflen]
+        p.stokes, p.laplace, p.peclet = int(self.stokes), int(self.laplace), int(self.peclet)
+        p.exact, p.nu, p.rho = float(self.exact), float(self.nu), float(self.rho)
+        p.has_source = int(self.source is not None)
+        if self.source is not None:
+            for i, v in enumerate(self.source):
+                p.f[i] = float(v)
+        return p
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(_LIB_PATH)
+        dp = C.POINTER(C.c_double)
+        ip32 = C.POINTER(C.c_int32)
+        ip64 = C.POINTER(C.c_int64)
+        L.or_geometry.argtypes = [C.c_int, dp, C.POINTER(_Geo)]
+        L.or_side_ray.argtypes = [C.c_int, dp, dp, dp, C.POINTER(C.c_int), dp, dp]
+        L.or_upwind.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp]
+        L.or_elem_A.argtypes = [C.POINTER(_Params), C.c_int, dp, dp, dp, C.c_double, dp, dp]
+        L.or_elem_M.argtypes = [C.POINTER(_Params), C.c_int, dp, dp, dp, dp]
+        L.or_elem_rhs.argtypes = [C.POINTER(_Params), C.c_int, dp, dp]
+        L.or_assemble.argtypes = [C.POINTER(_Params), C.c_int, dp, C.c_int64, ip32, C.c_int64, C.c_int64,
+                                  dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                  ip64, ip32, dp, dp]
+        L.or_assemble_abs.argtypes = [C.POINTER(_Params), C.c_int, dp, C.c_int64, ip32, C.c_int64,
+                                      dp, dp, C.c_double, C.c_double, C.c_double, C.c_int,
+                                      ip64, ip32, dp, dp]
+        L.or_pattern.argtypes = [C.c_int, C.c_int64, ip32, C.c_int64, ip64, ip32, ip64]
+        L.or_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"oracle error {code}: {msg}")
+        self.code = code
+
+
+def _chk(rc: int):
+    if rc != 0:
+        raise OracleError(rc, lib().or_last_error().decode())
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def geometry(et: int, corners) -> dict:
+    co = _f64(corners)
+    g = _Geo()
+    _chk(lib().or_geometry(et, _dp(co), C.byref(g)))
+    dim, nsh, nip = g.dim, g.nsh, g.nip
+    a = lambda x: np.ctypeslib.as_array(x)
+    return {
+        "from": a(g.frm)[:nip].copy(), "to": a(g.to)[:nip].copy(),
+        "ipl": a(g.ipl)[:nip, :dim].copy(), "ipg": a(g.ipg)[:nip, :dim].copy(),
+        "n": a(g.n)[:nip, :dim].copy(), "phi": a(g.phi)[:nip, :nsh].copy(),
+        "grad": a(g.g)[:nip, :nsh, :dim].copy(), "c0": a(g.c0)[:nip, :dim].copy(),
+        "c2": a(g.c2)[:nip, :dim].copy(), "vol": a(g.vol)[:nsh].copy(),
+    }
+
+
+def side_ray(et: int, corners, frm, direction):
+    co, f, d = _f64(corners), _f64(frm), _f64(direction)
+    side = C.c_int(-1)
+    gi = np.zeros(3)
+    li = np.zeros(3)
+    _chk(lib().or_side_ray(et, _dp(co), _dp(f), _dp(d), C.byref(side), _dp(gi), _dp(li)))
+    return side.value, gi[: DIM[et]], li[: DIM[et]]
+
+
+def upwind(kind: str, et: int, corners, ipvel):
+    co, v = _f64(corners), _f64(ipvel)
+    U = np.zeros((12, 8))
+    L = np.zeros(12)
+    _chk(lib().or_upwind(UPW[kind], et, _dp(co), _dp(v), _dp(U), _dp(L)))
+    return U[: NIP[et], : NSH[et]], L[: NIP[et]]
+
+
+def elem_A(p: Params, et: int, corners, uloc, uold=None, dt=0.0):
+    nl = NSH[et] * (DIM[et] + 1)
+    J = np.zeros((nl, nl))
+    d = np.zeros(nl)
+    pc = p.c()
+    _chk(lib().or_elem_A(C.byref(pc), et, _dp(_f64(corners)), _dp(_f64(uloc)), _dp(_f64(uold)),
+                         float(dt), _dp(J), _dp(d)))
+    return J, d
+
+
+def elem_M(p: Params, et: int, corners, uloc):
+    nl = NSH[et] * (DIM[et] + 1)
+    J = np.zeros((nl, nl))
+    d = np.zeros(nl)
+    pc = p.c()
+    _chk(lib().or_elem_M(C.byref(pc), et, _dp(_f64(corners)), _dp(_f64(uloc)), _dp(J), _dp(d)))
+    return J, d
+
+
+def elem_rhs(p: Params, et: int, corners):
+    nl = NSH[et] * (DIM[et] + 1)
+    f = np.zeros(nl)
+    pc = p.c()
+    _chk(lib().or_elem_rhs(C.byref(pc), et, _dp(_f64(corners)), _dp(f)))
+    return f
+
+
+def pattern(et: int, nv: int, conn):
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    ne = conn.shape[0]
+    nc = DIM[et] + 1
+    rowptr = np.zeros(nv * nc + 1, dtype=np.int64)
+    nnz = C.c_int64(0)
+    L = lib()
+    ip32 = C.POINTER(C.c_int32)
+    ip64 = C.POINTER(C.c_int64)
+    _chk(L.or_pattern(et, nv, conn.ctypes.data_as(ip32), ne, rowptr.ctypes.data_as(ip64), None, C.byref(nnz)))
+    colidx = np.zeros(nnz.value, dtype=np.int32)
+    _chk(L.or_pattern(et, nv, conn.ctypes.data_as(ip32), ne, rowptr.ctypes.data_as(ip64),
+                      colidx.ctypes.data_as(ip32), C.byref(nnz)))
+    return rowptr, colidx
+
+
+def assemble(p: Params, et: int, xyz, conn, u, uold=None, dt=0.0, s_a=1.0, s_m=0.0, s_f=0.0,
+             parts=1, rowptr=None, colidx=None, e0=0, e1=None, want_J=True, want_d=True):
+    xyz, u, uold = _f64(xyz), _f64(u), _f64(uold)
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    nv = xyz.shape[0]
+    if rowptr is None:
+        rowptr, colidx = pattern(et, nv, conn)
+    ndof = nv * (DIM[et] + 1)
+    J = np.zeros(int(rowptr[-1])) if want_J else None
+    d = np.zeros(ndof) if want_d else None
+    e1 = conn.shape[0] if e1 is None else e1
+    pc = p.c()
+    _chk(lib().or_assemble(C.byref(pc), et, _dp(xyz), nv, conn.ctypes.data_as(C.POINTER(C.c_int32)), e0, e1,
+                           _dp(u), _dp(uold), float(dt), float(s_a), float(s_m), float(s_f), int(parts),
+                           rowptr.ctypes.data_as(C.POINTER(C.c_int64)),
+                           colidx.ctypes.data_as(C.POINTER(C.c_int32)), _dp(J), _dp(d)))
+    return J, d
+
+
+def assemble_abs(p: Params, et: int, xyz, conn, u, uold=None, dt=0.0, s_a=1.0, s_m=0.0, parts=1,
+                 rowptr=None, colidx=None):
+    """Sum of |element contributions| per entry: the scale for mixed tolerances."""
+    xyz, u, uold = _f64(xyz), _f64(u), _f64(uold)
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    nv = xyz.shape[0]
+    if rowptr is None:
+        rowptr, colidx = pattern(et, nv, conn)
+    J = np.zeros(int(rowptr[-1]))
+    d = np.zeros(nv * (DIM[et] + 1))
+    pc = p.c()
+    _chk(lib().or_assemble_abs(C.byref(pc), et, _dp(xyz), nv, conn.ctypes.data_as(C.POINTER(C.c_int32)),
+                               conn.shape[0], _dp(u), _dp(uold), float(dt), float(s_a), float(s_m), int(parts),
+                               rowptr.ctypes.data_as(C.POINTER(C.c_int64)),
+                               colidx.ctypes.data_as(C.POINTER(C.c_int32)), _dp(J), _dp(d)))
+    return J, d

@@ -1,0 +1,1132 @@
+/*
+ * fv1_oracle.c -- CPU restatement of the FV1 incompressible Navier-Stokes
+ * element assembly of UG4/plugin_NavierStokes, in plain C, FP64.
+ *
+ * TEST INFRASTRUCTURE ONLY (the oracle).  Loaded by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg; never by the
+ * product library.
+ *
+ * Each function cites the reference file:line it restates.  The geometry
+ * (FV1Geometry), the ray/side intersection and the CSR scatter are ugcore
+ * code that is not present; they are restated here from the box-method
+ * construction:
+ *   - SCVF i <-> reference edge i, from/to = the edge's corners;
+ *   - 2D SCVF = segment [edge midpoint, element centre];
+ *     3D SCVF = quad [edge midpoint, centre of face A, element centre,
+ *     centre of face B], A and B the two faces containing the edge;
+ *   - ip = average of the SCVF corners (local and global);
+ *   - normal: 2D the rotated segment, 3D half the cross product of the
+ *     diagonals; its sign makes n.(x_to - x_from) > 0;
+ *   - shapes and gradients of the P1/Q1 Lagrange space at the local ip,
+ *     gradients mapped with J^-T;
+ *   - SCV of corner k: bounded by the corner, the midpoints of its edges, the
+ *     centres of its faces and the element centre; 2D area by the diagonal
+ *     cross product, 3D volume = exact volume of the trilinear hexahedron
+ *     through those eight points (2x2x2 Gauss on det J).
+ */
+#include "fv1_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static char g_err[256];
+static void set_err(const char* m) { snprintf(g_err, sizeof g_err, "%s", m); }
+const char* or_last_error(void) { return g_err; }
+
+/* ------------------------------------------------------------------ */
+/* reference elements (UG4 corner / edge / side numbering)            */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int dim, nsh, nip, nside;
+    double lc[8][3];
+    int edge[12][2];
+    int side_n[6];
+    int side[6][4];
+} refelem;
+
+static const refelem REF[4] = {
+    /* triangle */
+    {2, 3, 3, 3,
+     {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}},
+     {{0, 1}, {1, 2}, {2, 0}},
+     {2, 2, 2},
+     {{0, 1}, {1, 2}, {2, 0}}},
+    /* quadrilateral */
+    {2, 4, 4, 4,
+     {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}},
+     {{0, 1}, {1, 2}, {2, 3}, {3, 0}},
+     {2, 2, 2, 2},
+     {{0, 1}, {1, 2}, {2, 3}, {3, 0}}},
+    /* tetrahedron */
+    {3, 4, 6, 4,
+     {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}},
+     {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}},
+     {3, 3, 3, 3},
+     {{0, 2, 1}, {1, 2, 3}, {0, 3, 2}, {0, 1, 3}}},
+    /* hexahedron */
+    {3, 8, 12, 6,
+     {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}},
+     {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 4}, {1, 5}, {2, 6}, {3, 7}, {4, 5}, {5, 6}, {6, 7}, {7, 4}},
+     {4, 4, 4, 4, 4, 4},
+     {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {0, 4, 7, 3}, {4, 5, 6, 7}}},
+};
+
+int or_nsh(int et) { return (et >= 0 && et < 4) ? REF[et].nsh : -1; }
+int or_nip(int et) { return (et >= 0 && et < 4) ? REF[et].nip : -1; }
+int or_dim(int et) { return (et >= 0 && et < 4) ? REF[et].dim : -1; }
+
+static int side_has(const refelem* R, int s, int c)
+{
+    for (int i = 0; i < R->side_n[s]; ++i)
+        if (R->side[s][i] == c) return 1;
+    return 0;
+}
+
+/* the two sides containing edge e, in side order (3D) */
+static void edge_faces(const refelem* R, int e, int* A, int* B)
+{
+    int k = 0;
+    *A = *B = -1;
+    for (int s = 0; s < R->nside && k < 2; ++s)
+        if (side_has(R, s, R->edge[e][0]) && side_has(R, s, R->edge[e][1])) {
+            if (k == 0) *A = s; else *B = s;
+            ++k;
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* P1 / Q1 Lagrange shapes on the reference elements                  */
+/* ------------------------------------------------------------------ */
+static void shapes(int et, const double* x, double* phi, double (*gl)[3])
+{
+    switch (et) {
+    case OR_TRI:
+        phi[0] = 1.0 - x[0] - x[1]; phi[1] = x[0]; phi[2] = x[1];
+        gl[0][0] = -1; gl[0][1] = -1; gl[1][0] = 1; gl[1][1] = 0; gl[2][0] = 0; gl[2][1] = 1;
+        break;
+    case OR_QUAD: {
+        const double a = x[0], b = x[1];
+        phi[0] = (1 - a) * (1 - b); phi[1] = a * (1 - b); phi[2] = a * b; phi[3] = (1 - a) * b;
+        gl[0][0] = -(1 - b); gl[0][1] = -(1 - a);
+        gl[1][0] = (1 - b);  gl[1][1] = -a;
+        gl[2][0] = b;        gl[2][1] = a;
+        gl[3][0] = -b;       gl[3][1] = (1 - a);
+        break;
+    }
+    case OR_TET:
+        phi[0] = 1.0 - x[0] - x[1] - x[2]; phi[1] = x[0]; phi[2] = x[1]; phi[3] = x[2];
+        for (int k = 0; k < 4; ++k) for (int j = 0; j < 3; ++j) gl[k][j] = 0;
+        gl[0][0] = gl[0][1] = gl[0][2] = -1;
+        gl[1][0] = 1; gl[2][1] = 1; gl[3][2] = 1;
+        break;
+    case OR_HEX:
+        for (int k = 0; k < 8; ++k) {
+            const double* c = REF[OR_HEX].lc[k];
+            double f[3], df[3];
+            for (int j = 0; j < 3; ++j) {
+                f[j] = c[j] > 0.5 ? x[j] : 1.0 - x[j];
+                df[j] = c[j] > 0.5 ? 1.0 : -1.0;
+            }
+            phi[k] = f[0] * f[1] * f[2];
+            gl[k][0] = df[0] * f[1] * f[2];
+            gl[k][1] = f[0] * df[1] * f[2];
+            gl[k][2] = f[0] * f[1] * df[2];
+        }
+        break;
+    }
+}
+
+static double dot(int dim, const double* a, const double* b)
+{
+    double s = 0;
+    for (int i = 0; i < dim; ++i) s += a[i] * b[i];
+    return s;
+}
+static double dist2(int dim, const double* a, const double* b)
+{
+    double s = 0;
+    for (int i = 0; i < dim; ++i) s += (a[i] - b[i]) * (a[i] - b[i]);
+    return s;
+}
+static void cross(const double* a, const double* b, double* c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* volume of the trilinear hexahedron through p[8] (hex corner order) */
+static double trilinear_hex_volume(const double p[8][3])
+{
+    const double g0 = 0.5 - 0.5 / sqrt(3.0), g1 = 0.5 + 0.5 / sqrt(3.0);
+    const double gp[2] = {g0, g1};
+    double v = 0;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c) {
+                double x[3] = {gp[a], gp[b], gp[c]}, phi[8], gl[8][3], J[3][3] = {{0}};
+                shapes(OR_HEX, x, phi, gl);
+                for (int k = 0; k < 8; ++k)
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) J[i][j] += p[k][i] * gl[k][j];
+                double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1])
+                           - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0])
+                           + J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+                v += 0.125 * det;
+            }
+    return fabs(v);
+}
+
+/* ------------------------------------------------------------------ */
+/* FV1Geometry::update (ugcore, restated); used at                      */
+/* navier_stokes_fv1.cpp:214-216                                       */
+/* ------------------------------------------------------------------ */
+int or_geometry(int et, const double* corners, or_geo* G)
+{
+    if (et < 0 || et > 3) { set_err("unknown element type"); return -1; }
+    const refelem* R = &REF[et];
+    const int dim = R->dim, nsh = R->nsh, nip = R->nip;
+    memset(G, 0, sizeof *G);
+    G->et = et; G->dim = dim; G->nsh = nsh; G->nip = nip;
+    for (int k = 0; k < nsh; ++k)
+        for (int i = 0; i < dim; ++i) G->co[k][i] = corners[k * dim + i];
+
+    double c[3] = {0, 0, 0}, cl[3] = {0, 0, 0};
+    for (int k = 0; k < nsh; ++k)
+        for (int i = 0; i < dim; ++i) { c[i] += G->co[k][i]; cl[i] += R->lc[k][i]; }
+    for (int i = 0; i < dim; ++i) { c[i] /= nsh; cl[i] /= nsh; }
+
+    double me[12][3], mel[12][3];
+    for (int e = 0; e < nip; ++e)
+        for (int i = 0; i < dim; ++i) {
+            me[e][i] = 0.5 * (G->co[R->edge[e][0]][i] + G->co[R->edge[e][1]][i]);
+            mel[e][i] = 0.5 * (R->lc[R->edge[e][0]][i] + R->lc[R->edge[e][1]][i]);
+        }
+    double fc[6][3] = {{0}}, fcl[6][3] = {{0}};
+    if (dim == 3)
+        for (int s = 0; s < R->nside; ++s) {
+            for (int j = 0; j < R->side_n[s]; ++j)
+                for (int i = 0; i < 3; ++i) {
+                    fc[s][i] += G->co[R->side[s][j]][i];
+                    fcl[s][i] += R->lc[R->side[s][j]][i];
+                }
+            for (int i = 0; i < 3; ++i) { fc[s][i] /= R->side_n[s]; fcl[s][i] /= R->side_n[s]; }
+        }
+
+    for (int ip = 0; ip < nip; ++ip) {
+        const int f = R->edge[ip][0], t = R->edge[ip][1];
+        G->from[ip] = f; G->to[ip] = t;
+        double n[3] = {0, 0, 0};
+        if (dim == 2) {
+            for (int i = 0; i < 2; ++i) {
+                G->ipg[ip][i] = 0.5 * (me[ip][i] + c[i]);
+                G->ipl[ip][i] = 0.5 * (mel[ip][i] + cl[i]);
+                G->c0[ip][i] = me[ip][i];
+                G->c2[ip][i] = c[i];
+            }
+            const double d0 = c[0] - me[ip][0], d1 = c[1] - me[ip][1];
+            n[0] = d1; n[1] = -d0;
+        } else {
+            int A, B;
+            edge_faces(R, ip, &A, &B);
+            for (int i = 0; i < 3; ++i) {
+                G->ipg[ip][i] = 0.25 * (me[ip][i] + fc[A][i] + c[i] + fc[B][i]);
+                G->ipl[ip][i] = 0.25 * (mel[ip][i] + fcl[A][i] + cl[i] + fcl[B][i]);
+                G->c0[ip][i] = me[ip][i];
+                G->c2[ip][i] = c[i];
+            }
+            double d1[3], d2[3];
+            for (int i = 0; i < 3; ++i) { d1[i] = c[i] - me[ip][i]; d2[i] = fc[B][i] - fc[A][i]; }
+            cross(d1, d2, n);
+            for (int i = 0; i < 3; ++i) n[i] *= 0.5;
+        }
+        double e[3] = {0, 0, 0};
+        for (int i = 0; i < dim; ++i) e[i] = G->co[t][i] - G->co[f][i];
+        if (dot(dim, n, e) < 0)
+            for (int i = 0; i < dim; ++i) n[i] = -n[i];
+        for (int i = 0; i < dim; ++i) G->n[ip][i] = n[i];
+
+        /* shapes and global gradients at the local ip */
+        double gl[8][3] = {{0}};
+        shapes(et, G->ipl[ip], G->phi[ip], gl);
+        double J[3][3] = {{0}};
+        for (int k = 0; k < nsh; ++k)
+            for (int i = 0; i < dim; ++i)
+                for (int j = 0; j < dim; ++j) J[i][j] += G->co[k][i] * gl[k][j];
+        double Ji[3][3] = {{0}};
+        if (dim == 2) {
+            const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+            if (det == 0.0) { set_err("degenerate element"); return -2; }
+            Ji[0][0] = J[1][1] / det;  Ji[0][1] = -J[0][1] / det;
+            Ji[1][0] = -J[1][0] / det; Ji[1][1] = J[0][0] / det;
+        } else {
+            const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1])
+                             - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0])
+                             + J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+            if (det == 0.0) { set_err("degenerate element"); return -2; }
+            Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+            Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+            Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+            Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+            Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+            Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+            Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+            Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+            Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+        }
+        /* grad_k = J^-T gl_k */
+        for (int k = 0; k < nsh; ++k)
+            for (int i = 0; i < dim; ++i) {
+                double s = 0;
+                for (int j = 0; j < dim; ++j) s += Ji[j][i] * gl[k][j];
+                G->g[ip][k][i] = s;
+            }
+    }
+
+    /* SCV volumes */
+    for (int k = 0; k < nsh; ++k) {
+        int es[3], ne = 0;
+        for (int e = 0; e < nip; ++e)
+            if (R->edge[e][0] == k || R->edge[e][1] == k) es[ne++] = e;
+        if (dim == 2) {
+            /* quad [x_k, m_ea, c, m_eb] */
+            const double d1x = c[0] - G->co[k][0], d1y = c[1] - G->co[k][1];
+            const double d2x = me[es[1]][0] - me[es[0]][0], d2y = me[es[1]][1] - me[es[0]][1];
+            G->vol[k] = 0.5 * fabs(d1x * d2y - d1y * d2x);
+        } else {
+            int fs[3]; /* face of (e0,e1), (e0,e2), (e1,e2) */
+            const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+            for (int q = 0; q < 3; ++q) {
+                fs[q] = -1;
+                const int ea = es[pairs[q][0]], eb = es[pairs[q][1]];
+                for (int s = 0; s < R->nside; ++s)
+                    if (side_has(R, s, R->edge[ea][0]) && side_has(R, s, R->edge[ea][1]) &&
+                        side_has(R, s, R->edge[eb][0]) && side_has(R, s, R->edge[eb][1])) { fs[q] = s; break; }
+            }
+            double p[8][3];
+            for (int i = 0; i < 3; ++i) {
+                p[0][i] = G->co[k][i];
+                p[1][i] = me[es[0]][i];
+                p[2][i] = fc[fs[0]][i];
+                p[3][i] = me[es[1]][i];
+                p[4][i] = me[es[2]][i];
+                p[5][i] = fc[fs[1]][i];
+                p[6][i] = c[i];
+                p[7][i] = fc[fs[2]][i];
+            }
+            G->vol[k] = trilinear_hex_volume(p);
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* ElementSideRayIntersection (ugcore, restated), searching upwind:     */
+/* the first side in reference order whose intersection parameters lie  */
+/* in the side (tolerance 1e-10) at ray parameter t <= 0.               */
+/* called from upwind.cpp:351-353 and 547-549                          */
+/* ------------------------------------------------------------------ */
+#define OR_RAY_EPS 1e-10
+
+static int ray_tri(const double* p0, const double* p1, const double* p2,
+                   const double* from, const double* dir, double* b1o, double* b2o, double* to)
+{
+    double e1[3], e2[3], pv[3], tv[3], qv[3];
+    for (int i = 0; i < 3; ++i) { e1[i] = p1[i] - p0[i]; e2[i] = p2[i] - p0[i]; }
+    cross(dir, e2, pv);
+    const double det = dot(3, e1, pv);
+    if (det == 0.0) return 0;
+    const double inv = 1.0 / det;
+    for (int i = 0; i < 3; ++i) tv[i] = from[i] - p0[i];
+    const double b1 = dot(3, tv, pv) * inv;
+    cross(tv, e1, qv);
+    const double b2 = dot(3, dir, qv) * inv;
+    const double t = dot(3, e2, qv) * inv;
+    if (b1 < -OR_RAY_EPS || b2 < -OR_RAY_EPS || b1 + b2 > 1.0 + OR_RAY_EPS) return 0;
+    if (t > 0.0) return 0;
+    *b1o = b1; *b2o = b2; *to = t;
+    return 1;
+}
+
+int or_side_ray(int et, const double* corners, const double* from, const double* dir,
+                int* side, double* glob, double* loc)
+{
+    const refelem* R = &REF[et];
+    const int dim = R->dim;
+#define CO(k) (corners + (k) * dim)
+    if (dim == 2) {
+        for (int s = 0; s < R->nside; ++s) {
+            const double* p0 = CO(R->side[s][0]);
+            const double* p1 = CO(R->side[s][1]);
+            const double ex = p1[0] - p0[0], ey = p1[1] - p0[1];
+            const double rx = p0[0] - from[0], ry = p0[1] - from[1];
+            const double det = -dir[0] * ey + ex * dir[1];
+            if (det == 0.0) continue;
+            const double t = (-rx * ey + ex * ry) / det;
+            const double lam = (dir[0] * ry - dir[1] * rx) / det;
+            if (lam < -OR_RAY_EPS || lam > 1.0 + OR_RAY_EPS) continue;
+            if (t > 0.0) continue;
+            *side = s;
+            glob[0] = p0[0] + lam * ex;
+            glob[1] = p0[1] + lam * ey;
+            const double* l0 = R->lc[R->side[s][0]];
+            const double* l1 = R->lc[R->side[s][1]];
+            for (int i = 0; i < 2; ++i) loc[i] = (1.0 - lam) * l0[i] + lam * l1[i];
+            return 0;
+        }
+    } else {
+        for (int s = 0; s < R->nside; ++s) {
+            const int ntri = R->side_n[s] == 3 ? 1 : 2;
+            for (int tr = 0; tr < ntri; ++tr) {
+                const int a = R->side[s][0];
+                const int b = R->side[s][tr == 0 ? 1 : 2];
+                const int cc = R->side[s][tr == 0 ? 2 : 3];
+                double b1, b2, t;
+                if (!ray_tri(CO(a), CO(b), CO(cc), from, dir, &b1, &b2, &t)) continue;
+                *side = s;
+                for (int i = 0; i < 3; ++i) {
+                    glob[i] = CO(a)[i] + b1 * (CO(b)[i] - CO(a)[i]) + b2 * (CO(cc)[i] - CO(a)[i]);
+                    loc[i] = (1.0 - b1 - b2) * R->lc[a][i] + b1 * R->lc[b][i] + b2 * R->lc[cc][i];
+                }
+                return 0;
+            }
+        }
+    }
+#undef CO
+    set_err("Cannot find cut side.");
+    return -10;
+}
+
+/* ------------------------------------------------------------------ */
+/* upwind shapes: upwind.cpp:52-80 (no), 133-172 (full),                */
+/* 337-430 (skewed + GetNodeNextToCut), 505-575 (LPS)                   */
+/* ------------------------------------------------------------------ */
+static int upwind_geo(int type, const or_geo* G, const double (*vel)[3], double (*U)[8], double* convlen)
+{
+    const int dim = G->dim, nip = G->nip, nsh = G->nsh;
+    double cflat[24];
+    for (int k = 0; k < nsh; ++k)
+        for (int i = 0; i < dim; ++i) cflat[k * dim + i] = G->co[k][i];
+    for (int ip = 0; ip < nip; ++ip) {
+        switch (type) {
+        case OR_UPW_NO:
+            for (int sh = 0; sh < nsh; ++sh) U[ip][sh] = G->phi[ip][sh];
+            convlen[ip] = 1.0;
+            break;
+        case OR_UPW_FULL: {
+            for (int sh = 0; sh < nsh; ++sh) U[ip][sh] = 0.0;
+            const double flux = dot(dim, G->n[ip], vel[ip]);
+            if (flux > 0.0) {
+                U[ip][G->from[ip]] = 1.0;
+                convlen[ip] = sqrt(dist2(dim, G->ipg[ip], G->co[G->from[ip]]));
+            } else {
+                U[ip][G->to[ip]] = 1.0;
+                convlen[ip] = sqrt(dist2(dim, G->ipg[ip], G->co[G->to[ip]]));
+            }
+            break;
+        }
+        case OR_UPW_SKEWED:
+        case OR_UPW_LPS: {
+            for (int sh = 0; sh < nsh; ++sh) U[ip][sh] = 0.0;
+            if (sqrt(dot(dim, vel[ip], vel[ip])) < 1e-14) { convlen[ip] = 1.0; break; }
+            int side;
+            double gi[3] = {0, 0, 0}, li[3] = {0, 0, 0};
+            if (or_side_ray(G->et, cflat, G->ipg[ip], vel[ip], &side, gi, li)) return -10;
+            const refelem* R = &REF[G->et];
+            if (type == OR_UPW_SKEWED) {
+                double mn = DBL_MAX;
+                int coOut = 0;
+                for (int i = 0; i < R->side_n[side]; ++i) {
+                    const int co = R->side[side][i];
+                    const double dd = dist2(dim, gi, G->co[co]);
+                    if (dd < mn) { mn = dd; coOut = co; }
+                }
+                U[ip][coOut] = 1.0;
+                convlen[ip] = sqrt(dist2(dim, G->ipg[ip], G->co[coOut]));
+            } else {
+                double phi[8], gl[8][3];
+                shapes(G->et, li, phi, gl);
+                for (int j = 0; j < R->side_n[side]; ++j) {
+                    const int co = R->side[side][j];
+                    U[ip][co] = phi[co];
+                }
+                convlen[ip] = sqrt(dist2(dim, G->ipg[ip], gi));
+            }
+            break;
+        }
+        default:
+            set_err("upwind not set");
+            return -4;
+        }
+    }
+    return 0;
+}
+
+int or_upwind(int type, int et, const double* corners, const double* ipvel, double* U, double* convlen)
+{
+    or_geo G;
+    int rc = or_geometry(et, corners, &G);
+    if (rc) return rc;
+    double vel[12][3] = {{0}};
+    for (int ip = 0; ip < G.nip; ++ip)
+        for (int i = 0; i < G.dim; ++i) vel[ip][i] = ipvel[ip * G.dim + i];
+    double Ul[12][8];
+    rc = upwind_geo(type, &G, (const double (*)[3])vel, Ul, convlen);
+    if (rc) return rc;
+    for (int ip = 0; ip < G.nip; ++ip)
+        for (int k = 0; k < 8; ++k) U[ip * 8 + k] = k < G.nsh ? Ul[ip][k] : 0.0;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* diffusion length: diffusion_length.h:48-89 (FIVEPOINT),               */
+/* 91-133 (RAW), 135-199 (COR); selection stabilization.cpp:102-116     */
+/* ------------------------------------------------------------------ */
+static void diff_length(int method, const or_geo* G, double* L2inv)
+{
+    const int dim = G->dim, nip = G->nip;
+    if (method == OR_DL_COR) {
+        double minN = DBL_MAX, minD = DBL_MAX, avgN = 0.0;
+        for (int i = 0; i < nip; ++i) {
+            const double nn = dot(dim, G->n[i], G->n[i]);
+            if (nn < minN) minN = nn;
+            avgN += nn;
+            if (dim == 3) {
+                const double ds = dist2(3, G->c0[i], G->c2[i]);
+                if (ds < minD) minD = ds;
+            }
+        }
+        avgN /= nip;
+        for (int i = 0; i < nip; ++i) {
+            double a = 0.5 * (G->vol[G->from[i]] + G->vol[G->to[i]]);
+            a *= a;
+            if (dim == 2) L2inv[i] = 2.0 * minN / a + 8.0 / (3.0 * avgN);
+            else L2inv[i] = 2.0 * minN / a + 8.0 * minD / (3.0 * avgN);
+        }
+        return;
+    }
+    for (int i = 0; i < nip; ++i) {
+        const double nn = dot(dim, G->n[i], G->n[i]);
+        double a = 0.5 * (G->vol[G->from[i]] + G->vol[G->to[i]]);
+        a *= a;
+        if (method == OR_DL_RAW) {
+            if (dim == 2) L2inv[i] = 1. / (0.5 * a / nn + 3.0 * nn / 8.);
+            else L2inv[i] = 1. / (0.5 * a / nn + 3.0 * dist2(3, G->c0[i], G->c2[i]) / 8.);
+        } else {
+            if (dim == 2) L2inv[i] = 2.0 * nn / a + 8.0 / nn;
+            else L2inv[i] = 2.0 * nn / a + 8.0 * dist2(3, G->c0[i], G->c2[i]) / nn;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* stabilisation update: FIELDS diagonal case stabilization.cpp:122-242, */
+/* without stabilisation stabilization.cpp:805-850                       */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    double vel[12][3];
+    double Svel[12][3][8]; /* stab_shape_vel(ip, d, d, sh) */
+    double Sp[12][3][8];   /* stab_shape_p(ip, d, sh) */
+    double diag[12];       /* diagonal of the FIELDS ip system */
+} stabdata;
+
+static void u_at(const double* u, int nc, int d, int sh, double* out) { *out = u[sh * nc + d]; }
+#define UL(d, sh) (uloc[(sh) * nc + (d)])
+#define UO(d, sh) (uold[(sh) * nc + (d)])
+
+static int stab_update(const or_params* p, const or_geo* G, const double* uloc, const double* uold, double dt,
+                       const double (*stdvel)[3], const double (*Ust)[8], const double* convlen_st, stabdata* S)
+{
+    const int dim = G->dim, nip = G->nip, nsh = G->nsh, nc = dim + 1;
+    (void)u_at;
+    if (p->stab_type == OR_STAB_WITHOUT) {
+        for (int ip = 0; ip < nip; ++ip) {
+            for (int i = 0; i < dim; ++i)
+                for (int sh = 0; sh < nsh; ++sh) { S->Sp[ip][i][sh] = 0; S->Svel[ip][i][sh] = G->phi[ip][sh]; }
+            for (int i = 0; i < dim; ++i) S->vel[ip][i] = stdvel[ip][i];
+        }
+        return 0;
+    }
+    if (p->stab_type != OR_STAB_FIELDS) { set_err("Stabilization has not been set."); return -3; }
+    double L2inv[12];
+    diff_length(p->difflen, G, L2inv);
+    double vvisc[12], vconv[12];
+    for (int ip = 0; ip < nip; ++ip) vvisc[ip] = p->nu * L2inv[ip];
+    if (!p->stokes)
+        for (int ip = 0; ip < nip; ++ip)
+            vconv[ip] = sqrt(dot(dim, stdvel[ip], stdvel[ip])) / convlen_st[ip];
+    for (int ip = 0; ip < nip; ++ip) {
+        double diag = vvisc[ip];
+        if (uold) diag += 1. / dt;
+        if (!p->stokes) diag += vconv[ip];
+        S->diag[ip] = diag;
+        for (int d = 0; d < dim; ++d) {
+            double rhs = 0.0;
+            if (p->has_source) rhs = p->f[d];
+            if (uold) {
+                double o = 0.0;
+                for (int sh = 0; sh < nsh; ++sh) o += G->phi[ip][sh] * UO(d, sh);
+                rhs += o / dt;
+            }
+            for (int k = 0; k < nsh; ++k) {
+                double sumVel = vvisc[ip] * G->phi[ip][k];
+                if (!p->stokes) sumVel += vconv[ip] * Ust[ip][k];
+                rhs += sumVel * UL(d, k);
+                S->Svel[ip][d][k] = sumVel / diag;
+                const double sumP = -1.0 * G->g[ip][k][d] / p->rho;
+                rhs += sumP * UL(dim, k);
+                S->Sp[ip][d][k] = sumP / diag;
+            }
+            S->vel[ip][d] = rhs / diag;
+        }
+    }
+    return 0;
+}
+
+/* peclet_blend: navier_stokes_fv1.cpp:871-892 */
+static double peclet_blend(const or_params* p, const or_geo* G, int ip, const double* stdvel, double* upvel)
+{
+    const int dim = G->dim;
+    const double Pe = dot(dim, stdvel, G->n[ip]) / dot(dim, G->n[ip], G->n[ip])
+                    * sqrt(dist2(dim, G->co[G->to[ip]], G->co[G->from[ip]])) / p->nu;
+    const double Pe2 = Pe * Pe;
+    const double w = Pe2 / (5.0 + Pe2);
+    for (int i = 0; i < dim; ++i) upvel[i] = w * upvel[i] + (1.0 - w) * stdvel[i];
+    return w;
+}
+
+/* common prologue of add_jac_A_elem / add_def_A_elem:
+   navier_stokes_fv1.cpp:282-308 and 631-658 */
+typedef struct {
+    double stdvel[12][3];
+    double Uc[12][8], lc[12]; /* convective upwind (if used) */
+    stabdata S;
+} prolog;
+
+static int prologue(const or_params* p, const or_geo* G, const double* uloc, const double* uold, double dt, prolog* W)
+{
+    const int dim = G->dim, nip = G->nip, nsh = G->nsh, nc = dim + 1;
+    if (p->stab_type < 0) { set_err("Stabilization has not been set."); return -3; }
+    if (!p->stokes && p->conv_upwind < 0 && !p->pac) {
+        set_err("Upwinding for convective Term in Momentum eq. not set.");
+        return -4;
+    }
+    for (int ip = 0; ip < nip; ++ip) {
+        for (int d = 0; d < dim; ++d) W->stdvel[ip][d] = 0.0;
+        for (int sh = 0; sh < nsh; ++sh)
+            for (int d = 0; d < dim; ++d) W->stdvel[ip][d] += UL(d, sh) * G->phi[ip][sh];
+    }
+    /* stabilisation (computes its upwind first unless Stokes) */
+    double Ust[12][8] = {{0}}, lst[12];
+    for (int ip = 0; ip < 12; ++ip) lst[ip] = 1.0;
+    if (!p->stokes) {
+        if (p->stab_upwind < 0) { set_err("No upwind object"); return -4; }
+        int rc = upwind_geo(p->stab_upwind, G, (const double (*)[3])W->stdvel, Ust, lst);
+        if (rc) return rc;
+    }
+    int rc = stab_update(p, G, uloc, uold, dt, (const double (*)[3])W->stdvel, (const double (*)[8])Ust, lst, &W->S);
+    if (rc) return rc;
+    if (!p->stokes && !p->pac) {
+        if (p->conv_upwind == p->stab_upwind) {
+            memcpy(W->Uc, Ust, sizeof Ust);
+            memcpy(W->lc, lst, sizeof lst);
+        } else {
+            rc = upwind_geo(p->conv_upwind, G, (const double (*)[3])W->stdvel, W->Uc, W->lc);
+            if (rc) return rc;
+        }
+    }
+    return 0;
+}
+
+/* add_jac_A_elem: navier_stokes_fv1.cpp:250-595 */
+static int jac_A(const or_params* p, const or_geo* G, const double* uloc, const prolog* W, double* Jl)
+{
+    const int dim = G->dim, nip = G->nip, nsh = G->nsh, nc = dim + 1, P = dim, nl = nsh * nc;
+    const double nu = p->nu, rho = p->rho;
+#define JL(a, i, b, j) Jl[((i) * nc + (a)) * nl + (j) * nc + (b)]
+    for (int ip = 0; ip < nip; ++ip) {
+        const int fr = G->from[ip], to = G->to[ip];
+        const double* n = G->n[ip];
+        for (int sh = 0; sh < nsh; ++sh) {
+            const double* gsh = G->g[ip][sh];
+            /* diffusion :336-356 */
+            const double flux_sh = -1.0 * nu * rho * dot(dim, gsh, n);
+            for (int d1 = 0; d1 < dim; ++d1) { JL(d1, fr, d1, sh) += flux_sh; JL(d1, to, d1, sh) -= flux_sh; }
+            if (!p->laplace)
+                for (int d1 = 0; d1 < dim; ++d1)
+                    for (int d2 = 0; d2 < dim; ++d2) {
+                        const double f2 = -1.0 * nu * rho * gsh[d1] * n[d2];
+                        JL(d1, fr, d2, sh) += f2; JL(d1, to, d2, sh) -= f2;
+                    }
+            /* pressure :363-368 */
+            for (int d1 = 0; d1 < dim; ++d1) {
+                const double fp = G->phi[ip][sh] * n[d1];
+                JL(d1, fr, P, sh) += fp; JL(d1, to, P, sh) -= fp;
+            }
+            /* convection :375-553 */
+            if (!p->stokes) {
+                double upv[3] = {0, 0, 0};
+                if (!p->pac) {
+                    for (int k = 0; k < nsh; ++k)
+                        for (int d = 0; d < dim; ++d) upv[d] += W->Uc[ip][k] * UL(d, k);
+                } else {
+                    for (int d = 0; d < dim; ++d) upv[d] = W->S.vel[ip][d];
+                }
+                double w = 1.0;
+                if (p->peclet) w = peclet_blend(p, G, ip, W->stdvel[ip], upv);
+                const double prod = dot(dim, W->stdvel[ip], n) * rho;
+                if (p->pac) {
+                    for (int d1 = 0; d1 < dim; ++d1) {
+                        const double cv = prod * w * W->S.Svel[ip][d1][sh];
+                        JL(d1, fr, d1, sh) += cv; JL(d1, to, d1, sh) -= cv;
+                    }
+                    for (int d1 = 0; d1 < dim; ++d1) {
+                        const double cp = prod * w * W->S.Sp[ip][d1][sh];
+                        JL(d1, fr, P, sh) += cp; JL(d1, to, P, sh) -= cp;
+                    }
+                } else {
+                    const double cv = W->Uc[ip][sh] * prod * w;
+                    for (int d1 = 0; d1 < dim; ++d1) { JL(d1, fr, d1, sh) += cv; JL(d1, to, d1, sh) -= cv; }
+                }
+                if (p->peclet) {
+                    const double cpe = prod * (1.0 - w) * G->phi[ip][sh];
+                    for (int d1 = 0; d1 < dim; ++d1) { JL(d1, fr, d1, sh) += cpe; JL(d1, to, d1, sh) -= cpe; }
+                }
+                if (p->exact != 0.0) {
+                    if (p->pac) {
+                        for (int d1 = 0; d1 < dim; ++d1) {
+                            for (int d2 = 0; d2 < dim; ++d2) {
+                                double pv = W->S.Svel[ip][d1][sh] * n[d1];
+                                pv *= p->exact * rho;
+                                JL(d1, fr, d2, sh) += pv * upv[d1]; JL(d1, to, d2, sh) -= pv * upv[d1];
+                            }
+                            double pp = 0.0;
+                            for (int k = 0; k < dim; ++k) pp += W->S.Sp[ip][k][sh] * n[k];
+                            pp *= p->exact * rho;
+                            JL(d1, fr, P, sh) += pp * upv[d1]; JL(d1, to, P, sh) -= pp * upv[d1];
+                        }
+                    } else {
+                        for (int d1 = 0; d1 < dim; ++d1)
+                            for (int d2 = 0; d2 < dim; ++d2) {
+                                const double pv = w * W->Uc[ip][sh] * n[d2] * rho;
+                                JL(d1, fr, d2, sh) += pv * upv[d1]; JL(d1, to, d2, sh) -= pv * upv[d1];
+                            }
+                    }
+                    if (p->peclet)
+                        for (int d1 = 0; d1 < dim; ++d1)
+                            for (int d2 = 0; d2 < dim; ++d2) {
+                                const double cpe = upv[d1] * (1.0 - w) * G->phi[ip][sh] * n[d2] * rho;
+                                JL(d1, fr, d2, sh) += cpe; JL(d1, to, d2, sh) -= cpe;
+                            }
+                }
+            }
+            /* continuity :561-592 (vel_comp_connected == false) */
+            for (int d1 = 0; d1 < dim; ++d1) {
+                const double cv = W->S.Svel[ip][d1][sh] * n[d1] * rho;
+                JL(P, fr, d1, sh) += cv; JL(P, to, d1, sh) -= cv;
+            }
+            double cpp = 0.0;
+            for (int d1 = 0; d1 < dim; ++d1) cpp += W->S.Sp[ip][d1][sh] * n[d1] * rho;
+            JL(P, fr, P, sh) += cpp; JL(P, to, P, sh) -= cpp;
+        }
+    }
+#undef JL
+    return 0;
+}
+
+/* add_def_A_elem: navier_stokes_fv1.cpp:597-778 */
+static int def_A(const or_params* p, const or_geo* G, const double* uloc, const prolog* W, double* dl)
+{
+    const int dim = G->dim, nip = G->nip, nsh = G->nsh, nc = dim + 1, P = dim;
+    const double nu = p->nu, rho = p->rho;
+#define DL(a, i) dl[(i) * nc + (a)]
+    for (int ip = 0; ip < nip; ++ip) {
+        const int fr = G->from[ip], to = G->to[ip];
+        const double* n = G->n[ip];
+        double gv[3][3];
+        for (int d1 = 0; d1 < dim; ++d1)
+            for (int d2 = 0; d2 < dim; ++d2) {
+                gv[d1][d2] = 0.0;
+                for (int sh = 0; sh < nsh; ++sh) gv[d1][d2] += G->g[ip][sh][d2] * UL(d1, sh);
+            }
+        double df[3];
+        for (int d1 = 0; d1 < dim; ++d1) {
+            df[d1] = 0.0;
+            for (int d2 = 0; d2 < dim; ++d2) df[d1] += gv[d1][d2] * n[d2];
+        }
+        if (!p->laplace)
+            for (int d1 = 0; d1 < dim; ++d1)
+                for (int d2 = 0; d2 < dim; ++d2) df[d1] += gv[d2][d1] * n[d2];
+        for (int d1 = 0; d1 < dim; ++d1) df[d1] *= (-1.0) * nu * rho;
+        for (int d1 = 0; d1 < dim; ++d1) { DL(d1, fr) += df[d1]; DL(d1, to) -= df[d1]; }
+        if (!p->stokes) {
+            double upv[3] = {0, 0, 0};
+            if (!p->pac) {
+                for (int k = 0; k < nsh; ++k)
+                    for (int d = 0; d < dim; ++d) upv[d] += W->Uc[ip][k] * UL(d, k);
+            } else {
+                for (int d = 0; d < dim; ++d) upv[d] = W->S.vel[ip][d];
+            }
+            if (p->peclet) peclet_blend(p, G, ip, W->stdvel[ip], upv);
+            const double prod = dot(dim, W->stdvel[ip], n) * rho;
+            for (int d1 = 0; d1 < dim; ++d1) { DL(d1, fr) += upv[d1] * prod; DL(d1, to) -= upv[d1] * prod; }
+        }
+        double pr = 0.0;
+        for (int sh = 0; sh < nsh; ++sh) pr += G->phi[ip][sh] * UL(P, sh);
+        for (int d1 = 0; d1 < dim; ++d1) { DL(d1, fr) += pr * n[d1]; DL(d1, to) -= pr * n[d1]; }
+        const double cf = dot(dim, W->S.vel[ip], n) * rho;
+        DL(P, fr) += cf; DL(P, to) -= cf;
+    }
+#undef DL
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Term-magnitude scale: for every entry the sum of |terms| that the     */
+/* restated loops add to it, with every interpolated factor (StdVel,    */
+/* upwind velocity, velocity gradient, ip pressure, stabilised velocity,*/
+/* ip flux) replaced by the sum of the magnitudes it is formed from.    */
+/* This bounds the rounding of any evaluation order; the parity tests   */
+/* state their tolerances against it.                                   */
+/* ------------------------------------------------------------------ */
+static void elem_A_abs(const or_params* p, const or_geo* G, const double* uloc, const double* uold, double dt,
+                       const prolog* W, double* Ja, double* da)
+{
+    const int dim = G->dim, nip = G->nip, nsh = G->nsh, nc = dim + 1, P = dim, nl = nsh * nc;
+    const double nu = fabs(p->nu), rho = fabs(p->rho);
+#define JA(a, i, b, j) Ja[((i) * nc + (a)) * nl + (j) * nc + (b)]
+#define DA(a, i) da[(i) * nc + (a)]
+#define ADDJ(a, b, j, X) do { const double _x = fabs(X); if (Ja) { JA(a, fr, b, j) += _x; JA(a, to, b, j) += _x; } } while (0)
+#define ADDD(a, X) do { const double _x = fabs(X); if (da) { DA(a, fr) += _x; DA(a, to) += _x; } } while (0)
+    for (int ip = 0; ip < nip; ++ip) {
+        const int fr = G->from[ip], to = G->to[ip];
+        double n[3] = {0, 0, 0};
+        for (int i = 0; i < dim; ++i) n[i] = fabs(G->n[ip][i]);
+        double sva[3] = {0, 0, 0}, pra = 0, upa[3] = {0, 0, 0}, gva[3][3] = {{0}}, sta[3] = {0, 0, 0};
+        for (int sh = 0; sh < nsh; ++sh) {
+            for (int d = 0; d < dim; ++d) sva[d] += fabs(UL(d, sh) * G->phi[ip][sh]);
+            pra += fabs(G->phi[ip][sh] * UL(P, sh));
+            for (int d = 0; d < dim; ++d) upa[d] += fabs(W->Uc[ip][sh] * UL(d, sh));
+            for (int d1 = 0; d1 < dim; ++d1)
+                for (int d2 = 0; d2 < dim; ++d2) gva[d1][d2] += fabs(G->g[ip][sh][d2] * UL(d1, sh));
+        }
+        /* stabilised velocity magnitude bound */
+        if (p->stab_type == OR_STAB_WITHOUT) {
+            for (int d = 0; d < dim; ++d) sta[d] = sva[d];
+        } else {
+            const double diag = W->S.diag[ip];
+            for (int d = 0; d < dim; ++d) {
+                double a = p->has_source ? fabs(p->f[d]) : 0.0;
+                if (uold) {
+                    double o = 0;
+                    for (int sh = 0; sh < nsh; ++sh) o += fabs(G->phi[ip][sh] * UO(d, sh));
+                    a += o / dt;
+                }
+                for (int k = 0; k < nsh; ++k) {
+                    a += fabs(W->S.Svel[ip][d][k] * diag * UL(d, k));
+                    a += fabs(G->g[ip][k][d] / p->rho * UL(P, k));
+                }
+                sta[d] = a / diag;
+            }
+        }
+        if (p->pac)
+            for (int d = 0; d < dim; ++d) upa[d] = sta[d];
+        double proda = 0;
+        for (int d = 0; d < dim; ++d) proda += sva[d] * n[d];
+        proda *= rho;
+        double w = 1.0, upv_w[3];
+        for (int d = 0; d < dim; ++d) upv_w[d] = upa[d];
+        if (p->peclet && !p->stokes) {
+            double sv[3] = {0, 0, 0};
+            for (int d = 0; d < dim; ++d) sv[d] = W->stdvel[ip][d];
+            double tmp[3] = {0, 0, 0};
+            w = peclet_blend(p, G, ip, sv, tmp);
+            for (int d = 0; d < dim; ++d) upv_w[d] = fabs(w) * upa[d] + fabs(1.0 - w) * sva[d];
+        }
+        /* defect */
+        for (int d1 = 0; d1 < dim; ++d1) {
+            double df = 0;
+            for (int d2 = 0; d2 < dim; ++d2) df += gva[d1][d2] * n[d2];
+            if (!p->laplace)
+                for (int d2 = 0; d2 < dim; ++d2) df += gva[d2][d1] * n[d2];
+            ADDD(d1, nu * rho * df);
+            if (!p->stokes) ADDD(d1, upv_w[d1] * proda);
+            ADDD(d1, pra * n[d1]);
+        }
+        {
+            double cf = 0;
+            for (int d = 0; d < dim; ++d) cf += sta[d] * n[d];
+            ADDD(P, cf * rho);
+        }
+        /* Jacobian */
+        for (int sh = 0; sh < nsh; ++sh) {
+            const double* g = G->g[ip][sh];
+            double gn = 0;
+            for (int i = 0; i < dim; ++i) gn += fabs(g[i]) * n[i];
+            for (int d1 = 0; d1 < dim; ++d1) {
+                ADDJ(d1, d1, sh, nu * rho * gn);
+                if (!p->laplace)
+                    for (int d2 = 0; d2 < dim; ++d2) ADDJ(d1, d2, sh, nu * rho * g[d1] * n[d2]);
+                ADDJ(d1, P, sh, G->phi[ip][sh] * n[d1]);
+            }
+            if (!p->stokes) {
+                for (int d1 = 0; d1 < dim; ++d1) {
+                    if (p->pac) {
+                        ADDJ(d1, d1, sh, proda * w * W->S.Svel[ip][d1][sh]);
+                        ADDJ(d1, P, sh, proda * w * W->S.Sp[ip][d1][sh]);
+                    } else {
+                        ADDJ(d1, d1, sh, W->Uc[ip][sh] * proda * w);
+                    }
+                    if (p->peclet) ADDJ(d1, d1, sh, proda * (1.0 - w) * G->phi[ip][sh]);
+                    if (p->exact != 0.0) {
+                        for (int d2 = 0; d2 < dim; ++d2) {
+                            const double pv = p->pac ? W->S.Svel[ip][d1][sh] * n[d1] * p->exact * rho
+                                                     : w * W->Uc[ip][sh] * n[d2] * rho;
+                            ADDJ(d1, d2, sh, pv * upv_w[d1]);
+                            if (p->peclet) ADDJ(d1, d2, sh, upv_w[d1] * (1.0 - w) * G->phi[ip][sh] * n[d2] * rho);
+                        }
+                        if (p->pac) {
+                            double pp = 0;
+                            for (int k = 0; k < dim; ++k) pp += fabs(W->S.Sp[ip][k][sh]) * n[k];
+                            ADDJ(d1, P, sh, pp * p->exact * rho * upv_w[d1]);
+                        }
+                    }
+                }
+            }
+            double cpp = 0;
+            for (int d1 = 0; d1 < dim; ++d1) {
+                ADDJ(P, d1, sh, W->S.Svel[ip][d1][sh] * n[d1] * rho);
+                cpp += fabs(W->S.Sp[ip][d1][sh]) * n[d1] * rho;
+            }
+            ADDJ(P, P, sh, cpp);
+        }
+    }
+#undef JA
+#undef DA
+#undef ADDJ
+#undef ADDD
+}
+
+int or_elem_A_abs(const or_params* p, int et, const double* corners, const double* uloc,
+                  const double* uold, double dt, double* Ja, double* da)
+{
+    or_geo G;
+    int rc = or_geometry(et, corners, &G);
+    if (rc) return rc;
+    prolog W;
+    rc = prologue(p, &G, uloc, uold, dt, &W);
+    if (rc) return rc;
+    elem_A_abs(p, &G, uloc, uold, dt, &W, Ja, da);
+    return 0;
+}
+
+int or_elem_A(const or_params* p, int et, const double* corners, const double* uloc,
+              const double* uold, double dt, double* Jl, double* dl)
+{
+    or_geo G;
+    int rc = or_geometry(et, corners, &G);
+    if (rc) return rc;
+    prolog W;
+    rc = prologue(p, &G, uloc, uold, dt, &W);
+    if (rc) return rc;
+    if (Jl) jac_A(p, &G, uloc, &W, Jl);
+    if (dl) def_A(p, &G, uloc, &W, dl);
+    return 0;
+}
+
+/* add_jac_M_elem / add_def_M_elem: navier_stokes_fv1.cpp:781-838 */
+int or_elem_M(const or_params* p, int et, const double* corners, const double* uloc, double* Jl, double* dl)
+{
+    or_geo G;
+    int rc = or_geometry(et, corners, &G);
+    if (rc) return rc;
+    const int dim = G.dim, nsh = G.nsh, nc = dim + 1, nl = nsh * nc;
+    for (int sh = 0; sh < nsh; ++sh)
+        for (int d1 = 0; d1 < dim; ++d1) {
+            if (Jl) Jl[(sh * nc + d1) * nl + sh * nc + d1] += G.vol[sh] * p->rho;
+            if (dl) dl[sh * nc + d1] += UL(d1, sh) * G.vol[sh] * p->rho;
+        }
+    return 0;
+}
+
+/* add_rhs_elem: navier_stokes_fv1.cpp:841-869 */
+int or_elem_rhs(const or_params* p, int et, const double* corners, double* fl)
+{
+    if (!p->has_source) return 0;
+    or_geo G;
+    int rc = or_geometry(et, corners, &G);
+    if (rc) return rc;
+    const int dim = G.dim, nsh = G.nsh, nc = dim + 1;
+    for (int sh = 0; sh < nsh; ++sh)
+        for (int d1 = 0; d1 < dim; ++d1) fl[sh * nc + d1] += p->f[d1] * G.vol[sh] * p->rho;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* global scatter (ugcore's local->global add, restated)               */
+/* ------------------------------------------------------------------ */
+static int64_t find_col(const int64_t* rowptr, const int32_t* colidx, int64_t row, int32_t col)
+{
+    int64_t lo = rowptr[row], hi = rowptr[row + 1] - 1;
+    while (lo <= hi) {
+        const int64_t mid = (lo + hi) / 2;
+        if (colidx[mid] == col) return mid;
+        if (colidx[mid] < col) lo = mid + 1; else hi = mid - 1;
+    }
+    return -1;
+}
+
+static int assemble_impl(const or_params* p, int et, const double* xyz, int64_t nv,
+                         const int32_t* conn, int64_t e0, int64_t e1,
+                         const double* u, const double* uold, double dt,
+                         double s_a, double s_m, double s_f, int parts,
+                         const int64_t* rowptr, const int32_t* colidx,
+                         double* J, double* d, int absmode)
+{
+    if (et < 0 || et > 3) { set_err("unknown element type"); return -1; }
+    const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1, nl = nsh * nc;
+    if (p->dim != dim) { set_err("Wrong number of functions"); return -5; }
+    const int64_t ndof = nv * nc;
+    if (J) memset(J, 0, sizeof(double) * rowptr[ndof]);
+    if (d) memset(d, 0, sizeof(double) * ndof);
+    double co[24], ul[32], uo[32], Jl[1024], Ja[1024], dl[32], da[32], fl[32];
+    for (int64_t e = e0; e < e1; ++e) {
+        const int32_t* c = conn + e * nsh;
+        for (int k = 0; k < nsh; ++k) {
+            for (int i = 0; i < dim; ++i) co[k * dim + i] = xyz[(int64_t)c[k] * dim + i];
+            for (int a = 0; a < nc; ++a) {
+                ul[k * nc + a] = u[(int64_t)c[k] * nc + a];
+                if (uold) uo[k * nc + a] = uold[(int64_t)c[k] * nc + a];
+            }
+        }
+        memset(Jl, 0, sizeof(double) * nl * nl);
+        memset(dl, 0, sizeof(double) * nl);
+        memset(Ja, 0, sizeof(double) * nl * nl);
+        memset(da, 0, sizeof(double) * nl);
+        int rc;
+        if (parts & 1) {
+            double Jt[1024], dt_[32];
+            memset(Jt, 0, sizeof(double) * nl * nl);
+            memset(dt_, 0, sizeof(double) * nl);
+            rc = or_elem_A(p, et, co, ul, uold ? uo : NULL, dt, J ? Jt : NULL, d ? dt_ : NULL);
+            if (rc) return rc;
+            if (absmode) {
+                double Jb[1024], db[32];
+                memset(Jb, 0, sizeof(double) * nl * nl);
+                memset(db, 0, sizeof(double) * nl);
+                rc = or_elem_A_abs(p, et, co, ul, uold ? uo : NULL, dt, J ? Jb : NULL, d ? db : NULL);
+                if (rc) return rc;
+                for (int i = 0; i < nl * nl; ++i) Ja[i] += fabs(s_a) * Jb[i];
+                for (int i = 0; i < nl; ++i) da[i] += fabs(s_a) * db[i];
+            }
+            for (int i = 0; i < nl * nl; ++i) Jl[i] += s_a * Jt[i];
+            for (int i = 0; i < nl; ++i) dl[i] += s_a * dt_[i];
+        }
+        if (parts & 2) {
+            double Jt[1024], dt_[32];
+            memset(Jt, 0, sizeof(double) * nl * nl);
+            memset(dt_, 0, sizeof(double) * nl);
+            rc = or_elem_M(p, et, co, ul, J ? Jt : NULL, d ? dt_ : NULL);
+            if (rc) return rc;
+            for (int i = 0; i < nl * nl; ++i) { Jl[i] += s_m * Jt[i]; Ja[i] += fabs(s_m * Jt[i]); }
+            for (int i = 0; i < nl; ++i) { dl[i] += s_m * dt_[i]; da[i] += fabs(s_m * dt_[i]); }
+        }
+        if ((parts & 4) && d) {
+            memset(fl, 0, sizeof(double) * nl);
+            rc = or_elem_rhs(p, et, co, fl);
+            if (rc) return rc;
+            for (int i = 0; i < nl; ++i) { dl[i] += s_f * fl[i]; da[i] += fabs(s_f * fl[i]); }
+        }
+        const double* Jsrc = absmode ? Ja : Jl;
+        const double* dsrc = absmode ? da : dl;
+        for (int i = 0; i < nsh; ++i)
+            for (int a = 0; a < nc; ++a) {
+                const int64_t row = (int64_t)c[i] * nc + a;
+                if (d) d[row] += dsrc[i * nc + a];
+                if (J)
+                    for (int j = 0; j < nsh; ++j)
+                        for (int b = 0; b < nc; ++b) {
+                            const int64_t pos = find_col(rowptr, colidx, row, (int32_t)((int64_t)c[j] * nc + b));
+                            if (pos < 0) { set_err("pattern misses an entry"); return -6; }
+                            J[pos] += Jsrc[(i * nc + a) * nl + j * nc + b];
+                        }
+            }
+    }
+    return 0;
+}
+
+int or_assemble(const or_params* p, int et, const double* xyz, int64_t nv,
+                const int32_t* conn, int64_t e0, int64_t e1,
+                const double* u, const double* uold, double dt,
+                double s_a, double s_m, double s_f, int parts,
+                const int64_t* rowptr, const int32_t* colidx,
+                double* J, double* d)
+{
+    return assemble_impl(p, et, xyz, nv, conn, e0, e1, u, uold, dt, s_a, s_m, s_f, parts,
+                         rowptr, colidx, J, d, 0);
+}
+
+int or_assemble_abs(const or_params* p, int et, const double* xyz, int64_t nv,
+                    const int32_t* conn, int64_t ne,
+                    const double* u, const double* uold, double dt,
+                    double s_a, double s_m, int parts,
+                    const int64_t* rowptr, const int32_t* colidx,
+                    double* Jabs, double* dabs)
+{
+    return assemble_impl(p, et, xyz, nv, conn, 0, ne, u, uold, dt, s_a, s_m, 1.0, parts,
+                         rowptr, colidx, Jabs, dabs, 1);
+}
+
+/* ------------------------------------------------------------------ */
+/* CSR pattern: node adjacency closure (x) (dim+1)^2, sorted columns    */
+/* ------------------------------------------------------------------ */
+static int cmp_i32(const void* a, const void* b)
+{
+    const int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+    return (x > y) - (x < y);
+}
+
+int or_pattern(int et, int64_t nv, const int32_t* conn, int64_t ne,
+               int64_t* rowptr, int32_t* colidx, int64_t* nnz_out)
+{
+    const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1;
+    /* node -> element incidence */
+    int64_t* cnt = (int64_t*)calloc((size_t)nv + 1, sizeof(int64_t));
+    for (int64_t e = 0; e < ne; ++e)
+        for (int k = 0; k < nsh; ++k) cnt[conn[e * nsh + k] + 1]++;
+    for (int64_t v = 0; v < nv; ++v) cnt[v + 1] += cnt[v];
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)nv);
+    int64_t* inc = (int64_t*)malloc(sizeof(int64_t) * (size_t)cnt[nv]);
+    for (int64_t v = 0; v < nv; ++v) fill[v] = cnt[v];
+    for (int64_t e = 0; e < ne; ++e)
+        for (int k = 0; k < nsh; ++k) inc[fill[conn[e * nsh + k]]++] = e;
+    int32_t* nb = (int32_t*)malloc(sizeof(int32_t) * 512);
+    int64_t nnz = 0;
+    rowptr[0] = 0;
+    for (int64_t v = 0; v < nv; ++v) {
+        int m = 0;
+        for (int64_t q = cnt[v]; q < cnt[v + 1]; ++q)
+            for (int k = 0; k < nsh; ++k) {
+                const int32_t w = conn[inc[q] * nsh + k];
+                int dup = 0;
+                for (int t = 0; t < m; ++t) if (nb[t] == w) { dup = 1; break; }
+                if (!dup && m < 512) nb[m++] = w;
+            }
+        if (m == 0) nb[m++] = (int32_t)v; /* isolated node: keep the diagonal */
+        qsort(nb, (size_t)m, sizeof(int32_t), cmp_i32);
+        for (int a = 0; a < nc; ++a) {
+            const int64_t row = v * nc + a;
+            if (colidx)
+                for (int t = 0; t < m; ++t)
+                    for (int b = 0; b < nc; ++b) colidx[rowptr[row] + t * nc + b] = nb[t] * nc + b;
+            rowptr[row + 1] = rowptr[row] + (int64_t)m * nc;
+        }
+        nnz += (int64_t)m * nc * nc;
+    }
+    *nnz_out = nnz;
+    free(cnt); free(fill); free(inc); free(nb);
+    return 0;
+}

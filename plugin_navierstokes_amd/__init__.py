@@ -1,0 +1,252 @@
+"""MI355X-native FV1 incompressible Navier-Stokes assembly.
+
+``NavierStokesFV1`` mirrors the Lua/registry surface of the reference
+``NavierStokesFV1`` element discretisation
+(``incompressible/fv1/register_fv1.cpp:167-184``,
+``register_navier_stokes.cpp:104-126``,
+``incompressible/incompressible_navier_stokes_plugin.cpp:244-266``): the same
+setter names, string keys, defaults and error cases.  Every call goes through
+the C ABI of ``libnsfv1.so`` (declared in ``include/nsfv1.h``); device
+buffers are torch tensors.  There is no CPU fallback: without the built HIP
+library or a GPU the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import meshes  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libnsfv1.so")
+
+TRIANGLE, QUADRILATERAL, TETRAHEDRON, HEXAHEDRON = 0, 1, 2, 3
+PART_A, PART_M, PART_RHS = 1, 2, 4
+F32, F64 = 32, 64
+
+ERRORS = {
+    -1: "ARGUMENT", -2: "FUNCTION_COUNT", -3: "TRIAL_SPACE", -4: "NO_STABILIZATION", -5: "NO_UPWIND",
+    -6: "NO_VISCOSITY", -7: "HANGING_NODES", -8: "TIME_SERIES", -9: "CUT_SIDE", -10: "UNKNOWN_NAME",
+    -11: "NOT_IMPLEMENTED", -12: "NO_MESH", -13: "DEVICE", -14: "DEGENERATE",
+}
+
+# every symbol the C ABI exports (include/nsfv1.h)
+EXPORTS = [
+    "nsfv1_last_error", "nsfv1_create", "nsfv1_destroy", "nsfv1_prepare_setting", "nsfv1_disc_type",
+    "nsfv1_set_kinematic_viscosity", "nsfv1_set_source", "nsfv1_set_exact_jacobian", "nsfv1_set_density",
+    "nsfv1_set_stokes", "nsfv1_set_laplace", "nsfv1_set_peclet_blend", "nsfv1_set_grad_div",
+    "nsfv1_set_upwind", "nsfv1_set_stabilization", "nsfv1_set_stabilization_without",
+    "nsfv1_set_stabilization_upwind", "nsfv1_set_pac_upwind", "nsfv1_set_mesh", "nsfv1_num_dofs", "nsfv1_nnz",
+    "nsfv1_get_pattern", "nsfv1_pattern_device", "nsfv1_assemble", "nsfv1_sync_errors", "nsfv1_set_launch",
+]
+
+
+class NSFV1Error(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"NSFV1_ERR_{ERRORS.get(code, code)} ({code}): {msg}")
+        self.code = code
+        self.name = ERRORS.get(code, str(code))
+
+
+_lib = None
+
+
+def load_library(path: str | None = None):
+    """Load the HIP library; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m plugin_navierstokes_amd.build`")
+    L = C.CDLL(path)
+    vp, dp = C.c_void_p, C.c_double
+    H = C.c_void_p
+    L.nsfv1_last_error.restype = C.c_char_p
+    L.nsfv1_create.argtypes = [C.c_int, C.c_char_p, C.c_char_p, C.POINTER(H)]
+    L.nsfv1_destroy.argtypes = [H]
+    L.nsfv1_prepare_setting.argtypes = [H, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.c_int]
+    L.nsfv1_disc_type.argtypes = [H, C.c_char_p, C.c_size_t]
+    for n in ["nsfv1_set_kinematic_viscosity", "nsfv1_set_exact_jacobian", "nsfv1_set_density", "nsfv1_set_grad_div"]:
+        getattr(L, n).argtypes = [H, dp]
+    for n in ["nsfv1_set_stokes", "nsfv1_set_laplace", "nsfv1_set_peclet_blend", "nsfv1_set_pac_upwind"]:
+        getattr(L, n).argtypes = [H, C.c_int]
+    L.nsfv1_set_source.argtypes = [H, C.POINTER(dp), C.c_int]
+    L.nsfv1_set_upwind.argtypes = [H, C.c_char_p]
+    L.nsfv1_set_stabilization.argtypes = [H, C.c_char_p, C.c_char_p]
+    L.nsfv1_set_stabilization_without.argtypes = [H]
+    L.nsfv1_set_stabilization_upwind.argtypes = [H, C.c_char_p]
+    L.nsfv1_set_mesh.argtypes = [H, C.c_int, vp, C.c_int64, vp, C.c_int64]
+    L.nsfv1_num_dofs.argtypes = [H, C.POINTER(C.c_int64)]
+    L.nsfv1_nnz.argtypes = [H, C.POINTER(C.c_int64)]
+    L.nsfv1_get_pattern.argtypes = [H, vp, vp]
+    L.nsfv1_pattern_device.argtypes = [H, C.POINTER(vp), C.POINTER(vp)]
+    L.nsfv1_assemble.argtypes = [H, vp, vp, dp, dp, dp, dp, C.c_int, vp, C.c_int, vp, vp]
+    L.nsfv1_sync_errors.argtypes = [H, vp]
+    L.nsfv1_set_launch.argtypes = [H, C.c_int, C.c_int]
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise NSFV1Error(rc, load_library().nsfv1_last_error().decode())
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    return C.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+
+
+class NavierStokesFV1:
+    """Host object of the GPU assembly; mirrors ``NavierStokesFV1{2d,3d}``."""
+
+    def __init__(self, functions="u,v,p", subsets="Inner", dim: int | None = None):
+        if isinstance(functions, (list, tuple)):
+            functions = ",".join(functions)
+        if isinstance(subsets, (list, tuple)):
+            subsets = ",".join(subsets)
+        nf = len([f for f in functions.split(",") if f.strip()])
+        self.dim = dim if dim is not None else nf - 1
+        L = load_library()
+        h = C.c_void_p()
+        _check(L.nsfv1_create(self.dim, functions.encode(), subsets.encode(), C.byref(h)))
+        self._h = h
+        self.mesh = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.nsfv1_destroy(h)
+            self._h = None
+
+    # --- NavierStokesBase ---
+    def set_kinematic_viscosity(self, nu: float):
+        _check(_lib.nsfv1_set_kinematic_viscosity(self._h, float(nu)))
+
+    def set_source(self, f):
+        arr = (C.c_double * len(f))(*[float(x) for x in f]) if f is not None else None
+        _check(_lib.nsfv1_set_source(self._h, arr, 0 if f is None else len(f)))
+
+    def set_exact_jacobian(self, factor):
+        _check(_lib.nsfv1_set_exact_jacobian(self._h, (1.0 if factor else 0.0) if isinstance(factor, bool) else float(factor)))
+
+    def disc_type(self) -> str:
+        buf = C.create_string_buffer(16)
+        _check(_lib.nsfv1_disc_type(self._h, buf, 16))
+        return buf.value.decode()
+
+    # --- IncompressibleNavierStokesBase ---
+    def set_density(self, rho: float):
+        _check(_lib.nsfv1_set_density(self._h, float(rho)))
+
+    def set_stokes(self, on: bool):
+        _check(_lib.nsfv1_set_stokes(self._h, int(bool(on))))
+
+    def set_laplace(self, on: bool):
+        _check(_lib.nsfv1_set_laplace(self._h, int(bool(on))))
+
+    def set_peclet_blend(self, on: bool):
+        _check(_lib.nsfv1_set_peclet_blend(self._h, int(bool(on))))
+
+    def set_grad_div(self, factor: float):
+        _check(_lib.nsfv1_set_grad_div(self._h, float(factor)))
+
+    # --- NavierStokesFV1 ---
+    def set_upwind(self, name: str):
+        _check(_lib.nsfv1_set_upwind(self._h, name.encode()))
+
+    def set_stabilization(self, name: str, diff_length: str | None = None):
+        _check(_lib.nsfv1_set_stabilization(self._h, name.encode(), None if diff_length is None else diff_length.encode()))
+
+    def set_stabilization_without(self):
+        _check(_lib.nsfv1_set_stabilization_without(self._h))
+
+    def set_stabilization_upwind(self, name: str):
+        _check(_lib.nsfv1_set_stabilization_upwind(self._h, name.encode()))
+
+    def set_pac_upwind(self, on: bool):
+        _check(_lib.nsfv1_set_pac_upwind(self._h, int(bool(on))))
+
+    def prepare_setting(self, lfe_types, orders, non_regular_grid=False):
+        n = len(lfe_types)
+        t = (C.c_int * n)(*lfe_types)
+        o = (C.c_int * n)(*orders)
+        _check(_lib.nsfv1_prepare_setting(self._h, t, o, n, int(bool(non_regular_grid))))
+
+    # --- mesh / pattern ---
+    def set_mesh(self, mesh=None, et=None, xyz=None, conn=None):
+        if mesh is not None:
+            et, xyz, conn = mesh.et, mesh.xyz, mesh.conn
+            self.mesh = mesh
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        conn = np.ascontiguousarray(conn, dtype=np.int32)
+        _check(_lib.nsfv1_set_mesh(self._h, int(et), xyz.ctypes.data, xyz.shape[0], conn.ctypes.data, conn.shape[0]))
+        self.nv = xyz.shape[0]
+        self.ne = conn.shape[0]
+
+    def num_dofs(self) -> int:
+        n = C.c_int64()
+        _check(_lib.nsfv1_num_dofs(self._h, C.byref(n)))
+        return n.value
+
+    def nnz(self) -> int:
+        n = C.c_int64()
+        _check(_lib.nsfv1_nnz(self._h, C.byref(n)))
+        return n.value
+
+    def pattern(self):
+        rowptr = np.zeros(self.num_dofs() + 1, dtype=np.int64)
+        colidx = np.zeros(self.nnz(), dtype=np.int32)
+        _check(_lib.nsfv1_get_pattern(self._h, rowptr.ctypes.data, colidx.ctypes.data))
+        return rowptr, colidx
+
+    def pattern_device(self):
+        r, c = C.c_void_p(), C.c_void_p()
+        _check(_lib.nsfv1_pattern_device(self._h, C.byref(r), C.byref(c)))
+        return r.value, c.value
+
+    def set_launch(self, threads=0, lds_bytes=0):
+        _check(_lib.nsfv1_set_launch(self._h, int(threads), int(lds_bytes)))
+
+    # --- assembly ---
+    def assemble(self, u, u_old=None, dt=0.0, s_a=1.0, s_m=0.0, s_f=0.0, parts=PART_A, jprec=F64,
+                 J=None, d=None, want_J=True, want_d=True, stream=None, check=True):
+        """Assemble J = s_a J_A + s_m J_M and d = s_a d_A + s_m d_M + s_f f.
+
+        ``u``/``u_old``: CUDA float64 tensors [nv, dim+1]; returns (J, d)."""
+        import torch
+
+        dev = u.device
+        if want_J and J is None:
+            J = torch.empty(self.nnz(), dtype=torch.float32 if jprec == F32 else torch.float64, device=dev)
+        if want_d and d is None:
+            d = torch.empty(self.num_dofs(), dtype=torch.float64, device=dev)
+        _check(_lib.nsfv1_assemble(self._h, _ptr(u), _ptr(u_old), float(dt), float(s_a), float(s_m), float(s_f),
+                                   int(parts), _ptr(J) if want_J else None, int(jprec), _ptr(d) if want_d else None,
+                                   _stream_ptr(stream)))
+        if check:
+            self.sync_errors(stream)
+        return (J if want_J else None), (d if want_d else None)
+
+    def sync_errors(self, stream=None):
+        _check(_lib.nsfv1_sync_errors(self._h, _stream_ptr(stream)))
+
+
+def NavierStokes(fcts, subsets, disc_type: str = "fv1"):
+    """``NavierStokes(fcts, subsets, discType)`` of lua/lua-include.lua:36-47,
+    restricted to the FV1 discretisation built here."""
+    if disc_type is None:
+        disc_type = "fv1"
+    if disc_type != "fv1":
+        raise NSFV1Error(-11, f"NavierStokes: no disc type '{disc_type}' available in this build")
+    return NavierStokesFV1(fcts, subsets)

@@ -1,0 +1,653 @@
+// nsfv1_api.cpp -- C++ host layer and C ABI of the MI355X FV1 Navier-Stokes
+// assembly.  The handle is the host-side counterpart of the reference's
+// NavierStokesFV1<TDomain> object (incompressible/fv1/navier_stokes_fv1.h:
+// 145-629): it carries the same configuration state (viscosity, density,
+// source, exact-Jacobian factor, Stokes/Laplace/Peclet flags, the
+// stabilisation object and the upwind objects with their sharing rules) and
+// replaces ugcore's element loop by one device launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/nsfv1.h"
+#include "nsfv1_internal.hpp"
+
+using namespace nsfv1;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+std::string trim_lower(const char* s)
+{
+    std::string n = s ? s : "";
+    size_t a = n.find_first_not_of(" \t\n\r"), b = n.find_last_not_of(" \t\n\r");
+    n = (a == std::string::npos) ? std::string() : n.substr(a, b - a + 1);
+    std::transform(n.begin(), n.end(), n.begin(), [](unsigned char c) { return (char)std::tolower(c); });
+    return n;
+}
+
+// ---------------------------------------------------------------------
+// reference-element topology (UG4 numbering) and the derived FV1 tables
+// ---------------------------------------------------------------------
+struct TopoSrc {
+    int dim, nsh, nip, nside;
+    double lc[8][3];
+    int edge[12][2];
+    int side_n[6];
+    int side[6][4];
+};
+
+const TopoSrc kTopo[4] = {
+    {2, 3, 3, 3, {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}}, {{0, 1}, {1, 2}, {2, 0}}, {2, 2, 2}, {{0, 1}, {1, 2}, {2, 0}}},
+    {2, 4, 4, 4, {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}}, {{0, 1}, {1, 2}, {2, 3}, {3, 0}}, {2, 2, 2, 2},
+     {{0, 1}, {1, 2}, {2, 3}, {3, 0}}},
+    {3, 4, 6, 4, {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}}, {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}},
+     {3, 3, 3, 3}, {{0, 2, 1}, {1, 2, 3}, {0, 3, 2}, {0, 1, 3}}},
+    {3, 8, 12, 6, {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}},
+     {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 4}, {1, 5}, {2, 6}, {3, 7}, {4, 5}, {5, 6}, {6, 7}, {7, 4}},
+     {4, 4, 4, 4, 4, 4}, {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {0, 4, 7, 3}, {4, 5, 6, 7}}},
+};
+
+void build_topology(RefTopo* out)
+{
+    std::memset(out, 0, sizeof(RefTopo) * 4);
+    for (int et = 0; et < 4; ++et) {
+        const TopoSrc& S = kTopo[et];
+        RefTopo& T = out[et];
+        for (int i = 0; i < S.nip; ++i) { T.edge[i][0] = (int8_t)S.edge[i][0]; T.edge[i][1] = (int8_t)S.edge[i][1]; }
+        for (int s = 0; s < S.nside; ++s) {
+            T.side_n[s] = (int8_t)S.side_n[s];
+            for (int j = 0; j < S.side_n[s]; ++j) T.side[s][j] = (int8_t)S.side[s][j];
+        }
+        for (int k = 0; k < S.nsh; ++k)
+            for (int i = 0; i < 3; ++i) T.lc[k][i] = S.lc[k][i];
+        auto has = [&](int s, int c) {
+            for (int j = 0; j < S.side_n[s]; ++j)
+                if (S.side[s][j] == c) return true;
+            return false;
+        };
+        for (int e = 0; e < S.nip; ++e) {
+            int m = 0;
+            T.ef[e][0] = T.ef[e][1] = -1;
+            if (S.dim == 3)
+                for (int s = 0; s < S.nside && m < 2; ++s)
+                    if (has(s, S.edge[e][0]) && has(s, S.edge[e][1])) T.ef[e][m++] = (int8_t)s;
+        }
+        for (int k = 0; k < 8; ++k)
+            for (int q = 0; q < 3; ++q) T.cip[k][q] = -1;
+        for (int k = 0; k < S.nsh; ++k) {
+            int m = 0;
+            for (int e = 0; e < S.nip; ++e)
+                if (S.edge[e][0] == k || S.edge[e][1] == k) T.cip[k][m++] = (int8_t)e;
+        }
+    }
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t bytes)
+    {
+        release();
+        n = bytes;
+        return hipMalloc(&p, bytes ? bytes : 16);
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Plan {
+    bool valid = false;
+    int threads = 0, nwg = 0;
+    size_t lds = 0;
+    DevBuf wg_node, wg_rounds;
+};
+
+}  // namespace
+
+// enum values of the configuration state
+enum { UPW_UNSET = -1 };
+enum { STAB_UNSET = -1 };
+
+struct nsfv1_handle {
+    int dim = 2;
+    int device = 0;
+    std::vector<std::string> functions, subsets;
+    // NavierStokesBase / IncompressibleNavierStokesBase state
+    double nu = 0.0;
+    bool nu_set = false;
+    double rho = 1.0;  // navier_stokes_fv1.cpp:82
+    bool has_source = false;
+    double f[3] = {0, 0, 0};
+    double exact = 0.0;  // navier_stokes_base.cpp:57
+    bool stokes = false, laplace = false, peclet = false;  // incompressible_navier_stokes_base.cpp:57-59
+    double grad_div = 0.0;
+    // stabilisation / upwind objects (types + sharing)
+    int stab_type = STAB_UNSET;
+    int stab_upwind = UPW_UNSET;
+    int difflen = DL_RAW;
+    int conv_upwind = UPW_UNSET;
+    bool pac = false;
+    // mesh
+    int et = -1;
+    int64_t nv = 0, ne = 0, ndof = 0, nnz = 0, ninc = 0;
+    DevBuf xyz, conn, rowptr, colidx, inc_elem, inc_node, inc_info, node_inc, errflag;
+    std::vector<int64_t> h_rowptr, h_node_inc;
+    std::vector<int32_t> h_colidx_unused;
+    int max_threads = 256;
+    size_t lds_cap = 65536;
+    Plan plan32, plan64;
+};
+
+extern "C" {
+
+const char* nsfv1_last_error(void) { return g_last_error.c_str(); }
+
+int nsfv1_create(int dim, const char* functions, const char* subsets, nsfv1_handle** out)
+{
+    if (!out) return fail(NSFV1_ERR_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (dim != 2 && dim != 3) return fail(NSFV1_ERR_ARGUMENT, "NavierStokesFV1: dim must be 2 or 3");
+    std::vector<std::string> fct;
+    {
+        std::string s = functions ? functions : "";
+        size_t p = 0;
+        while (p <= s.size()) {
+            size_t q = s.find(',', p);
+            if (q == std::string::npos) q = s.size();
+            std::string t = s.substr(p, q - p);
+            t.erase(0, t.find_first_not_of(" \t"));
+            t.erase(t.find_last_not_of(" \t") + 1);
+            if (!t.empty()) fct.push_back(t);
+            p = q + 1;
+        }
+    }
+    // navier_stokes_fv1.cpp:65-68
+    if ((int)fct.size() != dim + 1)
+        return fail(NSFV1_ERR_FUNCTION_COUNT, "Wrong number of functions: The ElemDisc 'NavierStokes' needs exactly " +
+                                                  std::to_string(dim + 1) + " symbolic function.");
+    static bool topo_done_dev[64] = {false};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "no HIP device");
+    if (dev >= 0 && dev < 64 && !topo_done_dev[dev]) {
+        RefTopo topo[4];
+        build_topology(topo);
+        hipError_t e = upload_topology(topo);
+        if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("topology upload: ") + hipGetErrorString(e));
+        topo_done_dev[dev] = true;
+    }
+    auto h = std::make_unique<nsfv1_handle>();
+    h->dim = dim;
+    h->device = dev;
+    h->functions = fct;
+    h->subsets.push_back(subsets ? subsets : "");
+    if (h->errflag.alloc(sizeof(int)) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "hipMalloc");
+    if (hipMemset(h->errflag.p, 0, sizeof(int)) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "hipMemset");
+    *out = h.release();
+    return NSFV1_OK;
+}
+
+int nsfv1_destroy(nsfv1_handle* h)
+{
+    delete h;
+    return NSFV1_OK;
+}
+
+int nsfv1_prepare_setting(nsfv1_handle* h, const int* lfe_type, const int* order, int n, int non_regular)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (non_regular) return fail(NSFV1_ERR_HANGING_NODES, "NavierStokes: only regular grid implemented.");
+    if (n != h->dim + 1) return fail(NSFV1_ERR_TRIAL_SPACE, "NavierStokes: Need exactly " + std::to_string(h->dim + 1) + " functions");
+    for (int d = 0; d < n; ++d)
+        if (!lfe_type || !order || lfe_type[d] != 0 || order[d] != 1)
+            return fail(NSFV1_ERR_TRIAL_SPACE, "NavierStokes: 'fv1' expects Lagrange P1 trial space for velocity and pressure.");
+    return NSFV1_OK;
+}
+
+int nsfv1_disc_type(nsfv1_handle* h, char* buf, size_t n)
+{
+    if (!h || !buf || n < 4) return fail(NSFV1_ERR_ARGUMENT, "bad argument");
+    std::snprintf(buf, n, "fv1");
+    return NSFV1_OK;
+}
+
+int nsfv1_set_kinematic_viscosity(nsfv1_handle* h, double nu)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    h->nu = nu;
+    h->nu_set = true;
+    return NSFV1_OK;
+}
+
+int nsfv1_set_source(nsfv1_handle* h, const double* f, int n)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (!f || n <= 0) {
+        h->has_source = false;
+        return NSFV1_OK;
+    }
+    for (int i = 0; i < 3; ++i) h->f[i] = i < n && i < h->dim ? f[i] : 0.0;
+    h->has_source = true;
+    return NSFV1_OK;
+}
+
+int nsfv1_set_exact_jacobian(nsfv1_handle* h, double factor)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    h->exact = factor;
+    return NSFV1_OK;
+}
+
+int nsfv1_set_density(nsfv1_handle* h, double rho)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    h->rho = rho;
+    return NSFV1_OK;
+}
+
+int nsfv1_set_stokes(nsfv1_handle* h, int on)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    h->stokes = on != 0;
+    return NSFV1_OK;
+}
+
+int nsfv1_set_laplace(nsfv1_handle* h, int on)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    h->laplace = on != 0;
+    return NSFV1_OK;
+}
+
+int nsfv1_set_peclet_blend(nsfv1_handle* h, int on)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    h->peclet = on != 0;
+    return NSFV1_OK;
+}
+
+int nsfv1_set_grad_div(nsfv1_handle* h, double factor)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    h->grad_div = factor;  // used by the FVCR discretisation only
+    return NSFV1_OK;
+}
+
+// upwind factory: upwind_interface.cpp:43-62
+static int parse_upwind(const char* name, int* type)
+{
+    const std::string n = trim_lower(name);
+    if (n == "no") { *type = UPW_NO; return NSFV1_OK; }
+    if (n == "full") { *type = UPW_FULL; return NSFV1_OK; }
+    if (n == "skewed") { *type = UPW_SKEWED; return NSFV1_OK; }
+    if (n == "linearprofileskewed" || n == "lps") { *type = UPW_LPS; return NSFV1_OK; }
+    if (n == "positive" || n == "pos" || n == "regular" || n == "reg")
+        return fail(NSFV1_ERR_NOT_IMPLEMENTED, "NavierStokes: upwind type '" + std::string(name) +
+                                                   "' needs the coupled ip system of the positive/regular upwinds, "
+                                                   "which this build does not provide");
+    return fail(NSFV1_ERR_UNKNOWN_NAME, "NavierStokes: upwind type '" + std::string(name ? name : "") +
+                                            "' not found. Options are: no, full, weighted, skewed, "
+                                            "LinearProfileSkewed (lps), positive (pos), regular (reg)");
+}
+
+int nsfv1_set_upwind(nsfv1_handle* h, const char* name)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    int t;
+    int rc = parse_upwind(name, &t);
+    if (rc) return rc;
+    // navier_stokes_fv1.h:213-215
+    h->pac = false;
+    h->conv_upwind = t;
+    if (h->stab_type != STAB_UNSET && h->stab_upwind == UPW_UNSET) h->stab_upwind = t;
+    return NSFV1_OK;
+}
+
+int nsfv1_set_stabilization(nsfv1_handle* h, const char* name, const char* diff_length)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    const std::string n = trim_lower(name);
+    if (n == "flow")
+        return fail(NSFV1_ERR_NOT_IMPLEMENTED, "NavierStokes: the FLOW stabilization (coupled velocity components) "
+                                               "is not provided by this build");
+    if (n != "fields")
+        return fail(NSFV1_ERR_UNKNOWN_NAME, "NavierStokes: stabilization type '" + std::string(name ? name : "") +
+                                                "' not a valid name of a Schneider-Raw stabilization. Options are: fields, flow");
+    int dl = DL_RAW;
+    if (diff_length) {
+        const std::string d = trim_lower(diff_length);
+        if (d == "raw") dl = DL_RAW;
+        else if (d == "fivepoint") dl = DL_FIVEPOINT;
+        else if (d == "cor") dl = DL_COR;
+        else return fail(NSFV1_ERR_UNKNOWN_NAME, "Diffusion Length calculation method not found. Use one of [Raw, Fivepoint, Cor].");
+    }
+    // navier_stokes_fv1.h:190-200: a new stabilisation object, handed the
+    // convective upwind if one is set
+    h->stab_type = STAB_FIELDS;
+    h->difflen = dl;
+    h->stab_upwind = (h->conv_upwind != UPW_UNSET) ? h->conv_upwind : UPW_UNSET;
+    return NSFV1_OK;
+}
+
+int nsfv1_set_stabilization_without(nsfv1_handle* h)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    // object form of set_stabilization: no upwind is handed over
+    h->stab_type = STAB_WITHOUT;
+    h->stab_upwind = UPW_UNSET;
+    return NSFV1_OK;
+}
+
+int nsfv1_set_stabilization_upwind(nsfv1_handle* h, const char* name)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (h->stab_type == STAB_UNSET) return fail(NSFV1_ERR_NO_STABILIZATION, "Stabilization has not been set.");
+    int t;
+    int rc = parse_upwind(name, &t);
+    if (rc) return rc;
+    h->stab_upwind = t;
+    return NSFV1_OK;
+}
+
+int nsfv1_set_pac_upwind(nsfv1_handle* h, int on)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (!on) return NSFV1_OK;
+    // navier_stokes_fv1.h:217-226
+    if (h->conv_upwind == UPW_UNSET) return fail(NSFV1_ERR_NO_UPWIND, "Upwind must be specified previously.\n");
+    if (h->stab_type == STAB_UNSET) return fail(NSFV1_ERR_NO_STABILIZATION, "Stabilization must be specified previously.\n");
+    h->stab_upwind = h->conv_upwind;
+    h->pac = true;
+    h->conv_upwind = UPW_UNSET;
+    return NSFV1_OK;
+}
+
+static int nsh_of(int et) { return et == ET_TRI ? 3 : (et == ET_QUAD || et == ET_TET) ? 4 : 8; }
+static int dim_of(int et) { return et <= ET_QUAD ? 2 : 3; }
+
+int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const int32_t* conn, int64_t ne)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (et < 0 || et > 3) return fail(NSFV1_ERR_ARGUMENT, "unknown element type");
+    if (dim_of(et) != h->dim) return fail(NSFV1_ERR_ARGUMENT, "element type does not match the domain dimension");
+    if (nv <= 0 || ne <= 0 || !xyz || !conn) return fail(NSFV1_ERR_ARGUMENT, "empty mesh");
+    if (nv >= (int64_t(1) << 31) / (h->dim + 1)) return fail(NSFV1_ERR_ARGUMENT, "too many nodes for int32 column indices");
+    const int nsh = nsh_of(et), nc = h->dim + 1;
+    for (int64_t i = 0; i < ne * nsh; ++i)
+        if (conn[i] < 0 || conn[i] >= nv) return fail(NSFV1_ERR_ARGUMENT, "connectivity index out of range");
+
+    // node -> (element, corner) incidences, sorted by node then element
+    std::vector<int64_t> cnt(nv + 1, 0);
+    for (int64_t i = 0; i < ne * nsh; ++i) cnt[conn[i] + 1]++;
+    for (int64_t v = 0; v < nv; ++v) cnt[v + 1] += cnt[v];
+    const int64_t ninc = cnt[nv];
+    if (ninc >= (int64_t(1) << 31)) return fail(NSFV1_ERR_ARGUMENT, "too many incidences");
+    std::vector<int32_t> ie(ninc), in(ninc), ii(ninc);
+    {
+        std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+        for (int64_t e = 0; e < ne; ++e)
+            for (int k = 0; k < nsh; ++k) {
+                const int32_t v = conn[e * nsh + k];
+                const int64_t q = fill[v]++;
+                ie[q] = (int32_t)e;
+                in[q] = v;
+                ii[q] = k | (int32_t)((q - cnt[v]) << 8);
+            }
+    }
+    int maxinc = 0;
+    for (int64_t v = 0; v < nv; ++v) maxinc = std::max<int>(maxinc, (int)(cnt[v + 1] - cnt[v]));
+    if (maxinc > 255 || maxinc > h->max_threads)
+        return fail(NSFV1_ERR_ARGUMENT, "a node has more incident elements than one workgroup can hold");
+
+    // CSR pattern: node adjacency closure (x) (dim+1)^2, sorted columns
+    std::vector<int64_t> rowptr(nv * nc + 1);
+    std::vector<int32_t> deg(nv);
+    {
+        std::vector<int32_t> nb;
+        nb.reserve(256);
+        for (int64_t v = 0; v < nv; ++v) {
+            nb.clear();
+            for (int64_t q = cnt[v]; q < cnt[v + 1]; ++q)
+                for (int k = 0; k < nsh; ++k) nb.push_back(conn[(int64_t)ie[q] * nsh + k]);
+            if (nb.empty()) nb.push_back((int32_t)v);
+            std::sort(nb.begin(), nb.end());
+            deg[v] = (int32_t)(std::unique(nb.begin(), nb.end()) - nb.begin());
+        }
+        rowptr[0] = 0;
+        for (int64_t v = 0; v < nv; ++v)
+            for (int a = 0; a < nc; ++a) rowptr[v * nc + a + 1] = rowptr[v * nc + a] + (int64_t)deg[v] * nc;
+    }
+    const int64_t nnz = rowptr[nv * nc];
+    std::vector<int32_t> colidx;
+    try {
+        colidx.resize(nnz);
+    } catch (...) {
+        return fail(NSFV1_ERR_ARGUMENT, "pattern too large for host memory");
+    }
+    {
+        std::vector<int32_t> nb;
+        nb.reserve(256);
+        for (int64_t v = 0; v < nv; ++v) {
+            nb.clear();
+            for (int64_t q = cnt[v]; q < cnt[v + 1]; ++q)
+                for (int k = 0; k < nsh; ++k) nb.push_back(conn[(int64_t)ie[q] * nsh + k]);
+            if (nb.empty()) nb.push_back((int32_t)v);
+            std::sort(nb.begin(), nb.end());
+            nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+            for (int a = 0; a < nc; ++a) {
+                int32_t* out = colidx.data() + rowptr[v * nc + a];
+                for (size_t t = 0; t < nb.size(); ++t)
+                    for (int b = 0; b < nc; ++b) out[t * nc + b] = nb[t] * nc + b;
+            }
+        }
+    }
+
+    auto up = [](DevBuf& b, const void* src, size_t bytes) -> bool {
+        if (b.alloc(bytes) != hipSuccess) return false;
+        return hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    bool ok = up(h->xyz, xyz, sizeof(double) * nv * h->dim) && up(h->conn, conn, sizeof(int32_t) * ne * nsh) &&
+              up(h->rowptr, rowptr.data(), sizeof(int64_t) * rowptr.size()) &&
+              up(h->colidx, colidx.data(), sizeof(int32_t) * nnz) &&
+              up(h->inc_elem, ie.data(), sizeof(int32_t) * ninc) && up(h->inc_node, in.data(), sizeof(int32_t) * ninc) &&
+              up(h->inc_info, ii.data(), sizeof(int32_t) * ninc) && up(h->node_inc, cnt.data(), sizeof(int64_t) * cnt.size());
+    if (!ok) return fail(NSFV1_ERR_DEVICE, std::string("mesh upload: ") + hipGetErrorString(hipGetLastError()));
+    h->et = et;
+    h->nv = nv;
+    h->ne = ne;
+    h->ndof = nv * nc;
+    h->nnz = nnz;
+    h->ninc = ninc;
+    h->h_rowptr.swap(rowptr);
+    h->h_node_inc.swap(cnt);
+    h->plan32.valid = h->plan64.valid = false;
+    return NSFV1_OK;
+}
+
+int nsfv1_num_dofs(nsfv1_handle* h, int64_t* ndof)
+{
+    if (!h || !ndof) return fail(NSFV1_ERR_ARGUMENT, "bad argument");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    *ndof = h->ndof;
+    return NSFV1_OK;
+}
+
+int nsfv1_nnz(nsfv1_handle* h, int64_t* nnz)
+{
+    if (!h || !nnz) return fail(NSFV1_ERR_ARGUMENT, "bad argument");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    *nnz = h->nnz;
+    return NSFV1_OK;
+}
+
+int nsfv1_get_pattern(nsfv1_handle* h, int64_t* rowptr, int32_t* colidx)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    if (rowptr) std::memcpy(rowptr, h->h_rowptr.data(), sizeof(int64_t) * (h->ndof + 1));
+    if (colidx && hipMemcpy(colidx, h->colidx.p, sizeof(int32_t) * h->nnz, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "pattern copy");
+    return NSFV1_OK;
+}
+
+int nsfv1_pattern_device(nsfv1_handle* h, const int64_t** rowptr, const int32_t** colidx)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    if (rowptr) *rowptr = h->rowptr.as<int64_t>();
+    if (colidx) *colidx = h->colidx.as<int32_t>();
+    return NSFV1_OK;
+}
+
+int nsfv1_set_launch(nsfv1_handle* h, int threads, int lds_bytes)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (threads) {
+        if (threads < 64 || threads > 256 || threads % 64) return fail(NSFV1_ERR_ARGUMENT, "threads must be 64..256, a multiple of 64");
+        h->max_threads = threads;
+    }
+    if (lds_bytes) {
+        if (lds_bytes < 4096 || lds_bytes > 65536) return fail(NSFV1_ERR_ARGUMENT, "LDS budget must be 4..64 KiB");
+        h->lds_cap = (size_t)lds_bytes;
+    }
+    h->plan32.valid = h->plan64.valid = false;
+    return NSFV1_OK;
+}
+
+// Workgroup plan: consecutive nodes while the incidences fit the thread
+// budget and the node rows (plus defect) fit the LDS budget.
+static int make_plan(nsfv1_handle* h, int jbytes, Plan& P)
+{
+    const int nc = h->dim + 1;
+    std::vector<int32_t> wn, wr;
+    wn.push_back(0);
+    int64_t v = 0;
+    size_t maxlds = 0;
+    while (v < h->nv) {
+        const int64_t v0 = v;
+        int rounds = 0;
+        while (v < h->nv) {
+            const int64_t incs = h->h_node_inc[v + 1] - h->h_node_inc[v0];
+            const int64_t vals = h->h_rowptr[(v + 1) * nc] - h->h_rowptr[v0 * nc];
+            const size_t lds = ((size_t)vals * jbytes + 15) / 16 * 16 + (size_t)(v + 1 - v0) * nc * 8;
+            if (v > v0 && (incs > h->max_threads || lds > h->lds_cap)) break;
+            if (lds > h->lds_cap) return fail(NSFV1_ERR_ARGUMENT, "one node row exceeds the LDS budget");
+            rounds = std::max<int>(rounds, (int)(h->h_node_inc[v + 1] - h->h_node_inc[v]));
+            maxlds = std::max(maxlds, lds);
+            ++v;
+        }
+        wn.push_back((int32_t)v);
+        wr.push_back(rounds);
+    }
+    P.nwg = (int)wr.size();
+    P.threads = h->max_threads;
+    P.lds = maxlds;
+    if (P.wg_node.alloc(sizeof(int32_t) * wn.size()) != hipSuccess ||
+        hipMemcpy(P.wg_node.p, wn.data(), sizeof(int32_t) * wn.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        P.wg_rounds.alloc(sizeof(int32_t) * wr.size()) != hipSuccess ||
+        hipMemcpy(P.wg_rounds.p, wr.data(), sizeof(int32_t) * wr.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "plan upload");
+    P.valid = true;
+    return NSFV1_OK;
+}
+
+// checks of prep_elem_loop (navier_stokes_fv1.cpp:136-182)
+static int check_setup(nsfv1_handle* h)
+{
+    if (h->stab_type == STAB_UNSET) return fail(NSFV1_ERR_NO_STABILIZATION, "Stabilization has not been set.");
+    if (!h->stokes) {
+        if (!h->pac && h->conv_upwind == UPW_UNSET)
+            return fail(NSFV1_ERR_NO_UPWIND, "Upwinding for convective Term in Momentum eq. not set.");
+        if (h->stab_upwind == UPW_UNSET)
+            return fail(NSFV1_ERR_NO_UPWIND, "The stabilization has no upwind object (INavierStokesFV1Stabilization::set_upwind).");
+    }
+    if (!h->nu_set)
+        return fail(NSFV1_ERR_NO_VISCOSITY, "NavierStokes::prep_elem_loop: Kinematic Viscosity has not been set, but is required.");
+    return NSFV1_OK;
+}
+
+int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double dt, double s_a, double s_m,
+                   double s_f, int parts, void* jvals, int jprec, double* defect, void* stream)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    if (!u) return fail(NSFV1_ERR_ARGUMENT, "u is NULL");
+    if (jprec != NSFV1_F32 && jprec != NSFV1_F64) return fail(NSFV1_ERR_ARGUMENT, "jprec must be 32 or 64");
+    if (parts & ~7) return fail(NSFV1_ERR_ARGUMENT, "unknown part bits");
+    if (u_old && !(dt > 0.0)) return fail(NSFV1_ERR_TIME_SERIES, "NavierStokes: time step must be positive for the time-dependent stabilization");
+    if (parts & NSFV1_PART_A) {
+        int rc = check_setup(h);
+        if (rc) return rc;
+    }
+    Plan& P = jprec == NSFV1_F64 ? h->plan64 : h->plan32;
+    if (!P.valid) {
+        int rc = make_plan(h, jprec == NSFV1_F64 ? 8 : 4, P);
+        if (rc) return rc;
+    }
+    KParams k{};
+    k.conv_upwind = h->conv_upwind;
+    k.stab_upwind = h->stab_upwind;
+    k.stab_type = h->stab_type;
+    k.pac = h->pac;
+    k.difflen = h->difflen;
+    k.stokes = h->stokes;
+    k.laplace = h->laplace;
+    k.peclet = h->peclet;
+    k.has_source = h->has_source;
+    k.has_old = u_old != nullptr;
+    k.parts = parts;
+    k.exact = h->exact;
+    k.nu = h->nu;
+    k.rho = h->rho;
+    k.dt = dt;
+    for (int i = 0; i < 3; ++i) k.f[i] = h->f[i];
+    k.s_a = s_a;
+    k.s_m = s_m;
+    k.s_f = s_f;
+    KMeshPtrs m{h->xyz.as<double>(), h->conn.as<int32_t>(), h->rowptr.as<int64_t>(), h->colidx.as<int32_t>(),
+                h->inc_elem.as<int32_t>(), h->inc_node.as<int32_t>(), h->inc_info.as<int32_t>(),
+                h->node_inc.as<int64_t>(), P.wg_node.as<int32_t>(), P.wg_rounds.as<int32_t>()};
+    hipError_t e = launch_assemble(h->et, jprec, m, k, u, u_old, jvals, defect, h->errflag.as<int>(), P.nwg, P.threads,
+                                   P.lds, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("assemble launch: ") + hipGetErrorString(e));
+    return NSFV1_OK;
+}
+
+// Synchronises the stream and reports (then clears) errors raised inside
+// the kernels: a skewed/LPS ray that finds no side (upwind.cpp:354) or a
+// degenerate element.
+int nsfv1_sync_errors(nsfv1_handle* h, void* stream)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    int flag = 0;
+    if (hipMemcpyAsync(&flag, h->errflag.p, sizeof(int), hipMemcpyDeviceToHost, (hipStream_t)stream) != hipSuccess ||
+        hipStreamSynchronize((hipStream_t)stream) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "error-flag copy");
+    if (flag) {
+        (void)hipMemset(h->errflag.p, 0, sizeof(int));
+        if (flag & KERR_CUT_SIDE) return fail(NSFV1_ERR_CUT_SIDE, "GetNodeNextToCut: Cannot find cut side.");
+        return fail(NSFV1_ERR_DEGENERATE, "degenerate element");
+    }
+    return NSFV1_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,175 @@
+"""Synthetic seeded meshes and states for the lid-driven cavity and channel
+workloads (input generation; the assembly itself lives in the HIP library).
+
+Element corner order follows the UG4 reference elements: triangles and quads
+counter-clockwise from the lower-left corner, tetrahedra (0,0,0),(1,0,0),
+(0,1,0),(0,0,1), hexahedra bottom face counter-clockwise then the top face.
+Node numbering is lexicographic (x fastest); DoF = node*(dim+1)+component.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+TRI, QUAD, TET, HEX = 0, 1, 2, 3
+ELEM_NAMES = {TRI: "triangle", QUAD: "quadrilateral", TET: "tetrahedron", HEX: "hexahedron"}
+DIM = {TRI: 2, QUAD: 2, TET: 3, HEX: 3}
+NSH = {TRI: 3, QUAD: 4, TET: 4, HEX: 8}
+
+
+@dataclass
+class Mesh:
+    et: int
+    xyz: np.ndarray         # [nv, dim] float64
+    conn: np.ndarray        # [ne, nsh] int32
+    boundary: dict          # name -> node index array
+    shape: tuple = ()       # structured sizes, informational
+
+    @property
+    def dim(self) -> int:
+        return DIM[self.et]
+
+    @property
+    def nv(self) -> int:
+        return self.xyz.shape[0]
+
+    @property
+    def ne(self) -> int:
+        return self.conn.shape[0]
+
+
+def _jitter(xyz: np.ndarray, interior: np.ndarray, h: float, amount: float, seed: int) -> np.ndarray:
+    if amount <= 0:
+        return xyz
+    rng = np.random.default_rng(seed)
+    d = rng.uniform(-amount * h, amount * h, size=xyz.shape)
+    out = xyz.copy()
+    out[interior] += d[interior]
+    return out
+
+
+def cavity_2d(n: int, et: int = QUAD, jitter: float = 0.0, seed: int = 0, lx: float = 1.0) -> Mesh:
+    """Unit square, n x n cells; quads, or each square split into 2 triangles
+    along the (i,j)-(i+1,j+1) diagonal.  Interior nodes jittered uniformly
+    within +-jitter*h per coordinate."""
+    h = lx / n
+    i, j = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="xy")
+    xyz = np.stack([i.ravel() * h, j.ravel() * h], axis=1).astype(np.float64)
+    v = lambda a, b: b * (n + 1) + a
+    ci, cj = np.meshgrid(np.arange(n), np.arange(n), indexing="xy")
+    ci, cj = ci.ravel(), cj.ravel()
+    v00, v10, v11, v01 = v(ci, cj), v(ci + 1, cj), v(ci + 1, cj + 1), v(ci, cj + 1)
+    if et == QUAD:
+        conn = np.stack([v00, v10, v11, v01], axis=1)
+    elif et == TRI:
+        t1 = np.stack([v00, v10, v11], axis=1)
+        t2 = np.stack([v00, v11, v01], axis=1)
+        conn = np.stack([t1, t2], axis=1).reshape(-1, 3)
+    else:
+        raise ValueError("2D cavity needs TRI or QUAD")
+    ii, jj = i.ravel(), j.ravel()
+    interior = (ii > 0) & (ii < n) & (jj > 0) & (jj < n)
+    xyz = _jitter(xyz, interior, h, jitter, seed)
+    lid = np.nonzero(jj == n)[0]
+    wall = np.nonzero(((ii == 0) | (ii == n) | (jj == 0)) & (jj < n))[0]
+    return Mesh(et, xyz, conn.astype(np.int32), {"lid": lid, "wall": wall}, (n, n))
+
+
+def cavity_3d(n: int, et: int = HEX, jitter: float = 0.0, seed: int = 0) -> Mesh:
+    """Unit cube, n^3 hexahedra (or 6 Kuhn tetrahedra per hexahedron)."""
+    return box_3d((n, n, n), (1.0, 1.0, 1.0), et=et, jitter=jitter, seed=seed, kind="cavity")
+
+
+# Kuhn split of the unit cube into 6 tetrahedra sharing the (0,0,0)-(1,1,1)
+# diagonal; each listed with positive orientation.
+_KUHN = [
+    (0, 1, 2, 6), (0, 2, 3, 6), (0, 3, 7, 6), (0, 7, 4, 6), (0, 4, 5, 6), (0, 5, 1, 6),
+]
+
+
+def box_3d(cells, lengths, et: int = HEX, jitter: float = 0.0, seed: int = 0, kind: str = "cavity",
+           keep_cell=None) -> Mesh:
+    nx, ny, nz = cells
+    hx, hy, hz = lengths[0] / nx, lengths[1] / ny, lengths[2] / nz
+    k, j, i = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    ii, jj, kk = i.ravel(), j.ravel(), k.ravel()
+    xyz = np.stack([ii * hx, jj * hy, kk * hz], axis=1).astype(np.float64)
+    v = lambda a, b, c: (c * (ny + 1) + b) * (nx + 1) + a
+    ck, cj, ci = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ci, cj, ck = ci.ravel(), cj.ravel(), ck.ravel()
+    if keep_cell is not None:
+        m = keep_cell(ci, cj, ck)
+        ci, cj, ck = ci[m], cj[m], ck[m]
+    hexc = np.stack([v(ci, cj, ck), v(ci + 1, cj, ck), v(ci + 1, cj + 1, ck), v(ci, cj + 1, ck),
+                     v(ci, cj, ck + 1), v(ci + 1, cj, ck + 1), v(ci + 1, cj + 1, ck + 1), v(ci, cj + 1, ck + 1)],
+                    axis=1)
+    if et == HEX:
+        conn = hexc
+    elif et == TET:
+        conn = np.stack([hexc[:, list(t)] for t in _KUHN], axis=1).reshape(-1, 4)
+    else:
+        raise ValueError("3D box needs TET or HEX")
+    interior = (ii > 0) & (ii < nx) & (jj > 0) & (jj < ny) & (kk > 0) & (kk < nz)
+    xyz = _jitter(xyz, interior, min(hx, hy, hz), jitter, seed)
+    if kind == "cavity":
+        lid = np.nonzero(kk == nz)[0]
+        wall = np.nonzero(((ii == 0) | (ii == nx) | (jj == 0) | (jj == ny) | (kk == 0)) & (kk < nz))[0]
+        bnd = {"lid": lid, "wall": wall}
+    else:
+        inflow = np.nonzero(ii == 0)[0]
+        outflow = np.nonzero(ii == nx)[0]
+        wall = np.nonzero(((jj == 0) | (jj == ny) | (kk == 0) | (kk == nz)) & (ii > 0) & (ii < nx))[0]
+        bnd = {"inflow": inflow, "outflow": outflow, "wall": wall}
+    used = np.zeros(xyz.shape[0], dtype=bool)
+    used[conn.ravel()] = True
+    bnd["unused"] = np.nonzero(~used)[0]
+    return Mesh(et, xyz, conn.astype(np.int32), bnd, (nx, ny, nz))
+
+
+def random_state(mesh: Mesh, seed: int = 1, scale: float = 1.0) -> np.ndarray:
+    """Seeded random nodal state [nv, dim+1] (velocity, pressure)."""
+    rng = np.random.default_rng(seed)
+    return scale * rng.uniform(-1.0, 1.0, size=(mesh.nv, mesh.dim + 1))
+
+
+def divergence_free_perturbation(mesh: Mesh, amplitude: float = 1e-3, seed: int = 0, modes: int = 3) -> np.ndarray:
+    """Velocity = curl of a seeded smooth vector potential that vanishes on the
+    unit-cube boundary; pressure zero.  Returns [nv, 4]."""
+    rng = np.random.default_rng(seed)
+    x, y, z = mesh.xyz[:, 0], mesh.xyz[:, 1], mesh.xyz[:, 2]
+    u = np.zeros((mesh.nv, 4))
+    for _ in range(modes):
+        a = rng.uniform(-1, 1, size=3)
+        kx, ky, kz = rng.integers(1, 4, size=3)
+        # psi_c = a_c * B(x,y,z) with B = sin^2 products (zero value and gradient on the boundary)
+        sx, sy, sz = np.sin(np.pi * kx * x), np.sin(np.pi * ky * y), np.sin(np.pi * kz * z)
+        dsx = 2 * np.pi * kx * sx * np.cos(np.pi * kx * x)
+        dsy = 2 * np.pi * ky * sy * np.cos(np.pi * ky * y)
+        dsz = 2 * np.pi * kz * sz * np.cos(np.pi * kz * z)
+        B = sx**2 * sy**2 * sz**2
+        Bx, By, Bz = dsx * sy**2 * sz**2, sx**2 * dsy * sz**2, sx**2 * sy**2 * dsz
+        # curl(a B) = grad B x a
+        gx, gy, gz = Bx, By, Bz
+        u[:, 0] += gy * a[2] - gz * a[1]
+        u[:, 1] += gz * a[0] - gx * a[2]
+        u[:, 2] += gx * a[1] - gy * a[0]
+    m = np.abs(u[:, :3]).max()
+    if m > 0:
+        u[:, :3] *= amplitude / m
+    return u
+
+
+def cavity_lid_state(mesh: Mesh, lid_speed: float = 1.0, base: np.ndarray | None = None) -> np.ndarray:
+    """Apply the regularised lid (u = lid_speed along x on lid nodes except the
+    lid's boundary edges, where u = 0) and no-slip walls to a state."""
+    u = np.zeros((mesh.nv, mesh.dim + 1)) if base is None else base.copy()
+    u[mesh.boundary["wall"], : mesh.dim] = 0.0
+    lid = mesh.boundary["lid"]
+    u[lid, : mesh.dim] = 0.0
+    x = mesh.xyz[lid]
+    inner = np.ones(len(lid), dtype=bool)
+    for c in range(mesh.dim - 1):
+        inner &= (x[:, c] > 1e-12) & (x[:, c] < 1 - 1e-12)
+    u[lid[inner], 0] = lid_speed
+    return u

@@ -1,0 +1,167 @@
+"""GPU parity: the HIP assembly (through the C ABI) against the CPU oracle on
+the same seeded meshes and states, every element type, upwind,
+stabilisation and flag of the reference path
+(incompressible/fv1/navier_stokes_fv1.cpp:250-869, upwind.cpp:52-575,
+incompressible/fv1/stabilization.cpp:102-242,805-850,
+incompressible/fv1/diffusion_length.h:48-199)."""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle as O
+from plugin_navierstokes_amd import NavierStokesFV1, NSFV1Error, PART_A, PART_M, PART_RHS, F32, F64
+from plugin_navierstokes_amd import meshes as M
+from tests._parity import (configure_disc, fp32_violation, fp64_violation, max_rel_err, oracle_params,
+                           small_mesh, worst, FP32_REL, FP32_ABS)
+
+pytestmark = pytest.mark.gpu
+
+ETS = [M.TRI, M.QUAD, M.TET, M.HEX]
+ET_NAMES = {M.TRI: "tri", M.QUAD: "quad", M.TET: "tet", M.HEX: "hex"}
+
+
+def _run(cuda, et, cfg, mesh=None, seed=5, time_dep=False, parts=PART_A, s=(1.0, 0.0, 0.0), jprec=F64):
+    import torch
+
+    mesh = mesh or small_mesh(et)
+    dim = mesh.dim
+    u = M.random_state(mesh, seed)
+    uold = M.random_state(mesh, seed + 100) if time_dep else None
+    dt = 0.01 if time_dep else 0.0
+    disc = NavierStokesFV1("u,v,p" if dim == 2 else "u,v,w,p", "Inner")
+    configure_disc(disc, cfg)
+    disc.set_mesh(mesh)
+    ut = torch.from_numpy(u).to(cuda)
+    uot = torch.from_numpy(uold).to(cuda) if time_dep else None
+    J, d = disc.assemble(ut, uot, dt, s_a=s[0], s_m=s[1], s_f=s[2], parts=parts, jprec=jprec)
+    rowptr, colidx = disc.pattern()
+    p = oracle_params(dim, cfg)
+    Jr, dr = O.assemble(p, et, mesh.xyz, mesh.conn, u, uold, dt, s_a=s[0], s_m=s[1], s_f=s[2], parts=parts,
+                        rowptr=rowptr, colidx=colidx)
+    Ja, da = O.assemble_abs(p, et, mesh.xyz, mesh.conn, u, uold, dt, s_a=s[0], s_m=s[1], parts=parts,
+                            rowptr=rowptr, colidx=colidx)
+    if parts & PART_RHS and cfg.get("source") is not None:
+        da = da + np.abs(dr)
+    return J.cpu().numpy(), d.cpu().numpy(), Jr, dr, Ja, da, disc
+
+
+@pytest.mark.parametrize("et", ETS, ids=lambda e: ET_NAMES[e])
+def test_pattern_bitexact(cuda, et):
+    mesh = small_mesh(et)
+    disc = NavierStokesFV1("u,v,p" if mesh.dim == 2 else "u,v,w,p", "Inner")
+    disc.set_mesh(mesh)
+    r, c = disc.pattern()
+    r0, c0 = O.pattern(et, mesh.nv, mesh.conn)
+    assert np.array_equal(r, r0) and np.array_equal(c, c0)
+
+
+CASES = []
+for up in ["no", "full", "skewed", "lps"]:
+    CASES.append({"upwind": up})
+CASES += [
+    {"upwind": "full", "difflen": "fivepoint"},
+    {"upwind": "skewed", "difflen": "cor"},
+    {"upwind": "lps", "stab": "without"},
+    {"upwind": "full", "stokes": True},
+    {"upwind": "skewed", "laplace": True},
+    {"upwind": "full", "peclet": True},
+    {"upwind": "lps", "exact": 1.0},
+    {"upwind": "full", "exact": 0.5, "peclet": True},
+    {"upwind": "skewed", "pac": True},
+    {"upwind": "full", "pac": True, "exact": 1.0},
+    {"upwind": "full", "stab_upwind": "skewed"},
+    {"upwind": "lps", "rho": 1.7, "nu": 0.003, "source": (0.3, -0.2, 0.1)},
+]
+
+
+def _case_id(c):
+    return "-".join(f"{k}={v}" for k, v in c.items())
+
+
+@pytest.mark.parametrize("cfg", CASES, ids=_case_id)
+@pytest.mark.parametrize("et", ETS, ids=lambda e: ET_NAMES[e])
+def test_A_part_fp64(cuda, et, cfg):
+    print(f"case et={ET_NAMES[et]} cfg={cfg}", flush=True)
+    J, d, Jr, dr, Ja, da, _ = _run(cuda, et, cfg)
+    assert fp64_violation(J, Jr, Ja) <= 1.0, worst(J, Jr, Ja)
+    assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+
+
+@pytest.mark.parametrize("et", ETS, ids=lambda e: ET_NAMES[e])
+@pytest.mark.parametrize("up", ["full", "skewed", "lps"])
+def test_time_dependent_fp64(cuda, et, up):
+    cfg = {"upwind": up, "source": (0.1, 0.2, -0.3)}
+    J, d, Jr, dr, Ja, da, _ = _run(cuda, et, cfg, time_dep=True, parts=PART_A | PART_M | PART_RHS,
+                                   s=(0.7, 1.3, -0.7))
+    assert fp64_violation(J, Jr, Ja) <= 1.0, worst(J, Jr, Ja)
+    assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+
+
+@pytest.mark.parametrize("et", ETS, ids=lambda e: ET_NAMES[e])
+def test_mass_and_rhs_parts(cuda, et):
+    cfg = {"upwind": "full", "source": (1.0, -2.0, 0.5), "rho": 1.3}
+    for parts, s in [(PART_M, (0.0, 1.0, 0.0)), (PART_RHS, (0.0, 0.0, 1.0))]:
+        J, d, Jr, dr, Ja, da, _ = _run(cuda, et, cfg, parts=parts, s=s)
+        assert fp64_violation(J, Jr, Ja) <= 1.0, worst(J, Jr, Ja)
+        assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+
+
+@pytest.mark.parametrize("et", ETS, ids=lambda e: ET_NAMES[e])
+@pytest.mark.parametrize("up", ["full", "skewed", "lps"])
+def test_fp32_jacobian_bound(cuda, et, up):
+    J, d, Jr, dr, Ja, da, _ = _run(cuda, et, {"upwind": up}, jprec=F32)
+    assert J.dtype == np.float32
+    assert fp32_violation(J, Jr, Ja) <= 1.0, worst(J, Jr, Ja, FP32_REL, FP32_ABS)
+    assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+
+
+@pytest.mark.parametrize("et", ETS, ids=lambda e: ET_NAMES[e])
+def test_bitwise_repeatable(cuda, et):
+    import torch
+
+    mesh = small_mesh(et, n=6)
+    disc = NavierStokesFV1("u,v,p" if mesh.dim == 2 else "u,v,w,p", "Inner")
+    configure_disc(disc, {"upwind": "skewed"})
+    disc.set_mesh(mesh)
+    u = torch.from_numpy(M.random_state(mesh, 9)).to(cuda)
+    J1, d1 = disc.assemble(u, jprec=F32)
+    J2, d2 = disc.assemble(u, jprec=F32)
+    assert torch.equal(J1, J2) and torch.equal(d1, d2)
+
+
+def test_cavity_configs_medium(cuda):
+    """Jittered lid-driven cavity states of the 2D configurations."""
+    import torch
+
+    for et, up, n in [(M.QUAD, "full", 16), (M.TRI, "lps", 16)]:
+        mesh = M.cavity_2d(n, et, 0.1, seed=11)
+        u = M.cavity_lid_state(mesh, 1.0, base=0.2 * M.random_state(mesh, 3))
+        cfg = {"upwind": up, "nu": 1e-2 if et == M.QUAD else 1e-3}
+        J, d, Jr, dr, Ja, da, _ = _run(cuda, et, cfg, mesh=mesh)
+        assert fp64_violation(J, Jr, Ja) <= 1.0, worst(J, Jr, Ja)
+        assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+
+
+def test_error_cases(cuda):
+    import torch
+
+    mesh = small_mesh(M.QUAD)
+    disc = NavierStokesFV1("u,v,p", "Inner")
+    disc.set_mesh(mesh)
+    u = torch.zeros(mesh.nv, 3, dtype=torch.float64, device=cuda)
+    with pytest.raises(NSFV1Error) as e:
+        disc.assemble(u)
+    assert e.value.name == "NO_STABILIZATION"
+    disc.set_stabilization("fields")
+    with pytest.raises(NSFV1Error) as e:
+        disc.assemble(u)
+    assert e.value.name == "NO_UPWIND"
+    disc.set_upwind("full")
+    with pytest.raises(NSFV1Error) as e:
+        disc.assemble(u)
+    assert e.value.name == "NO_VISCOSITY"
+    disc.set_kinematic_viscosity(0.1)
+    disc.assemble(u)
