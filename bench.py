@@ -1,0 +1,158 @@
+"""Benchmark of the FV1 Navier-Stokes assembly hot path on one MI355X.
+
+Workload: 3D lid-driven cavity on the unit cube, 128^3 Q1 hexahedra
+(2,097,152 elements, 2,146,689 nodes), Re = 400 (nu = 2.5e-3), skewed upwind
++ FIELDS stabilisation (RAW diffusion length), time-dependent stabilisation
+with the previous time level (dt = 0.01), FP32 Jacobian + FP64 defect.  One
+step = one whole-mesh Jacobian + defect assembly with the state resident in
+HBM.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "elem/s Jacobian+defect assembly; s/timestep learned vs plain at 1e-8 residual"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(nv, ne, nnz, dim, nsh, jbytes, T):
+    """Compulsory HBM bytes of one assembly: coordinates, T solution levels,
+    connectivity, Jacobian values, defect (CSR indices excluded)."""
+    return nv * dim * 8 + T * nv * (dim + 1) * 8 + ne * nsh * 4 + nnz * jbytes + nv * (dim + 1) * 8
+
+
+def cpu_baseline(n_full: int, target_s: float = 10.0):
+    """Time the CPU oracle (1 thread) on a sub-block of the same cavity: the
+    same element shape, state construction and configuration."""
+    import oracle as O
+    from plugin_navierstokes_amd import meshes as M
+
+    def run(m):
+        mesh = M.cavity_3d(m, M.HEX)
+        u = M.cavity_lid_state(mesh, 1.0, base=M.divergence_free_perturbation(mesh, 1e-3, seed=0))
+        uold = 0.999 * u
+        p = O.Params(dim=3, upwind="skewed", nu=1.0 / 400.0)
+        rp, ci = O.pattern(O.HEX, mesh.nv, mesh.conn)
+        t0 = time.perf_counter()
+        O.assemble(p, O.HEX, mesh.xyz, mesh.conn, u, uold, 0.01, parts=1, rowptr=rp, colidx=ci)
+        return time.perf_counter() - t0, mesh.ne
+
+    t, ne = run(8)
+    rate = ne / t
+    m = int(max(8, min(64, round((target_s * rate) ** (1.0 / 3.0)))))
+    t, ne = run(m)
+    return {"value": ne / t, "unit": "elem/s", "cores": 1, "kind": "port",
+            "sample": f"CPU oracle (C, -O2, 1 thread), full FP64 Jacobian+defect assembly of a {m}^3-hexahedron "
+                      f"cavity ({ne} elements, same element size class, state and configuration), {t:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (rocprof PMC byte calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only; the multi-GPU sharding of the assembly is "
+              "described in DESIGN.md and is not run.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from plugin_navierstokes_amd import F32, NavierStokesFV1, PART_A
+    from plugin_navierstokes_amd import meshes as M
+
+    dev = torch.device("cuda:0")
+    n = args.n
+    t0 = time.perf_counter()
+    mesh = M.cavity_3d(n, M.HEX)
+    u = M.cavity_lid_state(mesh, 1.0, base=M.divergence_free_perturbation(mesh, 1e-3, seed=0))
+    uold = 0.999 * u
+    disc = NavierStokesFV1("u,v,w,p", "Inner")
+    disc.set_stabilization("fields")
+    disc.set_upwind("skewed")
+    disc.set_kinematic_viscosity(1.0 / 400.0)
+    disc.set_mesh(mesh)
+    print(f"bench: mesh {n}^3 hex, nv={mesh.nv} ne={mesh.ne} nnz={disc.nnz()} setup {time.perf_counter()-t0:.1f}s",
+          file=sys.stderr, flush=True)
+    ut = torch.from_numpy(u).to(dev)
+    uot = torch.from_numpy(uold).to(dev)
+    J = torch.empty(disc.nnz(), dtype=torch.float32, device=dev)
+    d = torch.empty(disc.num_dofs(), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+
+    if args.copy_calibration:
+        a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+
+    def step():
+        disc.assemble(ut, uot, 0.01, parts=PART_A, jprec=F32, J=J, d=d, stream=stream, check=False)
+
+    for _ in range(args.warmup):
+        step()
+    disc.sync_errors(stream)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s, e in ev:
+        s.record(stream)
+        step()
+        e.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    disc.sync_errors(stream)
+    kms = [s.elapsed_time(e) for s, e in ev]
+    ms_step = wall * 1000.0 / args.steps
+    kavg = float(np.mean(kms))
+    nnz = disc.nnz()
+    B = algorithmic_bytes(mesh.nv, mesh.ne, nnz, 3, 8, 4, 2)
+    achieved = B / (kavg * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": mesh.ne / (ms_step * 1e-3),
+        "unit": "elem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (Jacobian stored f32)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"3D lid-driven cavity, {n}^3 Q1 hexahedra, Re=400, skewed upwind + FIELDS (RAW), "
+                        f"time-dependent stabilisation (u_old, dt=0.01), FP32 Jacobian + FP64 defect",
+            "elements": mesh.ne, "nodes": mesh.nv, "nnz": nnz, "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "nsfv1::assemble_rows<3, float>", "kernel_ms": kavg, "algorithmic_bytes": B,
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
