@@ -162,7 +162,8 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
    element with a zero Jacobian determinant. */
 int nsfv1_sync_errors(nsfv1_handle* h, void* stream);
 
-/* Launch shape of the assembly (threads per workgroup, LDS budget in bytes);
+/* Launch shape of the assembly: incidences (threads) per workgroup (64..256,
+   a multiple of 64) and the LDS budget per workgroup in bytes (4..64 KiB);
    0 keeps the current value.  Used by the autotuner. */
 int nsfv1_set_launch(nsfv1_handle* h, int threads, int lds_bytes);
 

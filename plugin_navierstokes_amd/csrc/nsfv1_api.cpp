@@ -39,63 +39,6 @@ std::string trim_lower(const char* s)
     return n;
 }
 
-// ---------------------------------------------------------------------
-// reference-element topology (UG4 numbering) and the derived FV1 tables
-// ---------------------------------------------------------------------
-struct TopoSrc {
-    int dim, nsh, nip, nside;
-    double lc[8][3];
-    int edge[12][2];
-    int side_n[6];
-    int side[6][4];
-};
-
-const TopoSrc kTopo[4] = {
-    {2, 3, 3, 3, {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}}, {{0, 1}, {1, 2}, {2, 0}}, {2, 2, 2}, {{0, 1}, {1, 2}, {2, 0}}},
-    {2, 4, 4, 4, {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}}, {{0, 1}, {1, 2}, {2, 3}, {3, 0}}, {2, 2, 2, 2},
-     {{0, 1}, {1, 2}, {2, 3}, {3, 0}}},
-    {3, 4, 6, 4, {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}}, {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}},
-     {3, 3, 3, 3}, {{0, 2, 1}, {1, 2, 3}, {0, 3, 2}, {0, 1, 3}}},
-    {3, 8, 12, 6, {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}},
-     {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 4}, {1, 5}, {2, 6}, {3, 7}, {4, 5}, {5, 6}, {6, 7}, {7, 4}},
-     {4, 4, 4, 4, 4, 4}, {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {0, 4, 7, 3}, {4, 5, 6, 7}}},
-};
-
-void build_topology(RefTopo* out)
-{
-    std::memset(out, 0, sizeof(RefTopo) * 4);
-    for (int et = 0; et < 4; ++et) {
-        const TopoSrc& S = kTopo[et];
-        RefTopo& T = out[et];
-        for (int i = 0; i < S.nip; ++i) { T.edge[i][0] = (int8_t)S.edge[i][0]; T.edge[i][1] = (int8_t)S.edge[i][1]; }
-        for (int s = 0; s < S.nside; ++s) {
-            T.side_n[s] = (int8_t)S.side_n[s];
-            for (int j = 0; j < S.side_n[s]; ++j) T.side[s][j] = (int8_t)S.side[s][j];
-        }
-        for (int k = 0; k < S.nsh; ++k)
-            for (int i = 0; i < 3; ++i) T.lc[k][i] = S.lc[k][i];
-        auto has = [&](int s, int c) {
-            for (int j = 0; j < S.side_n[s]; ++j)
-                if (S.side[s][j] == c) return true;
-            return false;
-        };
-        for (int e = 0; e < S.nip; ++e) {
-            int m = 0;
-            T.ef[e][0] = T.ef[e][1] = -1;
-            if (S.dim == 3)
-                for (int s = 0; s < S.nside && m < 2; ++s)
-                    if (has(s, S.edge[e][0]) && has(s, S.edge[e][1])) T.ef[e][m++] = (int8_t)s;
-        }
-        for (int k = 0; k < 8; ++k)
-            for (int q = 0; q < 3; ++q) T.cip[k][q] = -1;
-        for (int k = 0; k < S.nsh; ++k) {
-            int m = 0;
-            for (int e = 0; e < S.nip; ++e)
-                if (S.edge[e][0] == k || S.edge[e][1] == k) T.cip[k][m++] = (int8_t)e;
-        }
-    }
-}
-
 struct DevBuf {
     void* p = nullptr;
     size_t n = 0;
@@ -117,7 +60,7 @@ struct DevBuf {
 
 struct Plan {
     bool valid = false;
-    int threads = 0, nwg = 0;
+    int nwg = 0;
     size_t lds = 0;
     DevBuf wg_node, wg_rounds;
 };
@@ -150,12 +93,13 @@ struct nsfv1_handle {
     // mesh
     int et = -1;
     int64_t nv = 0, ne = 0, ndof = 0, nnz = 0, ninc = 0;
-    DevBuf xyz, conn, rowptr, colidx, inc_elem, inc_node, inc_info, node_inc, errflag;
+    DevBuf xyz, conn, rowptr, colidx, errflag, l2inv, vol, node_inc, inc_elem, inc_node, inc_info;
     std::vector<int64_t> h_rowptr, h_node_inc;
-    std::vector<int32_t> h_colidx_unused;
     int max_threads = 256;
-    size_t lds_cap = 65536;
+    size_t lds_cap = 64 * 1024;
     Plan plan32, plan64;
+    bool geo_valid = false;
+    int geo_difflen = -1;
 };
 
 extern "C" {
@@ -185,16 +129,8 @@ int nsfv1_create(int dim, const char* functions, const char* subsets, nsfv1_hand
     if ((int)fct.size() != dim + 1)
         return fail(NSFV1_ERR_FUNCTION_COUNT, "Wrong number of functions: The ElemDisc 'NavierStokes' needs exactly " +
                                                   std::to_string(dim + 1) + " symbolic function.");
-    static bool topo_done_dev[64] = {false};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "no HIP device");
-    if (dev >= 0 && dev < 64 && !topo_done_dev[dev]) {
-        RefTopo topo[4];
-        build_topology(topo);
-        hipError_t e = upload_topology(topo);
-        if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("topology upload: ") + hipGetErrorString(e));
-        topo_done_dev[dev] = true;
-    }
     auto h = std::make_unique<nsfv1_handle>();
     h->dim = dim;
     h->device = dev;
@@ -461,6 +397,7 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
         }
     }
 
+    std::vector<int32_t> ni32(cnt.begin(), cnt.end());
     auto up = [](DevBuf& b, const void* src, size_t bytes) -> bool {
         if (b.alloc(bytes) != hipSuccess) return false;
         return hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
@@ -469,7 +406,10 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
               up(h->rowptr, rowptr.data(), sizeof(int64_t) * rowptr.size()) &&
               up(h->colidx, colidx.data(), sizeof(int32_t) * nnz) &&
               up(h->inc_elem, ie.data(), sizeof(int32_t) * ninc) && up(h->inc_node, in.data(), sizeof(int32_t) * ninc) &&
-              up(h->inc_info, ii.data(), sizeof(int32_t) * ninc) && up(h->node_inc, cnt.data(), sizeof(int64_t) * cnt.size());
+              up(h->inc_info, ii.data(), sizeof(int32_t) * ninc) &&
+              up(h->node_inc, ni32.data(), sizeof(int32_t) * ni32.size()) &&
+              h->l2inv.alloc(sizeof(double) * ne * (et == ET_HEX ? 12 : et == ET_TET ? 6 : et == ET_QUAD ? 4 : 3)) == hipSuccess &&
+              h->vol.alloc(sizeof(double) * ne * nsh) == hipSuccess;
     if (!ok) return fail(NSFV1_ERR_DEVICE, std::string("mesh upload: ") + hipGetErrorString(hipGetLastError()));
     h->et = et;
     h->nv = nv;
@@ -480,6 +420,7 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
     h->h_rowptr.swap(rowptr);
     h->h_node_inc.swap(cnt);
     h->plan32.valid = h->plan64.valid = false;
+    h->geo_valid = false;
     return NSFV1_OK;
 }
 
@@ -533,8 +474,8 @@ int nsfv1_set_launch(nsfv1_handle* h, int threads, int lds_bytes)
     return NSFV1_OK;
 }
 
-// Workgroup plan: consecutive nodes while the incidences fit the thread
-// budget and the node rows (plus defect) fit the LDS budget.
+// Workgroup plan: consecutive nodes while their incidences fit the thread
+// budget and their rows (plus defect) fit the LDS budget (<= 64 KiB).
 static int make_plan(nsfv1_handle* h, int jbytes, Plan& P)
 {
     const int nc = h->dim + 1;
@@ -549,8 +490,10 @@ static int make_plan(nsfv1_handle* h, int jbytes, Plan& P)
             const int64_t incs = h->h_node_inc[v + 1] - h->h_node_inc[v0];
             const int64_t vals = h->h_rowptr[(v + 1) * nc] - h->h_rowptr[v0 * nc];
             const size_t lds = ((size_t)vals * jbytes + 15) / 16 * 16 + (size_t)(v + 1 - v0) * nc * 8;
-            if (v > v0 && (incs > h->max_threads || lds > h->lds_cap)) break;
-            if (lds > h->lds_cap) return fail(NSFV1_ERR_ARGUMENT, "one node row exceeds the LDS budget");
+            if (incs > h->max_threads || lds > h->lds_cap) {
+                if (v == v0) return fail(NSFV1_ERR_ARGUMENT, "one node row exceeds the workgroup budget");
+                break;
+            }
             rounds = std::max<int>(rounds, (int)(h->h_node_inc[v + 1] - h->h_node_inc[v]));
             maxlds = std::max(maxlds, lds);
             ++v;
@@ -559,13 +502,12 @@ static int make_plan(nsfv1_handle* h, int jbytes, Plan& P)
         wr.push_back(rounds);
     }
     P.nwg = (int)wr.size();
-    P.threads = h->max_threads;
     P.lds = maxlds;
-    if (P.wg_node.alloc(sizeof(int32_t) * wn.size()) != hipSuccess ||
-        hipMemcpy(P.wg_node.p, wn.data(), sizeof(int32_t) * wn.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        P.wg_rounds.alloc(sizeof(int32_t) * wr.size()) != hipSuccess ||
-        hipMemcpy(P.wg_rounds.p, wr.data(), sizeof(int32_t) * wr.size(), hipMemcpyHostToDevice) != hipSuccess)
-        return fail(NSFV1_ERR_DEVICE, "plan upload");
+    auto up = [](DevBuf& b, const std::vector<int32_t>& x) {
+        return b.alloc(sizeof(int32_t) * std::max<size_t>(x.size(), 1)) == hipSuccess &&
+               (x.empty() || hipMemcpy(b.p, x.data(), sizeof(int32_t) * x.size(), hipMemcpyHostToDevice) == hipSuccess);
+    };
+    if (!up(P.wg_node, wn) || !up(P.wg_rounds, wr)) return fail(NSFV1_ERR_DEVICE, "plan upload");
     P.valid = true;
     return NSFV1_OK;
 }
@@ -623,11 +565,20 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
     k.s_a = s_a;
     k.s_m = s_m;
     k.s_f = s_f;
+    if (!h->geo_valid || h->geo_difflen != h->difflen) {
+        hipError_t ge = launch_geometry_cache(h->et, h->xyz.as<double>(), h->conn.as<int32_t>(), h->ne, h->difflen,
+                                              h->l2inv.as<double>(), h->vol.as<double>(), h->errflag.as<int>(),
+                                              (hipStream_t)stream);
+        if (ge != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("geometry cache: ") + hipGetErrorString(ge));
+        h->geo_valid = true;
+        h->geo_difflen = h->difflen;
+    }
     KMeshPtrs m{h->xyz.as<double>(), h->conn.as<int32_t>(), h->rowptr.as<int64_t>(), h->colidx.as<int32_t>(),
-                h->inc_elem.as<int32_t>(), h->inc_node.as<int32_t>(), h->inc_info.as<int32_t>(),
-                h->node_inc.as<int64_t>(), P.wg_node.as<int32_t>(), P.wg_rounds.as<int32_t>()};
-    hipError_t e = launch_assemble(h->et, jprec, m, k, u, u_old, jvals, defect, h->errflag.as<int>(), P.nwg, P.threads,
-                                   P.lds, (hipStream_t)stream);
+                h->node_inc.as<int32_t>(), h->inc_elem.as<int32_t>(), h->inc_node.as<int32_t>(),
+                h->inc_info.as<int32_t>(), P.wg_node.as<int32_t>(), P.wg_rounds.as<int32_t>(), h->l2inv.as<double>(),
+                h->vol.as<double>()};
+    hipError_t e = launch_assemble(h->et, jprec, m, k, u, u_old, jvals, defect, h->errflag.as<int>(), P.nwg, P.lds,
+                                   (hipStream_t)stream);
     if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("assemble launch: ") + hipGetErrorString(e));
     return NSFV1_OK;
 }

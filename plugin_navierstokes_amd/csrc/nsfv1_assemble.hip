@@ -1,420 +1,1015 @@
 // nsfv1_assemble.hip -- row-owner gather assembly of the FV1 incompressible
 // Navier-Stokes defect and Jacobian (gfx950).
 //
-// One thread per (row node v, incident element e) incidence.  The thread
-// forms the sub-control-volume faces of e that touch v (3 for hexahedra and
-// tetrahedra, 2 for 2D elements), evaluates the upwind, the FIELDS
-// stabilisation and the flux linearisation of navier_stokes_fv1.cpp:250-778
-// for those faces only, and holds the element's contribution to v's block
-// row: (dim+1) x nsh x (dim+1) Jacobian values and dim+1 defect values.
-// A workgroup owns a contiguous range of nodes, hence a contiguous range of
-// CSR values; it accumulates them in LDS in rounds ordered by the
-// incidence slot (no two threads of one round share a row, so the sum
-// order is fixed and the result is bitwise reproducible, without atomics)
-// and streams the finished rows to HBM with coalesced stores.
+// Work decomposition.  Every SCVF of an element adds its flux to exactly two
+// rows (from, to: navier_stokes_fv1.cpp:342-343, 713-714).  The row of node
+// v therefore only needs, for each incident element e, the SCVFs of e that
+// touch v.  One lane handles one (v, e) pair: it forms those SCVFs (3 for
+// hexahedra and tetrahedra, 2 in 2D), evaluates the upwind, the FIELDS
+// stabilisation and the fixpoint linearisation for them, and holds e's
+// contribution to v's block row ((dim+1) x nsh x (dim+1) Jacobian values
+// and dim+1 defect values) in registers.
+//
+// Canonical frame.  The lane loads e's corners permuted by a symmetry of the
+// reference element that puts v at reference corner 0, so a single code
+// path, with every SCVF index, shape value and reference gradient a
+// compile-time constant, serves every lane, and no register array is
+// indexed dynamically.  Normals point away from v, so all row contributions
+// carry the same sign.  The side/ray intersection of the skewed upwinds runs
+// on the original corner order and side order of the reference element.
+//
+// A workgroup owns consecutive nodes (a contiguous range of CSR values that
+// fits 64 KiB of LDS).  Lanes add their rows into LDS in rounds ordered by
+// the incidence slot (no two lanes of one round share a row), so the sums
+// are bitwise reproducible without atomics; finished rows leave with
+// coalesced stores.  Static geometry (diffusion lengths per SCVF, SCV
+// volumes) comes from a per-mesh cache built by geometry_cache<>.
+//
+// Decision-carrying quantities (centres, ip, normal, interpolated velocity,
+// flux sign, side/ray intersection) are evaluated without FMA contraction.
 #include "nsfv1_internal.hpp"
+#include "nsfv1_topo.hpp"
 
 namespace nsfv1 {
 
-__constant__ RefTopo c_topo[4];
+// ---------------------------------------------------------------------
+// compile-time tables per element type
+// ---------------------------------------------------------------------
+template <int ET>
+struct Tab {
+    double phi[12][8];
+    double gl[12][8][3];
+    int cip[8][3];
+    int efA[12], efB[12];
+};
 
-// Mesh arrays (device):  xyz [nv][dim], conn [ne][nsh], CSR rowptr/colidx,
-// incidences sorted by node (inc_elem, inc_node, inc_info = corner | slot<<8,
-// node_inc [nv+1]), workgroup plan wg_node [nwg+1] / wg_rounds [nwg].
-using KMesh = KMeshPtrs;
+template <int ET>
+constexpr Tab<ET> make_tab()
+{
+    Tab<ET> t{};
+    for (int ip = 0; ip < Topo<ET>::nip; ++ip) {
+        for (int k = 0; k < Topo<ET>::nsh; ++k) {
+            t.phi[ip][k] = ip_shape<ET>(ip, k);
+            for (int j = 0; j < Topo<ET>::dim; ++j) t.gl[ip][k][j] = ip_grad<ET>(ip, k, j);
+        }
+        t.efA[ip] = edge_face<ET>(ip, 0);
+        t.efB[ip] = edge_face<ET>(ip, 1);
+    }
+    for (int k = 0; k < Topo<ET>::nsh; ++k)
+        for (int q = 0; q < Topo<ET>::cip; ++q) t.cip[k][q] = corner_ip<ET>(k, q);
+    return t;
+}
+
+template <int ET>
+struct TabH {
+    static constexpr Tab<ET> t = make_tab<ET>();
+};
+
+#define NSFV1_DBL_MAX 1.7976931348623157e308
+
+template <int D>
+__device__ inline double dot_nc(const double* a, const double* b)
+{
+#pragma clang fp contract(off)
+    double s = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) s += a[i] * b[i];
+    return s;
+}
+
+template <int D>
+__device__ inline double dist2_nc(const double* a, const double* b)
+{
+#pragma clang fp contract(off)
+    double s = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) s += (a[i] - b[i]) * (a[i] - b[i]);
+    return s;
+}
+
+__device__ inline void cross_nc(const double* a, const double* b, double* c)
+{
+#pragma clang fp contract(off)
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// element centre (average of the corners, oracle operation order)
+template <int ET>
+__device__ inline void elem_centre(const double (&co)[Topo<ET>::nsh][Topo<ET>::dim], double* c)
+{
+#pragma clang fp contract(off)
+    constexpr int N = Topo<ET>::nsh, D = Topo<ET>::dim;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        double s = 0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) s += co[k][i];
+        c[i] = s / N;
+    }
+}
+
+// ip, normal and the edge midpoint of SCVF IP -- restated ugcore FV1Geometry
+template <int ET, int IP>
+__device__ inline void scvf_frame(const double (&co)[Topo<ET>::nsh][Topo<ET>::dim], const double* c,
+                                  double* ipg, double* n, double* me)
+{
+#pragma clang fp contract(off)
+    using T = Topo<ET>;
+    constexpr int D = T::dim;
+    constexpr int f = T::edge[IP][0], t = T::edge[IP][1];
+#pragma unroll
+    for (int i = 0; i < D; ++i) me[i] = 0.5 * (co[f][i] + co[t][i]);
+    double nn[3] = {0, 0, 0};
+    if constexpr (D == 2) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) ipg[i] = 0.5 * (me[i] + c[i]);
+        const double d0 = c[0] - me[0], d1 = c[1] - me[1];
+        nn[0] = d1;
+        nn[1] = -d0;
+    } else {
+        constexpr int A = TabH<ET>::t.efA[IP], B = TabH<ET>::t.efB[IP];
+        constexpr int na = T::side_n[A], nb = T::side_n[B];
+        double fa[3], fb[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            double sa = 0, sb = 0;
+#pragma unroll
+            for (int j = 0; j < na; ++j) sa += co[T::side[A][j]][i];
+#pragma unroll
+            for (int j = 0; j < nb; ++j) sb += co[T::side[B][j]][i];
+            fa[i] = sa / na;
+            fb[i] = sb / nb;
+        }
+        double d1[3], d2[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            ipg[i] = 0.25 * (me[i] + fa[i] + c[i] + fb[i]);
+            d1[i] = c[i] - me[i];
+            d2[i] = fb[i] - fa[i];
+        }
+        cross_nc(d1, d2, nn);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) nn[i] *= 0.5;
+    }
+    double e[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < D; ++i) e[i] = co[t][i] - co[f][i];
+    const double s = dot_nc<D>(nn, e);
+#pragma unroll
+    for (int i = 0; i < D; ++i) n[i] = s < 0 ? -nn[i] : nn[i];
+}
+
+// J^-1 of the element map at SCVF IP (J_ij = dx_i/dxi_j); false if singular
+template <int ET, int IP>
+__device__ inline bool inv_jacobian(const double (&co)[Topo<ET>::nsh][Topo<ET>::dim],
+                                    double (&Ji)[Topo<ET>::dim][Topo<ET>::dim])
+{
+    constexpr int D = Topo<ET>::dim, N = Topo<ET>::nsh;
+    double J[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            double s = 0;
+#pragma unroll
+            for (int k = 0; k < N; ++k) s += co[k][i] * TabH<ET>::t.gl[IP][k][j];
+            J[i][j] = s;
+        }
+    if constexpr (D == 2) {
+        const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        if (det == 0.0) return false;
+        const double id = 1.0 / det;
+        Ji[0][0] = J[1][1] * id;  Ji[0][1] = -J[0][1] * id;
+        Ji[1][0] = -J[1][0] * id; Ji[1][1] = J[0][0] * id;
+    } else {
+        const double a00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        const double a01 = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+        const double a02 = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+        const double det = J[0][0] * a00 + J[1][0] * a01 + J[2][0] * a02;
+        if (det == 0.0) return false;
+        const double id = 1.0 / det;
+        Ji[0][0] = a00 * id; Ji[0][1] = a01 * id; Ji[0][2] = a02 * id;
+        Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+        Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+        Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+        Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
+        Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+        Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------
+// side/ray intersection (ugcore ElementSideRayIntersection, restated):
+// the first side in reference order hit at ray parameter t <= 0 with side
+// parameters inside up to 1e-10, searched upwind from the ip along the ip
+// velocity (upwind.cpp:351-353, 547-549).
+// ---------------------------------------------------------------------
+#define NSFV1_RAY_EPS 1e-10
+
+__device__ inline bool ray_tri_nc(const double* p0, const double* p1, const double* p2, const double* from,
+                                  const double* dir, double& b1o, double& b2o)
+{
+#pragma clang fp contract(off)
+    double e1[3], e2[3], pv[3], tv[3], qv[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { e1[i] = p1[i] - p0[i]; e2[i] = p2[i] - p0[i]; }
+    cross_nc(dir, e2, pv);
+    const double det = dot_nc<3>(e1, pv);
+    if (det == 0.0) return false;
+    const double inv = 1.0 / det;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) tv[i] = from[i] - p0[i];
+    const double b1 = dot_nc<3>(tv, pv) * inv;
+    cross_nc(tv, e1, qv);
+    const double b2 = dot_nc<3>(dir, qv) * inv;
+    const double t = dot_nc<3>(e2, qv) * inv;
+    if (b1 < -NSFV1_RAY_EPS || b2 < -NSFV1_RAY_EPS || b1 + b2 > 1.0 + NSFV1_RAY_EPS) return false;
+    if (t > 0.0) return false;
+    b1o = b1;
+    b2o = b2;
+    return true;
+}
+
+template <int ET, int S, int TR>
+__device__ inline void try_side(const double (&co)[Topo<ET>::nsh][Topo<ET>::dim], const double* from,
+                                const double* dir, int& side, double* gi, double* li)
+{
+#pragma clang fp contract(off)
+    using T = Topo<ET>;
+    if (side >= 0) return;
+    if constexpr (T::dim == 2) {
+        if constexpr (TR == 0) {
+            constexpr int a = T::side[S][0], b = T::side[S][1];
+            const double ex = co[b][0] - co[a][0], ey = co[b][1] - co[a][1];
+            const double rx = co[a][0] - from[0], ry = co[a][1] - from[1];
+            const double det = -dir[0] * ey + ex * dir[1];
+            if (det == 0.0) return;
+            const double t = (-rx * ey + ex * ry) / det;
+            const double lam = (dir[0] * ry - dir[1] * rx) / det;
+            if (lam < -NSFV1_RAY_EPS || lam > 1.0 + NSFV1_RAY_EPS) return;
+            if (t > 0.0) return;
+            side = S;
+            gi[0] = co[a][0] + lam * ex;
+            gi[1] = co[a][1] + lam * ey;
+#pragma unroll
+            for (int i = 0; i < 2; ++i) li[i] = (1.0 - lam) * T::lc[a][i] + lam * T::lc[b][i];
+        }
+    } else {
+        if constexpr (TR == 0 || T::side_n[S] == 4) {
+            constexpr int a = T::side[S][0];
+            constexpr int b = T::side[S][TR == 0 ? 1 : 2];
+            constexpr int c = T::side[S][TR == 0 ? 2 : 3];
+            double b1, b2;
+            if (!ray_tri_nc(co[a], co[b], co[c], from, dir, b1, b2)) return;
+            side = S;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                gi[i] = co[a][i] + b1 * (co[b][i] - co[a][i]) + b2 * (co[c][i] - co[a][i]);
+                li[i] = (1.0 - b1 - b2) * T::lc[a][i] + b1 * T::lc[b][i] + b2 * T::lc[c][i];
+            }
+        }
+    }
+}
+
+template <int ET, int S>
+__device__ inline void side_loop(const double (&co)[Topo<ET>::nsh][Topo<ET>::dim], const double* from,
+                                 const double* dir, int& side, double* gi, double* li)
+{
+    if constexpr (S < Topo<ET>::nside) {
+        try_side<ET, S, 0>(co, from, dir, side, gi, li);
+        try_side<ET, S, 1>(co, from, dir, side, gi, li);
+        side_loop<ET, S + 1>(co, from, dir, side, gi, li);
+    }
+}
+
+// ---------------------------------------------------------------------
+// per-mesh static geometry: SCV volumes and the diffusion-length factor
+// 1/L^2 of every SCVF (diffusion_length.h:48-199 -- geometry only)
+// ---------------------------------------------------------------------
+template <int ET, int IP>
+__device__ inline void cache_ips(const double (&co)[Topo<ET>::nsh][Topo<ET>::dim], const double* c, double* nn,
+                                 double* ds)
+{
+    if constexpr (IP < Topo<ET>::nip) {
+        constexpr int D = Topo<ET>::dim;
+        double ipg[3], n[3], me[3];
+        scvf_frame<ET, IP>(co, c, ipg, n, me);
+        nn[IP] = dot_nc<D>(n, n);
+        ds[IP] = D == 3 ? dist2_nc<D>(me, c) : 0.0;
+        cache_ips<ET, IP + 1>(co, c, nn, ds);
+    }
+}
+
+template <int ET>
+__global__ void __launch_bounds__(256)
+geometry_cache(const double* __restrict__ xyz, const int32_t* __restrict__ conn, int64_t ne, int difflen,
+               double* __restrict__ l2inv, double* __restrict__ volc, int* __restrict__ err)
+{
+    using T = Topo<ET>;
+    constexpr int D = T::dim, N = T::nsh, NIP = T::nip;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    double co[N][D];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const int64_t v = conn[e * N + k];
+#pragma unroll
+        for (int i = 0; i < D; ++i) co[k][i] = xyz[v * D + i];
+    }
+    double vol[N];
+    scv_volumes<ET>(co, vol);
+#pragma unroll
+    for (int k = 0; k < N; ++k) volc[e * N + k] = vol[k];
+    double c[D];
+    elem_centre<ET>(co, c);
+    double nn[NIP], ds[NIP];
+    cache_ips<ET, 0>(co, c, nn, ds);
+    double minN = NSFV1_DBL_MAX, minD = NSFV1_DBL_MAX, avgN = 0.0;
+#pragma unroll
+    for (int ip = 0; ip < NIP; ++ip) {
+        minN = nn[ip] < minN ? nn[ip] : minN;
+        avgN += nn[ip];
+        minD = ds[ip] < minD ? ds[ip] : minD;
+    }
+    avgN /= NIP;
+    bool ok = true;
+#pragma unroll
+    for (int ip = 0; ip < NIP; ++ip) {
+        double a2 = 0.5 * (vol[T::edge[ip][0]] + vol[T::edge[ip][1]]);
+        a2 *= a2;
+        double L;
+        if (difflen == DL_RAW)
+            L = (D == 2) ? 1. / (0.5 * a2 / nn[ip] + 3.0 * nn[ip] / 8.) : 1. / (0.5 * a2 / nn[ip] + 3.0 * ds[ip] / 8.);
+        else if (difflen == DL_FIVEPOINT)
+            L = (D == 2) ? 2.0 * nn[ip] / a2 + 8.0 / nn[ip] : 2.0 * nn[ip] / a2 + 8.0 * ds[ip] / nn[ip];
+        else
+            L = (D == 2) ? 2.0 * minN / a2 + 8.0 / (3.0 * avgN) : 2.0 * minN / a2 + 8.0 * minD / (3.0 * avgN);
+        ok &= (nn[ip] > 0.0) && (a2 > 0.0);
+        l2inv[e * NIP + ip] = L;
+    }
+    if (!ok) err[0] = KERR_DEGENERATE;
+}
+
+// ---------------------------------------------------------------------
+// canonical frame tables (lane-varying lookups -> constant memory)
+// ---------------------------------------------------------------------
+template <int ET>
+struct Canon {
+    int perm[8][8];   // canonical corner k' -> original corner
+    int oip[8][3];    // original SCVF of the q-th canonical SCVF at corner 0
+    int from[8][3];   // 1 if kv is the `from` corner of that SCVF
+};
+
+template <int ET>
+constexpr Canon<ET> make_canon()
+{
+    Canon<ET> c{};
+    for (int kv = 0; kv < Topo<ET>::nsh; ++kv) {
+        for (int k = 0; k < Topo<ET>::nsh; ++k) c.perm[kv][k] = canon_perm<ET>(kv, k);
+        for (int q = 0; q < Topo<ET>::cip; ++q) {
+            c.oip[kv][q] = canon_orig_ip<ET>(kv, q);
+            c.from[kv][q] = canon_is_from<ET>(kv, q) ? 1 : 0;
+        }
+    }
+    return c;
+}
+
+template <int ET>
+struct CanonH {
+    static constexpr Canon<ET> c = make_canon<ET>();
+};
+
+// configuration: FAST >= 0 is the common case (FIELDS stabilisation, one
+// upwind object of type FAST for stabilisation and convection, no Stokes,
+// PAC, Peclet blend or exact Jacobian) compiled with those switches fixed;
+// FAST < 0 reads every switch at run time.
+template <int FAST>
+struct Cfg {
+    const KParams& P;
+    __device__ bool stokes() const { return FAST >= 0 ? false : P.stokes != 0; }
+    __device__ bool pac() const { return FAST >= 0 ? false : P.pac != 0; }
+    __device__ bool peclet() const { return FAST >= 0 ? false : P.peclet != 0; }
+    __device__ bool exact() const { return FAST >= 0 ? false : P.exact != 0.0; }
+    __device__ bool fields() const { return FAST >= 0 ? true : P.stab_type == STAB_FIELDS; }
+    __device__ int stab_up() const { return FAST >= 0 ? FAST : P.stab_upwind; }
+    __device__ int conv_up() const { return FAST >= 0 ? FAST : P.conv_upwind; }
+    __device__ bool two_upwinds() const { return FAST >= 0 ? false : (!P.pac && P.conv_upwind != P.stab_upwind); }
+};
+
+// compact per-SCVF data that the Jacobian expansion needs (precision JT)
+template <int ET, typename JT>
+struct Rec {
+    static constexpr int D = Topo<ET>::dim, N = Topo<ET>::nsh;
+    JT n[D], Ji[D][D], pw, pw1, vv, vc, sp, w, up[D];
+    JT Us[N], Uc[N];
+};
 
 template <int ET, typename JT>
+struct Lane {
+    static constexpr int D = Topo<ET>::dim, N = Topo<ET>::nsh, NC = D + 1;
+    JT jacc[NC][N][NC];
+    double dacc[NC];
+};
+
+// upwind shapes in canonical indexing.  `pk` maps canonical -> original
+// corners; the ray cast runs on the original corners (`coo`) in original
+// side order (upwind.cpp:337-430, 505-575).
+template <int ET, int Q>
+__device__ inline bool upwind_canon(int type, bool kvfrom, const double (&co)[Topo<ET>::nsh][Topo<ET>::dim],
+                                    const double (&coo)[Topo<ET>::nsh][Topo<ET>::dim], const int* pk,
+                                    const double* ipg, const double* n, const double* sv,
+                                    double (&U)[Topo<ET>::nsh], double& convlen, bool& have_cut, int& side,
+                                    double* gi, double* li)
+{
+#pragma clang fp contract(off)
+    using T = Topo<ET>;
+    constexpr int D = T::dim, N = T::nsh;
+    constexpr int CIP = corner_ip<ET>(0, Q);
+    constexpr int OT = T::edge[CIP][0] == 0 ? T::edge[CIP][1] : T::edge[CIP][0];  // other end of the edge
+    if (type == UPW_NO) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) U[k] = TabH<ET>::t.phi[CIP][k];
+        convlen = 1.0;
+        return true;
+    }
+    if (type == UPW_FULL) {
+        // n points away from canonical corner 0 (= kv): the reference picks
+        // `from` iff n_(from->to) . v > 0 (upwind.cpp:150-166)
+        const double fl = dot_nc<D>(n, sv);
+        const bool pick0 = kvfrom ? (fl > 0.0) : !(fl < 0.0);
+#pragma unroll
+        for (int k = 0; k < N; ++k) U[k] = (k == 0) ? (pick0 ? 1.0 : 0.0) : (k == OT ? (pick0 ? 0.0 : 1.0) : 0.0);
+        double cc[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) cc[i] = pick0 ? co[0][i] : co[OT][i];
+        convlen = sqrt(dist2_nc<D>(ipg, cc));
+        return true;
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) U[k] = 0.0;
+    if (sqrt(dot_nc<D>(sv, sv)) < 1e-14) {
+        convlen = 1.0;
+        return true;
+    }
+    if (!have_cut) {
+        side = -1;
+        side_loop<ET, 0>(coo, ipg, sv, side, gi, li);
+        have_cut = true;
+    }
+    if (side < 0) return false;
+    if (type == UPW_SKEWED) {
+        double mn = NSFV1_DBL_MAX;
+        int best = 0;
+#pragma unroll
+        for (int s = 0; s < T::nside; ++s) {
+            if (s != side) continue;
+#pragma unroll
+            for (int j = 0; j < T::side_n[s]; ++j) {
+                const double dd = dist2_nc<D>(gi, coo[T::side[s][j]]);
+                if (dd < mn) { mn = dd; best = T::side[s][j]; }
+            }
+        }
+        double cc[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) cc[i] = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const bool on = (pk[k] == best);
+            U[k] = on ? 1.0 : 0.0;
+#pragma unroll
+            for (int i = 0; i < D; ++i) cc[i] = on ? co[k][i] : cc[i];
+        }
+        convlen = sqrt(dist2_nc<D>(ipg, cc));
+    } else {
+        double val[N];
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+            bool on = false;
+#pragma unroll
+            for (int s = 0; s < T::nside; ++s) on |= (s == side) && side_has<ET>(s, c);
+            double v;
+            if constexpr (ET == 0) v = c == 0 ? 1.0 - li[0] - li[1] : (c == 1 ? li[0] : li[1]);
+            else if constexpr (ET == 1) {
+                const double a = li[0], b = li[1];
+                v = c == 0 ? (1 - a) * (1 - b) : (c == 1 ? a * (1 - b) : (c == 2 ? a * b : (1 - a) * b));
+            } else if constexpr (ET == 2) v = c == 0 ? 1.0 - li[0] - li[1] - li[2] : li[c - 1];
+            else {
+                const double f0 = T::lc[c][0] > 0.5 ? li[0] : 1.0 - li[0];
+                const double f1 = T::lc[c][1] > 0.5 ? li[1] : 1.0 - li[1];
+                const double f2 = T::lc[c][2] > 0.5 ? li[2] : 1.0 - li[2];
+                v = f0 * f1 * f2;
+            }
+            val[c] = on ? v : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            double x = 0.0;
+#pragma unroll
+            for (int c = 0; c < N; ++c) x = (pk[k] == c) ? val[c] : x;
+            U[k] = x;
+        }
+        convlen = sqrt(dist2_nc<D>(ipg, gi));
+    }
+    return true;
+}
+
+// FP64 part of canonical SCVF Q at corner 0 (= the row node): geometry,
+// upwind, stabilisation, defect; emits the record for the Jacobian.
+template <int ET, int FAST, int Q, typename JT>
+__device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET>::nsh][Topo<ET>::dim],
+                                  const double (&coo)[Topo<ET>::nsh][Topo<ET>::dim],
+                                  const double (&ul)[Topo<ET>::nsh][Topo<ET>::dim + 1], const int* pk,
+                                  const int32_t* ccn, const double* c, const double* __restrict__ uold,
+                                  double l2, bool kvfrom, bool wantJ, Lane<ET, JT>& L, Rec<ET, JT>& R,
+                                  bool& okg, bool& oku, bool& have_cut, int& side, double* gi, double* li)
+{
+    using T = Topo<ET>;
+    constexpr int D = T::dim, N = T::nsh;
+    constexpr int IP = corner_ip<ET>(0, Q);
+    constexpr int OT = T::edge[IP][0] == 0 ? T::edge[IP][1] : T::edge[IP][0];
+    const KParams& P = C.P;
+    const double nu = P.nu, rho = P.rho;
+
+    double ipg[D], n[D], me[D];
+    {
+#pragma clang fp contract(off)
+        // frame of canonical SCVF IP with the normal pointing away from
+        // corner 0 (the row node): row contributions are then all `+`
+        double nn_[D];
+        scvf_frame<ET, IP>(co, c, ipg, nn_, me);
+        double e[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) e[i] = co[OT][i] - co[0][i];
+        const double s = dot_nc<D>(nn_, e);
+#pragma unroll
+        for (int i = 0; i < D; ++i) n[i] = s < 0 ? -nn_[i] : nn_[i];
+    }
+    double Ji[D][D];
+    okg &= inv_jacobian<ET, IP>(co, Ji);
+    const double sgn = P.s_a;
+
+    double sv[D];
+    {
+#pragma clang fp contract(off)
+#pragma unroll
+        for (int d = 0; d < D; ++d) sv[d] = 0.0;
+#pragma unroll
+        for (int sh = 0; sh < N; ++sh)
+#pragma unroll
+            for (int d = 0; d < D; ++d) sv[d] += ul[sh][d] * TabH<ET>::t.phi[IP][sh];
+    }
+    double gv[D + 1][D];
+    {
+        double gh[D + 1][D];
+#pragma unroll
+        for (int a = 0; a <= D; ++a)
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                double acc = 0;
+#pragma unroll
+                for (int k = 0; k < N; ++k) acc += ul[k][a] * TabH<ET>::t.gl[IP][k][j];
+                gh[a][j] = acc;
+            }
+#pragma unroll
+        for (int a = 0; a <= D; ++a)
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                double acc = 0;
+#pragma unroll
+                for (int j = 0; j < D; ++j) acc += gh[a][j] * Ji[j][i];
+                gv[a][i] = acc;
+            }
+    }
+    double pr = 0;
+#pragma unroll
+    for (int sh = 0; sh < N; ++sh) pr += TabH<ET>::t.phi[IP][sh] * ul[sh][D];
+
+    // upwinds (the stabilisation's and the convective term's)
+    double Ls = 1.0, upS[D], upC[D];
+    {
+        double Us[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) Us[k] = 0.0;
+        if (!C.stokes()) oku &= upwind_canon<ET, Q>(C.stab_up(), kvfrom, co, coo, pk, ipg, n, sv, Us, Ls, have_cut, side, gi, li);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            double acc = 0;
+#pragma unroll
+            for (int k = 0; k < N; ++k) acc += Us[k] * ul[k][d];
+            upS[d] = acc;
+            upC[d] = acc;
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) { R.Us[k] = JT(Us[k]); R.Uc[k] = JT(Us[k]); }
+        if (!C.stokes() && C.two_upwinds()) {
+            double Uc[N], Lc;
+            oku &= upwind_canon<ET, Q>(C.conv_up(), kvfrom, co, coo, pk, ipg, n, sv, Uc, Lc, have_cut, side, gi, li);
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                double acc = 0;
+#pragma unroll
+                for (int k = 0; k < N; ++k) acc += Uc[k] * ul[k][d];
+                upC[d] = acc;
+            }
+#pragma unroll
+            for (int k = 0; k < N; ++k) R.Uc[k] = JT(Uc[k]);
+        }
+    }
+
+    // stabilisation: stabilization.cpp:122-242 (FIELDS), 805-850 (without)
+    double svel[D], vv = 0.0, vc = 0.0, idg = 0.0;
+    if (C.fields()) {
+        vv = nu * l2;
+        vc = C.stokes() ? 0.0 : sqrt(dot_nc<D>(sv, sv)) / Ls;
+        double diag = vv;
+        if (P.has_old) diag += 1. / P.dt;
+        if (!C.stokes()) diag += vc;
+        idg = 1.0 / diag;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            double rhs = P.has_source ? P.f[d] : 0.0;
+            if (P.has_old) {
+                double o = 0.0;
+#pragma unroll
+                for (int sh = 0; sh < N; ++sh) o += TabH<ET>::t.phi[IP][sh] * uold[(int64_t)ccn[sh] * (D + 1) + d];
+                rhs += o / P.dt;
+            }
+            rhs += vv * sv[d] + vc * upS[d] - gv[D][d] / rho;
+            svel[d] = rhs * idg;
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < D; ++d) svel[d] = sv[d];
+    }
+
+    // defect: navier_stokes_fv1.cpp:667-777
+#pragma unroll
+    for (int d1 = 0; d1 < D; ++d1) {
+        double acc = 0;
+#pragma unroll
+        for (int d2 = 0; d2 < D; ++d2) acc += gv[d1][d2] * n[d2];
+        if (!P.laplace)
+#pragma unroll
+            for (int d2 = 0; d2 < D; ++d2) acc += gv[d2][d1] * n[d2];
+        L.dacc[d1] += sgn * (acc * (-1.0 * nu * rho) + pr * n[d1]);
+    }
+    double prod = 0.0, wpe = 1.0, upv[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) upv[d] = 0.0;
+    if (!C.stokes()) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) upv[d] = C.pac() ? svel[d] : upC[d];
+        if (C.peclet()) {
+            // peclet_blend: navier_stokes_fv1.cpp:871-892
+            double nn = 0;
+#pragma unroll
+            for (int i = 0; i < D; ++i) nn += n[i] * n[i];
+            const double Pe = dot_nc<D>(sv, n) / nn * sqrt(dist2_nc<D>(co[OT], co[0])) / nu;
+            wpe = Pe * Pe / (5.0 + Pe * Pe);
+#pragma unroll
+            for (int d = 0; d < D; ++d) upv[d] = wpe * upv[d] + (1.0 - wpe) * sv[d];
+        }
+        prod = dot_nc<D>(sv, n) * rho;
+#pragma unroll
+        for (int d1 = 0; d1 < D; ++d1) L.dacc[d1] += sgn * (upv[d1] * prod);
+    }
+    L.dacc[D] += sgn * (dot_nc<D>(svel, n) * rho);
+
+    if (!wantJ) return;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        R.n[i] = JT(n[i]);
+        R.up[i] = JT(upv[i]);
+#pragma unroll
+        for (int j = 0; j < D; ++j) R.Ji[i][j] = JT(Ji[i][j]);
+    }
+    R.pw = JT(prod * wpe);
+    R.pw1 = JT(prod * (1.0 - wpe));
+    R.vv = JT(vv * idg);
+    R.vc = JT(vc * idg);
+    R.sp = JT(-1.0 / rho * idg);
+    R.w = JT(wpe);
+}
+
+// Jacobian row contributions of canonical SCVF Q from its record:
+// navier_stokes_fv1.cpp:317-593
+template <int ET, int FAST, int Q, typename JT>
+__device__ inline void scvf_jac(const Cfg<FAST>& C, const Rec<ET, JT>& R, Lane<ET, JT>& L)
+{
+    using T = Topo<ET>;
+    constexpr int D = T::dim, N = T::nsh;
+    constexpr int IP = corner_ip<ET>(0, Q);
+    const KParams& P = C.P;
+    const bool fields = C.fields();
+    const JT s_ = JT(P.s_a), rh = JT(P.rho), nr = JT(-1.0 * P.nu * P.rho), ex = JT(P.exact * P.rho);
+#pragma unroll
+    for (int sh = 0; sh < N; ++sh) {
+        JT g[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            JT acc = 0;
+#pragma unroll
+            for (int j = 0; j < D; ++j) acc += R.Ji[j][i] * JT(TabH<ET>::t.gl[IP][sh][j]);
+            g[i] = acc;
+        }
+        const JT ph = JT(TabH<ET>::t.phi[IP][sh]);
+        const JT Sv = fields ? R.vv * ph + R.vc * R.Us[sh] : ph;
+        JT Sp[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) Sp[d] = fields ? g[d] * R.sp : JT(0);
+        JT gn = 0;
+#pragma unroll
+        for (int i = 0; i < D; ++i) gn += g[i] * R.n[i];
+#pragma unroll
+        for (int d1 = 0; d1 < D; ++d1) {
+            L.jacc[d1][sh][d1] += s_ * (nr * gn);
+            if (!P.laplace)
+#pragma unroll
+                for (int d2 = 0; d2 < D; ++d2) L.jacc[d1][sh][d2] += s_ * (nr * g[d1] * R.n[d2]);
+            L.jacc[d1][sh][D] += s_ * (ph * R.n[d1]);
+        }
+        if (!C.stokes()) {
+            if (C.pac()) {
+#pragma unroll
+                for (int d1 = 0; d1 < D; ++d1) {
+                    L.jacc[d1][sh][d1] += s_ * (R.pw * Sv);
+                    L.jacc[d1][sh][D] += s_ * (R.pw * Sp[d1]);
+                }
+            } else {
+                const JT cv = R.Uc[sh] * R.pw;
+#pragma unroll
+                for (int d1 = 0; d1 < D; ++d1) L.jacc[d1][sh][d1] += s_ * cv;
+            }
+            if (C.peclet())
+#pragma unroll
+                for (int d1 = 0; d1 < D; ++d1) L.jacc[d1][sh][d1] += s_ * (R.pw1 * ph);
+            if (C.exact()) {
+                if (C.pac()) {
+                    JT pp = 0;
+#pragma unroll
+                    for (int k = 0; k < D; ++k) pp += Sp[k] * R.n[k];
+#pragma unroll
+                    for (int d1 = 0; d1 < D; ++d1) {
+                        const JT pv = Sv * R.n[d1] * ex;
+#pragma unroll
+                        for (int d2 = 0; d2 < D; ++d2) L.jacc[d1][sh][d2] += s_ * (pv * R.up[d1]);
+                        L.jacc[d1][sh][D] += s_ * (pp * ex * R.up[d1]);
+                    }
+                } else {
+#pragma unroll
+                    for (int d1 = 0; d1 < D; ++d1)
+#pragma unroll
+                        for (int d2 = 0; d2 < D; ++d2) L.jacc[d1][sh][d2] += s_ * (R.w * R.Uc[sh] * R.n[d2] * rh * R.up[d1]);
+                }
+                if (C.peclet())
+#pragma unroll
+                    for (int d1 = 0; d1 < D; ++d1)
+#pragma unroll
+                        for (int d2 = 0; d2 < D; ++d2)
+                            L.jacc[d1][sh][d2] += s_ * (R.up[d1] * (JT(1) - R.w) * ph * R.n[d2] * rh);
+            }
+        }
+        JT cpp = 0;
+#pragma unroll
+        for (int d1 = 0; d1 < D; ++d1) {
+            L.jacc[D][sh][d1] += s_ * (Sv * R.n[d1] * rh);
+            cpp += Sp[d1] * R.n[d1] * rh;
+        }
+        L.jacc[D][sh][D] += s_ * cpp;
+    }
+}
+
+template <int ET, int FAST, int Q, typename JT>
+__device__ inline void scvf_all(const Cfg<FAST>& C, const double (&co)[Topo<ET>::nsh][Topo<ET>::dim],
+                                const double (&coo)[Topo<ET>::nsh][Topo<ET>::dim],
+                                const double (&ul)[Topo<ET>::nsh][Topo<ET>::dim + 1], const int* pk,
+                                const int32_t* ccn, const double* c, const double* uold, const double* l2row,
+                                int kv, bool wantJ, Lane<ET, JT>& L, Rec<ET, JT>* R, bool& okg, bool& oku)
+{
+    if constexpr (Q < Topo<ET>::cip) {
+        bool have_cut = false;
+        int side = -1;
+        double gi[3] = {0, 0, 0}, li[3] = {0, 0, 0};
+        const double l2 = l2row[CanonH<ET>::c.oip[kv][Q]];
+        const bool kvfrom = CanonH<ET>::c.from[kv][Q] != 0;
+        scvf_canon<ET, FAST, Q, JT>(C, co, coo, ul, pk, ccn, c, uold, l2, kvfrom, wantJ, L, R[Q], okg, oku, have_cut,
+                                    side, gi, li);
+        scvf_all<ET, FAST, Q + 1, JT>(C, co, coo, ul, pk, ccn, c, uold, l2row, kv, wantJ, L, R, okg, oku);
+    }
+}
+
+template <int ET, int FAST, int Q, typename JT>
+__device__ inline void jac_all(const Cfg<FAST>& C, const Rec<ET, JT>* R, Lane<ET, JT>& L)
+{
+    if constexpr (Q < Topo<ET>::cip) {
+        scvf_jac<ET, FAST, Q, JT>(C, R[Q], L);
+        jac_all<ET, FAST, Q + 1, JT>(C, R, L);
+    }
+}
+
+// ---------------------------------------------------------------------
+// assembly kernel: one lane per (row node, incident element), node-major;
+// a workgroup owns the consecutive nodes [v0, v1) and their contiguous CSR
+// values; rows are accumulated in LDS in rounds of the incidence slot
+// ---------------------------------------------------------------------
+template <int ET, typename JT, int FAST>
 __global__ void __launch_bounds__(256)
-assemble_rows(KMesh M, KParams P, const double* __restrict__ u, const double* __restrict__ uold,
+assemble_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const double* __restrict__ uold,
               JT* __restrict__ J, double* __restrict__ dout, int* __restrict__ err)
 {
-    constexpr int D = Shape<ET>::dim, N = Shape<ET>::nsh, NC = D + 1, CIP = Shape<ET>::cip;
+    using T = Topo<ET>;
+    constexpr int D = T::dim, N = T::nsh, NC = D + 1, NIP = T::nip;
     extern __shared__ __align__(16) unsigned char smem[];
-    const RefTopo& R = c_topo[ET];
+    const Cfg<FAST> C{P};
 
     const int w = blockIdx.x;
     const int v0 = M.wg_node[w], v1 = M.wg_node[w + 1];
     const int64_t jbase = M.rowptr[(int64_t)v0 * NC];
-    const int64_t jlen = M.rowptr[(int64_t)v1 * NC] - jbase;
+    const int jlen = (int)(M.rowptr[(int64_t)v1 * NC] - jbase);
     const int dlen = (v1 - v0) * NC;
     JT* jbuf = reinterpret_cast<JT*>(smem);
-    double* dbuf = reinterpret_cast<double*>(smem + ((jlen * sizeof(JT) + 15) & ~(int64_t)15));
-    for (int64_t i = threadIdx.x; i < jlen; i += blockDim.x) jbuf[i] = JT(0);
+    double* dbuf = reinterpret_cast<double*>(smem + ((jlen * (int)sizeof(JT) + 15) & ~15));
+    const bool wantJ = J != nullptr;
+    if (wantJ)
+        for (int i = threadIdx.x; i < jlen; i += blockDim.x) jbuf[i] = JT(0);
     for (int i = threadIdx.x; i < dlen; i += blockDim.x) dbuf[i] = 0.0;
 
-    const int64_t i0 = M.node_inc[v0], i1 = M.node_inc[v1];
-    const int64_t inc = i0 + threadIdx.x;
+    const int i0 = M.node_inc[v0], i1 = M.node_inc[v1];
+    const int inc = i0 + (int)threadIdx.x;
     const bool active = inc < i1;
 
-    JT jacc[NC][N][NC];
-    double dacc[NC];
+    Lane<ET, JT> L;
 #pragma unroll
     for (int a = 0; a < NC; ++a) {
-        dacc[a] = 0.0;
+        L.dacc[a] = 0.0;
 #pragma unroll
         for (int k = 0; k < N; ++k)
 #pragma unroll
-            for (int b = 0; b < NC; ++b) jacc[a][k][b] = JT(0);
+            for (int b = 0; b < NC; ++b) L.jacc[a][k][b] = JT(0);
     }
-    int v = 0, kv = 0, slot = -1;
-    int32_t cn[N];
+    int v = v0, slot = -1;
     int pos[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) pos[k] = 0;
     if (active) {
         const int e = M.inc_elem[inc];
         v = M.inc_node[inc];
         const int info = M.inc_info[inc];
-        kv = info & 0xff;
+        const int kv = info & 0xff;
         slot = info >> 8;
-        double co[N][D], ul[N][NC], uo[N][NC];
+        int pk[N];
+        int32_t ccn[N];
+        double co[N][D], coo[N][D], ul[N][NC];
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-            cn[k] = M.conn[(int64_t)e * N + k];
+            pk[k] = CanonH<ET>::c.perm[kv][k];
+            ccn[k] = M.conn[(int64_t)e * N + pk[k]];
+            const int32_t on = M.conn[(int64_t)e * N + k];
 #pragma unroll
-            for (int i = 0; i < D; ++i) co[k][i] = M.xyz[(int64_t)cn[k] * D + i];
-#pragma unroll
-            for (int a = 0; a < NC; ++a) {
-                ul[k][a] = u[(int64_t)cn[k] * NC + a];
-                uo[k][a] = P.has_old ? uold[(int64_t)cn[k] * NC + a] : 0.0;
+            for (int i = 0; i < D; ++i) {
+                co[k][i] = M.xyz[(int64_t)ccn[k] * D + i];
+                coo[k][i] = M.xyz[(int64_t)on * D + i];
             }
+#pragma unroll
+            for (int a = 0; a < NC; ++a) ul[k][a] = u[(int64_t)ccn[k] * NC + a];
         }
-        double vol[N];
-        scv_volumes<ET>(co, vol);
-
-        // COR diffusion length needs element-wide extrema of |n|^2 (and of
-        // the edge-midpoint/centre distance in 3D): diffusion_length.h:135-199
-        double corMinN = 0, corAvgN = 0, corMinD = 0;
-        bool ok = true, okg = true;
-        if (P.stab_type == STAB_FIELDS && P.difflen == DL_COR) {
-            corMinN = 1.7976931348623157e308; corMinD = 1.7976931348623157e308;
-            for (int ip = 0; ip < Shape<ET>::nip; ++ip) {
-                IpGeo<ET> Gq;
-                okg &= ip_geometry<ET>(R, co, ip, Gq);
-                const double nn = dotd<D>(Gq.n, Gq.n);
-                corMinN = nn < corMinN ? nn : corMinN;
-                corAvgN += nn;
-                if (D == 3) {
-                    const double ds = dist2d<D>(Gq.c0, Gq.c2);
-                    corMinD = ds < corMinD ? ds : corMinD;
-                }
-            }
-            corAvgN /= Shape<ET>::nip;
+        bool okg = true, oku = true;
+        if (P.parts & 1) {
+            double c[D];
+            elem_centre<ET>(co, c);
+            Rec<ET, JT> R[T::cip];
+            scvf_all<ET, FAST, 0, JT>(C, co, coo, ul, pk, ccn, c, uold, M.l2inv + (int64_t)e * NIP, kv, wantJ, L, R,
+                                      okg, oku);
+            if (wantJ) jac_all<ET, FAST, 0, JT>(C, R, L);
         }
-
-        const double nu = P.nu, rho = P.rho;
-        const int ncip = (P.parts & 1) ? CIP : 0;
-        for (int q = 0; q < ncip; ++q) {
-            const int ip = R.cip[kv][q];
-            const int fr = R.edge[ip][0], to = R.edge[ip][1];
-            const double sgn = (fr == kv) ? P.s_a : -P.s_a;
-            IpGeo<ET> G;
-            okg &= ip_geometry<ET>(R, co, ip, G);
-
-            // StdVel: navier_stokes_fv1.cpp:631-642
-            double sv[D];
-            {
-#pragma clang fp contract(off)
-#pragma unroll
-                for (int d = 0; d < D; ++d) sv[d] = 0.0;
-#pragma unroll
-                for (int sh = 0; sh < N; ++sh)
-#pragma unroll
-                    for (int d = 0; d < D; ++d) sv[d] += ul[sh][d] * G.phi[sh];
-            }
-            // upwind of the stabilisation and of the convective term
-            double Us[N], Ls = 1.0;
-#pragma unroll
-            for (int k = 0; k < N; ++k) Us[k] = 0.0;
-            if (!P.stokes) ok &= upwind_ip<ET>(P.stab_upwind, R, co, ip, G, sv, Us, Ls);
-            double Uc[N];
-#pragma unroll
-            for (int k = 0; k < N; ++k) Uc[k] = Us[k];
-            if (!P.stokes && !P.pac && P.conv_upwind != P.stab_upwind) {
-                double Lc;
-                ok &= upwind_ip<ET>(P.conv_upwind, R, co, ip, G, sv, Uc, Lc);
-            }
-
-            // stabilisation: stabilization.cpp:122-242 (FIELDS, diagonal),
-            // 805-850 (without)
-            double nn = 0;
-#pragma unroll
-            for (int i = 0; i < D; ++i) nn += G.n[i] * G.n[i];
-            double svel[D], Sv[N], Sp[D][N];
-            if (P.stab_type == STAB_FIELDS) {
-                double a2 = 0.5 * (vol[fr] + vol[to]);
-                a2 *= a2;
-                double dist = 0;
-                if (D == 3) dist = dist2d<D>(G.c0, G.c2);
-                double L2inv;
-                if (P.difflen == DL_RAW) {
-                    L2inv = (D == 2) ? 1. / (0.5 * a2 / nn + 3.0 * nn / 8.) : 1. / (0.5 * a2 / nn + 3.0 * dist / 8.);
-                } else if (P.difflen == DL_FIVEPOINT) {
-                    L2inv = (D == 2) ? 2.0 * nn / a2 + 8.0 / nn : 2.0 * nn / a2 + 8.0 * dist / nn;
-                } else {
-                    L2inv = (D == 2) ? 2.0 * corMinN / a2 + 8.0 / (3.0 * corAvgN)
-                                     : 2.0 * corMinN / a2 + 8.0 * corMinD / (3.0 * corAvgN);
-                }
-                const double vv = nu * L2inv;
-                const double vc = P.stokes ? 0.0 : sqrt(dotd<D>(sv, sv)) / Ls;
-                double diag = vv;
-                if (P.has_old) diag += 1. / P.dt;
-                if (!P.stokes) diag += vc;
-#pragma unroll
-                for (int k = 0; k < N; ++k) Sv[k] = (vv * G.phi[k] + vc * Us[k]);
+        // mass part and source: navier_stokes_fv1.cpp:781-869
+        if (P.parts & 6) {
+            const double volk = M.vol[(int64_t)e * N + kv];
+            if (P.parts & 2) {
+                const double m = volk * P.rho;
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
-                    double rhs = P.has_source ? P.f[d] : 0.0;
-                    if (P.has_old) {
-                        double o = 0.0;
-#pragma unroll
-                        for (int sh = 0; sh < N; ++sh) o += G.phi[sh] * uo[sh][d];
-                        rhs += o / P.dt;
-                    }
-#pragma unroll
-                    for (int k = 0; k < N; ++k) {
-                        rhs += Sv[k] * ul[k][d];
-                        const double sumP = -1.0 * G.g[k][d] / rho;
-                        rhs += sumP * ul[k][D];
-                        Sp[d][k] = sumP / diag;
-                    }
-                    svel[d] = rhs / diag;
-                }
-#pragma unroll
-                for (int k = 0; k < N; ++k) Sv[k] = Sv[k] / diag;
-            } else {
-#pragma unroll
-                for (int k = 0; k < N; ++k) {
-                    Sv[k] = G.phi[k];
-#pragma unroll
-                    for (int d = 0; d < D; ++d) Sp[d][k] = 0.0;
-                }
-#pragma unroll
-                for (int d = 0; d < D; ++d) svel[d] = sv[d];
-            }
-
-            // ---- defect: navier_stokes_fv1.cpp:667-777 ----
-            {
-                double gv[D][D];
-#pragma unroll
-                for (int d1 = 0; d1 < D; ++d1)
-#pragma unroll
-                    for (int d2 = 0; d2 < D; ++d2) {
-                        double s = 0;
-#pragma unroll
-                        for (int sh = 0; sh < N; ++sh) s += G.g[sh][d2] * ul[sh][d1];
-                        gv[d1][d2] = s;
-                    }
-                double df[D];
-#pragma unroll
-                for (int d1 = 0; d1 < D; ++d1) {
-                    double s = 0;
-#pragma unroll
-                    for (int d2 = 0; d2 < D; ++d2) s += gv[d1][d2] * G.n[d2];
-                    if (!P.laplace)
-#pragma unroll
-                        for (int d2 = 0; d2 < D; ++d2) s += gv[d2][d1] * G.n[d2];
-                    df[d1] = s * (-1.0 * nu * rho);
-                }
-                double pr = 0;
-#pragma unroll
-                for (int sh = 0; sh < N; ++sh) pr += G.phi[sh] * ul[sh][D];
-#pragma unroll
-                for (int d1 = 0; d1 < D; ++d1) dacc[d1] += sgn * (df[d1] + pr * G.n[d1]);
-                if (!P.stokes) {
-                    double upv[D];
-                    if (!P.pac) {
-#pragma unroll
-                        for (int d = 0; d < D; ++d) {
-                            double s = 0;
-#pragma unroll
-                            for (int k = 0; k < N; ++k) s += Uc[k] * ul[k][d];
-                            upv[d] = s;
-                        }
-                    } else {
-#pragma unroll
-                        for (int d = 0; d < D; ++d) upv[d] = svel[d];
-                    }
-                    double wpe = 1.0;
-                    if (P.peclet) {
-                        // peclet_blend: navier_stokes_fv1.cpp:871-892
-                        const double Pe = dotd<D>(sv, G.n) / nn * sqrt(dist2d<D>(co[to], co[fr])) / nu;
-                        wpe = Pe * Pe / (5.0 + Pe * Pe);
-#pragma unroll
-                        for (int d = 0; d < D; ++d) upv[d] = wpe * upv[d] + (1.0 - wpe) * sv[d];
-                    }
-                    const double prod = dotd<D>(sv, G.n) * rho;
-#pragma unroll
-                    for (int d1 = 0; d1 < D; ++d1) dacc[d1] += sgn * (upv[d1] * prod);
-
-                    // ---- Jacobian convection: navier_stokes_fv1.cpp:375-553 ----
-                    if (J) {
-#pragma unroll
-                        for (int sh = 0; sh < N; ++sh) {
-                            if (P.pac) {
-#pragma unroll
-                                for (int d1 = 0; d1 < D; ++d1) {
-                                    jacc[d1][sh][d1] += JT(sgn * (prod * wpe * Sv[sh]));
-                                    jacc[d1][sh][D] += JT(sgn * (prod * wpe * Sp[d1][sh]));
-                                }
-                            } else {
-                                const double cv = Uc[sh] * prod * wpe;
-#pragma unroll
-                                for (int d1 = 0; d1 < D; ++d1) jacc[d1][sh][d1] += JT(sgn * cv);
-                            }
-                            if (P.peclet) {
-                                const double cpe = prod * (1.0 - wpe) * G.phi[sh];
-#pragma unroll
-                                for (int d1 = 0; d1 < D; ++d1) jacc[d1][sh][d1] += JT(sgn * cpe);
-                            }
-                            if (P.exact != 0.0) {
-                                if (P.pac) {
-#pragma unroll
-                                    for (int d1 = 0; d1 < D; ++d1) {
-                                        const double pv = Sv[sh] * G.n[d1] * (P.exact * rho);
-#pragma unroll
-                                        for (int d2 = 0; d2 < D; ++d2) jacc[d1][sh][d2] += JT(sgn * (pv * upv[d1]));
-                                        double pp = 0;
-#pragma unroll
-                                        for (int k = 0; k < D; ++k) pp += Sp[k][sh] * G.n[k];
-                                        pp *= P.exact * rho;
-                                        jacc[d1][sh][D] += JT(sgn * (pp * upv[d1]));
-                                    }
-                                } else {
-#pragma unroll
-                                    for (int d1 = 0; d1 < D; ++d1)
-#pragma unroll
-                                        for (int d2 = 0; d2 < D; ++d2)
-                                            jacc[d1][sh][d2] += JT(sgn * (wpe * Uc[sh] * G.n[d2] * rho * upv[d1]));
-                                }
-                                if (P.peclet)
-#pragma unroll
-                                    for (int d1 = 0; d1 < D; ++d1)
-#pragma unroll
-                                        for (int d2 = 0; d2 < D; ++d2)
-                                            jacc[d1][sh][d2] += JT(sgn * (upv[d1] * (1.0 - wpe) * G.phi[sh] * G.n[d2] * rho));
-                            }
-                        }
-                    }
-                }
-                dacc[D] += sgn * (dotd<D>(svel, G.n) * rho);
-            }
-
-            // ---- Jacobian: diffusion, pressure, continuity ----
-            if (J) {
-                const double nr = -1.0 * nu * rho;
-#pragma unroll
-                for (int sh = 0; sh < N; ++sh) {
-                    double gn = 0;
-#pragma unroll
-                    for (int i = 0; i < D; ++i) gn += G.g[sh][i] * G.n[i];
-                    const double flux_sh = nr * gn;
-#pragma unroll
-                    for (int d1 = 0; d1 < D; ++d1) {
-                        jacc[d1][sh][d1] += JT(sgn * flux_sh);
-                        if (!P.laplace)
-#pragma unroll
-                            for (int d2 = 0; d2 < D; ++d2) jacc[d1][sh][d2] += JT(sgn * (nr * G.g[sh][d1] * G.n[d2]));
-                        jacc[d1][sh][D] += JT(sgn * (G.phi[sh] * G.n[d1]));
-                        jacc[D][sh][d1] += JT(sgn * (Sv[sh] * G.n[d1] * rho));
-                    }
-                    double cpp = 0;
-#pragma unroll
-                    for (int d1 = 0; d1 < D; ++d1) cpp += Sp[d1][sh] * G.n[d1] * rho;
-                    jacc[D][sh][D] += JT(sgn * cpp);
+                    if (wantJ) L.jacc[d][0][d] += JT(P.s_m * m);
+                    L.dacc[d] += P.s_m * (ul[0][d] * m);
                 }
             }
+            if ((P.parts & 4) && P.has_source)
+#pragma unroll
+                for (int d = 0; d < D; ++d) L.dacc[d] += P.s_f * (P.f[d] * volk * P.rho);
         }
-
-        // ---- mass part and source: navier_stokes_fv1.cpp:781-869 ----
-        if (P.parts & 2) {
-            const double m = vol[kv] * rho;
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                if (J) jacc[d][kv][d] += JT(P.s_m * m);
-                dacc[d] += P.s_m * (ul[kv][d] * m);
-            }
-        }
-        if ((P.parts & 4) && P.has_source) {
-#pragma unroll
-            for (int d = 0; d < D; ++d) dacc[d] += P.s_f * (P.f[d] * vol[kv] * rho);
-        }
-        if (!ok || !okg) err[0] = (ok ? 0 : KERR_CUT_SIDE) | (okg ? 0 : KERR_DEGENERATE);
-
-        // column slots of the element's corners in v's block row
+        if (!okg || !oku) err[0] = (oku ? 0 : KERR_CUT_SIDE) | (okg ? 0 : KERR_DEGENERATE);
+        // column slots of the (canonically ordered) corners in v's block row
         const int64_t rs = M.rowptr[(int64_t)v * NC];
         const int deg = (int)((M.rowptr[(int64_t)v * NC + 1] - rs) / NC);
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-            const int32_t key = cn[k] * NC;
+            const int32_t key = ccn[k] * NC;
             int lo = 0, hi = deg - 1, found = 0;
             while (lo <= hi) {
                 const int mid = (lo + hi) >> 1;
-                const int32_t c = M.colidx[rs + (int64_t)mid * NC];
-                if (c == key) { found = mid; break; }
-                if (c < key) lo = mid + 1; else hi = mid - 1;
+                const int32_t cc = M.colidx[rs + (int64_t)mid * NC];
+                if (cc == key) { found = mid; break; }
+                if (cc < key) lo = mid + 1; else hi = mid - 1;
             }
             pos[k] = found * NC;
         }
     }
+    int rbase = 0, rlen = 0;
+    if (active) {
+        const int64_t rs = M.rowptr[(int64_t)v * NC];
+        rbase = (int)(rs - jbase);
+        rlen = (int)(M.rowptr[(int64_t)v * NC + 1] - rs);
+    }
     __syncthreads();
 
-    // ---- ordered accumulation of the rows in LDS ----
+    // ordered accumulation: round r adds the incidences of slot r
     const int rounds = M.wg_rounds[w];
     for (int r = 0; r < rounds; ++r) {
         if (active && slot == r) {
-            const int64_t rs = M.rowptr[(int64_t)v * NC] - jbase;
-            const int64_t rlen = M.rowptr[(int64_t)v * NC + 1] - M.rowptr[(int64_t)v * NC];
-            if (J) {
+            if (wantJ) {
 #pragma unroll
-                for (int a = 0; a < NC; ++a)
+                for (int a = 0; a < NC; ++a) {
+                    const int ra = rbase + a * rlen;
 #pragma unroll
                     for (int k = 0; k < N; ++k)
 #pragma unroll
-                        for (int b = 0; b < NC; ++b) jbuf[rs + a * rlen + pos[k] + b] += jacc[a][k][b];
+                        for (int b = 0; b < NC; ++b) jbuf[ra + pos[k] + b] += L.jacc[a][k][b];
+                }
             }
 #pragma unroll
-            for (int a = 0; a < NC; ++a) dbuf[(v - v0) * NC + a] += dacc[a];
+            for (int a = 0; a < NC; ++a) dbuf[(v - v0) * NC + a] += L.dacc[a];
         }
         __syncthreads();
     }
 
-    // ---- coalesced write-back ----
-    if (J)
-        for (int64_t i = threadIdx.x; i < jlen; i += blockDim.x) J[jbase + i] = jbuf[i];
+    if (wantJ)
+        for (int i = threadIdx.x; i < jlen; i += blockDim.x) J[jbase + i] = jbuf[i];
     if (dout)
         for (int i = threadIdx.x; i < dlen; i += blockDim.x) dout[(int64_t)v0 * NC + i] = dbuf[i];
 }
 
-}  // namespace nsfv1
-
-namespace nsfv1 {
-
-hipError_t upload_topology(const RefTopo* topo4)
+// ---------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------
+template <int ET, typename JT>
+static hipError_t launch_cfg(int fast, const KMeshPtrs& m, const KParams& p, const double* u, const double* uold,
+                             JT* J, double* d, int* err, int nwg, size_t lds, hipStream_t s)
 {
-    return hipMemcpyToSymbol(HIP_SYMBOL(c_topo), topo4, sizeof(RefTopo) * 4);
+    switch (fast) {
+    case UPW_NO: assemble_rows<ET, JT, UPW_NO><<<nwg, 256, lds, s>>>(m, p, u, uold, J, d, err); break;
+    case UPW_FULL: assemble_rows<ET, JT, UPW_FULL><<<nwg, 256, lds, s>>>(m, p, u, uold, J, d, err); break;
+    case UPW_SKEWED: assemble_rows<ET, JT, UPW_SKEWED><<<nwg, 256, lds, s>>>(m, p, u, uold, J, d, err); break;
+    case UPW_LPS: assemble_rows<ET, JT, UPW_LPS><<<nwg, 256, lds, s>>>(m, p, u, uold, J, d, err); break;
+    default: assemble_rows<ET, JT, -1><<<nwg, 256, lds, s>>>(m, p, u, uold, J, d, err); break;
+    }
+    return hipGetLastError();
+}
+
+// the common configuration runs the specialised kernel
+static int fast_path(const KParams& p)
+{
+    const bool common = p.stab_type == STAB_FIELDS && !p.stokes && !p.pac && !p.peclet && p.exact == 0.0 &&
+                        p.conv_upwind == p.stab_upwind && p.conv_upwind >= 0;
+    return common ? p.conv_upwind : -1;
 }
 
 template <int ET>
 static hipError_t launch_et(int jprec, const KMeshPtrs& m, const KParams& p, const double* u, const double* uold,
-                            void* J, double* d, int* err, int nwg, int threads, size_t lds, hipStream_t s)
+                            void* J, double* d, int* err, int nwg, size_t lds, hipStream_t s)
 {
-    if (jprec == 64)
-        assemble_rows<ET, double><<<nwg, threads, lds, s>>>(m, p, u, uold, static_cast<double*>(J), d, err);
-    else
-        assemble_rows<ET, float><<<nwg, threads, lds, s>>>(m, p, u, uold, static_cast<float*>(J), d, err);
-    return hipGetLastError();
+    const int fast = fast_path(p);
+    if (jprec == 64) return launch_cfg<ET, double>(fast, m, p, u, uold, static_cast<double*>(J), d, err, nwg, lds, s);
+    return launch_cfg<ET, float>(fast, m, p, u, uold, static_cast<float*>(J), d, err, nwg, lds, s);
 }
 
-hipError_t launch_assemble(int et, int jprec, const KMeshPtrs& m, const KParams& p,
-                           const double* u, const double* uold, void* J, double* d, int* err,
-                           int nwg, int threads, size_t lds, hipStream_t s)
+hipError_t launch_assemble(int et, int jprec, const KMeshPtrs& m, const KParams& p, const double* u,
+                           const double* uold, void* J, double* d, int* err, int nwg, size_t lds, hipStream_t s)
 {
+    if (lds > 65536) return hipErrorInvalidValue;
     switch (et) {
-    case ET_TRI:  return launch_et<ET_TRI>(jprec, m, p, u, uold, J, d, err, nwg, threads, lds, s);
-    case ET_QUAD: return launch_et<ET_QUAD>(jprec, m, p, u, uold, J, d, err, nwg, threads, lds, s);
-    case ET_TET:  return launch_et<ET_TET>(jprec, m, p, u, uold, J, d, err, nwg, threads, lds, s);
-    case ET_HEX:  return launch_et<ET_HEX>(jprec, m, p, u, uold, J, d, err, nwg, threads, lds, s);
+    case ET_TRI:  return launch_et<ET_TRI>(jprec, m, p, u, uold, J, d, err, nwg, lds, s);
+    case ET_QUAD: return launch_et<ET_QUAD>(jprec, m, p, u, uold, J, d, err, nwg, lds, s);
+    case ET_TET:  return launch_et<ET_TET>(jprec, m, p, u, uold, J, d, err, nwg, lds, s);
+    case ET_HEX:  return launch_et<ET_HEX>(jprec, m, p, u, uold, J, d, err, nwg, lds, s);
     }
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_geometry_cache(int et, const double* xyz, const int32_t* conn, int64_t ne, int difflen,
+                                 double* l2inv, double* vol, int* err, hipStream_t s)
+{
+    const int blocks = (int)((ne + 255) / 256);
+    switch (et) {
+    case ET_TRI:  geometry_cache<ET_TRI><<<blocks, 256, 0, s>>>(xyz, conn, ne, difflen, l2inv, vol, err); break;
+    case ET_QUAD: geometry_cache<ET_QUAD><<<blocks, 256, 0, s>>>(xyz, conn, ne, difflen, l2inv, vol, err); break;
+    case ET_TET:  geometry_cache<ET_TET><<<blocks, 256, 0, s>>>(xyz, conn, ne, difflen, l2inv, vol, err); break;
+    case ET_HEX:  geometry_cache<ET_HEX><<<blocks, 256, 0, s>>>(xyz, conn, ne, difflen, l2inv, vol, err); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 }  // namespace nsfv1

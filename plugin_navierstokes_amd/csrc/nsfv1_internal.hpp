@@ -9,23 +9,27 @@
 
 namespace nsfv1 {
 
+// Device arrays of one mesh and one launch plan.
 struct KMeshPtrs {
-    const double* xyz;
-    const int32_t* conn;
-    const int64_t* rowptr;
-    const int32_t* colidx;
-    const int32_t* inc_elem;
-    const int32_t* inc_node;
-    const int32_t* inc_info;
-    const int64_t* node_inc;
-    const int32_t* wg_node;
-    const int32_t* wg_rounds;
+    const double* xyz;        // [nv][dim]
+    const int32_t* conn;      // [ne][nsh]
+    const int64_t* rowptr;    // [ndof+1]
+    const int32_t* colidx;    // [nnz]
+    const int32_t* node_inc;  // [nv+1] incidences of each node (node-major)
+    const int32_t* inc_elem;  // [ninc] element of the incidence
+    const int32_t* inc_node;  // [ninc] node of the incidence
+    const int32_t* inc_info;  // [ninc] corner | slot << 8
+    const int32_t* wg_node;   // [nwg+1] first node of each workgroup
+    const int32_t* wg_rounds; // [nwg] incidence slots of the workgroup
+    const double* l2inv;      // [ne][nip] diffusion-length factor per SCVF
+    const double* vol;        // [ne][nsh] SCV volumes
 };
 
-hipError_t upload_topology(const RefTopo* topo4);
+hipError_t launch_assemble(int et, int jprec, const KMeshPtrs& m, const KParams& p, const double* u,
+                           const double* uold, void* J, double* d, int* err, int nwg, size_t lds_bytes,
+                           hipStream_t stream);
 
-hipError_t launch_assemble(int et, int jprec, const KMeshPtrs& m, const KParams& p,
-                           const double* u, const double* uold, void* J, double* d, int* err,
-                           int nwg, int threads, size_t lds_bytes, hipStream_t stream);
+hipError_t launch_geometry_cache(int et, const double* xyz, const int32_t* conn, int64_t ne, int difflen,
+                                 double* l2inv, double* vol, int* err, hipStream_t stream);
 
 }  // namespace nsfv1
