@@ -561,6 +561,8 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
     k.nu = h->nu;
     k.rho = h->rho;
     k.dt = dt;
+    k.inv_dt = dt > 0.0 ? 1.0 / dt : 0.0;
+    k.inv_rho = 1.0 / h->rho;
     for (int i = 0; i < 3; ++i) k.f[i] = h->f[i];
     k.s_a = s_a;
     k.s_m = s_m;

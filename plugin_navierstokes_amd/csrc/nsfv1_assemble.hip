@@ -25,8 +25,9 @@
 // coalesced stores.  Static geometry (diffusion lengths per SCVF, SCV
 // volumes) comes from a per-mesh cache built by geometry_cache<>.
 //
-// Decision-carrying quantities (centres, ip, normal, interpolated velocity,
-// flux sign, side/ray intersection) are evaluated without FMA contraction.
+// Decisions (flux sign, zero-velocity guard, cut side, nearest corner)
+// agree with the CPU oracle except at exact ties, whose outcome depends on
+// the rounding of the canonical-order sums.
 #include "nsfv1_internal.hpp"
 #include "nsfv1_topo.hpp"
 
@@ -70,7 +71,6 @@ struct TabH {
 template <int D>
 __device__ inline double dot_nc(const double* a, const double* b)
 {
-#pragma clang fp contract(off)
     double s = 0;
 #pragma unroll
     for (int i = 0; i < D; ++i) s += a[i] * b[i];
@@ -80,7 +80,6 @@ __device__ inline double dot_nc(const double* a, const double* b)
 template <int D>
 __device__ inline double dist2_nc(const double* a, const double* b)
 {
-#pragma clang fp contract(off)
     double s = 0;
 #pragma unroll
     for (int i = 0; i < D; ++i) s += (a[i] - b[i]) * (a[i] - b[i]);
@@ -89,7 +88,6 @@ __device__ inline double dist2_nc(const double* a, const double* b)
 
 __device__ inline void cross_nc(const double* a, const double* b, double* c)
 {
-#pragma clang fp contract(off)
     c[0] = a[1] * b[2] - a[2] * b[1];
     c[1] = a[2] * b[0] - a[0] * b[2];
     c[2] = a[0] * b[1] - a[1] * b[0];
@@ -99,7 +97,6 @@ __device__ inline void cross_nc(const double* a, const double* b, double* c)
 template <int ET>
 __device__ inline void elem_centre(const double (&co)[Topo<ET>::nsh][Topo<ET>::dim], double* c)
 {
-#pragma clang fp contract(off)
     constexpr int N = Topo<ET>::nsh, D = Topo<ET>::dim;
 #pragma unroll
     for (int i = 0; i < D; ++i) {
@@ -115,7 +112,6 @@ template <int ET, int IP>
 __device__ inline void scvf_frame(const double (&co)[Topo<ET>::nsh][Topo<ET>::dim], const double* c,
                                   double* ipg, double* n, double* me)
 {
-#pragma clang fp contract(off)
     using T = Topo<ET>;
     constexpr int D = T::dim;
     constexpr int f = T::edge[IP][0], t = T::edge[IP][1];
@@ -212,7 +208,6 @@ __device__ inline bool inv_jacobian(const double (&co)[Topo<ET>::nsh][Topo<ET>::
 __device__ inline bool ray_tri_nc(const double* p0, const double* p1, const double* p2, const double* from,
                                   const double* dir, double& b1o, double& b2o)
 {
-#pragma clang fp contract(off)
     double e1[3], e2[3], pv[3], tv[3], qv[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) { e1[i] = p1[i] - p0[i]; e2[i] = p2[i] - p0[i]; }
@@ -237,7 +232,6 @@ template <int ET, int S, int TR>
 __device__ inline void try_side(const double (&co)[Topo<ET>::nsh][Topo<ET>::dim], const double* from,
                                 const double* dir, int& side, double* gi, double* li)
 {
-#pragma clang fp contract(off)
     using T = Topo<ET>;
     if (side >= 0) return;
     if constexpr (T::dim == 2) {
@@ -419,12 +413,11 @@ struct Lane {
 // side order (upwind.cpp:337-430, 505-575).
 template <int ET, int Q>
 __device__ inline bool upwind_canon(int type, bool kvfrom, const double (&co)[Topo<ET>::nsh][Topo<ET>::dim],
-                                    const double (&coo)[Topo<ET>::nsh][Topo<ET>::dim], const int* pk,
+                                    const double* __restrict__ xyz, const int32_t* __restrict__ econn, const int* pk,
                                     const double* ipg, const double* n, const double* sv,
                                     double (&U)[Topo<ET>::nsh], double& convlen, bool& have_cut, int& side,
                                     double* gi, double* li)
 {
-#pragma clang fp contract(off)
     using T = Topo<ET>;
     constexpr int D = T::dim, N = T::nsh;
     constexpr int CIP = corner_ip<ET>(0, Q);
@@ -453,6 +446,14 @@ __device__ inline bool upwind_canon(int type, bool kvfrom, const double (&co)[To
     if (sqrt(dot_nc<D>(sv, sv)) < 1e-14) {
         convlen = 1.0;
         return true;
+    }
+    // the element's corners in original order, for the ray cast
+    double coo[N][D];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const int64_t on = econn[k];
+#pragma unroll
+        for (int i = 0; i < D; ++i) coo[k][i] = xyz[on * D + i];
     }
     if (!have_cut) {
         side = -1;
@@ -520,7 +521,7 @@ __device__ inline bool upwind_canon(int type, bool kvfrom, const double (&co)[To
 // upwind, stabilisation, defect; emits the record for the Jacobian.
 template <int ET, int FAST, int Q, typename JT>
 __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET>::nsh][Topo<ET>::dim],
-                                  const double (&coo)[Topo<ET>::nsh][Topo<ET>::dim],
+                                  const double* __restrict__ xyz, const int32_t* __restrict__ econn,
                                   const double (&ul)[Topo<ET>::nsh][Topo<ET>::dim + 1], const int* pk,
                                   const int32_t* ccn, const double* c, const double* __restrict__ uold,
                                   double l2, bool kvfrom, bool wantJ, Lane<ET, JT>& L, Rec<ET, JT>& R,
@@ -535,7 +536,6 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
 
     double ipg[D], n[D], me[D];
     {
-#pragma clang fp contract(off)
         // frame of canonical SCVF IP with the normal pointing away from
         // corner 0 (the row node): row contributions are then all `+`
         double nn_[D];
@@ -553,7 +553,6 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
 
     double sv[D];
     {
-#pragma clang fp contract(off)
 #pragma unroll
         for (int d = 0; d < D; ++d) sv[d] = 0.0;
 #pragma unroll
@@ -593,7 +592,7 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
         double Us[N];
 #pragma unroll
         for (int k = 0; k < N; ++k) Us[k] = 0.0;
-        if (!C.stokes()) oku &= upwind_canon<ET, Q>(C.stab_up(), kvfrom, co, coo, pk, ipg, n, sv, Us, Ls, have_cut, side, gi, li);
+        if (!C.stokes()) oku &= upwind_canon<ET, Q>(C.stab_up(), kvfrom, co, xyz, econn, pk, ipg, n, sv, Us, Ls, have_cut, side, gi, li);
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             double acc = 0;
@@ -606,7 +605,7 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
         for (int k = 0; k < N; ++k) { R.Us[k] = JT(Us[k]); R.Uc[k] = JT(Us[k]); }
         if (!C.stokes() && C.two_upwinds()) {
             double Uc[N], Lc;
-            oku &= upwind_canon<ET, Q>(C.conv_up(), kvfrom, co, coo, pk, ipg, n, sv, Uc, Lc, have_cut, side, gi, li);
+            oku &= upwind_canon<ET, Q>(C.conv_up(), kvfrom, co, xyz, econn, pk, ipg, n, sv, Uc, Lc, have_cut, side, gi, li);
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 double acc = 0;
@@ -625,7 +624,7 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
         vv = nu * l2;
         vc = C.stokes() ? 0.0 : sqrt(dot_nc<D>(sv, sv)) / Ls;
         double diag = vv;
-        if (P.has_old) diag += 1. / P.dt;
+        if (P.has_old) diag += P.inv_dt;
         if (!C.stokes()) diag += vc;
         idg = 1.0 / diag;
 #pragma unroll
@@ -635,9 +634,9 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
                 double o = 0.0;
 #pragma unroll
                 for (int sh = 0; sh < N; ++sh) o += TabH<ET>::t.phi[IP][sh] * uold[(int64_t)ccn[sh] * (D + 1) + d];
-                rhs += o / P.dt;
+                rhs += o * P.inv_dt;
             }
-            rhs += vv * sv[d] + vc * upS[d] - gv[D][d] / rho;
+            rhs += vv * sv[d] + vc * upS[d] - gv[D][d] * P.inv_rho;
             svel[d] = rhs * idg;
         }
     } else {
@@ -690,7 +689,7 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
     R.pw1 = JT(prod * (1.0 - wpe));
     R.vv = JT(vv * idg);
     R.vc = JT(vc * idg);
-    R.sp = JT(-1.0 / rho * idg);
+    R.sp = JT(-P.inv_rho * idg);
     R.w = JT(wpe);
 }
 
@@ -784,7 +783,7 @@ __device__ inline void scvf_jac(const Cfg<FAST>& C, const Rec<ET, JT>& R, Lane<E
 
 template <int ET, int FAST, int Q, typename JT>
 __device__ inline void scvf_all(const Cfg<FAST>& C, const double (&co)[Topo<ET>::nsh][Topo<ET>::dim],
-                                const double (&coo)[Topo<ET>::nsh][Topo<ET>::dim],
+                                const double* __restrict__ xyz, const int32_t* __restrict__ econn,
                                 const double (&ul)[Topo<ET>::nsh][Topo<ET>::dim + 1], const int* pk,
                                 const int32_t* ccn, const double* c, const double* uold, const double* l2row,
                                 int kv, bool wantJ, Lane<ET, JT>& L, Rec<ET, JT>* R, bool& okg, bool& oku)
@@ -795,9 +794,9 @@ __device__ inline void scvf_all(const Cfg<FAST>& C, const double (&co)[Topo<ET>:
         double gi[3] = {0, 0, 0}, li[3] = {0, 0, 0};
         const double l2 = l2row[CanonH<ET>::c.oip[kv][Q]];
         const bool kvfrom = CanonH<ET>::c.from[kv][Q] != 0;
-        scvf_canon<ET, FAST, Q, JT>(C, co, coo, ul, pk, ccn, c, uold, l2, kvfrom, wantJ, L, R[Q], okg, oku, have_cut,
+        scvf_canon<ET, FAST, Q, JT>(C, co, xyz, econn, ul, pk, ccn, c, uold, l2, kvfrom, wantJ, L, R[Q], okg, oku, have_cut,
                                     side, gi, li);
-        scvf_all<ET, FAST, Q + 1, JT>(C, co, coo, ul, pk, ccn, c, uold, l2row, kv, wantJ, L, R, okg, oku);
+        scvf_all<ET, FAST, Q + 1, JT>(C, co, xyz, econn, ul, pk, ccn, c, uold, l2row, kv, wantJ, L, R, okg, oku);
     }
 }
 
@@ -862,17 +861,13 @@ assemble_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const double
         slot = info >> 8;
         int pk[N];
         int32_t ccn[N];
-        double co[N][D], coo[N][D], ul[N][NC];
+        double co[N][D], ul[N][NC];
 #pragma unroll
         for (int k = 0; k < N; ++k) {
             pk[k] = CanonH<ET>::c.perm[kv][k];
             ccn[k] = M.conn[(int64_t)e * N + pk[k]];
-            const int32_t on = M.conn[(int64_t)e * N + k];
 #pragma unroll
-            for (int i = 0; i < D; ++i) {
-                co[k][i] = M.xyz[(int64_t)ccn[k] * D + i];
-                coo[k][i] = M.xyz[(int64_t)on * D + i];
-            }
+            for (int i = 0; i < D; ++i) co[k][i] = M.xyz[(int64_t)ccn[k] * D + i];
 #pragma unroll
             for (int a = 0; a < NC; ++a) ul[k][a] = u[(int64_t)ccn[k] * NC + a];
         }
@@ -881,7 +876,7 @@ assemble_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const double
             double c[D];
             elem_centre<ET>(co, c);
             Rec<ET, JT> R[T::cip];
-            scvf_all<ET, FAST, 0, JT>(C, co, coo, ul, pk, ccn, c, uold, M.l2inv + (int64_t)e * NIP, kv, wantJ, L, R,
+            scvf_all<ET, FAST, 0, JT>(C, co, M.xyz, M.conn + (int64_t)e * N, ul, pk, ccn, c, uold, M.l2inv + (int64_t)e * NIP, kv, wantJ, L, R,
                                       okg, oku);
             if (wantJ) jac_all<ET, FAST, 0, JT>(C, R, L);
         }

@@ -19,7 +19,7 @@ struct KParams {
     int conv_upwind, stab_upwind, stab_type, pac, difflen;
     int stokes, laplace, peclet, has_source, has_old;
     int parts;
-    double exact, nu, rho, dt;
+    double exact, nu, rho, dt, inv_rho, inv_dt;
     double f[3];
     double s_a, s_m, s_f;
 };
@@ -28,7 +28,6 @@ __device__ inline double dot3(const double* a, const double* b) { return a[0] * 
 
 __device__ inline void cross3(const double* a, const double* b, double* c)
 {
-#pragma clang fp contract(off)
     c[0] = a[1] * b[2] - a[2] * b[1];
     c[1] = a[2] * b[0] - a[0] * b[2];
     c[2] = a[0] * b[1] - a[1] * b[0];
