@@ -162,6 +162,32 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
    element with a zero Jacobian determinant. */
 int nsfv1_sync_errors(nsfv1_handle* h, void* stream);
 
+/* ---- Newton-loop services (ugcore DirichletBoundary / linear solver, which
+   the reference delegates to; not part of the plugin) ---- */
+
+/* Dirichlet DoFs (HOST array of DoF indices).  Replaces the constraint rows
+   that ugcore's DirichletBoundary sets for NavierStokesWall /
+   NavierStokesInflow (incompressible/bnd/wall_impl.h:57-69,
+   incompressible/bnd/inflow_base_impl.h:44-63). */
+int nsfv1_set_dirichlet(nsfv1_handle* h, const int64_t* dofs, int64_t n);
+
+/* Rows of the Dirichlet DoFs: J row -> identity, defect -> u - g, where g
+   (device, one value per DoF in set_dirichlet order) may be NULL (g = 0).
+   jvals or defect may be NULL.  Device pointers, asynchronous. */
+int nsfv1_apply_dirichlet(nsfv1_handle* h, void* jvals, int jprec, double* defect, const double* u,
+                          const double* g, void* stream);
+
+/* y = J x with the assembled Jacobian (device pointers). */
+int nsfv1_spmv(nsfv1_handle* h, const void* jvals, int jprec, const double* x, double* y, void* stream);
+
+/* Restarted GMRES(restart), right-preconditioned with point-block Jacobi
+   ((dim+1)x(dim+1) node blocks), on J x = b: device FP64 b and x (x is
+   overwritten; zero initial guess), J in jprec.  Synchronises the stream.
+   iters/relres (host, may be NULL) receive the inner iterations and
+   ||b - J x|| / ||b||. */
+int nsfv1_linear_solve(nsfv1_handle* h, const void* jvals, int jprec, const double* b, double* x, double rtol,
+                       int maxit, int restart, int* iters, double* relres, void* stream);
+
 /* Launch shape of the assembly: incidences (threads) per workgroup (64..256,
    a multiple of 64) and the LDS budget per workgroup in bytes (4..64 KiB);
    0 keeps the current value.  Used by the autotuner. */

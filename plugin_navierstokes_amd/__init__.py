@@ -40,6 +40,7 @@ EXPORTS = [
     "nsfv1_set_upwind", "nsfv1_set_stabilization", "nsfv1_set_stabilization_without",
     "nsfv1_set_stabilization_upwind", "nsfv1_set_pac_upwind", "nsfv1_set_mesh", "nsfv1_num_dofs", "nsfv1_nnz",
     "nsfv1_get_pattern", "nsfv1_pattern_device", "nsfv1_assemble", "nsfv1_sync_errors", "nsfv1_set_launch",
+    "nsfv1_set_dirichlet", "nsfv1_apply_dirichlet", "nsfv1_spmv", "nsfv1_linear_solve",
 ]
 
 
@@ -86,6 +87,11 @@ def load_library(path: str | None = None):
     L.nsfv1_assemble.argtypes = [H, vp, vp, dp, dp, dp, dp, C.c_int, vp, C.c_int, vp, vp]
     L.nsfv1_sync_errors.argtypes = [H, vp]
     L.nsfv1_set_launch.argtypes = [H, C.c_int, C.c_int]
+    L.nsfv1_set_dirichlet.argtypes = [H, vp, C.c_int64]
+    L.nsfv1_apply_dirichlet.argtypes = [H, vp, C.c_int, vp, vp, vp, vp]
+    L.nsfv1_spmv.argtypes = [H, vp, C.c_int, vp, vp, vp]
+    L.nsfv1_linear_solve.argtypes = [H, vp, C.c_int, vp, vp, dp, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_double), vp]
     _lib = L
     return L
 
@@ -237,6 +243,26 @@ class NavierStokesFV1:
         if check:
             self.sync_errors(stream)
         return (J if want_J else None), (d if want_d else None)
+
+    # --- Newton-loop services ---
+    def set_dirichlet(self, dofs):
+        dofs = np.ascontiguousarray(dofs, dtype=np.int64)
+        self._ndir = len(dofs)
+        _check(_lib.nsfv1_set_dirichlet(self._h, dofs.ctypes.data, len(dofs)))
+
+    def apply_dirichlet(self, J=None, jprec=F64, d=None, u=None, g=None, stream=None):
+        _check(_lib.nsfv1_apply_dirichlet(self._h, _ptr(J), int(jprec), _ptr(d), _ptr(u), _ptr(g),
+                                          _stream_ptr(stream)))
+
+    def spmv(self, J, x, y, jprec=F64, stream=None):
+        _check(_lib.nsfv1_spmv(self._h, _ptr(J), int(jprec), _ptr(x), _ptr(y), _stream_ptr(stream)))
+
+    def linear_solve(self, J, b, x, jprec=F64, rtol=1e-6, maxit=500, restart=30, stream=None):
+        it = C.c_int(0)
+        rr = C.c_double(0.0)
+        _check(_lib.nsfv1_linear_solve(self._h, _ptr(J), int(jprec), _ptr(b), _ptr(x), float(rtol), int(maxit),
+                                       int(restart), C.byref(it), C.byref(rr), _stream_ptr(stream)))
+        return it.value, rr.value
 
     def sync_errors(self, stream=None):
         _check(_lib.nsfv1_sync_errors(self._h, _stream_ptr(stream)))

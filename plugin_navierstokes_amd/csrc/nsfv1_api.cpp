@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cctype>
 #include <cstdio>
 #include <cstring>
@@ -17,6 +18,7 @@
 
 #include "../../include/nsfv1.h"
 #include "nsfv1_internal.hpp"
+#include "nsfv1_solver.hpp"
 
 using namespace nsfv1;
 
@@ -98,6 +100,10 @@ struct nsfv1_handle {
     int max_threads = 256;
     size_t lds_cap = 64 * 1024;
     Plan plan32, plan64;
+    // Newton-loop linear algebra
+    DevBuf dir_rows, gm_V, gm_Binv, gm_part, gm_small, gm_w;
+    int64_t ndir = 0;
+    int gm_restart = 0;
     bool geo_valid = false;
     int geo_difflen = -1;
 };
@@ -600,6 +606,162 @@ int nsfv1_sync_errors(nsfv1_handle* h, void* stream)
         if (flag & KERR_CUT_SIDE) return fail(NSFV1_ERR_CUT_SIDE, "GetNodeNextToCut: Cannot find cut side.");
         return fail(NSFV1_ERR_DEGENERATE, "degenerate element");
     }
+    return NSFV1_OK;
+}
+
+
+// Dirichlet DoFs (host array, any order); rows of J become identity rows and
+// the defect entries u - g (ugcore DirichletBoundary row treatment).
+int nsfv1_set_dirichlet(nsfv1_handle* h, const int64_t* dofs, int64_t n)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    std::vector<int64_t> d(dofs, dofs + (n > 0 ? n : 0));
+    for (int64_t x : d)
+        if (x < 0 || x >= h->ndof) return fail(NSFV1_ERR_ARGUMENT, "Dirichlet DoF out of range");
+    if (h->dir_rows.alloc(sizeof(int64_t) * std::max<int64_t>(n, 1)) != hipSuccess ||
+        (n > 0 && hipMemcpy(h->dir_rows.p, d.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice) != hipSuccess))
+        return fail(NSFV1_ERR_DEVICE, "Dirichlet upload");
+    h->ndir = n;
+    return NSFV1_OK;
+}
+
+int nsfv1_apply_dirichlet(nsfv1_handle* h, void* jvals, int jprec, double* defect, const double* u,
+                          const double* g, void* stream)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    if (defect && !u) return fail(NSFV1_ERR_ARGUMENT, "u is NULL");
+    hipError_t e = launch_dirichlet(jprec, h->rowptr.as<int64_t>(), h->colidx.as<int32_t>(), h->dir_rows.as<int64_t>(),
+                                    h->ndir, jvals, defect, u, g, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("dirichlet: ") + hipGetErrorString(e));
+    return NSFV1_OK;
+}
+
+int nsfv1_spmv(nsfv1_handle* h, const void* jvals, int jprec, const double* x, double* y, void* stream)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    hipError_t e = launch_spmv(jprec, h->rowptr.as<int64_t>(), h->colidx.as<int32_t>(), jvals, x, y, h->ndof,
+                               (hipStream_t)stream);
+    if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("spmv: ") + hipGetErrorString(e));
+    return NSFV1_OK;
+}
+
+// Restarted GMRES(m), right-preconditioned with point-block Jacobi, for
+// J x = b with the device Jacobian (jprec) and FP64 vectors.  x is
+// overwritten (zero initial guess).  Stops when ||b - J x|| <= rtol ||b||
+// or after maxit inner iterations.
+int nsfv1_linear_solve(nsfv1_handle* h, const void* jvals, int jprec, const double* b, double* x, double rtol,
+                       int maxit, int restart, int* iters, double* relres, void* stream)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    if (restart < 1 || restart > GMRES_MAX_RESTART) return fail(NSFV1_ERR_ARGUMENT, "restart must be 1..60");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = h->ndof, nv = h->nv;
+    const int nc = h->dim + 1, m = restart;
+    if (h->gm_restart != m) {
+        if (h->gm_V.alloc(sizeof(double) * n * (m + 1)) != hipSuccess || h->gm_w.alloc(sizeof(double) * n * 2) != hipSuccess ||
+            h->gm_Binv.alloc(sizeof(double) * nv * nc * nc) != hipSuccess ||
+            h->gm_part.alloc(sizeof(double) * RED_PARTIALS) != hipSuccess ||
+            h->gm_small.alloc(sizeof(double) * 256) != hipSuccess)
+            return fail(NSFV1_ERR_DEVICE, "GMRES workspace");
+        h->gm_restart = m;
+    }
+    double* V = h->gm_V.as<double>();
+    double* w = h->gm_w.as<double>();
+    double* z = w + n;
+    double* Binv = h->gm_Binv.as<double>();
+    double* part = h->gm_part.as<double>();
+    double* dsm = h->gm_small.as<double>();
+    const int64_t* rp = h->rowptr.as<int64_t>();
+    const int32_t* ci = h->colidx.as<int32_t>();
+    auto chk = [&](hipError_t e) { return e == hipSuccess; };
+    auto dot = [&](const double* Vb, int k, const double* vec, double* out) -> bool {
+        if (!chk(launch_multi_dot(Vb, n, k, vec, n, part, dsm, s))) return false;
+        return hipMemcpyAsync(out, dsm, sizeof(double) * k, hipMemcpyDeviceToHost, s) == hipSuccess &&
+               hipStreamSynchronize(s) == hipSuccess;
+    };
+    if (!chk(launch_block_jacobi_setup(jprec, nc, rp, ci, jvals, nv, Binv, h->errflag.as<int>(), s)))
+        return fail(NSFV1_ERR_DEVICE, "block Jacobi setup");
+    if (hipMemsetAsync(x, 0, sizeof(double) * n, s) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "memset");
+    double bn2;
+    if (!dot(b, 1, b, &bn2)) return fail(NSFV1_ERR_DEVICE, "dot");
+    const double bnorm = std::sqrt(bn2);
+    int it = 0;
+    double res = bnorm;
+    if (bnorm == 0.0) {
+        if (iters) *iters = 0;
+        if (relres) *relres = 0.0;
+        return NSFV1_OK;
+    }
+    std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+    while (it < maxit) {
+        // r = b - J x  -> V0
+        if (!chk(launch_spmv(jprec, rp, ci, jvals, x, w, n, s)) || !chk(launch_axpby(1.0, b, -1.0, w, n, s)))
+            return fail(NSFV1_ERR_DEVICE, "residual");
+        double rn2;
+        if (!dot(w, 1, w, &rn2)) return fail(NSFV1_ERR_DEVICE, "dot");
+        const double beta = std::sqrt(rn2);
+        res = beta;
+        if (beta <= rtol * bnorm) break;
+        if (!chk(launch_axpby(1.0 / beta, w, 0.0, V, n, s))) return fail(NSFV1_ERR_DEVICE, "scale");
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int j = 0;
+        for (; j < m && it < maxit; ++j, ++it) {
+            if (!chk(launch_block_jacobi_apply(nc, Binv, V + (int64_t)j * n, z, nv, s)) ||
+                !chk(launch_spmv(jprec, rp, ci, jvals, z, w, n, s)))
+                return fail(NSFV1_ERR_DEVICE, "arnoldi");
+            // classical Gram-Schmidt, twice
+            double hcol[GMRES_MAX_RESTART + 1], h2[GMRES_MAX_RESTART + 1];
+            for (int pass = 0; pass < 2; ++pass) {
+                double* hc = pass == 0 ? hcol : h2;
+                if (!dot(V, j + 1, w, hc)) return fail(NSFV1_ERR_DEVICE, "dot");
+                if (hipMemcpyAsync(dsm + 128, hc, sizeof(double) * (j + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
+                    !chk(launch_multi_axpy(V, n, j + 1, dsm + 128, w, n, s)))
+                    return fail(NSFV1_ERR_DEVICE, "orthogonalise");
+            }
+            for (int i = 0; i <= j; ++i) H[i * m + j] = hcol[i] + h2[i];
+            double wn2;
+            if (!dot(w, 1, w, &wn2)) return fail(NSFV1_ERR_DEVICE, "dot");
+            const double hn = std::sqrt(wn2);
+            H[(j + 1) * m + j] = hn;
+            if (hn > 0 && !chk(launch_axpby(1.0 / hn, w, 0.0, V + (int64_t)(j + 1) * n, n, s)))
+                return fail(NSFV1_ERR_DEVICE, "scale");
+            // Givens rotations
+            for (int i = 0; i < j; ++i) {
+                const double t = cs[i] * H[i * m + j] + sn[i] * H[(i + 1) * m + j];
+                H[(i + 1) * m + j] = -sn[i] * H[i * m + j] + cs[i] * H[(i + 1) * m + j];
+                H[i * m + j] = t;
+            }
+            const double a = H[j * m + j], c = H[(j + 1) * m + j];
+            const double r = std::hypot(a, c);
+            cs[j] = r > 0 ? a / r : 1.0;
+            sn[j] = r > 0 ? c / r : 0.0;
+            H[j * m + j] = r;
+            H[(j + 1) * m + j] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            res = std::fabs(g[j + 1]);
+            if (res <= rtol * bnorm || hn == 0.0) { ++j; ++it; break; }
+        }
+        // x += M^-1 V y
+        for (int i = j - 1; i >= 0; --i) {
+            double t = g[i];
+            for (int k = i + 1; k < j; ++k) t -= H[i * m + k] * y[k];
+            y[i] = H[i * m + i] != 0.0 ? t / H[i * m + i] : 0.0;
+        }
+        if (hipMemcpyAsync(dsm + 128, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemsetAsync(w, 0, sizeof(double) * n, s) != hipSuccess || !chk(launch_multi_add(V, n, j, dsm + 128, w, n, s)) ||
+            !chk(launch_block_jacobi_apply(nc, Binv, w, z, nv, s)) || !chk(launch_axpby(1.0, z, 1.0, x, n, s)))
+            return fail(NSFV1_ERR_DEVICE, "update");
+        if (res <= rtol * bnorm) break;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "sync");
+    if (iters) *iters = it;
+    if (relres) *relres = res / bnorm;
     return NSFV1_OK;
 }
 

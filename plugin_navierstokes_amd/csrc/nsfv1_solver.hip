@@ -1,0 +1,281 @@
+// nsfv1_solver.hip -- device linear algebra for the Newton loop around the
+// FV1 assembly: Dirichlet rows, CSR SpMV with the (FP32 or FP64) Jacobian on
+// FP64 vectors, point-block Jacobi (one (dim+1)x(dim+1) block per node), and
+// the BLAS-1 kernels of restarted GMRES.  All reductions use a fixed
+// decomposition and order, so results are bitwise reproducible.
+//
+// The reference delegates this to ugcore (DirichletBoundary, NewtonSolver,
+// linear solvers), which is not part of the plugin; these kernels provide the
+// same service for the device-resident Jacobian and defect.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nsfv1_solver.hpp"
+
+namespace nsfv1 {
+
+// ---------------------------------------------------------------------
+// Dirichlet rows: J row -> identity, d = u - g (ugcore DirichletBoundary
+// row treatment, restated)
+// ---------------------------------------------------------------------
+template <typename JT>
+__global__ void dirichlet_rows(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+                               const int64_t* __restrict__ rows, int64_t nrows, JT* __restrict__ J,
+                               double* __restrict__ d, const double* __restrict__ u, const double* __restrict__ g)
+{
+    const int64_t r = (int64_t)blockIdx.x;
+    if (r >= nrows) return;
+    const int64_t row = rows[r];
+    if (J)
+        for (int64_t k = rowptr[row] + threadIdx.x; k < rowptr[row + 1]; k += blockDim.x)
+            J[k] = colidx[k] == row ? JT(1) : JT(0);
+    if (threadIdx.x == 0 && d) d[row] = u[row] - (g ? g[r] : 0.0);
+}
+
+// ---------------------------------------------------------------------
+// y = A x, 16 lanes per row, fixed-order shuffle reduction
+// ---------------------------------------------------------------------
+template <typename JT>
+__global__ void __launch_bounds__(256)
+spmv_csr(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colidx, const JT* __restrict__ A,
+         const double* __restrict__ x, double* __restrict__ y, int64_t n)
+{
+    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int sub = threadIdx.x & 15;
+    double s = 0.0;
+    if (row < n) {
+        const int64_t a = rowptr[row], b = rowptr[row + 1];
+        for (int64_t k = a + sub; k < b; k += 16) s += (double)A[k] * x[colidx[k]];
+    }
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
+    if (row < n && sub == 0) y[row] = s;
+}
+
+// ---------------------------------------------------------------------
+// point-block Jacobi: invert the (nc x nc) diagonal block of every node
+// (Gauss-Jordan with partial pivoting in FP64)
+// ---------------------------------------------------------------------
+template <typename JT, int NC>
+__global__ void block_jacobi_setup(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+                                   const JT* __restrict__ A, int64_t nv, double* __restrict__ Binv,
+                                   int* __restrict__ err)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    double M[NC][2 * NC];
+    const int64_t r0 = rowptr[v * NC];
+    const int deg = (int)((rowptr[v * NC + 1] - r0) / NC);
+    // slot of the diagonal block in the row
+    int lo = 0, hi = deg - 1, j = 0;
+    const int32_t key = (int32_t)(v * NC);
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int32_t c = colidx[r0 + (int64_t)mid * NC];
+        if (c == key) { j = mid; break; }
+        if (c < key) lo = mid + 1; else hi = mid - 1;
+    }
+#pragma unroll
+    for (int a = 0; a < NC; ++a) {
+        const int64_t ra = rowptr[v * NC + a] + (int64_t)j * NC;
+#pragma unroll
+        for (int b = 0; b < NC; ++b) {
+            M[a][b] = (double)A[ra + b];
+            M[a][NC + b] = a == b ? 1.0 : 0.0;
+        }
+    }
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        int p = c;
+        double best = fabs(M[c][c]);
+#pragma unroll
+        for (int r = c + 1; r < NC; ++r)
+            if (fabs(M[r][c]) > best) { best = fabs(M[r][c]); p = r; }
+        if (best == 0.0) { ok = false; break; }
+        if (p != c)
+#pragma unroll
+            for (int k = 0; k < 2 * NC; ++k) { const double t = M[c][k]; M[c][k] = M[p][k]; M[p][k] = t; }
+        const double ip = 1.0 / M[c][c];
+#pragma unroll
+        for (int k = 0; k < 2 * NC; ++k) M[c][k] *= ip;
+#pragma unroll
+        for (int r = 0; r < NC; ++r)
+            if (r != c) {
+                const double f = M[r][c];
+#pragma unroll
+                for (int k = 0; k < 2 * NC; ++k) M[r][k] -= f * M[c][k];
+            }
+    }
+#pragma unroll
+    for (int a = 0; a < NC; ++a)
+#pragma unroll
+        for (int b = 0; b < NC; ++b) Binv[(v * NC + a) * NC + b] = ok ? M[a][NC + b] : (a == b ? 1.0 : 0.0);
+    if (!ok) err[0] = 4;
+}
+
+template <int NC>
+__global__ void block_jacobi_apply(const double* __restrict__ Binv, const double* __restrict__ x,
+                                   double* __restrict__ y, int64_t nv)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    double xv[NC];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) xv[a] = x[v * NC + a];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) {
+        double s = 0;
+#pragma unroll
+        for (int b = 0; b < NC; ++b) s += Binv[(v * NC + a) * NC + b] * xv[b];
+        y[v * NC + a] = s;
+    }
+}
+
+// ---------------------------------------------------------------------
+// BLAS-1 with deterministic two-stage reductions
+// ---------------------------------------------------------------------
+constexpr int RED_BLOCKS = 1024;
+
+__device__ inline double block_sum(double s, double* sh)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) sh[w] = s;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[i];
+    __syncthreads();
+    return t;
+}
+
+// partial[b][i] = sum over block b's share of V_i . w, i < k
+__global__ void __launch_bounds__(256)
+multi_dot_partial(const double* __restrict__ V, int64_t ld, int k, const double* __restrict__ w, int64_t n,
+                  double* __restrict__ partial)
+{
+    __shared__ double sh[4];
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t a = (int64_t)blockIdx.x * chunk, b = a + chunk < n ? a + chunk : n;
+    for (int i = 0; i < k; ++i) {
+        double s = 0;
+        for (int64_t t = a + threadIdx.x; t < b; t += blockDim.x) s += V[i * ld + t] * w[t];
+        const double tot = block_sum(s, sh);
+        if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * 64 + i] = tot;
+    }
+}
+
+__global__ void multi_dot_final(const double* __restrict__ partial, int nblk, int k, double* __restrict__ out)
+{
+    const int i = threadIdx.x;
+    if (i >= k) return;
+    double s = 0;
+    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * 64 + i];
+    out[i] = s;
+}
+
+// w -= sum_i h_i V_i
+__global__ void multi_axpy(const double* __restrict__ V, int64_t ld, int k, const double* __restrict__ h,
+                           double* __restrict__ w, int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double s = w[t];
+    for (int i = 0; i < k; ++i) s -= h[i] * V[i * ld + t];
+    w[t] = s;
+}
+
+// y = a x + b y
+__global__ void axpby(double a, const double* __restrict__ x, double b, double* __restrict__ y, int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) y[t] = a * x[t] + b * y[t];
+}
+
+// x += sum_i c_i V_i
+__global__ void multi_add(const double* __restrict__ V, int64_t ld, int k, const double* __restrict__ c,
+                          double* __restrict__ x, int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double s = x[t];
+    for (int i = 0; i < k; ++i) s += c[i] * V[i * ld + t];
+    x[t] = s;
+}
+
+// ---------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------
+static inline int blocks_for(int64_t n, int t) { return (int)((n + t - 1) / t); }
+
+hipError_t launch_dirichlet(int jprec, const int64_t* rowptr, const int32_t* colidx, const int64_t* rows,
+                            int64_t nrows, void* J, double* d, const double* u, const double* g, hipStream_t s)
+{
+    if (nrows <= 0) return hipSuccess;
+    if (jprec == 64)
+        dirichlet_rows<double><<<(unsigned)nrows, 64, 0, s>>>(rowptr, colidx, rows, nrows, (double*)J, d, u, g);
+    else
+        dirichlet_rows<float><<<(unsigned)nrows, 64, 0, s>>>(rowptr, colidx, rows, nrows, (float*)J, d, u, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_spmv(int jprec, const int64_t* rowptr, const int32_t* colidx, const void* A, const double* x,
+                       double* y, int64_t n, hipStream_t s)
+{
+    const int bl = blocks_for(n * 16, 256);
+    if (jprec == 64) spmv_csr<double><<<bl, 256, 0, s>>>(rowptr, colidx, (const double*)A, x, y, n);
+    else spmv_csr<float><<<bl, 256, 0, s>>>(rowptr, colidx, (const float*)A, x, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_jacobi_setup(int jprec, int nc, const int64_t* rowptr, const int32_t* colidx, const void* A,
+                                     int64_t nv, double* Binv, int* err, hipStream_t s)
+{
+    const int bl = blocks_for(nv, 128);
+    if (nc == 3) {
+        if (jprec == 64) block_jacobi_setup<double, 3><<<bl, 128, 0, s>>>(rowptr, colidx, (const double*)A, nv, Binv, err);
+        else block_jacobi_setup<float, 3><<<bl, 128, 0, s>>>(rowptr, colidx, (const float*)A, nv, Binv, err);
+    } else {
+        if (jprec == 64) block_jacobi_setup<double, 4><<<bl, 128, 0, s>>>(rowptr, colidx, (const double*)A, nv, Binv, err);
+        else block_jacobi_setup<float, 4><<<bl, 128, 0, s>>>(rowptr, colidx, (const float*)A, nv, Binv, err);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_block_jacobi_apply(int nc, const double* Binv, const double* x, double* y, int64_t nv, hipStream_t s)
+{
+    const int bl = blocks_for(nv, 256);
+    if (nc == 3) block_jacobi_apply<3><<<bl, 256, 0, s>>>(Binv, x, y, nv);
+    else block_jacobi_apply<4><<<bl, 256, 0, s>>>(Binv, x, y, nv);
+    return hipGetLastError();
+}
+
+hipError_t launch_multi_dot(const double* V, int64_t ld, int k, const double* w, int64_t n, double* partial,
+                            double* out, hipStream_t s)
+{
+    multi_dot_partial<<<RED_BLOCKS, 256, 0, s>>>(V, ld, k, w, n, partial);
+    multi_dot_final<<<1, 64, 0, s>>>(partial, RED_BLOCKS, k, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_multi_axpy(const double* V, int64_t ld, int k, const double* h, double* w, int64_t n, hipStream_t s)
+{
+    multi_axpy<<<blocks_for(n, 256), 256, 0, s>>>(V, ld, k, h, w, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_multi_add(const double* V, int64_t ld, int k, const double* c, double* x, int64_t n, hipStream_t s)
+{
+    multi_add<<<blocks_for(n, 256), 256, 0, s>>>(V, ld, k, c, x, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t s)
+{
+    axpby<<<blocks_for(n, 256), 256, 0, s>>>(a, x, b, y, n);
+    return hipGetLastError();
+}
+
+}  // namespace nsfv1
