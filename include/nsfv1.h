@@ -181,12 +181,41 @@ int nsfv1_apply_dirichlet(nsfv1_handle* h, void* jvals, int jprec, double* defec
 int nsfv1_spmv(nsfv1_handle* h, const void* jvals, int jprec, const double* x, double* y, void* stream);
 
 /* Restarted GMRES(restart), right-preconditioned with point-block Jacobi
-   ((dim+1)x(dim+1) node blocks), on J x = b: device FP64 b and x (x is
-   overwritten; zero initial guess), J in jprec.  Synchronises the stream.
-   iters/relres (host, may be NULL) receive the inner iterations and
-   ||b - J x|| / ||b||. */
+   ((dim+1)x(dim+1) node blocks) or, when selected, one geometric multigrid
+   V-cycle, on J x = b: device FP64 b and x (x is overwritten; zero initial
+   guess), J in jprec.  Synchronises the stream.  iters/relres (host, may be
+   NULL) receive the inner iterations and ||b - J x|| / ||b||.
+   Replaces the linear solver ugcore's NewtonSolver calls on the assembled
+   operator (not part of the plugin; the plugin's own preconditioner
+   incompressible/cr_ilut.h is an ILUT for the Crouzeix-Raviart variant). */
 int nsfv1_linear_solve(nsfv1_handle* h, const void* jvals, int jprec, const double* b, double* x, double rtol,
                        int maxit, int restart, int* iters, double* relres, void* stream);
+
+/* Preconditioner of nsfv1_linear_solve: "block-jacobi" (default) or "gmg"
+   (geometric multigrid, needs nsfv1_mg_set_hierarchy and nsfv1_mg_setup).
+   No counterpart in the plugin: ugcore's GMG plays this role for it. */
+int nsfv1_set_preconditioner(nsfv1_handle* h, const char* name);
+
+/* Multigrid hierarchy: level 0 is h, levels 1..nlev-1 are the handles
+   coarse[0..nlev-2] with meshes (and Dirichlet rows) set; they take h's
+   configuration.  For level l -> l+1 (array index l): prol_rowptr/colidx/
+   weights is a node-level CSR with rows = nodes of level l and columns =
+   nodes of level l+1 (applied to every component; restriction is its
+   transpose), inject[l][w] = node of level l that coarse node w sits on.
+   The coarsest level must have at most 1200 DoFs (dense inverse).  pre/post
+   damped (omega) point-block Jacobi sweeps per level.  Coarse handles must
+   outlive h's use of them. */
+int nsfv1_mg_set_hierarchy(nsfv1_handle* h, int nlev, nsfv1_handle* const* coarse, const int64_t* const* prol_rowptr,
+                           const int32_t* const* prol_colidx, const double* const* prol_weights,
+                           const int64_t* const* inject, int pre, int post, double omega);
+
+/* Re-assembles every coarse level's Jacobian at the injected state (same
+   time-series, scalings and parts as the fine Jacobian), applies the coarse
+   Dirichlet rows, factors the point blocks and inverts the coarsest level on
+   the host.  Call after each fine assembly whose Jacobian the next
+   nsfv1_linear_solve uses.  Synchronises the stream. */
+int nsfv1_mg_setup(nsfv1_handle* h, int jprec, const double* u, const double* u_old, double dt, double s_a,
+                   double s_m, int parts, void* stream);
 
 /* Launch shape of the assembly: incidences (threads) per workgroup (64..256,
    a multiple of 64) and the LDS budget per workgroup in bytes (4..64 KiB);

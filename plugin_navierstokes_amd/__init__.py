@@ -41,6 +41,7 @@ EXPORTS = [
     "nsfv1_set_stabilization_upwind", "nsfv1_set_pac_upwind", "nsfv1_set_mesh", "nsfv1_num_dofs", "nsfv1_nnz",
     "nsfv1_get_pattern", "nsfv1_pattern_device", "nsfv1_assemble", "nsfv1_sync_errors", "nsfv1_set_launch",
     "nsfv1_set_dirichlet", "nsfv1_apply_dirichlet", "nsfv1_spmv", "nsfv1_linear_solve",
+    "nsfv1_set_preconditioner", "nsfv1_mg_set_hierarchy", "nsfv1_mg_setup",
 ]
 
 
@@ -92,6 +93,10 @@ def load_library(path: str | None = None):
     L.nsfv1_spmv.argtypes = [H, vp, C.c_int, vp, vp, vp]
     L.nsfv1_linear_solve.argtypes = [H, vp, C.c_int, vp, vp, dp, C.c_int, C.c_int, C.POINTER(C.c_int),
                                      C.POINTER(C.c_double), vp]
+    L.nsfv1_set_preconditioner.argtypes = [H, C.c_char_p]
+    L.nsfv1_mg_set_hierarchy.argtypes = [H, C.c_int, C.POINTER(H), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
+                                         C.POINTER(vp), C.c_int, C.c_int, dp]
+    L.nsfv1_mg_setup.argtypes = [H, C.c_int, vp, vp, dp, dp, dp, C.c_int, vp]
     _lib = L
     return L
 
@@ -263,6 +268,34 @@ class NavierStokesFV1:
         _check(_lib.nsfv1_linear_solve(self._h, _ptr(J), int(jprec), _ptr(b), _ptr(x), float(rtol), int(maxit),
                                        int(restart), C.byref(it), C.byref(rr), _stream_ptr(stream)))
         return it.value, rr.value
+
+    def set_preconditioner(self, name: str):
+        _check(_lib.nsfv1_set_preconditioner(self._h, name.encode()))
+
+    def mg_set_hierarchy(self, coarse, prolongations, injections, pre=2, post=2, omega=0.7):
+        """coarse: list of NavierStokesFV1 (meshes and Dirichlet rows set),
+        prolongations: list of (rowptr int64, colidx int32, weights float64)
+        node-level CSR per level pair, injections: int64 arrays."""
+        nl = len(coarse) + 1
+        Hs = (C.c_void_p * (nl - 1))(*[c._h.value for c in coarse])
+        keep = []
+        def arr(xs, dt):
+            out = []
+            for x in xs:
+                a = np.ascontiguousarray(x, dtype=dt)
+                keep.append(a)
+                out.append(a.ctypes.data)
+            return (C.c_void_p * (nl - 1))(*out)
+        rp = arr([p[0] for p in prolongations], np.int64)
+        ci = arr([p[1] for p in prolongations], np.int32)
+        w = arr([p[2] for p in prolongations], np.float64)
+        ij = arr(injections, np.int64)
+        _check(_lib.nsfv1_mg_set_hierarchy(self._h, nl, Hs, rp, ci, w, ij, int(pre), int(post), float(omega)))
+        self._mg_levels = list(coarse)  # the library borrows the coarse handles
+
+    def mg_setup(self, jprec, u, u_old=None, dt=0.0, s_a=1.0, s_m=0.0, parts=PART_A, stream=None):
+        _check(_lib.nsfv1_mg_setup(self._h, int(jprec), _ptr(u), _ptr(u_old), float(dt), float(s_a), float(s_m),
+                                   int(parts), _stream_ptr(stream)))
 
     def sync_errors(self, stream=None):
         _check(_lib.nsfv1_sync_errors(self._h, _stream_ptr(stream)))

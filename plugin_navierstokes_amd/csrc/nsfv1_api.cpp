@@ -69,6 +69,17 @@ struct Plan {
 
 }  // namespace
 
+// one coarse level of the geometric multigrid preconditioner
+struct MGLevel {
+    nsfv1_handle* h = nullptr;      // coarse discretisation (owned by the caller)
+    int64_t nv_fine = 0;             // nodes of the next finer level
+    DevBuf P_rp, P_ci, P_w;          // prolongation: rows = finer nodes, cols = this level's nodes
+    DevBuf R_rp, R_ci, R_w;          // restriction = transpose of P
+    DevBuf inj;                      // this level's node -> finer node (state injection)
+    DevBuf J, Binv, u, uold, b, x, r;
+    int jprec = 0;
+};
+
 // enum values of the configuration state
 enum { UPW_UNSET = -1 };
 enum { STAB_UNSET = -1 };
@@ -104,6 +115,14 @@ struct nsfv1_handle {
     DevBuf dir_rows, gm_V, gm_Binv, gm_part, gm_small, gm_w;
     int64_t ndir = 0;
     int gm_restart = 0;
+    // geometric multigrid preconditioner
+    std::vector<std::unique_ptr<MGLevel>> mg;
+    int mg_pre = 2, mg_post = 2;
+    double mg_omega = 0.7;
+    bool mg_ready = false, use_mg = false;
+    int mg_jprec = 0;
+    int mg_ncoarse = 0;
+    DevBuf mg_r0, mg_dense;
     bool geo_valid = false;
     int geo_difflen = -1;
 };
@@ -427,6 +446,11 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
     h->h_node_inc.swap(cnt);
     h->plan32.valid = h->plan64.valid = false;
     h->geo_valid = false;
+    // solver state sized for the previous mesh
+    h->gm_restart = 0;
+    h->ndir = 0;
+    h->mg.clear();
+    h->mg_ready = false;
     return NSFV1_OK;
 }
 
@@ -648,6 +672,249 @@ int nsfv1_spmv(nsfv1_handle* h, const void* jvals, int jprec, const double* x, d
     return NSFV1_OK;
 }
 
+
+// ---------------------------------------------------------------------
+// geometric multigrid preconditioner (V-cycle, damped point-block Jacobi
+// smoothing, Galerkin-free: every coarse level re-assembles the same
+// discretisation on its own mesh at the injected state)
+// ---------------------------------------------------------------------
+int nsfv1_set_preconditioner(nsfv1_handle* h, const char* name)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    const std::string n = trim_lower(name);
+    if (n == "block-jacobi" || n == "jacobi") h->use_mg = false;
+    else if (n == "gmg" || n == "multigrid") h->use_mg = true;
+    else return fail(NSFV1_ERR_UNKNOWN_NAME, "preconditioner '" + std::string(name ? name : "") + "' not found. Options are: block-jacobi, gmg");
+    return NSFV1_OK;
+}
+
+int nsfv1_mg_set_hierarchy(nsfv1_handle* h, int nlev, nsfv1_handle* const* coarse, const int64_t* const* prol_rowptr,
+                           const int32_t* const* prol_colidx, const double* const* prol_weights,
+                           const int64_t* const* inject, int pre, int post, double omega)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    if (nlev < 2 || nlev > MG_MAX_LEVELS) return fail(NSFV1_ERR_ARGUMENT, "multigrid needs 2.." + std::to_string(MG_MAX_LEVELS) + " levels");
+    if (!coarse || !prol_rowptr || !prol_colidx || !prol_weights || !inject) return fail(NSFV1_ERR_ARGUMENT, "NULL level arrays");
+    if (pre < 1 || post < 0 || pre > 16 || post > 16 || !(omega > 0.0 && omega <= 1.5))
+        return fail(NSFV1_ERR_ARGUMENT, "smoothing steps must be 1..16 (pre) / 0..16 (post), omega in (0, 1.5]");
+    std::vector<std::unique_ptr<MGLevel>> lv;
+    int64_t nv_f = h->nv;
+    for (int l = 1; l < nlev; ++l) {
+        nsfv1_handle* c = coarse[l - 1];
+        if (!c || c == h) return fail(NSFV1_ERR_ARGUMENT, "bad coarse handle");
+        if (c->et < 0) return fail(NSFV1_ERR_NO_MESH, "coarse level without mesh");
+        if (c->dim != h->dim) return fail(NSFV1_ERR_ARGUMENT, "coarse level of another dimension");
+        const int64_t nv_c = c->nv;
+        const int64_t* rp = prol_rowptr[l - 1];
+        const int32_t* ci = prol_colidx[l - 1];
+        const double* w = prol_weights[l - 1];
+        const int64_t* ij = inject[l - 1];
+        if (!rp || !ci || !w || !ij) return fail(NSFV1_ERR_ARGUMENT, "NULL transfer operator");
+        if (rp[0] != 0) return fail(NSFV1_ERR_ARGUMENT, "prolongation rowptr must start at 0");
+        for (int64_t r = 0; r < nv_f; ++r)
+            if (rp[r + 1] < rp[r]) return fail(NSFV1_ERR_ARGUMENT, "prolongation rowptr not monotone");
+        const int64_t pnnz = rp[nv_f];
+        for (int64_t k = 0; k < pnnz; ++k)
+            if (ci[k] < 0 || ci[k] >= nv_c) return fail(NSFV1_ERR_ARGUMENT, "prolongation column out of range");
+        for (int64_t k = 0; k < nv_c; ++k)
+            if (ij[k] < 0 || ij[k] >= nv_f) return fail(NSFV1_ERR_ARGUMENT, "injection index out of range");
+        // restriction = P^T (host transpose, sorted by fine node)
+        std::vector<int64_t> rrp(nv_c + 1, 0);
+        for (int64_t k = 0; k < pnnz; ++k) ++rrp[ci[k] + 1];
+        for (int64_t r = 0; r < nv_c; ++r) rrp[r + 1] += rrp[r];
+        std::vector<int32_t> rci(pnnz);
+        std::vector<double> rw(pnnz);
+        std::vector<int64_t> fill(rrp.begin(), rrp.end() - 1);
+        for (int64_t r = 0; r < nv_f; ++r)
+            for (int64_t k = rp[r]; k < rp[r + 1]; ++k) {
+                const int64_t q = fill[ci[k]]++;
+                rci[q] = (int32_t)r;
+                rw[q] = w[k];
+            }
+        auto L = std::make_unique<MGLevel>();
+        L->h = c;
+        L->nv_fine = nv_f;
+        auto up = [](DevBuf& b, const void* src, size_t bytes) {
+            return b.alloc(bytes) == hipSuccess && (bytes == 0 || hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice) == hipSuccess);
+        };
+        if (!up(L->P_rp, rp, sizeof(int64_t) * (nv_f + 1)) || !up(L->P_ci, ci, sizeof(int32_t) * pnnz) ||
+            !up(L->P_w, w, sizeof(double) * pnnz) || !up(L->R_rp, rrp.data(), sizeof(int64_t) * (nv_c + 1)) ||
+            !up(L->R_ci, rci.data(), sizeof(int32_t) * pnnz) || !up(L->R_w, rw.data(), sizeof(double) * pnnz) ||
+            !up(L->inj, ij, sizeof(int64_t) * nv_c))
+            return fail(NSFV1_ERR_DEVICE, "multigrid transfer upload");
+        const int64_t n = c->ndof;
+        if (L->Binv.alloc(sizeof(double) * c->nv * (c->dim + 1) * (c->dim + 1)) != hipSuccess ||
+            L->u.alloc(sizeof(double) * n) != hipSuccess || L->uold.alloc(sizeof(double) * n) != hipSuccess ||
+            L->b.alloc(sizeof(double) * n) != hipSuccess || L->x.alloc(sizeof(double) * n) != hipSuccess ||
+            L->r.alloc(sizeof(double) * n) != hipSuccess)
+            return fail(NSFV1_ERR_DEVICE, "multigrid workspace");
+        lv.push_back(std::move(L));
+        nv_f = nv_c;
+    }
+    const int64_t ncoarse = lv.back()->h->ndof;
+    if (ncoarse > MG_DENSE_MAX)
+        return fail(NSFV1_ERR_ARGUMENT, "coarsest level has " + std::to_string(ncoarse) + " DoFs; at most " +
+                                            std::to_string(MG_DENSE_MAX) + " are solved densely");
+    if (h->mg_r0.alloc(sizeof(double) * h->ndof) != hipSuccess ||
+        h->mg_dense.alloc(sizeof(double) * ncoarse * ncoarse) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "multigrid workspace");
+    // coarse levels carry the configuration of the fine discretisation
+    for (auto& L : lv) {
+        nsfv1_handle* c = L->h;
+        c->nu = h->nu;
+        c->nu_set = h->nu_set;
+        c->rho = h->rho;
+        c->has_source = h->has_source;
+        for (int i = 0; i < 3; ++i) c->f[i] = h->f[i];
+        c->exact = h->exact;
+        c->stokes = h->stokes;
+        c->laplace = h->laplace;
+        c->peclet = h->peclet;
+        c->grad_div = h->grad_div;
+        c->stab_type = h->stab_type;
+        c->stab_upwind = h->stab_upwind;
+        c->difflen = h->difflen;
+        c->conv_upwind = h->conv_upwind;
+        c->pac = h->pac;
+    }
+    h->mg.swap(lv);
+    h->mg_pre = pre;
+    h->mg_post = post;
+    h->mg_omega = omega;
+    h->mg_ncoarse = (int)ncoarse;
+    h->mg_ready = false;
+    return NSFV1_OK;
+}
+
+// dense Gauss-Jordan inverse with partial pivoting (coarsest level only)
+static bool dense_inverse(std::vector<double>& A, int n, std::vector<double>& inv)
+{
+    inv.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int p = c;
+        double best = std::fabs(A[(size_t)c * n + c]);
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(A[(size_t)r * n + c]) > best) { best = std::fabs(A[(size_t)r * n + c]); p = r; }
+        if (best == 0.0) return false;
+        if (p != c)
+            for (int k = 0; k < n; ++k) {
+                std::swap(A[(size_t)c * n + k], A[(size_t)p * n + k]);
+                std::swap(inv[(size_t)c * n + k], inv[(size_t)p * n + k]);
+            }
+        const double ip = 1.0 / A[(size_t)c * n + c];
+        for (int k = 0; k < n; ++k) {
+            A[(size_t)c * n + k] *= ip;
+            inv[(size_t)c * n + k] *= ip;
+        }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            const double f = A[(size_t)r * n + c];
+            if (f == 0.0) continue;
+            for (int k = 0; k < n; ++k) {
+                A[(size_t)r * n + k] -= f * A[(size_t)c * n + k];
+                inv[(size_t)r * n + k] -= f * inv[(size_t)c * n + k];
+            }
+        }
+    }
+    return true;
+}
+
+int nsfv1_mg_setup(nsfv1_handle* h, int jprec, const double* u, const double* u_old, double dt, double s_a,
+                   double s_m, int parts, void* stream)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (h->mg.empty()) return fail(NSFV1_ERR_ARGUMENT, "no multigrid hierarchy set");
+    if (!u) return fail(NSFV1_ERR_ARGUMENT, "u is NULL");
+    if (jprec != NSFV1_F32 && jprec != NSFV1_F64) return fail(NSFV1_ERR_ARGUMENT, "jprec must be 32 or 64");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t js = jprec == NSFV1_F64 ? 8 : 4;
+    const double* uf = u;
+    const double* uof = u_old;
+    for (auto& Lp : h->mg) {
+        MGLevel& L = *Lp;
+        nsfv1_handle* c = L.h;
+        const int nc = c->dim + 1;
+        if (L.jprec != jprec || L.J.p == nullptr) {
+            if (L.J.alloc(js * c->nnz) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "multigrid Jacobian");
+            L.jprec = jprec;
+        }
+        hipError_t e = launch_inject(L.inj.as<int64_t>(), uf, L.u.as<double>(), c->nv, nc, s);
+        if (e == hipSuccess && uof) e = launch_inject(L.inj.as<int64_t>(), uof, L.uold.as<double>(), c->nv, nc, s);
+        if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, "multigrid injection");
+        int rc = nsfv1_assemble(c, L.u.as<double>(), uof ? L.uold.as<double>() : nullptr, dt, s_a, s_m, 0.0, parts,
+                                L.J.p, jprec, nullptr, stream);
+        if (rc) return rc;
+        e = launch_dirichlet(jprec, c->rowptr.as<int64_t>(), c->colidx.as<int32_t>(), c->dir_rows.as<int64_t>(), c->ndir,
+                             L.J.p, nullptr, nullptr, nullptr, s);
+        if (e == hipSuccess)
+            e = launch_block_jacobi_setup(jprec, nc, c->rowptr.as<int64_t>(), c->colidx.as<int32_t>(), L.J.p, c->nv,
+                                          L.Binv.as<double>(), h->errflag.as<int>(), s);
+        if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, "multigrid level setup");
+        uf = L.u.as<double>();
+        uof = uof ? L.uold.as<double>() : nullptr;
+    }
+    // coarsest level: dense inverse on the host
+    MGLevel& C = *h->mg.back();
+    nsfv1_handle* c = C.h;
+    const int n = (int)c->ndof;
+    std::vector<char> jh(js * c->nnz);
+    if (hipMemcpyAsync(jh.data(), C.J.p, jh.size(), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "coarse Jacobian download");
+    std::vector<int32_t> cci(c->nnz);
+    if (hipMemcpy(cci.data(), c->colidx.p, sizeof(int32_t) * c->nnz, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "coarse pattern download");
+    std::vector<double> A((size_t)n * n, 0.0), inv;
+    for (int r = 0; r < n; ++r)
+        for (int64_t k = c->h_rowptr[r]; k < c->h_rowptr[r + 1]; ++k)
+            A[(size_t)r * n + cci[k]] = jprec == NSFV1_F64 ? reinterpret_cast<const double*>(jh.data())[k]
+                                                           : (double)reinterpret_cast<const float*>(jh.data())[k];
+    if (!dense_inverse(A, n, inv)) return fail(NSFV1_ERR_ARGUMENT, "coarsest multigrid level is singular");
+    if (hipMemcpyAsync(h->mg_dense.p, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "coarse inverse upload");
+    h->mg_jprec = jprec;
+    h->mg_ready = true;
+    return NSFV1_OK;
+}
+
+// x = V-cycle(b) on level l (l = 0: the fine system J0 with block inverse Binv0)
+static hipError_t mg_cycle(nsfv1_handle* h, size_t l, const void* J0, const double* Binv0, const double* b, double* x,
+                           hipStream_t s)
+{
+    const bool fine = l == 0;
+    nsfv1_handle* c = fine ? h : h->mg[l - 1]->h;
+    const int nc = c->dim + 1;
+    const int jp = h->mg_jprec;
+    if (l == h->mg.size()) return launch_dense_matvec(h->mg_dense.as<double>(), b, x, (int)c->ndof, s);
+    const void* J = fine ? J0 : h->mg[l - 1]->J.p;
+    const double* Bi = fine ? Binv0 : h->mg[l - 1]->Binv.as<double>();
+    double* r = fine ? h->mg_r0.as<double>() : h->mg[l - 1]->r.as<double>();
+    const int64_t* rp = c->rowptr.as<int64_t>();
+    const int32_t* ci = c->colidx.as<int32_t>();
+    const int64_t n = c->ndof, nv = c->nv;
+    const double w = h->mg_omega;
+    hipError_t e = launch_bj_update(nc, Bi, b, x, w, 1, nv, s);
+    for (int k = 1; k < h->mg_pre && e == hipSuccess; ++k) {
+        e = launch_residual(jp, rp, ci, J, x, b, r, n, s);
+        if (e == hipSuccess) e = launch_bj_update(nc, Bi, r, x, w, 0, nv, s);
+    }
+    if (e != hipSuccess) return e;
+    MGLevel& C = *h->mg[l];
+    nsfv1_handle* cc = C.h;
+    e = launch_residual(jp, rp, ci, J, x, b, r, n, s);
+    if (e == hipSuccess) e = launch_node_transfer(C.R_rp.as<int64_t>(), C.R_ci.as<int32_t>(), C.R_w.as<double>(), r, C.b.as<double>(), cc->nv, nc, 0, s);
+    if (e == hipSuccess) e = launch_zero_rows(cc->dir_rows.as<int64_t>(), cc->ndir, C.b.as<double>(), s);
+    if (e == hipSuccess) e = mg_cycle(h, l + 1, J0, Binv0, C.b.as<double>(), C.x.as<double>(), s);
+    if (e == hipSuccess) e = launch_node_transfer(C.P_rp.as<int64_t>(), C.P_ci.as<int32_t>(), C.P_w.as<double>(), C.x.as<double>(), x, nv, nc, 1, s);
+    for (int k = 0; k < h->mg_post && e == hipSuccess; ++k) {
+        e = launch_residual(jp, rp, ci, J, x, b, r, n, s);
+        if (e == hipSuccess) e = launch_bj_update(nc, Bi, r, x, w, 0, nv, s);
+    }
+    return e;
+}
+
 // Restarted GMRES(m), right-preconditioned with point-block Jacobi, for
 // J x = b with the device Jacobian (jprec) and FP64 vectors.  x is
 // overwritten (zero initial guess).  Stops when ||b - J x|| <= rtol ||b||
@@ -683,8 +950,14 @@ int nsfv1_linear_solve(nsfv1_handle* h, const void* jvals, int jprec, const doub
         return hipMemcpyAsync(out, dsm, sizeof(double) * k, hipMemcpyDeviceToHost, s) == hipSuccess &&
                hipStreamSynchronize(s) == hipSuccess;
     };
+    if (h->use_mg && (!h->mg_ready || h->mg_jprec != jprec))
+        return fail(NSFV1_ERR_ARGUMENT, "multigrid preconditioner selected but not set up for this Jacobian precision (call nsfv1_mg_setup)");
     if (!chk(launch_block_jacobi_setup(jprec, nc, rp, ci, jvals, nv, Binv, h->errflag.as<int>(), s)))
         return fail(NSFV1_ERR_DEVICE, "block Jacobi setup");
+    auto precond = [&](const double* in, double* out) -> hipError_t {
+        if (h->use_mg) return mg_cycle(h, 0, jvals, Binv, in, out, s);
+        return launch_block_jacobi_apply(nc, Binv, in, out, nv, s);
+    };
     if (hipMemsetAsync(x, 0, sizeof(double) * n, s) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "memset");
     double bn2;
     if (!dot(b, 1, b, &bn2)) return fail(NSFV1_ERR_DEVICE, "dot");
@@ -711,8 +984,7 @@ int nsfv1_linear_solve(nsfv1_handle* h, const void* jvals, int jprec, const doub
         g[0] = beta;
         int j = 0;
         for (; j < m && it < maxit; ++j, ++it) {
-            if (!chk(launch_block_jacobi_apply(nc, Binv, V + (int64_t)j * n, z, nv, s)) ||
-                !chk(launch_spmv(jprec, rp, ci, jvals, z, w, n, s)))
+            if (!chk(precond(V + (int64_t)j * n, z)) || !chk(launch_spmv(jprec, rp, ci, jvals, z, w, n, s)))
                 return fail(NSFV1_ERR_DEVICE, "arnoldi");
             // classical Gram-Schmidt, twice
             double hcol[GMRES_MAX_RESTART + 1], h2[GMRES_MAX_RESTART + 1];
@@ -755,7 +1027,7 @@ int nsfv1_linear_solve(nsfv1_handle* h, const void* jvals, int jprec, const doub
         }
         if (hipMemcpyAsync(dsm + 128, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, s) != hipSuccess ||
             hipMemsetAsync(w, 0, sizeof(double) * n, s) != hipSuccess || !chk(launch_multi_add(V, n, j, dsm + 128, w, n, s)) ||
-            !chk(launch_block_jacobi_apply(nc, Binv, w, z, nv, s)) || !chk(launch_axpby(1.0, z, 1.0, x, n, s)))
+            !chk(precond(w, z)) || !chk(launch_axpby(1.0, z, 1.0, x, n, s)))
             return fail(NSFV1_ERR_DEVICE, "update");
         if (res <= rtol * bnorm) break;
     }

@@ -205,6 +205,95 @@ __global__ void multi_add(const double* __restrict__ V, int64_t ld, int k, const
     x[t] = s;
 }
 
+
+// ---------------------------------------------------------------------
+// geometric multigrid pieces: residual, damped block-Jacobi update,
+// node-level transfer operators, Dirichlet masking, state injection and
+// the dense coarsest-level solve
+// ---------------------------------------------------------------------
+
+// r = b - A x, 16 lanes per row
+template <typename JT>
+__global__ void __launch_bounds__(256)
+residual_csr(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colidx, const JT* __restrict__ A,
+             const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ r, int64_t n)
+{
+    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int sub = threadIdx.x & 15;
+    double s = 0.0;
+    if (row < n) {
+        const int64_t a = rowptr[row], e = rowptr[row + 1];
+        for (int64_t k = a + sub; k < e; k += 16) s += (double)A[k] * x[colidx[k]];
+    }
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
+    if (row < n && sub == 0) r[row] = b[row] - s;
+}
+
+// x = (first ? 0 : x) + omega * Binv r, per node block
+template <int NC>
+__global__ void bj_update(const double* __restrict__ Binv, const double* __restrict__ r, double* __restrict__ x,
+                          double omega, int first, int64_t nv)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    double rv[NC];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) rv[a] = r[v * NC + a];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) {
+        double s = 0;
+#pragma unroll
+        for (int b = 0; b < NC; ++b) s += Binv[(v * NC + a) * NC + b] * rv[b];
+        x[v * NC + a] = (first ? 0.0 : x[v * NC + a]) + omega * s;
+    }
+}
+
+// y[row, c] (= or +=) sum_k w_k x[col_k, c] for every component c < nc
+__global__ void node_transfer(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+                              const double* __restrict__ w, const double* __restrict__ x, double* __restrict__ y,
+                              int64_t nrows, int nc, int accumulate)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrows * nc) return;
+    const int64_t row = t / nc;
+    const int c = (int)(t - row * nc);
+    double s = 0.0;
+    for (int64_t k = rowptr[row]; k < rowptr[row + 1]; ++k) s += w[k] * x[(int64_t)colidx[k] * nc + c];
+    y[t] = accumulate ? y[t] + s : s;
+}
+
+__global__ void zero_rows(const int64_t* __restrict__ rows, int64_t nrows, double* __restrict__ y)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nrows) y[rows[t]] = 0.0;
+}
+
+// coarse state: uc[w, c] = uf[inj[w], c]
+__global__ void inject_state(const int64_t* __restrict__ inj, const double* __restrict__ uf, double* __restrict__ uc,
+                             int64_t nvc, int nc)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nvc * nc) return;
+    const int64_t w = t / nc;
+    const int c = (int)(t - w * nc);
+    uc[t] = uf[inj[w] * nc + c];
+}
+
+// y = A x with a dense row-major n x n matrix, one wavefront per row
+__global__ void __launch_bounds__(256)
+dense_matvec(const double* __restrict__ A, const double* __restrict__ x, double* __restrict__ y, int n)
+{
+    const int row = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    double s = 0.0;
+    if (row < n)
+        for (int k = lane; k < n; k += 64) s += A[(int64_t)row * n + k] * x[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (row < n && lane == 0) y[row] = s;
+}
+
 // ---------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------
@@ -275,6 +364,50 @@ hipError_t launch_multi_add(const double* V, int64_t ld, int k, const double* c,
 hipError_t launch_axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t s)
 {
     axpby<<<blocks_for(n, 256), 256, 0, s>>>(a, x, b, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(int jprec, const int64_t* rowptr, const int32_t* colidx, const void* A, const double* x,
+                           const double* b, double* r, int64_t n, hipStream_t s)
+{
+    const int bl = blocks_for(n * 16, 256);
+    if (jprec == 64) residual_csr<double><<<bl, 256, 0, s>>>(rowptr, colidx, (const double*)A, x, b, r, n);
+    else residual_csr<float><<<bl, 256, 0, s>>>(rowptr, colidx, (const float*)A, x, b, r, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_bj_update(int nc, const double* Binv, const double* r, double* x, double omega, int first,
+                            int64_t nv, hipStream_t s)
+{
+    const int bl = blocks_for(nv, 256);
+    if (nc == 3) bj_update<3><<<bl, 256, 0, s>>>(Binv, r, x, omega, first, nv);
+    else bj_update<4><<<bl, 256, 0, s>>>(Binv, r, x, omega, first, nv);
+    return hipGetLastError();
+}
+
+hipError_t launch_node_transfer(const int64_t* rowptr, const int32_t* colidx, const double* w, const double* x,
+                                double* y, int64_t nrows, int nc, int accumulate, hipStream_t s)
+{
+    node_transfer<<<blocks_for(nrows * nc, 256), 256, 0, s>>>(rowptr, colidx, w, x, y, nrows, nc, accumulate);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_rows(const int64_t* rows, int64_t nrows, double* y, hipStream_t s)
+{
+    if (nrows <= 0) return hipSuccess;
+    zero_rows<<<blocks_for(nrows, 256), 256, 0, s>>>(rows, nrows, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_inject(const int64_t* inj, const double* uf, double* uc, int64_t nvc, int nc, hipStream_t s)
+{
+    inject_state<<<blocks_for(nvc * nc, 256), 256, 0, s>>>(inj, uf, uc, nvc, nc);
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_matvec(const double* A, const double* x, double* y, int n, hipStream_t s)
+{
+    dense_matvec<<<blocks_for((int64_t)n * 64, 256), 256, 0, s>>>(A, x, y, n);
     return hipGetLastError();
 }
 

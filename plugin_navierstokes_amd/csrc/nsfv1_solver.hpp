@@ -9,6 +9,8 @@ namespace nsfv1 {
 
 constexpr int GMRES_MAX_RESTART = 60;
 constexpr int RED_PARTIALS = 1024 * 64;
+constexpr int MG_MAX_LEVELS = 12;
+constexpr int MG_DENSE_MAX = 1200;  // largest coarsest-level system inverted densely
 
 hipError_t launch_dirichlet(int jprec, const int64_t* rowptr, const int32_t* colidx, const int64_t* rows,
                             int64_t nrows, void* J, double* d, const double* u, const double* g, hipStream_t s);
@@ -22,5 +24,14 @@ hipError_t launch_multi_dot(const double* V, int64_t ld, int k, const double* w,
 hipError_t launch_multi_axpy(const double* V, int64_t ld, int k, const double* h, double* w, int64_t n, hipStream_t s);
 hipError_t launch_multi_add(const double* V, int64_t ld, int k, const double* c, double* x, int64_t n, hipStream_t s);
 hipError_t launch_axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t s);
+hipError_t launch_residual(int jprec, const int64_t* rowptr, const int32_t* colidx, const void* A, const double* x,
+                           const double* b, double* r, int64_t n, hipStream_t s);
+hipError_t launch_bj_update(int nc, const double* Binv, const double* r, double* x, double omega, int first,
+                            int64_t nv, hipStream_t s);
+hipError_t launch_node_transfer(const int64_t* rowptr, const int32_t* colidx, const double* w, const double* x,
+                                double* y, int64_t nrows, int nc, int accumulate, hipStream_t s);
+hipError_t launch_zero_rows(const int64_t* rows, int64_t nrows, double* y, hipStream_t s);
+hipError_t launch_inject(const int64_t* inj, const double* uf, double* uc, int64_t nvc, int nc, hipStream_t s);
+hipError_t launch_dense_matvec(const double* A, const double* x, double* y, int n, hipStream_t s);
 
 }  // namespace nsfv1

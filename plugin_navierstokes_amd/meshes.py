@@ -24,7 +24,8 @@ class Mesh:
     xyz: np.ndarray         # [nv, dim] float64
     conn: np.ndarray        # [ne, nsh] int32
     boundary: dict          # name -> node index array
-    shape: tuple = ()       # structured sizes, informational
+    shape: tuple = ()       # structured cell counts (nx, ny[, nz])
+    recipe: dict | None = None  # how to rebuild the mesh at another resolution
 
     @property
     def dim(self) -> int:
@@ -73,7 +74,8 @@ def cavity_2d(n: int, et: int = QUAD, jitter: float = 0.0, seed: int = 0, lx: fl
     xyz = _jitter(xyz, interior, h, jitter, seed)
     lid = np.nonzero(jj == n)[0]
     wall = np.nonzero(((ii == 0) | (ii == n) | (jj == 0)) & (jj < n))[0]
-    return Mesh(et, xyz, conn.astype(np.int32), {"lid": lid, "wall": wall}, (n, n))
+    return Mesh(et, xyz, conn.astype(np.int32), {"lid": lid, "wall": wall}, (n, n),
+                {"fn": "cavity_2d", "et": et, "lx": lx})
 
 
 def cavity_3d(n: int, et: int = HEX, jitter: float = 0.0, seed: int = 0) -> Mesh:
@@ -89,7 +91,11 @@ _KUHN = [
 
 
 def box_3d(cells, lengths, et: int = HEX, jitter: float = 0.0, seed: int = 0, kind: str = "cavity",
-           keep_cell=None) -> Mesh:
+           keep_cell=None, solid=None) -> Mesh:
+    """Box [0,lx]x[0,ly]x[0,lz] of nx*ny*nz hexahedra (or 6 Kuhn tetrahedra
+    each).  keep_cell(ci, cj, ck) or solid(x, y, z) (evaluated at cell
+    centres; True = inside an obstacle) drop cells; their nodes stay in the
+    numbering and are listed in boundary["unused"]."""
     nx, ny, nz = cells
     hx, hy, hz = lengths[0] / nx, lengths[1] / ny, lengths[2] / nz
     k, j, i = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
@@ -100,6 +106,9 @@ def box_3d(cells, lengths, et: int = HEX, jitter: float = 0.0, seed: int = 0, ki
     ci, cj, ck = ci.ravel(), cj.ravel(), ck.ravel()
     if keep_cell is not None:
         m = keep_cell(ci, cj, ck)
+        ci, cj, ck = ci[m], cj[m], ck[m]
+    if solid is not None:
+        m = ~solid((ci + 0.5) * hx, (cj + 0.5) * hy, (ck + 0.5) * hz)
         ci, cj, ck = ci[m], cj[m], ck[m]
     hexc = np.stack([v(ci, cj, ck), v(ci + 1, cj, ck), v(ci + 1, cj + 1, ck), v(ci, cj + 1, ck),
                      v(ci, cj, ck + 1), v(ci + 1, cj, ck + 1), v(ci + 1, cj + 1, ck + 1), v(ci, cj + 1, ck + 1)],
@@ -124,7 +133,73 @@ def box_3d(cells, lengths, et: int = HEX, jitter: float = 0.0, seed: int = 0, ki
     used = np.zeros(xyz.shape[0], dtype=bool)
     used[conn.ravel()] = True
     bnd["unused"] = np.nonzero(~used)[0]
-    return Mesh(et, xyz, conn.astype(np.int32), bnd, (nx, ny, nz))
+    recipe = None if keep_cell is not None else {"fn": "box_3d", "et": et, "lengths": tuple(lengths), "kind": kind,
+                                                 "solid": solid}
+    return Mesh(et, xyz, conn.astype(np.int32), bnd, (nx, ny, nz), recipe)
+
+
+def _grid_index(shape):
+    """Structured node coordinates (i, j[, k]) of a lexicographic numbering."""
+    dims = [s + 1 for s in shape]
+    idx = np.indices(dims[::-1]).reshape(len(dims), -1)[::-1]
+    return idx  # idx[0] = i (fastest)
+
+
+def coarsen(mesh: Mesh):
+    """Next coarser level of a structured mesh (every cell count halved).
+
+    Returns (coarse, prolongation, injection): the coarse mesh (same element
+    type and boundary sets; node positions taken from the fine mesh at the
+    coinciding nodes, so jitter carries over), the node-level prolongation as
+    CSR (rowptr, colidx, weights) with rows = fine nodes and columns = coarse
+    nodes (multilinear interpolation in grid index space), and the injection
+    coarse node -> fine node."""
+    if mesh.recipe is None or any(c % 2 for c in mesh.shape):
+        raise ValueError("mesh cannot be coarsened (needs a recipe and even cell counts)")
+    r = mesh.recipe
+    cshape = tuple(c // 2 for c in mesh.shape)
+    if r["fn"] == "cavity_2d":
+        coarse = cavity_2d(cshape[0], r["et"], lx=r["lx"])
+    else:
+        coarse = box_3d(cshape, r["lengths"], et=r["et"], kind=r["kind"], solid=r["solid"])
+    fdims = [c + 1 for c in mesh.shape]
+    cdims = [c + 1 for c in cshape]
+    fidx = _grid_index(mesh.shape)
+    cidx = _grid_index(cshape)
+    # injection: coarse (I, J, K) sits on fine (2I, 2J, 2K)
+    inj = np.zeros(coarse.nv, dtype=np.int64)
+    stride = 1
+    for d in range(len(fdims)):
+        inj += 2 * cidx[d] * stride
+        stride *= fdims[d]
+    coarse.xyz = mesh.xyz[inj].copy()
+    # prolongation: tensor product of 1D weights
+    nf = mesh.nv
+    rows = [np.arange(nf)]
+    cols = [np.zeros(nf, dtype=np.int64)]
+    wts = [np.ones(nf)]
+    cstride = 1
+    for d in range(len(fdims)):
+        i = fidx[d]
+        lo = i // 2
+        odd = (i % 2) == 1
+        nr, nc_, nw = [], [], []
+        for rr, cc, ww in zip(rows, cols, wts):
+            # even: one parent weight 1; odd: parents lo and lo+1 weight 1/2
+            nr.append(rr); nc_.append(cc + lo[rr] * cstride); nw.append(ww * np.where(odd[rr], 0.5, 1.0))
+            m = odd[rr]
+            nr.append(rr[m]); nc_.append(cc[m] + (lo[rr[m]] + 1) * cstride); nw.append(ww[m] * 0.5)
+        rows = [np.concatenate(nr)]
+        cols = [np.concatenate(nc_)]
+        wts = [np.concatenate(nw)]
+        cstride *= cdims[d]
+    rows, cols, wts = rows[0], cols[0], wts[0]
+    order = np.lexsort((cols, rows))
+    rows, cols, wts = rows[order], cols[order], wts[order]
+    rowptr = np.zeros(nf + 1, dtype=np.int64)
+    np.add.at(rowptr, rows + 1, 1)
+    rowptr = np.cumsum(rowptr)
+    return coarse, (rowptr, cols.astype(np.int32), wts), inj
 
 
 def random_state(mesh: Mesh, seed: int = 1, scale: float = 1.0) -> np.ndarray:

@@ -37,13 +37,50 @@ class NewtonStats:
     seconds: float = 0.0
 
 
+def build_multigrid(disc, mesh, dirichlet, max_coarse_dofs=1200, pre=2, post=2, omega=0.7):
+    """Geometric multigrid hierarchy for a structured mesh: coarsen until the
+    coarsest level has at most max_coarse_dofs DoFs (or the cell counts stop
+    being even), create a discretisation per coarse level (configuration is
+    taken from disc), set its Dirichlet rows from dirichlet(mesh) -> (dofs,
+    vals), and register the hierarchy with disc.  Returns the coarse discs."""
+    from . import NavierStokesFV1
+    from .meshes import coarsen
+
+    coarse, prol, inj = [], [], []
+    m = mesh
+    while m.nv * (m.dim + 1) > max_coarse_dofs:
+        if m.recipe is None or any(c % 2 for c in m.shape) or min(m.shape) < 2:
+            break
+        c, P, I = coarsen(m)
+        d = NavierStokesFV1(",".join(["u", "v", "w"][: m.dim] + ["p"]), "Inner", dim=m.dim)
+        d.set_mesh(c)
+        dofs, _ = dirichlet(c)
+        d.set_dirichlet(np.sort(np.asarray(dofs, dtype=np.int64)))
+        coarse.append(d)
+        prol.append(P)
+        inj.append(I)
+        m = c
+    if not coarse:
+        raise ValueError("mesh too small or not coarsenable for multigrid")
+    if m.nv * (m.dim + 1) > max_coarse_dofs:
+        raise ValueError(f"coarsest level has {m.nv * (m.dim + 1)} DoFs > {max_coarse_dofs}")
+    disc.mg_set_hierarchy(coarse, prol, inj, pre=pre, post=post, omega=omega)
+    disc.set_preconditioner("gmg")
+    return coarse
+
+
 class NewtonSolver:
-    """Fixpoint/Newton iteration for d(u) = 0 with device buffers."""
+    """Fixpoint/Newton iteration for d(u) = 0 with device buffers.
+
+    With multigrid=True the correction equation is preconditioned by a
+    geometric multigrid V-cycle (the disc needs a hierarchy from
+    build_multigrid); otherwise by point-block Jacobi."""
 
     def __init__(self, disc, dirichlet_dofs, dirichlet_vals, jprec=F32, rtol=1e-8, atol=1e-14, maxit=60,
-                 lin_rtol=1e-3, lin_maxit=400, restart=30):
+                 lin_rtol=1e-3, lin_maxit=400, restart=30, multigrid=False):
         import torch
 
+        self.multigrid = multigrid
         self.disc = disc
         self.jprec = jprec
         self.rtol, self.atol, self.maxit = rtol, atol, maxit
@@ -72,6 +109,8 @@ class NewtonSolver:
         if const is not None:
             self.d += const
         self.disc.apply_dirichlet(self.J if want_J else None, self.jprec, self.d, u, self.g)
+        if want_J and self.multigrid:
+            self.disc.mg_setup(self.jprec, u, u_old, dt, s_a=1.0, s_m=s_m, parts=parts)
         return self.d
 
     def solve(self, u, u_old=None, dt=0.0, s_m=0.0, const=None) -> NewtonStats:
@@ -156,4 +195,9 @@ def cavity_dirichlet(mesh, lid_speed=1.0, pin_pressure=True):
     if pin_pressure:
         dofs = np.append(dofs, mesh.dim)
         vals = np.append(vals, 0.0)
+    unused = mesh.boundary.get("unused", np.zeros(0, dtype=np.int64))
+    if len(unused):
+        extra = (np.asarray(unused)[:, None] * nc + np.arange(nc)[None, :]).ravel()
+        dofs = np.append(dofs, extra)
+        vals = np.append(vals, np.zeros(len(extra)))
     return dofs.astype(np.int64), vals

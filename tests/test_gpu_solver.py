@@ -1,14 +1,17 @@
-"""GPU Newton loop (Dirichlet rows, GMRES with the FP32/FP64 Jacobian,
-FP64 defect) against the CPU fixpoint solve on the oracle."""
+"""GPU Newton loop (Dirichlet rows, GMRES with point-block Jacobi or
+geometric multigrid, FP32/FP64 Jacobian, FP64 defect) against the CPU
+direct-solve Newton on the oracle."""
 from __future__ import annotations
+
+import re
 
 import numpy as np
 import pytest
 
 import oracle as O
-from plugin_navierstokes_amd import F32, F64, NavierStokesFV1
+from plugin_navierstokes_amd import F32, F64, NavierStokesFV1, PART_A, PART_M
 from plugin_navierstokes_amd import meshes as M
-from plugin_navierstokes_amd.solver import BDFStepper, NewtonSolver, cavity_dirichlet
+from plugin_navierstokes_amd.solver import BDFStepper, NewtonSolver, build_multigrid, cavity_dirichlet
 from tests._cpu_newton import newton as cpu_newton
 
 pytestmark = pytest.mark.gpu
@@ -23,25 +26,69 @@ def _disc(mesh, upwind, nu):
     return disc
 
 
-def test_linear_solve_residual(cuda):
+def _sorted_dirichlet(mesh):
+    dofs, vals = cavity_dirichlet(mesh)
+    o = np.argsort(dofs)
+    return dofs[o], vals[o]
+
+
+def _system(cuda, mesh, disc, jprec, s_m=0.0, dt=0.0, seed=1):
     import torch
 
-    mesh = M.cavity_2d(16, M.QUAD, 0.1, 2)
-    disc = _disc(mesh, "full", 0.01)
-    dofs, vals = cavity_dirichlet(mesh)
+    dofs, vals = _sorted_dirichlet(mesh)
     disc.set_dirichlet(dofs)
-    u = torch.from_numpy(M.cavity_lid_state(mesh, 1.0, 0.1 * M.random_state(mesh, 1))).to(cuda)
-    for jprec in [F64, F32]:
-        J, d = disc.assemble(u, jprec=jprec)
-        disc.apply_dirichlet(J, jprec, d, u, torch.as_tensor(vals, device=cuda))
-        b = torch.randn(disc.num_dofs(), dtype=torch.float64, device=cuda)
-        x = torch.empty_like(b)
-        it, rr = disc.linear_solve(J, b, x, jprec=jprec, rtol=1e-8, maxit=2000, restart=40)
-        y = torch.empty_like(b)
-        disc.spmv(J, x, y, jprec=jprec)
-        true = float(torch.linalg.vector_norm(y - b) / torch.linalg.vector_norm(b))
-        print(f"jprec={jprec} GMRES its={it} est={rr:.2e} true={true:.2e}")
-        assert rr <= 1e-8 and true <= 1e-7
+    base = 0.1 * M.random_state(mesh, seed)
+    if mesh.dim == 3:
+        base = M.divergence_free_perturbation(mesh, 1e-3, seed=seed)
+    u = torch.from_numpy(M.cavity_lid_state(mesh, 1.0, base)).to(cuda)
+    parts = PART_A | (PART_M if s_m else 0)
+    uold = u if s_m else None
+    J, d = disc.assemble(u, uold, dt, s_m=s_m, parts=parts, jprec=jprec)
+    disc.apply_dirichlet(J, jprec, d, u, torch.as_tensor(vals, device=cuda))
+    return u, uold, parts, J
+
+
+def _check_solve(cuda, disc, J, jprec, rtol, maxit, restart):
+    import torch
+
+    b = torch.randn(disc.num_dofs(), dtype=torch.float64, device=cuda, generator=torch.Generator(cuda).manual_seed(0))
+    x = torch.empty_like(b)
+    it, rr = disc.linear_solve(J, b, x, jprec=jprec, rtol=rtol, maxit=maxit, restart=restart)
+    y = torch.empty_like(b)
+    disc.spmv(J, x, y, jprec=jprec)
+    true = float(torch.linalg.vector_norm(y - b) / torch.linalg.vector_norm(b))
+    return it, rr, true
+
+
+@pytest.mark.parametrize("jprec", [F64, F32])
+def test_block_jacobi_gmres(cuda, jprec):
+    mesh = M.cavity_2d(8, M.QUAD, 0.1, 2)
+    disc = _disc(mesh, "full", 0.01)
+    _, _, _, J = _system(cuda, mesh, disc, jprec)
+    it, rr, true = _check_solve(cuda, disc, J, jprec, 1e-8, 4000, 40)
+    print(f"block-Jacobi GMRES jprec={jprec}: its={it} est={rr:.2e} true={true:.2e}")
+    assert rr <= 1e-8 and true <= 1e-7
+
+
+@pytest.mark.parametrize("case", ["quad32-steady", "tri32-steady", "hex16-bdf2", "tet8-bdf2"])
+@pytest.mark.parametrize("jprec", [F64, F32])
+def test_multigrid_gmres(cuda, case, jprec):
+    """One V-cycle per GMRES step keeps the iteration count small and the
+    estimated residual equals the true one."""
+    kind, n, _ = re.match(r"([a-z]+)(\d+)-(\w+)", case).groups()
+    et = {"quad": M.QUAD, "tri": M.TRI, "hex": M.HEX, "tet": M.TET}[kind]
+    n = int(n)
+    mesh = M.cavity_2d(n, et, 0.1, 3) if et in (M.QUAD, M.TRI) else M.cavity_3d(n, et)
+    inst = case.endswith("bdf2")
+    disc = _disc(mesh, "full" if mesh.dim == 2 else "skewed", 0.01 if mesh.dim == 2 else 1 / 400)
+    s_m, dt = (1.5 / 0.01, 0.01) if inst else (0.0, 0.0)
+    u, uold, parts, J = _system(cuda, mesh, disc, jprec, s_m, dt)
+    levels = build_multigrid(disc, mesh, lambda m: _sorted_dirichlet(m))
+    disc.mg_setup(jprec, u, uold, dt, s_m=s_m, parts=parts)
+    it, rr, true = _check_solve(cuda, disc, J, jprec, 1e-8, 300, 30)
+    print(f"GMG GMRES {case} jprec={jprec} levels={len(levels) + 1}: its={it} est={rr:.2e} true={true:.2e}")
+    assert rr <= 1e-8 and true <= 2e-8
+    assert it <= 60
 
 
 @pytest.mark.parametrize("jprec", [F64, F32])
@@ -52,45 +99,48 @@ def test_steady_cavity_matches_cpu(cuda, jprec):
 
     mesh = M.cavity_2d(24, M.QUAD, 0.1, 5)
     disc = _disc(mesh, "full", 0.01)
-    dofs, vals = cavity_dirichlet(mesh)
-    nt = NewtonSolver(disc, dofs, vals, jprec=jprec, rtol=1e-10, maxit=80, lin_rtol=1e-6, lin_maxit=2000, restart=40)
+    dofs, vals = _sorted_dirichlet(mesh)
+    build_multigrid(disc, mesh, _sorted_dirichlet)
+    nt = NewtonSolver(disc, dofs, vals, jprec=jprec, rtol=1e-11, maxit=80, lin_rtol=1e-4, lin_maxit=300, restart=30,
+                      multigrid=True)
     u = torch.zeros(mesh.nv, 3, dtype=torch.float64, device=cuda)
     st = nt.solve(u)
     print(f"GPU Newton jprec={jprec}: its={st.iterations} lin={st.linear_iterations} "
           f"res {st.residuals[0]:.3e} -> {st.residuals[-1]:.3e}")
     assert st.converged
     p = O.Params(dim=2, upwind="full", nu=0.01)
-    ucpu, hist = cpu_newton(p, mesh, dofs=np.sort(dofs), vals=vals[np.argsort(dofs)], rtol=1e-12, maxit=80)
+    ucpu, hist = cpu_newton(p, mesh, dofs=dofs, vals=vals, rtol=1e-13, maxit=80)
     err = np.abs(u.cpu().numpy() - ucpu).max()
     print(f"max |u_gpu - u_cpu| = {err:.3e}")
-    assert err < 1e-7
+    assert err < 1e-8
 
 
 def test_bdf2_steps_match_cpu(cuda):
-    """Two BDF steps (BDF1 start, BDF2) of a small 3D hexahedral cavity with
-    skewed upwind against the same scheme on the CPU oracle."""
+    """Three BDF steps (BDF1 start, then BDF2) of an 8^3 hexahedral cavity with
+    skewed upwind, FP32 Jacobian + multigrid, against the same scheme on the
+    CPU oracle with direct solves."""
     import torch
 
-    mesh = M.cavity_3d(4, M.HEX)
+    mesh = M.cavity_3d(8, M.HEX)
     nu, dt = 1.0 / 400.0, 0.01
     disc = _disc(mesh, "skewed", nu)
-    dofs, vals = cavity_dirichlet(mesh)
-    nt = NewtonSolver(disc, dofs, vals, jprec=F64, rtol=1e-11, maxit=60, lin_rtol=1e-8, lin_maxit=3000, restart=50)
+    dofs, vals = _sorted_dirichlet(mesh)
+    build_multigrid(disc, mesh, _sorted_dirichlet)
+    nt = NewtonSolver(disc, dofs, vals, jprec=F32, rtol=1e-11, maxit=40, lin_rtol=1e-4, lin_maxit=300, restart=30,
+                      multigrid=True)
     stepper = BDFStepper(nt, dt)
     u0 = M.cavity_lid_state(mesh, 1.0, M.divergence_free_perturbation(mesh, 1e-3, seed=3))
     u_n = torch.from_numpy(u0).to(cuda)
     gpu = []
-    for _ in range(2):
+    for _ in range(3):
         u_n, st = stepper.step(u_n, guess=stepper.extrapolate(u_n))
+        print(f"GPU step: newton its={st.iterations} lin={st.linear_iterations} res={st.residuals[-1]:.2e}")
         assert st.converged
         gpu.append(u_n.cpu().numpy())
-    # CPU: same BDF1 / BDF2 residuals with the oracle's M and A parts
     p = O.Params(dim=3, upwind="skewed", nu=nu)
     rp, ci = O.pattern(O.HEX, mesh.nv, mesh.conn)
-    order = np.argsort(dofs)
-    dd, dv = dofs[order], vals[order]
     prev, cur = None, u0
-    for step in range(2):
+    for step in range(3):
         if prev is None:
             s_m = 1.0 / dt
             _, cM = O.assemble(p, O.HEX, mesh.xyz, mesh.conn, cur, s_a=0.0, s_m=1.0, parts=2, rowptr=rp, colidx=ci)
@@ -100,8 +150,8 @@ def test_bdf2_steps_match_cpu(cuda):
             w = -2.0 * cur + 0.5 * prev
             _, cM = O.assemble(p, O.HEX, mesh.xyz, mesh.conn, w, s_a=0.0, s_m=1.0, parts=2, rowptr=rp, colidx=ci)
             const = cM / dt
-        unew, hist = cpu_newton(p, mesh, u0=(cur if prev is None else 2 * cur - prev), dofs=dd, vals=dv, rtol=1e-12,
-                                maxit=60, uold=cur, dt=dt, s_m=s_m, const=const)
+        unew, hist = cpu_newton(p, mesh, u0=(cur if prev is None else 2 * cur - prev), dofs=dofs, vals=vals,
+                                rtol=1e-13, maxit=60, uold=cur, dt=dt, s_m=s_m, const=const)
         prev, cur = cur, unew
         err = np.abs(gpu[step] - cur).max()
         print(f"BDF step {step}: max |u_gpu - u_cpu| = {err:.3e}")
