@@ -156,6 +156,19 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
                    double s_a, double s_m, double s_f, int parts,
                    void* jvals, int jprec, double* defect, void* stream);
 
+/* Skewed-upwind decisions (the corner nearest the ray's cut point,
+   upwind.cpp:381-430, or the zero-velocity guard) of later assemblies:
+   NSFV1_UPWIND_FRESH (default) decides anew on every assembly;
+   NSFV1_UPWIND_RECORD decides anew and stores each SCVF's decision;
+   NSFV1_UPWIND_REPLAY reuses the stored decisions (no ray cast).  No
+   counterpart in the plugin: the corner choice is a discontinuous function
+   of the velocity, so a Newton iteration that lands next to a switching
+   point can cycle between two states; replaying the decisions of one
+   iterate makes the defect smooth in u for the rest of a solve.  Other
+   upwinds are not affected. */
+enum { NSFV1_UPWIND_FRESH = 0, NSFV1_UPWIND_RECORD = 1, NSFV1_UPWIND_REPLAY = 2 };
+int nsfv1_set_upwind_decisions(nsfv1_handle* h, int mode);
+
 /* Synchronises `stream` and reports (then clears) errors raised inside the
    kernels since the last call: NSFV1_ERR_CUT_SIDE when a skewed/LPS upwind
    ray found no side (upwind.cpp:354, 422, 550), NSFV1_ERR_DEGENERATE for an

@@ -41,8 +41,9 @@ EXPORTS = [
     "nsfv1_set_stabilization_upwind", "nsfv1_set_pac_upwind", "nsfv1_set_mesh", "nsfv1_num_dofs", "nsfv1_nnz",
     "nsfv1_get_pattern", "nsfv1_pattern_device", "nsfv1_assemble", "nsfv1_sync_errors", "nsfv1_set_launch",
     "nsfv1_set_dirichlet", "nsfv1_apply_dirichlet", "nsfv1_spmv", "nsfv1_linear_solve",
-    "nsfv1_set_preconditioner", "nsfv1_mg_set_hierarchy", "nsfv1_mg_setup",
+    "nsfv1_set_preconditioner", "nsfv1_mg_set_hierarchy", "nsfv1_mg_setup", "nsfv1_set_upwind_decisions",
 ]
+UPWIND_FRESH, UPWIND_RECORD, UPWIND_REPLAY = 0, 1, 2
 
 
 class NSFV1Error(RuntimeError):
@@ -94,6 +95,7 @@ def load_library(path: str | None = None):
     L.nsfv1_linear_solve.argtypes = [H, vp, C.c_int, vp, vp, dp, C.c_int, C.c_int, C.POINTER(C.c_int),
                                      C.POINTER(C.c_double), vp]
     L.nsfv1_set_preconditioner.argtypes = [H, C.c_char_p]
+    L.nsfv1_set_upwind_decisions.argtypes = [H, C.c_int]
     L.nsfv1_mg_set_hierarchy.argtypes = [H, C.c_int, C.POINTER(H), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
                                          C.POINTER(vp), C.c_int, C.c_int, dp]
     L.nsfv1_mg_setup.argtypes = [H, C.c_int, vp, vp, dp, dp, dp, C.c_int, vp]
@@ -268,6 +270,9 @@ class NavierStokesFV1:
         _check(_lib.nsfv1_linear_solve(self._h, _ptr(J), int(jprec), _ptr(b), _ptr(x), float(rtol), int(maxit),
                                        int(restart), C.byref(it), C.byref(rr), _stream_ptr(stream)))
         return it.value, rr.value
+
+    def set_upwind_decisions(self, mode: int):
+        _check(_lib.nsfv1_set_upwind_decisions(self._h, int(mode)))
 
     def set_preconditioner(self, name: str):
         _check(_lib.nsfv1_set_preconditioner(self._h, name.encode()))

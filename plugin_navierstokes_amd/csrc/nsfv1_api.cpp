@@ -125,6 +125,10 @@ struct nsfv1_handle {
     DevBuf mg_r0, mg_dense;
     bool geo_valid = false;
     int geo_difflen = -1;
+    // recorded skewed-upwind decisions
+    DevBuf upw_dec;
+    int upw_mode = 0;
+    bool upw_recorded = false;
 };
 
 extern "C" {
@@ -451,6 +455,8 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
     h->ndir = 0;
     h->mg.clear();
     h->mg_ready = false;
+    h->upw_dec.release();
+    h->upw_recorded = false;
     return NSFV1_OK;
 }
 
@@ -605,13 +611,30 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
         h->geo_valid = true;
         h->geo_difflen = h->difflen;
     }
+    k.upw_mode = h->upw_mode;
+    if (h->upw_mode == NSFV1_UPWIND_REPLAY && !h->upw_recorded)
+        return fail(NSFV1_ERR_ARGUMENT, "no recorded upwind decisions to replay");
+    if (h->upw_mode == NSFV1_UPWIND_RECORD && h->upw_dec.p == nullptr) {
+        const int nip = h->et == 0 ? 3 : (h->et == 1 ? 4 : (h->et == 2 ? 6 : 12));
+        if (h->upw_dec.alloc((size_t)h->ne * nip) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "upwind record buffer");
+    }
     KMeshPtrs m{h->xyz.as<double>(), h->conn.as<int32_t>(), h->rowptr.as<int64_t>(), h->colidx.as<int32_t>(),
                 h->node_inc.as<int32_t>(), h->inc_elem.as<int32_t>(), h->inc_node.as<int32_t>(),
                 h->inc_info.as<int32_t>(), P.wg_node.as<int32_t>(), P.wg_rounds.as<int32_t>(), h->l2inv.as<double>(),
-                h->vol.as<double>()};
+                h->vol.as<double>(), h->upw_dec.as<uint8_t>()};
     hipError_t e = launch_assemble(h->et, jprec, m, k, u, u_old, jvals, defect, h->errflag.as<int>(), P.nwg, P.lds,
                                    (hipStream_t)stream);
     if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("assemble launch: ") + hipGetErrorString(e));
+    if (h->upw_mode == NSFV1_UPWIND_RECORD && (parts & NSFV1_PART_A)) h->upw_recorded = true;
+    return NSFV1_OK;
+}
+
+int nsfv1_set_upwind_decisions(nsfv1_handle* h, int mode)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (mode != NSFV1_UPWIND_FRESH && mode != NSFV1_UPWIND_RECORD && mode != NSFV1_UPWIND_REPLAY)
+        return fail(NSFV1_ERR_ARGUMENT, "upwind decision mode must be 0 (fresh), 1 (record) or 2 (replay)");
+    h->upw_mode = mode;
     return NSFV1_OK;
 }
 

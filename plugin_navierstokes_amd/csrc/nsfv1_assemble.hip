@@ -416,7 +416,7 @@ __device__ inline bool upwind_canon(int type, bool kvfrom, const double (&co)[To
                                     const double* __restrict__ xyz, const int32_t* __restrict__ econn, const int* pk,
                                     const double* ipg, const double* n, const double* sv,
                                     double (&U)[Topo<ET>::nsh], double& convlen, bool& have_cut, int& side,
-                                    double* gi, double* li)
+                                    double* gi, double* li, uint8_t* decp, int mode)
 {
     using T = Topo<ET>;
     constexpr int D = T::dim, N = T::nsh;
@@ -443,7 +443,28 @@ __device__ inline bool upwind_canon(int type, bool kvfrom, const double (&co)[To
     }
 #pragma unroll
     for (int k = 0; k < N; ++k) U[k] = 0.0;
+    if (type == UPW_SKEWED && mode == 2 && decp) {
+        // replay a recorded decision (no ray cast)
+        const int best = *decp;
+        if (best == 255) {
+            convlen = 1.0;
+            return true;
+        }
+        double cc[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) cc[i] = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            const bool on = (pk[k] == best);
+            U[k] = on ? 1.0 : 0.0;
+#pragma unroll
+            for (int i = 0; i < D; ++i) cc[i] = on ? co[k][i] : cc[i];
+        }
+        convlen = sqrt(dist2_nc<D>(ipg, cc));
+        return true;
+    }
     if (sqrt(dot_nc<D>(sv, sv)) < 1e-14) {
+        if (type == UPW_SKEWED && mode == 1 && decp && kvfrom) *decp = 255;
         convlen = 1.0;
         return true;
     }
@@ -473,6 +494,7 @@ __device__ inline bool upwind_canon(int type, bool kvfrom, const double (&co)[To
                 if (dd < mn) { mn = dd; best = T::side[s][j]; }
             }
         }
+        if (mode == 1 && decp && kvfrom) *decp = (uint8_t)best;
         double cc[D];
 #pragma unroll
         for (int i = 0; i < D; ++i) cc[i] = 0.0;
@@ -525,7 +547,8 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
                                   const double (&ul)[Topo<ET>::nsh][Topo<ET>::dim + 1], const int* pk,
                                   const int32_t* ccn, const double* c, const double* __restrict__ uold,
                                   double l2, bool kvfrom, bool wantJ, Lane<ET, JT>& L, Rec<ET, JT>& R,
-                                  bool& okg, bool& oku, bool& have_cut, int& side, double* gi, double* li)
+                                  bool& okg, bool& oku, bool& have_cut, int& side, double* gi, double* li,
+                                  uint8_t* decp)
 {
     using T = Topo<ET>;
     constexpr int D = T::dim, N = T::nsh;
@@ -592,7 +615,7 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
         double Us[N];
 #pragma unroll
         for (int k = 0; k < N; ++k) Us[k] = 0.0;
-        if (!C.stokes()) oku &= upwind_canon<ET, Q>(C.stab_up(), kvfrom, co, xyz, econn, pk, ipg, n, sv, Us, Ls, have_cut, side, gi, li);
+        if (!C.stokes()) oku &= upwind_canon<ET, Q>(C.stab_up(), kvfrom, co, xyz, econn, pk, ipg, n, sv, Us, Ls, have_cut, side, gi, li, decp, P.upw_mode);
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             double acc = 0;
@@ -605,7 +628,7 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
         for (int k = 0; k < N; ++k) { R.Us[k] = JT(Us[k]); R.Uc[k] = JT(Us[k]); }
         if (!C.stokes() && C.two_upwinds()) {
             double Uc[N], Lc;
-            oku &= upwind_canon<ET, Q>(C.conv_up(), kvfrom, co, xyz, econn, pk, ipg, n, sv, Uc, Lc, have_cut, side, gi, li);
+            oku &= upwind_canon<ET, Q>(C.conv_up(), kvfrom, co, xyz, econn, pk, ipg, n, sv, Uc, Lc, have_cut, side, gi, li, decp, P.upw_mode);
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 double acc = 0;
@@ -786,17 +809,19 @@ __device__ inline void scvf_all(const Cfg<FAST>& C, const double (&co)[Topo<ET>:
                                 const double* __restrict__ xyz, const int32_t* __restrict__ econn,
                                 const double (&ul)[Topo<ET>::nsh][Topo<ET>::dim + 1], const int* pk,
                                 const int32_t* ccn, const double* c, const double* uold, const double* l2row,
-                                int kv, bool wantJ, Lane<ET, JT>& L, Rec<ET, JT>* R, bool& okg, bool& oku)
+                                uint8_t* decrow, int kv, bool wantJ, Lane<ET, JT>& L, Rec<ET, JT>* R, bool& okg,
+                                bool& oku)
 {
     if constexpr (Q < Topo<ET>::cip) {
         bool have_cut = false;
         int side = -1;
         double gi[3] = {0, 0, 0}, li[3] = {0, 0, 0};
-        const double l2 = l2row[CanonH<ET>::c.oip[kv][Q]];
+        const int oip = CanonH<ET>::c.oip[kv][Q];
+        const double l2 = l2row[oip];
         const bool kvfrom = CanonH<ET>::c.from[kv][Q] != 0;
         scvf_canon<ET, FAST, Q, JT>(C, co, xyz, econn, ul, pk, ccn, c, uold, l2, kvfrom, wantJ, L, R[Q], okg, oku, have_cut,
-                                    side, gi, li);
-        scvf_all<ET, FAST, Q + 1, JT>(C, co, xyz, econn, ul, pk, ccn, c, uold, l2row, kv, wantJ, L, R, okg, oku);
+                                    side, gi, li, decrow ? decrow + oip : nullptr);
+        scvf_all<ET, FAST, Q + 1, JT>(C, co, xyz, econn, ul, pk, ccn, c, uold, l2row, decrow, kv, wantJ, L, R, okg, oku);
     }
 }
 
@@ -876,7 +901,8 @@ assemble_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const double
             double c[D];
             elem_centre<ET>(co, c);
             Rec<ET, JT> R[T::cip];
-            scvf_all<ET, FAST, 0, JT>(C, co, M.xyz, M.conn + (int64_t)e * N, ul, pk, ccn, c, uold, M.l2inv + (int64_t)e * NIP, kv, wantJ, L, R,
+            scvf_all<ET, FAST, 0, JT>(C, co, M.xyz, M.conn + (int64_t)e * N, ul, pk, ccn, c, uold, M.l2inv + (int64_t)e * NIP,
+                                      (P.upw_mode != 0 && M.upw_dec) ? M.upw_dec + (int64_t)e * NIP : nullptr, kv, wantJ, L, R,
                                       okg, oku);
             if (wantJ) jac_all<ET, FAST, 0, JT>(C, R, L);
         }

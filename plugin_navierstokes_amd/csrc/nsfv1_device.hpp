@@ -19,6 +19,7 @@ struct KParams {
     int conv_upwind, stab_upwind, stab_type, pac, difflen;
     int stokes, laplace, peclet, has_source, has_old;
     int parts;
+    int upw_mode;  // skewed-upwind decisions: 0 fresh, 1 fresh + record, 2 replay the record
     double exact, nu, rho, dt, inv_rho, inv_dt;
     double f[3];
     double s_a, s_m, s_f;

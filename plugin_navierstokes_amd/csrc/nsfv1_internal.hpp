@@ -23,6 +23,7 @@ struct KMeshPtrs {
     const int32_t* wg_rounds; // [nwg] incidence slots of the workgroup
     const double* l2inv;      // [ne][nip] diffusion-length factor per SCVF
     const double* vol;        // [ne][nsh] SCV volumes
+    uint8_t* upw_dec;         // [ne][nip] recorded skewed-upwind corner (255: zero velocity), or null
 };
 
 hipError_t launch_assemble(int et, int jprec, const KMeshPtrs& m, const KParams& p, const double* u,

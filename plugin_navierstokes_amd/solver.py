@@ -25,7 +25,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import F32, F64, PART_A, PART_M
+from . import F32, F64, PART_A, PART_M, UPWIND_FRESH, UPWIND_RECORD, UPWIND_REPLAY
 
 
 @dataclass
@@ -35,6 +35,8 @@ class NewtonStats:
     residuals: list = field(default_factory=list)
     converged: bool = False
     seconds: float = 0.0
+    reference: float = 0.0  # the norm the relative tolerance applies to
+    fresh_residual: float | None = None  # with frozen upwind decisions: final defect with fresh decisions
 
 
 def build_multigrid(disc, mesh, dirichlet, max_coarse_dofs=1200, pre=2, post=2, omega=0.7):
@@ -77,10 +79,13 @@ class NewtonSolver:
     build_multigrid); otherwise by point-block Jacobi."""
 
     def __init__(self, disc, dirichlet_dofs, dirichlet_vals, jprec=F32, rtol=1e-8, atol=1e-14, maxit=60,
-                 lin_rtol=1e-3, lin_maxit=400, restart=30, multigrid=False):
+                 lin_rtol=1e-3, lin_maxit=400, restart=30, multigrid=False, freeze_upwind_after=None):
         import torch
 
         self.multigrid = multigrid
+        # skewed upwind: decide anew for the first k-1 iterations, record at
+        # iteration k-1, replay afterwards (None: always decide anew)
+        self.freeze_after = freeze_upwind_after
         self.disc = disc
         self.jprec = jprec
         self.rtol, self.atol, self.maxit = rtol, atol, maxit
@@ -102,29 +107,60 @@ class NewtonSolver:
         """Set the Dirichlet values in u (flat view)."""
         u.view(-1)[self.dofs_t] = self.g
 
-    def defect(self, u, u_old=None, dt=0.0, s_m=0.0, const=None, want_J=True):
+    def defect(self, u, u_old=None, dt=0.0, s_m=0.0, const=None, want_J=True, mg_setup=True):
         parts = PART_A | (PART_M if s_m else 0)
         self.disc.assemble(u, u_old, dt, s_a=1.0, s_m=s_m, parts=parts, jprec=self.jprec,
                            J=self.J if want_J else None, d=self.d, want_J=want_J, check=False)
         if const is not None:
             self.d += const
         self.disc.apply_dirichlet(self.J if want_J else None, self.jprec, self.d, u, self.g)
-        if want_J and self.multigrid:
+        if want_J and self.multigrid and mg_setup:
             self.disc.mg_setup(self.jprec, u, u_old, dt, s_a=1.0, s_m=s_m, parts=parts)
         return self.d
 
-    def solve(self, u, u_old=None, dt=0.0, s_m=0.0, const=None) -> NewtonStats:
+    def defect_norm(self, u, u_old=None, dt=0.0, s_m=0.0, const=None) -> float:
+        """||d(u)|| in FP64 (defect only, no Jacobian)."""
+        import torch
+
+        return float(torch.linalg.vector_norm(self.defect(u, u_old, dt, s_m, const, want_J=False)))
+
+    def solve(self, u, u_old=None, dt=0.0, s_m=0.0, const=None, ref_norm=None, mg_lag=False) -> NewtonStats:
+        """Iterate until ||d(u)|| <= max(rtol * ref, atol), ref = ref_norm or
+        the defect norm of the initial guess.  mg_lag keeps the multigrid
+        levels of the first iteration for the whole solve (the Krylov
+        operator is always the current Jacobian)."""
         import torch
 
         st = NewtonStats()
         t0 = time.perf_counter()
         self.impose(u)
         uf = u.view(-1)
+        try:
+            self._iterate(st, u, uf, u_old, dt, s_m, const, ref_norm, mg_lag)
+        finally:
+            if self.freeze_after is not None:
+                self.disc.set_upwind_decisions(UPWIND_FRESH)
+        if self.freeze_after is not None and st.iterations >= self.freeze_after:
+            st.fresh_residual = self.defect_norm(u, u_old, dt, s_m, const)
+        self.disc.sync_errors()
+        torch.cuda.synchronize()
+        st.seconds = time.perf_counter() - t0
+        return st
+
+    def _iterate(self, st, u, uf, u_old, dt, s_m, const, ref_norm, mg_lag):
+        import torch
+
+        k = self.freeze_after
         for it in range(self.maxit + 1):
-            d = self.defect(u, u_old, dt, s_m, const)
+            if k is not None:
+                self.disc.set_upwind_decisions(UPWIND_FRESH if it < k - 1 else
+                                               (UPWIND_RECORD if it == k - 1 else UPWIND_REPLAY))
+            d = self.defect(u, u_old, dt, s_m, const, mg_setup=(it == 0 or not mg_lag))
             r = float(torch.linalg.vector_norm(d))
             st.residuals.append(r)
-            if r <= max(self.rtol * st.residuals[0], self.atol):
+            ref = ref_norm if ref_norm is not None else st.residuals[0]
+            st.reference = ref
+            if r <= max(self.rtol * ref, self.atol):
                 st.converged = True
                 break
             if it == self.maxit:
@@ -134,20 +170,24 @@ class NewtonSolver:
             st.linear_iterations += its
             uf -= self.delta
             st.iterations += 1
-        self.disc.sync_errors()
-        torch.cuda.synchronize()
-        st.seconds = time.perf_counter() - t0
-        return st
 
 
 class BDFStepper:
-    """BDF1 start, then BDF2 with constant dt, for M du/dt + A(u) = 0."""
+    """BDF1 start, then BDF2 with constant dt, for M du/dt + A(u) = 0.
 
-    def __init__(self, newton: NewtonSolver, dt: float):
+    reference="old" measures convergence against the defect of the step's
+    system at the old state u^n, so the stopping point does not depend on
+    the initial guess (the guess only changes how many iterations it takes);
+    reference="guess" uses the defect of the initial guess."""
+
+    def __init__(self, newton: NewtonSolver, dt: float, reference: str = "guess", mg_lag: bool = False):
         self.nt = newton
         self.dt = dt
         self.u_prev = None  # u^{n-1}
         self.steps = 0
+        self.reference = reference
+        self.mg_lag = mg_lag
+        self.last_choice = None
 
     def mass_const(self, w, scale):
         """scale * M w as a device vector (defect of the mass part, linear)."""
@@ -158,16 +198,30 @@ class BDFStepper:
                               want_J=False, check=False)
         return c
 
-    def step(self, u_n, guess=None):
-        """Returns (u^{n+1}, NewtonStats); u_n is not modified."""
+    def step(self, u_n, guess=None, candidates=None):
+        """Returns (u^{n+1}, NewtonStats); u_n is not modified.  candidates:
+        optional list of initial guesses; the one with the smallest FP64
+        defect starts the Newton iteration (index kept in last_choice)."""
         dt = self.dt
-        u = (guess if guess is not None else u_n).clone()
         if self.u_prev is None:
             s_m, const = 1.0 / dt, self.mass_const(u_n, -1.0 / dt)
         else:
             w = -2.0 * u_n + 0.5 * self.u_prev
             s_m, const = 1.5 / dt, self.mass_const(w, 1.0 / dt)
-        st = self.nt.solve(u, u_old=u_n, dt=dt, s_m=s_m, const=const)
+        ref = None
+        if self.reference == "old":
+            ref = self.nt.defect_norm(u_n, u_n, dt, s_m, const)
+        if candidates:
+            norms = []
+            for c in candidates:
+                self.nt.impose(c)
+                norms.append(self.nt.defect_norm(c, u_n, dt, s_m, const))
+            k = int(np.argmin(norms))
+            self.last_choice = k
+            u = candidates[k].clone()
+        else:
+            u = (guess if guess is not None else u_n).clone()
+        st = self.nt.solve(u, u_old=u_n, dt=dt, s_m=s_m, const=const, ref_norm=ref, mg_lag=self.mg_lag)
         self.u_prev = u_n.clone()
         self.steps += 1
         return u, st

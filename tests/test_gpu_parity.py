@@ -165,3 +165,34 @@ def test_error_cases(cuda):
     assert e.value.name == "NO_VISCOSITY"
     disc.set_kinematic_viscosity(0.1)
     disc.assemble(u)
+
+
+@pytest.mark.parametrize("et", [M.HEX, M.TET, M.QUAD, M.TRI])
+def test_upwind_record_replay(cuda, et):
+    """Replaying the decisions recorded at u reproduces the fresh assembly at
+    u bitwise; replay at another state uses the recorded corners."""
+    import torch
+
+    from plugin_navierstokes_amd import UPWIND_FRESH, UPWIND_RECORD, UPWIND_REPLAY
+
+    mesh = M.cavity_2d(6, et, 0.1, 4) if et in (M.QUAD, M.TRI) else M.cavity_3d(4, et, 0.1, 4)
+    disc = NavierStokesFV1("u,v,p" if mesh.dim == 2 else "u,v,w,p", "Inner")
+    disc.set_stabilization("fields")
+    disc.set_upwind("skewed")
+    disc.set_kinematic_viscosity(0.01)
+    disc.set_mesh(mesh)
+    u = torch.from_numpy(M.random_state(mesh, 7)).to(cuda)
+    uold = 0.9 * u
+    J0, d0 = disc.assemble(u, uold, 0.01, jprec=F64)
+    disc.set_upwind_decisions(UPWIND_RECORD)
+    J1, d1 = disc.assemble(u, uold, 0.01, jprec=F64)
+    disc.set_upwind_decisions(UPWIND_REPLAY)
+    J2, d2 = disc.assemble(u, uold, 0.01, jprec=F64)
+    assert torch.equal(J0, J1) and torch.equal(d0, d1)
+    assert torch.equal(J0, J2) and torch.equal(d0, d2)
+    # a reversed flow field replayed with the old decisions differs from a fresh assembly
+    J3, d3 = disc.assemble(-u, -uold, 0.01, jprec=F64)
+    disc.set_upwind_decisions(UPWIND_FRESH)
+    J4, d4 = disc.assemble(-u, -uold, 0.01, jprec=F64)
+    assert not torch.equal(d3, d4)
+    disc.sync_errors()
