@@ -98,6 +98,21 @@ def timestep_leg(mesh, disc, steps: int, dt: float = 0.01):
             "workload": f"{mesh.shape[0]}^3 hex cavity, Re=400, skewed+FIELDS, BDF2 dt={dt}, FP32 J + FP64 defect"}
 
 
+def learned_leg(steps: int):
+    """Plain vs learned initial guess on a held-out 96x48x48x6-tetrahedra
+    channel (Re, obstacles and seed not in the training set): BDF2 dt=0.005,
+    LPS + FIELDS, FP32 J + FP64 defect, 1e-8 relative FP64 residual; 6
+    untimed warm-up steps, then `steps` timed steps per method."""
+    from plugin_navierstokes_amd.learned import compare
+
+    w = os.path.join(ROOT, "plugin_navierstokes_amd", "data", "predictor_channel.npz")
+    r = compare(w, seeds=(100,), steps=steps, warm=6, log=lambda m: print("bench: " + m, file=sys.stderr, flush=True))
+    r["workload"] = ("channel 96x48x48 hexahedra x 6 tetrahedra (1.26M elements after obstacles), held-out seed 100, "
+                     "BDF2 dt=0.005, LPS+FIELDS, FP32 J + FP64 defect, residual 1e-8 x ||d(u^n)||")
+    r["learned_vs_plain"] = r["learned_s_per_step"] / r["plain_s_per_step"]
+    return r
+
+
 def measured_traffic(n: int):
     """HBM bytes per assembly launch from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE passes scaled by a plain-copy calibration);
@@ -211,6 +226,9 @@ def main():
         del J, d
         torch.cuda.empty_cache()
         out["timestep"] = {"plain": timestep_leg(mesh, disc, args.timestep_steps)}
+        del disc
+        torch.cuda.empty_cache()
+        out["timestep"]["channel"] = learned_leg(args.timestep_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(out), flush=True)

@@ -230,6 +230,42 @@ int nsfv1_mg_set_hierarchy(nsfv1_handle* h, int nlev, nsfv1_handle* const* coars
 int nsfv1_mg_setup(nsfv1_handle* h, int jprec, const double* u, const double* u_old, double dt, double s_a,
                    double s_m, int parts, void* stream);
 
+/* ---- Learned Newton initial guess (no counterpart in the plugin) ----
+   A per-node MLP 12 -> 128 -> 128 -> 4 with ReLU, BF16 weights and
+   activations, FP32 accumulation (MFMA).  params: FP32 blob of nparam
+   floats, layout W1[128][16] (input columns 12..15 zero), b1[128],
+   W2[128][128], b2[128], W3[16][128] (output rows 4..15 zero), b3[16];
+   wbf16: the BF16 copy (same offsets) made by nsfv1_mlp_pack or
+   nsfv1_mlp_adam.  All buffers are device pointers; x [n][16] and y [n][4]
+   FP32, 16-byte aligned. */
+int nsfv1_mlp_sizes(int64_t n, int64_t* nparam, int64_t* work_floats);
+int nsfv1_mlp_pack(const float* params, uint16_t* wbf16, void* stream);
+/* y = MLP(x); z1, z2 (nullable, [n][128]) receive the pre-activations for
+   nsfv1_mlp_backward. */
+int nsfv1_mlp_forward(const float* params, const uint16_t* wbf16, const float* x, int64_t n, float* y, float* z1,
+                      float* z2, void* stream);
+/* dy = d(mean squared error)/dy for targets t [n][4]; work holds at least
+   ceil(4n/256) floats; loss (host, nullable) receives the MSE (synchronises). */
+int nsfv1_mlp_mse(const float* y, const float* t, int64_t n, float* dy, float* work, double* loss, void* stream);
+/* grad [nparam] = d loss / d params for the batch (deterministic order);
+   work: nsfv1_mlp_sizes(n) floats. */
+int nsfv1_mlp_backward(const uint16_t* wbf16, const float* x, const float* z1, const float* z2, const float* dy,
+                       int64_t n, float* grad, float* work, void* stream);
+/* One Adam step (bias-corrected) on params with moments m, v; refreshes wbf16. */
+int nsfv1_mlp_adam(float* params, float* m, float* v, const float* grad, int step, double lr, double beta1,
+                   double beta2, double eps, uint16_t* wbf16, void* stream);
+/* Features of every node of h's mesh (3D, [nv][4] states):
+   x[v] = (u^n, u^n - u^{n-1}, 1-ring mean of u^n - u^n) * scale[12]. */
+int nsfv1_predict_features(nsfv1_handle* h, const double* u, const double* u_prev, const float* scale, float* x,
+                           void* stream);
+/* guess = 2 u^n - u^{n-1} + y * out_scale[4] (linear extrapolation plus the
+   learned correction). */
+int nsfv1_predict_guess(nsfv1_handle* h, const double* u, const double* u_prev, const float* y, const float* out_scale,
+                        double* guess, void* stream);
+/* training target t = (u^{n+1} - 2 u^n + u^{n-1}) / out_scale. */
+int nsfv1_predict_target(nsfv1_handle* h, const double* u_next, const double* u, const double* u_prev,
+                         const float* out_scale, float* t, void* stream);
+
 /* Launch shape of the assembly: incidences (threads) per workgroup (64..256,
    a multiple of 64) and the LDS budget per workgroup in bytes (4..64 KiB);
    0 keeps the current value.  Used by the autotuner. */

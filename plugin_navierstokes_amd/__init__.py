@@ -42,6 +42,8 @@ EXPORTS = [
     "nsfv1_get_pattern", "nsfv1_pattern_device", "nsfv1_assemble", "nsfv1_sync_errors", "nsfv1_set_launch",
     "nsfv1_set_dirichlet", "nsfv1_apply_dirichlet", "nsfv1_spmv", "nsfv1_linear_solve",
     "nsfv1_set_preconditioner", "nsfv1_mg_set_hierarchy", "nsfv1_mg_setup", "nsfv1_set_upwind_decisions",
+    "nsfv1_mlp_sizes", "nsfv1_mlp_pack", "nsfv1_mlp_forward", "nsfv1_mlp_mse", "nsfv1_mlp_backward", "nsfv1_mlp_adam",
+    "nsfv1_predict_features", "nsfv1_predict_guess", "nsfv1_predict_target",
 ]
 UPWIND_FRESH, UPWIND_RECORD, UPWIND_REPLAY = 0, 1, 2
 
@@ -96,6 +98,15 @@ def load_library(path: str | None = None):
                                      C.POINTER(C.c_double), vp]
     L.nsfv1_set_preconditioner.argtypes = [H, C.c_char_p]
     L.nsfv1_set_upwind_decisions.argtypes = [H, C.c_int]
+    L.nsfv1_mlp_sizes.argtypes = [C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.nsfv1_mlp_pack.argtypes = [vp, vp, vp]
+    L.nsfv1_mlp_forward.argtypes = [vp, vp, vp, C.c_int64, vp, vp, vp, vp]
+    L.nsfv1_mlp_mse.argtypes = [vp, vp, C.c_int64, vp, vp, C.POINTER(C.c_double), vp]
+    L.nsfv1_mlp_backward.argtypes = [vp, vp, vp, vp, vp, C.c_int64, vp, vp, vp]
+    L.nsfv1_mlp_adam.argtypes = [vp, vp, vp, vp, C.c_int, dp, dp, dp, dp, vp, vp]
+    L.nsfv1_predict_features.argtypes = [H, vp, vp, vp, vp, vp]
+    L.nsfv1_predict_guess.argtypes = [H, vp, vp, vp, vp, vp, vp]
+    L.nsfv1_predict_target.argtypes = [H, vp, vp, vp, vp, vp, vp]
     L.nsfv1_mg_set_hierarchy.argtypes = [H, C.c_int, C.POINTER(H), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
                                          C.POINTER(vp), C.c_int, C.c_int, dp]
     L.nsfv1_mg_setup.argtypes = [H, C.c_int, vp, vp, dp, dp, dp, C.c_int, vp]

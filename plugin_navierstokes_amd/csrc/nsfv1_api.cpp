@@ -19,6 +19,7 @@
 #include "../../include/nsfv1.h"
 #include "nsfv1_internal.hpp"
 #include "nsfv1_solver.hpp"
+#include "nsfv1_mlp.hpp"
 
 using namespace nsfv1;
 
@@ -926,16 +927,133 @@ static hipError_t mg_cycle(nsfv1_handle* h, size_t l, const void* J0, const doub
     if (e != hipSuccess) return e;
     MGLevel& C = *h->mg[l];
     nsfv1_handle* cc = C.h;
+    // Dirichlet rows neither send residual to nor take corrections from the
+    // coarse level (their equations are decoupled identities)
     e = launch_residual(jp, rp, ci, J, x, b, r, n, s);
+    if (e == hipSuccess) e = launch_zero_rows(c->dir_rows.as<int64_t>(), c->ndir, r, s);
     if (e == hipSuccess) e = launch_node_transfer(C.R_rp.as<int64_t>(), C.R_ci.as<int32_t>(), C.R_w.as<double>(), r, C.b.as<double>(), cc->nv, nc, 0, s);
     if (e == hipSuccess) e = launch_zero_rows(cc->dir_rows.as<int64_t>(), cc->ndir, C.b.as<double>(), s);
     if (e == hipSuccess) e = mg_cycle(h, l + 1, J0, Binv0, C.b.as<double>(), C.x.as<double>(), s);
-    if (e == hipSuccess) e = launch_node_transfer(C.P_rp.as<int64_t>(), C.P_ci.as<int32_t>(), C.P_w.as<double>(), C.x.as<double>(), x, nv, nc, 1, s);
+    if (e == hipSuccess) e = launch_node_transfer(C.P_rp.as<int64_t>(), C.P_ci.as<int32_t>(), C.P_w.as<double>(), C.x.as<double>(), r, nv, nc, 0, s);
+    if (e == hipSuccess) e = launch_zero_rows(c->dir_rows.as<int64_t>(), c->ndir, r, s);
+    if (e == hipSuccess) e = launch_axpby(1.0, r, 1.0, x, n, s);
     for (int k = 0; k < h->mg_post && e == hipSuccess; ++k) {
         e = launch_residual(jp, rp, ci, J, x, b, r, n, s);
         if (e == hipSuccess) e = launch_bj_update(nc, Bi, r, x, w, 0, nv, s);
     }
     return e;
+}
+
+
+// ---------------------------------------------------------------------
+// learned initial guess (MLP) and its training
+// ---------------------------------------------------------------------
+int nsfv1_mlp_sizes(int64_t n, int64_t* nparam, int64_t* work_floats)
+{
+    if (n < 0) return fail(NSFV1_ERR_ARGUMENT, "negative node count");
+    if (nparam) *nparam = MLP_NPARAM;
+    if (work_floats) *work_floats = mlp_work_floats(n);
+    return NSFV1_OK;
+}
+
+int nsfv1_mlp_pack(const float* params, uint16_t* wbf16, void* stream)
+{
+    if (!params || !wbf16) return fail(NSFV1_ERR_ARGUMENT, "NULL buffer");
+    if (launch_mlp_pack(params, wbf16, (hipStream_t)stream) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "mlp pack");
+    return NSFV1_OK;
+}
+
+int nsfv1_mlp_forward(const float* params, const uint16_t* wbf16, const float* x, int64_t n, float* y, float* z1,
+                      float* z2, void* stream)
+{
+    if (!params || !wbf16 || (n > 0 && (!x || !y))) return fail(NSFV1_ERR_ARGUMENT, "NULL buffer");
+    if (((uintptr_t)x | (uintptr_t)y | (uintptr_t)z1 | (uintptr_t)z2) & 15)
+        return fail(NSFV1_ERR_ARGUMENT, "MLP buffers must be 16-byte aligned");
+    if (launch_mlp_forward(params, wbf16, x, n, y, z1, z2, (hipStream_t)stream) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "mlp forward");
+    return NSFV1_OK;
+}
+
+int nsfv1_mlp_mse(const float* y, const float* t, int64_t n, float* dy, float* work, double* loss, void* stream)
+{
+    if (!y || !t || !dy || !work) return fail(NSFV1_ERR_ARGUMENT, "NULL buffer");
+    if (n <= 0) return fail(NSFV1_ERR_ARGUMENT, "empty batch");
+    int np = 0;
+    hipStream_t s = (hipStream_t)stream;
+    if (launch_mlp_mse(y, t, n, dy, work, &np, s) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "mlp mse");
+    if (loss) {
+        std::vector<float> h(np);
+        if (hipMemcpyAsync(h.data(), work, sizeof(float) * np, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return fail(NSFV1_ERR_DEVICE, "mlp loss copy");
+        double a = 0;
+        for (float v : h) a += v;
+        *loss = a / (double)(n * MLP_OUT);
+    }
+    return NSFV1_OK;
+}
+
+int nsfv1_mlp_backward(const uint16_t* wbf16, const float* x, const float* z1, const float* z2, const float* dy,
+                       int64_t n, float* grad, float* work, void* stream)
+{
+    if (!wbf16 || !x || !z1 || !z2 || !dy || !grad || !work) return fail(NSFV1_ERR_ARGUMENT, "NULL buffer");
+    if (launch_mlp_backward(wbf16, x, z1, z2, dy, n, grad, work, (hipStream_t)stream) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "mlp backward");
+    return NSFV1_OK;
+}
+
+int nsfv1_mlp_adam(float* params, float* m, float* v, const float* grad, int step, double lr, double beta1,
+                   double beta2, double eps, uint16_t* wbf16, void* stream)
+{
+    if (!params || !m || !v || !grad || !wbf16) return fail(NSFV1_ERR_ARGUMENT, "NULL buffer");
+    if (step < 1) return fail(NSFV1_ERR_ARGUMENT, "Adam step counts from 1");
+    if (launch_mlp_adam(params, m, v, grad, step, (float)lr, (float)beta1, (float)beta2, (float)eps, wbf16,
+                        (hipStream_t)stream) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "mlp adam");
+    return NSFV1_OK;
+}
+
+static int predict_check(nsfv1_handle* h)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    if (h->dim != 3) return fail(NSFV1_ERR_ARGUMENT, "the learned predictor takes 3D states (4 components per node)");
+    return NSFV1_OK;
+}
+
+int nsfv1_predict_features(nsfv1_handle* h, const double* u, const double* u_prev, const float* scale, float* x,
+                           void* stream)
+{
+    int rc = predict_check(h);
+    if (rc) return rc;
+    if (!u || !u_prev || !scale || !x) return fail(NSFV1_ERR_ARGUMENT, "NULL buffer");
+    if ((uintptr_t)x & 15) return fail(NSFV1_ERR_ARGUMENT, "feature buffer must be 16-byte aligned");
+    if (launch_predict_features(h->rowptr.as<int64_t>(), h->colidx.as<int32_t>(), u, u_prev, h->nv, scale, x,
+                                (hipStream_t)stream) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "predict features");
+    return NSFV1_OK;
+}
+
+int nsfv1_predict_guess(nsfv1_handle* h, const double* u, const double* u_prev, const float* y, const float* out_scale,
+                        double* guess, void* stream)
+{
+    int rc = predict_check(h);
+    if (rc) return rc;
+    if (!u || !u_prev || !y || !out_scale || !guess) return fail(NSFV1_ERR_ARGUMENT, "NULL buffer");
+    if (launch_predict_guess(u, u_prev, y, out_scale, h->nv, guess, (hipStream_t)stream) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "predict guess");
+    return NSFV1_OK;
+}
+
+int nsfv1_predict_target(nsfv1_handle* h, const double* u_next, const double* u, const double* u_prev,
+                         const float* out_scale, float* t, void* stream)
+{
+    int rc = predict_check(h);
+    if (rc) return rc;
+    if (!u_next || !u || !u_prev || !out_scale || !t) return fail(NSFV1_ERR_ARGUMENT, "NULL buffer");
+    if (launch_predict_target(u_next, u, u_prev, out_scale, h->nv, t, (hipStream_t)stream) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "predict target");
+    return NSFV1_OK;
 }
 
 // Restarted GMRES(m), right-preconditioned with point-block Jacobi, for

@@ -1,0 +1,440 @@
+// nsfv1_mlp.hip -- learned Newton initial guess: a per-node MLP
+// 12 -> 128 -> 128 -> 4 (ReLU), BF16 weights and activations with FP32
+// accumulation on the MFMA units (v_mfma_f32_16x16x16_bf16), plus the FP32
+// backward pass, deterministic gradient reduction and Adam used to train it
+// on trajectories of the plain solver, and the feature / guess kernels that
+// connect it to the discretisation.  No counterpart in the reference.
+//
+// Parameter blob (FP32 master copy, MLP_NPARAM floats):
+//   W1 [128][16] (inputs padded 12 -> 16 with zero columns), b1 [128],
+//   W2 [128][128], b2 [128], W3 [16][128] (outputs padded 4 -> 16 with zero
+//   rows), b3 [16].
+// The BF16 copy (MLP_NPARAM uint16) uses the same offsets; only its weight
+// matrices are read.
+//
+// MFMA mapping (one wavefront = 16 nodes).  The layers are evaluated in
+// transposed form, H^T = W X^T, so that the accumulator layout of one layer
+// (lane l holds rows 4*(l/16)+j, column l%16) is exactly the B-operand
+// layout of the next (k = 4*(l/16)+j, column l%16): activations never leave
+// registers.  A operands (weights) come from LDS.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nsfv1_mlp.hpp"
+
+namespace nsfv1 {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __host__ inline uint16_t f2bf(float f)
+{
+    // round to nearest even (finite inputs)
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ inline float bf2f(uint16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
+
+__device__ inline float relu_bf(float z) { return bf2f(f2bf(z > 0.f ? z : 0.f)); }
+
+// ---------------------------------------------------------------------
+// forward: y[n][0..3] = W3 relu(W2 relu(W1 x + b1) + b2) + b3
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+mlp_forward(const float* __restrict__ P, const uint16_t* __restrict__ WB, const float* __restrict__ X, int64_t n,
+            float* __restrict__ Y, float* __restrict__ Z1, float* __restrict__ Z2)
+{
+    __shared__ __align__(16) uint16_t w1[MLP_H * MLP_KIN];
+    __shared__ __align__(16) uint16_t w2[MLP_H * MLP_H];
+    __shared__ __align__(16) uint16_t w3[MLP_KOUT * MLP_H];
+    __shared__ float bias[2 * MLP_H + MLP_KOUT];
+    for (int i = threadIdx.x; i < MLP_H * MLP_KIN; i += blockDim.x) w1[i] = WB[MLP_OFF_W1 + i];
+    for (int i = threadIdx.x; i < MLP_H * MLP_H; i += blockDim.x) w2[i] = WB[MLP_OFF_W2 + i];
+    for (int i = threadIdx.x; i < MLP_KOUT * MLP_H; i += blockDim.x) w3[i] = WB[MLP_OFF_W3 + i];
+    for (int i = threadIdx.x; i < MLP_H; i += blockDim.x) {
+        bias[i] = P[MLP_OFF_B1 + i];
+        bias[MLP_H + i] = P[MLP_OFF_B2 + i];
+    }
+    if (threadIdx.x < MLP_KOUT) bias[2 * MLP_H + threadIdx.x] = P[MLP_OFF_B3 + threadIdx.x];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, kq = (lane >> 4) * 4;
+    const int64_t node = ((int64_t)blockIdx.x * 4 + wave) * 16 + col;
+    const bool valid = node < n;
+
+    // layer 1: B = X^T tile (k = input feature, column = node)
+    bf16x4 xb;
+    {
+        float xv[4] = {0.f, 0.f, 0.f, 0.f};
+        if (valid) {
+            const float4 t = *reinterpret_cast<const float4*>(X + node * MLP_KIN + kq);
+            xv[0] = t.x; xv[1] = t.y; xv[2] = t.z; xv[3] = t.w;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) xb[j] = (short)f2bf(xv[j]);
+    }
+    bf16x4 h1[MLP_H / 16];
+#pragma unroll
+    for (int ot = 0; ot < MLP_H / 16; ++ot) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4*>(&w1[(ot * 16 + col) * MLP_KIN + kq]);
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, xb, acc, 0, 0, 0);
+        float z[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            z[j] = acc[j] + bias[ot * 16 + kq + j];
+            h1[ot][j] = (short)f2bf(z[j] > 0.f ? z[j] : 0.f);
+        }
+        if (Z1 && valid) *reinterpret_cast<float4*>(Z1 + node * MLP_H + ot * 16 + kq) = make_float4(z[0], z[1], z[2], z[3]);
+    }
+    // layer 2
+    bf16x4 h2[MLP_H / 16];
+#pragma unroll
+    for (int ot = 0; ot < MLP_H / 16; ++ot) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < MLP_H / 16; ++ks) {
+            const bf16x4 a = *reinterpret_cast<const bf16x4*>(&w2[(ot * 16 + col) * MLP_H + ks * 16 + kq]);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h1[ks], acc, 0, 0, 0);
+        }
+        float z[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            z[j] = acc[j] + bias[MLP_H + ot * 16 + kq + j];
+            h2[ot][j] = (short)f2bf(z[j] > 0.f ? z[j] : 0.f);
+        }
+        if (Z2 && valid) *reinterpret_cast<float4*>(Z2 + node * MLP_H + ot * 16 + kq) = make_float4(z[0], z[1], z[2], z[3]);
+    }
+    // layer 3 (rows 0..3 of the padded 16-row output)
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < MLP_H / 16; ++ks) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4*>(&w3[col * MLP_H + ks * 16 + kq]);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h2[ks], acc, 0, 0, 0);
+    }
+    if (valid && kq == 0)
+        *reinterpret_cast<float4*>(Y + node * MLP_OUT) =
+            make_float4(acc[0] + bias[2 * MLP_H + 0], acc[1] + bias[2 * MLP_H + 1], acc[2] + bias[2 * MLP_H + 2],
+                        acc[3] + bias[2 * MLP_H + 3]);
+}
+
+// ---------------------------------------------------------------------
+// backward through the activations, one thread per node (FP32):
+//   dz2 = (W3^T dy) * [z2 > 0],  dz1 = (W2^T dz2) * [z1 > 0]
+// with the BF16-rounded weights the forward used (straight-through for the
+// activation rounding)
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+mlp_backward_act(const uint16_t* __restrict__ WB, const float* __restrict__ Z1, const float* __restrict__ Z2,
+                 const float* __restrict__ DY, int64_t n, float* __restrict__ DZ1, float* __restrict__ DZ2)
+{
+    __shared__ __align__(16) uint16_t w2[MLP_H * MLP_H];
+    __shared__ float w3[MLP_OUT * MLP_H];
+    for (int i = threadIdx.x; i < MLP_H * MLP_H; i += blockDim.x) w2[i] = WB[MLP_OFF_W2 + i];
+    for (int i = threadIdx.x; i < MLP_OUT * MLP_H; i += blockDim.x) w3[i] = bf2f(WB[MLP_OFF_W3 + i]);
+    __syncthreads();
+    const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= n) return;
+    float dy[MLP_OUT];
+#pragma unroll
+    for (int r = 0; r < MLP_OUT; ++r) dy[r] = DY[node * MLP_OUT + r];
+    float dz2[MLP_H];
+#pragma unroll
+    for (int o = 0; o < MLP_H; ++o) {
+        float s = 0.f;
+#pragma unroll
+        for (int r = 0; r < MLP_OUT; ++r) s += w3[r * MLP_H + o] * dy[r];
+        const float z = Z2[node * MLP_H + o];
+        dz2[o] = z > 0.f ? s : 0.f;
+        DZ2[node * MLP_H + o] = dz2[o];
+    }
+    for (int i = 0; i < MLP_H; ++i) {
+        float s = 0.f;
+#pragma unroll
+        for (int o = 0; o < MLP_H; ++o) s += bf2f(w2[o * MLP_H + i]) * dz2[o];
+        const float z = Z1[node * MLP_H + i];
+        DZ1[node * MLP_H + i] = z > 0.f ? s : 0.f;
+    }
+}
+
+// ---------------------------------------------------------------------
+// weight gradients: partial sums over a chunk of nodes per workgroup,
+// written to part[block][MLP_NPARAM]; reduced in block order afterwards
+// ---------------------------------------------------------------------
+constexpr int WG_SUB = 16;  // nodes staged in LDS at a time
+
+__global__ void __launch_bounds__(256)
+mlp_wgrad_partial(const float* __restrict__ X, const float* __restrict__ Z1, const float* __restrict__ Z2,
+                  const float* __restrict__ DY, const float* __restrict__ DZ1, const float* __restrict__ DZ2,
+                  int64_t n, int64_t chunk, float* __restrict__ part)
+{
+    __shared__ float sx[WG_SUB][MLP_KIN];
+    __shared__ float sh1[WG_SUB][MLP_H];
+    __shared__ float sh2[WG_SUB][MLP_H];
+    __shared__ float sdz1[WG_SUB][MLP_H];
+    __shared__ float sdz2[WG_SUB][MLP_H];
+    __shared__ float sdy[WG_SUB][MLP_KOUT];
+    constexpr int PER = (MLP_NPARAM + 255) / 256;
+    float acc[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) acc[q] = 0.f;
+    const int64_t a = (int64_t)blockIdx.x * chunk;
+    const int64_t b = a + chunk < n ? a + chunk : n;
+    for (int64_t s0 = a; s0 < b; s0 += WG_SUB) {
+        const int cnt = (int)((b - s0) < WG_SUB ? (b - s0) : WG_SUB);
+        __syncthreads();
+        for (int i = threadIdx.x; i < WG_SUB * MLP_H; i += blockDim.x) {
+            const int t = i / MLP_H, f = i % MLP_H;
+            const bool ok = t < cnt;
+            const int64_t nd = s0 + t;
+            sh1[t][f] = ok ? relu_bf(Z1[nd * MLP_H + f]) : 0.f;
+            sh2[t][f] = ok ? relu_bf(Z2[nd * MLP_H + f]) : 0.f;
+            sdz1[t][f] = ok ? DZ1[nd * MLP_H + f] : 0.f;
+            sdz2[t][f] = ok ? DZ2[nd * MLP_H + f] : 0.f;
+        }
+        for (int i = threadIdx.x; i < WG_SUB * MLP_KIN; i += blockDim.x) {
+            const int t = i / MLP_KIN, f = i % MLP_KIN;
+            sx[t][f] = t < cnt ? bf2f(f2bf(X[(s0 + t) * MLP_KIN + f])) : 0.f;
+        }
+        for (int i = threadIdx.x; i < WG_SUB * MLP_KOUT; i += blockDim.x) {
+            const int t = i / MLP_KOUT, f = i % MLP_KOUT;
+            sdy[t][f] = (t < cnt && f < MLP_OUT) ? DY[(s0 + t) * MLP_OUT + f] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int p = threadIdx.x + q * 256;
+            if (p >= MLP_NPARAM) continue;
+            float s = 0.f;
+            if (p < MLP_OFF_B1) {
+                const int o = p / MLP_KIN, k = p % MLP_KIN;
+                for (int t = 0; t < WG_SUB; ++t) s += sdz1[t][o] * sx[t][k];
+            } else if (p < MLP_OFF_W2) {
+                const int o = p - MLP_OFF_B1;
+                for (int t = 0; t < WG_SUB; ++t) s += sdz1[t][o];
+            } else if (p < MLP_OFF_B2) {
+                const int o = (p - MLP_OFF_W2) / MLP_H, i = (p - MLP_OFF_W2) % MLP_H;
+                for (int t = 0; t < WG_SUB; ++t) s += sdz2[t][o] * sh1[t][i];
+            } else if (p < MLP_OFF_W3) {
+                const int o = p - MLP_OFF_B2;
+                for (int t = 0; t < WG_SUB; ++t) s += sdz2[t][o];
+            } else if (p < MLP_OFF_B3) {
+                const int r = (p - MLP_OFF_W3) / MLP_H, i = (p - MLP_OFF_W3) % MLP_H;
+                for (int t = 0; t < WG_SUB; ++t) s += sdy[t][r] * sh2[t][i];
+            } else {
+                const int r = p - MLP_OFF_B3;
+                for (int t = 0; t < WG_SUB; ++t) s += sdy[t][r];
+            }
+            acc[q] += s;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int p = threadIdx.x + q * 256;
+        if (p < MLP_NPARAM) part[(int64_t)blockIdx.x * MLP_NPARAM + p] = acc[q];
+    }
+}
+
+__global__ void mlp_reduce(const float* __restrict__ part, int nblk, float* __restrict__ grad)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= MLP_NPARAM) return;
+    float s = 0.f;
+    for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * MLP_NPARAM + p];
+    grad[p] = s;
+}
+
+// MSE loss gradient on the 4 outputs: dy = 2 (y - t) / (4 n); loss partials
+__global__ void mlp_mse(const float* __restrict__ Y, const float* __restrict__ T, int64_t n, float* __restrict__ DY,
+                        float* __restrict__ lpart)
+{
+    __shared__ float sh[256];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.f;
+    if (i < n * MLP_OUT) {
+        const float e = Y[i] - T[i];
+        DY[i] = 2.f * e / (float)(n * MLP_OUT);
+        l = e * e;
+    }
+    sh[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lpart[blockIdx.x] = sh[0];
+}
+
+// Adam on the FP32 master copy; refreshes the BF16 copy.  Padding entries
+// (zero columns of W1, zero rows of W3 and b3) stay zero.
+__global__ void mlp_adam(float* __restrict__ P, float* __restrict__ M, float* __restrict__ V,
+                         const float* __restrict__ G, float lr, float b1, float b2, float eps, float c1, float c2,
+                         uint16_t* __restrict__ WB)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= MLP_NPARAM) return;
+    bool pad = false;
+    if (p < MLP_OFF_B1) pad = (p % MLP_KIN) >= MLP_IN;
+    else if (p >= MLP_OFF_W3 && p < MLP_OFF_B3) pad = (p - MLP_OFF_W3) / MLP_H >= MLP_OUT;
+    else if (p >= MLP_OFF_B3) pad = (p - MLP_OFF_B3) >= MLP_OUT;
+    float w = P[p];
+    if (!pad) {
+        const float g = G[p];
+        const float m = b1 * M[p] + (1.f - b1) * g;
+        const float v = b2 * V[p] + (1.f - b2) * g * g;
+        M[p] = m;
+        V[p] = v;
+        w -= lr * (m * c1) / (sqrtf(v * c2) + eps);
+    } else {
+        w = 0.f;
+    }
+    P[p] = w;
+    WB[p] = f2bf(w);
+}
+
+__global__ void mlp_pack(const float* __restrict__ P, uint16_t* __restrict__ WB)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < MLP_NPARAM) WB[p] = f2bf(P[p]);
+}
+
+// ---------------------------------------------------------------------
+// features and guess (node-major states [nv][4], FP64)
+// ---------------------------------------------------------------------
+// x[v] = ( u^n, u^n - u^{n-1}, mean over the 1-ring of u^n - u^n ) * scale,
+// padded to 16.  The 1-ring is the node's column blocks in the CSR pattern
+// (rows of component 0), the node itself excluded.
+__global__ void predict_features(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
+                                 const double* __restrict__ u, const double* __restrict__ up, int64_t nv,
+                                 const float* __restrict__ scale, float* __restrict__ X)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    double un[4], d[4], m[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        un[c] = u[v * 4 + c];
+        d[c] = un[c] - up[v * 4 + c];
+    }
+    const int64_t r0 = rowptr[v * 4], r1 = rowptr[v * 4 + 1];
+    int cnt = 0;
+    for (int64_t k = r0; k < r1; k += 4) {
+        const int64_t w = colidx[k] / 4;
+        if (w == v) continue;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) m[c] += u[w * 4 + c];
+        ++cnt;
+    }
+    const double ic = cnt > 0 ? 1.0 / cnt : 0.0;
+    float out[MLP_KIN];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        out[c] = (float)(un[c]) * scale[c];
+        out[4 + c] = (float)(d[c]) * scale[4 + c];
+        out[8 + c] = (float)(cnt > 0 ? m[c] * ic - un[c] : 0.0) * scale[8 + c];
+    }
+#pragma unroll
+    for (int c = MLP_IN; c < MLP_KIN; ++c) out[c] = 0.f;
+#pragma unroll
+    for (int q = 0; q < MLP_KIN; q += 4)
+        *reinterpret_cast<float4*>(X + v * MLP_KIN + q) = make_float4(out[q], out[q + 1], out[q + 2], out[q + 3]);
+}
+
+// guess = 2 u^n - u^{n-1} + y * out_scale (per component)
+__global__ void predict_guess(const double* __restrict__ u, const double* __restrict__ up, const float* __restrict__ Y,
+                              const float* __restrict__ oscale, int64_t nv, double* __restrict__ g)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv * 4) return;
+    const int c = (int)(t & 3);
+    g[t] = 2.0 * u[t] - up[t] + (double)Y[t] * (double)oscale[c];
+}
+
+// training target: t = ((u^{n+1} - u^n) - (u^n - u^{n-1})) / out_scale
+__global__ void predict_target(const double* __restrict__ un1, const double* __restrict__ u,
+                               const double* __restrict__ up, const float* __restrict__ oscale, int64_t nv,
+                               float* __restrict__ T)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv * 4) return;
+    const int c = (int)(t & 3);
+    T[t] = (float)((un1[t] - 2.0 * u[t] + up[t]) / (double)oscale[c]);
+}
+
+// ---------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------
+static inline unsigned nblocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+hipError_t launch_mlp_forward(const float* P, const uint16_t* WB, const float* X, int64_t n, float* Y, float* Z1,
+                              float* Z2, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    mlp_forward<<<nblocks(n, 64), 256, 0, s>>>(P, WB, X, n, Y, Z1, Z2);
+    return hipGetLastError();
+}
+
+int64_t mlp_work_floats(int64_t n) { return 2 * n * MLP_H + (int64_t)MLP_WG_BLOCKS * MLP_NPARAM + n * MLP_OUT + 1024; }
+
+hipError_t launch_mlp_backward(const uint16_t* WB, const float* X, const float* Z1, const float* Z2, const float* DY,
+                               int64_t n, float* grad, float* work, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    float* dz1 = work;
+    float* dz2 = dz1 + n * MLP_H;
+    float* part = dz2 + n * MLP_H;
+    mlp_backward_act<<<nblocks(n, 256), 256, 0, s>>>(WB, Z1, Z2, DY, n, dz1, dz2);
+    const int64_t chunk = (n + MLP_WG_BLOCKS - 1) / MLP_WG_BLOCKS;
+    mlp_wgrad_partial<<<MLP_WG_BLOCKS, 256, 0, s>>>(X, Z1, Z2, DY, dz1, dz2, n, chunk, part);
+    mlp_reduce<<<nblocks(MLP_NPARAM, 256), 256, 0, s>>>(part, MLP_WG_BLOCKS, grad);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_mse(const float* Y, const float* T, int64_t n, float* DY, float* lpart, int* nparts,
+                          hipStream_t s)
+{
+    const unsigned nb = nblocks(n * MLP_OUT, 256);
+    *nparts = (int)nb;
+    mlp_mse<<<nb, 256, 0, s>>>(Y, T, n, DY, lpart);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_adam(float* P, float* M, float* V, const float* G, int step, float lr, float b1, float b2,
+                           float eps, uint16_t* WB, hipStream_t s)
+{
+    const float c1 = 1.f / (1.f - powf(b1, (float)step)), c2 = 1.f / (1.f - powf(b2, (float)step));
+    mlp_adam<<<nblocks(MLP_NPARAM, 256), 256, 0, s>>>(P, M, V, G, lr, b1, b2, eps, c1, c2, WB);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_pack(const float* P, uint16_t* WB, hipStream_t s)
+{
+    mlp_pack<<<nblocks(MLP_NPARAM, 256), 256, 0, s>>>(P, WB);
+    return hipGetLastError();
+}
+
+hipError_t launch_predict_features(const int64_t* rowptr, const int32_t* colidx, const double* u, const double* up,
+                                   int64_t nv, const float* scale, float* X, hipStream_t s)
+{
+    predict_features<<<nblocks(nv, 256), 256, 0, s>>>(rowptr, colidx, u, up, nv, scale, X);
+    return hipGetLastError();
+}
+
+hipError_t launch_predict_guess(const double* u, const double* up, const float* Y, const float* oscale, int64_t nv,
+                                double* g, hipStream_t s)
+{
+    predict_guess<<<nblocks(nv * 4, 256), 256, 0, s>>>(u, up, Y, oscale, nv, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_predict_target(const double* un1, const double* u, const double* up, const float* oscale, int64_t nv,
+                                 float* T, hipStream_t s)
+{
+    predict_target<<<nblocks(nv * 4, 256), 256, 0, s>>>(un1, u, up, oscale, nv, T);
+    return hipGetLastError();
+}
+
+}  // namespace nsfv1

@@ -162,6 +162,9 @@ def coarsen(mesh: Mesh):
         coarse = cavity_2d(cshape[0], r["et"], lx=r["lx"])
     else:
         coarse = box_3d(cshape, r["lengths"], et=r["et"], kind=r["kind"], solid=r["solid"])
+        if "boxes" in r:
+            coarse.recipe["boxes"] = r["boxes"]
+            coarse.boundary["obstacle"] = obstacle_surface_nodes(coarse)
     fdims = [c + 1 for c in mesh.shape]
     cdims = [c + 1 for c in cshape]
     fidx = _grid_index(mesh.shape)
@@ -200,6 +203,101 @@ def coarsen(mesh: Mesh):
     np.add.at(rowptr, rows + 1, 1)
     rowptr = np.cumsum(rowptr)
     return coarse, (rowptr, cols.astype(np.int32), wts), inj
+
+
+def obstacle_boxes(n_boxes: int, seed: int, lengths=(2.0, 1.0, 1.0), grid: float = 1.0 / 6.0):
+    """Seeded axis-aligned obstacle boxes inside a channel, with faces on a
+    grid of spacing `grid` (so every multigrid level resolves them) and a
+    free inflow and outflow section.  Returns a list of (lo, hi) corners."""
+    rng = np.random.default_rng(seed)
+    boxes = []
+    L = np.asarray(lengths)
+    for _ in range(n_boxes):
+        size = rng.integers(1, 3, size=3) * grid
+        lo = np.array([rng.integers(2, int(round((L[0] - 0.5) / grid)) - 1) * grid,
+                       rng.integers(0, int(round((L[1] - size[1]) / grid)) + 1) * grid,
+                       rng.integers(0, int(round((L[2] - size[2]) / grid)) + 1) * grid])
+        boxes.append((lo, lo + size))
+    return boxes
+
+
+def solid_from_boxes(boxes):
+    def solid(x, y, z):
+        m = np.zeros(np.shape(x), dtype=bool)
+        for lo, hi in boxes:
+            m |= (x > lo[0]) & (x < hi[0]) & (y > lo[1]) & (y < hi[1]) & (z > lo[2]) & (z < hi[2])
+        return m
+    return solid
+
+
+def channel_3d(cells=(96, 48, 48), lengths=(2.0, 1.0, 1.0), et: int = TET, n_boxes: int = 2, seed: int = 0) -> Mesh:
+    """Channel [0,2]x[0,1]x[0,1] of hexahedra split into 6 Kuhn tetrahedra
+    each, with seeded obstacle boxes removed; inflow x = 0, outflow x = 2."""
+    boxes = obstacle_boxes(n_boxes, seed, lengths)
+    m = box_3d(cells, lengths, et=et, kind="channel", solid=solid_from_boxes(boxes))
+    m.boundary["obstacle"] = obstacle_surface_nodes(m)
+    m.recipe["boxes"] = boxes
+    return m
+
+
+def obstacle_surface_nodes(mesh: Mesh) -> np.ndarray:
+    """Used nodes with a removed (solid) cell among their incident grid
+    cells: the no-slip faces of the obstacles."""
+    nx, ny, nz = mesh.shape
+    lx, ly, lz = mesh.recipe["lengths"]
+    solid = mesh.recipe["solid"]
+    ck, cj, ci = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    removed = solid((ci + 0.5) * lx / nx, (cj + 0.5) * ly / ny, (ck + 0.5) * lz / nz)  # [nz, ny, nx]
+    pad = np.zeros((nz + 2, ny + 2, nx + 2), dtype=bool)
+    pad[1:-1, 1:-1, 1:-1] = removed
+    touch = np.zeros((nz + 1, ny + 1, nx + 1), dtype=bool)
+    for dk in (0, 1):
+        for dj in (0, 1):
+            for di in (0, 1):
+                touch |= pad[dk:dk + nz + 1, dj:dj + ny + 1, di:di + nx + 1]
+    used = np.zeros(mesh.nv, dtype=bool)
+    used[mesh.conn.ravel()] = True
+    return np.nonzero(touch.ravel() & used)[0]
+
+
+def channel_dirichlet(mesh: Mesh, peak: float = 1.0):
+    """Parabolic inflow and outflow u = 16 peak y(1-y) z(1-z) (fully
+    developed, peak at the centre), no-slip walls and obstacle faces,
+    every DoF of unused nodes fixed, pressure pinned at the outflow centre
+    node.  Returns (dofs, vals), sorted by DoF."""
+    nc = 4
+    x, y, z = mesh.xyz[:, 0], mesh.xyz[:, 1], mesh.xyz[:, 2]
+    prof = 16.0 * peak * y * (1 - y) * z * (1 - z)
+    vel_nodes = np.unique(np.concatenate([mesh.boundary["inflow"], mesh.boundary["outflow"], mesh.boundary["wall"],
+                                          mesh.boundary.get("obstacle", np.zeros(0, np.int64))]))
+    dofs = (vel_nodes[:, None] * nc + np.arange(3)[None, :]).ravel()
+    vals = np.zeros((len(vel_nodes), 3))
+    io = np.isin(vel_nodes, np.concatenate([mesh.boundary["inflow"], mesh.boundary["outflow"]]))
+    wall = np.isin(vel_nodes, np.concatenate([mesh.boundary["wall"], mesh.boundary.get("obstacle", np.zeros(0, np.int64))]))
+    vals[io & ~wall, 0] = prof[vel_nodes[io & ~wall]]
+    vals = vals.ravel()
+    out = mesh.boundary["outflow"]
+    centre = out[np.argmin((y[out] - 0.5) ** 2 + (z[out] - 0.5) ** 2)]
+    dofs = np.append(dofs, centre * nc + 3)
+    vals = np.append(vals, 0.0)
+    unused = mesh.boundary.get("unused", np.zeros(0, np.int64))
+    if len(unused):
+        ex = (np.asarray(unused)[:, None] * nc + np.arange(nc)[None, :]).ravel()
+        dofs = np.append(dofs, ex)
+        vals = np.append(vals, np.zeros(len(ex)))
+    o = np.argsort(dofs)
+    return dofs[o].astype(np.int64), vals[o]
+
+
+def channel_initial_state(mesh: Mesh, dirichlet, peak: float = 1.0) -> np.ndarray:
+    """Parabolic profile in the whole channel, zero in and on obstacles,
+    Dirichlet values imposed; pressure zero."""
+    y, z = mesh.xyz[:, 1], mesh.xyz[:, 2]
+    u = np.zeros((mesh.nv, 4))
+    u[:, 0] = 16.0 * peak * y * (1 - y) * z * (1 - z)
+    dofs, vals = dirichlet
+    u.reshape(-1)[dofs] = vals
+    return u
 
 
 def random_state(mesh: Mesh, seed: int = 1, scale: float = 1.0) -> np.ndarray:
