@@ -113,6 +113,39 @@ def learned_leg(steps: int):
     return r
 
 
+def batched_leg(steps: int = 20, count: int = 256):
+    """256 cavity instances (side ~ U{32..192}, Re log-uniform [10, 2000],
+    lid ~ U[0.5, 1.5], one seed) assembled in one launch: autotuned launch
+    shape, assembly + Dirichlet rows captured in a HIP graph and replayed."""
+    import torch
+
+    from plugin_navierstokes_amd.batch import BatchedCavities, autotune, draw_instances
+
+    t0 = time.perf_counter()
+    b = BatchedCavities(draw_instances(count, seed=0))
+    u = b.state(seed=1)
+    print(f"bench: batch of {count} cavities, {b.elements} quads, setup {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    best, table = autotune(b.disc, lambda: b.step(u), reps=3)
+    b.capture(u)
+    for _ in range(3):
+        b.replay()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(steps):
+        b.replay()
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / steps
+    return {"workload": f"{count} 2D cavities in one launch (side U{{32..192}}, Re log-U[10,2000], lid U[0.5,1.5], "
+                        f"seed 0), full upwind + FIELDS, FP32 J + FP64 defect, assembly + Dirichlet rows per HIP-graph "
+                        f"replay",
+            "elements": int(b.elements), "ms_per_step": ms, "elem_per_s": b.elements / (ms * 1e-3),
+            "launch": {"threads": best[0], "lds_bytes": best[1]},
+            "autotune": [{"threads": t, "lds": l, "ms": m} for t, l, m in table]}
+
+
 def measured_traffic(n: int):
     """HBM bytes per assembly launch from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE passes scaled by a plain-copy calibration);
@@ -137,6 +170,7 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timestep-steps", type=int, default=4, help="timed BDF2 steps of the time-stepping leg (0: skip)")
+    ap.add_argument("--no-batched", dest="batched", action="store_false", help="skip the batched-instances leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (rocprof PMC byte calibration)")
     args = ap.parse_args()
@@ -229,6 +263,8 @@ def main():
         del disc
         torch.cuda.empty_cache()
         out["timestep"]["channel"] = learned_leg(args.timestep_steps)
+    if args.batched:
+        out["batched"] = batched_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(out), flush=True)

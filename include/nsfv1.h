@@ -156,6 +156,17 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
                    double s_a, double s_m, double s_f, int parts,
                    void* jvals, int jprec, double* defect, void* stream);
 
+/* Batched instances: the handle's mesh is the disjoint union of ninst
+   independent problems (e.g. cavities of different sizes); instance i owns
+   elements [elem_offset[i], elem_offset[i+1]) and has kinematic viscosity
+   nu[i], which replaces set_kinematic_viscosity for its elements.  One
+   nsfv1_assemble call then assembles every instance in one launch.  Host
+   arrays (copied); ninst = 0 returns to the single viscosity.  Per-instance
+   Dirichlet values go through nsfv1_set_dirichlet / nsfv1_apply_dirichlet
+   on the union.  No counterpart in the plugin (one problem per element
+   disc). */
+int nsfv1_batch_set_instances(nsfv1_handle* h, int64_t ninst, const int64_t* elem_offset, const double* nu);
+
 /* Skewed-upwind decisions (the corner nearest the ray's cut point,
    upwind.cpp:381-430, or the zero-velocity guard) of later assemblies:
    NSFV1_UPWIND_FRESH (default) decides anew on every assembly;

@@ -64,8 +64,9 @@ struct DevBuf {
 struct Plan {
     bool valid = false;
     int nwg = 0;
+    int block = 256;
     size_t lds = 0;
-    DevBuf wg_node, wg_rounds;
+    DevBuf wg_node, wg_rounds, perm;
 };
 
 }  // namespace
@@ -107,7 +108,8 @@ struct nsfv1_handle {
     // mesh
     int et = -1;
     int64_t nv = 0, ne = 0, ndof = 0, nnz = 0, ninc = 0;
-    DevBuf xyz, conn, rowptr, colidx, errflag, l2inv, vol, node_inc, inc_elem, inc_node, inc_info;
+    DevBuf xyz, conn, rowptr, colidx, errflag, l2inv, vol, affine, node_inc, inc_elem, inc_node, inc_info, inc_pos;
+    std::vector<int32_t> h_inc_info;
     std::vector<int64_t> h_rowptr, h_node_inc;
     int max_threads = 256;
     size_t lds_cap = 64 * 1024;
@@ -126,6 +128,9 @@ struct nsfv1_handle {
     DevBuf mg_r0, mg_dense;
     bool geo_valid = false;
     int geo_difflen = -1;
+    // batched instances: per-element viscosity
+    DevBuf nu_elem;
+    int64_t ninst = 0;
     // recorded skewed-upwind decisions
     DevBuf upw_dec;
     int upw_mode = 0;
@@ -403,9 +408,13 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
             for (int a = 0; a < nc; ++a) rowptr[v * nc + a + 1] = rowptr[v * nc + a] + (int64_t)deg[v] * nc;
     }
     const int64_t nnz = rowptr[nv * nc];
+    for (int64_t v = 0; v < nv; ++v)
+        if (deg[v] > 255) return fail(NSFV1_ERR_ARGUMENT, "a node has more than 255 neighbours");
     std::vector<int32_t> colidx;
+    std::vector<uint64_t> ipos;
     try {
         colidx.resize(nnz);
+        ipos.resize(ninc);
     } catch (...) {
         return fail(NSFV1_ERR_ARGUMENT, "pattern too large for host memory");
     }
@@ -419,6 +428,15 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
             if (nb.empty()) nb.push_back((int32_t)v);
             std::sort(nb.begin(), nb.end());
             nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+            for (int64_t q = cnt[v]; q < cnt[v + 1]; ++q) {
+                uint64_t pk = 0;
+                for (int k = 0; k < nsh; ++k) {
+                    const int32_t w = conn[(int64_t)ie[q] * nsh + k];
+                    const uint64_t slot = (uint64_t)(std::lower_bound(nb.begin(), nb.end(), w) - nb.begin());
+                    pk |= slot << (8 * k);
+                }
+                ipos[q] = pk;
+            }
             for (int a = 0; a < nc; ++a) {
                 int32_t* out = colidx.data() + rowptr[v * nc + a];
                 for (size_t t = 0; t < nb.size(); ++t)
@@ -436,10 +454,10 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
               up(h->rowptr, rowptr.data(), sizeof(int64_t) * rowptr.size()) &&
               up(h->colidx, colidx.data(), sizeof(int32_t) * nnz) &&
               up(h->inc_elem, ie.data(), sizeof(int32_t) * ninc) && up(h->inc_node, in.data(), sizeof(int32_t) * ninc) &&
-              up(h->inc_info, ii.data(), sizeof(int32_t) * ninc) &&
+              up(h->inc_info, ii.data(), sizeof(int32_t) * ninc) && up(h->inc_pos, ipos.data(), sizeof(uint64_t) * ninc) &&
               up(h->node_inc, ni32.data(), sizeof(int32_t) * ni32.size()) &&
               h->l2inv.alloc(sizeof(double) * ne * (et == ET_HEX ? 12 : et == ET_TET ? 6 : et == ET_QUAD ? 4 : 3)) == hipSuccess &&
-              h->vol.alloc(sizeof(double) * ne * nsh) == hipSuccess;
+              h->vol.alloc(sizeof(double) * ne * nsh) == hipSuccess && h->affine.alloc((size_t)ne) == hipSuccess;
     if (!ok) return fail(NSFV1_ERR_DEVICE, std::string("mesh upload: ") + hipGetErrorString(hipGetLastError()));
     h->et = et;
     h->nv = nv;
@@ -449,6 +467,7 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
     h->ninc = ninc;
     h->h_rowptr.swap(rowptr);
     h->h_node_inc.swap(cnt);
+    h->h_inc_info.swap(ii);
     h->plan32.valid = h->plan64.valid = false;
     h->geo_valid = false;
     // solver state sized for the previous mesh
@@ -458,6 +477,8 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
     h->mg_ready = false;
     h->upw_dec.release();
     h->upw_recorded = false;
+    h->nu_elem.release();
+    h->ninst = 0;
     return NSFV1_OK;
 }
 
@@ -540,11 +561,23 @@ static int make_plan(nsfv1_handle* h, int jbytes, Plan& P)
     }
     P.nwg = (int)wr.size();
     P.lds = maxlds;
+    P.block = h->max_threads;
+    // lane order inside each workgroup: by incidence slot, then node, so a
+    // wavefront takes part in as few accumulation rounds as possible
+    std::vector<int32_t> perm((size_t)h->ninc);
+    for (int w = 0; w < P.nwg; ++w) {
+        const int64_t i0 = h->h_node_inc[wn[w]], i1 = h->h_node_inc[wn[w + 1]];
+        std::vector<std::pair<int32_t, int32_t>> key;
+        key.reserve((size_t)(i1 - i0));
+        for (int64_t q = i0; q < i1; ++q) key.push_back({h->h_inc_info[q] >> 8, (int32_t)q});
+        std::stable_sort(key.begin(), key.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+        for (int64_t t = 0; t < i1 - i0; ++t) perm[i0 + t] = key[t].second;
+    }
     auto up = [](DevBuf& b, const std::vector<int32_t>& x) {
         return b.alloc(sizeof(int32_t) * std::max<size_t>(x.size(), 1)) == hipSuccess &&
                (x.empty() || hipMemcpy(b.p, x.data(), sizeof(int32_t) * x.size(), hipMemcpyHostToDevice) == hipSuccess);
     };
-    if (!up(P.wg_node, wn) || !up(P.wg_rounds, wr)) return fail(NSFV1_ERR_DEVICE, "plan upload");
+    if (!up(P.wg_node, wn) || !up(P.wg_rounds, wr) || !up(P.perm, perm)) return fail(NSFV1_ERR_DEVICE, "plan upload");
     P.valid = true;
     return NSFV1_OK;
 }
@@ -606,8 +639,8 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
     k.s_f = s_f;
     if (!h->geo_valid || h->geo_difflen != h->difflen) {
         hipError_t ge = launch_geometry_cache(h->et, h->xyz.as<double>(), h->conn.as<int32_t>(), h->ne, h->difflen,
-                                              h->l2inv.as<double>(), h->vol.as<double>(), h->errflag.as<int>(),
-                                              (hipStream_t)stream);
+                                              h->l2inv.as<double>(), h->vol.as<double>(), h->affine.as<uint8_t>(),
+                                              h->errflag.as<int>(), (hipStream_t)stream);
         if (ge != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("geometry cache: ") + hipGetErrorString(ge));
         h->geo_valid = true;
         h->geo_difflen = h->difflen;
@@ -622,11 +655,38 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
     KMeshPtrs m{h->xyz.as<double>(), h->conn.as<int32_t>(), h->rowptr.as<int64_t>(), h->colidx.as<int32_t>(),
                 h->node_inc.as<int32_t>(), h->inc_elem.as<int32_t>(), h->inc_node.as<int32_t>(),
                 h->inc_info.as<int32_t>(), P.wg_node.as<int32_t>(), P.wg_rounds.as<int32_t>(), h->l2inv.as<double>(),
-                h->vol.as<double>(), h->upw_dec.as<uint8_t>()};
+                h->vol.as<double>(), h->upw_dec.as<uint8_t>(), h->affine.as<uint8_t>(), h->inc_pos.as<uint64_t>(),
+                P.perm.as<int32_t>(), h->nu_elem.as<double>(), P.block};
     hipError_t e = launch_assemble(h->et, jprec, m, k, u, u_old, jvals, defect, h->errflag.as<int>(), P.nwg, P.lds,
                                    (hipStream_t)stream);
     if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("assemble launch: ") + hipGetErrorString(e));
     if (h->upw_mode == NSFV1_UPWIND_RECORD && (parts & NSFV1_PART_A)) h->upw_recorded = true;
+    return NSFV1_OK;
+}
+
+int nsfv1_batch_set_instances(nsfv1_handle* h, int64_t ninst, const int64_t* elem_offset, const double* nu)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    if (ninst == 0) {
+        h->nu_elem.release();
+        h->ninst = 0;
+        return NSFV1_OK;
+    }
+    if (ninst < 0 || !elem_offset || !nu) return fail(NSFV1_ERR_ARGUMENT, "bad instance arrays");
+    if (elem_offset[0] != 0 || elem_offset[ninst] != h->ne)
+        return fail(NSFV1_ERR_ARGUMENT, "instance element ranges must cover the mesh: offset[0] = 0, offset[n] = ne");
+    std::vector<double> ne_nu((size_t)h->ne);
+    for (int64_t i = 0; i < ninst; ++i) {
+        if (elem_offset[i + 1] < elem_offset[i]) return fail(NSFV1_ERR_ARGUMENT, "instance element ranges not monotone");
+        if (!(nu[i] > 0.0)) return fail(NSFV1_ERR_ARGUMENT, "instance viscosity must be positive");
+        for (int64_t e = elem_offset[i]; e < elem_offset[i + 1]; ++e) ne_nu[e] = nu[i];
+    }
+    if (h->nu_elem.alloc(sizeof(double) * h->ne) != hipSuccess ||
+        hipMemcpy(h->nu_elem.p, ne_nu.data(), sizeof(double) * h->ne, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "instance viscosity upload");
+    h->ninst = ninst;
+    h->nu_set = true;
     return NSFV1_OK;
 }
 

@@ -279,6 +279,151 @@ __device__ inline void side_loop(const double (&co)[Topo<ET>::nsh][Topo<ET>::dim
     }
 }
 
+
+
+// reference-element facts used by ray_affine (constant-folded after unrolling)
+template <int ET>
+constexpr int side_axis(int S)  // box elements: the reference axis normal to side S
+{
+    using T = Topo<ET>;
+    int r = 0;
+    for (int a = 0; a < T::dim; ++a) {
+        bool same = true;
+        for (int j = 1; j < T::side_n[S]; ++j)
+            if (T::lc[T::side[S][j]][a] != T::lc[T::side[S][0]][a]) same = false;
+        if (same) r = a;
+    }
+    return r;
+}
+template <int ET>
+constexpr bool side_high(int S)
+{
+    return Topo<ET>::lc[Topo<ET>::side[S][0]][side_axis<ET>(S)] > 0.5;
+}
+template <int ET>
+constexpr int side_opposite(int S)  // simplices: the corner not on side S
+{
+    for (int c = 0; c < Topo<ET>::nsh; ++c)
+        if (!side_has<ET>(S, c)) return c;
+    return 0;
+}
+
+// Ray cast of an affine element (every simplex; parallelogram quads and
+// parallelepiped hexahedra per the geometry cache) in reference
+// coordinates: xi(s) = xi_ip + s * J^-1 (-v), s >= 0 (upwind).  The side
+// whose plane the ray leaves through is found by one division per
+// candidate side instead of a ray/triangle test per side triangle; among
+// sides that pass the inside test (tolerance NSFV1_RAY_EPS) the first in
+// reference order wins, as in the general test.  Leaves side < 0 when no
+// side passes (the caller then runs the general test).
+template <int ET>
+__device__ inline void ray_affine(const double (&co)[Topo<ET>::nsh][Topo<ET>::dim], const double* ipg,
+                                  const double* sv, int& side, double* gi, double* li)
+{
+    using T = Topo<ET>;
+    constexpr int D = T::dim;
+    constexpr bool box = (ET == 1 || ET == 3);
+    constexpr int a1 = 1, a2 = box ? 3 : 2, a3 = box ? 4 : 3;
+    double Jm[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        Jm[i][0] = co[a1][i] - co[0][i];
+        Jm[i][1] = co[a2][i] - co[0][i];
+        if constexpr (D == 3) Jm[i][2] = co[a3][i] - co[0][i];
+    }
+    double K[D][D];
+    if constexpr (D == 2) {
+        const double det = Jm[0][0] * Jm[1][1] - Jm[0][1] * Jm[1][0];
+        if (det == 0.0) return;
+        const double id = 1.0 / det;
+        K[0][0] = Jm[1][1] * id; K[0][1] = -Jm[0][1] * id;
+        K[1][0] = -Jm[1][0] * id; K[1][1] = Jm[0][0] * id;
+    } else {
+        const double c00 = Jm[1][1] * Jm[2][2] - Jm[1][2] * Jm[2][1];
+        const double c01 = Jm[1][2] * Jm[2][0] - Jm[1][0] * Jm[2][2];
+        const double c02 = Jm[1][0] * Jm[2][1] - Jm[1][1] * Jm[2][0];
+        const double det = Jm[0][0] * c00 + Jm[0][1] * c01 + Jm[0][2] * c02;
+        if (det == 0.0) return;
+        const double id = 1.0 / det;
+        K[0][0] = c00 * id;
+        K[1][0] = c01 * id;
+        K[2][0] = c02 * id;
+        K[0][1] = (Jm[0][2] * Jm[2][1] - Jm[0][1] * Jm[2][2]) * id;
+        K[1][1] = (Jm[0][0] * Jm[2][2] - Jm[0][2] * Jm[2][0]) * id;
+        K[2][1] = (Jm[0][1] * Jm[2][0] - Jm[0][0] * Jm[2][1]) * id;
+        K[0][2] = (Jm[0][1] * Jm[1][2] - Jm[0][2] * Jm[1][1]) * id;
+        K[1][2] = (Jm[0][2] * Jm[1][0] - Jm[0][0] * Jm[1][2]) * id;
+        K[2][2] = (Jm[0][0] * Jm[1][1] - Jm[0][1] * Jm[1][0]) * id;
+    }
+    double xi[D], dh[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        double x = 0.0, d = 0.0;
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            x += K[a][i] * (ipg[i] - co[0][i]);
+            d -= K[a][i] * sv[i];
+        }
+        xi[a] = x;
+        dh[a] = d;
+    }
+    constexpr double eps = NSFV1_RAY_EPS;
+    double xs[D];
+    bool found = false;
+#pragma unroll
+    for (int S = 0; S < T::nside; ++S) {
+        if (found) continue;
+        if constexpr (box) {
+            // side S lies on the plane xi_ax = hi (reference corner coordinates)
+            const int ax = side_axis<ET>(S);
+            const bool hi = side_high<ET>(S);
+            const bool toward = hi ? dh[ax] > 0.0 : dh[ax] < 0.0;
+            if (!toward) continue;
+            const double sp = (hi ? 1.0 - xi[ax] : -xi[ax]) / dh[ax];
+            bool in = true;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                const double p = a == ax ? (hi ? 1.0 : 0.0) : xi[a] + sp * dh[a];
+                xs[a] = p;
+                if (a != ax) in &= (p >= -eps) && (p <= 1.0 + eps);
+            }
+            if (in) { side = S; found = true; }
+        } else {
+            // side S is opposite the corner it does not contain
+            const int opp = side_opposite<ET>(S);
+            double lam[D + 1], dl[D + 1];
+            lam[0] = 1.0;
+            dl[0] = 0.0;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                lam[0] -= xi[a];
+                dl[0] -= dh[a];
+                lam[a + 1] = xi[a];
+                dl[a + 1] = dh[a];
+            }
+            if (!(dl[opp] < 0.0)) continue;
+            const double sp = -lam[opp] / dl[opp];
+            bool in = true;
+#pragma unroll
+            for (int c = 0; c <= D; ++c)
+                if (c != opp) in &= (lam[c] + sp * dl[c] >= -eps);
+#pragma unroll
+            for (int a = 0; a < D; ++a) xs[a] = (a + 1 == opp) ? 0.0 : xi[a] + sp * dh[a];
+            if (in) { side = S; found = true; }
+        }
+    }
+    if (!found) return;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        double x = co[0][i];
+#pragma unroll
+        for (int a = 0; a < D; ++a) x += Jm[i][a] * xs[a];
+        gi[i] = x;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) li[a] = a < D ? xs[a] : 0.0;
+}
+
 // ---------------------------------------------------------------------
 // per-mesh static geometry: SCV volumes and the diffusion-length factor
 // 1/L^2 of every SCVF (diffusion_length.h:48-199 -- geometry only)
@@ -300,7 +445,8 @@ __device__ inline void cache_ips(const double (&co)[Topo<ET>::nsh][Topo<ET>::dim
 template <int ET>
 __global__ void __launch_bounds__(256)
 geometry_cache(const double* __restrict__ xyz, const int32_t* __restrict__ conn, int64_t ne, int difflen,
-               double* __restrict__ l2inv, double* __restrict__ volc, int* __restrict__ err)
+               double* __restrict__ l2inv, double* __restrict__ volc, uint8_t* __restrict__ affine,
+               int* __restrict__ err)
 {
     using T = Topo<ET>;
     constexpr int D = T::dim, N = T::nsh, NIP = T::nip;
@@ -317,6 +463,28 @@ geometry_cache(const double* __restrict__ xyz, const int32_t* __restrict__ conn,
     scv_volumes<ET>(co, vol);
 #pragma unroll
     for (int k = 0; k < N; ++k) volc[e * N + k] = vol[k];
+    {
+        // parallelogram / parallelepiped test (simplices are always affine)
+        bool aff = true;
+        if constexpr (ET == 1 || ET == 3) {
+            double scale = 0.0, dev = 0.0;
+#pragma unroll
+            for (int k = 1; k < N; ++k)
+#pragma unroll
+                for (int i = 0; i < D; ++i) scale = fmax(scale, fabs(co[k][i] - co[0][i]));
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                dev = fmax(dev, fabs(co[1][i] + co[3][i] - co[0][i] - co[2][i]));
+                if constexpr (ET == 3) {
+                    dev = fmax(dev, fabs(co[1][i] + co[4][i] - co[0][i] - co[5][i]));
+                    dev = fmax(dev, fabs(co[3][i] + co[4][i] - co[0][i] - co[7][i]));
+                    dev = fmax(dev, fabs(co[1][i] + co[3][i] + co[4][i] - 2.0 * co[0][i] - co[6][i]));
+                }
+            }
+            aff = dev <= 1e-12 * scale;
+        }
+        affine[e] = aff ? 1 : 0;
+    }
     double c[D];
     elem_centre<ET>(co, c);
     double nn[NIP], ds[NIP];
@@ -383,6 +551,7 @@ struct CanonH {
 template <int FAST>
 struct Cfg {
     const KParams& P;
+    double nu;  // kinematic viscosity of the lane's element (batched instances carry their own)
     __device__ bool stokes() const { return FAST >= 0 ? false : P.stokes != 0; }
     __device__ bool pac() const { return FAST >= 0 ? false : P.pac != 0; }
     __device__ bool peclet() const { return FAST >= 0 ? false : P.peclet != 0; }
@@ -416,7 +585,7 @@ __device__ inline bool upwind_canon(int type, bool kvfrom, const double (&co)[To
                                     const double* __restrict__ xyz, const int32_t* __restrict__ econn, const int* pk,
                                     const double* ipg, const double* n, const double* sv,
                                     double (&U)[Topo<ET>::nsh], double& convlen, bool& have_cut, int& side,
-                                    double* gi, double* li, uint8_t* decp, int mode)
+                                    double* gi, double* li, uint8_t* decp, int mode, bool affine)
 {
     using T = Topo<ET>;
     constexpr int D = T::dim, N = T::nsh;
@@ -478,7 +647,8 @@ __device__ inline bool upwind_canon(int type, bool kvfrom, const double (&co)[To
     }
     if (!have_cut) {
         side = -1;
-        side_loop<ET, 0>(coo, ipg, sv, side, gi, li);
+        if (affine) ray_affine<ET>(coo, ipg, sv, side, gi, li);
+        if (side < 0) side_loop<ET, 0>(coo, ipg, sv, side, gi, li);
         have_cut = true;
     }
     if (side < 0) return false;
@@ -548,14 +718,14 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
                                   const int32_t* ccn, const double* c, const double* __restrict__ uold,
                                   double l2, bool kvfrom, bool wantJ, Lane<ET, JT>& L, Rec<ET, JT>& R,
                                   bool& okg, bool& oku, bool& have_cut, int& side, double* gi, double* li,
-                                  uint8_t* decp)
+                                  uint8_t* decp, bool affine)
 {
     using T = Topo<ET>;
     constexpr int D = T::dim, N = T::nsh;
     constexpr int IP = corner_ip<ET>(0, Q);
     constexpr int OT = T::edge[IP][0] == 0 ? T::edge[IP][1] : T::edge[IP][0];
     const KParams& P = C.P;
-    const double nu = P.nu, rho = P.rho;
+    const double nu = C.nu, rho = P.rho;
 
     double ipg[D], n[D], me[D];
     {
@@ -615,7 +785,7 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
         double Us[N];
 #pragma unroll
         for (int k = 0; k < N; ++k) Us[k] = 0.0;
-        if (!C.stokes()) oku &= upwind_canon<ET, Q>(C.stab_up(), kvfrom, co, xyz, econn, pk, ipg, n, sv, Us, Ls, have_cut, side, gi, li, decp, P.upw_mode);
+        if (!C.stokes()) oku &= upwind_canon<ET, Q>(C.stab_up(), kvfrom, co, xyz, econn, pk, ipg, n, sv, Us, Ls, have_cut, side, gi, li, decp, P.upw_mode, affine);
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             double acc = 0;
@@ -628,7 +798,7 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
         for (int k = 0; k < N; ++k) { R.Us[k] = JT(Us[k]); R.Uc[k] = JT(Us[k]); }
         if (!C.stokes() && C.two_upwinds()) {
             double Uc[N], Lc;
-            oku &= upwind_canon<ET, Q>(C.conv_up(), kvfrom, co, xyz, econn, pk, ipg, n, sv, Uc, Lc, have_cut, side, gi, li, decp, P.upw_mode);
+            oku &= upwind_canon<ET, Q>(C.conv_up(), kvfrom, co, xyz, econn, pk, ipg, n, sv, Uc, Lc, have_cut, side, gi, li, decp, P.upw_mode, affine);
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 double acc = 0;
@@ -726,7 +896,7 @@ __device__ inline void scvf_jac(const Cfg<FAST>& C, const Rec<ET, JT>& R, Lane<E
     constexpr int IP = corner_ip<ET>(0, Q);
     const KParams& P = C.P;
     const bool fields = C.fields();
-    const JT s_ = JT(P.s_a), rh = JT(P.rho), nr = JT(-1.0 * P.nu * P.rho), ex = JT(P.exact * P.rho);
+    const JT s_ = JT(P.s_a), rh = JT(P.rho), nr = JT(-1.0 * C.nu * P.rho), ex = JT(P.exact * P.rho);
 #pragma unroll
     for (int sh = 0; sh < N; ++sh) {
         JT g[D];
@@ -810,7 +980,7 @@ __device__ inline void scvf_all(const Cfg<FAST>& C, const double (&co)[Topo<ET>:
                                 const double (&ul)[Topo<ET>::nsh][Topo<ET>::dim + 1], const int* pk,
                                 const int32_t* ccn, const double* c, const double* uold, const double* l2row,
                                 uint8_t* decrow, int kv, bool wantJ, Lane<ET, JT>& L, Rec<ET, JT>* R, bool& okg,
-                                bool& oku)
+                                bool& oku, bool affine)
 {
     if constexpr (Q < Topo<ET>::cip) {
         bool have_cut = false;
@@ -820,8 +990,9 @@ __device__ inline void scvf_all(const Cfg<FAST>& C, const double (&co)[Topo<ET>:
         const double l2 = l2row[oip];
         const bool kvfrom = CanonH<ET>::c.from[kv][Q] != 0;
         scvf_canon<ET, FAST, Q, JT>(C, co, xyz, econn, ul, pk, ccn, c, uold, l2, kvfrom, wantJ, L, R[Q], okg, oku, have_cut,
-                                    side, gi, li, decrow ? decrow + oip : nullptr);
-        scvf_all<ET, FAST, Q + 1, JT>(C, co, xyz, econn, ul, pk, ccn, c, uold, l2row, decrow, kv, wantJ, L, R, okg, oku);
+                                    side, gi, li, decrow ? decrow + oip : nullptr, affine);
+        scvf_all<ET, FAST, Q + 1, JT>(C, co, xyz, econn, ul, pk, ccn, c, uold, l2row, decrow, kv, wantJ, L, R, okg, oku,
+                                      affine);
     }
 }
 
@@ -847,9 +1018,14 @@ assemble_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const double
     using T = Topo<ET>;
     constexpr int D = T::dim, N = T::nsh, NC = D + 1, NIP = T::nip;
     extern __shared__ __align__(16) unsigned char smem[];
-    const Cfg<FAST> C{P};
 
-    const int w = blockIdx.x;
+    // consecutive workgroups (neighbouring node ranges, shared element data)
+    // go to the same XCD: hardware hands block i to XCD i % 8
+    const int w = [] {
+        const int nwg = (int)gridDim.x, i = (int)blockIdx.x;
+        const int per = nwg >> 3, rem = nwg & 7, x = i & 7, k = i >> 3;
+        return x * per + (x < rem ? x : rem) + k;
+    }();
     const int v0 = M.wg_node[w], v1 = M.wg_node[w + 1];
     const int64_t jbase = M.rowptr[(int64_t)v0 * NC];
     const int jlen = (int)(M.rowptr[(int64_t)v1 * NC] - jbase);
@@ -862,8 +1038,8 @@ assemble_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const double
     for (int i = threadIdx.x; i < dlen; i += blockDim.x) dbuf[i] = 0.0;
 
     const int i0 = M.node_inc[v0], i1 = M.node_inc[v1];
-    const int inc = i0 + (int)threadIdx.x;
-    const bool active = inc < i1;
+    const int lane_inc = i0 + (int)threadIdx.x;
+    const bool active = lane_inc < i1;
 
     Lane<ET, JT> L;
 #pragma unroll
@@ -879,6 +1055,7 @@ assemble_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const double
 #pragma unroll
     for (int k = 0; k < N; ++k) pos[k] = 0;
     if (active) {
+        const int inc = M.perm[lane_inc];
         const int e = M.inc_elem[inc];
         v = M.inc_node[inc];
         const int info = M.inc_info[inc];
@@ -901,9 +1078,10 @@ assemble_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const double
             double c[D];
             elem_centre<ET>(co, c);
             Rec<ET, JT> R[T::cip];
+            const Cfg<FAST> C{P, M.nu_elem ? M.nu_elem[e] : P.nu};
             scvf_all<ET, FAST, 0, JT>(C, co, M.xyz, M.conn + (int64_t)e * N, ul, pk, ccn, c, uold, M.l2inv + (int64_t)e * NIP,
                                       (P.upw_mode != 0 && M.upw_dec) ? M.upw_dec + (int64_t)e * NIP : nullptr, kv, wantJ, L, R,
-                                      okg, oku);
+                                      okg, oku, (ET == 0 || ET == 2) || (M.affine != nullptr && M.affine[e] != 0));
             if (wantJ) jac_all<ET, FAST, 0, JT>(C, R, L);
         }
         // mass part and source: navier_stokes_fv1.cpp:781-869
@@ -923,20 +1101,9 @@ assemble_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const double
         }
         if (!okg || !oku) err[0] = (oku ? 0 : KERR_CUT_SIDE) | (okg ? 0 : KERR_DEGENERATE);
         // column slots of the (canonically ordered) corners in v's block row
-        const int64_t rs = M.rowptr[(int64_t)v * NC];
-        const int deg = (int)((M.rowptr[(int64_t)v * NC + 1] - rs) / NC);
+        const uint64_t pw = M.inc_pos[inc];
 #pragma unroll
-        for (int k = 0; k < N; ++k) {
-            const int32_t key = ccn[k] * NC;
-            int lo = 0, hi = deg - 1, found = 0;
-            while (lo <= hi) {
-                const int mid = (lo + hi) >> 1;
-                const int32_t cc = M.colidx[rs + (int64_t)mid * NC];
-                if (cc == key) { found = mid; break; }
-                if (cc < key) lo = mid + 1; else hi = mid - 1;
-            }
-            pos[k] = found * NC;
-        }
+        for (int k = 0; k < N; ++k) pos[k] = (int)((pw >> (8 * pk[k])) & 0xffu) * NC;
     }
     int rbase = 0, rlen = 0;
     if (active) {
@@ -980,11 +1147,11 @@ static hipError_t launch_cfg(int fast, const KMeshPtrs& m, const KParams& p, con
                              JT* J, double* d, int* err, int nwg, size_t lds, hipStream_t s)
 {
     switch (fast) {
-    case UPW_NO: assemble_rows<ET, JT, UPW_NO><<<nwg, 256, lds, s>>>(m, p, u, uold, J, d, err); break;
-    case UPW_FULL: assemble_rows<ET, JT, UPW_FULL><<<nwg, 256, lds, s>>>(m, p, u, uold, J, d, err); break;
-    case UPW_SKEWED: assemble_rows<ET, JT, UPW_SKEWED><<<nwg, 256, lds, s>>>(m, p, u, uold, J, d, err); break;
-    case UPW_LPS: assemble_rows<ET, JT, UPW_LPS><<<nwg, 256, lds, s>>>(m, p, u, uold, J, d, err); break;
-    default: assemble_rows<ET, JT, -1><<<nwg, 256, lds, s>>>(m, p, u, uold, J, d, err); break;
+    case UPW_NO: assemble_rows<ET, JT, UPW_NO><<<nwg, m.block, lds, s>>>(m, p, u, uold, J, d, err); break;
+    case UPW_FULL: assemble_rows<ET, JT, UPW_FULL><<<nwg, m.block, lds, s>>>(m, p, u, uold, J, d, err); break;
+    case UPW_SKEWED: assemble_rows<ET, JT, UPW_SKEWED><<<nwg, m.block, lds, s>>>(m, p, u, uold, J, d, err); break;
+    case UPW_LPS: assemble_rows<ET, JT, UPW_LPS><<<nwg, m.block, lds, s>>>(m, p, u, uold, J, d, err); break;
+    default: assemble_rows<ET, JT, -1><<<nwg, m.block, lds, s>>>(m, p, u, uold, J, d, err); break;
     }
     return hipGetLastError();
 }
@@ -1009,7 +1176,7 @@ static hipError_t launch_et(int jprec, const KMeshPtrs& m, const KParams& p, con
 hipError_t launch_assemble(int et, int jprec, const KMeshPtrs& m, const KParams& p, const double* u,
                            const double* uold, void* J, double* d, int* err, int nwg, size_t lds, hipStream_t s)
 {
-    if (lds > 65536) return hipErrorInvalidValue;
+    if (lds > 65536 || m.block < 64 || m.block > 256 || m.block % 64) return hipErrorInvalidValue;
     switch (et) {
     case ET_TRI:  return launch_et<ET_TRI>(jprec, m, p, u, uold, J, d, err, nwg, lds, s);
     case ET_QUAD: return launch_et<ET_QUAD>(jprec, m, p, u, uold, J, d, err, nwg, lds, s);
@@ -1020,14 +1187,14 @@ hipError_t launch_assemble(int et, int jprec, const KMeshPtrs& m, const KParams&
 }
 
 hipError_t launch_geometry_cache(int et, const double* xyz, const int32_t* conn, int64_t ne, int difflen,
-                                 double* l2inv, double* vol, int* err, hipStream_t s)
+                                 double* l2inv, double* vol, uint8_t* affine, int* err, hipStream_t s)
 {
     const int blocks = (int)((ne + 255) / 256);
     switch (et) {
-    case ET_TRI:  geometry_cache<ET_TRI><<<blocks, 256, 0, s>>>(xyz, conn, ne, difflen, l2inv, vol, err); break;
-    case ET_QUAD: geometry_cache<ET_QUAD><<<blocks, 256, 0, s>>>(xyz, conn, ne, difflen, l2inv, vol, err); break;
-    case ET_TET:  geometry_cache<ET_TET><<<blocks, 256, 0, s>>>(xyz, conn, ne, difflen, l2inv, vol, err); break;
-    case ET_HEX:  geometry_cache<ET_HEX><<<blocks, 256, 0, s>>>(xyz, conn, ne, difflen, l2inv, vol, err); break;
+    case ET_TRI:  geometry_cache<ET_TRI><<<blocks, 256, 0, s>>>(xyz, conn, ne, difflen, l2inv, vol, affine, err); break;
+    case ET_QUAD: geometry_cache<ET_QUAD><<<blocks, 256, 0, s>>>(xyz, conn, ne, difflen, l2inv, vol, affine, err); break;
+    case ET_TET:  geometry_cache<ET_TET><<<blocks, 256, 0, s>>>(xyz, conn, ne, difflen, l2inv, vol, affine, err); break;
+    case ET_HEX:  geometry_cache<ET_HEX><<<blocks, 256, 0, s>>>(xyz, conn, ne, difflen, l2inv, vol, affine, err); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

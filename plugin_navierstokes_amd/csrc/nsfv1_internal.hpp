@@ -24,6 +24,11 @@ struct KMeshPtrs {
     const double* l2inv;      // [ne][nip] diffusion-length factor per SCVF
     const double* vol;        // [ne][nsh] SCV volumes
     uint8_t* upw_dec;         // [ne][nip] recorded skewed-upwind corner (255: zero velocity), or null
+    const uint8_t* affine;    // [ne] 1 = parallelogram / parallelepiped (quadrilaterals, hexahedra)
+    const uint64_t* inc_pos;  // [ninc] byte c = column block of corner c in the node's block row
+    const int32_t* perm;      // [ninc] lane order of the plan: incidences of a workgroup sorted by slot
+    const double* nu_elem;    // [ne] per-element kinematic viscosity (batched instances), or null
+    int block;                // threads per workgroup of the launch (64..256)
 };
 
 hipError_t launch_assemble(int et, int jprec, const KMeshPtrs& m, const KParams& p, const double* u,
@@ -31,6 +36,6 @@ hipError_t launch_assemble(int et, int jprec, const KMeshPtrs& m, const KParams&
                            hipStream_t stream);
 
 hipError_t launch_geometry_cache(int et, const double* xyz, const int32_t* conn, int64_t ne, int difflen,
-                                 double* l2inv, double* vol, int* err, hipStream_t stream);
+                                 double* l2inv, double* vol, uint8_t* affine, int* err, hipStream_t stream);
 
 }  // namespace nsfv1

@@ -196,3 +196,31 @@ def test_upwind_record_replay(cuda, et):
     J4, d4 = disc.assemble(-u, -uold, 0.01, jprec=F64)
     assert not torch.equal(d3, d4)
     disc.sync_errors()
+
+
+def _affine_mesh(et, shear):
+    """Unjittered structured mesh, optionally mapped by a fixed linear map:
+    every quadrilateral / hexahedron is a parallelogram / parallelepiped, so
+    the kernel's affine ray cast runs (simplices always take it)."""
+    mesh = M.cavity_2d(5, et) if et in (M.TRI, M.QUAD) else M.cavity_3d(3, et)
+    if shear:
+        A = np.array([[1.0, 0.3], [-0.1, 1.2]]) if mesh.dim == 2 else \
+            np.array([[1.0, 0.3, 0.1], [0.0, 1.1, -0.2], [0.05, 0.0, 0.9]])
+        mesh.xyz = mesh.xyz @ A.T
+    return mesh
+
+
+@pytest.mark.parametrize("shear", [False, True], ids=["unit", "sheared"])
+@pytest.mark.parametrize("up", ["skewed", "lps"])
+@pytest.mark.parametrize("et", ETS, ids=lambda e: ET_NAMES[e])
+def test_affine_elements_ray_cast(cuda, et, up, shear):
+    print(f"case et={ET_NAMES[et]} up={up} shear={shear}", flush=True)
+    mesh = _affine_mesh(et, shear)
+    for jprec in (F64, F32):
+        J, d, Jr, dr, Ja, da, _ = _run(cuda, et, {"upwind": up}, mesh=mesh, seed=11, time_dep=True,
+                                       parts=PART_A | PART_M, s=(1.0, 2.0, 0.0), jprec=jprec)
+        if jprec == F64:
+            assert fp64_violation(J, Jr, Ja) <= 1.0, worst(J, Jr, Ja)
+        else:
+            assert fp32_violation(J, Jr, Ja) <= 1.0, worst(J, Jr, Ja, FP32_REL, FP32_ABS)
+        assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
