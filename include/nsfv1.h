@@ -277,6 +277,13 @@ int nsfv1_predict_guess(nsfv1_handle* h, const double* u, const double* u_prev, 
 int nsfv1_predict_target(nsfv1_handle* h, const double* u_next, const double* u, const double* u_prev,
                          const float* out_scale, float* t, void* stream);
 
+/* Assembly path: 0 (default) evaluates every SCVF once into a per-SCVF
+   record (FP32 Jacobian, common configuration: FIELDS, one upwind object,
+   no Stokes/PAC/Peclet/exact terms) and gathers rows from the records; 1
+   always runs the fused kernel (each SCVF evaluated by both of its rows).
+   Both are deterministic.  No counterpart in the plugin. */
+int nsfv1_set_assembly_path(nsfv1_handle* h, int path);
+
 /* Launch shape of the assembly: incidences (threads) per workgroup (64..256,
    a multiple of 64) and the LDS budget per workgroup in bytes (4..64 KiB);
    0 keeps the current value.  Used by the autotuner. */

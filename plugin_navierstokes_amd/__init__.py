@@ -44,6 +44,7 @@ EXPORTS = [
     "nsfv1_set_preconditioner", "nsfv1_mg_set_hierarchy", "nsfv1_mg_setup", "nsfv1_set_upwind_decisions",
     "nsfv1_mlp_sizes", "nsfv1_mlp_pack", "nsfv1_mlp_forward", "nsfv1_mlp_mse", "nsfv1_mlp_backward", "nsfv1_mlp_adam",
     "nsfv1_predict_features", "nsfv1_predict_guess", "nsfv1_predict_target", "nsfv1_batch_set_instances",
+    "nsfv1_set_assembly_path",
 ]
 UPWIND_FRESH, UPWIND_RECORD, UPWIND_REPLAY = 0, 1, 2
 
@@ -99,6 +100,7 @@ def load_library(path: str | None = None):
     L.nsfv1_set_preconditioner.argtypes = [H, C.c_char_p]
     L.nsfv1_set_upwind_decisions.argtypes = [H, C.c_int]
     L.nsfv1_batch_set_instances.argtypes = [H, C.c_int64, vp, vp]
+    L.nsfv1_set_assembly_path.argtypes = [H, C.c_int]
     L.nsfv1_mlp_sizes.argtypes = [C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.nsfv1_mlp_pack.argtypes = [vp, vp, vp]
     L.nsfv1_mlp_forward.argtypes = [vp, vp, vp, C.c_int64, vp, vp, vp, vp]
@@ -282,6 +284,9 @@ class NavierStokesFV1:
         _check(_lib.nsfv1_linear_solve(self._h, _ptr(J), int(jprec), _ptr(b), _ptr(x), float(rtol), int(maxit),
                                        int(restart), C.byref(it), C.byref(rr), _stream_ptr(stream)))
         return it.value, rr.value
+
+    def set_assembly_path(self, path: int):
+        _check(_lib.nsfv1_set_assembly_path(self._h, int(path)))
 
     def batch_set_instances(self, elem_offset, nu):
         off = np.ascontiguousarray(elem_offset, dtype=np.int64)

@@ -128,6 +128,9 @@ struct nsfv1_handle {
     DevBuf mg_r0, mg_dense;
     bool geo_valid = false;
     int geo_difflen = -1;
+    // split assembly path (per-SCVF records)
+    DevBuf recs;
+    int path = 0;  // 0: split where supported, 1: fused kernel only
     // batched instances: per-element viscosity
     DevBuf nu_elem;
     int64_t ninst = 0;
@@ -479,6 +482,7 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
     h->upw_recorded = false;
     h->nu_elem.release();
     h->ninst = 0;
+    h->recs.release();
     return NSFV1_OK;
 }
 
@@ -657,8 +661,16 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
                 h->inc_info.as<int32_t>(), P.wg_node.as<int32_t>(), P.wg_rounds.as<int32_t>(), h->l2inv.as<double>(),
                 h->vol.as<double>(), h->upw_dec.as<uint8_t>(), h->affine.as<uint8_t>(), h->inc_pos.as<uint64_t>(),
                 P.perm.as<int32_t>(), h->nu_elem.as<double>(), P.block};
-    hipError_t e = launch_assemble(h->et, jprec, m, k, u, u_old, jvals, defect, h->errflag.as<int>(), P.nwg, P.lds,
-                                   (hipStream_t)stream);
+    hipError_t e;
+    if (h->path == 0 && jprec == NSFV1_F32 && jvals && (parts & NSFV1_PART_A) && split_supported(h->et, k)) {
+        const size_t need = sizeof(float) * (size_t)split_record_floats(h->et, h->ne);
+        if (h->recs.n < need && h->recs.alloc(need) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "record buffer");
+        e = launch_assemble_split(h->et, m, k, u, u_old, static_cast<float*>(jvals), defect, h->errflag.as<int>(), P.nwg,
+                                  P.lds, h->ne, h->recs.as<float>(), (hipStream_t)stream);
+    } else {
+        e = launch_assemble(h->et, jprec, m, k, u, u_old, jvals, defect, h->errflag.as<int>(), P.nwg, P.lds,
+                            (hipStream_t)stream);
+    }
     if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("assemble launch: ") + hipGetErrorString(e));
     if (h->upw_mode == NSFV1_UPWIND_RECORD && (parts & NSFV1_PART_A)) h->upw_recorded = true;
     return NSFV1_OK;
@@ -687,6 +699,14 @@ int nsfv1_batch_set_instances(nsfv1_handle* h, int64_t ninst, const int64_t* ele
         return fail(NSFV1_ERR_DEVICE, "instance viscosity upload");
     h->ninst = ninst;
     h->nu_set = true;
+    return NSFV1_OK;
+}
+
+int nsfv1_set_assembly_path(nsfv1_handle* h, int path)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (path != 0 && path != 1) return fail(NSFV1_ERR_ARGUMENT, "path must be 0 (split where supported) or 1 (fused)");
+    h->path = path;
     return NSFV1_OK;
 }
 

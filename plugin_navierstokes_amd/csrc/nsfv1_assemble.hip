@@ -523,6 +523,9 @@ struct Canon {
     int perm[8][8];   // canonical corner k' -> original corner
     int oip[8][3];    // original SCVF of the q-th canonical SCVF at corner 0
     int from[8][3];   // 1 if kv is the `from` corner of that SCVF
+    int A[8][3][3];   // original reference coordinates = A * canonical ones + corner offset
+    int Ainv[8][3][3];
+    int kq[12];       // canonical index of SCVF ip in the frame of its `from` corner
 };
 
 template <int ET>
@@ -535,6 +538,46 @@ constexpr Canon<ET> make_canon()
             c.oip[kv][q] = canon_orig_ip<ET>(kv, q);
             c.from[kv][q] = canon_is_from<ET>(kv, q) ? 1 : 0;
         }
+        // reference-coordinate map of the frame: columns are the images of
+        // the canonical unit corners
+        constexpr bool box = (ET == 1 || ET == 3);
+        constexpr int D = Topo<ET>::dim;
+        const int o = canon_perm<ET>(kv, 0);
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) c.A[kv][i][j] = 0;
+        for (int j = 0; j < D; ++j) {
+            const int kj = box ? (j == 0 ? 1 : (j == 1 ? 3 : 4)) : j + 1;
+            const int t = canon_perm<ET>(kv, kj);
+            for (int i = 0; i < D; ++i) c.A[kv][i][j] = (int)Topo<ET>::lc[t][i] - (int)Topo<ET>::lc[o][i];
+        }
+        int (&a)[3][3] = c.A[kv];
+        int (&b)[3][3] = c.Ainv[kv];
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) b[i][j] = 0;
+        if (D == 2) {
+            const int det = a[0][0] * a[1][1] - a[0][1] * a[1][0];
+            b[0][0] = a[1][1] * det; b[0][1] = -a[0][1] * det;
+            b[1][0] = -a[1][0] * det; b[1][1] = a[0][0] * det;
+        } else {
+            const int det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) -
+                            a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                            a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);  // +-1
+            b[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) * det;
+            b[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) * det;
+            b[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) * det;
+            b[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) * det;
+            b[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) * det;
+            b[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) * det;
+            b[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) * det;
+            b[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) * det;
+            b[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) * det;
+        }
+    }
+    for (int ip = 0; ip < 12; ++ip) c.kq[ip] = 0;
+    for (int ip = 0; ip < Topo<ET>::nip; ++ip) {
+        const int kv = Topo<ET>::edge[ip][0];
+        for (int q = 0; q < Topo<ET>::cip; ++q)
+            if (c.oip[kv][q] == ip) c.kq[ip] = q;
     }
     return c;
 }
@@ -889,7 +932,8 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
 // Jacobian row contributions of canonical SCVF Q from its record:
 // navier_stokes_fv1.cpp:317-593
 template <int ET, int FAST, int Q, typename JT>
-__device__ inline void scvf_jac(const Cfg<FAST>& C, const Rec<ET, JT>& R, Lane<ET, JT>& L)
+__device__ inline void scvf_jac(const Cfg<FAST>& C, const Rec<ET, JT>& R, Lane<ET, JT>& L,
+                                const JT (*G)[Topo<ET>::dim] = nullptr)
 {
     using T = Topo<ET>;
     constexpr int D = T::dim, N = T::nsh;
@@ -902,10 +946,14 @@ __device__ inline void scvf_jac(const Cfg<FAST>& C, const Rec<ET, JT>& R, Lane<E
         JT g[D];
 #pragma unroll
         for (int i = 0; i < D; ++i) {
-            JT acc = 0;
+            if (G) {
+                g[i] = G[sh][i];
+            } else {
+                JT acc = 0;
 #pragma unroll
-            for (int j = 0; j < D; ++j) acc += R.Ji[j][i] * JT(TabH<ET>::t.gl[IP][sh][j]);
-            g[i] = acc;
+                for (int j = 0; j < D; ++j) acc += R.Ji[j][i] * JT(TabH<ET>::t.gl[IP][sh][j]);
+                g[i] = acc;
+            }
         }
         const JT ph = JT(TabH<ET>::t.phi[IP][sh]);
         const JT Sv = fields ? R.vv * ph + R.vc * R.Us[sh] : ph;
@@ -1139,6 +1187,389 @@ assemble_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const double
         for (int i = threadIdx.x; i < dlen; i += blockDim.x) dout[(int64_t)v0 * NC + i] = dbuf[i];
 }
 
+
+// ---------------------------------------------------------------------
+// Split path (FP32 Jacobian, common configuration): every SCVF is
+// evaluated once.
+//   scvf_records: one lane per (SCVF, element), in the frame of the SCVF's
+//     `from` corner; writes the FP64 flux (added to `from`, subtracted
+//     from `to`) and the FP32 record of its Jacobian block, expressed in
+//     the original corner order and reference frame (128 B per SCVF).
+//   assemble_rows_split: the row-owner gather of assemble_rows, reading the
+//     three records of each (node, element) pair instead of recomputing
+//     them; the `to` row takes the negated block (every term of the block is
+//     linear in the normal or in the flux weight).
+// ---------------------------------------------------------------------
+constexpr int REC_FLOATS = 32;
+// record layout: box elements keep the inverse reference Jacobian (their
+// frame changes are axis reflections, exact in FP32); simplices keep the
+// physical shape gradients (a frame change would round)
+template <int ET>
+struct RecLayout {
+    static constexpr bool box = (ET == 1 || ET == 3);
+    static constexpr int G = 0;               // Ji[D][D] (box) or grad[N][D] (simplex), original frame/corners
+    static constexpr int Nrm = box ? 9 : 12;  // normal, from -> to
+    static constexpr int S = Nrm + 3;         // pw, vv, vc, sp
+    static constexpr int U = S + 4;           // upwind shapes, original corners
+    static constexpr int F = 24;              // flux (4 x FP64)
+    static_assert(U + Topo<ET>::nsh <= F, "record layout");
+};
+
+template <int ET, int FAST, int Q>
+__device__ inline void record_one(const Cfg<FAST>& C, const double (&co)[Topo<ET>::nsh][Topo<ET>::dim],
+                                  const double* __restrict__ xyz, const int32_t* __restrict__ econn,
+                                  const double (&ul)[Topo<ET>::nsh][Topo<ET>::dim + 1], const int* pk,
+                                  const int32_t* ccn, const double* c, const double* __restrict__ uold, double l2,
+                                  int kv, float* __restrict__ rec, bool& okg, bool& oku, uint8_t* decp, bool affine)
+{
+    using T = Topo<ET>;
+    constexpr int D = T::dim, N = T::nsh;
+    Lane<ET, float> L;
+#pragma unroll
+    for (int a = 0; a <= D; ++a) L.dacc[a] = 0.0;
+    Rec<ET, float> R;
+    bool have_cut = false;
+    int side = -1;
+    double gi[3] = {0, 0, 0}, li[3] = {0, 0, 0};
+    scvf_canon<ET, FAST, Q, float>(C, co, xyz, econn, ul, pk, ccn, c, uold, l2, true, true, L, R, okg, oku, have_cut,
+                                   side, gi, li, decp, affine);
+    using RL = RecLayout<ET>;
+    float rv[RL::F];
+#pragma unroll
+    for (int i = 0; i < RL::F; ++i) rv[i] = 0.f;
+    if constexpr (RL::box) {
+        // canonical -> original reference frame: Ji_o = A Ji_c (A = diag(+-1))
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                float acc = 0.f;
+#pragma unroll
+                for (int m = 0; m < D; ++m) acc += float(CanonH<ET>::c.A[kv][i][m]) * R.Ji[m][j];
+                rv[RL::G + i * D + j] = acc;
+            }
+    } else {
+        // physical gradients of every corner, computed once in the original
+        // frame in FP64 (constant on a simplex): J = sum_k x_k (x) grad_ref
+        // phi_k, cofactor inverse, grad phi_c = J^-T grad_ref phi_c
+        double J[D][D], Ki[D][D];
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    const double glk = k == 0 ? -1.0 : (k == j + 1 ? 1.0 : 0.0);
+                    acc += xyz[(int64_t)econn[k] * D + i] * glk;
+                }
+                J[i][j] = acc;
+            }
+        if constexpr (D == 2) {
+            const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+            Ki[0][0] = J[1][1] / det;  Ki[0][1] = -J[0][1] / det;
+            Ki[1][0] = -J[1][0] / det; Ki[1][1] = J[0][0] / det;
+        } else {
+            const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                               J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                               J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+            Ki[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+            Ki[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+            Ki[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+            Ki[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+            Ki[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+            Ki[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+            Ki[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+            Ki[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+            Ki[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+        }
+#pragma unroll
+        for (int c = 0; c < N; ++c)
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                double acc = 0.0;
+#pragma unroll
+                for (int j = 0; j < D; ++j) acc += Ki[j][i] * (c == 0 ? -1.0 : (c == j + 1 ? 1.0 : 0.0));
+                rv[RL::G + c * D + i] = float(acc);
+            }
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) rv[RL::Nrm + i] = R.n[i];
+    rv[RL::S + 0] = R.pw;
+    rv[RL::S + 1] = R.vv;
+    rv[RL::S + 2] = R.vc;
+    rv[RL::S + 3] = R.sp;
+    if constexpr (FAST == UPW_FULL || FAST == UPW_SKEWED) {
+        // one-hot shapes: the original number of the upwind corner (-1: none)
+        int idx = -1;
+#pragma unroll
+        for (int k = 0; k < N; ++k) idx = R.Us[k] != 0.f ? pk[k] : idx;
+        rv[RL::U] = __int_as_float(idx);
+    } else if constexpr (FAST == UPW_LPS) {
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+            float x = 0.f;
+#pragma unroll
+            for (int k = 0; k < N; ++k) x = pk[k] == c ? R.Us[k] : x;
+            rv[RL::U + c] = x;
+        }
+    }
+    float4* r4 = reinterpret_cast<float4*>(rec);
+#pragma unroll
+    for (int q = 0; q < RL::F / 4; ++q) r4[q] = make_float4(rv[4 * q], rv[4 * q + 1], rv[4 * q + 2], rv[4 * q + 3]);
+    double2* f2 = reinterpret_cast<double2*>(rec + RL::F);
+    f2[0] = make_double2(L.dacc[0], L.dacc[1]);
+    f2[1] = make_double2(L.dacc[2], D == 3 ? L.dacc[3] : 0.0);
+}
+
+template <int ET, int Q>
+struct IpsOfQ {
+    static constexpr int count = [] {
+        int n = 0;
+        for (int ip = 0; ip < Topo<ET>::nip; ++ip)
+            if (CanonH<ET>::c.kq[ip] == Q) ++n;
+        return n;
+    }();
+    static constexpr auto list = [] {
+        struct L { int ip[12]; } l{};
+        int n = 0;
+        for (int ip = 0; ip < Topo<ET>::nip; ++ip)
+            if (CanonH<ET>::c.kq[ip] == Q) l.ip[n++] = ip;
+        return l;
+    }();
+};
+
+template <int ET, int FAST, int Q>
+__global__ void __launch_bounds__(256)
+scvf_records(KMeshPtrs M, KParams P, const double* __restrict__ u, const double* __restrict__ uold, int64_t ne,
+             int64_t ne_pad, float* __restrict__ recs, int* __restrict__ err)
+{
+    using T = Topo<ET>;
+    constexpr int D = T::dim, N = T::nsh, NC = D + 1, NIP = T::nip;
+    const int64_t L = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int idx = (int)(L / ne_pad);  // uniform across a wavefront (ne_pad % 64 == 0)
+    const int64_t e = L - (int64_t)idx * ne_pad;
+    if (idx >= IpsOfQ<ET, Q>::count || e >= ne) return;
+    const int ip = IpsOfQ<ET, Q>::list.ip[idx];
+    const int kv = T::edge[ip][0];
+    int pk[N];
+    int32_t ccn[N];
+    double co[N][D], ul[N][NC];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        pk[k] = CanonH<ET>::c.perm[kv][k];
+        ccn[k] = M.conn[e * N + pk[k]];
+#pragma unroll
+        for (int i = 0; i < D; ++i) co[k][i] = M.xyz[(int64_t)ccn[k] * D + i];
+#pragma unroll
+        for (int a = 0; a < NC; ++a) ul[k][a] = u[(int64_t)ccn[k] * NC + a];
+    }
+    double c[D];
+    elem_centre<ET>(co, c);
+    const Cfg<FAST> C{P, M.nu_elem ? M.nu_elem[e] : P.nu};
+    const double l2 = M.l2inv[e * NIP + ip];
+    uint8_t* decp = (P.upw_mode != 0 && M.upw_dec) ? M.upw_dec + e * NIP + ip : nullptr;
+    const bool affine = (ET == 0 || ET == 2) || (M.affine != nullptr && M.affine[e] != 0);
+    float* rec = recs + ((int64_t)ip * ne_pad + e) * REC_FLOATS;
+    bool okg = true, oku = true;
+    record_one<ET, FAST, Q>(C, co, M.xyz, M.conn + e * N, ul, pk, ccn, c, uold, l2, kv, rec, okg, oku, decp, affine);
+    if (!okg || !oku) err[0] = (oku ? 0 : KERR_CUT_SIDE) | (okg ? 0 : KERR_DEGENERATE);
+}
+
+template <int ET, int FAST, int Q>
+__device__ inline void gather_jac(const Cfg<FAST>& C, const float* __restrict__ recs, int64_t ne_pad, int64_t e,
+                                  int kv, const int* pk, Lane<ET, float>& L, bool wantJ)
+{
+    using T = Topo<ET>;
+    if constexpr (Q < T::cip) {
+        constexpr int D = T::dim, N = T::nsh;
+        const int ip = CanonH<ET>::c.oip[kv][Q];
+        const float sg = CanonH<ET>::c.from[kv][Q] ? 1.f : -1.f;
+        const float* rec = recs + ((int64_t)ip * ne_pad + e) * REC_FLOATS;
+        using RL = RecLayout<ET>;
+        const double2* f2 = reinterpret_cast<const double2*>(rec + RL::F);
+        const double2 fa = f2[0], fb = f2[1];
+        L.dacc[0] += double(sg) * fa.x;
+        L.dacc[1] += double(sg) * fa.y;
+        L.dacc[2] += double(sg) * fb.x;
+        if constexpr (D == 3) L.dacc[3] += double(sg) * fb.y;
+        if (wantJ) {
+            float rv[RL::F];
+            const float4* r4 = reinterpret_cast<const float4*>(rec);
+#pragma unroll
+            for (int q = 0; q < RL::F / 4; ++q) {
+                const float4 t = r4[q];
+                rv[4 * q] = t.x;
+                rv[4 * q + 1] = t.y;
+                rv[4 * q + 2] = t.z;
+                rv[4 * q + 3] = t.w;
+            }
+            Rec<ET, float> R;
+            float G[N][D];
+            if constexpr (RL::box) {
+                // original -> this lane's canonical frame: Ji_c = A^-1 Ji_o
+#pragma unroll
+                for (int i = 0; i < D; ++i)
+#pragma unroll
+                    for (int j = 0; j < D; ++j) {
+                        float acc = 0.f;
+#pragma unroll
+                        for (int m = 0; m < D; ++m) acc += float(CanonH<ET>::c.Ainv[kv][i][m]) * rv[RL::G + m * D + j];
+                        R.Ji[i][j] = acc;
+                    }
+            } else {
+#pragma unroll
+                for (int k = 0; k < N; ++k)
+#pragma unroll
+                    for (int i = 0; i < D; ++i) {
+                        float x = 0.f;
+#pragma unroll
+                        for (int c = 0; c < N; ++c) x = pk[k] == c ? rv[RL::G + c * D + i] : x;
+                        G[k][i] = x;
+                    }
+            }
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                R.n[i] = sg * rv[RL::Nrm + i];
+                R.up[i] = 0.f;
+            }
+            R.pw = sg * rv[RL::S + 0];
+            R.pw1 = 0.f;
+            R.vv = rv[RL::S + 1];
+            R.vc = rv[RL::S + 2];
+            R.sp = rv[RL::S + 3];
+            R.w = 1.f;
+            if constexpr (FAST == UPW_FULL || FAST == UPW_SKEWED) {
+                const int idx = __float_as_int(rv[RL::U]);
+#pragma unroll
+                for (int k = 0; k < N; ++k) R.Us[k] = pk[k] == idx ? 1.f : 0.f;
+            } else if constexpr (FAST == UPW_LPS) {
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    float x = 0.f;
+#pragma unroll
+                    for (int c = 0; c < N; ++c) x = pk[k] == c ? rv[RL::U + c] : x;
+                    R.Us[k] = x;
+                }
+            } else {
+                constexpr int IP = corner_ip<ET>(0, Q);
+#pragma unroll
+                for (int k = 0; k < N; ++k) R.Us[k] = float(TabH<ET>::t.phi[IP][k]);
+            }
+#pragma unroll
+            for (int k = 0; k < N; ++k) R.Uc[k] = R.Us[k];
+            if constexpr (RL::box) scvf_jac<ET, FAST, Q, float>(C, R, L);
+            else scvf_jac<ET, FAST, Q, float>(C, R, L, G);
+        }
+        gather_jac<ET, FAST, Q + 1>(C, recs, ne_pad, e, kv, pk, L, wantJ);
+    }
+}
+
+template <int ET, int FAST>
+__global__ void __launch_bounds__(256)
+assemble_rows_split(KMeshPtrs M, KParams P, const double* __restrict__ u, const float* __restrict__ recs,
+                    int64_t ne_pad, float* __restrict__ J, double* __restrict__ dout)
+{
+    using T = Topo<ET>;
+    constexpr int D = T::dim, N = T::nsh, NC = D + 1;
+    using JT = float;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int w = [] {
+        const int nwg = (int)gridDim.x, i = (int)blockIdx.x;
+        const int per = nwg >> 3, rem = nwg & 7, x = i & 7, k = i >> 3;
+        return x * per + (x < rem ? x : rem) + k;
+    }();
+    const int v0 = M.wg_node[w], v1 = M.wg_node[w + 1];
+    const int64_t jbase = M.rowptr[(int64_t)v0 * NC];
+    const int jlen = (int)(M.rowptr[(int64_t)v1 * NC] - jbase);
+    const int dlen = (v1 - v0) * NC;
+    JT* jbuf = reinterpret_cast<JT*>(smem);
+    double* dbuf = reinterpret_cast<double*>(smem + ((jlen * (int)sizeof(JT) + 15) & ~15));
+    const bool wantJ = J != nullptr;
+    if (wantJ)
+        for (int i = threadIdx.x; i < jlen; i += blockDim.x) jbuf[i] = JT(0);
+    for (int i = threadIdx.x; i < dlen; i += blockDim.x) dbuf[i] = 0.0;
+
+    const int i0 = M.node_inc[v0], i1 = M.node_inc[v1];
+    const int lane_inc = i0 + (int)threadIdx.x;
+    const bool active = lane_inc < i1;
+
+    Lane<ET, JT> L;
+#pragma unroll
+    for (int a = 0; a < NC; ++a) {
+        L.dacc[a] = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+#pragma unroll
+            for (int b = 0; b < NC; ++b) L.jacc[a][k][b] = JT(0);
+    }
+    int v = v0, slot = -1;
+    int pos[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) pos[k] = 0;
+    if (active) {
+        const int inc = M.perm[lane_inc];
+        const int e = M.inc_elem[inc];
+        v = M.inc_node[inc];
+        const int info = M.inc_info[inc];
+        const int kv = info & 0xff;
+        slot = info >> 8;
+        int pk[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) pk[k] = CanonH<ET>::c.perm[kv][k];
+        if (P.parts & 1) {
+            const Cfg<FAST> C{P, M.nu_elem ? M.nu_elem[e] : P.nu};
+            gather_jac<ET, FAST, 0>(C, recs, ne_pad, e, kv, pk, L, wantJ);
+        }
+        if (P.parts & 6) {
+            const double volk = M.vol[(int64_t)e * N + kv];
+            if (P.parts & 2) {
+                const double m = volk * P.rho;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    if (wantJ) L.jacc[d][0][d] += JT(P.s_m * m);
+                    L.dacc[d] += P.s_m * (u[(int64_t)v * NC + d] * m);
+                }
+            }
+            if ((P.parts & 4) && P.has_source)
+#pragma unroll
+                for (int d = 0; d < D; ++d) L.dacc[d] += P.s_f * (P.f[d] * volk * P.rho);
+        }
+        const uint64_t pw = M.inc_pos[inc];
+#pragma unroll
+        for (int k = 0; k < N; ++k) pos[k] = (int)((pw >> (8 * pk[k])) & 0xffu) * NC;
+    }
+    int rbase = 0, rlen = 0;
+    if (active) {
+        const int64_t rs = M.rowptr[(int64_t)v * NC];
+        rbase = (int)(rs - jbase);
+        rlen = (int)(M.rowptr[(int64_t)v * NC + 1] - rs);
+    }
+    __syncthreads();
+    const int rounds = M.wg_rounds[w];
+    for (int r = 0; r < rounds; ++r) {
+        if (active && slot == r) {
+            if (wantJ) {
+#pragma unroll
+                for (int a = 0; a < NC; ++a) {
+                    const int ra = rbase + a * rlen;
+#pragma unroll
+                    for (int k = 0; k < N; ++k)
+#pragma unroll
+                        for (int b = 0; b < NC; ++b) jbuf[ra + pos[k] + b] += L.jacc[a][k][b];
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < NC; ++a) dbuf[(v - v0) * NC + a] += L.dacc[a];
+        }
+        __syncthreads();
+    }
+    if (wantJ)
+        for (int i = threadIdx.x; i < jlen; i += blockDim.x) J[jbase + i] = jbuf[i];
+    if (dout)
+        for (int i = threadIdx.x; i < dlen; i += blockDim.x) dout[(int64_t)v0 * NC + i] = dbuf[i];
+}
+
 // ---------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------
@@ -1182,6 +1613,68 @@ hipError_t launch_assemble(int et, int jprec, const KMeshPtrs& m, const KParams&
     case ET_QUAD: return launch_et<ET_QUAD>(jprec, m, p, u, uold, J, d, err, nwg, lds, s);
     case ET_TET:  return launch_et<ET_TET>(jprec, m, p, u, uold, J, d, err, nwg, lds, s);
     case ET_HEX:  return launch_et<ET_HEX>(jprec, m, p, u, uold, J, d, err, nwg, lds, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+
+// split path: records then row gather (FP32 Jacobian, common configuration)
+template <int ET, int FAST>
+static hipError_t launch_split_cfg(const KMeshPtrs& m, const KParams& p, const double* u, const double* uold,
+                                   float* J, double* d, int* err, int nwg, size_t lds, int64_t ne, float* recs,
+                                   hipStream_t s)
+{
+    const int64_t ne_pad = (ne + 63) / 64 * 64;
+    if (p.parts & 1) {
+        // one launch per frame index Q that has SCVFs (triangles use Q = 0 only)
+        if constexpr (IpsOfQ<ET, 0>::count > 0)
+            scvf_records<ET, FAST, 0><<<(unsigned)((IpsOfQ<ET, 0>::count * ne_pad + 255) / 256), 256, 0, s>>>(
+                m, p, u, uold, ne, ne_pad, recs, err);
+        if constexpr (IpsOfQ<ET, 1>::count > 0)
+            scvf_records<ET, FAST, 1><<<(unsigned)((IpsOfQ<ET, 1>::count * ne_pad + 255) / 256), 256, 0, s>>>(
+                m, p, u, uold, ne, ne_pad, recs, err);
+        if constexpr (Topo<ET>::cip > 2 && IpsOfQ<ET, 2>::count > 0)
+            scvf_records<ET, FAST, 2><<<(unsigned)((IpsOfQ<ET, 2>::count * ne_pad + 255) / 256), 256, 0, s>>>(
+                m, p, u, uold, ne, ne_pad, recs, err);
+    }
+    assemble_rows_split<ET, FAST><<<nwg, m.block, lds, s>>>(m, p, u, recs, ne_pad, J, d);
+    return hipGetLastError();
+}
+
+template <int ET>
+static hipError_t launch_split_et(const KMeshPtrs& m, const KParams& p, const double* u, const double* uold, float* J,
+                                  double* d, int* err, int nwg, size_t lds, int64_t ne, float* recs, hipStream_t s)
+{
+    switch (fast_path(p)) {
+    case UPW_NO: return launch_split_cfg<ET, UPW_NO>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, s);
+    case UPW_FULL: return launch_split_cfg<ET, UPW_FULL>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, s);
+    case UPW_SKEWED: return launch_split_cfg<ET, UPW_SKEWED>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, s);
+    case UPW_LPS: return launch_split_cfg<ET, UPW_LPS>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+// Tetrahedra stay on the fused kernel: a record built in the `from` corner's
+// frame rounds some exactly-zero normal/gradient products of axis-aligned
+// tetrahedra differently from a row's own frame (O(1e-19) where the
+// reference arithmetic gives an exact 0).
+bool split_supported(int et, const KParams& p) { return et != ET_TET && fast_path(p) >= 0; }
+
+int64_t split_record_floats(int et, int64_t ne)
+{
+    const int nip = et == ET_HEX ? 12 : et == ET_TET ? 6 : et == ET_QUAD ? 4 : 3;
+    return (int64_t)nip * ((ne + 63) / 64 * 64) * REC_FLOATS;
+}
+
+hipError_t launch_assemble_split(int et, const KMeshPtrs& m, const KParams& p, const double* u, const double* uold,
+                                 float* J, double* d, int* err, int nwg, size_t lds, int64_t ne, float* recs,
+                                 hipStream_t s)
+{
+    if (lds > 65536 || m.block < 64 || m.block > 256 || m.block % 64 || !split_supported(et, p)) return hipErrorInvalidValue;
+    switch (et) {
+    case ET_TRI:  return launch_split_et<ET_TRI>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, s);
+    case ET_QUAD: return launch_split_et<ET_QUAD>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, s);
+    case ET_HEX:  return launch_split_et<ET_HEX>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, s);
     }
     return hipErrorInvalidValue;
 }

@@ -35,6 +35,14 @@ hipError_t launch_assemble(int et, int jprec, const KMeshPtrs& m, const KParams&
                            const double* uold, void* J, double* d, int* err, int nwg, size_t lds_bytes,
                            hipStream_t stream);
 
+// split path (FP32 Jacobian, common configuration): per-SCVF records, then
+// the row gather; recs holds split_record_floats(et, ne) floats
+bool split_supported(int et, const KParams& p);
+int64_t split_record_floats(int et, int64_t ne);
+hipError_t launch_assemble_split(int et, const KMeshPtrs& m, const KParams& p, const double* u, const double* uold,
+                                 float* J, double* d, int* err, int nwg, size_t lds, int64_t ne, float* recs,
+                                 hipStream_t stream);
+
 hipError_t launch_geometry_cache(int et, const double* xyz, const int32_t* conn, int64_t ne, int difflen,
                                  double* l2inv, double* vol, uint8_t* affine, int* err, hipStream_t stream);
 
