@@ -142,8 +142,8 @@ def batched_leg(steps: int = 20, count: int = 256):
                         f"seed 0), full upwind + FIELDS, FP32 J + FP64 defect, assembly + Dirichlet rows per HIP-graph "
                         f"replay",
             "elements": int(b.elements), "ms_per_step": ms, "elem_per_s": b.elements / (ms * 1e-3),
-            "launch": {"threads": best[0], "lds_bytes": best[1]},
-            "autotune": [{"threads": t, "lds": l, "ms": m} for t, l, m in table]}
+            "launch": {"path": best[0], "threads": best[1], "lds_bytes": best[2]},
+            "autotune": [{"path": p, "threads": t, "lds": l, "ms": m} for p, t, l, m in table]}
 
 
 def measured_traffic(n: int):
@@ -171,6 +171,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timestep-steps", type=int, default=4, help="timed BDF2 steps of the time-stepping leg (0: skip)")
     ap.add_argument("--no-batched", dest="batched", action="store_false", help="skip the batched-instances leg")
+    ap.add_argument("--launch", type=int, nargs=3, metavar=("PATH", "THREADS", "LDS"),
+                    help="fixed assembly path / launch shape instead of the autotuner (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (rocprof PMC byte calibration)")
     args = ap.parse_args()
@@ -212,6 +214,16 @@ def main():
     def step():
         disc.assemble(ut, uot, 0.01, parts=PART_A, jprec=F32, J=J, d=d, stream=stream, check=False)
 
+    # launch shape (incidences per workgroup, LDS budget) from the autotuner
+    from plugin_navierstokes_amd.batch import autotune
+
+    if args.launch:
+        disc.set_assembly_path(args.launch[0])
+        disc.set_launch(args.launch[1], args.launch[2])
+        launch = {"path": args.launch[0], "threads": args.launch[1], "lds_bytes": args.launch[2], "autotuned": False}
+    else:
+        best, _ = autotune(disc, step, reps=2)
+        launch = {"path": best[0], "threads": best[1], "lds_bytes": best[2], "autotuned": True}
     for _ in range(args.warmup):
         step()
     disc.sync_errors(stream)
@@ -253,7 +265,8 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n),
-            "kernel": "nsfv1::assemble_rows<3, float, 2>", "kernel_ms": kavg, "algorithmic_bytes": B,
+            "kernel": "one assembly = nsfv1::scvf_records<3,2,Q> (Q=0,1,2) + nsfv1::assemble_rows_split<3,2>",
+            "kernel_ms": kavg, "algorithmic_bytes": B, "launch": launch,
         },
     }
     if args.timestep_steps > 0:

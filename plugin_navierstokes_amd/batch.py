@@ -126,14 +126,17 @@ class BatchedCavities:
 
 
 def autotune(disc, run, candidates=None, reps: int = 5, log=None):
-    """Time run() under every (threads, lds_bytes) launch shape, keep the
-    fastest on disc and return (best, table of (threads, lds, ms))."""
+    """Time run() under every (assembly path, threads, lds_bytes) launch
+    shape, keep the fastest on disc and return (best, table of (path,
+    threads, lds, ms)).  Path 0 evaluates each SCVF once into records, path 1
+    is the fused kernel."""
     import torch
 
     if candidates is None:
-        candidates = [(t, l) for t in (64, 128, 192, 256) for l in (16384, 32768, 49152, 65536)]
+        candidates = [(p, t, l) for p in (0, 1) for t in (64, 128, 256) for l in (16384, 32768, 65536)]
     table = []
-    for t, l in candidates:
+    for p, t, l in candidates:
+        disc.set_assembly_path(p)
         disc.set_launch(t, l)
         run()  # builds the plan for this shape
         torch.cuda.synchronize()
@@ -144,11 +147,12 @@ def autotune(disc, run, candidates=None, reps: int = 5, log=None):
         e.record()
         torch.cuda.synchronize()
         ms = s.elapsed_time(e) / reps
-        table.append((t, l, ms))
+        table.append((p, t, l, ms))
         if log:
-            log(f"launch threads={t} lds={l}: {ms:.3f} ms")
-    best = min(table, key=lambda r: r[2])
-    disc.set_launch(best[0], best[1])
+            log(f"launch path={p} threads={t} lds={l}: {ms:.3f} ms")
+    best = min(table, key=lambda r: r[3])
+    disc.set_assembly_path(best[0])
+    disc.set_launch(best[1], best[2])
     run()
     torch.cuda.synchronize()
     return best, table

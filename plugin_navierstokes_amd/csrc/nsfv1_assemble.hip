@@ -1346,10 +1346,14 @@ scvf_records(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
 {
     using T = Topo<ET>;
     constexpr int D = T::dim, N = T::nsh, NC = D + 1, NIP = T::nip;
+    // wavefront w takes SCVF w % count of the 64 elements block w / count:
+    // consecutive wavefronts share their elements' data in L2
+    constexpr int CNT = IpsOfQ<ET, Q>::count;
     const int64_t L = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int idx = (int)(L / ne_pad);  // uniform across a wavefront (ne_pad % 64 == 0)
-    const int64_t e = L - (int64_t)idx * ne_pad;
-    if (idx >= IpsOfQ<ET, Q>::count || e >= ne) return;
+    const int64_t wave = L >> 6;
+    const int idx = (int)(wave % CNT);
+    const int64_t e = (wave / CNT) * 64 + (L & 63);
+    if (e >= ne || e >= ne_pad) return;
     const int ip = IpsOfQ<ET, Q>::list.ip[idx];
     const int kv = T::edge[ip][0];
     int pk[N];
