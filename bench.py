@@ -152,7 +152,7 @@ def measured_traffic(n: int):
     None when no summary covers this workload."""
     if n != 128:
         return None
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "assemble_rows_hex128_fp32.md")
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "assembly_hex128_fp32.md")
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_dispatch"):
