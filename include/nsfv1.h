@@ -180,6 +180,17 @@ int nsfv1_batch_set_instances(nsfv1_handle* h, int64_t ninst, const int64_t* ele
 enum { NSFV1_UPWIND_FRESH = 0, NSFV1_UPWIND_RECORD = 1, NSFV1_UPWIND_REPLAY = 2 };
 int nsfv1_set_upwind_decisions(nsfv1_handle* h, int mode);
 
+/* NavierStokesNoNormalStressOutflow (incompressible/bnd/no_normal_stress_
+   outflow_base.h:70-89 with the FV1 assembly of incompressible/fv1/bnd/
+   no_normal_stress_outflow_fv1.cpp:188-426): the element sides (element
+   index, UG4 local side index; host arrays) on the outflow boundary.
+   nsfv1_assemble then adds, per FV1 boundary face, the tangential viscous
+   flux, the outgoing convective flux and the continuity flux (part A, scaled
+   by s_a) to the rows of the side's nodes.  n = 0 removes the condition.
+   The master's viscosity, density, Laplace and Stokes flags apply (the
+   reference copies them from the master disc, :49-56). */
+int nsfv1_set_outflow(nsfv1_handle* h, const int32_t* elem, const int32_t* side, int64_t n);
+
 /* Synchronises `stream` and reports (then clears) errors raised inside the
    kernels since the last call: NSFV1_ERR_CUT_SIDE when a skewed/LPS upwind
    ray found no side (upwind.cpp:354, 422, 550), NSFV1_ERR_DEGENERATE for an

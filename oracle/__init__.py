@@ -48,6 +48,14 @@ class _Params(C.Structure):
     ]
 
 
+class _BFGeo(C.Structure):
+    _fields_ = [
+        ("nbf", C.c_int), ("node", C.c_int * 4),
+        ("n", (C.c_double * 3) * 4), ("ipl", (C.c_double * 3) * 4), ("ipg", (C.c_double * 3) * 4),
+        ("phi", (C.c_double * 8) * 4), ("g", ((C.c_double * 3) * 8) * 4),
+    ]
+
+
 class _Geo(C.Structure):
     _fields_ = [
         ("et", C.c_int), ("dim", C.c_int), ("nsh", C.c_int), ("nip", C.c_int),
@@ -117,6 +125,10 @@ def lib():
                                       dp, dp, C.c_double, C.c_double, C.c_double, C.c_int,
                                       ip64, ip32, dp, dp]
         L.or_pattern.argtypes = [C.c_int, C.c_int64, ip32, C.c_int64, ip64, ip32, ip64]
+        L.or_bnd_faces.argtypes = [C.c_int, dp, C.c_int, C.POINTER(_BFGeo)]
+        L.or_outflow_elem.argtypes = [C.POINTER(_Params), C.c_int, dp, C.c_int, dp, dp, dp]
+        L.or_outflow.argtypes = [C.POINTER(_Params), C.c_int, dp, C.c_int64, ip32, ip32, ip32, C.c_int64, dp,
+                                 C.c_double, ip64, ip32, dp, dp]
         L.or_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -250,4 +262,47 @@ def assemble_abs(p: Params, et: int, xyz, conn, u, uold=None, dt=0.0, s_a=1.0, s
                                conn.shape[0], _dp(u), _dp(uold), float(dt), float(s_a), float(s_m), int(parts),
                                rowptr.ctypes.data_as(C.POINTER(C.c_int64)),
                                colidx.ctypes.data_as(C.POINTER(C.c_int32)), _dp(J), _dp(d)))
+    return J, d
+
+
+def bnd_faces(et: int, corners, side: int) -> dict:
+    """FV1 boundary faces of one element side (restated ugcore FV1Geometry BF)."""
+    co = _f64(corners)
+    b = _BFGeo()
+    _chk(lib().or_bnd_faces(et, _dp(co), int(side), C.byref(b)))
+    dim, nsh, k = DIM[et], NSH[et], b.nbf
+    a = lambda x: np.ctypeslib.as_array(x)
+    return {"node": a(b.node)[:k].copy(), "n": a(b.n)[:k, :dim].copy(), "ipl": a(b.ipl)[:k, :dim].copy(),
+            "ipg": a(b.ipg)[:k, :dim].copy(), "phi": a(b.phi)[:k, :nsh].copy(), "grad": a(b.g)[:k, :nsh, :dim].copy()}
+
+
+def outflow_elem(p: Params, et: int, corners, side: int, uloc):
+    """Element-local (J, d) of the no-normal-stress outflow condition on one side."""
+    nl = NSH[et] * (DIM[et] + 1)
+    J = np.zeros(nl * nl)
+    d = np.zeros(nl)
+    pc = p.c()
+    _chk(lib().or_outflow_elem(C.byref(pc), et, _dp(_f64(corners)), int(side), _dp(_f64(uloc)), _dp(J), _dp(d)))
+    return J.reshape(nl, nl), d
+
+
+def outflow(p: Params, et: int, xyz, conn, belem, bside, u, s_a=1.0, rowptr=None, colidx=None, J=None, d=None):
+    """Adds s_a x the outflow contributions of the (element, side) list to J, d
+    (new zero arrays when None).  Returns (J, d)."""
+    xyz, u = _f64(xyz), _f64(u)
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    belem = np.ascontiguousarray(belem, dtype=np.int32)
+    bside = np.ascontiguousarray(bside, dtype=np.int32)
+    nv = xyz.shape[0]
+    if rowptr is None:
+        rowptr, colidx = pattern(et, nv, conn)
+    if J is None:
+        J = np.zeros(int(rowptr[-1]))
+    if d is None:
+        d = np.zeros(nv * (DIM[et] + 1))
+    pc = p.c()
+    i32 = C.POINTER(C.c_int32)
+    _chk(lib().or_outflow(C.byref(pc), et, _dp(xyz), nv, conn.ctypes.data_as(i32), belem.ctypes.data_as(i32),
+                          bside.ctypes.data_as(i32), len(belem), _dp(u), float(s_a),
+                          rowptr.ctypes.data_as(C.POINTER(C.c_int64)), colidx.ctypes.data_as(i32), _dp(J), _dp(d)))
     return J, d

@@ -1130,3 +1130,222 @@ int or_pattern(int et, int64_t nv, const int32_t* conn, int64_t ne,
     free(cnt); free(fill); free(inc); free(nb);
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* FV1 boundary faces (ugcore FV1Geometry BF, restated): on side s of  */
+/* the element, corner c owns the part of the side inside its SCV.     */
+/* 2D: [x_c, edge midpoint]; 3D: [x_c, mid(c,next), side centre,       */
+/* mid(prev,c)].  ip = average of the part's corners, normal = outward */
+/* area vector (2D: rotated segment; 3D: half the cross product of the */
+/* diagonals), shapes and global gradients at the local ip.            */
+/* ------------------------------------------------------------------ */
+int or_bnd_faces(int et, const double* corners, int side, or_bfgeo* B)
+{
+    if (et < 0 || et > 3) { set_err("unknown element type"); return -1; }
+    const refelem* R = &REF[et];
+    if (side < 0 || side >= R->nside) { set_err("side index out of range"); return -1; }
+    const int dim = R->dim, nsh = R->nsh, k = R->side_n[side];
+    memset(B, 0, sizeof *B);
+    double co[8][3] = {{0}}, c[3] = {0, 0, 0};
+    for (int q = 0; q < nsh; ++q)
+        for (int i = 0; i < dim; ++i) { co[q][i] = corners[q * dim + i]; c[i] += co[q][i] / nsh; }
+    double fc[3] = {0, 0, 0}, fcl[3] = {0, 0, 0};
+    for (int j = 0; j < k; ++j)
+        for (int i = 0; i < dim; ++i) {
+            fc[i] += co[R->side[side][j]][i] / k;
+            fcl[i] += R->lc[R->side[side][j]][i] / k;
+        }
+    B->nbf = k;
+    for (int j = 0; j < k; ++j) {
+        const int cc = R->side[side][j];
+        const int nx = R->side[side][(j + 1) % k], pv = R->side[side][(j + k - 1) % k];
+        double p[4][3] = {{0}}, pl[4][3] = {{0}};
+        int np;
+        if (dim == 2) {
+            /* the side is the edge (side[0], side[1]); the part of corner cc ends at the midpoint */
+            const int other = (j == 0) ? R->side[side][1] : R->side[side][0];
+            for (int i = 0; i < 2; ++i) {
+                p[0][i] = co[cc][i];
+                p[1][i] = 0.5 * (co[cc][i] + co[other][i]);
+                pl[0][i] = R->lc[cc][i];
+                pl[1][i] = 0.5 * (R->lc[cc][i] + R->lc[other][i]);
+            }
+            np = 2;
+        } else {
+            for (int i = 0; i < 3; ++i) {
+                p[0][i] = co[cc][i];
+                p[1][i] = 0.5 * (co[cc][i] + co[nx][i]);
+                p[2][i] = fc[i];
+                p[3][i] = 0.5 * (co[pv][i] + co[cc][i]);
+                pl[0][i] = R->lc[cc][i];
+                pl[1][i] = 0.5 * (R->lc[cc][i] + R->lc[nx][i]);
+                pl[2][i] = fcl[i];
+                pl[3][i] = 0.5 * (R->lc[pv][i] + R->lc[cc][i]);
+            }
+            np = 4;
+        }
+        B->node[j] = cc;
+        for (int i = 0; i < dim; ++i) {
+            double a = 0, b = 0;
+            for (int q = 0; q < np; ++q) { a += p[q][i]; b += pl[q][i]; }
+            B->ipg[j][i] = a / np;
+            B->ipl[j][i] = b / np;
+        }
+        double n[3] = {0, 0, 0};
+        if (dim == 2) {
+            n[0] = p[1][1] - p[0][1];
+            n[1] = -(p[1][0] - p[0][0]);
+        } else {
+            double d1[3], d2[3];
+            for (int i = 0; i < 3; ++i) { d1[i] = p[2][i] - p[0][i]; d2[i] = p[3][i] - p[1][i]; }
+            cross(d1, d2, n);
+            for (int i = 0; i < 3; ++i) n[i] *= 0.5;
+        }
+        double out[3] = {0, 0, 0};
+        for (int i = 0; i < dim; ++i) out[i] = B->ipg[j][i] - c[i];
+        if (dot(dim, n, out) < 0)
+            for (int i = 0; i < dim; ++i) n[i] = -n[i];
+        for (int i = 0; i < dim; ++i) B->n[j][i] = n[i];
+        /* shapes and global gradients at the local ip */
+        double gl[8][3] = {{0}};
+        shapes(et, B->ipl[j], B->phi[j], gl);
+        double Jm[3][3] = {{0}};
+        for (int q = 0; q < nsh; ++q)
+            for (int i = 0; i < dim; ++i)
+                for (int jj = 0; jj < dim; ++jj) Jm[i][jj] += co[q][i] * gl[q][jj];
+        double Ji[3][3] = {{0}};
+        if (dim == 2) {
+            const double det = Jm[0][0] * Jm[1][1] - Jm[0][1] * Jm[1][0];
+            if (det == 0.0) { set_err("degenerate element"); return -2; }
+            Ji[0][0] = Jm[1][1] / det;  Ji[0][1] = -Jm[0][1] / det;
+            Ji[1][0] = -Jm[1][0] / det; Ji[1][1] = Jm[0][0] / det;
+        } else {
+            const double det = Jm[0][0] * (Jm[1][1] * Jm[2][2] - Jm[1][2] * Jm[2][1])
+                             - Jm[0][1] * (Jm[1][0] * Jm[2][2] - Jm[1][2] * Jm[2][0])
+                             + Jm[0][2] * (Jm[1][0] * Jm[2][1] - Jm[1][1] * Jm[2][0]);
+            if (det == 0.0) { set_err("degenerate element"); return -2; }
+            Ji[0][0] = (Jm[1][1] * Jm[2][2] - Jm[1][2] * Jm[2][1]) / det;
+            Ji[0][1] = (Jm[0][2] * Jm[2][1] - Jm[0][1] * Jm[2][2]) / det;
+            Ji[0][2] = (Jm[0][1] * Jm[1][2] - Jm[0][2] * Jm[1][1]) / det;
+            Ji[1][0] = (Jm[1][2] * Jm[2][0] - Jm[1][0] * Jm[2][2]) / det;
+            Ji[1][1] = (Jm[0][0] * Jm[2][2] - Jm[0][2] * Jm[2][0]) / det;
+            Ji[1][2] = (Jm[0][2] * Jm[1][0] - Jm[0][0] * Jm[1][2]) / det;
+            Ji[2][0] = (Jm[1][0] * Jm[2][1] - Jm[1][1] * Jm[2][0]) / det;
+            Ji[2][1] = (Jm[0][1] * Jm[2][0] - Jm[0][0] * Jm[2][1]) / det;
+            Ji[2][2] = (Jm[0][0] * Jm[1][1] - Jm[0][1] * Jm[1][0]) / det;
+        }
+        for (int q = 0; q < nsh; ++q)
+            for (int i = 0; i < dim; ++i) {
+                double s = 0;
+                for (int jj = 0; jj < dim; ++jj) s += Ji[jj][i] * gl[q][jj];
+                B->g[j][q][i] = s;
+            }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* NavierStokesNoNormalStressOutflowFV1 (restated):                    */
+/* incompressible/fv1/bnd/no_normal_stress_outflow_fv1.cpp:188-426.   */
+/* Per boundary face (node c, normal n, ip):                           */
+/*   momentum: tangential part of -nu rho (grad u + grad u^T) n, with  */
+/*     the normal part removed as F - (F.n) n (n is the area vector,   */
+/*     as in :261); convection max(rho u.n, 0) u (:305-309, 329-335);  */
+/*   continuity: rho u.n (:424).                                       */
+/* Jacobian: the same terms with the convective factor frozen          */
+/* (:291-311, 360-368).  Element-local J (nl x nl) and d (nl), ADDED.  */
+/* ------------------------------------------------------------------ */
+int or_outflow_elem(const or_params* p, int et, const double* corners, int side, const double* uloc,
+                    double* Jl, double* dl)
+{
+    or_bfgeo B;
+    int rc = or_bnd_faces(et, corners, side, &B);
+    if (rc) return rc;
+    const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1, nl = nsh * nc;
+    const double nu = p->nu, rho = p->rho;
+    for (int f = 0; f < B.nbf; ++f) {
+        const double* n = B.n[f];
+        const int row = B.node[f];
+        double vel[3] = {0, 0, 0};
+        for (int d1 = 0; d1 < dim; ++d1)
+            for (int sh = 0; sh < nsh; ++sh) vel[d1] += uloc[sh * nc + d1] * B.phi[f][sh];
+        double flux = dot(dim, vel, n) * rho;
+        if (flux < 0) flux = 0;
+        if (dl) {
+            double gv[3][3] = {{0}}, F[3] = {0, 0, 0};
+            for (int d1 = 0; d1 < dim; ++d1)
+                for (int d2 = 0; d2 < dim; ++d2)
+                    for (int sh = 0; sh < nsh; ++sh) gv[d1][d2] += B.g[f][sh][d2] * uloc[sh * nc + d1];
+            for (int d1 = 0; d1 < dim; ++d1)
+                for (int d2 = 0; d2 < dim; ++d2) F[d1] += gv[d1][d2] * n[d2];
+            if (!p->laplace)
+                for (int d1 = 0; d1 < dim; ++d1)
+                    for (int d2 = 0; d2 < dim; ++d2) F[d1] += gv[d2][d1] * n[d2];
+            const double fn = dot(dim, F, n);
+            for (int d1 = 0; d1 < dim; ++d1) F[d1] -= fn * n[d1];
+            for (int d1 = 0; d1 < dim; ++d1) {
+                dl[row * nc + d1] += F[d1] * (-nu * rho);
+                if (!p->stokes) dl[row * nc + d1] += flux * vel[d1];
+            }
+            dl[row * nc + dim] += dot(dim, vel, n) * rho;
+        }
+        if (Jl) {
+            for (int sh = 0; sh < nsh; ++sh) {
+                const double* g = B.g[f][sh];
+                double df[3][3] = {{0}}, ns[3] = {0, 0, 0};
+                const double gn = dot(dim, g, n);
+                for (int d1 = 0; d1 < dim; ++d1) df[d1][d1] = gn;
+                if (!p->laplace)
+                    for (int d1 = 0; d1 < dim; ++d1)
+                        for (int d2 = 0; d2 < dim; ++d2) df[d1][d2] += g[d1] * n[d2];
+                for (int d2 = 0; d2 < dim; ++d2)
+                    for (int d1 = 0; d1 < dim; ++d1) ns[d2] += df[d1][d2] * n[d1];
+                for (int d2 = 0; d2 < dim; ++d2)
+                    for (int d1 = 0; d1 < dim; ++d1) df[d1][d2] -= n[d1] * ns[d2];
+                for (int d1 = 0; d1 < dim; ++d1)
+                    for (int d2 = 0; d2 < dim; ++d2)
+                        Jl[(row * nc + d1) * nl + sh * nc + d2] += df[d1][d2] * (-nu * rho);
+                if (!p->stokes)
+                    for (int d1 = 0; d1 < dim; ++d1) Jl[(row * nc + d1) * nl + sh * nc + d1] += flux * B.phi[f][sh];
+                for (int d2 = 0; d2 < dim; ++d2)
+                    Jl[(row * nc + dim) * nl + sh * nc + d2] += B.phi[f][sh] * n[d2] * rho;
+            }
+        }
+    }
+    return 0;
+}
+
+/* adds s_a * (outflow contributions of the listed boundary sides) to J and d */
+int or_outflow(const or_params* p, int et, const double* xyz, int64_t nv, const int32_t* conn,
+               const int32_t* belem, const int32_t* bside, int64_t nb, const double* u, double s_a,
+               const int64_t* rowptr, const int32_t* colidx, double* J, double* d)
+{
+    if (et < 0 || et > 3) { set_err("unknown element type"); return -1; }
+    const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1, nl = nsh * nc;
+    (void)nv;
+    double co[24], ul[32], Jl[1024], dl[32];
+    for (int64_t b = 0; b < nb; ++b) {
+        const int32_t* c = conn + (int64_t)belem[b] * nsh;
+        for (int k = 0; k < nsh; ++k) {
+            for (int i = 0; i < dim; ++i) co[k * dim + i] = xyz[(int64_t)c[k] * dim + i];
+            for (int a = 0; a < nc; ++a) ul[k * nc + a] = u[(int64_t)c[k] * nc + a];
+        }
+        memset(Jl, 0, sizeof(double) * nl * nl);
+        memset(dl, 0, sizeof(double) * nl);
+        int rc = or_outflow_elem(p, et, co, bside[b], ul, J ? Jl : NULL, d ? dl : NULL);
+        if (rc) return rc;
+        for (int i = 0; i < nsh; ++i)
+            for (int a = 0; a < nc; ++a) {
+                const int64_t row = (int64_t)c[i] * nc + a;
+                if (d) d[row] += s_a * dl[i * nc + a];
+                if (J)
+                    for (int j = 0; j < nsh; ++j)
+                        for (int bb = 0; bb < nc; ++bb) {
+                            const int64_t pos = find_col(rowptr, colidx, row, (int32_t)((int64_t)c[j] * nc + bb));
+                            if (pos < 0) { set_err("pattern misses an entry"); return -6; }
+                            J[pos] += s_a * Jl[(i * nc + a) * nl + j * nc + bb];
+                        }
+            }
+    }
+    return 0;
+}

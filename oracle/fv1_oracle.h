@@ -113,6 +113,25 @@ int or_pattern(int et, int64_t nv, const int32_t* conn, int64_t ne,
 
 const char* or_last_error(void);
 
+/* FV1 boundary faces of one element side (restated ugcore FV1Geometry BF). */
+typedef struct {
+    int nbf;
+    int node[4];
+    double n[4][3], ipl[4][3], ipg[4][3];
+    double phi[4][8], g[4][8][3];
+} or_bfgeo;
+int or_bnd_faces(int et, const double* corners, int side, or_bfgeo* B);
+
+/* No-normal-stress outflow boundary disc
+   (incompressible/fv1/bnd/no_normal_stress_outflow_fv1.cpp:188-426):
+   element-local contributions of one boundary side, ADDED to Jl/dl. */
+int or_outflow_elem(const or_params* p, int et, const double* corners, int side, const double* uloc,
+                    double* Jl, double* dl);
+/* adds s_a times the contributions of nb boundary sides (element, side) to J/d */
+int or_outflow(const or_params* p, int et, const double* xyz, int64_t nv, const int32_t* conn,
+               const int32_t* belem, const int32_t* bside, int64_t nb, const double* u, double s_a,
+               const int64_t* rowptr, const int32_t* colidx, double* J, double* d);
+
 #ifdef __cplusplus
 }
 #endif

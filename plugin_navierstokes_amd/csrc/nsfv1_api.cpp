@@ -20,6 +20,8 @@
 #include "nsfv1_internal.hpp"
 #include "nsfv1_solver.hpp"
 #include "nsfv1_mlp.hpp"
+#include "nsfv1_boundary.hpp"
+#include "nsfv1_topo.hpp"
 
 using namespace nsfv1;
 
@@ -109,7 +111,10 @@ struct nsfv1_handle {
     int et = -1;
     int64_t nv = 0, ne = 0, ndof = 0, nnz = 0, ninc = 0;
     DevBuf xyz, conn, rowptr, colidx, errflag, l2inv, vol, affine, node_inc, inc_elem, inc_node, inc_info, inc_pos;
-    std::vector<int32_t> h_inc_info;
+    std::vector<int32_t> h_inc_info, h_inc_elem;
+    // no-normal-stress outflow boundary disc
+    DevBuf out_ptr, out_node, out_inc, out_side;
+    int64_t out_nbn = 0;
     std::vector<int64_t> h_rowptr, h_node_inc;
     int max_threads = 256;
     size_t lds_cap = 64 * 1024;
@@ -471,6 +476,8 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
     h->h_rowptr.swap(rowptr);
     h->h_node_inc.swap(cnt);
     h->h_inc_info.swap(ii);
+    h->h_inc_elem.swap(ie);
+    h->out_nbn = 0;
     h->plan32.valid = h->plan64.valid = false;
     h->geo_valid = false;
     // solver state sized for the previous mesh
@@ -672,6 +679,14 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
                             (hipStream_t)stream);
     }
     if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("assemble launch: ") + hipGetErrorString(e));
+    if (h->out_nbn > 0 && (parts & NSFV1_PART_A)) {
+        OutflowArgs oa{h->xyz.as<double>(), h->conn.as<int32_t>(), h->rowptr.as<int64_t>(), h->inc_elem.as<int32_t>(),
+                       h->inc_info.as<int32_t>(), h->inc_pos.as<uint64_t>(), h->out_ptr.as<int32_t>(),
+                       h->out_node.as<int64_t>(), h->out_inc.as<int32_t>(), h->out_side.as<int32_t>(), h->out_nbn,
+                       h->nu, h->rho, s_a, h->laplace ? 1 : 0, h->stokes ? 1 : 0};
+        e = launch_outflow(h->et, jprec, oa, u, jvals, defect, h->errflag.as<int>(), (hipStream_t)stream);
+        if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("outflow launch: ") + hipGetErrorString(e));
+    }
     if (h->upw_mode == NSFV1_UPWIND_RECORD && (parts & NSFV1_PART_A)) h->upw_recorded = true;
     return NSFV1_OK;
 }
@@ -699,6 +714,75 @@ int nsfv1_batch_set_instances(nsfv1_handle* h, int64_t ninst, const int64_t* ele
         return fail(NSFV1_ERR_DEVICE, "instance viscosity upload");
     h->ninst = ninst;
     h->nu_set = true;
+    return NSFV1_OK;
+}
+
+}  // extern "C"
+
+template <int ET>
+static void side_corners(int s, int* n, int* c)
+{
+    *n = Topo<ET>::side_n[s];
+    for (int j = 0; j < *n; ++j) c[j] = Topo<ET>::side[s][j];
+}
+
+extern "C" {
+
+int nsfv1_set_outflow(nsfv1_handle* h, const int32_t* elem, const int32_t* side, int64_t n)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    h->out_nbn = 0;
+    if (n == 0) return NSFV1_OK;
+    if (n < 0 || !elem || !side) return fail(NSFV1_ERR_ARGUMENT, "bad boundary side arrays");
+    const int nsh = h->et == ET_HEX ? 8 : (h->et == ET_TRI ? 3 : 4);
+    const int nside = h->et == ET_HEX ? 6 : (h->et == ET_QUAD ? 4 : (h->et == ET_TET ? 4 : 3));
+    std::vector<int32_t> cn;  // conn rows are needed: download once
+    cn.resize((size_t)h->ne * nsh);
+    if (hipMemcpy(cn.data(), h->conn.p, sizeof(int32_t) * cn.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "connectivity download");
+    // (node, item) pairs in input order, grouped by node
+    std::vector<std::pair<int64_t, std::pair<int32_t, int32_t>>> items;
+    for (int64_t b = 0; b < n; ++b) {
+        const int64_t e = elem[b];
+        const int s = side[b];
+        if (e < 0 || e >= h->ne) return fail(NSFV1_ERR_ARGUMENT, "boundary element out of range");
+        if (s < 0 || s >= nside) return fail(NSFV1_ERR_ARGUMENT, "boundary side out of range");
+        int k = 0, c[4];
+        switch (h->et) {
+        case 0: side_corners<0>(s, &k, c); break;
+        case 1: side_corners<1>(s, &k, c); break;
+        case 2: side_corners<2>(s, &k, c); break;
+        default: side_corners<3>(s, &k, c); break;
+        }
+        for (int j = 0; j < k; ++j) {
+            const int64_t v = cn[e * nsh + c[j]];
+            int64_t q = -1;
+            for (int64_t t = h->h_node_inc[v]; t < h->h_node_inc[v + 1]; ++t)
+                if (h->h_inc_elem[t] == e) { q = t; break; }
+            if (q < 0) return fail(NSFV1_ERR_ARGUMENT, "boundary side corner without incidence");
+            items.push_back({v, {(int32_t)q, (int32_t)s}});
+        }
+    }
+    std::stable_sort(items.begin(), items.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    std::vector<int64_t> nodes;
+    std::vector<int32_t> ptr{0}, iq, is;
+    for (size_t i = 0; i < items.size(); ++i) {
+        if (nodes.empty() || nodes.back() != items[i].first) {
+            if (!nodes.empty()) ptr.push_back((int32_t)iq.size());
+            nodes.push_back(items[i].first);
+        }
+        iq.push_back(items[i].second.first);
+        is.push_back(items[i].second.second);
+    }
+    ptr.push_back((int32_t)iq.size());
+    auto up = [](DevBuf& b, const void* src, size_t bytes) {
+        return b.alloc(bytes) == hipSuccess && hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    if (!up(h->out_ptr, ptr.data(), sizeof(int32_t) * ptr.size()) || !up(h->out_node, nodes.data(), sizeof(int64_t) * nodes.size()) ||
+        !up(h->out_inc, iq.data(), sizeof(int32_t) * iq.size()) || !up(h->out_side, is.data(), sizeof(int32_t) * is.size()))
+        return fail(NSFV1_ERR_DEVICE, "outflow upload");
+    h->out_nbn = (int64_t)nodes.size();
     return NSFV1_OK;
 }
 

@@ -16,6 +16,30 @@ TRI, QUAD, TET, HEX = 0, 1, 2, 3
 ELEM_NAMES = {TRI: "triangle", QUAD: "quadrilateral", TET: "tetrahedron", HEX: "hexahedron"}
 DIM = {TRI: 2, QUAD: 2, TET: 3, HEX: 3}
 NSH = {TRI: 3, QUAD: 4, TET: 4, HEX: 8}
+# element sides in UG4 reference-element numbering (corners of each side)
+SIDES = {
+    TRI: [(0, 1), (1, 2), (2, 0)],
+    QUAD: [(0, 1), (1, 2), (2, 3), (3, 0)],
+    TET: [(0, 2, 1), (1, 2, 3), (0, 3, 2), (0, 1, 3)],
+    HEX: [(0, 3, 2, 1), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (0, 4, 7, 3), (4, 5, 6, 7)],
+}
+
+
+def boundary_sides(mesh: "Mesh", nodes) -> tuple:
+    """(element, side) pairs of every element side whose corners all lie in
+    `nodes` (e.g. the outflow plane).  Returns int32 arrays."""
+    mask = np.zeros(mesh.nv, dtype=bool)
+    mask[np.asarray(nodes, dtype=np.int64)] = True
+    belem, bside = [], []
+    for s, corners in enumerate(SIDES[mesh.et]):
+        on = np.all(mask[mesh.conn[:, list(corners)]], axis=1)
+        e = np.nonzero(on)[0]
+        belem.append(e)
+        bside.append(np.full(len(e), s))
+    belem = np.concatenate(belem)
+    bside = np.concatenate(bside)
+    o = np.lexsort((bside, belem))
+    return belem[o].astype(np.int32), bside[o].astype(np.int32)
 
 
 @dataclass

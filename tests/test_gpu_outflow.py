@@ -1,0 +1,83 @@
+"""GPU parity of the no-normal-stress outflow boundary disc
+(incompressible/fv1/bnd/no_normal_stress_outflow_fv1.cpp:188-426): element
+assembly plus outflow rows through the C ABI against the CPU oracle."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+from plugin_navierstokes_amd import F32, F64, NavierStokesFV1, PART_A
+from plugin_navierstokes_amd import meshes as M
+from tests._parity import configure_disc, oracle_params
+
+pytestmark = pytest.mark.gpu
+
+ETS = [M.TRI, M.QUAD, M.TET, M.HEX]
+
+
+def _case(et):
+    mesh = M.cavity_2d(5, et, 0.12, 8) if et in (M.TRI, M.QUAD) else M.cavity_3d(3, et, 0.12, 8)
+    x = mesh.xyz[:, 0]
+    out_nodes = np.nonzero(np.isclose(x, x.max()))[0]
+    top = np.nonzero(np.isclose(mesh.xyz[:, 1], mesh.xyz[:, 1].max()))[0]
+    be1, bs1 = M.boundary_sides(mesh, out_nodes)
+    be2, bs2 = M.boundary_sides(mesh, top)
+    return mesh, np.concatenate([be1, be2]), np.concatenate([bs1, bs2])
+
+
+@pytest.mark.parametrize("cfg", [{"upwind": "full"}, {"upwind": "lps", "laplace": True}, {"upwind": "full", "stokes": True}],
+                         ids=["full", "lps-laplace", "stokes"])
+@pytest.mark.parametrize("jprec", [F64, F32])
+@pytest.mark.parametrize("et", ETS)
+def test_outflow_parity(cuda, et, jprec, cfg):
+    import torch
+
+    mesh, belem, bside = _case(et)
+    assert len(belem) > 0
+    u = M.random_state(mesh, 21)
+    disc = NavierStokesFV1("u,v,p" if mesh.dim == 2 else "u,v,w,p", "Inner")
+    configure_disc(disc, cfg)
+    disc.set_mesh(mesh)
+    disc.set_outflow(belem, bside)
+    J, d = disc.assemble(torch.from_numpy(u).to(cuda), s_a=0.8, parts=PART_A, jprec=jprec)
+    J, d = J.cpu().numpy().astype(np.float64), d.cpu().numpy()
+    rp, ci = disc.pattern()
+    p = oracle_params(mesh.dim, cfg)
+    Jr, dr = O.assemble(p, et, mesh.xyz, mesh.conn, u, s_a=0.8, parts=1, rowptr=rp, colidx=ci)
+    Ja, da = O.assemble_abs(p, et, mesh.xyz, mesh.conn, u, s_a=0.8, parts=1, rowptr=rp, colidx=ci)
+    Jo, do = O.outflow(p, et, mesh.xyz, mesh.conn, belem, bside, u, s_a=0.8, rowptr=rp, colidx=ci)
+    assert np.abs(Jo).max() > 0 and np.abs(do).max() > 0
+    # per-row magnitude of the outflow terms as their rounding scale
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    rmax = np.zeros(len(rp) - 1)
+    np.maximum.at(rmax, rows, np.abs(Jo))
+    SJ = Ja + rmax[rows]
+    Sd = da + np.abs(do)
+    Jw, dw = Jr + Jo, dr + do
+    if jprec == F64:
+        tolJ = 1e-12 * np.abs(Jw) + 1e-13 * SJ
+    else:
+        tolJ = 2.0 ** -17 * SJ + 2.0 ** -20 * np.abs(Jw)
+    told = 1e-12 * np.abs(dw) + 1e-13 * Sd
+    vJ = np.max(np.abs(J - Jw) / (tolJ + 1e-300))
+    vd = np.max(np.abs(d - dw) / (told + 1e-300))
+    print(f"et={et} jprec={jprec} cfg={cfg}: violation J {vJ:.3f} d {vd:.3f}, outflow sides {len(belem)}")
+    assert vJ <= 1.0 and vd <= 1.0
+
+
+def test_outflow_cleared(cuda):
+    import torch
+
+    mesh, belem, bside = _case(M.QUAD)
+    u = torch.from_numpy(M.random_state(mesh, 2)).to(cuda)
+    disc = NavierStokesFV1("u,v,p", "Inner")
+    configure_disc(disc, {"upwind": "full"})
+    disc.set_mesh(mesh)
+    J0, d0 = disc.assemble(u)
+    disc.set_outflow(belem, bside)
+    J1, d1 = disc.assemble(u)
+    disc.set_outflow([], [])
+    J2, d2 = disc.assemble(u)
+    assert not torch.equal(d0, d1)
+    assert torch.equal(J0, J2) and torch.equal(d0, d2)

@@ -1,0 +1,108 @@
+"""Pins the oracle's FV1 boundary faces and its no-normal-stress outflow
+condition (incompressible/fv1/bnd/no_normal_stress_outflow_fv1.cpp:188-426):
+the discrete divergence theorem on every sub-control volume (interior
+SCVF normals plus boundary-face normals close), boundary-face areas,
+the Picard identity d = J u, and finite differences."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+from plugin_navierstokes_amd import meshes as M
+
+ETS = [O.TRI, O.QUAD, O.TET, O.HEX]
+
+
+def _elem(et, seed, jitter=0.15):
+    rng = np.random.default_rng(seed)
+    lc = {O.TRI: [[0, 0], [1, 0], [0, 1]], O.QUAD: [[0, 0], [1, 0], [1, 1], [0, 1]],
+          O.TET: [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]],
+          O.HEX: [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]]}[et]
+    co = np.asarray(lc, dtype=float) * 1.3 + rng.uniform(-jitter, jitter, size=(len(lc), O.DIM[et]))
+    return co
+
+
+@pytest.mark.parametrize("et", ETS)
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_sub_control_volumes_close(et, seed):
+    """Sum over the boundary of every SCV of the area vectors = 0."""
+    co = _elem(et, seed)
+    g = O.geometry(et, co)
+    nsh = O.NSH[et]
+    tot = np.zeros((nsh, O.DIM[et]))
+    for ip in range(len(g["from"])):
+        tot[g["from"][ip]] += g["n"][ip]      # n points from -> to: outward for `from`
+        tot[g["to"][ip]] -= g["n"][ip]
+    for s in range(len(M.SIDES[et])):
+        b = O.bnd_faces(et, co, s)
+        for j, c in enumerate(b["node"]):
+            tot[c] += b["n"][j]
+    scale = np.abs(g["n"]).max()
+    assert np.abs(tot).max() <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("et", ETS)
+def test_boundary_face_areas_and_ips(et):
+    """Unit reference element scaled by 2: the parts of a side split its area
+    equally (planar, affine sides), normals are outward and axis-aligned, the
+    ips lie on the side."""
+    lc = np.asarray({O.TRI: [[0, 0], [1, 0], [0, 1]], O.QUAD: [[0, 0], [1, 0], [1, 1], [0, 1]],
+                     O.TET: [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]],
+                     O.HEX: [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1],
+                             [0, 1, 1]]}[et], dtype=float)
+    co = 2.0 * lc
+    c = co.mean(0)
+    for s, corners in enumerate(M.SIDES[et]):
+        b = O.bnd_faces(et, co, s)
+        pts = co[list(corners)]
+        if O.DIM[et] == 2:
+            area = np.linalg.norm(pts[1] - pts[0])
+        elif len(corners) == 3:
+            area = 0.5 * np.linalg.norm(np.cross(pts[1] - pts[0], pts[2] - pts[0]))
+        else:
+            area = np.linalg.norm(np.cross(pts[2] - pts[0], pts[3] - pts[1])) / 2
+        a = np.linalg.norm(b["n"], axis=1)
+        assert abs(a.sum() - area) <= 1e-14 * area
+        if len(corners) != 3 or O.DIM[et] == 2:
+            np.testing.assert_allclose(a, area / len(corners), rtol=1e-14)
+        nrm = b["n"] / a[:, None]
+        assert np.all(np.einsum("ij,ij->i", nrm, b["ipg"] - c) > 0)
+        # ips are in the side plane: (ip - corner0) . n = 0
+        assert np.all(np.abs(np.einsum("ij,ij->i", b["ipg"] - pts[0], nrm)) <= 1e-14)
+
+
+@pytest.mark.parametrize("et", ETS)
+@pytest.mark.parametrize("flags", [{}, {"laplace": True}, {"stokes": True}])
+def test_outflow_picard_identity(et, flags):
+    co = _elem(et, 7)
+    rng = np.random.default_rng(3)
+    nsh, nc = O.NSH[et], O.DIM[et] + 1
+    p = O.Params(dim=O.DIM[et], upwind="full", nu=0.03, rho=1.4, **flags)
+    for s in range(len(M.SIDES[et])):
+        u = rng.normal(size=nsh * nc)
+        J, d = O.outflow_elem(p, et, co, s, u)
+        np.testing.assert_allclose(J @ u, d, rtol=1e-12, atol=1e-14 * np.abs(J).max())
+        # rows of corners not on the side stay empty
+        on = set(M.SIDES[et][s])
+        for k in range(nsh):
+            if k not in on:
+                assert not J[k * nc:(k + 1) * nc].any() and not d[k * nc:(k + 1) * nc].any()
+
+
+@pytest.mark.parametrize("et", ETS)
+def test_outflow_finite_differences(et):
+    """Away from the switch of max(u.n, 0) the defect's derivative is the
+    fixpoint Jacobian plus the derivative of the frozen convective factor."""
+    co = _elem(et, 11)
+    rng = np.random.default_rng(4)
+    nsh, nc = O.NSH[et], O.DIM[et] + 1
+    p = O.Params(dim=O.DIM[et], upwind="full", nu=0.05, rho=1.0, stokes=True)
+    u = rng.normal(size=nsh * nc)
+    J, d = O.outflow_elem(p, et, co, 0, u)
+    h = 1e-6
+    for j in range(nsh * nc):
+        e = np.zeros(nsh * nc)
+        e[j] = h
+        fd = (O.outflow_elem(p, et, co, 0, u + e)[1] - O.outflow_elem(p, et, co, 0, u - e)[1]) / (2 * h)
+        np.testing.assert_allclose(fd, J[:, j], rtol=1e-6, atol=1e-8)
