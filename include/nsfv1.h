@@ -191,6 +191,15 @@ int nsfv1_set_upwind_decisions(nsfv1_handle* h, int mode);
    reference copies them from the master disc, :49-56). */
 int nsfv1_set_outflow(nsfv1_handle* h, const int32_t* elem, const int32_t* side, int64_t n);
 
+/* NavierStokesSymBCFV1 (incompressible/fv1/bnd/symmetric_boundary_fv1.cpp:
+   224-386, Lua name "NavierStokesSymBCFV1", its add() of boundary subsets
+   replaced by the side list as in nsfv1_set_outflow): per FV1 boundary face
+   the normal viscous stress component-wise times n and the pressure times n
+   go into the momentum rows of the face's node (part A, scaled by s_a);
+   the continuity rows get nothing.  n = 0 removes the condition.  The
+   master's viscosity, density and Laplace flag apply. */
+int nsfv1_set_symmetry(nsfv1_handle* h, const int32_t* elem, const int32_t* side, int64_t n);
+
 /* Synchronises `stream` and reports (then clears) errors raised inside the
    kernels since the last call: NSFV1_ERR_CUT_SIDE when a skewed/LPS upwind
    ray found no side (upwind.cpp:354, 422, 550), NSFV1_ERR_DEGENERATE for an

@@ -129,6 +129,9 @@ def lib():
         L.or_outflow_elem.argtypes = [C.POINTER(_Params), C.c_int, dp, C.c_int, dp, dp, dp]
         L.or_outflow.argtypes = [C.POINTER(_Params), C.c_int, dp, C.c_int64, ip32, ip32, ip32, C.c_int64, dp,
                                  C.c_double, ip64, ip32, dp, dp]
+        L.or_symmetry_elem.argtypes = [C.POINTER(_Params), C.c_int, dp, C.c_int, dp, dp, dp]
+        L.or_symmetry.argtypes = [C.POINTER(_Params), C.c_int, dp, C.c_int64, ip32, ip32, ip32, C.c_int64, dp,
+                                  C.c_double, ip64, ip32, dp, dp]
         L.or_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -276,17 +279,20 @@ def bnd_faces(et: int, corners, side: int) -> dict:
             "ipg": a(b.ipg)[:k, :dim].copy(), "phi": a(b.phi)[:k, :nsh].copy(), "grad": a(b.g)[:k, :nsh, :dim].copy()}
 
 
-def outflow_elem(p: Params, et: int, corners, side: int, uloc):
-    """Element-local (J, d) of the no-normal-stress outflow condition on one side."""
+def outflow_elem(p: Params, et: int, corners, side: int, uloc, kind: str = "outflow"):
+    """Element-local (J, d) of a boundary condition ("outflow": no normal
+    stress; "symmetry") on one side."""
     nl = NSH[et] * (DIM[et] + 1)
     J = np.zeros(nl * nl)
     d = np.zeros(nl)
     pc = p.c()
-    _chk(lib().or_outflow_elem(C.byref(pc), et, _dp(_f64(corners)), int(side), _dp(_f64(uloc)), _dp(J), _dp(d)))
+    f = lib().or_outflow_elem if kind == "outflow" else lib().or_symmetry_elem
+    _chk(f(C.byref(pc), et, _dp(_f64(corners)), int(side), _dp(_f64(uloc)), _dp(J), _dp(d)))
     return J.reshape(nl, nl), d
 
 
-def outflow(p: Params, et: int, xyz, conn, belem, bside, u, s_a=1.0, rowptr=None, colidx=None, J=None, d=None):
+def outflow(p: Params, et: int, xyz, conn, belem, bside, u, s_a=1.0, rowptr=None, colidx=None, J=None, d=None,
+            kind: str = "outflow"):
     """Adds s_a x the outflow contributions of the (element, side) list to J, d
     (new zero arrays when None).  Returns (J, d)."""
     xyz, u = _f64(xyz), _f64(u)
@@ -302,7 +308,8 @@ def outflow(p: Params, et: int, xyz, conn, belem, bside, u, s_a=1.0, rowptr=None
         d = np.zeros(nv * (DIM[et] + 1))
     pc = p.c()
     i32 = C.POINTER(C.c_int32)
-    _chk(lib().or_outflow(C.byref(pc), et, _dp(xyz), nv, conn.ctypes.data_as(i32), belem.ctypes.data_as(i32),
+    f = lib().or_outflow if kind == "outflow" else lib().or_symmetry
+    _chk(f(C.byref(pc), et, _dp(xyz), nv, conn.ctypes.data_as(i32), belem.ctypes.data_as(i32),
                           bside.ctypes.data_as(i32), len(belem), _dp(u), float(s_a),
                           rowptr.ctypes.data_as(C.POINTER(C.c_int64)), colidx.ctypes.data_as(i32), _dp(J), _dp(d)))
     return J, d

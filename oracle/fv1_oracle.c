@@ -1349,3 +1349,86 @@ int or_outflow(const or_params* p, int et, const double* xyz, int64_t nv, const 
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* NavierStokesSymBCFV1 (restated):                                    */
+/* incompressible/fv1/bnd/symmetric_boundary_fv1.cpp:224-386.          */
+/* Per boundary face (node c, area vector n, ip):                      */
+/*   momentum d1: F[d1] n[d1] with F = -nu rho (grad u + grad u^T) n   */
+/*     (component-wise product, as :373-375), plus p(ip) n[d1];        */
+/*   continuity: nothing (u.n = 0 on the boundary).                    */
+/* Jacobian (:290-317) matches; element-local, ADDED.                  */
+/* ------------------------------------------------------------------ */
+int or_symmetry_elem(const or_params* p, int et, const double* corners, int side, const double* uloc,
+                     double* Jl, double* dl)
+{
+    or_bfgeo B;
+    int rc = or_bnd_faces(et, corners, side, &B);
+    if (rc) return rc;
+    const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1, nl = nsh * nc;
+    const double nr = -1.0 * p->nu * p->rho;
+    for (int f = 0; f < B.nbf; ++f) {
+        const double* n = B.n[f];
+        const int row = B.node[f];
+        if (dl) {
+            double gv[3][3] = {{0}}, F[3] = {0, 0, 0};
+            for (int d1 = 0; d1 < dim; ++d1)
+                for (int d2 = 0; d2 < dim; ++d2)
+                    for (int sh = 0; sh < nsh; ++sh) gv[d1][d2] += B.g[f][sh][d2] * uloc[sh * nc + d1];
+            for (int d1 = 0; d1 < dim; ++d1)
+                for (int d2 = 0; d2 < dim; ++d2) F[d1] += gv[d1][d2] * n[d2];
+            if (!p->laplace)
+                for (int d1 = 0; d1 < dim; ++d1)
+                    for (int d2 = 0; d2 < dim; ++d2) F[d1] += gv[d2][d1] * n[d2];
+            double pr = 0;
+            for (int sh = 0; sh < nsh; ++sh) pr += B.phi[f][sh] * uloc[sh * nc + dim];
+            for (int d1 = 0; d1 < dim; ++d1) dl[row * nc + d1] += F[d1] * nr * n[d1] + pr * n[d1];
+        }
+        if (Jl)
+            for (int sh = 0; sh < nsh; ++sh) {
+                const double* g = B.g[f][sh];
+                const double fl = nr * dot(dim, g, n);
+                for (int d1 = 0; d1 < dim; ++d1) Jl[(row * nc + d1) * nl + sh * nc + d1] += fl * n[d1];
+                if (!p->laplace)
+                    for (int d1 = 0; d1 < dim; ++d1)
+                        for (int d2 = 0; d2 < dim; ++d2)
+                            Jl[(row * nc + d1) * nl + sh * nc + d2] += nr * g[d1] * n[d2] * n[d1];
+                for (int d1 = 0; d1 < dim; ++d1) Jl[(row * nc + d1) * nl + sh * nc + dim] += B.phi[f][sh] * n[d1];
+            }
+    }
+    return 0;
+}
+
+int or_symmetry(const or_params* p, int et, const double* xyz, int64_t nv, const int32_t* conn,
+                const int32_t* belem, const int32_t* bside, int64_t nb, const double* u, double s_a,
+                const int64_t* rowptr, const int32_t* colidx, double* J, double* d)
+{
+    if (et < 0 || et > 3) { set_err("unknown element type"); return -1; }
+    const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1, nl = nsh * nc;
+    (void)nv;
+    double co[24], ul[32], Jl[1024], dl[32];
+    for (int64_t b = 0; b < nb; ++b) {
+        const int32_t* c = conn + (int64_t)belem[b] * nsh;
+        for (int k = 0; k < nsh; ++k) {
+            for (int i = 0; i < dim; ++i) co[k * dim + i] = xyz[(int64_t)c[k] * dim + i];
+            for (int a = 0; a < nc; ++a) ul[k * nc + a] = u[(int64_t)c[k] * nc + a];
+        }
+        memset(Jl, 0, sizeof(double) * nl * nl);
+        memset(dl, 0, sizeof(double) * nl);
+        int rc = or_symmetry_elem(p, et, co, bside[b], ul, J ? Jl : NULL, d ? dl : NULL);
+        if (rc) return rc;
+        for (int i = 0; i < nsh; ++i)
+            for (int a = 0; a < nc; ++a) {
+                const int64_t row = (int64_t)c[i] * nc + a;
+                if (d) d[row] += s_a * dl[i * nc + a];
+                if (J)
+                    for (int j = 0; j < nsh; ++j)
+                        for (int bb = 0; bb < nc; ++bb) {
+                            const int64_t pos = find_col(rowptr, colidx, row, (int32_t)((int64_t)c[j] * nc + bb));
+                            if (pos < 0) { set_err("pattern misses an entry"); return -6; }
+                            J[pos] += s_a * Jl[(i * nc + a) * nl + j * nc + bb];
+                        }
+            }
+    }
+    return 0;
+}

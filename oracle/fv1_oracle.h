@@ -132,6 +132,14 @@ int or_outflow(const or_params* p, int et, const double* xyz, int64_t nv, const 
                const int32_t* belem, const int32_t* bside, int64_t nb, const double* u, double s_a,
                const int64_t* rowptr, const int32_t* colidx, double* J, double* d);
 
+/* Symmetry boundary disc (incompressible/fv1/bnd/symmetric_boundary_fv1.cpp:
+   224-386): element-local contributions of one boundary side, ADDED. */
+int or_symmetry_elem(const or_params* p, int et, const double* corners, int side, const double* uloc,
+                     double* Jl, double* dl);
+int or_symmetry(const or_params* p, int et, const double* xyz, int64_t nv, const int32_t* conn,
+                const int32_t* belem, const int32_t* bside, int64_t nb, const double* u, double s_a,
+                const int64_t* rowptr, const int32_t* colidx, double* J, double* d);
+
 #ifdef __cplusplus
 }
 #endif

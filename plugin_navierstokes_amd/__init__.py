@@ -44,7 +44,7 @@ EXPORTS = [
     "nsfv1_set_preconditioner", "nsfv1_mg_set_hierarchy", "nsfv1_mg_setup", "nsfv1_set_upwind_decisions",
     "nsfv1_mlp_sizes", "nsfv1_mlp_pack", "nsfv1_mlp_forward", "nsfv1_mlp_mse", "nsfv1_mlp_backward", "nsfv1_mlp_adam",
     "nsfv1_predict_features", "nsfv1_predict_guess", "nsfv1_predict_target", "nsfv1_batch_set_instances",
-    "nsfv1_set_assembly_path", "nsfv1_set_outflow",
+    "nsfv1_set_assembly_path", "nsfv1_set_outflow", "nsfv1_set_symmetry",
 ]
 UPWIND_FRESH, UPWIND_RECORD, UPWIND_REPLAY = 0, 1, 2
 
@@ -102,6 +102,7 @@ def load_library(path: str | None = None):
     L.nsfv1_batch_set_instances.argtypes = [H, C.c_int64, vp, vp]
     L.nsfv1_set_assembly_path.argtypes = [H, C.c_int]
     L.nsfv1_set_outflow.argtypes = [H, vp, vp, C.c_int64]
+    L.nsfv1_set_symmetry.argtypes = [H, vp, vp, C.c_int64]
     L.nsfv1_mlp_sizes.argtypes = [C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.nsfv1_mlp_pack.argtypes = [vp, vp, vp]
     L.nsfv1_mlp_forward.argtypes = [vp, vp, vp, C.c_int64, vp, vp, vp, vp]
@@ -291,6 +292,12 @@ class NavierStokesFV1:
         e = np.ascontiguousarray(elem, dtype=np.int32)
         s = np.ascontiguousarray(side, dtype=np.int32)
         _check(_lib.nsfv1_set_outflow(self._h, e.ctypes.data, s.ctypes.data, len(e)))
+
+    def set_symmetry(self, elem, side):
+        """Symmetry condition (NavierStokesSymBCFV1) on the (element, local side) list."""
+        e = np.ascontiguousarray(elem, dtype=np.int32)
+        s = np.ascontiguousarray(side, dtype=np.int32)
+        _check(_lib.nsfv1_set_symmetry(self._h, e.ctypes.data, s.ctypes.data, len(e)))
 
     def set_assembly_path(self, path: int):
         _check(_lib.nsfv1_set_assembly_path(self._h, int(path)))

@@ -112,9 +112,12 @@ struct nsfv1_handle {
     int64_t nv = 0, ne = 0, ndof = 0, nnz = 0, ninc = 0;
     DevBuf xyz, conn, rowptr, colidx, errflag, l2inv, vol, affine, node_inc, inc_elem, inc_node, inc_info, inc_pos;
     std::vector<int32_t> h_inc_info, h_inc_elem;
-    // no-normal-stress outflow boundary disc
-    DevBuf out_ptr, out_node, out_inc, out_side;
-    int64_t out_nbn = 0;
+    // boundary discs (no-normal-stress outflow, symmetry): per boundary node,
+    // its boundary faces as (incidence, side) items
+    struct BndList {
+        DevBuf ptr, node, inc, side;
+        int64_t nbn = 0;
+    } bnd[2];
     std::vector<int64_t> h_rowptr, h_node_inc;
     int max_threads = 256;
     size_t lds_cap = 64 * 1024;
@@ -477,7 +480,7 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
     h->h_node_inc.swap(cnt);
     h->h_inc_info.swap(ii);
     h->h_inc_elem.swap(ie);
-    h->out_nbn = 0;
+    h->bnd[BND_OUTFLOW].nbn = h->bnd[BND_SYMMETRY].nbn = 0;
     h->plan32.valid = h->plan64.valid = false;
     h->geo_valid = false;
     // solver state sized for the previous mesh
@@ -679,13 +682,15 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
                             (hipStream_t)stream);
     }
     if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("assemble launch: ") + hipGetErrorString(e));
-    if (h->out_nbn > 0 && (parts & NSFV1_PART_A)) {
+    for (int kind = 0; kind < 2; ++kind) {
+        const auto& B = h->bnd[kind];
+        if (B.nbn == 0 || !(parts & NSFV1_PART_A)) continue;
         OutflowArgs oa{h->xyz.as<double>(), h->conn.as<int32_t>(), h->rowptr.as<int64_t>(), h->inc_elem.as<int32_t>(),
-                       h->inc_info.as<int32_t>(), h->inc_pos.as<uint64_t>(), h->out_ptr.as<int32_t>(),
-                       h->out_node.as<int64_t>(), h->out_inc.as<int32_t>(), h->out_side.as<int32_t>(), h->out_nbn,
-                       h->nu, h->rho, s_a, h->laplace ? 1 : 0, h->stokes ? 1 : 0};
+                       h->inc_info.as<int32_t>(), h->inc_pos.as<uint64_t>(), B.ptr.as<int32_t>(), B.node.as<int64_t>(),
+                       B.inc.as<int32_t>(), B.side.as<int32_t>(), B.nbn, h->nu, h->rho, s_a, h->laplace ? 1 : 0,
+                       h->stokes ? 1 : 0, kind};
         e = launch_outflow(h->et, jprec, oa, u, jvals, defect, h->errflag.as<int>(), (hipStream_t)stream);
-        if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("outflow launch: ") + hipGetErrorString(e));
+        if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("boundary disc launch: ") + hipGetErrorString(e));
     }
     if (h->upw_mode == NSFV1_UPWIND_RECORD && (parts & NSFV1_PART_A)) h->upw_recorded = true;
     return NSFV1_OK;
@@ -726,13 +731,14 @@ static void side_corners(int s, int* n, int* c)
     for (int j = 0; j < *n; ++j) c[j] = Topo<ET>::side[s][j];
 }
 
-extern "C" {
-
-int nsfv1_set_outflow(nsfv1_handle* h, const int32_t* elem, const int32_t* side, int64_t n)
+// Groups the corners of the listed (element, side) pairs by node, keeping the
+// input order within a node, and uploads them as boundary list `kind`.
+static int set_boundary_list(nsfv1_handle* h, int kind, const int32_t* elem, const int32_t* side, int64_t n)
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
     if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
-    h->out_nbn = 0;
+    auto& B = h->bnd[kind];
+    B.nbn = 0;
     if (n == 0) return NSFV1_OK;
     if (n < 0 || !elem || !side) return fail(NSFV1_ERR_ARGUMENT, "bad boundary side arrays");
     const int nsh = h->et == ET_HEX ? 8 : (h->et == ET_TRI ? 3 : 4);
@@ -779,11 +785,23 @@ int nsfv1_set_outflow(nsfv1_handle* h, const int32_t* elem, const int32_t* side,
     auto up = [](DevBuf& b, const void* src, size_t bytes) {
         return b.alloc(bytes) == hipSuccess && hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
     };
-    if (!up(h->out_ptr, ptr.data(), sizeof(int32_t) * ptr.size()) || !up(h->out_node, nodes.data(), sizeof(int64_t) * nodes.size()) ||
-        !up(h->out_inc, iq.data(), sizeof(int32_t) * iq.size()) || !up(h->out_side, is.data(), sizeof(int32_t) * is.size()))
-        return fail(NSFV1_ERR_DEVICE, "outflow upload");
-    h->out_nbn = (int64_t)nodes.size();
+    if (!up(B.ptr, ptr.data(), sizeof(int32_t) * ptr.size()) || !up(B.node, nodes.data(), sizeof(int64_t) * nodes.size()) ||
+        !up(B.inc, iq.data(), sizeof(int32_t) * iq.size()) || !up(B.side, is.data(), sizeof(int32_t) * is.size()))
+        return fail(NSFV1_ERR_DEVICE, "boundary list upload");
+    B.nbn = (int64_t)nodes.size();
     return NSFV1_OK;
+}
+
+extern "C" {
+
+int nsfv1_set_outflow(nsfv1_handle* h, const int32_t* elem, const int32_t* side, int64_t n)
+{
+    return set_boundary_list(h, BND_OUTFLOW, elem, side, n);
+}
+
+int nsfv1_set_symmetry(nsfv1_handle* h, const int32_t* elem, const int32_t* side, int64_t n)
+{
+    return set_boundary_list(h, BND_SYMMETRY, elem, side, n);
 }
 
 int nsfv1_set_assembly_path(nsfv1_handle* h, int path)

@@ -1,6 +1,8 @@
-// nsfv1_boundary.hip -- FV1 boundary element disc: the no-normal-stress
-// outflow condition (incompressible/fv1/bnd/no_normal_stress_outflow_fv1.cpp:
-// 188-426) added to the rows of the nodes on the listed boundary sides.
+// nsfv1_boundary.hip -- FV1 boundary element discs added to the rows of the
+// nodes on the listed boundary sides: the no-normal-stress outflow condition
+// (incompressible/fv1/bnd/no_normal_stress_outflow_fv1.cpp:188-426) and the
+// symmetry condition (incompressible/fv1/bnd/symmetric_boundary_fv1.cpp:
+// 224-386).
 //
 // Boundary faces (ugcore FV1Geometry BF, restated): on side s of an element,
 // corner c owns the part of the side inside its sub-control volume --
@@ -15,6 +17,12 @@
 //               outgoing convective flux max(rho u.n, 0) u;
 //   continuity: rho u.n;
 // and the matching fixpoint Jacobian (:291-311, 360-368).
+//
+// The symmetry condition adds per boundary face
+//   momentum d: F[d] n[d] with F = -nu rho (grad u + grad u^T) n (the
+//               component-wise product of :373-375) plus p(ip) n[d];
+//   continuity: nothing (u.n = 0 there);
+// and its Jacobian (:290-317).  It is linear in u.
 //
 // One lane per boundary node walks that node's boundary faces in a fixed
 // order and adds into its own rows of J and d (read-modify-write of rows no
@@ -225,6 +233,50 @@ outflow_rows(OutflowArgs A, const double* __restrict__ u, JT* __restrict__ J, do
         double flux = vn * rho;
         if (flux < 0) flux = 0;
         const double sa = A.s_a;
+        if (A.kind == BND_SYMMETRY) {
+            if (dout) {
+                double gv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, F[3] = {0, 0, 0}, pr = 0;
+#pragma unroll
+                for (int d1 = 0; d1 < D; ++d1)
+#pragma unroll
+                    for (int d2 = 0; d2 < D; ++d2)
+#pragma unroll
+                        for (int k = 0; k < N; ++k) gv[d1][d2] += g[k][d2] * ul[k][d1];
+#pragma unroll
+                for (int d1 = 0; d1 < D; ++d1)
+#pragma unroll
+                    for (int d2 = 0; d2 < D; ++d2) F[d1] += gv[d1][d2] * n[d2];
+                if (!A.laplace)
+#pragma unroll
+                    for (int d1 = 0; d1 < D; ++d1)
+#pragma unroll
+                        for (int d2 = 0; d2 < D; ++d2) F[d1] += gv[d2][d1] * n[d2];
+#pragma unroll
+                for (int k = 0; k < N; ++k) pr += phi[k] * ul[k][D];
+#pragma unroll
+                for (int d1 = 0; d1 < D; ++d1) dsum[d1] += sa * (F[d1] * (-nu * rho) * n[d1] + pr * n[d1]);
+            }
+            if (J) {
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    const int col = (int)((pw >> (8 * k)) & 0xffu) * NC;
+                    double gn = 0;
+#pragma unroll
+                    for (int i = 0; i < D; ++i) gn += g[k][i] * n[i];
+#pragma unroll
+                    for (int d1 = 0; d1 < D; ++d1) {
+#pragma unroll
+                        for (int d2 = 0; d2 < D; ++d2) {
+                            double val = (d1 == d2) ? gn * n[d1] : 0.0;
+                            if (!A.laplace) val += g[k][d1] * n[d2] * n[d1];
+                            J[rows[d1] + col + d2] += JT(sa * (-nu * rho) * val);
+                        }
+                        J[rows[d1] + col + D] += JT(sa * (phi[k] * n[d1]));
+                    }
+                }
+            }
+            continue;
+        }
         // defect
         if (dout) {
             double gv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, F[3] = {0, 0, 0};

@@ -1,5 +1,5 @@
-// nsfv1_boundary.hpp -- launcher of the FV1 no-normal-stress outflow
-// boundary disc (rows of the nodes on the listed boundary sides).
+// nsfv1_boundary.hpp -- launcher of the FV1 boundary discs (no-normal-stress
+// outflow, symmetry) on the rows of the nodes of the listed boundary sides.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,7 +21,10 @@ struct OutflowArgs {
     int64_t nbn;
     double nu, rho, s_a;
     int laplace, stokes;
+    int kind;  // BND_OUTFLOW or BND_SYMMETRY
 };
+
+enum { BND_OUTFLOW = 0, BND_SYMMETRY = 1 };
 
 hipError_t launch_outflow(int et, int jprec, const OutflowArgs& a, const double* u, void* J, double* d, int* err,
                           hipStream_t s);

@@ -1,6 +1,8 @@
-"""GPU parity of the no-normal-stress outflow boundary disc
-(incompressible/fv1/bnd/no_normal_stress_outflow_fv1.cpp:188-426): element
-assembly plus outflow rows through the C ABI against the CPU oracle."""
+"""GPU parity of the FV1 boundary discs -- no-normal-stress outflow
+(incompressible/fv1/bnd/no_normal_stress_outflow_fv1.cpp:188-426) and
+symmetry (incompressible/fv1/bnd/symmetric_boundary_fv1.cpp:224-386):
+element assembly plus boundary rows through the C ABI against the CPU
+oracle."""
 from __future__ import annotations
 
 import numpy as np
@@ -26,11 +28,18 @@ def _case(et):
     return mesh, np.concatenate([be1, be2]), np.concatenate([bs1, bs2])
 
 
+def _bottom(mesh):
+    """Sides on y = min (the symmetry plane of the combined case)."""
+    y = mesh.xyz[:, 1]
+    return M.boundary_sides(mesh, np.nonzero(np.isclose(y, y.min()))[0])
+
+
 @pytest.mark.parametrize("cfg", [{"upwind": "full"}, {"upwind": "lps", "laplace": True}, {"upwind": "full", "stokes": True}],
                          ids=["full", "lps-laplace", "stokes"])
 @pytest.mark.parametrize("jprec", [F64, F32])
 @pytest.mark.parametrize("et", ETS)
-def test_outflow_parity(cuda, et, jprec, cfg):
+@pytest.mark.parametrize("kind", ["outflow", "symmetry", "both"])
+def test_outflow_parity(cuda, et, jprec, cfg, kind):
     import torch
 
     mesh, belem, bside = _case(et)
@@ -39,14 +48,26 @@ def test_outflow_parity(cuda, et, jprec, cfg):
     disc = NavierStokesFV1("u,v,p" if mesh.dim == 2 else "u,v,w,p", "Inner")
     configure_disc(disc, cfg)
     disc.set_mesh(mesh)
-    disc.set_outflow(belem, bside)
-    J, d = disc.assemble(torch.from_numpy(u).to(cuda), s_a=0.8, parts=PART_A, jprec=jprec)
-    J, d = J.cpu().numpy().astype(np.float64), d.cpu().numpy()
     rp, ci = disc.pattern()
     p = oracle_params(mesh.dim, cfg)
+    Jo, do = 0.0, 0.0
+    if kind in ("outflow", "both"):
+        disc.set_outflow(belem, bside)
+        Jx, dx = O.outflow(p, et, mesh.xyz, mesh.conn, belem, bside, u, s_a=0.8, rowptr=rp, colidx=ci)
+        Jo, do = Jo + np.abs(Jx), do + np.abs(dx)
+        Jr_b, dr_b = Jx, dx
+    else:
+        Jr_b, dr_b = 0.0, 0.0
+    if kind in ("symmetry", "both"):
+        se, ss = (belem, bside) if kind == "symmetry" else _bottom(mesh)
+        disc.set_symmetry(se, ss)
+        Jx, dx = O.outflow(p, et, mesh.xyz, mesh.conn, se, ss, u, s_a=0.8, rowptr=rp, colidx=ci, kind="symmetry")
+        Jo, do = Jo + np.abs(Jx), do + np.abs(dx)
+        Jr_b, dr_b = Jr_b + Jx, dr_b + dx
+    J, d = disc.assemble(torch.from_numpy(u).to(cuda), s_a=0.8, parts=PART_A, jprec=jprec)
+    J, d = J.cpu().numpy().astype(np.float64), d.cpu().numpy()
     Jr, dr = O.assemble(p, et, mesh.xyz, mesh.conn, u, s_a=0.8, parts=1, rowptr=rp, colidx=ci)
     Ja, da = O.assemble_abs(p, et, mesh.xyz, mesh.conn, u, s_a=0.8, parts=1, rowptr=rp, colidx=ci)
-    Jo, do = O.outflow(p, et, mesh.xyz, mesh.conn, belem, bside, u, s_a=0.8, rowptr=rp, colidx=ci)
     assert np.abs(Jo).max() > 0 and np.abs(do).max() > 0
     # per-row magnitude of the outflow terms as their rounding scale
     rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
@@ -54,7 +75,7 @@ def test_outflow_parity(cuda, et, jprec, cfg):
     np.maximum.at(rmax, rows, np.abs(Jo))
     SJ = Ja + rmax[rows]
     Sd = da + np.abs(do)
-    Jw, dw = Jr + Jo, dr + do
+    Jw, dw = Jr + Jr_b, dr + dr_b
     if jprec == F64:
         tolJ = 1e-12 * np.abs(Jw) + 1e-13 * SJ
     else:
@@ -62,7 +83,7 @@ def test_outflow_parity(cuda, et, jprec, cfg):
     told = 1e-12 * np.abs(dw) + 1e-13 * Sd
     vJ = np.max(np.abs(J - Jw) / (tolJ + 1e-300))
     vd = np.max(np.abs(d - dw) / (told + 1e-300))
-    print(f"et={et} jprec={jprec} cfg={cfg}: violation J {vJ:.3f} d {vd:.3f}, outflow sides {len(belem)}")
+    print(f"et={et} jprec={jprec} cfg={cfg} kind={kind}: violation J {vJ:.3f} d {vd:.3f}, outflow sides {len(belem)}")
     assert vJ <= 1.0 and vd <= 1.0
 
 
@@ -81,3 +102,9 @@ def test_outflow_cleared(cuda):
     J2, d2 = disc.assemble(u)
     assert not torch.equal(d0, d1)
     assert torch.equal(J0, J2) and torch.equal(d0, d2)
+    disc.set_symmetry(belem, bside)
+    _, d3 = disc.assemble(u)
+    disc.set_symmetry([], [])
+    J4, d4 = disc.assemble(u)
+    assert not torch.equal(d0, d3)
+    assert torch.equal(J0, J4) and torch.equal(d0, d4)

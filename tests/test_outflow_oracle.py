@@ -72,16 +72,17 @@ def test_boundary_face_areas_and_ips(et):
         assert np.all(np.abs(np.einsum("ij,ij->i", b["ipg"] - pts[0], nrm)) <= 1e-14)
 
 
+@pytest.mark.parametrize("kind", ["outflow", "symmetry"])
 @pytest.mark.parametrize("et", ETS)
 @pytest.mark.parametrize("flags", [{}, {"laplace": True}, {"stokes": True}])
-def test_outflow_picard_identity(et, flags):
+def test_outflow_picard_identity(et, flags, kind):
     co = _elem(et, 7)
     rng = np.random.default_rng(3)
     nsh, nc = O.NSH[et], O.DIM[et] + 1
     p = O.Params(dim=O.DIM[et], upwind="full", nu=0.03, rho=1.4, **flags)
     for s in range(len(M.SIDES[et])):
         u = rng.normal(size=nsh * nc)
-        J, d = O.outflow_elem(p, et, co, s, u)
+        J, d = O.outflow_elem(p, et, co, s, u, kind)
         np.testing.assert_allclose(J @ u, d, rtol=1e-12, atol=1e-14 * np.abs(J).max())
         # rows of corners not on the side stay empty
         on = set(M.SIDES[et][s])
@@ -106,3 +107,28 @@ def test_outflow_finite_differences(et):
         e[j] = h
         fd = (O.outflow_elem(p, et, co, 0, u + e)[1] - O.outflow_elem(p, et, co, 0, u - e)[1]) / (2 * h)
         np.testing.assert_allclose(fd, J[:, j], rtol=1e-6, atol=1e-8)
+
+
+@pytest.mark.parametrize("et", ETS)
+def test_symmetry_linear_and_pressure(et):
+    """The symmetry condition is linear in u; a constant pressure p0 adds
+    p0 n[d] to momentum row d of the face's node, nothing else."""
+    co = _elem(et, 13)
+    nsh, nc, dim = O.NSH[et], O.DIM[et] + 1, O.DIM[et]
+    p = O.Params(dim=dim, upwind="full", nu=0.02, rho=1.1)
+    rng = np.random.default_rng(6)
+    u, w = rng.normal(size=nsh * nc), rng.normal(size=nsh * nc)
+    for s in range(len(M.SIDES[et])):
+        Ju, du = O.outflow_elem(p, et, co, s, u, "symmetry")
+        Jw, dw = O.outflow_elem(p, et, co, s, w, "symmetry")
+        _, duw = O.outflow_elem(p, et, co, s, 2.0 * u - 0.5 * w, "symmetry")
+        np.testing.assert_allclose(duw, 2.0 * du - 0.5 * dw, rtol=1e-12, atol=1e-13)
+        assert np.array_equal(Ju, Jw)
+        q = np.zeros(nsh * nc)
+        q[dim::nc] = 3.0
+        _, dq = O.outflow_elem(p, et, co, s, q, "symmetry")
+        b = O.bnd_faces(et, co, s)
+        want = np.zeros(nsh * nc)
+        for j, c in enumerate(b["node"]):
+            want[c * nc:c * nc + dim] += 3.0 * b["n"][j]
+        np.testing.assert_allclose(dq, want, rtol=1e-13, atol=1e-14)
