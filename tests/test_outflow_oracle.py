@@ -1,8 +1,11 @@
-"""Pins the oracle's FV1 boundary faces and its no-normal-stress outflow
-condition (incompressible/fv1/bnd/no_normal_stress_outflow_fv1.cpp:188-426):
-the discrete divergence theorem on every sub-control volume (interior
-SCVF normals plus boundary-face normals close), boundary-face areas,
-the Picard identity d = J u, and finite differences."""
+"""Pins the oracle's FV1 boundary faces, its no-normal-stress outflow
+condition (incompressible/fv1/bnd/no_normal_stress_outflow_fv1.cpp:188-426)
+and its symmetry condition (incompressible/fv1/bnd/symmetric_boundary_fv1.cpp:
+224-386): the discrete divergence theorem on every sub-control volume
+(interior SCVF normals plus boundary-face normals close), boundary-face
+areas, the Picard identity d = J u, finite differences, linearity, and the
+closed form of the symmetry stress for a linear velocity on an axis-aligned
+side."""
 from __future__ import annotations
 
 import numpy as np
@@ -132,3 +135,52 @@ def test_symmetry_linear_and_pressure(et):
         for j, c in enumerate(b["node"]):
             want[c * nc:c * nc + dim] += 3.0 * b["n"][j]
         np.testing.assert_allclose(dq, want, rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("laplace", [False, True])
+@pytest.mark.parametrize("et", [O.QUAD, O.HEX])
+def test_symmetry_closed_form_axis_aligned(et, laplace):
+    """Undistorted box element of size h, linear velocity u = A x: on side s
+    with outward axis e_k every corner owns a face of area a = (h/2)^(dim-1),
+    n = a e_k, so momentum row k of each side corner gets
+    -nu rho a^2 (A_kk + A_kk [not laplace]) and the other momentum rows 0."""
+    dim = O.DIM[et]
+    h = 0.7
+    lc = {O.QUAD: [[0, 0], [1, 0], [1, 1], [0, 1]],
+          O.HEX: [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]]}[et]
+    co = np.asarray(lc, dtype=float) * h + 0.3
+    nsh, nc = O.NSH[et], dim + 1
+    rng = np.random.default_rng(9)
+    A = rng.normal(size=(dim, dim))
+    u = np.zeros((nsh, nc))
+    u[:, :dim] = co @ A.T
+    nu, rho = 0.03, 1.4
+    p = O.Params(dim=dim, upwind="full", nu=nu, rho=rho, laplace=laplace)
+    a = (h / 2) ** (dim - 1)
+    for s in range(len(M.SIDES[et])):
+        side = list(M.SIDES[et][s])
+        cen = co[side].mean(axis=0)
+        k = int(np.argmax(np.abs(cen - co.mean(axis=0))))
+        _, d = O.outflow_elem(p, et, co, s, u.reshape(-1), "symmetry")
+        d = d.reshape(nsh, nc)
+        want = np.zeros((nsh, nc))
+        for c in side:
+            want[c, k] = -nu * rho * a * a * A[k, k] * (1 if laplace else 2)
+        np.testing.assert_allclose(d, want, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("et", ETS)
+def test_symmetry_finite_differences(et):
+    """The symmetry defect is linear: central differences equal J."""
+    co = _elem(et, 12)
+    rng = np.random.default_rng(5)
+    nsh, nc = O.NSH[et], O.DIM[et] + 1
+    p = O.Params(dim=O.DIM[et], upwind="full", nu=0.05, rho=1.2)
+    u = rng.normal(size=nsh * nc)
+    J, _ = O.outflow_elem(p, et, co, 1, u, "symmetry")
+    h = 1e-3
+    for j in range(nsh * nc):
+        e = np.zeros(nsh * nc)
+        e[j] = h
+        fd = (O.outflow_elem(p, et, co, 1, u + e, "symmetry")[1] - O.outflow_elem(p, et, co, 1, u - e, "symmetry")[1]) / (2 * h)
+        np.testing.assert_allclose(fd, J[:, j], rtol=1e-9, atol=1e-11)
