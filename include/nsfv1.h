@@ -200,6 +200,31 @@ int nsfv1_set_outflow(nsfv1_handle* h, const int32_t* elem, const int32_t* side,
    master's viscosity, density and Laplace flag apply. */
 int nsfv1_set_symmetry(nsfv1_handle* h, const int32_t* elem, const int32_t* side, int64_t n);
 
+/* Data exports of the disc (navier_stokes_fv1.cpp:894-1491; the export
+   objects of navier_stokes_fv1.cpp:1573-1577): values at points of every
+   element, written to the device array out[ne][points][components] (FP64)
+   on `stream`.
+     what  NSFV1_EX_VELOCITY       ex_nodal_velocity   (dim components)
+           NSFV1_EX_VELOCITY_GRAD  ex_velocity_grad    (dim*dim, row d1 = velocity component, column d2)
+           NSFV1_EX_VELOCITY_IP    ex_velocity_ip      (dim; at SCVF ips the stabilised velocity, which
+                                                        needs the stabilisation as the assembly does and,
+                                                        time-dependent, u_old and dt > 0)
+           NSFV1_EX_PRESSURE       ex_nodal_pressure   (1)
+           NSFV1_EX_PRESSURE_GRAD  ex_pressure_grad    (dim)
+     at    NSFV1_AT_SCVF   the element's FV1 SCVF ips (the reference's special case)
+           NSFV1_AT_SCV    its SCV ips, i.e. its corners
+           NSFV1_AT_LOCAL  the nip (1..32) local points local_ips[nip][dim] (host array), the
+                           reference's general case; its velocity gradient there is taken in
+                           reference-element coordinates, as ex_velocity_grad :1072-1080 does.
+   The derivatives (bDeriv) are the shape values and gradients themselves
+   and are not exported.  nsfv1_export_size gives the number of doubles. */
+enum { NSFV1_EX_VELOCITY = 0, NSFV1_EX_VELOCITY_GRAD = 1, NSFV1_EX_VELOCITY_IP = 2, NSFV1_EX_PRESSURE = 3,
+       NSFV1_EX_PRESSURE_GRAD = 4 };
+enum { NSFV1_AT_SCVF = 0, NSFV1_AT_SCV = 1, NSFV1_AT_LOCAL = 2 };
+int nsfv1_export_size(nsfv1_handle* h, int what, int at, int nip, int64_t* count);
+int nsfv1_export(nsfv1_handle* h, int what, int at, const double* local_ips, int nip, const double* u,
+                 const double* u_old, double dt, double* out, void* stream);
+
 /* Synchronises `stream` and reports (then clears) errors raised inside the
    kernels since the last call: NSFV1_ERR_CUT_SIDE when a skewed/LPS upwind
    ray found no side (upwind.cpp:354, 422, 550), NSFV1_ERR_DEGENERATE for an

@@ -132,6 +132,10 @@ def lib():
         L.or_symmetry_elem.argtypes = [C.POINTER(_Params), C.c_int, dp, C.c_int, dp, dp, dp]
         L.or_symmetry.argtypes = [C.POINTER(_Params), C.c_int, dp, C.c_int64, ip32, ip32, ip32, C.c_int64, dp,
                                   C.c_double, ip64, ip32, dp, dp]
+        L.or_export_elem.argtypes = [C.POINTER(_Params), C.c_int, dp, dp, dp, C.c_double, C.c_int, C.c_int, dp,
+                                     C.c_int, dp]
+        L.or_export.argtypes = [C.POINTER(_Params), C.c_int, dp, ip32, C.c_int64, dp, dp, C.c_double, C.c_int, C.c_int,
+                                dp, C.c_int, dp]
         L.or_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -313,3 +317,33 @@ def outflow(p: Params, et: int, xyz, conn, belem, bside, u, s_a=1.0, rowptr=None
                           bside.ctypes.data_as(i32), len(belem), _dp(u), float(s_a),
                           rowptr.ctypes.data_as(C.POINTER(C.c_int64)), colidx.ctypes.data_as(i32), _dp(J), _dp(d)))
     return J, d
+
+
+EXPORTS = {"velocity": 0, "velocity_grad": 1, "velocity_ip": 2, "pressure": 3, "pressure_grad": 4}
+AT = {"scvf": 0, "scv": 1, "local": 2}
+
+
+def export_shape(et: int, what: str, at: str, nip: int = 0):
+    """(points per element, components) of an export."""
+    dim = DIM[et]
+    npt = {"scvf": NIP[et], "scv": NSH[et], "local": nip}[at]
+    ncomp = {"velocity": dim, "velocity_grad": dim * dim, "velocity_ip": dim, "pressure": 1,
+             "pressure_grad": dim}[what]
+    return npt, ncomp
+
+
+def export(p: Params, et: int, xyz, conn, u, what: str, at: str = "scvf", local_ips=None, u_old=None, dt=0.0):
+    """Data export of every element: [ne][points][components]."""
+    xyz, u = _f64(xyz), _f64(u)
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    ne = conn.shape[0]
+    lips = _f64(local_ips if local_ips is not None else np.zeros((1, DIM[et])))
+    nip = 0 if local_ips is None else lips.shape[0]
+    npt, ncomp = export_shape(et, what, at, nip)
+    out = np.zeros(ne * npt * ncomp)
+    pc = p.c()
+    uo = _f64(u_old) if u_old is not None else None
+    _chk(lib().or_export(C.byref(pc), et, _dp(xyz), conn.ctypes.data_as(C.POINTER(C.c_int32)), ne, _dp(u),
+                         _dp(uo) if uo is not None else None, float(dt), EXPORTS[what], AT[at], _dp(lips), nip,
+                         _dp(out)))
+    return out.reshape(ne, npt, ncomp)

@@ -1432,3 +1432,159 @@ int or_symmetry(const or_params* p, int et, const double* xyz, int64_t nv, const
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* Data exports (restated): navier_stokes_fv1.cpp:894-1491.            */
+/* what: 0 velocity (ex_nodal_velocity :898-990), 1 velocity gradient  */
+/* (ex_velocity_grad :997-1098), 2 stabilised ip velocity              */
+/* (ex_velocity_ip :1105-1278), 3 pressure (ex_nodal_pressure          */
+/* :1285-1392), 4 pressure gradient (ex_pressure_grad :1399-1491).     */
+/* at: 0 the element's SCVF ips, 1 its SCV ips (the corners), 2 the    */
+/* nip local points lips[nip][dim] ("general case").                   */
+/* out: [nip][ncomp], ncomp = dim, dim*dim (row d1, column d2), dim, 1,*/
+/* dim.                                                                */
+/* ------------------------------------------------------------------ */
+int or_export_ncomp(int what, int dim)
+{
+    switch (what) {
+    case 0: case 2: case 4: return dim;
+    case 1: return dim * dim;
+    case 3: return 1;
+    default: return -1;
+    }
+}
+
+/* shapes, reference and global gradients at local point x */
+static int point_grads(int et, const double (*co)[3], const double* x, double* phi, double (*gl)[3], double (*g)[3])
+{
+    const int dim = REF[et].dim, nsh = REF[et].nsh;
+    shapes(et, x, phi, gl);
+    double J[3][3] = {{0}}, Ji[3][3] = {{0}};
+    for (int i = 0; i < dim; ++i)
+        for (int j = 0; j < dim; ++j)
+            for (int k = 0; k < nsh; ++k) J[i][j] += co[k][i] * gl[k][j];
+    double det;
+    if (dim == 2) {
+        det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        if (det == 0.0) { set_err("degenerate element"); return -2; }
+        Ji[0][0] = J[1][1] / det;  Ji[0][1] = -J[0][1] / det;
+        Ji[1][0] = -J[1][0] / det; Ji[1][1] = J[0][0] / det;
+    } else {
+        det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+              J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        if (det == 0.0) { set_err("degenerate element"); return -2; }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                /* cofactor of J[j][i] / det */
+                const int a0 = (j + 1) % 3, a1 = (j + 2) % 3, b0 = (i + 1) % 3, b1 = (i + 2) % 3;
+                Ji[i][j] = (J[a0][b0] * J[a1][b1] - J[a0][b1] * J[a1][b0]) / det;
+            }
+    }
+    for (int k = 0; k < nsh; ++k)
+        for (int i = 0; i < dim; ++i) {
+            double s = 0;
+            for (int j = 0; j < dim; ++j) s += Ji[j][i] * gl[k][j];
+            g[k][i] = s;
+        }
+    return 0;
+}
+
+int or_export_elem(const or_params* p, int et, const double* corners, const double* uloc, const double* uold,
+                   double dt, int what, int at, const double* lips, int nip, double* out)
+{
+    if (et < 0 || et > 3) { set_err("unknown element type"); return -1; }
+    const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1, P = dim;
+    const int ncomp = or_export_ncomp(what, dim);
+    if (ncomp < 0 || at < 0 || at > 2) { set_err("unknown export"); return -1; }
+    or_geo G;
+    int rc = or_geometry(et, corners, &G);
+    if (rc) return rc;
+    if (at == 0) {
+        if (what == 2) {
+            /* the stabilisation's update only (ex_velocity_ip :1146-1189) */
+            double Ust[12][8] = {{0}}, lst[12], stdvel[12][3];
+            stabdata S;
+            for (int ip = 0; ip < G.nip; ++ip) {
+                lst[ip] = 1.0;
+                for (int d = 0; d < dim; ++d) {
+                    stdvel[ip][d] = 0.0;
+                    for (int sh = 0; sh < nsh; ++sh) stdvel[ip][d] += UL(d, sh) * G.phi[ip][sh];
+                }
+            }
+            if (p->stab_type < 0) { set_err("Stabilization has not been set."); return -3; }
+            if (!p->stokes) {
+                if (p->stab_upwind < 0) { set_err("No upwind object"); return -4; }
+                rc = upwind_geo(p->stab_upwind, &G, (const double (*)[3])stdvel, Ust, lst);
+                if (rc) return rc;
+            }
+            rc = stab_update(p, &G, uloc, uold, dt, (const double (*)[3])stdvel, (const double (*)[8])Ust, lst, &S);
+            if (rc) return rc;
+            for (int ip = 0; ip < G.nip; ++ip)
+                for (int d = 0; d < dim; ++d) out[ip * ncomp + d] = S.vel[ip][d];
+            return 0;
+        }
+        for (int ip = 0; ip < G.nip; ++ip) {
+            double* o = out + ip * ncomp;
+            for (int c = 0; c < ncomp; ++c) o[c] = 0.0;
+            for (int sh = 0; sh < nsh; ++sh) {
+                if (what == 0)
+                    for (int d = 0; d < dim; ++d) o[d] += UL(d, sh) * G.phi[ip][sh];
+                else if (what == 1)
+                    for (int d1 = 0; d1 < dim; ++d1)
+                        for (int d2 = 0; d2 < dim; ++d2) o[d1 * dim + d2] += UL(d1, sh) * G.g[ip][sh][d2];
+                else if (what == 3)
+                    o[0] += UL(P, sh) * G.phi[ip][sh];
+                else
+                    for (int d = 0; d < dim; ++d) o[d] += UL(P, sh) * G.g[ip][sh][d];
+            }
+        }
+        return 0;
+    }
+    const int npt = at == 1 ? nsh : nip;
+    for (int q = 0; q < npt; ++q) {
+        double x[3] = {0, 0, 0};
+        for (int i = 0; i < dim; ++i) x[i] = at == 1 ? REF[et].lc[q][i] : lips[q * dim + i];
+        double phi[8], gl[8][3], g[8][3];
+        rc = point_grads(et, (const double (*)[3])G.co, x, phi, gl, g);
+        if (rc) return rc;
+        double* o = out + q * ncomp;
+        for (int c = 0; c < ncomp; ++c) o[c] = 0.0;
+        if (what == 3 && at == 1) { o[0] = UL(P, q); continue; }  /* SCV branch :1342-1364 */
+        for (int sh = 0; sh < nsh; ++sh) {
+            if (what == 0 || what == 2)
+                for (int d = 0; d < dim; ++d) o[d] += UL(d, sh) * phi[sh];
+            else if (what == 1)  /* reference-element gradient, as :1072-1080 */
+                for (int d1 = 0; d1 < dim; ++d1)
+                    for (int d2 = 0; d2 < dim; ++d2) o[d1 * dim + d2] += UL(d1, sh) * gl[sh][d2];
+            else if (what == 3)
+                o[0] += UL(P, sh) * phi[sh];
+            else
+                for (int d = 0; d < dim; ++d) o[d] += UL(P, sh) * g[sh][d];
+        }
+    }
+    return 0;
+}
+
+int or_export(const or_params* p, int et, const double* xyz, const int32_t* conn, int64_t ne, const double* u,
+              const double* uold, double dt, int what, int at, const double* lips, int nip, double* out)
+{
+    if (et < 0 || et > 3) { set_err("unknown element type"); return -1; }
+    const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1;
+    const int npt = at == 0 ? REF[et].nip : (at == 1 ? nsh : nip);
+    const int ncomp = or_export_ncomp(what, dim);
+    if (ncomp < 0) { set_err("unknown export"); return -1; }
+    double co[24], ul[32], uo[32];
+    for (int64_t e = 0; e < ne; ++e) {
+        const int32_t* c = conn + e * nsh;
+        for (int k = 0; k < nsh; ++k) {
+            for (int i = 0; i < dim; ++i) co[k * dim + i] = xyz[(int64_t)c[k] * dim + i];
+            for (int a = 0; a < nc; ++a) {
+                ul[k * nc + a] = u[(int64_t)c[k] * nc + a];
+                if (uold) uo[k * nc + a] = uold[(int64_t)c[k] * nc + a];
+            }
+        }
+        int rc = or_export_elem(p, et, co, ul, uold ? uo : NULL, dt, what, at, lips, nip, out + e * npt * ncomp);
+        if (rc) return rc;
+    }
+    return 0;
+}

@@ -140,6 +140,16 @@ int or_symmetry(const or_params* p, int et, const double* xyz, int64_t nv, const
                 const int32_t* belem, const int32_t* bside, int64_t nb, const double* u, double s_a,
                 const int64_t* rowptr, const int32_t* colidx, double* J, double* d);
 
+/* Data exports (navier_stokes_fv1.cpp:894-1491).  what: 0 velocity,
+   1 velocity gradient, 2 stabilised ip velocity, 3 pressure, 4 pressure
+   gradient; at: 0 SCVF ips, 1 SCV ips (corners), 2 the local points
+   lips[nip][dim].  out: [ne][points][ncomp]. */
+int or_export_ncomp(int what, int dim);
+int or_export_elem(const or_params* p, int et, const double* corners, const double* uloc, const double* uold,
+                   double dt, int what, int at, const double* lips, int nip, double* out);
+int or_export(const or_params* p, int et, const double* xyz, const int32_t* conn, int64_t ne, const double* u,
+              const double* uold, double dt, int what, int at, const double* lips, int nip, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,8 +44,11 @@ EXPORTS = [
     "nsfv1_set_preconditioner", "nsfv1_mg_set_hierarchy", "nsfv1_mg_setup", "nsfv1_set_upwind_decisions",
     "nsfv1_mlp_sizes", "nsfv1_mlp_pack", "nsfv1_mlp_forward", "nsfv1_mlp_mse", "nsfv1_mlp_backward", "nsfv1_mlp_adam",
     "nsfv1_predict_features", "nsfv1_predict_guess", "nsfv1_predict_target", "nsfv1_batch_set_instances",
-    "nsfv1_set_assembly_path", "nsfv1_set_outflow", "nsfv1_set_symmetry",
+    "nsfv1_set_assembly_path", "nsfv1_set_outflow", "nsfv1_set_symmetry", "nsfv1_export_size", "nsfv1_export",
 ]
+# data exports (navier_stokes_fv1.cpp:1573-1577) and their points
+EX = {"velocity": 0, "velocity_grad": 1, "velocity_ip": 2, "pressure": 3, "pressure_grad": 4}
+AT = {"scvf": 0, "scv": 1, "local": 2}
 UPWIND_FRESH, UPWIND_RECORD, UPWIND_REPLAY = 0, 1, 2
 
 
@@ -103,6 +106,8 @@ def load_library(path: str | None = None):
     L.nsfv1_set_assembly_path.argtypes = [H, C.c_int]
     L.nsfv1_set_outflow.argtypes = [H, vp, vp, C.c_int64]
     L.nsfv1_set_symmetry.argtypes = [H, vp, vp, C.c_int64]
+    L.nsfv1_export_size.argtypes = [H, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64)]
+    L.nsfv1_export.argtypes = [H, C.c_int, C.c_int, vp, C.c_int, vp, vp, C.c_double, vp, vp]
     L.nsfv1_mlp_sizes.argtypes = [C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.nsfv1_mlp_pack.argtypes = [vp, vp, vp]
     L.nsfv1_mlp_forward.argtypes = [vp, vp, vp, C.c_int64, vp, vp, vp, vp]
@@ -222,6 +227,7 @@ class NavierStokesFV1:
         _check(_lib.nsfv1_set_mesh(self._h, int(et), xyz.ctypes.data, xyz.shape[0], conn.ctypes.data, conn.shape[0]))
         self.nv = xyz.shape[0]
         self.ne = conn.shape[0]
+        self.dim = 2 if int(et) in (0, 1) else 3
 
     def num_dofs(self) -> int:
         n = C.c_int64()
@@ -298,6 +304,27 @@ class NavierStokesFV1:
         e = np.ascontiguousarray(elem, dtype=np.int32)
         s = np.ascontiguousarray(side, dtype=np.int32)
         _check(_lib.nsfv1_set_symmetry(self._h, e.ctypes.data, s.ctypes.data, len(e)))
+
+    def export(self, u, what: str, at: str = "scvf", local_ips=None, u_old=None, dt=0.0, out=None, stream=None,
+               check=True):
+        """Data export ("velocity", "velocity_grad", "velocity_ip", "pressure",
+        "pressure_grad") at the SCVF ips, the corners ("scv") or the given
+        local points of every element: a CUDA float64 tensor
+        [ne, points, components]."""
+        import torch
+
+        lips = None if local_ips is None else np.ascontiguousarray(local_ips, dtype=np.float64)
+        nip = 0 if lips is None else lips.shape[0]
+        cnt = C.c_int64(0)
+        _check(_lib.nsfv1_export_size(self._h, EX[what], AT[at], nip, C.byref(cnt)))
+        if out is None:
+            out = torch.empty(cnt.value, dtype=torch.float64, device=u.device)
+        _check(_lib.nsfv1_export(self._h, EX[what], AT[at], None if lips is None else lips.ctypes.data, nip, _ptr(u),
+                                 _ptr(u_old), float(dt), _ptr(out), _stream_ptr(stream)))
+        if check:
+            self.sync_errors(stream)
+        ncomp = self.dim * self.dim if what == "velocity_grad" else (1 if what == "pressure" else self.dim)
+        return out.view(self.ne, -1, ncomp)
 
     def set_assembly_path(self, path: int):
         _check(_lib.nsfv1_set_assembly_path(self._h, int(path)))

@@ -21,6 +21,7 @@
 #include "nsfv1_solver.hpp"
 #include "nsfv1_mlp.hpp"
 #include "nsfv1_boundary.hpp"
+#include "nsfv1_export.hpp"
 #include "nsfv1_topo.hpp"
 
 using namespace nsfv1;
@@ -596,6 +597,20 @@ static int make_plan(nsfv1_handle* h, int jbytes, Plan& P)
     return NSFV1_OK;
 }
 
+// per-element SCV volumes and diffusion-length factors, built on first use
+// and again when the diffusion-length method changes
+static int ensure_geometry(nsfv1_handle* h, void* stream)
+{
+    if (h->geo_valid && h->geo_difflen == h->difflen) return NSFV1_OK;
+    hipError_t ge = launch_geometry_cache(h->et, h->xyz.as<double>(), h->conn.as<int32_t>(), h->ne, h->difflen,
+                                          h->l2inv.as<double>(), h->vol.as<double>(), h->affine.as<uint8_t>(),
+                                          h->errflag.as<int>(), (hipStream_t)stream);
+    if (ge != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("geometry cache: ") + hipGetErrorString(ge));
+    h->geo_valid = true;
+    h->geo_difflen = h->difflen;
+    return NSFV1_OK;
+}
+
 // checks of prep_elem_loop (navier_stokes_fv1.cpp:136-182)
 static int check_setup(nsfv1_handle* h)
 {
@@ -651,13 +666,9 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
     k.s_a = s_a;
     k.s_m = s_m;
     k.s_f = s_f;
-    if (!h->geo_valid || h->geo_difflen != h->difflen) {
-        hipError_t ge = launch_geometry_cache(h->et, h->xyz.as<double>(), h->conn.as<int32_t>(), h->ne, h->difflen,
-                                              h->l2inv.as<double>(), h->vol.as<double>(), h->affine.as<uint8_t>(),
-                                              h->errflag.as<int>(), (hipStream_t)stream);
-        if (ge != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("geometry cache: ") + hipGetErrorString(ge));
-        h->geo_valid = true;
-        h->geo_difflen = h->difflen;
+    {
+        int rc = ensure_geometry(h, stream);
+        if (rc) return rc;
     }
     k.upw_mode = h->upw_mode;
     if (h->upw_mode == NSFV1_UPWIND_REPLAY && !h->upw_recorded)
@@ -802,6 +813,87 @@ int nsfv1_set_outflow(nsfv1_handle* h, const int32_t* elem, const int32_t* side,
 int nsfv1_set_symmetry(nsfv1_handle* h, const int32_t* elem, const int32_t* side, int64_t n)
 {
     return set_boundary_list(h, BND_SYMMETRY, elem, side, n);
+}
+
+int nsfv1_export_size(nsfv1_handle* h, int what, int at, int nip, int64_t* count)
+{
+    if (!h || !count) return fail(NSFV1_ERR_ARGUMENT, "NULL argument");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    if (what < NSFV1_EX_VELOCITY || what > NSFV1_EX_PRESSURE_GRAD) return fail(NSFV1_ERR_ARGUMENT, "unknown export");
+    if (at < NSFV1_AT_SCVF || at > NSFV1_AT_LOCAL) return fail(NSFV1_ERR_ARGUMENT, "unknown export points");
+    if (at == NSFV1_AT_LOCAL && (nip < 1 || nip > EX_MAX_LOCAL))
+        return fail(NSFV1_ERR_ARGUMENT, "local points per call must be 1..32");
+    const int dim = (h->et == ET_TRI || h->et == ET_QUAD) ? 2 : 3;
+    const int nsh = h->et == ET_HEX ? 8 : (h->et == ET_TRI ? 3 : 4);
+    const int nscvf = h->et == ET_TRI ? 3 : (h->et == ET_QUAD ? 4 : (h->et == ET_TET ? 6 : 12));
+    const int npt = at == NSFV1_AT_SCVF ? nscvf : (at == NSFV1_AT_SCV ? nsh : nip);
+    const int ncomp = what == NSFV1_EX_VELOCITY_GRAD ? dim * dim : (what == NSFV1_EX_PRESSURE ? 1 : dim);
+    *count = h->ne * npt * ncomp;
+    return NSFV1_OK;
+}
+
+int nsfv1_export(nsfv1_handle* h, int what, int at, const double* local_ips, int nip, const double* u,
+                 const double* u_old, double dt, double* out, void* stream)
+{
+    int64_t count = 0;
+    int rc = nsfv1_export_size(h, what, at, nip, &count);
+    if (rc) return rc;
+    if (!u || !out) return fail(NSFV1_ERR_ARGUMENT, "NULL buffer");
+    if (at == NSFV1_AT_LOCAL && !local_ips) return fail(NSFV1_ERR_ARGUMENT, "local points are NULL");
+    const int dim = (h->et == ET_TRI || h->et == ET_QUAD) ? 2 : 3;
+    if (what == NSFV1_EX_VELOCITY_IP && at == NSFV1_AT_SCVF) {
+        // the stabilisation's update (ex_velocity_ip :1146-1189)
+        if (h->stab_type == STAB_UNSET) return fail(NSFV1_ERR_NO_STABILIZATION, "Stabilization has not been set.");
+        if (!h->stokes && h->stab_upwind == UPW_UNSET)
+            return fail(NSFV1_ERR_NO_UPWIND, "The stabilization has no upwind object (INavierStokesFV1Stabilization::set_upwind).");
+        if (!h->nu_set) return fail(NSFV1_ERR_NO_VISCOSITY, "Kinematic Viscosity has not been set, but is required.");
+        if (u_old && !(dt > 0.0)) return fail(NSFV1_ERR_TIME_SERIES, "NavierStokes: time step must be positive for the time-dependent stabilization");
+        rc = ensure_geometry(h, stream);
+        if (rc) return rc;
+        KParams k{};
+        k.stab_upwind = h->stab_upwind;
+        k.conv_upwind = h->stab_upwind;  // the convective upwind plays no part in the stabilised velocity
+        k.stab_type = h->stab_type;
+        k.difflen = h->difflen;
+        k.stokes = h->stokes;
+        k.laplace = h->laplace;
+        k.has_source = h->has_source;
+        k.has_old = u_old != nullptr;
+        k.parts = NSFV1_PART_A;
+        k.nu = h->nu;
+        k.rho = h->rho;
+        k.dt = dt;
+        k.inv_dt = u_old ? 1.0 / dt : 0.0;
+        k.inv_rho = 1.0 / h->rho;
+        for (int i = 0; i < 3; ++i) k.f[i] = h->f[i];
+        k.s_a = 1.0;
+        k.upw_mode = NSFV1_UPWIND_FRESH;
+        KMeshPtrs m{};
+        m.xyz = h->xyz.as<double>();
+        m.conn = h->conn.as<int32_t>();
+        m.l2inv = h->l2inv.as<double>();
+        m.vol = h->vol.as<double>();
+        m.affine = h->affine.as<uint8_t>();
+        m.nu_elem = h->nu_elem.as<double>();
+        hipError_t e = launch_export_stab_vel(h->et, m, k, u, u_old, h->ne, out, h->errflag.as<int>(), (hipStream_t)stream);
+        if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("export launch: ") + hipGetErrorString(e));
+        return NSFV1_OK;
+    }
+    ExportArgs a{};
+    a.xyz = h->xyz.as<double>();
+    a.conn = h->conn.as<int32_t>();
+    a.ne = h->ne;
+    a.what = what;
+    a.at = at;
+    a.npt = (int)(count / std::max<int64_t>(h->ne, 1) /
+                  (what == NSFV1_EX_VELOCITY_GRAD ? dim * dim : (what == NSFV1_EX_PRESSURE ? 1 : dim)));
+    a.ncomp = what == NSFV1_EX_VELOCITY_GRAD ? dim * dim : (what == NSFV1_EX_PRESSURE ? 1 : dim);
+    if (at == NSFV1_AT_LOCAL)
+        for (int q = 0; q < nip; ++q)
+            for (int i = 0; i < dim; ++i) a.lips[q][i] = local_ips[q * dim + i];
+    hipError_t e = launch_export_points(h->et, a, u, out, h->errflag.as<int>(), (hipStream_t)stream);
+    if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("export launch: ") + hipGetErrorString(e));
+    return NSFV1_OK;
 }
 
 int nsfv1_set_assembly_path(nsfv1_handle* h, int path)

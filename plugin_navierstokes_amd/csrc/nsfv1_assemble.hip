@@ -761,7 +761,7 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
                                   const int32_t* ccn, const double* c, const double* __restrict__ uold,
                                   double l2, bool kvfrom, bool wantJ, Lane<ET, JT>& L, Rec<ET, JT>& R,
                                   bool& okg, bool& oku, bool& have_cut, int& side, double* gi, double* li,
-                                  uint8_t* decp, bool affine)
+                                  uint8_t* decp, bool affine, double* sv_out = nullptr)
 {
     using T = Topo<ET>;
     constexpr int D = T::dim, N = T::nsh;
@@ -879,6 +879,9 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
 #pragma unroll
         for (int d = 0; d < D; ++d) svel[d] = sv[d];
     }
+    if (sv_out)
+#pragma unroll
+        for (int d = 0; d < D; ++d) sv_out[d] = svel[d];
 
     // defect: navier_stokes_fv1.cpp:667-777
 #pragma unroll
@@ -1692,6 +1695,85 @@ hipError_t launch_geometry_cache(int et, const double* xyz, const int32_t* conn,
     case ET_QUAD: geometry_cache<ET_QUAD><<<blocks, 256, 0, s>>>(xyz, conn, ne, difflen, l2inv, vol, affine, err); break;
     case ET_TET:  geometry_cache<ET_TET><<<blocks, 256, 0, s>>>(xyz, conn, ne, difflen, l2inv, vol, affine, err); break;
     case ET_HEX:  geometry_cache<ET_HEX><<<blocks, 256, 0, s>>>(xyz, conn, ne, difflen, l2inv, vol, affine, err); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------
+// stabilised SCVF velocities (ex_velocity_ip, navier_stokes_fv1.cpp:1105-
+// 1204): one lane per (element, corner); the lane evaluates the SCVFs whose
+// `from` corner it is, in that corner's canonical frame, and writes
+// out[e][ip][d].  Upwind decisions are always fresh.
+// ---------------------------------------------------------------------
+template <int ET, int Q>
+__device__ inline void stab_vel_all(const Cfg<-1>& C, const double (&co)[Topo<ET>::nsh][Topo<ET>::dim],
+                                    const double* __restrict__ xyz, const int32_t* __restrict__ econn,
+                                    const double (&ul)[Topo<ET>::nsh][Topo<ET>::dim + 1], const int* pk,
+                                    const int32_t* ccn, const double* c, const double* uold, const double* l2row, int kv,
+                                    double* oute, bool& okg, bool& oku, bool affine)
+{
+    if constexpr (Q < Topo<ET>::cip) {
+        if (CanonH<ET>::c.from[kv][Q]) {
+            bool have_cut = false;
+            int side = -1;
+            double gi[3] = {0, 0, 0}, li[3] = {0, 0, 0};
+            const int oip = CanonH<ET>::c.oip[kv][Q];
+            Lane<ET, double> L;
+#pragma unroll
+            for (int a = 0; a <= Topo<ET>::dim; ++a) L.dacc[a] = 0.0;
+            Rec<ET, double> R;
+            scvf_canon<ET, -1, Q, double>(C, co, xyz, econn, ul, pk, ccn, c, uold, l2row[oip], true, false, L, R, okg,
+                                          oku, have_cut, side, gi, li, nullptr, affine, oute + oip * Topo<ET>::dim);
+        }
+        stab_vel_all<ET, Q + 1>(C, co, xyz, econn, ul, pk, ccn, c, uold, l2row, kv, oute, okg, oku, affine);
+    }
+}
+
+template <int ET>
+__global__ void __launch_bounds__(256)
+export_stab_vel(KMeshPtrs M, KParams P, const double* __restrict__ u, const double* __restrict__ uold, int64_t ne,
+                double* __restrict__ out, int* __restrict__ err)
+{
+    using T = Topo<ET>;
+    constexpr int D = T::dim, N = T::nsh, NC = D + 1, NIP = T::nip;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ne * N) return;
+    const int64_t e = t / N;
+    const int kv = (int)(t - e * N);
+    int pk[N];
+    int32_t ccn[N];
+    double co[N][D], ul[N][NC];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        pk[k] = CanonH<ET>::c.perm[kv][k];
+        ccn[k] = M.conn[e * N + pk[k]];
+#pragma unroll
+        for (int i = 0; i < D; ++i) co[k][i] = M.xyz[(int64_t)ccn[k] * D + i];
+#pragma unroll
+        for (int a = 0; a < NC; ++a) ul[k][a] = u[(int64_t)ccn[k] * NC + a];
+    }
+    double c[D];
+    elem_centre<ET>(co, c);
+    bool okg = true, oku = true;
+    const Cfg<-1> C{P, M.nu_elem ? M.nu_elem[e] : P.nu};
+    stab_vel_all<ET, 0>(C, co, M.xyz, M.conn + e * N, ul, pk, ccn, c, uold, M.l2inv + e * NIP, kv,
+                        out + e * NIP * D, okg, oku, (ET == 0 || ET == 2) || (M.affine != nullptr && M.affine[e] != 0));
+    if (!okg || !oku) err[0] = (oku ? 0 : KERR_CUT_SIDE) | (okg ? 0 : KERR_DEGENERATE);
+}
+
+hipError_t launch_export_stab_vel(int et, const KMeshPtrs& m, const KParams& p, const double* u, const double* uold,
+                                  int64_t ne, double* out, int* err, hipStream_t s)
+{
+    const int nsh = et == ET_HEX ? 8 : (et == ET_TRI ? 3 : 4);
+    const int64_t lanes = ne * nsh;
+    if (lanes == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((lanes + 255) / 256);
+    switch (et) {
+    case ET_TRI:  export_stab_vel<ET_TRI><<<blocks, 256, 0, s>>>(m, p, u, uold, ne, out, err); break;
+    case ET_QUAD: export_stab_vel<ET_QUAD><<<blocks, 256, 0, s>>>(m, p, u, uold, ne, out, err); break;
+    case ET_TET:  export_stab_vel<ET_TET><<<blocks, 256, 0, s>>>(m, p, u, uold, ne, out, err); break;
+    case ET_HEX:  export_stab_vel<ET_HEX><<<blocks, 256, 0, s>>>(m, p, u, uold, ne, out, err); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -43,6 +43,10 @@ hipError_t launch_assemble_split(int et, const KMeshPtrs& m, const KParams& p, c
                                  float* J, double* d, int* err, int nwg, size_t lds, int64_t ne, float* recs,
                                  hipStream_t stream);
 
+// stabilised SCVF velocities of every element: out[ne][nip][dim]
+hipError_t launch_export_stab_vel(int et, const KMeshPtrs& m, const KParams& p, const double* u, const double* uold,
+                                  int64_t ne, double* out, int* err, hipStream_t stream);
+
 hipError_t launch_geometry_cache(int et, const double* xyz, const int32_t* conn, int64_t ne, int difflen,
                                  double* l2inv, double* vol, uint8_t* affine, int* err, hipStream_t stream);
 
