@@ -200,6 +200,18 @@ int nsfv1_set_outflow(nsfv1_handle* h, const int32_t* elem, const int32_t* side,
    master's viscosity, density and Laplace flag apply. */
 int nsfv1_set_symmetry(nsfv1_handle* h, const int32_t* elem, const int32_t* side, int64_t n);
 
+/* NavierStokesWSBCFV1, the sliding wall (incompressible/fv1/bnd/
+   wall_sliding_fv1.cpp:229-425; Lua "NavierStokesWSBCFV1" with add(),
+   set_sliding_factor(number), set_sliding_limit(number)): on the listed
+   sides, per FV1 boundary face, the tangential convective term, -k times
+   the tangential velocity and -limit times its direction go into the
+   momentum rows (part A, scaled by s_a), with the reference's approximate
+   Jacobian.  The velocity of the k and limit terms is interpolated from the
+   element's first dim corners only, as the reference does (:391-397).
+   The master's density and Stokes flag apply.  n = 0 removes the condition. */
+int nsfv1_set_sliding_wall(nsfv1_handle* h, const int32_t* elem, const int32_t* side, int64_t n,
+                           double sliding_factor, double sliding_limit);
+
 /* Data exports of the disc (navier_stokes_fv1.cpp:894-1491; the export
    objects of navier_stokes_fv1.cpp:1573-1577): values at points of every
    element, written to the device array out[ne][points][components] (FP64)

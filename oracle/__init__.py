@@ -132,6 +132,9 @@ def lib():
         L.or_symmetry_elem.argtypes = [C.POINTER(_Params), C.c_int, dp, C.c_int, dp, dp, dp]
         L.or_symmetry.argtypes = [C.POINTER(_Params), C.c_int, dp, C.c_int64, ip32, ip32, ip32, C.c_int64, dp,
                                   C.c_double, ip64, ip32, dp, dp]
+        L.or_sliding_elem.argtypes = [C.POINTER(_Params), C.c_int, dp, C.c_int, dp, C.c_double, C.c_double, dp, dp]
+        L.or_sliding.argtypes = [C.POINTER(_Params), C.c_int, dp, ip32, ip32, ip32, C.c_int64, dp, C.c_double,
+                                 C.c_double, C.c_double, ip64, ip32, dp, dp]
         L.or_export_elem.argtypes = [C.POINTER(_Params), C.c_int, dp, dp, dp, C.c_double, C.c_int, C.c_int, dp,
                                      C.c_int, dp]
         L.or_export.argtypes = [C.POINTER(_Params), C.c_int, dp, ip32, C.c_int64, dp, dp, C.c_double, C.c_int, C.c_int,
@@ -283,20 +286,26 @@ def bnd_faces(et: int, corners, side: int) -> dict:
             "ipg": a(b.ipg)[:k, :dim].copy(), "phi": a(b.phi)[:k, :nsh].copy(), "grad": a(b.g)[:k, :nsh, :dim].copy()}
 
 
-def outflow_elem(p: Params, et: int, corners, side: int, uloc, kind: str = "outflow"):
+def outflow_elem(p: Params, et: int, corners, side: int, uloc, kind: str = "outflow", k: float = 0.0,
+                 lim: float = 0.0):
     """Element-local (J, d) of a boundary condition ("outflow": no normal
-    stress; "symmetry") on one side."""
+    stress; "symmetry"; "sliding": sliding wall with factor k and limit lim)
+    on one side."""
     nl = NSH[et] * (DIM[et] + 1)
     J = np.zeros(nl * nl)
     d = np.zeros(nl)
     pc = p.c()
-    f = lib().or_outflow_elem if kind == "outflow" else lib().or_symmetry_elem
-    _chk(f(C.byref(pc), et, _dp(_f64(corners)), int(side), _dp(_f64(uloc)), _dp(J), _dp(d)))
+    co, ul = _dp(_f64(corners)), _dp(_f64(uloc))
+    if kind == "sliding":
+        _chk(lib().or_sliding_elem(C.byref(pc), et, co, int(side), ul, float(k), float(lim), _dp(J), _dp(d)))
+    else:
+        f = lib().or_outflow_elem if kind == "outflow" else lib().or_symmetry_elem
+        _chk(f(C.byref(pc), et, co, int(side), ul, _dp(J), _dp(d)))
     return J.reshape(nl, nl), d
 
 
 def outflow(p: Params, et: int, xyz, conn, belem, bside, u, s_a=1.0, rowptr=None, colidx=None, J=None, d=None,
-            kind: str = "outflow"):
+            kind: str = "outflow", k: float = 0.0, lim: float = 0.0):
     """Adds s_a x the outflow contributions of the (element, side) list to J, d
     (new zero arrays when None).  Returns (J, d)."""
     xyz, u = _f64(xyz), _f64(u)
@@ -312,10 +321,15 @@ def outflow(p: Params, et: int, xyz, conn, belem, bside, u, s_a=1.0, rowptr=None
         d = np.zeros(nv * (DIM[et] + 1))
     pc = p.c()
     i32 = C.POINTER(C.c_int32)
-    f = lib().or_outflow if kind == "outflow" else lib().or_symmetry
-    _chk(f(C.byref(pc), et, _dp(xyz), nv, conn.ctypes.data_as(i32), belem.ctypes.data_as(i32),
-                          bside.ctypes.data_as(i32), len(belem), _dp(u), float(s_a),
-                          rowptr.ctypes.data_as(C.POINTER(C.c_int64)), colidx.ctypes.data_as(i32), _dp(J), _dp(d)))
+    rp, ci = rowptr.ctypes.data_as(C.POINTER(C.c_int64)), colidx.ctypes.data_as(i32)
+    if kind == "sliding":
+        _chk(lib().or_sliding(C.byref(pc), et, _dp(xyz), conn.ctypes.data_as(i32), belem.ctypes.data_as(i32),
+                              bside.ctypes.data_as(i32), len(belem), _dp(u), float(s_a), float(k), float(lim), rp, ci,
+                              _dp(J), _dp(d)))
+    else:
+        f = lib().or_outflow if kind == "outflow" else lib().or_symmetry
+        _chk(f(C.byref(pc), et, _dp(xyz), nv, conn.ctypes.data_as(i32), belem.ctypes.data_as(i32),
+               bside.ctypes.data_as(i32), len(belem), _dp(u), float(s_a), rp, ci, _dp(J), _dp(d)))
     return J, d
 
 

@@ -1316,23 +1316,28 @@ int or_outflow_elem(const or_params* p, int et, const double* corners, int side,
 }
 
 /* adds s_a * (outflow contributions of the listed boundary sides) to J and d */
-int or_outflow(const or_params* p, int et, const double* xyz, int64_t nv, const int32_t* conn,
-               const int32_t* belem, const int32_t* bside, int64_t nb, const double* u, double s_a,
-               const int64_t* rowptr, const int32_t* colidx, double* J, double* d)
+/* element-local boundary contributions of one kind (0 outflow, 1 symmetry,
+   2 sliding wall with factor k and limit lim) */
+static int bnd_elem(int kind, const or_params* p, int et, const double* co, int side, const double* ul, double k,
+                    double lim, double* Jl, double* dl);
+
+/* adds s_a * (contributions of the listed boundary sides) to J and d */
+static int bnd_scatter(int kind, const or_params* p, int et, const double* xyz, const int32_t* conn,
+                       const int32_t* belem, const int32_t* bside, int64_t nb, const double* u, double s_a,
+                       double k, double lim, const int64_t* rowptr, const int32_t* colidx, double* J, double* d)
 {
     if (et < 0 || et > 3) { set_err("unknown element type"); return -1; }
     const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1, nl = nsh * nc;
-    (void)nv;
     double co[24], ul[32], Jl[1024], dl[32];
     for (int64_t b = 0; b < nb; ++b) {
         const int32_t* c = conn + (int64_t)belem[b] * nsh;
-        for (int k = 0; k < nsh; ++k) {
-            for (int i = 0; i < dim; ++i) co[k * dim + i] = xyz[(int64_t)c[k] * dim + i];
-            for (int a = 0; a < nc; ++a) ul[k * nc + a] = u[(int64_t)c[k] * nc + a];
+        for (int q = 0; q < nsh; ++q) {
+            for (int i = 0; i < dim; ++i) co[q * dim + i] = xyz[(int64_t)c[q] * dim + i];
+            for (int a = 0; a < nc; ++a) ul[q * nc + a] = u[(int64_t)c[q] * nc + a];
         }
         memset(Jl, 0, sizeof(double) * nl * nl);
         memset(dl, 0, sizeof(double) * nl);
-        int rc = or_outflow_elem(p, et, co, bside[b], ul, J ? Jl : NULL, d ? dl : NULL);
+        int rc = bnd_elem(kind, p, et, co, bside[b], ul, k, lim, J ? Jl : NULL, d ? dl : NULL);
         if (rc) return rc;
         for (int i = 0; i < nsh; ++i)
             for (int a = 0; a < nc; ++a) {
@@ -1348,6 +1353,14 @@ int or_outflow(const or_params* p, int et, const double* xyz, int64_t nv, const 
             }
     }
     return 0;
+}
+
+int or_outflow(const or_params* p, int et, const double* xyz, int64_t nv, const int32_t* conn,
+               const int32_t* belem, const int32_t* bside, int64_t nb, const double* u, double s_a,
+               const int64_t* rowptr, const int32_t* colidx, double* J, double* d)
+{
+    (void)nv;
+    return bnd_scatter(0, p, et, xyz, conn, belem, bside, nb, u, s_a, 0.0, 0.0, rowptr, colidx, J, d);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1403,34 +1416,99 @@ int or_symmetry(const or_params* p, int et, const double* xyz, int64_t nv, const
                 const int32_t* belem, const int32_t* bside, int64_t nb, const double* u, double s_a,
                 const int64_t* rowptr, const int32_t* colidx, double* J, double* d)
 {
-    if (et < 0 || et > 3) { set_err("unknown element type"); return -1; }
-    const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1, nl = nsh * nc;
     (void)nv;
-    double co[24], ul[32], Jl[1024], dl[32];
-    for (int64_t b = 0; b < nb; ++b) {
-        const int32_t* c = conn + (int64_t)belem[b] * nsh;
-        for (int k = 0; k < nsh; ++k) {
-            for (int i = 0; i < dim; ++i) co[k * dim + i] = xyz[(int64_t)c[k] * dim + i];
-            for (int a = 0; a < nc; ++a) ul[k * nc + a] = u[(int64_t)c[k] * nc + a];
+    return bnd_scatter(1, p, et, xyz, conn, belem, bside, nb, u, s_a, 0.0, 0.0, rowptr, colidx, J, d);
+}
+
+/* ------------------------------------------------------------------ */
+/* NavierStokesWSBCFV1, the sliding wall (restated):                   */
+/* incompressible/fv1/bnd/wall_sliding_fv1.cpp:229-425.  Per boundary  */
+/* face (node c, area vector n, unit normal nh = n/|n|, ip):           */
+/*  convection (not Stokes): v = sum_sh u_sh phi_sh, t = v - (v.nh)nh, */
+/*    f = (t.n) rho; d(d1,c) += f t[d1]; J(d1,c,d1,sh) += f phi_sh     */
+/*    (t is tangential, so f is a rounding residue, kept as written);  */
+/*  sliding factor k: w = sum over the FIRST dim shapes of u_sh phi_sh */
+/*    (the loop bound of :391-397), t = w - (w.nh)nh,                  */
+/*    d(d1,c) -= k t[d1]; J(d1,c,d2,sh) -= k P(d1,d2) for every sh,    */
+/*    with P = (sum_sh phi_sh) I - (sum_{sh<dim} phi_sh) nh nh^T       */
+/*    (:328-344);                                                      */
+/*  sliding limit lim: if |t| != 0, d(d1,c) -= lim t[d1]/|t|;          */
+/*    J(d1,c,d1,sh) -= lim s[d1] for every sh, s = (1,..,1)(sum phi)   */
+/*    normalised (:350-365).                                           */
+/* The Jacobian is the reference's approximation, not d'(u).           */
+/* ------------------------------------------------------------------ */
+int or_sliding_elem(const or_params* p, int et, const double* corners, int side, const double* uloc, double k,
+                    double lim, double* Jl, double* dl)
+{
+    or_bfgeo B;
+    int rc = or_bnd_faces(et, corners, side, &B);
+    if (rc) return rc;
+    const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1, nl = nsh * nc;
+    const double rho = p->rho;
+    for (int f = 0; f < B.nbf; ++f) {
+        const double* n = B.n[f];
+        const int row = B.node[f];
+        const double len = sqrt(dot(dim, n, n));
+        double nh[3] = {0, 0, 0};
+        for (int i = 0; i < dim; ++i) nh[i] = n[i] / len;
+        if (!p->stokes) {
+            double v[3] = {0, 0, 0};
+            for (int d1 = 0; d1 < dim; ++d1)
+                for (int sh = 0; sh < nsh; ++sh) v[d1] += uloc[sh * nc + d1] * B.phi[f][sh];
+            const double vn = dot(dim, v, nh);
+            for (int d1 = 0; d1 < dim; ++d1) v[d1] -= vn * nh[d1];
+            const double flux = dot(dim, v, n) * rho;
+            if (dl)
+                for (int d1 = 0; d1 < dim; ++d1) dl[row * nc + d1] += flux * v[d1];
+            if (Jl)
+                for (int sh = 0; sh < nsh; ++sh)
+                    for (int d1 = 0; d1 < dim; ++d1) Jl[(row * nc + d1) * nl + sh * nc + d1] += flux * B.phi[f][sh];
         }
-        memset(Jl, 0, sizeof(double) * nl * nl);
-        memset(dl, 0, sizeof(double) * nl);
-        int rc = or_symmetry_elem(p, et, co, bside[b], ul, J ? Jl : NULL, d ? dl : NULL);
-        if (rc) return rc;
-        for (int i = 0; i < nsh; ++i)
-            for (int a = 0; a < nc; ++a) {
-                const int64_t row = (int64_t)c[i] * nc + a;
-                if (d) d[row] += s_a * dl[i * nc + a];
-                if (J)
-                    for (int j = 0; j < nsh; ++j)
-                        for (int bb = 0; bb < nc; ++bb) {
-                            const int64_t pos = find_col(rowptr, colidx, row, (int32_t)((int64_t)c[j] * nc + bb));
-                            if (pos < 0) { set_err("pattern misses an entry"); return -6; }
-                            J[pos] += s_a * Jl[(i * nc + a) * nl + j * nc + bb];
-                        }
+        double sum_all = 0.0, sum_dim = 0.0;
+        for (int sh = 0; sh < nsh; ++sh) sum_all += B.phi[f][sh];
+        for (int sh = 0; sh < dim; ++sh) sum_dim += B.phi[f][sh];
+        if (dl) {
+            double w[3] = {0, 0, 0};
+            for (int d1 = 0; d1 < dim; ++d1)
+                for (int sh = 0; sh < dim; ++sh) w[d1] += uloc[sh * nc + d1] * B.phi[f][sh];
+            const double wn = dot(dim, w, nh);
+            for (int d1 = 0; d1 < dim; ++d1) w[d1] -= wn * nh[d1];
+            for (int d1 = 0; d1 < dim; ++d1) dl[row * nc + d1] += -1.0 * w[d1] * k;
+            const double wl = sqrt(dot(dim, w, w));
+            if (wl != 0.0)
+                for (int d1 = 0; d1 < dim; ++d1) dl[row * nc + d1] += -1.0 * (w[d1] / wl) * lim;
+        }
+        if (Jl) {
+            double P[3][3];
+            for (int d1 = 0; d1 < dim; ++d1)
+                for (int d2 = 0; d2 < dim; ++d2) P[d1][d2] = (d1 == d2 ? sum_all : 0.0) - sum_dim * nh[d1] * nh[d2];
+            double s[3] = {0, 0, 0};
+            for (int d1 = 0; d1 < dim; ++d1) s[d1] = sum_all;
+            const double sl = sqrt(dot(dim, s, s));
+            for (int sh = 0; sh < nsh; ++sh) {
+                for (int d1 = 0; d1 < dim; ++d1)
+                    for (int d2 = 0; d2 < dim; ++d2) Jl[(row * nc + d1) * nl + sh * nc + d2] += -1.0 * P[d1][d2] * k;
+                if (sl != 0.0)
+                    for (int d1 = 0; d1 < dim; ++d1) Jl[(row * nc + d1) * nl + sh * nc + d1] += -1.0 * (s[d1] / sl) * lim;
             }
+        }
     }
     return 0;
+}
+
+int or_sliding(const or_params* p, int et, const double* xyz, const int32_t* conn, const int32_t* belem,
+               const int32_t* bside, int64_t nb, const double* u, double s_a, double k, double lim,
+               const int64_t* rowptr, const int32_t* colidx, double* J, double* d)
+{
+    return bnd_scatter(2, p, et, xyz, conn, belem, bside, nb, u, s_a, k, lim, rowptr, colidx, J, d);
+}
+
+static int bnd_elem(int kind, const or_params* p, int et, const double* co, int side, const double* ul, double k,
+                    double lim, double* Jl, double* dl)
+{
+    if (kind == 0) return or_outflow_elem(p, et, co, side, ul, Jl, dl);
+    if (kind == 1) return or_symmetry_elem(p, et, co, side, ul, Jl, dl);
+    return or_sliding_elem(p, et, co, side, ul, k, lim, Jl, dl);
 }
 
 /* ------------------------------------------------------------------ */

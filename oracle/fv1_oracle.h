@@ -140,6 +140,14 @@ int or_symmetry(const or_params* p, int et, const double* xyz, int64_t nv, const
                 const int32_t* belem, const int32_t* bside, int64_t nb, const double* u, double s_a,
                 const int64_t* rowptr, const int32_t* colidx, double* J, double* d);
 
+/* Sliding wall disc (incompressible/fv1/bnd/wall_sliding_fv1.cpp:229-425)
+   with constant sliding factor k and sliding limit lim. */
+int or_sliding_elem(const or_params* p, int et, const double* corners, int side, const double* uloc, double k,
+                    double lim, double* Jl, double* dl);
+int or_sliding(const or_params* p, int et, const double* xyz, const int32_t* conn, const int32_t* belem,
+               const int32_t* bside, int64_t nb, const double* u, double s_a, double k, double lim,
+               const int64_t* rowptr, const int32_t* colidx, double* J, double* d);
+
 /* Data exports (navier_stokes_fv1.cpp:894-1491).  what: 0 velocity,
    1 velocity gradient, 2 stabilised ip velocity, 3 pressure, 4 pressure
    gradient; at: 0 SCVF ips, 1 SCV ips (corners), 2 the local points

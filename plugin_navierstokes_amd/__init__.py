@@ -44,7 +44,8 @@ EXPORTS = [
     "nsfv1_set_preconditioner", "nsfv1_mg_set_hierarchy", "nsfv1_mg_setup", "nsfv1_set_upwind_decisions",
     "nsfv1_mlp_sizes", "nsfv1_mlp_pack", "nsfv1_mlp_forward", "nsfv1_mlp_mse", "nsfv1_mlp_backward", "nsfv1_mlp_adam",
     "nsfv1_predict_features", "nsfv1_predict_guess", "nsfv1_predict_target", "nsfv1_batch_set_instances",
-    "nsfv1_set_assembly_path", "nsfv1_set_outflow", "nsfv1_set_symmetry", "nsfv1_export_size", "nsfv1_export",
+    "nsfv1_set_assembly_path", "nsfv1_set_outflow", "nsfv1_set_symmetry", "nsfv1_set_sliding_wall", "nsfv1_export_size",
+    "nsfv1_export",
 ]
 # data exports (navier_stokes_fv1.cpp:1573-1577) and their points
 EX = {"velocity": 0, "velocity_grad": 1, "velocity_ip": 2, "pressure": 3, "pressure_grad": 4}
@@ -106,6 +107,7 @@ def load_library(path: str | None = None):
     L.nsfv1_set_assembly_path.argtypes = [H, C.c_int]
     L.nsfv1_set_outflow.argtypes = [H, vp, vp, C.c_int64]
     L.nsfv1_set_symmetry.argtypes = [H, vp, vp, C.c_int64]
+    L.nsfv1_set_sliding_wall.argtypes = [H, vp, vp, C.c_int64, C.c_double, C.c_double]
     L.nsfv1_export_size.argtypes = [H, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64)]
     L.nsfv1_export.argtypes = [H, C.c_int, C.c_int, vp, C.c_int, vp, vp, C.c_double, vp, vp]
     L.nsfv1_mlp_sizes.argtypes = [C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
@@ -304,6 +306,13 @@ class NavierStokesFV1:
         e = np.ascontiguousarray(elem, dtype=np.int32)
         s = np.ascontiguousarray(side, dtype=np.int32)
         _check(_lib.nsfv1_set_symmetry(self._h, e.ctypes.data, s.ctypes.data, len(e)))
+
+    def set_sliding_wall(self, elem, side, sliding_factor: float, sliding_limit: float = 0.0):
+        """Sliding wall (NavierStokesWSBCFV1) on the (element, local side) list."""
+        e = np.ascontiguousarray(elem, dtype=np.int32)
+        s = np.ascontiguousarray(side, dtype=np.int32)
+        _check(_lib.nsfv1_set_sliding_wall(self._h, e.ctypes.data, s.ctypes.data, len(e), float(sliding_factor),
+                                           float(sliding_limit)))
 
     def export(self, u, what: str, at: str = "scvf", local_ips=None, u_old=None, dt=0.0, out=None, stream=None,
                check=True):

@@ -113,12 +113,13 @@ struct nsfv1_handle {
     int64_t nv = 0, ne = 0, ndof = 0, nnz = 0, ninc = 0;
     DevBuf xyz, conn, rowptr, colidx, errflag, l2inv, vol, affine, node_inc, inc_elem, inc_node, inc_info, inc_pos;
     std::vector<int32_t> h_inc_info, h_inc_elem;
-    // boundary discs (no-normal-stress outflow, symmetry): per boundary node,
-    // its boundary faces as (incidence, side) items
+    // boundary discs (no-normal-stress outflow, symmetry, sliding wall): per
+    // boundary node, its boundary faces as (incidence, side) items
     struct BndList {
         DevBuf ptr, node, inc, side;
         int64_t nbn = 0;
-    } bnd[2];
+    } bnd[BND_KINDS];
+    double slide_k = 0.0, slide_lim = 0.0;
     std::vector<int64_t> h_rowptr, h_node_inc;
     int max_threads = 256;
     size_t lds_cap = 64 * 1024;
@@ -481,7 +482,7 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
     h->h_node_inc.swap(cnt);
     h->h_inc_info.swap(ii);
     h->h_inc_elem.swap(ie);
-    h->bnd[BND_OUTFLOW].nbn = h->bnd[BND_SYMMETRY].nbn = 0;
+    for (auto& b : h->bnd) b.nbn = 0;
     h->plan32.valid = h->plan64.valid = false;
     h->geo_valid = false;
     // solver state sized for the previous mesh
@@ -693,13 +694,13 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
                             (hipStream_t)stream);
     }
     if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("assemble launch: ") + hipGetErrorString(e));
-    for (int kind = 0; kind < 2; ++kind) {
+    for (int kind = 0; kind < BND_KINDS; ++kind) {
         const auto& B = h->bnd[kind];
         if (B.nbn == 0 || !(parts & NSFV1_PART_A)) continue;
         OutflowArgs oa{h->xyz.as<double>(), h->conn.as<int32_t>(), h->rowptr.as<int64_t>(), h->inc_elem.as<int32_t>(),
                        h->inc_info.as<int32_t>(), h->inc_pos.as<uint64_t>(), B.ptr.as<int32_t>(), B.node.as<int64_t>(),
                        B.inc.as<int32_t>(), B.side.as<int32_t>(), B.nbn, h->nu, h->rho, s_a, h->laplace ? 1 : 0,
-                       h->stokes ? 1 : 0, kind};
+                       h->stokes ? 1 : 0, kind, h->slide_k, h->slide_lim};
         e = launch_outflow(h->et, jprec, oa, u, jvals, defect, h->errflag.as<int>(), (hipStream_t)stream);
         if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("boundary disc launch: ") + hipGetErrorString(e));
     }
@@ -813,6 +814,18 @@ int nsfv1_set_outflow(nsfv1_handle* h, const int32_t* elem, const int32_t* side,
 int nsfv1_set_symmetry(nsfv1_handle* h, const int32_t* elem, const int32_t* side, int64_t n)
 {
     return set_boundary_list(h, BND_SYMMETRY, elem, side, n);
+}
+
+int nsfv1_set_sliding_wall(nsfv1_handle* h, const int32_t* elem, const int32_t* side, int64_t n,
+                           double sliding_factor, double sliding_limit)
+{
+    if (h && n > 0 && !(std::isfinite(sliding_factor) && std::isfinite(sliding_limit)))
+        return fail(NSFV1_ERR_ARGUMENT, "sliding factor and limit must be finite");
+    int rc = set_boundary_list(h, BND_SLIDING, elem, side, n);
+    if (rc) return rc;
+    h->slide_k = sliding_factor;
+    h->slide_lim = sliding_limit;
+    return NSFV1_OK;
 }
 
 int nsfv1_export_size(nsfv1_handle* h, int what, int at, int nip, int64_t* count)

@@ -18,6 +18,13 @@
 //   continuity: rho u.n;
 // and the matching fixpoint Jacobian (:291-311, 360-368).
 //
+// The sliding wall (incompressible/fv1/bnd/wall_sliding_fv1.cpp:229-425)
+// adds per boundary face, with the unit normal nh: the convective term
+// (t.n) rho t of the tangential velocity t (zero up to rounding, kept as
+// the reference writes it), -k t_w and -lim t_w/|t_w| where t_w is the
+// tangential part of the interpolation over the first dim shapes only
+// (:391-397), and the reference's approximate Jacobian (:265-365).
+//
 // The symmetry condition adds per boundary face
 //   momentum d: F[d] n[d] with F = -nu rho (grad u + grad u^T) n (the
 //               component-wise product of :373-375) plus p(ip) n[d];
@@ -273,6 +280,72 @@ outflow_rows(OutflowArgs A, const double* __restrict__ u, JT* __restrict__ J, do
                         }
                         J[rows[d1] + col + D] += JT(sa * (phi[k] * n[d1]));
                     }
+                }
+            }
+            continue;
+        }
+        if (A.kind == BND_SLIDING) {
+            double len = 0, nh[3] = {0, 0, 0};
+#pragma unroll
+            for (int i = 0; i < D; ++i) len += n[i] * n[i];
+            len = sqrt(len);
+#pragma unroll
+            for (int i = 0; i < D; ++i) nh[i] = n[i] / len;
+            if (!A.stokes) {
+                double t[3] = {0, 0, 0}, tn = 0, f = 0;
+#pragma unroll
+                for (int i = 0; i < D; ++i) tn += vel[i] * nh[i];
+#pragma unroll
+                for (int i = 0; i < D; ++i) t[i] = vel[i] - tn * nh[i];
+#pragma unroll
+                for (int i = 0; i < D; ++i) f += t[i] * n[i];
+                f *= rho;
+#pragma unroll
+                for (int d1 = 0; d1 < D; ++d1) dsum[d1] += sa * (f * t[d1]);
+                if (J)
+#pragma unroll
+                    for (int k = 0; k < N; ++k) {
+                        const int col = (int)((pw >> (8 * k)) & 0xffu) * NC;
+#pragma unroll
+                        for (int d1 = 0; d1 < D; ++d1) J[rows[d1] + col + d1] += JT(sa * (f * phi[k]));
+                    }
+            }
+            double sum_all = 0, sum_dim = 0, w[3] = {0, 0, 0}, wn = 0;
+#pragma unroll
+            for (int k = 0; k < N; ++k) sum_all += phi[k];
+#pragma unroll
+            for (int k = 0; k < D; ++k) sum_dim += phi[k];
+#pragma unroll
+            for (int d1 = 0; d1 < D; ++d1)
+#pragma unroll
+                for (int k = 0; k < D; ++k) w[d1] += ul[k][d1] * phi[k];
+#pragma unroll
+            for (int i = 0; i < D; ++i) wn += w[i] * nh[i];
+#pragma unroll
+            for (int i = 0; i < D; ++i) w[i] -= wn * nh[i];
+            double wl = 0;
+#pragma unroll
+            for (int i = 0; i < D; ++i) wl += w[i] * w[i];
+            wl = sqrt(wl);
+#pragma unroll
+            for (int d1 = 0; d1 < D; ++d1) {
+                double r = -1.0 * w[d1] * A.slide_k;
+                if (wl != 0.0) r += -1.0 * (w[d1] / wl) * A.slide_lim;
+                dsum[d1] += sa * r;
+            }
+            if (J) {
+                const double sl = sqrt((double)D * sum_all * sum_all);
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    const int col = (int)((pw >> (8 * k)) & 0xffu) * NC;
+#pragma unroll
+                    for (int d1 = 0; d1 < D; ++d1)
+#pragma unroll
+                        for (int d2 = 0; d2 < D; ++d2) {
+                            double val = -1.0 * ((d1 == d2 ? sum_all : 0.0) - sum_dim * nh[d1] * nh[d2]) * A.slide_k;
+                            if (d1 == d2 && sl != 0.0) val += -1.0 * (sum_all / sl) * A.slide_lim;
+                            J[rows[d1] + col + d2] += JT(sa * val);
+                        }
                 }
             }
             continue;

@@ -1,5 +1,6 @@
 // nsfv1_boundary.hpp -- launcher of the FV1 boundary discs (no-normal-stress
-// outflow, symmetry) on the rows of the nodes of the listed boundary sides.
+// outflow, symmetry, sliding wall) on the rows of the nodes of the listed
+// boundary sides.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,10 +22,11 @@ struct OutflowArgs {
     int64_t nbn;
     double nu, rho, s_a;
     int laplace, stokes;
-    int kind;  // BND_OUTFLOW or BND_SYMMETRY
+    int kind;  // BND_OUTFLOW, BND_SYMMETRY or BND_SLIDING
+    double slide_k, slide_lim;  // sliding wall: sliding factor and sliding limit
 };
 
-enum { BND_OUTFLOW = 0, BND_SYMMETRY = 1 };
+enum { BND_OUTFLOW = 0, BND_SYMMETRY = 1, BND_SLIDING = 2, BND_KINDS = 3 };
 
 hipError_t launch_outflow(int et, int jprec, const OutflowArgs& a, const double* u, void* J, double* d, int* err,
                           hipStream_t s);

@@ -1,6 +1,7 @@
 """GPU parity of the FV1 boundary discs -- no-normal-stress outflow
-(incompressible/fv1/bnd/no_normal_stress_outflow_fv1.cpp:188-426) and
-symmetry (incompressible/fv1/bnd/symmetric_boundary_fv1.cpp:224-386):
+(incompressible/fv1/bnd/no_normal_stress_outflow_fv1.cpp:188-426),
+symmetry (incompressible/fv1/bnd/symmetric_boundary_fv1.cpp:224-386) and
+sliding wall (incompressible/fv1/bnd/wall_sliding_fv1.cpp:229-425):
 element assembly plus boundary rows through the C ABI against the CPU
 oracle."""
 from __future__ import annotations
@@ -28,6 +29,12 @@ def _case(et):
     return mesh, np.concatenate([be1, be2]), np.concatenate([bs1, bs2])
 
 
+def _left(mesh):
+    """Sides on x = min (the sliding wall of the combined case)."""
+    x = mesh.xyz[:, 0]
+    return M.boundary_sides(mesh, np.nonzero(np.isclose(x, x.min()))[0])
+
+
 def _bottom(mesh):
     """Sides on y = min (the symmetry plane of the combined case)."""
     y = mesh.xyz[:, 1]
@@ -38,7 +45,7 @@ def _bottom(mesh):
                          ids=["full", "lps-laplace", "stokes"])
 @pytest.mark.parametrize("jprec", [F64, F32])
 @pytest.mark.parametrize("et", ETS)
-@pytest.mark.parametrize("kind", ["outflow", "symmetry", "both"])
+@pytest.mark.parametrize("kind", ["outflow", "symmetry", "both", "sliding", "all"])
 def test_outflow_parity(cuda, et, jprec, cfg, kind):
     import torch
 
@@ -51,17 +58,24 @@ def test_outflow_parity(cuda, et, jprec, cfg, kind):
     rp, ci = disc.pattern()
     p = oracle_params(mesh.dim, cfg)
     Jo, do = 0.0, 0.0
-    if kind in ("outflow", "both"):
+    if kind in ("outflow", "both", "all"):
         disc.set_outflow(belem, bside)
         Jx, dx = O.outflow(p, et, mesh.xyz, mesh.conn, belem, bside, u, s_a=0.8, rowptr=rp, colidx=ci)
         Jo, do = Jo + np.abs(Jx), do + np.abs(dx)
         Jr_b, dr_b = Jx, dx
     else:
         Jr_b, dr_b = 0.0, 0.0
-    if kind in ("symmetry", "both"):
+    if kind in ("symmetry", "both", "all"):
         se, ss = (belem, bside) if kind == "symmetry" else _bottom(mesh)
         disc.set_symmetry(se, ss)
         Jx, dx = O.outflow(p, et, mesh.xyz, mesh.conn, se, ss, u, s_a=0.8, rowptr=rp, colidx=ci, kind="symmetry")
+        Jo, do = Jo + np.abs(Jx), do + np.abs(dx)
+        Jr_b, dr_b = Jr_b + Jx, dr_b + dx
+    if kind in ("sliding", "all"):
+        we, ws = (belem, bside) if kind == "sliding" else _left(mesh)
+        disc.set_sliding_wall(we, ws, 1.3, 0.2)
+        Jx, dx = O.outflow(p, et, mesh.xyz, mesh.conn, we, ws, u, s_a=0.8, rowptr=rp, colidx=ci, kind="sliding",
+                           k=1.3, lim=0.2)
         Jo, do = Jo + np.abs(Jx), do + np.abs(dx)
         Jr_b, dr_b = Jr_b + Jx, dr_b + dx
     J, d = disc.assemble(torch.from_numpy(u).to(cuda), s_a=0.8, parts=PART_A, jprec=jprec)
@@ -108,3 +122,9 @@ def test_outflow_cleared(cuda):
     J4, d4 = disc.assemble(u)
     assert not torch.equal(d0, d3)
     assert torch.equal(J0, J4) and torch.equal(d0, d4)
+    disc.set_sliding_wall(belem, bside, 2.0, 0.1)
+    _, d5 = disc.assemble(u)
+    disc.set_sliding_wall([], [], 0.0, 0.0)
+    J6, d6 = disc.assemble(u)
+    assert not torch.equal(d0, d5)
+    assert torch.equal(J0, J6) and torch.equal(d0, d6)

@@ -5,7 +5,9 @@ and its symmetry condition (incompressible/fv1/bnd/symmetric_boundary_fv1.cpp:
 (interior SCVF normals plus boundary-face normals close), boundary-face
 areas, the Picard identity d = J u, finite differences, linearity, and the
 closed form of the symmetry stress for a linear velocity on an axis-aligned
-side."""
+side; for the sliding wall (incompressible/fv1/bnd/wall_sliding_fv1.cpp:
+229-425) closed forms on axis-aligned sides, including its sums over the
+first dim shapes only, and the rounding-level convective term."""
 from __future__ import annotations
 
 import numpy as np
@@ -184,3 +186,93 @@ def test_symmetry_finite_differences(et):
         e[j] = h
         fd = (O.outflow_elem(p, et, co, 1, u + e, "symmetry")[1] - O.outflow_elem(p, et, co, 1, u - e, "symmetry")[1]) / (2 * h)
         np.testing.assert_allclose(fd, J[:, j], rtol=1e-9, atol=1e-11)
+
+
+def _box(et, h=0.8):
+    lc = {O.QUAD: [[0, 0], [1, 0], [1, 1], [0, 1]],
+          O.HEX: [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]]}[et]
+    return np.asarray(lc, dtype=float) * h + 0.2
+
+
+def _side_where(et, co, axis, low):
+    for s, side in enumerate(M.SIDES[et]):
+        x = co[list(side), axis]
+        if np.allclose(x, co[:, axis].min() if low else co[:, axis].max()):
+            return s
+    raise AssertionError
+
+
+def test_sliding_factor_closed_form_quad():
+    """Bottom side y = min of a box quad: the faces of corners 0 and 1 have
+    their ips at local (1/4, 0) and (3/4, 0), nh = (0, -1); the factor term
+    gives -k (tangential part of u0 phi0 + u1 phi1) = -k (u0x phi0 + u1x phi1, 0)."""
+    et = O.QUAD
+    co = _box(et)
+    s = _side_where(et, co, 1, True)
+    p = O.Params(dim=2, upwind="full", stokes=True)
+    u = np.zeros((4, 3))
+    u[:, 0] = [0.7, -0.4, 5.0, 3.0]
+    u[:, 1] = [0.2, 0.9, -1.0, 2.0]
+    k = 1.7
+    J, d = O.outflow_elem(p, et, co, s, u.reshape(-1), "sliding", k, 0.0)
+    d = d.reshape(4, 3)
+    want = np.zeros((4, 3))
+    want[0, 0] = -k * (0.75 * 0.7 + 0.25 * -0.4)
+    want[1, 0] = -k * (0.25 * 0.7 + 0.75 * -0.4)
+    np.testing.assert_allclose(d, want, rtol=1e-14, atol=1e-15)
+    # Jacobian: -k diag(1, 0) in every column block of the two rows
+    J = J.reshape(4, 3, 4, 3)
+    for c in (0, 1):
+        for sh in range(4):
+            np.testing.assert_allclose(J[c, :, sh, :], [[-k, 0, 0], [0, 0, 0], [0, 0, 0]], atol=1e-15)
+    assert not J[2:].any()
+
+
+def test_sliding_uses_first_dim_shapes_only():
+    """On the top side of a quad (corners 2, 3) the first two shapes vanish at
+    the face ips, so the factor and limit terms of the defect vanish for any u."""
+    et = O.QUAD
+    co = _box(et)
+    s = _side_where(et, co, 1, False)
+    p = O.Params(dim=2, upwind="full", stokes=True)
+    u = np.random.default_rng(3).normal(size=12)
+    _, d = O.outflow_elem(p, et, co, s, u, "sliding", 2.0, 0.5)
+    assert np.abs(d).max() < 1e-15
+
+
+def test_sliding_limit_closed_form():
+    """Limit term: -lim t/|t| with the tangential direction of the velocity;
+    Jacobian -lim/sqrt(dim) on the diagonal of every column block."""
+    for et, axis in ((O.QUAD, 1), (O.HEX, 2)):
+        co = _box(et)
+        dim = O.DIM[et]
+        s = _side_where(et, co, axis, True)
+        p = O.Params(dim=dim, upwind="full", stokes=True)
+        nsh = O.NSH[et]
+        u = np.zeros((nsh, dim + 1))
+        u[:, 0] = 0.4
+        u[:, axis] = -2.0  # normal component: projected out
+        lim = 0.3
+        J, d = O.outflow_elem(p, et, co, s, u.reshape(-1), "sliding", 0.0, lim)
+        d = d.reshape(nsh, dim + 1)
+        on = sorted(M.SIDES[et][s])
+        want = np.zeros((nsh, dim + 1))
+        want[on, 0] = -lim
+        np.testing.assert_allclose(d, want, rtol=1e-14, atol=1e-15)
+        J = J.reshape(nsh, dim + 1, nsh, dim + 1)
+        for c in on:
+            for sh in range(nsh):
+                np.testing.assert_allclose(np.diag(J[c, :dim, sh, :dim]), -lim / np.sqrt(dim), rtol=1e-14)
+
+
+@pytest.mark.parametrize("et", ETS)
+def test_sliding_convective_term_is_rounding_level(et):
+    """(t.n) with t the tangential velocity is zero up to rounding, so the
+    convective term is negligible against |u|^2 |n|."""
+    co = _elem(et, 21)
+    nsh, nc = O.NSH[et], O.DIM[et] + 1
+    p = O.Params(dim=O.DIM[et], upwind="full", rho=1.2)
+    u = np.random.default_rng(2).normal(size=nsh * nc)
+    J, d = O.outflow_elem(p, et, co, 0, u, "sliding", 0.0, 0.0)
+    assert np.abs(d).max() < 1e-14 * np.abs(u).max() ** 2 * 4
+    assert np.abs(J).max() < 1e-14 * np.abs(u).max() * 4
