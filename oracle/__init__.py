@@ -22,8 +22,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "lib", "libfv1oracle.so")
 
 TRI, QUAD, TET, HEX = 0, 1, 2, 3
-UPW = {"no": 0, "full": 1, "skewed": 2, "lps": 3, "linearprofileskewed": 3}
-STAB_FIELDS, STAB_WITHOUT = 0, 1
+UPW = {"no": 0, "full": 1, "skewed": 2, "lps": 3, "linearprofileskewed": 3, "positive": 4}
+STAB_FIELDS, STAB_WITHOUT, STAB_FLOW = 0, 1, 2
 DIFFLEN = {"raw": 0, "fivepoint": 1, "cor": 2}
 NSH = {TRI: 3, QUAD: 4, TET: 4, HEX: 8}
 NIP = {TRI: 3, QUAD: 4, TET: 6, HEX: 12}
@@ -44,7 +44,7 @@ class _Params(C.Structure):
         ("stab_upwind", C.c_int), ("pac", C.c_int), ("difflen", C.c_int),
         ("stokes", C.c_int), ("laplace", C.c_int), ("peclet", C.c_int),
         ("exact", C.c_double), ("nu", C.c_double), ("rho", C.c_double),
-        ("has_source", C.c_int), ("f", C.c_double * 3),
+        ("has_source", C.c_int), ("f", C.c_double * 3), ("force_matrix", C.c_int),
     ]
 
 
@@ -82,12 +82,13 @@ class Params:
     nu: float = 0.01
     rho: float = 1.0
     source: tuple | None = None
+    force_matrix: bool = False           # pinning knob: matrix solve of the ip system without ip shapes
 
     def c(self) -> _Params:
         p = _Params()
         p.dim = self.dim
         p.conv_upwind = -1 if (self.upwind is None or self.pac) else UPW[self.upwind]
-        p.stab_type = {None: -1, "fields": STAB_FIELDS, "without": STAB_WITHOUT}[self.stab]
+        p.stab_type = {None: -1, "fields": STAB_FIELDS, "without": STAB_WITHOUT, "flow": STAB_FLOW}[self.stab]
         su = self.stab_upwind if self.stab_upwind is not None else self.upwind
         p.stab_upwind = -1 if su is None else UPW[su]
         p.pac = int(self.pac)
@@ -95,6 +96,7 @@ class Params:
         p.stokes, p.laplace, p.peclet = int(self.stokes), int(self.laplace), int(self.peclet)
         p.exact, p.nu, p.rho = float(self.exact), float(self.nu), float(self.rho)
         p.has_source = int(self.source is not None)
+        p.force_matrix = int(self.force_matrix)
         if self.source is not None:
             for i, v in enumerate(self.source):
                 p.f[i] = float(v)
@@ -114,7 +116,7 @@ def lib():
         ip64 = C.POINTER(C.c_int64)
         L.or_geometry.argtypes = [C.c_int, dp, C.POINTER(_Geo)]
         L.or_side_ray.argtypes = [C.c_int, dp, dp, dp, C.POINTER(C.c_int), dp, dp]
-        L.or_upwind.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp]
+        L.or_upwind.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, dp]
         L.or_elem_A.argtypes = [C.POINTER(_Params), C.c_int, dp, dp, dp, C.c_double, dp, dp]
         L.or_elem_M.argtypes = [C.POINTER(_Params), C.c_int, dp, dp, dp, dp]
         L.or_elem_rhs.argtypes = [C.POINTER(_Params), C.c_int, dp, dp]
@@ -187,11 +189,16 @@ def side_ray(et: int, corners, frm, direction):
     return side.value, gi[: DIM[et]], li[: DIM[et]]
 
 
-def upwind(kind: str, et: int, corners, ipvel):
+def upwind(kind: str, et: int, corners, ipvel, ip_shapes: bool = False):
+    """Upwind corner shapes U [nip, nsh] and convection lengths [nip]
+    (and, with ip_shapes, the ip shapes W [nip, nip])."""
     co, v = _f64(corners), _f64(ipvel)
     U = np.zeros((12, 8))
+    W = np.zeros((12, 12))
     L = np.zeros(12)
-    _chk(lib().or_upwind(UPW[kind], et, _dp(co), _dp(v), _dp(U), _dp(L)))
+    _chk(lib().or_upwind(UPW[kind], et, _dp(co), _dp(v), _dp(U), _dp(W), _dp(L)))
+    if ip_shapes:
+        return U[: NIP[et], : NSH[et]], W[: NIP[et], : NIP[et]], L[: NIP[et]]
     return U[: NIP[et], : NSH[et]], L[: NIP[et]]
 
 

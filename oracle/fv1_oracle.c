@@ -401,12 +401,91 @@ int or_side_ray(int et, const double* corners, const double* from, const double*
 }
 
 /* ------------------------------------------------------------------ */
-/* upwind shapes: upwind.cpp:52-80 (no), 133-172 (full),                */
-/* 337-430 (skewed + GetNodeNextToCut), 505-575 (LPS)                   */
+/* positive upwind: upwind.cpp:643-786.  Corner shapes U and ip shapes  */
+/* W: the velocity at an ip is taken from the SCV it flows out of, as a */
+/* convex combination of that SCV's corner value and the inflowing ip   */
+/* velocities weighted by their share of the SCV's throughput F =       */
+/* max(inflow, outflow).  Faces with (numerically) no flux take the     */
+/* mean of their two corners.  W is zero for every other upwind.        */
 /* ------------------------------------------------------------------ */
-static int upwind_geo(int type, const or_geo* G, const double (*vel)[3], double (*U)[8], double* convlen)
+static void positive_upwind(const or_geo* G, const double (*vel)[3], double (*U)[8], double (*W)[12],
+                            double* convlen)
 {
     const int dim = G->dim, nip = G->nip, nsh = G->nsh;
+    const double eps = DBL_EPSILON * 10;
+    double mflux[12];
+    int has[12], nno = 0;
+    for (int ip = 0; ip < nip; ++ip) {
+        for (int sh = 0; sh < nsh; ++sh) U[ip][sh] = 0.0;
+        for (int j = 0; j < nip; ++j) W[ip][j] = 0.0;
+        mflux[ip] = 0.0;
+        has[ip] = 1;
+        const double nsq = dot(dim, vel[ip], vel[ip]);
+        if (fabs(nsq) <= eps) {
+            U[ip][G->from[ip]] = 0.5; U[ip][G->to[ip]] = 0.5;
+            has[ip] = 0; ++nno;
+            continue;
+        }
+        mflux[ip] = dot(dim, vel[ip], G->n[ip]);
+        const double v = sqrt(nsq), len = sqrt(dot(dim, G->n[ip], G->n[ip]));
+        if (fabs(mflux[ip] / sqrt(v * len)) <= eps) {
+            U[ip][G->from[ip]] = 0.5; U[ip][G->to[ip]] = 0.5;
+            has[ip] = 0; ++nno;
+        }
+    }
+    if (nno != nip)
+        for (int sh = 0; sh < nsh; ++sh) {
+            double m_in = 0, m_out = 0, fl[12];
+            int ips[12], cnt = 0;
+            for (int ip = 0; ip < nip; ++ip) {
+                if (!has[ip]) continue;
+                double f;
+                if (G->from[ip] == sh) f = mflux[ip];
+                else if (G->to[ip] == sh) f = -1.0 * mflux[ip];
+                else continue;
+                ips[cnt] = ip; fl[cnt] = f; ++cnt;
+                m_in += -1.0 * fmin(f, 0.0);
+                m_out += fmax(f, 0.0);
+            }
+            const double F = fmax(m_in, m_out);
+            for (int i = 0; i < cnt; ++i) {
+                if (!(fl[i] > 0)) continue;
+                double sum = 0.0;
+                for (int j = 0; j < cnt; ++j)
+                    if (fl[j] < 0) {
+                        W[ips[i]][ips[j]] = -1.0 * fl[j] / F;
+                        sum += W[ips[i]][ips[j]];
+                    }
+                U[ips[i]][sh] = 1.0 - sum;
+            }
+        }
+    for (int ip = 0; ip < nip; ++ip) {
+        double up[3] = {0, 0, 0};
+        for (int sh = 0; sh < nsh; ++sh)
+            for (int i = 0; i < dim; ++i) up[i] += U[ip][sh] * G->co[sh][i];
+        for (int j = 0; j < nip; ++j)
+            for (int i = 0; i < dim; ++i) up[i] += W[ip][j] * G->ipg[j][i];
+        convlen[ip] = sqrt(dist2(dim, G->ipg[ip], up));
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* upwind shapes: upwind.cpp:52-80 (no), 133-172 (full),                */
+/* 337-430 (skewed + GetNodeNextToCut), 505-575 (LPS), 643-786          */
+/* (positive).  W (ip shapes) may be NULL for the corner-only upwinds.  */
+/* ------------------------------------------------------------------ */
+static int upwind_geo(int type, const or_geo* G, const double (*vel)[3], double (*U)[8], double (*W)[12],
+                      double* convlen)
+{
+    const int dim = G->dim, nip = G->nip, nsh = G->nsh;
+    if (type == OR_UPW_POSITIVE) {
+        if (!W) { set_err("positive upwind needs ip shapes"); return -4; }
+        positive_upwind(G, vel, U, W, convlen);
+        return 0;
+    }
+    if (W)
+        for (int ip = 0; ip < nip; ++ip)
+            for (int j = 0; j < 12; ++j) W[ip][j] = 0.0;
     double cflat[24];
     for (int k = 0; k < nsh; ++k)
         for (int i = 0; i < dim; ++i) cflat[k * dim + i] = G->co[k][i];
@@ -465,7 +544,7 @@ static int upwind_geo(int type, const or_geo* G, const double (*vel)[3], double 
     return 0;
 }
 
-int or_upwind(int type, int et, const double* corners, const double* ipvel, double* U, double* convlen)
+int or_upwind(int type, int et, const double* corners, const double* ipvel, double* U, double* W, double* convlen)
 {
     or_geo G;
     int rc = or_geometry(et, corners, &G);
@@ -473,11 +552,14 @@ int or_upwind(int type, int et, const double* corners, const double* ipvel, doub
     double vel[12][3] = {{0}};
     for (int ip = 0; ip < G.nip; ++ip)
         for (int i = 0; i < G.dim; ++i) vel[ip][i] = ipvel[ip * G.dim + i];
-    double Ul[12][8];
-    rc = upwind_geo(type, &G, (const double (*)[3])vel, Ul, convlen);
+    double Ul[12][8], Wl[12][12];
+    rc = upwind_geo(type, &G, (const double (*)[3])vel, Ul, Wl, convlen);
     if (rc) return rc;
     for (int ip = 0; ip < G.nip; ++ip)
         for (int k = 0; k < 8; ++k) U[ip * 8 + k] = k < G.nsh ? Ul[ip][k] : 0.0;
+    if (W)
+        for (int ip = 0; ip < G.nip; ++ip)
+            for (int j = 0; j < 12; ++j) W[ip * 12 + j] = j < G.nip ? Wl[ip][j] : 0.0;
     return 0;
 }
 
@@ -528,59 +610,224 @@ static void diff_length(int method, const or_geo* G, double* L2inv)
 /* ------------------------------------------------------------------ */
 typedef struct {
     double vel[12][3];
-    double Svel[12][3][8]; /* stab_shape_vel(ip, d, d, sh) */
-    double Sp[12][3][8];   /* stab_shape_p(ip, d, sh) */
-    double diag[12];       /* diagonal of the FIELDS ip system */
+    double Svel[12][3][3][8]; /* stab_shape_vel(ip, d, d2, sh); d != d2 only when connected */
+    double Sp[12][3][8];      /* stab_shape_p(ip, d, sh) */
+    double diag[12];          /* divisor of the diagonal ip system */
+    double cst[12][3];        /* the part of vel from the source and the old time step */
+    int connected;            /* vel_comp_connected(): FLOW */
+    int nondiag;              /* the ip system was solved as a matrix */
 } stabdata;
+
+/* upwind shapes of one upwind evaluation */
+typedef struct {
+    double U[12][8], W[12][12], len[12];
+} updata;
 
 static void u_at(const double* u, int nc, int d, int sh, double* out) { *out = u[sh * nc + d]; }
 #define UL(d, sh) (uloc[(sh) * nc + (d)])
 #define UO(d, sh) (uold[(sh) * nc + (d)])
 
+/* inverse of the n x n ip system (Gauss-Jordan with partial pivoting);
+   returns 0 if singular (GetInverse failing, stabilization.cpp:289) */
+static int invert(int n, const double (*A)[12], double (*X)[12])
+{
+    double M[12][24];
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < 2 * n; ++j) M[i][j] = j < n ? A[i][j] : (j - n == i ? 1.0 : 0.0);
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (fabs(M[r][c]) > fabs(M[piv][c])) piv = r;
+        if (M[piv][c] == 0.0) return 0;
+        if (piv != c)
+            for (int j = 0; j < 2 * n; ++j) { const double t = M[c][j]; M[c][j] = M[piv][j]; M[piv][j] = t; }
+        const double ip = 1.0 / M[c][c];
+        for (int j = 0; j < 2 * n; ++j) M[c][j] *= ip;
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            const double f = M[r][c];
+            if (f == 0.0) continue;
+            for (int j = 0; j < 2 * n; ++j) M[r][j] -= f * M[c][j];
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) X[i][j] = M[i][n + j];
+    return 1;
+}
+
+/* x = A y */
+static void matvec(int n, const double (*A)[12], const double* y, double* x)
+{
+    for (int i = 0; i < n; ++i) {
+        double a = 0.0;
+        for (int j = 0; j < n; ++j) a += A[i][j] * y[j];
+        x[i] = a;
+    }
+}
+
+/* Stabilisation update: FIELDS stabilization.cpp:122-403 (diagonal case
+   :163-242, matrix case :244-403), FLOW :436-772 (diagonal :486-602, matrix
+   :604-772), without stabilisation :805-850.  `up` holds the upwind of
+   the stabilisation, `dn` its downwind (FLOW); the ip system is a matrix
+   when the upwind has ip shapes (positive upwind) and the equations are
+   not Stokes. */
 static int stab_update(const or_params* p, const or_geo* G, const double* uloc, const double* uold, double dt,
-                       const double (*stdvel)[3], const double (*Ust)[8], const double* convlen_st, stabdata* S)
+                       const double (*stdvel)[3], const updata* up, const updata* dn, int ipshapes, stabdata* S)
 {
     const int dim = G->dim, nip = G->nip, nsh = G->nsh, nc = dim + 1;
     (void)u_at;
+    memset(S->Svel, 0, sizeof S->Svel);
+    memset(S->cst, 0, sizeof S->cst);
+    S->connected = p->stab_type == OR_STAB_FLOW;
+    S->nondiag = 0;
     if (p->stab_type == OR_STAB_WITHOUT) {
         for (int ip = 0; ip < nip; ++ip) {
             for (int i = 0; i < dim; ++i)
-                for (int sh = 0; sh < nsh; ++sh) { S->Sp[ip][i][sh] = 0; S->Svel[ip][i][sh] = G->phi[ip][sh]; }
+                for (int sh = 0; sh < nsh; ++sh) { S->Sp[ip][i][sh] = 0; S->Svel[ip][i][i][sh] = G->phi[ip][sh]; }
             for (int i = 0; i < dim; ++i) S->vel[ip][i] = stdvel[ip][i];
+            S->diag[ip] = 1.0;
         }
         return 0;
     }
-    if (p->stab_type != OR_STAB_FIELDS) { set_err("Stabilization has not been set."); return -3; }
+    if (p->stab_type != OR_STAB_FIELDS && p->stab_type != OR_STAB_FLOW) {
+        set_err("Stabilization has not been set.");
+        return -3;
+    }
+    const int flow = p->stab_type == OR_STAB_FLOW;
     double L2inv[12];
     diff_length(p->difflen, G, L2inv);
-    double vvisc[12], vconv[12];
+    double vvisc[12], vconv[12] = {0}, vdown[12] = {0};
     for (int ip = 0; ip < nip; ++ip) vvisc[ip] = p->nu * L2inv[ip];
     if (!p->stokes)
+        for (int ip = 0; ip < nip; ++ip) {
+            const double nrm = sqrt(dot(dim, stdvel[ip], stdvel[ip]));
+            vconv[ip] = nrm / up->len[ip];
+            if (flow) vdown[ip] = nrm / (dn->len[ip] + up->len[ip]);
+        }
+    double src[3] = {0, 0, 0};
+    if (p->has_source)
+        for (int d = 0; d < dim; ++d) src[d] = p->f[d];
+    double old[12][3] = {{0}};
+    if (uold)
         for (int ip = 0; ip < nip; ++ip)
-            vconv[ip] = sqrt(dot(dim, stdvel[ip], stdvel[ip])) / convlen_st[ip];
-    for (int ip = 0; ip < nip; ++ip) {
-        double diag = vvisc[ip];
-        if (uold) diag += 1. / dt;
-        if (!p->stokes) diag += vconv[ip];
-        S->diag[ip] = diag;
-        for (int d = 0; d < dim; ++d) {
-            double rhs = 0.0;
-            if (p->has_source) rhs = p->f[d];
-            if (uold) {
+            for (int d = 0; d < dim; ++d) {
                 double o = 0.0;
                 for (int sh = 0; sh < nsh; ++sh) o += G->phi[ip][sh] * UO(d, sh);
-                rhs += o / dt;
+                old[ip][d] = o;
             }
+
+    if (p->stokes || !ipshapes) {
+        /* diagonal case: ip by ip */
+        for (int ip = 0; ip < nip; ++ip) {
+            double diag = vvisc[ip];
+            if (uold) diag += 1. / dt;
+            if (!p->stokes) diag += vconv[ip];
+            S->diag[ip] = diag;
+            const double* sv = stdvel[ip];
+            for (int d = 0; d < dim; ++d) {
+                double rhs = src[d];
+                if (uold) rhs += old[ip][d] / dt;
+                S->cst[ip][d] = rhs / diag;
+                for (int k = 0; k < nsh; ++k) {
+                    double sumVel = vvisc[ip] * G->phi[ip][k];
+                    if (!p->stokes) {
+                        sumVel += vconv[ip] * up->U[ip][k];
+                        if (flow) sumVel += vdown[ip] * (dn->U[ip][k] - up->U[ip][k]);
+                    }
+                    if (flow)
+                        for (int d2 = 0; d2 < dim; ++d2)
+                            if (d2 != d) sumVel -= sv[d2] * G->g[ip][k][d2];
+                    rhs += sumVel * UL(d, k);
+                    S->Svel[ip][d][d][k] = sumVel / diag;
+                    if (flow)
+                        for (int d2 = 0; d2 < dim; ++d2) {
+                            if (d2 == d) continue;
+                            const double sumVel2 = sv[d] * G->g[ip][k][d2];
+                            rhs += sumVel2 * UL(d2, k);
+                            S->Svel[ip][d][d2][k] = sumVel2 / diag;
+                        }
+                    const double sumP = -1.0 * G->g[ip][k][d] / p->rho;
+                    rhs += sumP * UL(dim, k);
+                    S->Sp[ip][d][k] = sumP / diag;
+                }
+                S->vel[ip][d] = rhs / diag;
+            }
+        }
+        return 0;
+    }
+
+    /* matrix case */
+    S->nondiag = 1;
+    double mat[12][12] = {{0}}, inv[12][12];
+    for (int ip = 0; ip < nip; ++ip) {
+        if (uold) mat[ip][ip] += 1. / dt;
+        mat[ip][ip] += vvisc[ip];
+        mat[ip][ip] += vconv[ip];
+        for (int ip2 = 0; ip2 < nip; ++ip2) {
+            mat[ip][ip2] -= up->W[ip][ip2] * vconv[ip];
+            if (flow) mat[ip][ip2] += vdown[ip] * (up->W[ip][ip2] - dn->W[ip][ip2]);
+        }
+        S->diag[ip] = mat[ip][ip];
+    }
+    if (!invert(nip, (const double (*)[12])mat, inv)) { set_err("Could not compute inverse."); return -11; }
+    const double (*iv)[12] = (const double (*)[12])inv;
+    for (int d = 0; d < dim; ++d) {
+        double contVel[3][8][12], contP[8][12], x[12], f[12], f0[12], c0[12];
+        for (int ip = 0; ip < nip; ++ip) {
+            const double* sv = stdvel[ip];
             for (int k = 0; k < nsh; ++k) {
-                double sumVel = vvisc[ip] * G->phi[ip][k];
-                if (!p->stokes) sumVel += vconv[ip] * Ust[ip][k];
-                rhs += sumVel * UL(d, k);
-                S->Svel[ip][d][k] = sumVel / diag;
-                const double sumP = -1.0 * G->g[ip][k][d] / p->rho;
-                rhs += sumP * UL(dim, k);
-                S->Sp[ip][d][k] = sumP / diag;
+                contP[k][ip] = -1.0 * G->g[ip][k][d] / p->rho;
+                contVel[d][k][ip] = vvisc[ip] * G->phi[ip][k];
+                contVel[d][k][ip] += vconv[ip] * up->U[ip][k];
+                if (flow) {
+                    contVel[d][k][ip] += vdown[ip] * (dn->U[ip][k] - up->U[ip][k]);
+                    for (int d2 = 0; d2 < dim; ++d2) {
+                        if (d2 == d) continue;
+                        contVel[d][k][ip] -= sv[d2] * G->g[ip][k][d2];
+                        contVel[d2][k][ip] = sv[d] * G->g[ip][k][d2];
+                    }
+                }
             }
-            S->vel[ip][d] = rhs / diag;
+        }
+        for (int k = 0; k < nsh; ++k) {
+            matvec(nip, iv, contP[k], x);
+            for (int ip = 0; ip < nip; ++ip) S->Sp[ip][d][k] = x[ip];
+            for (int d2 = 0; d2 < dim; ++d2) {
+                if (!flow && d2 != d) continue;
+                matvec(nip, iv, contVel[d2][k], x);
+                for (int ip = 0; ip < nip; ++ip) S->Svel[ip][d][d2][k] = x[ip];
+            }
+        }
+        for (int ip = 0; ip < nip; ++ip) {
+            f0[ip] = src[d];
+            if (uold) f0[ip] += old[ip][d] / dt;
+        }
+        if (!flow) {
+            /* FIELDS: source and old time first, then corner contributions */
+            for (int ip = 0; ip < nip; ++ip) f[ip] = f0[ip];
+            for (int k = 0; k < nsh; ++k)
+                for (int ip = 0; ip < nip; ++ip) {
+                    f[ip] = f[ip] + UL(d, k) * contVel[d][k][ip];
+                    f[ip] = f[ip] + UL(dim, k) * contP[k][ip];
+                }
+        } else {
+            /* FLOW: corner contributions first, then source and old time */
+            for (int ip = 0; ip < nip; ++ip) f[ip] = 0.0;
+            for (int k = 0; k < nsh; ++k)
+                for (int ip = 0; ip < nip; ++ip) {
+                    for (int d2 = 0; d2 < dim; ++d2) f[ip] = f[ip] + UL(d2, k) * contVel[d2][k][ip];
+                    f[ip] = f[ip] + UL(dim, k) * contP[k][ip];
+                }
+            for (int ip = 0; ip < nip; ++ip) {
+                if (p->has_source) f[ip] += src[d];
+                if (uold) f[ip] += old[ip][d] / dt;
+            }
+        }
+        matvec(nip, iv, f, x);
+        matvec(nip, iv, f0, c0);
+        for (int ip = 0; ip < nip; ++ip) {
+            S->vel[ip][d] = x[ip];
+            S->cst[ip][d] = c0[ip];
         }
     }
     return 0;
@@ -602,9 +849,36 @@ static double peclet_blend(const or_params* p, const or_geo* G, int ip, const do
    navier_stokes_fv1.cpp:282-308 and 631-658 */
 typedef struct {
     double stdvel[12][3];
-    double Uc[12][8], lc[12]; /* convective upwind (if used) */
+    double Uc[12][8], Wc[12][12], lc[12]; /* convective upwind (if used) */
+    int conv_ip;                          /* the convective upwind has ip shapes */
     stabdata S;
 } prolog;
+
+/* the stabilisation's update: its upwind (and for FLOW its downwind, the
+   same upwind of the negated ip velocities, upwind_interface.h:157-165),
+   then the ip system */
+static int stab_prologue(const or_params* p, const or_geo* G, const double* uloc, const double* uold, double dt,
+                         const double (*stdvel)[3], updata* st, stabdata* S)
+{
+    const int nip = G->nip, dim = G->dim;
+    updata dn;
+    memset(st, 0, sizeof *st);
+    memset(&dn, 0, sizeof dn);
+    for (int ip = 0; ip < 12; ++ip) st->len[ip] = dn.len[ip] = 1.0;
+    if (!p->stokes) {
+        if (p->stab_upwind < 0) { set_err("No upwind object"); return -4; }
+        int rc = upwind_geo(p->stab_upwind, G, stdvel, st->U, st->W, st->len);
+        if (rc) return rc;
+        if (p->stab_type == OR_STAB_FLOW) {
+            double neg[12][3] = {{0}};
+            for (int ip = 0; ip < nip; ++ip)
+                for (int d = 0; d < dim; ++d) neg[ip][d] = -1.0 * stdvel[ip][d];
+            rc = upwind_geo(p->stab_upwind, G, (const double (*)[3])neg, dn.U, dn.W, dn.len);
+            if (rc) return rc;
+        }
+    }
+    return stab_update(p, G, uloc, uold, dt, stdvel, st, &dn, p->stab_upwind == OR_UPW_POSITIVE || p->force_matrix, S);
+}
 
 static int prologue(const or_params* p, const or_geo* G, const double* uloc, const double* uold, double dt, prolog* W)
 {
@@ -619,26 +893,35 @@ static int prologue(const or_params* p, const or_geo* G, const double* uloc, con
         for (int sh = 0; sh < nsh; ++sh)
             for (int d = 0; d < dim; ++d) W->stdvel[ip][d] += UL(d, sh) * G->phi[ip][sh];
     }
-    /* stabilisation (computes its upwind first unless Stokes) */
-    double Ust[12][8] = {{0}}, lst[12];
-    for (int ip = 0; ip < 12; ++ip) lst[ip] = 1.0;
-    if (!p->stokes) {
-        if (p->stab_upwind < 0) { set_err("No upwind object"); return -4; }
-        int rc = upwind_geo(p->stab_upwind, G, (const double (*)[3])W->stdvel, Ust, lst);
-        if (rc) return rc;
-    }
-    int rc = stab_update(p, G, uloc, uold, dt, (const double (*)[3])W->stdvel, (const double (*)[8])Ust, lst, &W->S);
+    updata st;
+    int rc = stab_prologue(p, G, uloc, uold, dt, (const double (*)[3])W->stdvel, &st, &W->S);
     if (rc) return rc;
+    memset(W->Wc, 0, sizeof W->Wc);
+    W->conv_ip = 0;
     if (!p->stokes && !p->pac) {
         if (p->conv_upwind == p->stab_upwind) {
-            memcpy(W->Uc, Ust, sizeof Ust);
-            memcpy(W->lc, lst, sizeof lst);
+            memcpy(W->Uc, st.U, sizeof st.U);
+            memcpy(W->Wc, st.W, sizeof st.W);
+            memcpy(W->lc, st.len, sizeof st.len);
         } else {
-            rc = upwind_geo(p->conv_upwind, G, (const double (*)[3])W->stdvel, W->Uc, W->lc);
+            rc = upwind_geo(p->conv_upwind, G, (const double (*)[3])W->stdvel, W->Uc, W->Wc, W->lc);
             if (rc) return rc;
         }
+        W->conv_ip = p->conv_upwind == OR_UPW_POSITIVE;
     }
     return 0;
+}
+
+/* upwind velocity of the convective term: upwind_interface.h:335-358 */
+static void conv_upwind_vel(const or_geo* G, const double* uloc, const prolog* W, int ip, double* upv)
+{
+    const int dim = G->dim, nsh = G->nsh, nip = G->nip, nc = dim + 1;
+    for (int d = 0; d < dim; ++d) upv[d] = 0.0;
+    for (int k = 0; k < nsh; ++k)
+        for (int d = 0; d < dim; ++d) upv[d] += W->Uc[ip][k] * UL(d, k);
+    if (W->conv_ip)
+        for (int j = 0; j < nip; ++j)
+            for (int d = 0; d < dim; ++d) upv[d] += W->Wc[ip][j] * W->stdvel[j][d];
 }
 
 /* add_jac_A_elem: navier_stokes_fv1.cpp:250-595 */
@@ -670,8 +953,7 @@ static int jac_A(const or_params* p, const or_geo* G, const double* uloc, const 
             if (!p->stokes) {
                 double upv[3] = {0, 0, 0};
                 if (!p->pac) {
-                    for (int k = 0; k < nsh; ++k)
-                        for (int d = 0; d < dim; ++d) upv[d] += W->Uc[ip][k] * UL(d, k);
+                    conv_upwind_vel(G, uloc, W, ip, upv);
                 } else {
                     for (int d = 0; d < dim; ++d) upv[d] = W->S.vel[ip][d];
                 }
@@ -679,16 +961,27 @@ static int jac_A(const or_params* p, const or_geo* G, const double* uloc, const 
                 if (p->peclet) w = peclet_blend(p, G, ip, W->stdvel[ip], upv);
                 const double prod = dot(dim, W->stdvel[ip], n) * rho;
                 if (p->pac) {
-                    for (int d1 = 0; d1 < dim; ++d1) {
-                        const double cv = prod * w * W->S.Svel[ip][d1][sh];
-                        JL(d1, fr, d1, sh) += cv; JL(d1, to, d1, sh) -= cv;
-                    }
+                    if (W->S.connected)
+                        for (int d1 = 0; d1 < dim; ++d1)
+                            for (int d2 = 0; d2 < dim; ++d2) {
+                                const double cv = prod * w * W->S.Svel[ip][d1][d2][sh];
+                                JL(d1, fr, d2, sh) += cv; JL(d1, to, d2, sh) -= cv;
+                            }
+                    else
+                        for (int d1 = 0; d1 < dim; ++d1) {
+                            const double cv = prod * w * W->S.Svel[ip][d1][d1][sh];
+                            JL(d1, fr, d1, sh) += cv; JL(d1, to, d1, sh) -= cv;
+                        }
                     for (int d1 = 0; d1 < dim; ++d1) {
                         const double cp = prod * w * W->S.Sp[ip][d1][sh];
                         JL(d1, fr, P, sh) += cp; JL(d1, to, P, sh) -= cp;
                     }
                 } else {
-                    const double cv = W->Uc[ip][sh] * prod * w;
+                    /* ip shapes folded into corner shapes (:437-447) */
+                    double cv = W->Uc[ip][sh];
+                    if (W->conv_ip)
+                        for (int j = 0; j < nip; ++j) cv += G->phi[j][sh] * W->Wc[ip][j];
+                    cv *= prod * w;
                     for (int d1 = 0; d1 < dim; ++d1) { JL(d1, fr, d1, sh) += cv; JL(d1, to, d1, sh) -= cv; }
                 }
                 if (p->peclet) {
@@ -699,7 +992,11 @@ static int jac_A(const or_params* p, const or_geo* G, const double* uloc, const 
                     if (p->pac) {
                         for (int d1 = 0; d1 < dim; ++d1) {
                             for (int d2 = 0; d2 < dim; ++d2) {
-                                double pv = W->S.Svel[ip][d1][sh] * n[d1];
+                                double pv = 0.0;
+                                if (W->S.connected)
+                                    for (int k = 0; k < dim; ++k) pv += w * W->S.Svel[ip][k][d2][sh] * n[k];
+                                else
+                                    pv = W->S.Svel[ip][d1][d1][sh] * n[d1];
                                 pv *= p->exact * rho;
                                 JL(d1, fr, d2, sh) += pv * upv[d1]; JL(d1, to, d2, sh) -= pv * upv[d1];
                             }
@@ -723,9 +1020,13 @@ static int jac_A(const or_params* p, const or_geo* G, const double* uloc, const 
                             }
                 }
             }
-            /* continuity :561-592 (vel_comp_connected == false) */
+            /* continuity :561-592 */
             for (int d1 = 0; d1 < dim; ++d1) {
-                const double cv = W->S.Svel[ip][d1][sh] * n[d1] * rho;
+                double cv = 0.0;
+                if (W->S.connected)
+                    for (int d2 = 0; d2 < dim; ++d2) cv += W->S.Svel[ip][d2][d1][sh] * n[d2] * rho;
+                else
+                    cv = W->S.Svel[ip][d1][d1][sh] * n[d1] * rho;
                 JL(P, fr, d1, sh) += cv; JL(P, to, d1, sh) -= cv;
             }
             double cpp = 0.0;
@@ -765,8 +1066,7 @@ static int def_A(const or_params* p, const or_geo* G, const double* uloc, const 
         if (!p->stokes) {
             double upv[3] = {0, 0, 0};
             if (!p->pac) {
-                for (int k = 0; k < nsh; ++k)
-                    for (int d = 0; d < dim; ++d) upv[d] += W->Uc[ip][k] * UL(d, k);
+                conv_upwind_vel(G, uloc, W, ip, upv);
             } else {
                 for (int d = 0; d < dim; ++d) upv[d] = W->S.vel[ip][d];
             }
@@ -813,9 +1113,25 @@ static void elem_A_abs(const or_params* p, const or_geo* G, const double* uloc, 
             for (int d1 = 0; d1 < dim; ++d1)
                 for (int d2 = 0; d2 < dim; ++d2) gva[d1][d2] += fabs(G->g[ip][sh][d2] * UL(d1, sh));
         }
+        if (W->conv_ip)
+            for (int j = 0; j < nip; ++j)
+                for (int d = 0; d < dim; ++d) {
+                    double a = 0;
+                    for (int k = 0; k < nsh; ++k) a += fabs(G->phi[j][k] * UL(d, k));
+                    upa[d] += fabs(W->Wc[ip][j]) * a;
+                }
         /* stabilised velocity magnitude bound */
         if (p->stab_type == OR_STAB_WITHOUT) {
             for (int d = 0; d < dim; ++d) sta[d] = sva[d];
+        } else if (W->S.connected || W->S.nondiag) {
+            for (int d = 0; d < dim; ++d) {
+                double a = fabs(W->S.cst[ip][d]);
+                for (int k = 0; k < nsh; ++k) {
+                    for (int d2 = 0; d2 < dim; ++d2) a += fabs(W->S.Svel[ip][d][d2][k] * UL(d2, k));
+                    a += fabs(W->S.Sp[ip][d][k] * UL(P, k));
+                }
+                sta[d] = a;
+            }
         } else {
             const double diag = W->S.diag[ip];
             for (int d = 0; d < dim; ++d) {
@@ -826,7 +1142,7 @@ static void elem_A_abs(const or_params* p, const or_geo* G, const double* uloc, 
                     a += o / dt;
                 }
                 for (int k = 0; k < nsh; ++k) {
-                    a += fabs(W->S.Svel[ip][d][k] * diag * UL(d, k));
+                    a += fabs(W->S.Svel[ip][d][d][k] * diag * UL(d, k));
                     a += fabs(G->g[ip][k][d] / p->rho * UL(P, k));
                 }
                 sta[d] = a / diag;
@@ -875,16 +1191,29 @@ static void elem_A_abs(const or_params* p, const or_geo* G, const double* uloc, 
             if (!p->stokes) {
                 for (int d1 = 0; d1 < dim; ++d1) {
                     if (p->pac) {
-                        ADDJ(d1, d1, sh, proda * w * W->S.Svel[ip][d1][sh]);
+                        if (W->S.connected)
+                            for (int d2 = 0; d2 < dim; ++d2) ADDJ(d1, d2, sh, proda * w * W->S.Svel[ip][d1][d2][sh]);
+                        else
+                            ADDJ(d1, d1, sh, proda * w * W->S.Svel[ip][d1][d1][sh]);
                         ADDJ(d1, P, sh, proda * w * W->S.Sp[ip][d1][sh]);
                     } else {
-                        ADDJ(d1, d1, sh, W->Uc[ip][sh] * proda * w);
+                        double ue = fabs(W->Uc[ip][sh]);
+                        if (W->conv_ip)
+                            for (int j = 0; j < nip; ++j) ue += fabs(G->phi[j][sh] * W->Wc[ip][j]);
+                        ADDJ(d1, d1, sh, ue * proda * w);
                     }
                     if (p->peclet) ADDJ(d1, d1, sh, proda * (1.0 - w) * G->phi[ip][sh]);
                     if (p->exact != 0.0) {
                         for (int d2 = 0; d2 < dim; ++d2) {
-                            const double pv = p->pac ? W->S.Svel[ip][d1][sh] * n[d1] * p->exact * rho
-                                                     : w * W->Uc[ip][sh] * n[d2] * rho;
+                            double pv;
+                            if (p->pac && W->S.connected) {
+                                pv = 0.0;
+                                for (int k = 0; k < dim; ++k) pv += fabs(w * W->S.Svel[ip][k][d2][sh]) * n[k];
+                                pv *= p->exact * rho;
+                            } else {
+                                pv = p->pac ? W->S.Svel[ip][d1][d1][sh] * n[d1] * p->exact * rho
+                                            : w * W->Uc[ip][sh] * n[d2] * rho;
+                            }
                             ADDJ(d1, d2, sh, pv * upv_w[d1]);
                             if (p->peclet) ADDJ(d1, d2, sh, upv_w[d1] * (1.0 - w) * G->phi[ip][sh] * n[d2] * rho);
                         }
@@ -898,7 +1227,13 @@ static void elem_A_abs(const or_params* p, const or_geo* G, const double* uloc, 
             }
             double cpp = 0;
             for (int d1 = 0; d1 < dim; ++d1) {
-                ADDJ(P, d1, sh, W->S.Svel[ip][d1][sh] * n[d1] * rho);
+                if (W->S.connected) {
+                    double a = 0;
+                    for (int d2 = 0; d2 < dim; ++d2) a += fabs(W->S.Svel[ip][d2][d1][sh]) * n[d2];
+                    ADDJ(P, d1, sh, a * rho);
+                } else {
+                    ADDJ(P, d1, sh, W->S.Svel[ip][d1][d1][sh] * n[d1] * rho);
+                }
                 cpp += fabs(W->S.Sp[ip][d1][sh]) * n[d1] * rho;
             }
             ADDJ(P, P, sh, cpp);
@@ -1580,22 +1915,16 @@ int or_export_elem(const or_params* p, int et, const double* corners, const doub
     if (at == 0) {
         if (what == 2) {
             /* the stabilisation's update only (ex_velocity_ip :1146-1189) */
-            double Ust[12][8] = {{0}}, lst[12], stdvel[12][3];
+            double stdvel[12][3];
             stabdata S;
-            for (int ip = 0; ip < G.nip; ++ip) {
-                lst[ip] = 1.0;
+            updata st;
+            for (int ip = 0; ip < G.nip; ++ip)
                 for (int d = 0; d < dim; ++d) {
                     stdvel[ip][d] = 0.0;
                     for (int sh = 0; sh < nsh; ++sh) stdvel[ip][d] += UL(d, sh) * G.phi[ip][sh];
                 }
-            }
             if (p->stab_type < 0) { set_err("Stabilization has not been set."); return -3; }
-            if (!p->stokes) {
-                if (p->stab_upwind < 0) { set_err("No upwind object"); return -4; }
-                rc = upwind_geo(p->stab_upwind, &G, (const double (*)[3])stdvel, Ust, lst);
-                if (rc) return rc;
-            }
-            rc = stab_update(p, &G, uloc, uold, dt, (const double (*)[3])stdvel, (const double (*)[8])Ust, lst, &S);
+            rc = stab_prologue(p, &G, uloc, uold, dt, (const double (*)[3])stdvel, &st, &S);
             if (rc) return rc;
             for (int ip = 0; ip < G.nip; ++ip)
                 for (int d = 0; d < dim; ++d) out[ip * ncomp + d] = S.vel[ip][d];

@@ -30,8 +30,8 @@ extern "C" {
 #endif
 
 enum { OR_TRI = 0, OR_QUAD = 1, OR_TET = 2, OR_HEX = 3 };
-enum { OR_UPW_NONE_SET = -1, OR_UPW_NO = 0, OR_UPW_FULL = 1, OR_UPW_SKEWED = 2, OR_UPW_LPS = 3 };
-enum { OR_STAB_NONE_SET = -1, OR_STAB_FIELDS = 0, OR_STAB_WITHOUT = 1 };
+enum { OR_UPW_NONE_SET = -1, OR_UPW_NO = 0, OR_UPW_FULL = 1, OR_UPW_SKEWED = 2, OR_UPW_LPS = 3, OR_UPW_POSITIVE = 4 };
+enum { OR_STAB_NONE_SET = -1, OR_STAB_FIELDS = 0, OR_STAB_WITHOUT = 1, OR_STAB_FLOW = 2 };
 enum { OR_DL_RAW = 0, OR_DL_FIVEPOINT = 1, OR_DL_COR = 2 };
 
 typedef struct {
@@ -46,6 +46,7 @@ typedef struct {
     double nu, rho;
     int has_source;
     double f[3];
+    int force_matrix;  /* pinning knob: solve the ip system as a matrix even without ip shapes */
 } or_params;
 
 /* FV1 geometry of one element (restated ugcore FV1Geometry). */
@@ -70,9 +71,11 @@ int or_geometry(int et, const double* corners, or_geo* out);
 int or_side_ray(int et, const double* corners, const double* from, const double* dir,
                 int* side, double* glob, double* loc);
 
-/* Upwind shapes of one element: U[nip][8], convlen[nip]. */
+/* Upwind shapes of one element: corner shapes U[nip][8], ip shapes
+   W[nip][12] (NULL: not wanted; nonzero only for the positive upwind),
+   convlen[nip]. */
 int or_upwind(int type, int et, const double* corners, const double* ipvel /*[nip][dim]*/,
-              double* U, double* convlen);
+              double* U, double* W, double* convlen);
 
 /* Local A-part defect and Jacobian of one element.
    uloc/uold: [nsh][dim+1] (node-major, DoF interleave); uold may be NULL.
