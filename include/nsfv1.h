@@ -105,13 +105,17 @@ int nsfv1_set_grad_div(nsfv1_handle* h, double factor);
 
 /* NavierStokesFV1::set_upwind(name): navier_stokes_fv1.h:213-215 with the
    factory upwind_interface.cpp:43-62 (keys trimmed, case-insensitive: no,
-   full, skewed, linearprofileskewed|lps; positive|pos and regular|reg are
-   recognised but not provided by this build). */
+   full, skewed, linearprofileskewed|lps, positive|pos; regular|reg, 2D only,
+   is recognised but not provided by this build: it needs ugcore's SCVF-of-SCV
+   ray intersection).  The positive upwind has ip shapes: the assembly then
+   solves the stabilisation's numIp x numIp system per element. */
 int nsfv1_set_upwind(nsfv1_handle* h, const char* name);
 
 /* set_stabilization(name[, diffLength]): navier_stokes_fv1.h:189-200 with
    the factories stabilization.cpp:46-57 (fields, flow) and :86-100 (raw,
-   fivepoint, cor).  diff_length may be NULL (keeps RAW, stabilization.h:321-325). */
+   fivepoint, cor).  diff_length may be NULL (keeps RAW, stabilization.h:321-325).
+   FLOW (stabilization.cpp:436-772) couples the velocity components and uses
+   the downwind of its upwind as well. */
 int nsfv1_set_stabilization(nsfv1_handle* h, const char* name, const char* diff_length);
 
 /* set_stabilization(NavierStokesFV1WithoutStabilization): the object form

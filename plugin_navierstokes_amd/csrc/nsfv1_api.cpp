@@ -125,6 +125,7 @@ struct nsfv1_handle {
     size_t lds_cap = 64 * 1024;
     Plan plan32, plan64;
     // Newton-loop linear algebra
+    DevBuf cpl_recs;  // coupled path: per-SCVF row blocks
     DevBuf dir_rows, gm_V, gm_Binv, gm_part, gm_small, gm_w;
     int64_t ndir = 0;
     int gm_restart = 0;
@@ -284,9 +285,10 @@ static int parse_upwind(const char* name, int* type)
     if (n == "full") { *type = UPW_FULL; return NSFV1_OK; }
     if (n == "skewed") { *type = UPW_SKEWED; return NSFV1_OK; }
     if (n == "linearprofileskewed" || n == "lps") { *type = UPW_LPS; return NSFV1_OK; }
-    if (n == "positive" || n == "pos" || n == "regular" || n == "reg")
+    if (n == "positive" || n == "pos") { *type = UPW_POSITIVE; return NSFV1_OK; }
+    if (n == "regular" || n == "reg")
         return fail(NSFV1_ERR_NOT_IMPLEMENTED, "NavierStokes: upwind type '" + std::string(name) +
-                                                   "' needs the coupled ip system of the positive/regular upwinds, "
+                                                   "' (2D only) needs ugcore's SCVF-of-SCV ray intersection, "
                                                    "which this build does not provide");
     return fail(NSFV1_ERR_UNKNOWN_NAME, "NavierStokes: upwind type '" + std::string(name ? name : "") +
                                             "' not found. Options are: no, full, weighted, skewed, "
@@ -310,10 +312,7 @@ int nsfv1_set_stabilization(nsfv1_handle* h, const char* name, const char* diff_
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
     const std::string n = trim_lower(name);
-    if (n == "flow")
-        return fail(NSFV1_ERR_NOT_IMPLEMENTED, "NavierStokes: the FLOW stabilization (coupled velocity components) "
-                                               "is not provided by this build");
-    if (n != "fields")
+    if (n != "fields" && n != "flow")
         return fail(NSFV1_ERR_UNKNOWN_NAME, "NavierStokes: stabilization type '" + std::string(name ? name : "") +
                                                 "' not a valid name of a Schneider-Raw stabilization. Options are: fields, flow");
     int dl = DL_RAW;
@@ -326,7 +325,7 @@ int nsfv1_set_stabilization(nsfv1_handle* h, const char* name, const char* diff_
     }
     // navier_stokes_fv1.h:190-200: a new stabilisation object, handed the
     // convective upwind if one is set
-    h->stab_type = STAB_FIELDS;
+    h->stab_type = n == "flow" ? STAB_FLOW : STAB_FIELDS;
     h->difflen = dl;
     h->stab_upwind = (h->conv_upwind != UPW_UNSET) ? h->conv_upwind : UPW_UNSET;
     return NSFV1_OK;
@@ -495,6 +494,7 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
     h->nu_elem.release();
     h->ninst = 0;
     h->recs.release();
+    h->cpl_recs.release();
     return NSFV1_OK;
 }
 
@@ -598,6 +598,16 @@ static int make_plan(nsfv1_handle* h, int jbytes, Plan& P)
     return NSFV1_OK;
 }
 
+// configurations whose SCVFs are coupled within an element: the FLOW
+// stabilisation (connected velocity components) and upwinds with ip shapes
+static bool coupled_config(const nsfv1_handle* h)
+{
+    if (h->stab_type == STAB_FLOW) return true;
+    if (h->stokes) return false;
+    if (h->stab_type != STAB_WITHOUT && h->stab_upwind == UPW_POSITIVE) return true;
+    return !h->pac && h->conv_upwind == UPW_POSITIVE;
+}
+
 // per-element SCV volumes and diffusion-length factors, built on first use
 // and again when the diffusion-length method changes
 static int ensure_geometry(nsfv1_handle* h, void* stream)
@@ -684,7 +694,16 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
                 h->vol.as<double>(), h->upw_dec.as<uint8_t>(), h->affine.as<uint8_t>(), h->inc_pos.as<uint64_t>(),
                 P.perm.as<int32_t>(), h->nu_elem.as<double>(), P.block};
     hipError_t e;
-    if (h->path == 0 && jprec == NSFV1_F32 && jvals && (parts & NSFV1_PART_A) && split_supported(h->et, k)) {
+    if (coupled_config(h)) {
+        if (h->upw_mode != NSFV1_UPWIND_FRESH)
+            return fail(NSFV1_ERR_NOT_IMPLEMENTED, "recorded upwind decisions are not available with the FLOW "
+                                                   "stabilization or the positive upwind");
+        const size_t need = (size_t)coupled_record_bytes(h->et, jprec, h->ne);
+        if (h->cpl_recs.n < need && h->cpl_recs.alloc(need) != hipSuccess)
+            return fail(NSFV1_ERR_DEVICE, "coupled record buffer");
+        e = launch_assemble_coupled(h->et, jprec, m, k, u, u_old, jvals, defect, h->errflag.as<int>(), P.nwg, P.lds,
+                                    h->ne, h->cpl_recs.p, nullptr, (hipStream_t)stream);
+    } else if (h->path == 0 && jprec == NSFV1_F32 && jvals && (parts & NSFV1_PART_A) && split_supported(h->et, k)) {
         const size_t need = sizeof(float) * (size_t)split_record_floats(h->et, h->ne);
         if (h->recs.n < need && h->recs.alloc(need) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "record buffer");
         e = launch_assemble_split(h->et, m, k, u, u_old, static_cast<float*>(jvals), defect, h->errflag.as<int>(), P.nwg,
@@ -888,7 +907,15 @@ int nsfv1_export(nsfv1_handle* h, int what, int at, const double* local_ips, int
         m.vol = h->vol.as<double>();
         m.affine = h->affine.as<uint8_t>();
         m.nu_elem = h->nu_elem.as<double>();
-        hipError_t e = launch_export_stab_vel(h->et, m, k, u, u_old, h->ne, out, h->errflag.as<int>(), (hipStream_t)stream);
+        hipError_t e;
+        if (coupled_config(h)) {
+            k.conv_upwind = h->conv_upwind;
+            k.pac = h->pac;
+            e = launch_assemble_coupled(h->et, NSFV1_F64, m, k, u, u_old, nullptr, nullptr, h->errflag.as<int>(), 0, 0,
+                                        h->ne, nullptr, out, (hipStream_t)stream);
+        } else {
+            e = launch_export_stab_vel(h->et, m, k, u, u_old, h->ne, out, h->errflag.as<int>(), (hipStream_t)stream);
+        }
         if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("export launch: ") + hipGetErrorString(e));
         return NSFV1_OK;
     }

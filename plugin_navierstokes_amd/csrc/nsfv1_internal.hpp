@@ -47,6 +47,14 @@ hipError_t launch_assemble_split(int et, const KMeshPtrs& m, const KParams& p, c
 hipError_t launch_export_stab_vel(int et, const KMeshPtrs& m, const KParams& p, const double* u, const double* uold,
                                   int64_t ne, double* out, int* err, hipStream_t stream);
 
+// coupled path (FLOW stabilisation, upwinds with ip shapes): per-SCVF row
+// blocks in `recs` (coupled_record_bytes), then the row gather; with
+// vel_out set, only the stabilised SCVF velocities out[ne][nip][dim]
+int64_t coupled_record_bytes(int et, int jprec, int64_t ne);
+hipError_t launch_assemble_coupled(int et, int jprec, const KMeshPtrs& m, const KParams& p, const double* u,
+                                   const double* uold, void* J, double* d, int* err, int nwg, size_t lds, int64_t ne,
+                                   void* recs, double* vel_out, hipStream_t stream);
+
 hipError_t launch_geometry_cache(int et, const double* xyz, const int32_t* conn, int64_t ne, int difflen,
                                  double* l2inv, double* vol, uint8_t* affine, int* err, hipStream_t stream);
 

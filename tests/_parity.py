@@ -32,10 +32,38 @@ def oracle_params(dim, cfg: dict) -> O.Params:
                     source=cfg.get("source"))
 
 
+def run_case(cuda, et, cfg, mesh=None, seed=5, time_dep=False, parts=1, s=(1.0, 0.0, 0.0), jprec=64):
+    """GPU assembly through the C ABI and the oracle's on the same seeded
+    state: (J, d, J_oracle, d_oracle, J_scale, d_scale, disc)."""
+    import torch
+    from plugin_navierstokes_amd import NavierStokesFV1, PART_A, PART_RHS
+
+    mesh = mesh or small_mesh(et)
+    dim = mesh.dim
+    u = M.random_state(mesh, seed)
+    uold = M.random_state(mesh, seed + 100) if time_dep else None
+    dt = 0.01 if time_dep else 0.0
+    disc = NavierStokesFV1("u,v,p" if dim == 2 else "u,v,w,p", "Inner")
+    configure_disc(disc, cfg)
+    disc.set_mesh(mesh)
+    ut = torch.from_numpy(u).to(cuda)
+    uot = torch.from_numpy(uold).to(cuda) if time_dep else None
+    J, d = disc.assemble(ut, uot, dt, s_a=s[0], s_m=s[1], s_f=s[2], parts=parts, jprec=jprec)
+    rowptr, colidx = disc.pattern()
+    p = oracle_params(dim, cfg)
+    Jr, dr = O.assemble(p, et, mesh.xyz, mesh.conn, u, uold, dt, s_a=s[0], s_m=s[1], s_f=s[2], parts=parts,
+                        rowptr=rowptr, colidx=colidx)
+    Ja, da = O.assemble_abs(p, et, mesh.xyz, mesh.conn, u, uold, dt, s_a=s[0], s_m=s[1], parts=parts,
+                            rowptr=rowptr, colidx=colidx)
+    if parts & PART_RHS and cfg.get("source") is not None:
+        da = da + np.abs(dr)
+    return J.cpu().numpy(), d.cpu().numpy(), Jr, dr, Ja, da, disc
+
+
 def configure_disc(disc, cfg: dict):
     """Apply a case configuration through the reference-style setters."""
-    if cfg.get("stab", "fields") == "fields":
-        disc.set_stabilization("fields", cfg.get("difflen", "raw"))
+    if cfg.get("stab", "fields") in ("fields", "flow"):
+        disc.set_stabilization(cfg.get("stab", "fields"), cfg.get("difflen", "raw"))
     else:
         disc.set_stabilization_without()
     if cfg.get("upwind", "full") is not None:

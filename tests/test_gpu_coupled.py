@@ -1,0 +1,120 @@
+"""GPU parity of the coupled path -- the FLOW stabilisation
+(incompressible/fv1/stabilization.cpp:436-772), the positive upwind
+(upwind.cpp:643-786) with the matrix case of the stabilisation
+(stabilization.cpp:244-403) and ip shapes in the convective term
+(navier_stokes_fv1.cpp:430-457) -- through the C ABI against the CPU oracle,
+every element type, FP64 and FP32 Jacobians, steady and time-dependent,
+with the mass and source parts, and the stabilised-velocity export.
+
+Tolerances as in the other parity tests: FP64 |a-b| <= 1e-12|b| + 1e-13 S,
+FP32 Jacobian |a-b| <= 2^-17 S + 2^-20 |b| (S: the oracle's sum of term
+magnitudes)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+from plugin_navierstokes_amd import F32, F64, PART_A, PART_M, PART_RHS, NavierStokesFV1, NSFV1Error
+from plugin_navierstokes_amd import meshes as M
+from tests._parity import (FP32_ABS, FP32_REL, configure_disc, fp32_violation, fp64_violation, oracle_params,
+                           run_case, small_mesh, worst)
+
+pytestmark = pytest.mark.gpu
+
+ETS = [M.TRI, M.QUAD, M.TET, M.HEX]
+ET_NAMES = {M.TRI: "tri", M.QUAD: "quad", M.TET: "tet", M.HEX: "hex"}
+
+CASES = [
+    {"upwind": "positive"},
+    {"upwind": "positive", "difflen": "cor"},
+    {"upwind": "positive", "stab": "flow"},
+    {"upwind": "full", "stab": "flow"},
+    {"upwind": "no", "stab": "flow", "difflen": "fivepoint"},
+    {"upwind": "skewed", "stab": "flow"},
+    {"upwind": "lps", "stab": "flow"},
+    {"upwind": "positive", "pac": True},
+    {"upwind": "positive", "stab": "flow", "pac": True},
+    {"upwind": "full", "stab": "flow", "pac": True, "exact": 1.0},
+    {"upwind": "positive", "exact": 0.5, "peclet": True},
+    {"upwind": "full", "stab_upwind": "positive"},
+    {"upwind": "positive", "stab_upwind": "lps"},
+    {"upwind": "positive", "stab": "without"},
+    {"upwind": "full", "stab": "flow", "stokes": True},
+    {"upwind": "positive", "stab": "flow", "laplace": True, "rho": 1.4, "nu": 0.01},
+]
+
+
+def _cid(c):
+    return "-".join(f"{k}={v}" for k, v in c.items())
+
+
+@pytest.mark.parametrize("cfg", CASES, ids=_cid)
+@pytest.mark.parametrize("et", ETS, ids=lambda e: ET_NAMES[e])
+def test_coupled_A_part_fp64(cuda, et, cfg):
+    print(f"case et={ET_NAMES[et]} cfg={cfg}", flush=True)
+    J, d, Jr, dr, Ja, da, _ = run_case(cuda, et, cfg)
+    assert fp64_violation(J, Jr, Ja) <= 1.0, worst(J, Jr, Ja)
+    assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+
+
+@pytest.mark.parametrize("cfg", [{"upwind": "positive", "source": (0.1, 0.2, -0.3)},
+                                 {"upwind": "lps", "stab": "flow", "source": (0.3, -0.1, 0.2)},
+                                 {"upwind": "positive", "stab": "flow"}], ids=_cid)
+@pytest.mark.parametrize("et", ETS, ids=lambda e: ET_NAMES[e])
+def test_coupled_time_dependent(cuda, et, cfg):
+    print(f"case et={ET_NAMES[et]} cfg={cfg}", flush=True)
+    J, d, Jr, dr, Ja, da, _ = run_case(cuda, et, cfg, time_dep=True, parts=PART_A | PART_M | PART_RHS,
+                                       s=(0.7, 1.3, -0.7))
+    assert fp64_violation(J, Jr, Ja) <= 1.0, worst(J, Jr, Ja)
+    assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+
+
+@pytest.mark.parametrize("cfg", [{"upwind": "positive"}, {"upwind": "skewed", "stab": "flow"}], ids=_cid)
+@pytest.mark.parametrize("et", ETS, ids=lambda e: ET_NAMES[e])
+def test_coupled_fp32_jacobian(cuda, et, cfg):
+    J, d, Jr, dr, Ja, da, _ = run_case(cuda, et, cfg, jprec=F32)
+    assert J.dtype == np.float32
+    assert fp32_violation(J, Jr, Ja) <= 1.0, worst(J, Jr, Ja, FP32_REL, FP32_ABS)
+    assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+
+
+@pytest.mark.parametrize("et", ETS, ids=lambda e: ET_NAMES[e])
+def test_coupled_repeatable_and_larger_mesh(cuda, et):
+    import torch
+
+    mesh = small_mesh(et, n=8, seed=11, jitter=0.08)
+    cfg = {"upwind": "positive", "stab": "flow"}
+    J, d, Jr, dr, Ja, da, disc = run_case(cuda, et, cfg, mesh=mesh, seed=12)
+    assert fp64_violation(J, Jr, Ja) <= 1.0, worst(J, Jr, Ja)
+    assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+    u = torch.from_numpy(M.random_state(mesh, 12)).to(cuda)
+    J1, d1 = disc.assemble(u)
+    J2, d2 = disc.assemble(u)
+    assert torch.equal(J1, J2) and torch.equal(d1, d2)
+
+
+@pytest.mark.parametrize("cfg", [{"upwind": "positive"}, {"upwind": "positive", "stab": "flow"},
+                                 {"upwind": "skewed", "stab": "flow"}], ids=_cid)
+@pytest.mark.parametrize("et", ETS, ids=lambda e: ET_NAMES[e])
+def test_coupled_stabilised_velocity_export(cuda, et, cfg):
+    import torch
+
+    mesh = small_mesh(et, 5, seed=6, jitter=0.12)
+    disc = NavierStokesFV1("u,v,p" if mesh.dim == 2 else "u,v,w,p", "Inner")
+    configure_disc(disc, cfg)
+    disc.set_mesh(mesh)
+    u = M.random_state(mesh, 41)
+    uold = M.random_state(mesh, 42)
+    got = disc.export(torch.from_numpy(u).to(cuda), "velocity_ip", "scvf", u_old=torch.from_numpy(uold).to(cuda),
+                      dt=0.02).cpu().numpy()
+    want = O.export(oracle_params(mesh.dim, cfg), et, mesh.xyz, mesh.conn, u, "velocity_ip", "scvf", None, uold, 0.02)
+    tol = 1e-12 * np.abs(want) + 1e-12 * np.abs(want).max()
+    assert np.max(np.abs(got - want) / tol) <= 1.0
+
+
+def test_regular_upwind_not_provided(cuda):
+    disc = NavierStokesFV1("u,v,p", "Inner")
+    with pytest.raises(NSFV1Error) as e:
+        disc.set_upwind("regular")
+    assert e.value.name == "NOT_IMPLEMENTED"

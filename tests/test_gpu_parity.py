@@ -16,36 +16,12 @@ from plugin_navierstokes_amd import NavierStokesFV1, NSFV1Error, PART_A, PART_M,
 from plugin_navierstokes_amd import meshes as M
 from tests._parity import (configure_disc, fp32_violation, fp64_violation, max_rel_err, oracle_params,
                            small_mesh, worst, FP32_REL, FP32_ABS)
+from tests._parity import run_case as _run
 
 pytestmark = pytest.mark.gpu
 
 ETS = [M.TRI, M.QUAD, M.TET, M.HEX]
 ET_NAMES = {M.TRI: "tri", M.QUAD: "quad", M.TET: "tet", M.HEX: "hex"}
-
-
-def _run(cuda, et, cfg, mesh=None, seed=5, time_dep=False, parts=PART_A, s=(1.0, 0.0, 0.0), jprec=F64):
-    import torch
-
-    mesh = mesh or small_mesh(et)
-    dim = mesh.dim
-    u = M.random_state(mesh, seed)
-    uold = M.random_state(mesh, seed + 100) if time_dep else None
-    dt = 0.01 if time_dep else 0.0
-    disc = NavierStokesFV1("u,v,p" if dim == 2 else "u,v,w,p", "Inner")
-    configure_disc(disc, cfg)
-    disc.set_mesh(mesh)
-    ut = torch.from_numpy(u).to(cuda)
-    uot = torch.from_numpy(uold).to(cuda) if time_dep else None
-    J, d = disc.assemble(ut, uot, dt, s_a=s[0], s_m=s[1], s_f=s[2], parts=parts, jprec=jprec)
-    rowptr, colidx = disc.pattern()
-    p = oracle_params(dim, cfg)
-    Jr, dr = O.assemble(p, et, mesh.xyz, mesh.conn, u, uold, dt, s_a=s[0], s_m=s[1], s_f=s[2], parts=parts,
-                        rowptr=rowptr, colidx=colidx)
-    Ja, da = O.assemble_abs(p, et, mesh.xyz, mesh.conn, u, uold, dt, s_a=s[0], s_m=s[1], parts=parts,
-                            rowptr=rowptr, colidx=colidx)
-    if parts & PART_RHS and cfg.get("source") is not None:
-        da = da + np.abs(dr)
-    return J.cpu().numpy(), d.cpu().numpy(), Jr, dr, Ja, da, disc
 
 
 @pytest.mark.parametrize("et", ETS, ids=lambda e: ET_NAMES[e])
