@@ -718,10 +718,12 @@ coupled_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const JT* __r
     const int jlen = (int)(M.rowptr[(int64_t)v1 * NC] - jbase);
     const int dlen = (v1 - v0) * NC;
     JT* jbuf = reinterpret_cast<JT*>(smem);
-    double* dbuf = reinterpret_cast<double*>(smem + ((jlen * (int)sizeof(JT) + 15) & ~15));
+    // one pad element after each node's rows: consecutive nodes then start
+    // in different LDS banks, so the lanes of one round do not conflict
+    double* dbuf = reinterpret_cast<double*>(smem + (((jlen + (v1 - v0)) * (int)sizeof(JT) + 15) & ~15));
     const bool wantJ = J != nullptr;
     if (wantJ)
-        for (int i = threadIdx.x; i < jlen; i += blockDim.x) jbuf[i] = JT(0);
+        for (int i = threadIdx.x; i < jlen + (v1 - v0); i += blockDim.x) jbuf[i] = JT(0);
     for (int i = threadIdx.x; i < dlen; i += blockDim.x) dbuf[i] = 0.0;
 
     const int i0 = M.node_inc[v0], i1 = M.node_inc[v1];
@@ -795,7 +797,7 @@ coupled_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const JT* __r
     int rbase = 0, rlen = 0;
     if (active) {
         const int64_t rs = M.rowptr[(int64_t)v * NC];
-        rbase = (int)(rs - jbase);
+        rbase = (int)(rs - jbase) + (v - v0);
         rlen = (int)(M.rowptr[(int64_t)v * NC + 1] - rs);
     }
     __syncthreads();
@@ -818,7 +820,11 @@ coupled_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const JT* __r
         __syncthreads();
     }
     if (wantJ)
-        for (int i = threadIdx.x; i < jlen; i += blockDim.x) J[jbase + i] = jbuf[i];
+        for (int j = 0; j < v1 - v0; ++j) {
+            const int off = (int)(M.rowptr[(int64_t)(v0 + j) * NC] - jbase);
+            const int len = (int)(M.rowptr[(int64_t)(v0 + j + 1) * NC] - jbase) - off;
+            for (int i = threadIdx.x; i < len; i += blockDim.x) J[jbase + off + i] = jbuf[off + j + i];
+        }
     if (dout)
         for (int i = threadIdx.x; i < dlen; i += blockDim.x) dout[(int64_t)v0 * NC + i] = dbuf[i];
 }
