@@ -563,7 +563,7 @@ static int make_plan(nsfv1_handle* h, int jbytes, Plan& P)
         while (v < h->nv) {
             const int64_t incs = h->h_node_inc[v + 1] - h->h_node_inc[v0];
             const int64_t vals = h->h_rowptr[(v + 1) * nc] - h->h_rowptr[v0 * nc];
-            const size_t lds = ((size_t)(vals + (v + 1 - v0)) * jbytes + 15) / 16 * 16 + (size_t)(v + 1 - v0) * nc * 8;
+            const size_t lds = ((size_t)vals * jbytes + (size_t)(v + 1 - v0) * 16 + 15) / 16 * 16 + (size_t)(v + 1 - v0) * nc * 8;
             if (incs > h->max_threads || lds > h->lds_cap) {
                 if (v == v0) return fail(NSFV1_ERR_ARGUMENT, "one node row exceeds the workgroup budget");
                 break;

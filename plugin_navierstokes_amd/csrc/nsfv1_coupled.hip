@@ -718,12 +718,14 @@ coupled_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const JT* __r
     const int jlen = (int)(M.rowptr[(int64_t)v1 * NC] - jbase);
     const int dlen = (v1 - v0) * NC;
     JT* jbuf = reinterpret_cast<JT*>(smem);
-    // one pad element after each node's rows: consecutive nodes then start
-    // in different LDS banks, so the lanes of one round do not conflict
-    double* dbuf = reinterpret_cast<double*>(smem + (((jlen + (v1 - v0)) * (int)sizeof(JT) + 15) & ~15));
+    // 16 pad bytes after each node's rows: consecutive nodes then start in
+    // different LDS banks, so the lanes of one round do not conflict, and
+    // every (row, column block) stays 16-byte aligned for vector access
+    constexpr int PADE = 16 / (int)sizeof(JT);
+    double* dbuf = reinterpret_cast<double*>(smem + (((jlen + (v1 - v0) * PADE) * (int)sizeof(JT) + 15) & ~15));
     const bool wantJ = J != nullptr;
     if (wantJ)
-        for (int i = threadIdx.x; i < jlen + (v1 - v0); i += blockDim.x) jbuf[i] = JT(0);
+        for (int i = threadIdx.x; i < jlen + (v1 - v0) * PADE; i += blockDim.x) jbuf[i] = JT(0);
     for (int i = threadIdx.x; i < dlen; i += blockDim.x) dbuf[i] = 0.0;
 
     const int i0 = M.node_inc[v0], i1 = M.node_inc[v1];
@@ -797,7 +799,7 @@ coupled_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const JT* __r
     int rbase = 0, rlen = 0;
     if (active) {
         const int64_t rs = M.rowptr[(int64_t)v * NC];
-        rbase = (int)(rs - jbase) + (v - v0);
+        rbase = (int)(rs - jbase) + (v - v0) * PADE;
         rlen = (int)(M.rowptr[(int64_t)v * NC + 1] - rs);
     }
     __syncthreads();
@@ -809,9 +811,7 @@ coupled_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const JT* __r
                 for (int a = 0; a < NC; ++a) {
                     const int ra = rbase + a * rlen;
 #pragma unroll
-                    for (int k = 0; k < N; ++k)
-#pragma unroll
-                        for (int b = 0; b < NC; ++b) jbuf[ra + pos[k] + b] += jacc[a][k][b];
+                    for (int k = 0; k < N; ++k) lds_add_block<NC>(jbuf + ra + pos[k], jacc[a][k]);
                 }
             }
 #pragma unroll
@@ -823,7 +823,7 @@ coupled_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const JT* __r
         for (int j = 0; j < v1 - v0; ++j) {
             const int off = (int)(M.rowptr[(int64_t)(v0 + j) * NC] - jbase);
             const int len = (int)(M.rowptr[(int64_t)(v0 + j + 1) * NC] - jbase) - off;
-            for (int i = threadIdx.x; i < len; i += blockDim.x) J[jbase + off + i] = jbuf[off + j + i];
+            for (int i = threadIdx.x; i < len; i += blockDim.x) J[jbase + off + i] = jbuf[off + j * PADE + i];
         }
     if (dout)
         for (int i = threadIdx.x; i < dlen; i += blockDim.x) dout[(int64_t)v0 * NC + i] = dbuf[i];

@@ -1,6 +1,6 @@
 // nsfv1_ray.hpp -- device helpers shared by the assembly kernels: small
-// vector operations, the element centre and the side/ray intersection of
-// the skewed upwinds.
+// vector operations, the element centre, the side/ray intersection of the
+// skewed upwinds and the LDS row accumulation.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -129,6 +129,29 @@ __device__ inline void side_loop(const double (&co)[Topo<ET>::nsh][Topo<ET>::dim
         try_side<ET, S, 0>(co, from, dir, side, gi, li);
         try_side<ET, S, 1>(co, from, dir, side, gi, li);
         side_loop<ET, S + 1>(co, from, dir, side, gi, li);
+    }
+}
+
+// Adds a column block of NC values to a row in LDS.  Blocks of four are
+// 16-byte aligned (rows are padded to 16 bytes), so they move as 128-bit
+// LDS loads and stores.
+template <int NC, typename T>
+__device__ inline void lds_add_block(T* p, const T (&x)[NC])
+{
+    if constexpr (NC == 4 && sizeof(T) == 4) {
+        float4* q = reinterpret_cast<float4*>(p);
+        float4 t = *q;
+        t.x += x[0]; t.y += x[1]; t.z += x[2]; t.w += x[3];
+        *q = t;
+    } else if constexpr (NC == 4 && sizeof(T) == 8) {
+        double2* q = reinterpret_cast<double2*>(p);
+        double2 t0 = q[0], t1 = q[1];
+        t0.x += x[0]; t0.y += x[1]; t1.x += x[2]; t1.y += x[3];
+        q[0] = t0;
+        q[1] = t1;
+    } else {
+#pragma unroll
+        for (int b = 0; b < NC; ++b) p[b] += x[b];
     }
 }
 
