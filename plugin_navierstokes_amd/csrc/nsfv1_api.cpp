@@ -70,6 +70,12 @@ struct Plan {
     int block = 256;
     size_t lds = 0;
     DevBuf wg_node, wg_rounds, perm;
+    // the same plan packed for one dependent load per lane and per
+    // workgroup (split row kernel): lane_a[lane] = {element, node,
+    // corner | slot << 8 | row length << 16, LDS row base}, lane_pos[lane] =
+    // column blocks of the corners; wg_a[2w] = {v0, v1, i0, i1},
+    // wg_a[2w+1] = {jbase low, jbase high, jlen, rounds}
+    DevBuf lane_a, lane_pos, wg_a;
 };
 
 }  // namespace
@@ -594,6 +600,50 @@ static int make_plan(nsfv1_handle* h, int jbytes, Plan& P)
                (x.empty() || hipMemcpy(b.p, x.data(), sizeof(int32_t) * x.size(), hipMemcpyHostToDevice) == hipSuccess);
     };
     if (!up(P.wg_node, wn) || !up(P.wg_rounds, wr) || !up(P.perm, perm)) return fail(NSFV1_ERR_DEVICE, "plan upload");
+    {
+        const int pade = 16 / jbytes;
+        std::vector<int32_t> la((size_t)h->ninc * 4), wa((size_t)P.nwg * 8);
+        std::vector<uint64_t> lp((size_t)h->ninc);
+        std::vector<uint64_t> hpos((size_t)h->ninc);
+        if (h->ninc > 0 &&
+            hipMemcpy(hpos.data(), h->inc_pos.p, sizeof(uint64_t) * hpos.size(), hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(NSFV1_ERR_DEVICE, "plan download");
+        for (int w = 0; w < P.nwg; ++w) {
+            const int64_t v0 = wn[w], v1 = wn[w + 1];
+            const int64_t i0 = h->h_node_inc[v0], i1 = h->h_node_inc[v1];
+            const int64_t jbase = h->h_rowptr[v0 * nc];
+            for (int64_t t = i0; t < i1; ++t) {
+                const int32_t q = perm[t];
+                int64_t v = v0;
+                while (h->h_node_inc[v + 1] <= q) ++v;
+                const int32_t info = h->h_inc_info[q];
+                const int64_t rlen = h->h_rowptr[v * nc + 1] - h->h_rowptr[v * nc];
+                if (rlen >= (1 << 15)) return fail(NSFV1_ERR_ARGUMENT, "node row too long for the plan");
+                la[t * 4 + 0] = h->h_inc_elem[q];
+                la[t * 4 + 1] = (int32_t)v;
+                la[t * 4 + 2] = (info & 0xffff) | (int32_t)(rlen << 16);
+                la[t * 4 + 3] = (int32_t)(h->h_rowptr[v * nc] - jbase + (v - v0) * pade);
+                lp[t] = hpos[q];
+            }
+            const int64_t jlen = h->h_rowptr[v1 * nc] - jbase;
+            wa[w * 8 + 0] = (int32_t)v0;
+            wa[w * 8 + 1] = (int32_t)v1;
+            wa[w * 8 + 2] = (int32_t)i0;
+            wa[w * 8 + 3] = (int32_t)i1;
+            wa[w * 8 + 4] = (int32_t)(uint32_t)(jbase & 0xffffffffu);
+            wa[w * 8 + 5] = (int32_t)(jbase >> 32);
+            wa[w * 8 + 6] = (int32_t)jlen;
+            wa[w * 8 + 7] = wr[w];
+        }
+        auto upb = [](DevBuf& b, const void* x, size_t bytes) {
+            return b.alloc(std::max<size_t>(bytes, 16)) == hipSuccess &&
+                   (bytes == 0 || hipMemcpy(b.p, x, bytes, hipMemcpyHostToDevice) == hipSuccess);
+        };
+        if (!upb(P.lane_a, la.data(), sizeof(int32_t) * la.size()) ||
+            !upb(P.lane_pos, lp.data(), sizeof(uint64_t) * lp.size()) ||
+            !upb(P.wg_a, wa.data(), sizeof(int32_t) * wa.size()))
+            return fail(NSFV1_ERR_DEVICE, "plan upload");
+    }
     P.valid = true;
     return NSFV1_OK;
 }
@@ -692,7 +742,8 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
                 h->node_inc.as<int32_t>(), h->inc_elem.as<int32_t>(), h->inc_node.as<int32_t>(),
                 h->inc_info.as<int32_t>(), P.wg_node.as<int32_t>(), P.wg_rounds.as<int32_t>(), h->l2inv.as<double>(),
                 h->vol.as<double>(), h->upw_dec.as<uint8_t>(), h->affine.as<uint8_t>(), h->inc_pos.as<uint64_t>(),
-                P.perm.as<int32_t>(), h->nu_elem.as<double>(), P.block};
+                P.perm.as<int32_t>(), h->nu_elem.as<double>(), P.block, P.lane_a.as<int4>(),
+                P.lane_pos.as<uint64_t>(), P.wg_a.as<int4>()};
     hipError_t e;
     if (coupled_config(h)) {
         if (h->upw_mode != NSFV1_UPWIND_FRESH)

@@ -1368,9 +1368,11 @@ assemble_rows_split(KMeshPtrs M, KParams P, const double* __restrict__ u, const 
         const int per = nwg >> 3, rem = nwg & 7, x = i & 7, k = i >> 3;
         return x * per + (x < rem ? x : rem) + k;
     }();
-    const int v0 = M.wg_node[w], v1 = M.wg_node[w + 1];
-    const int64_t jbase = M.rowptr[(int64_t)v0 * NC];
-    const int jlen = (int)(M.rowptr[(int64_t)v1 * NC] - jbase);
+    // the workgroup's and the lane's plan data come packed, one load each
+    const int4 wa = M.wg_a[2 * w], wb = M.wg_a[2 * w + 1];
+    const int v0 = wa.x, v1 = wa.y;
+    const int64_t jbase = (int64_t)(uint32_t)wb.x | ((int64_t)wb.y << 32);
+    const int jlen = wb.z;
     const int dlen = (v1 - v0) * NC;
     JT* jbuf = reinterpret_cast<JT*>(smem);
     // 16 pad bytes after each node's rows: consecutive nodes then start in
@@ -1383,7 +1385,7 @@ assemble_rows_split(KMeshPtrs M, KParams P, const double* __restrict__ u, const 
         for (int i = threadIdx.x; i < jlen + (v1 - v0) * PADE; i += blockDim.x) jbuf[i] = JT(0);
     for (int i = threadIdx.x; i < dlen; i += blockDim.x) dbuf[i] = 0.0;
 
-    const int i0 = M.node_inc[v0], i1 = M.node_inc[v1];
+    const int i0 = wa.z, i1 = wa.w;
     const int lane_inc = i0 + (int)threadIdx.x;
     const bool active = lane_inc < i1;
 
@@ -1396,17 +1398,19 @@ assemble_rows_split(KMeshPtrs M, KParams P, const double* __restrict__ u, const 
 #pragma unroll
             for (int b = 0; b < NC; ++b) L.jacc[a][k][b] = JT(0);
     }
-    int v = v0, slot = -1;
+    int v = v0, slot = -1, rbase = 0, rlen = 0;
     int pos[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) pos[k] = 0;
     if (active) {
-        const int inc = M.perm[lane_inc];
-        const int e = M.inc_elem[inc];
-        v = M.inc_node[inc];
-        const int info = M.inc_info[inc];
-        const int kv = info & 0xff;
-        slot = info >> 8;
+        const int4 la = M.lane_a[lane_inc];
+        const uint64_t pw = M.lane_pos[lane_inc];
+        const int e = la.x;
+        v = la.y;
+        const int kv = la.z & 0xff;
+        slot = (la.z >> 8) & 0xff;
+        rlen = la.z >> 16;
+        rbase = la.w;
         int pk[N];
 #pragma unroll
         for (int k = 0; k < N; ++k) pk[k] = CanonH<ET>::c.perm[kv][k];
@@ -1428,18 +1432,11 @@ assemble_rows_split(KMeshPtrs M, KParams P, const double* __restrict__ u, const 
 #pragma unroll
                 for (int d = 0; d < D; ++d) L.dacc[d] += P.s_f * (P.f[d] * volk * P.rho);
         }
-        const uint64_t pw = M.inc_pos[inc];
 #pragma unroll
         for (int k = 0; k < N; ++k) pos[k] = (int)((pw >> (8 * pk[k])) & 0xffu) * NC;
     }
-    int rbase = 0, rlen = 0;
-    if (active) {
-        const int64_t rs = M.rowptr[(int64_t)v * NC];
-        rbase = (int)(rs - jbase) + (v - v0) * PADE;
-        rlen = (int)(M.rowptr[(int64_t)v * NC + 1] - rs);
-    }
     __syncthreads();
-    const int rounds = M.wg_rounds[w];
+    const int rounds = wb.w;
     for (int r = 0; r < rounds; ++r) {
         if (active && slot == r) {
             if (wantJ) {

@@ -29,6 +29,9 @@ struct KMeshPtrs {
     const int32_t* perm;      // [ninc] lane order of the plan: incidences of a workgroup sorted by slot
     const double* nu_elem;    // [ne] per-element kinematic viscosity (batched instances), or null
     int block;                // threads per workgroup of the launch (64..256)
+    const int4* lane_a;       // [ninc] in lane order: {element, node, corner | slot << 8 | row length << 16, LDS row base}
+    const uint64_t* lane_pos; // [ninc] in lane order: inc_pos of the lane's incidence
+    const int4* wg_a;         // [2 nwg]: {v0, v1, i0, i1}, {jbase low, jbase high, jlen, rounds}
 };
 
 hipError_t launch_assemble(int et, int jprec, const KMeshPtrs& m, const KParams& p, const double* u,
