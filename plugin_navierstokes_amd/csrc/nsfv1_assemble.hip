@@ -1086,6 +1086,11 @@ assemble_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const double
 //     linear in the normal or in the flux weight).
 // ---------------------------------------------------------------------
 constexpr int REC_FLOATS = 32;
+// A record is REC_CHUNKS 16-byte chunks.  Chunk c of SCVF ip of element e
+// lives at float4 index (ip * REC_CHUNKS + c) * ne_pad + e: the lanes of a
+// wavefront, which hold consecutive elements, read and write each chunk as
+// one contiguous run (whole cache lines per instruction).
+constexpr int REC_CHUNKS = REC_FLOATS / 4;
 // record layout: box elements keep the inverse reference Jacobian (their
 // frame changes are axis reflections, exact in FP32); simplices keep the
 // physical shape gradients (a frame change would round)
@@ -1105,7 +1110,8 @@ __device__ inline void record_one(const Cfg<FAST>& C, const double (&co)[Topo<ET
                                   const double* __restrict__ xyz, const int32_t* __restrict__ econn,
                                   const double (&ul)[Topo<ET>::nsh][Topo<ET>::dim + 1], const int* pk,
                                   const int32_t* ccn, const double* c, const double* __restrict__ uold, double l2,
-                                  int kv, float* __restrict__ rec, bool& okg, bool& oku, uint8_t* decp, bool affine)
+                                  int kv, float4* __restrict__ rec, int64_t cs, bool& okg, bool& oku, uint8_t* decp,
+                                  bool affine)
 {
     using T = Topo<ET>;
     constexpr int D = T::dim, N = T::nsh;
@@ -1199,12 +1205,10 @@ __device__ inline void record_one(const Cfg<FAST>& C, const double (&co)[Topo<ET
             rv[RL::U + c] = x;
         }
     }
-    float4* r4 = reinterpret_cast<float4*>(rec);
 #pragma unroll
-    for (int q = 0; q < RL::F / 4; ++q) r4[q] = make_float4(rv[4 * q], rv[4 * q + 1], rv[4 * q + 2], rv[4 * q + 3]);
-    double2* f2 = reinterpret_cast<double2*>(rec + RL::F);
-    f2[0] = make_double2(L.dacc[0], L.dacc[1]);
-    f2[1] = make_double2(L.dacc[2], D == 3 ? L.dacc[3] : 0.0);
+    for (int q = 0; q < RL::F / 4; ++q) rec[q * cs] = make_float4(rv[4 * q], rv[4 * q + 1], rv[4 * q + 2], rv[4 * q + 3]);
+    *reinterpret_cast<double2*>(rec + (RL::F / 4) * cs) = make_double2(L.dacc[0], L.dacc[1]);
+    *reinterpret_cast<double2*>(rec + (RL::F / 4 + 1) * cs) = make_double2(L.dacc[2], D == 3 ? L.dacc[3] : 0.0);
 }
 
 template <int ET, int Q>
@@ -1259,9 +1263,10 @@ scvf_records(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
     const double l2 = M.l2inv[e * NIP + ip];
     uint8_t* decp = (P.upw_mode != 0 && M.upw_dec) ? M.upw_dec + e * NIP + ip : nullptr;
     const bool affine = (ET == 0 || ET == 2) || (M.affine != nullptr && M.affine[e] != 0);
-    float* rec = recs + ((int64_t)ip * ne_pad + e) * REC_FLOATS;
+    float4* rec = reinterpret_cast<float4*>(recs) + (int64_t)ip * REC_CHUNKS * ne_pad + e;
     bool okg = true, oku = true;
-    record_one<ET, FAST, Q>(C, co, M.xyz, M.conn + e * N, ul, pk, ccn, c, uold, l2, kv, rec, okg, oku, decp, affine);
+    record_one<ET, FAST, Q>(C, co, M.xyz, M.conn + e * N, ul, pk, ccn, c, uold, l2, kv, rec, ne_pad, okg, oku, decp,
+                            affine);
     if (!okg || !oku) err[0] = (oku ? 0 : KERR_CUT_SIDE) | (okg ? 0 : KERR_DEGENERATE);
 }
 
@@ -1274,20 +1279,19 @@ __device__ inline void gather_jac(const Cfg<FAST>& C, const float* __restrict__ 
         constexpr int D = T::dim, N = T::nsh;
         const int ip = CanonH<ET>::c.oip[kv][Q];
         const float sg = CanonH<ET>::c.from[kv][Q] ? 1.f : -1.f;
-        const float* rec = recs + ((int64_t)ip * ne_pad + e) * REC_FLOATS;
+        const float4* rec = reinterpret_cast<const float4*>(recs) + (int64_t)ip * REC_CHUNKS * ne_pad + e;
         using RL = RecLayout<ET>;
-        const double2* f2 = reinterpret_cast<const double2*>(rec + RL::F);
-        const double2 fa = f2[0], fb = f2[1];
+        const double2 fa = *reinterpret_cast<const double2*>(rec + (RL::F / 4) * ne_pad);
+        const double2 fb = *reinterpret_cast<const double2*>(rec + (RL::F / 4 + 1) * ne_pad);
         L.dacc[0] += double(sg) * fa.x;
         L.dacc[1] += double(sg) * fa.y;
         L.dacc[2] += double(sg) * fb.x;
         if constexpr (D == 3) L.dacc[3] += double(sg) * fb.y;
         if (wantJ) {
             float rv[RL::F];
-            const float4* r4 = reinterpret_cast<const float4*>(rec);
 #pragma unroll
             for (int q = 0; q < RL::F / 4; ++q) {
-                const float4 t = r4[q];
+                const float4 t = rec[q * ne_pad];
                 rv[4 * q] = t.x;
                 rv[4 * q + 1] = t.y;
                 rv[4 * q + 2] = t.z;
