@@ -129,11 +129,13 @@ def autotune(disc, run, candidates=None, reps: int = 5, log=None):
     """Time run() under every (assembly path, threads, lds_bytes) launch
     shape, keep the fastest on disc and return (best, table of (path,
     threads, lds, ms)).  Path 0 evaluates each SCVF once into records, path 1
-    is the fused kernel."""
+    is the fused kernel, path 2 is path 0 with dim+1 gather lanes per
+    (node, element) (64 threads only)."""
     import torch
 
     if candidates is None:
         candidates = [(p, t, l) for p in (0, 1) for t in (64, 128, 256) for l in (16384, 32768, 65536)]
+        candidates += [(2, 64, l) for l in (16384, 65536)]
     table = []
     for p, t, l in candidates:
         disc.set_assembly_path(p)

@@ -76,6 +76,7 @@ struct Plan {
     // column blocks of the corners; wg_a[2w] = {v0, v1, i0, i1},
     // wg_a[2w+1] = {jbase low, jbase high, jlen, rounds}
     DevBuf lane_a, lane_pos, wg_a;
+    bool rowlane = false;  // built for the row-lane split gather
 };
 
 }  // namespace
@@ -563,6 +564,10 @@ static int make_plan(nsfv1_handle* h, int jbytes, Plan& P)
     wn.push_back(0);
     int64_t v = 0;
     size_t maxlds = 0;
+    // the row-lane split gather puts dim+1 lanes on each incidence of a
+    // 64-lane workgroup
+    const bool rowlane = h->path == 2 && jbytes == 4 && h->max_threads == 64;
+    const int64_t inc_budget = rowlane ? 64 / nc : h->max_threads;
     while (v < h->nv) {
         const int64_t v0 = v;
         int rounds = 0;
@@ -570,7 +575,7 @@ static int make_plan(nsfv1_handle* h, int jbytes, Plan& P)
             const int64_t incs = h->h_node_inc[v + 1] - h->h_node_inc[v0];
             const int64_t vals = h->h_rowptr[(v + 1) * nc] - h->h_rowptr[v0 * nc];
             const size_t lds = ((size_t)vals * jbytes + (size_t)(v + 1 - v0) * 16 + 15) / 16 * 16 + (size_t)(v + 1 - v0) * nc * 8;
-            if (incs > h->max_threads || lds > h->lds_cap) {
+            if (incs > inc_budget || lds > h->lds_cap) {
                 if (v == v0) return fail(NSFV1_ERR_ARGUMENT, "one node row exceeds the workgroup budget");
                 break;
             }
@@ -584,6 +589,7 @@ static int make_plan(nsfv1_handle* h, int jbytes, Plan& P)
     P.nwg = (int)wr.size();
     P.lds = maxlds;
     P.block = h->max_threads;
+    P.rowlane = rowlane;
     // lane order inside each workgroup: by incidence slot, then node, so a
     // wavefront takes part in as few accumulation rounds as possible
     std::vector<int32_t> perm((size_t)h->ninc);
@@ -754,11 +760,12 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
             return fail(NSFV1_ERR_DEVICE, "coupled record buffer");
         e = launch_assemble_coupled(h->et, jprec, m, k, u, u_old, jvals, defect, h->errflag.as<int>(), P.nwg, P.lds,
                                     h->ne, h->cpl_recs.p, nullptr, (hipStream_t)stream);
-    } else if (h->path == 0 && jprec == NSFV1_F32 && jvals && (parts & NSFV1_PART_A) && split_supported(h->et, k)) {
+    } else if ((h->path == 0 || h->path == 2) && jprec == NSFV1_F32 && jvals && (parts & NSFV1_PART_A) &&
+               split_supported(h->et, k)) {
         const size_t need = sizeof(float) * (size_t)split_record_floats(h->et, h->ne);
         if (h->recs.n < need && h->recs.alloc(need) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "record buffer");
         e = launch_assemble_split(h->et, m, k, u, u_old, static_cast<float*>(jvals), defect, h->errflag.as<int>(), P.nwg,
-                                  P.lds, h->ne, h->recs.as<float>(), (hipStream_t)stream);
+                                  P.lds, h->ne, h->recs.as<float>(), h->path == 2 && P.rowlane, (hipStream_t)stream);
     } else {
         e = launch_assemble(h->et, jprec, m, k, u, u_old, jvals, defect, h->errflag.as<int>(), P.nwg, P.lds,
                             (hipStream_t)stream);
@@ -990,7 +997,9 @@ int nsfv1_export(nsfv1_handle* h, int what, int at, const double* local_ips, int
 int nsfv1_set_assembly_path(nsfv1_handle* h, int path)
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
-    if (path != 0 && path != 1) return fail(NSFV1_ERR_ARGUMENT, "path must be 0 (split where supported) or 1 (fused)");
+    if (path < 0 || path > 2)
+        return fail(NSFV1_ERR_ARGUMENT, "path must be 0 (split where supported), 1 (fused) or 2 (split, row lanes)");
+    if (path != h->path) h->plan32.valid = h->plan64.valid = false;  // the row-lane plan packs fewer incidences
     h->path = path;
     return NSFV1_OK;
 }

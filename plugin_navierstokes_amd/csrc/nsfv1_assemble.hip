@@ -1469,6 +1469,248 @@ assemble_rows_split(KMeshPtrs M, KParams P, const double* __restrict__ u, const 
 // ---------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------
+// ---------------------------------------------------------------------
+// Row-lane variant of the split row gather: four lanes (dim+1 in general)
+// per (node, element) incidence, lane a forms row a of the element's block.
+// A lane holds its three records (loaded together, one memory round trip)
+// and one row of the block instead of all dim+1 rows, so more wavefronts
+// fit and each keeps all its loads in flight.  The terms and their order
+// of addition per entry are those of scvf_jac for the split configurations
+// (FIELDS, one upwind, no Stokes/PAC/Peclet/exact), so the result is
+// bitwise that of assemble_rows_split.
+// ---------------------------------------------------------------------
+template <int ET, int FAST, int Q>
+__device__ inline void row_terms(const Cfg<FAST>& C, const float4* __restrict__ rec, int64_t cs, int kv, const int* pk,
+                                 int a, float (&jrow)[Topo<ET>::nsh][Topo<ET>::dim + 1])
+{
+    using T = Topo<ET>;
+    constexpr int D = T::dim, N = T::nsh, NC = D + 1;
+    constexpr int IP = corner_ip<ET>(0, Q);
+    using RL = RecLayout<ET>;
+    using JT = float;
+    const KParams& P = C.P;
+    const float sg = CanonH<ET>::c.from[kv][Q] ? 1.f : -1.f;
+    float rv[RL::F];
+#pragma unroll
+    for (int q = 0; q < RL::F / 4; ++q) {
+        const float4 t = rec[q * cs];
+        rv[4 * q] = t.x;
+        rv[4 * q + 1] = t.y;
+        rv[4 * q + 2] = t.z;
+        rv[4 * q + 3] = t.w;
+    }
+    JT Ji[D][D], G[N][D], n[D], Us[N];
+    if constexpr (RL::box) {
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                float acc = 0.f;
+#pragma unroll
+                for (int m = 0; m < D; ++m) acc += float(CanonH<ET>::c.Ainv[kv][i][m]) * rv[RL::G + m * D + j];
+                Ji[i][j] = acc;
+            }
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                float x = 0.f;
+#pragma unroll
+                for (int c = 0; c < N; ++c) x = pk[k] == c ? rv[RL::G + c * D + i] : x;
+                G[k][i] = x;
+            }
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) n[i] = sg * rv[RL::Nrm + i];
+    const JT pw = sg * rv[RL::S + 0], vv = rv[RL::S + 1], vc = rv[RL::S + 2], sp = rv[RL::S + 3];
+    if constexpr (FAST == UPW_FULL || FAST == UPW_SKEWED) {
+        const int idx = __float_as_int(rv[RL::U]);
+#pragma unroll
+        for (int k = 0; k < N; ++k) Us[k] = pk[k] == idx ? 1.f : 0.f;
+    } else if constexpr (FAST == UPW_LPS) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            float x = 0.f;
+#pragma unroll
+            for (int c = 0; c < N; ++c) x = pk[k] == c ? rv[RL::U + c] : x;
+            Us[k] = x;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; ++k) Us[k] = float(TabH<ET>::t.phi[IP][k]);
+    }
+    const JT s_ = JT(P.s_a), rh = JT(P.rho), nr = JT(-1.0 * C.nu * P.rho);
+    const bool mom = a < D;
+#pragma unroll
+    for (int sh = 0; sh < N; ++sh) {
+        JT g[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            if constexpr (RL::box) {
+                JT acc = 0;
+#pragma unroll
+                for (int j = 0; j < D; ++j) acc += Ji[j][i] * JT(TabH<ET>::t.gl[IP][sh][j]);
+                g[i] = acc;
+            } else {
+                g[i] = G[sh][i];
+            }
+        }
+        const JT ph = JT(TabH<ET>::t.phi[IP][sh]);
+        JT ga = 0, na = 0;  // g[a], n[a] of the lane's momentum row
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            ga = a == i ? g[i] : ga;
+            na = a == i ? n[i] : na;
+        }
+        if (mom) {
+            JT gn = 0;
+#pragma unroll
+            for (int i = 0; i < D; ++i) gn += g[i] * n[i];
+            const JT cv = Us[sh] * pw;
+#pragma unroll
+            for (int b = 0; b < NC; ++b) {
+                JT x = jrow[sh][b];
+                if (b == a) x += s_ * (nr * gn);
+                if (b < D && !P.laplace) x += s_ * (nr * ga * n[b < D ? b : 0]);
+                if (b == D) x += s_ * (ph * na);
+                if (b == a) x += s_ * cv;
+                jrow[sh][b] = x;
+            }
+        } else {
+            const JT Sv = vv * ph + vc * Us[sh];
+            JT cpp = 0;
+#pragma unroll
+            for (int d1 = 0; d1 < D; ++d1) {
+                jrow[sh][d1] += s_ * (Sv * n[d1] * rh);
+                cpp += (g[d1] * sp) * n[d1] * rh;
+            }
+            jrow[sh][D] += s_ * cpp;
+        }
+    }
+}
+
+template <int ET, int FAST, int Q>
+__device__ inline void row_all(const Cfg<FAST>& C, const float4* const* recq, int64_t cs, int kv, const int* pk, int a,
+                               float (&jrow)[Topo<ET>::nsh][Topo<ET>::dim + 1])
+{
+    if constexpr (Q < Topo<ET>::cip) {
+        row_terms<ET, FAST, Q>(C, recq[Q], cs, kv, pk, a, jrow);
+        row_all<ET, FAST, Q + 1>(C, recq, cs, kv, pk, a, jrow);
+    }
+}
+
+template <int ET, int FAST>
+__global__ void __launch_bounds__(64)
+assemble_rows_rowlane(KMeshPtrs M, KParams P, const double* __restrict__ u, const float* __restrict__ recs,
+                      int64_t ne_pad, float* __restrict__ J, double* __restrict__ dout)
+{
+    using T = Topo<ET>;
+    constexpr int D = T::dim, N = T::nsh, NC = D + 1, CIP = T::cip;
+    using JT = float;
+    using RL = RecLayout<ET>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int w = [] {
+        const int nwg = (int)gridDim.x, i = (int)blockIdx.x;
+        const int per = nwg >> 3, rem = nwg & 7, x = i & 7, k = i >> 3;
+        return x * per + (x < rem ? x : rem) + k;
+    }();
+    const int4 wa = M.wg_a[2 * w], wb = M.wg_a[2 * w + 1];
+    const int v0 = wa.x, v1 = wa.y;
+    const int64_t jbase = (int64_t)(uint32_t)wb.x | ((int64_t)wb.y << 32);
+    const int jlen = wb.z;
+    const int dlen = (v1 - v0) * NC;
+    JT* jbuf = reinterpret_cast<JT*>(smem);
+    constexpr int PADE = 16 / (int)sizeof(JT);
+    double* dbuf = reinterpret_cast<double*>(smem + (((jlen + (v1 - v0) * PADE) * (int)sizeof(JT) + 15) & ~15));
+    const bool wantJ = J != nullptr;
+    if (wantJ)
+        for (int i = threadIdx.x; i < jlen + (v1 - v0) * PADE; i += blockDim.x) jbuf[i] = JT(0);
+    for (int i = threadIdx.x; i < dlen; i += blockDim.x) dbuf[i] = 0.0;
+
+    const int li = (int)threadIdx.x / NC, a = (int)threadIdx.x - li * NC;
+    const int i0 = wa.z, i1 = wa.w;
+    const int lane_inc = i0 + li;
+    const bool active = li < 64 / NC && lane_inc < i1;
+    float jrow[N][NC];
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+#pragma unroll
+        for (int b = 0; b < NC; ++b) jrow[k][b] = 0.f;
+    double dsum = 0.0;
+    int v = v0, slot = -1, rbase = 0, rlen = 0;
+    int pos[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) pos[k] = 0;
+    if (active) {
+        const int4 la = M.lane_a[lane_inc];
+        const uint64_t pw = M.lane_pos[lane_inc];
+        const int64_t e = la.x;
+        v = la.y;
+        const int kv = la.z & 0xff;
+        slot = (la.z >> 8) & 0xff;
+        rlen = la.z >> 16;
+        rbase = la.w;
+        int pk[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) pk[k] = CanonH<ET>::c.perm[kv][k];
+        if (P.parts & 1) {
+            const float4* recq[CIP];
+            double fl[CIP];
+#pragma unroll
+            for (int q = 0; q < CIP; ++q) {
+                const int ip = CanonH<ET>::c.oip[kv][q];
+                recq[q] = reinterpret_cast<const float4*>(recs) + (int64_t)ip * REC_CHUNKS * ne_pad + e;
+                const double2 f = *reinterpret_cast<const double2*>(recq[q] + (RL::F / 4 + (a >> 1)) * ne_pad);
+                fl[q] = (a & 1) ? f.y : f.x;
+            }
+#pragma unroll
+            for (int q = 0; q < CIP; ++q) dsum += double(CanonH<ET>::c.from[kv][q] ? 1.f : -1.f) * fl[q];
+            const Cfg<FAST> C{P, M.nu_elem ? M.nu_elem[e] : P.nu};
+            if (wantJ) row_all<ET, FAST, 0>(C, recq, ne_pad, kv, pk, a, jrow);
+        }
+        if (P.parts & 6) {
+            const double volk = M.vol[e * N + kv];
+            if ((P.parts & 2) && a < D) {
+                const double m = volk * P.rho;
+#pragma unroll
+                for (int b = 0; b < NC; ++b)
+                    if (b == a && wantJ) jrow[0][b] += JT(P.s_m * m);
+                dsum += P.s_m * (u[(int64_t)v * NC + a] * m);
+            }
+            if ((P.parts & 4) && P.has_source && a < D) {
+                double f = 0.0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) f = d == a ? P.f[d] : f;
+                dsum += P.s_f * (f * volk * P.rho);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) pos[k] = (int)((pw >> (8 * pk[k])) & 0xffu) * NC;
+    }
+    __syncthreads();
+    const int rounds = wb.w;
+    for (int r = 0; r < rounds; ++r) {
+        if (active && slot == r) {
+            if (wantJ) {
+                const int ra = rbase + a * rlen;
+#pragma unroll
+                for (int k = 0; k < N; ++k) lds_add_block<NC>(jbuf + ra + pos[k], jrow[k]);
+            }
+            dbuf[(v - v0) * NC + a] += dsum;
+        }
+        __syncthreads();
+    }
+    if (wantJ)
+        for (int j = 0; j < v1 - v0; ++j) {
+            const int off = (int)(M.rowptr[(int64_t)(v0 + j) * NC] - jbase);
+            const int len = (int)(M.rowptr[(int64_t)(v0 + j + 1) * NC] - jbase) - off;
+            for (int i = threadIdx.x; i < len; i += blockDim.x) J[jbase + off + i] = jbuf[off + j * PADE + i];
+        }
+    if (dout)
+        for (int i = threadIdx.x; i < dlen; i += blockDim.x) dout[(int64_t)v0 * NC + i] = dbuf[i];
+}
+
 template <int ET, typename JT>
 static hipError_t launch_cfg(int fast, const KMeshPtrs& m, const KParams& p, const double* u, const double* uold,
                              JT* J, double* d, int* err, int nwg, size_t lds, hipStream_t s)
@@ -1518,7 +1760,7 @@ hipError_t launch_assemble(int et, int jprec, const KMeshPtrs& m, const KParams&
 template <int ET, int FAST>
 static hipError_t launch_split_cfg(const KMeshPtrs& m, const KParams& p, const double* u, const double* uold,
                                    float* J, double* d, int* err, int nwg, size_t lds, int64_t ne, float* recs,
-                                   hipStream_t s)
+                                   bool rowlane, hipStream_t s)
 {
     const int64_t ne_pad = (ne + 63) / 64 * 64;
     if (p.parts & 1) {
@@ -1533,19 +1775,21 @@ static hipError_t launch_split_cfg(const KMeshPtrs& m, const KParams& p, const d
             scvf_records<ET, FAST, 2><<<(unsigned)((IpsOfQ<ET, 2>::count * ne_pad + 255) / 256), 256, 0, s>>>(
                 m, p, u, uold, ne, ne_pad, recs, err);
     }
-    assemble_rows_split<ET, FAST><<<nwg, m.block, lds, s>>>(m, p, u, recs, ne_pad, J, d);
+    if (rowlane) assemble_rows_rowlane<ET, FAST><<<nwg, 64, lds, s>>>(m, p, u, recs, ne_pad, J, d);
+    else assemble_rows_split<ET, FAST><<<nwg, m.block, lds, s>>>(m, p, u, recs, ne_pad, J, d);
     return hipGetLastError();
 }
 
 template <int ET>
 static hipError_t launch_split_et(const KMeshPtrs& m, const KParams& p, const double* u, const double* uold, float* J,
-                                  double* d, int* err, int nwg, size_t lds, int64_t ne, float* recs, hipStream_t s)
+                                  double* d, int* err, int nwg, size_t lds, int64_t ne, float* recs, bool rl,
+                                  hipStream_t s)
 {
     switch (fast_path(p)) {
-    case UPW_NO: return launch_split_cfg<ET, UPW_NO>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, s);
-    case UPW_FULL: return launch_split_cfg<ET, UPW_FULL>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, s);
-    case UPW_SKEWED: return launch_split_cfg<ET, UPW_SKEWED>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, s);
-    case UPW_LPS: return launch_split_cfg<ET, UPW_LPS>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, s);
+    case UPW_NO: return launch_split_cfg<ET, UPW_NO>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, rl, s);
+    case UPW_FULL: return launch_split_cfg<ET, UPW_FULL>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, rl, s);
+    case UPW_SKEWED: return launch_split_cfg<ET, UPW_SKEWED>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, rl, s);
+    case UPW_LPS: return launch_split_cfg<ET, UPW_LPS>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, rl, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1564,13 +1808,14 @@ int64_t split_record_floats(int et, int64_t ne)
 
 hipError_t launch_assemble_split(int et, const KMeshPtrs& m, const KParams& p, const double* u, const double* uold,
                                  float* J, double* d, int* err, int nwg, size_t lds, int64_t ne, float* recs,
-                                 hipStream_t s)
+                                 bool rowlane, hipStream_t s)
 {
     if (lds > 65536 || m.block < 64 || m.block > 256 || m.block % 64 || !split_supported(et, p)) return hipErrorInvalidValue;
+    if (rowlane && m.block != 64) return hipErrorInvalidValue;
     switch (et) {
-    case ET_TRI:  return launch_split_et<ET_TRI>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, s);
-    case ET_QUAD: return launch_split_et<ET_QUAD>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, s);
-    case ET_HEX:  return launch_split_et<ET_HEX>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, s);
+    case ET_TRI:  return launch_split_et<ET_TRI>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, rowlane, s);
+    case ET_QUAD: return launch_split_et<ET_QUAD>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, rowlane, s);
+    case ET_HEX:  return launch_split_et<ET_HEX>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, rowlane, s);
     }
     return hipErrorInvalidValue;
 }

@@ -42,9 +42,11 @@ hipError_t launch_assemble(int et, int jprec, const KMeshPtrs& m, const KParams&
 // the row gather; recs holds split_record_floats(et, ne) floats
 bool split_supported(int et, const KParams& p);
 int64_t split_record_floats(int et, int64_t ne);
+// rowlane: one lane per (node, element, row) in the gather (plan with
+// 64 / (dim+1) incidences per 64-lane workgroup)
 hipError_t launch_assemble_split(int et, const KMeshPtrs& m, const KParams& p, const double* u, const double* uold,
                                  float* J, double* d, int* err, int nwg, size_t lds, int64_t ne, float* recs,
-                                 hipStream_t stream);
+                                 bool rowlane, hipStream_t stream);
 
 // stabilised SCVF velocities of every element: out[ne][nip][dim]
 hipError_t launch_export_stab_vel(int et, const KMeshPtrs& m, const KParams& p, const double* u, const double* uold,

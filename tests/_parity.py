@@ -32,7 +32,7 @@ def oracle_params(dim, cfg: dict) -> O.Params:
                     source=cfg.get("source"))
 
 
-def run_case(cuda, et, cfg, mesh=None, seed=5, time_dep=False, parts=1, s=(1.0, 0.0, 0.0), jprec=64):
+def run_case(cuda, et, cfg, mesh=None, seed=5, time_dep=False, parts=1, s=(1.0, 0.0, 0.0), jprec=64, path=None):
     """GPU assembly through the C ABI and the oracle's on the same seeded
     state: (J, d, J_oracle, d_oracle, J_scale, d_scale, disc)."""
     import torch
@@ -46,6 +46,9 @@ def run_case(cuda, et, cfg, mesh=None, seed=5, time_dep=False, parts=1, s=(1.0, 
     disc = NavierStokesFV1("u,v,p" if dim == 2 else "u,v,w,p", "Inner")
     configure_disc(disc, cfg)
     disc.set_mesh(mesh)
+    if path is not None:
+        disc.set_launch(64, 16384)
+        disc.set_assembly_path(path)
     ut = torch.from_numpy(u).to(cuda)
     uot = torch.from_numpy(uold).to(cuda) if time_dep else None
     J, d = disc.assemble(ut, uot, dt, s_a=s[0], s_m=s[1], s_f=s[2], parts=parts, jprec=jprec)

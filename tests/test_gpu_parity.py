@@ -200,3 +200,26 @@ def test_affine_elements_ray_cast(cuda, et, up, shear):
         else:
             assert fp32_violation(J, Jr, Ja) <= 1.0, worst(J, Jr, Ja, FP32_REL, FP32_ABS)
         assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+
+
+@pytest.mark.parametrize("et", [M.TRI, M.QUAD, M.HEX], ids=lambda e: ET_NAMES[e])
+@pytest.mark.parametrize("up", ["no", "full", "skewed", "lps"])
+def test_split_row_lanes(cuda, et, up):
+    """The row-lane gather (path 2) against the oracle and against the split
+    gather (path 0), with the mass and source parts: the same terms in the
+    same order per entry; the compiler may contract different products into
+    FMAs, so the paths agree to FP32 rounding, not bitwise."""
+    cfg = {"upwind": up, "source": (0.2, -0.1, 0.3)}
+    mesh = small_mesh(et, n=6, seed=8)
+    res = {}
+    for path in (0, 2):
+        J, d, Jr, dr, Ja, da, disc = _run(cuda, et, cfg, mesh=mesh, seed=4, time_dep=True,
+                                              parts=PART_A | PART_M | PART_RHS, s=(0.8, 1.5, -0.5), jprec=F32,
+                                              path=path)
+        assert fp32_violation(J, Jr, Ja) <= 1.0, worst(J, Jr, Ja, FP32_REL, FP32_ABS)
+        assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+        res[path] = (J, d, Ja, da)
+    J0, d0, Ja, da = res[0]
+    J2, d2, _, _ = res[2]
+    assert np.max(np.abs(J2.astype(np.float64) - J0) / (2.0 ** -20 * (Ja + np.abs(J0)) + 1e-300)) <= 1.0
+    assert fp64_violation(d2, d0, da) <= 1.0
