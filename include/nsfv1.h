@@ -341,8 +341,11 @@ int nsfv1_predict_target(nsfv1_handle* h, const double* u_next, const double* u,
 /* Assembly path: 0 (default) evaluates every SCVF once into a per-SCVF
    record (FP32 Jacobian, common configuration: FIELDS, one upwind object,
    no Stokes/PAC/Peclet/exact terms) and gathers rows from the records; 1
-   always runs the fused kernel (each SCVF evaluated by both of its rows).
-   Both are deterministic.  No counterpart in the plugin. */
+   always runs the fused kernel (each SCVF evaluated by both of its rows);
+   2 is path 0 with dim+1 gather lanes per (node, element), one per row,
+   when the launch has 64 threads (nsfv1_set_launch).  All are
+   deterministic.  The FLOW stabilisation and the positive upwind always
+   use the coupled path.  No counterpart in the plugin. */
 int nsfv1_set_assembly_path(nsfv1_handle* h, int path);
 
 /* Launch shape of the assembly: incidences (threads) per workgroup (64..256,
