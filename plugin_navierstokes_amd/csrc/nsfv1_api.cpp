@@ -607,7 +607,7 @@ static int make_plan(nsfv1_handle* h, int jbytes, Plan& P)
     };
     if (!up(P.wg_node, wn) || !up(P.wg_rounds, wr) || !up(P.perm, perm)) return fail(NSFV1_ERR_DEVICE, "plan upload");
     {
-        const int pade = 16 / jbytes;
+        const int pade = nc == 4 ? 16 / jbytes : 0;  // the row kernels pad 3D node rows only
         std::vector<int32_t> la((size_t)h->ninc * 4), wa((size_t)P.nwg * 8);
         std::vector<uint64_t> lp((size_t)h->ninc);
         std::vector<uint64_t> hpos((size_t)h->ninc);

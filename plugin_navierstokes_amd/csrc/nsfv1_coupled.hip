@@ -721,7 +721,7 @@ coupled_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const JT* __r
     // 16 pad bytes after each node's rows: consecutive nodes then start in
     // different LDS banks, so the lanes of one round do not conflict, and
     // every (row, column block) stays 16-byte aligned for vector access
-    constexpr int PADE = 16 / (int)sizeof(JT);
+    constexpr int PADE = NC == 4 ? 16 / (int)sizeof(JT) : 0;  // 2D rows already have odd strides
     double* dbuf = reinterpret_cast<double*>(smem + (((jlen + (v1 - v0) * PADE) * (int)sizeof(JT) + 15) & ~15));
     const bool wantJ = J != nullptr;
     if (wantJ)
@@ -819,12 +819,17 @@ coupled_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const JT* __r
         }
         __syncthreads();
     }
-    if (wantJ)
-        for (int j = 0; j < v1 - v0; ++j) {
-            const int off = (int)(M.rowptr[(int64_t)(v0 + j) * NC] - jbase);
-            const int len = (int)(M.rowptr[(int64_t)(v0 + j + 1) * NC] - jbase) - off;
-            for (int i = threadIdx.x; i < len; i += blockDim.x) J[jbase + off + i] = jbuf[off + j * PADE + i];
+    if (wantJ) {
+        if constexpr (PADE == 0) {
+            for (int i = threadIdx.x; i < jlen; i += blockDim.x) J[jbase + i] = jbuf[i];
+        } else {
+            for (int j = 0; j < v1 - v0; ++j) {
+                const int off = (int)(M.rowptr[(int64_t)(v0 + j) * NC] - jbase);
+                const int len = (int)(M.rowptr[(int64_t)(v0 + j + 1) * NC] - jbase) - off;
+                for (int i = threadIdx.x; i < len; i += blockDim.x) J[jbase + off + i] = jbuf[off + j * PADE + i];
+            }
         }
+    }
     if (dout)
         for (int i = threadIdx.x; i < dlen; i += blockDim.x) dout[(int64_t)v0 * NC + i] = dbuf[i];
 }
