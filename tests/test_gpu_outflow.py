@@ -128,3 +128,24 @@ def test_outflow_cleared(cuda):
     J6, d6 = disc.assemble(u)
     assert not torch.equal(d0, d5)
     assert torch.equal(J0, J6) and torch.equal(d0, d6)
+
+
+def test_boundary_disc_errors(cuda):
+    from plugin_navierstokes_amd import NSFV1Error
+
+    mesh, belem, bside = _case(M.QUAD)
+    disc = NavierStokesFV1("u,v,p", "Inner")
+    configure_disc(disc, {"upwind": "full"})
+    disc.set_mesh(mesh)
+    for setter in (disc.set_outflow, disc.set_symmetry):
+        with pytest.raises(NSFV1Error) as e:
+            setter([mesh.ne], [0])
+        assert e.value.name == "ARGUMENT"
+        with pytest.raises(NSFV1Error):
+            setter([0], [4])  # a quadrilateral has sides 0..3
+    with pytest.raises(NSFV1Error):
+        disc.set_sliding_wall(belem, bside, float("nan"), 0.0)
+    # a side whose corner is not on the element: none exists by construction,
+    # so a valid list is accepted and an empty one removes the condition
+    disc.set_sliding_wall(belem, bside, 1.0, 0.5)
+    disc.set_sliding_wall([], [], 0.0, 0.0)
