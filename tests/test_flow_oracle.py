@@ -11,7 +11,9 @@ stabilisation (stabilization.cpp:436-772):
 - reproduction of a constant velocity by FIELDS and FLOW with the positive
   upwind (only true if the ip shapes enter with the right sign and index);
 - the Picard identity d = J u and conservation of the assembled defect, and
-  the continuity closure of the stabilised ip velocity."""
+  the continuity closure of the stabilised ip velocity;
+- the matrix case against numpy.linalg.solve of the ip system built from the
+  upwind's shapes on unit elements (closed-form diffusion length)."""
 from __future__ import annotations
 
 import numpy as np
@@ -190,3 +192,33 @@ def test_stabilised_velocity_closes_continuity(et, cfg):
         want[g["to"][ip]] -= f
     _, d = O.elem_A(p, et, co, u.ravel())
     np.testing.assert_allclose(d.reshape(nsh, dim + 1)[:, dim], want, rtol=1e-12, atol=1e-13 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("et,L2inv", [(O.QUAD, 32 / 7), (O.HEX, 3.2)])
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_matrix_case_against_numpy_solve(et, L2inv, seed):
+    """FIELDS with the positive upwind on the unit square / cube (RAW
+    diffusion length in closed form): assemble the ip system from the
+    upwind's corner and ip shapes and solve it with numpy.linalg.solve;
+    the oracle's stabilised velocities must agree (stabilization.cpp:
+    244-403, steady, no source)."""
+    co = np.asarray(LC[et], dtype=float)
+    dim, nsh, nip = O.DIM[et], O.NSH[et], O.NIP[et]
+    rng = np.random.default_rng(30 + seed)
+    u = rng.normal(size=(nsh, dim + 1))
+    nu, rho = 0.04, 1.3
+    g = O.geometry(et, co)
+    sv = g["phi"] @ u[:, :dim]
+    U, W, L = O.upwind("positive", et, co, sv, ip_shapes=True)
+    vv = nu * L2inv
+    vc = np.linalg.norm(sv, axis=1) / L
+    Mx = np.diag(vv + vc) - vc[:, None] * W
+    rhs = np.zeros((nip, dim))
+    for ip in range(nip):
+        cont = vv * g["phi"][ip] + vc[ip] * U[ip]
+        for d in range(dim):
+            rhs[ip, d] = cont @ u[:, d] - (g["grad"][ip][:, d] / rho) @ u[:, dim]
+    want = np.linalg.solve(Mx, rhs)
+    p = O.Params(dim=dim, upwind="positive", nu=nu, rho=rho)
+    got = O.export(p, et, co, np.arange(nsh, dtype=np.int32)[None], u, "velocity_ip", "scvf")[0]
+    np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-12 * np.abs(want).max())
