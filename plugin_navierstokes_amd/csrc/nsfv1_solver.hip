@@ -167,13 +167,16 @@ multi_dot_partial(const double* __restrict__ V, int64_t ld, int k, const double*
     }
 }
 
+// one workgroup per dot product: every thread sums a fixed strided subset
+// of the partials, then the fixed-order block reduction (deterministic)
 __global__ void multi_dot_final(const double* __restrict__ partial, int nblk, int k, double* __restrict__ out)
 {
-    const int i = threadIdx.x;
-    if (i >= k) return;
+    __shared__ double sh[4];
+    const int i = blockIdx.x;
     double s = 0;
-    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * 64 + i];
-    out[i] = s;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += partial[(int64_t)b * 64 + i];
+    const double tot = block_sum(s, sh);
+    if (threadIdx.x == 0) out[i] = tot;
 }
 
 // w -= sum_i h_i V_i
@@ -345,7 +348,7 @@ hipError_t launch_multi_dot(const double* V, int64_t ld, int k, const double* w,
                             double* out, hipStream_t s)
 {
     multi_dot_partial<<<RED_BLOCKS, 256, 0, s>>>(V, ld, k, w, n, partial);
-    multi_dot_final<<<1, 64, 0, s>>>(partial, RED_BLOCKS, k, out);
+    if (k > 0) multi_dot_final<<<k, 256, 0, s>>>(partial, RED_BLOCKS, k, out);
     return hipGetLastError();
 }
 
