@@ -118,6 +118,7 @@ struct nsfv1_handle {
     // mesh
     int et = -1;
     int64_t nv = 0, ne = 0, ndof = 0, nnz = 0, ninc = 0;
+    DevBuf adj;  // node-block neighbour lists: adj[rowptr[v nc] / nc^2 + s] = s-th neighbour of v
     DevBuf xyz, conn, rowptr, colidx, errflag, l2inv, vol, affine, node_inc, inc_elem, inc_node, inc_info, inc_pos;
     std::vector<int32_t> h_inc_info, h_inc_elem;
     // boundary discs (no-normal-stress outflow, symmetry, sliding wall): per
@@ -429,10 +430,11 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
     const int64_t nnz = rowptr[nv * nc];
     for (int64_t v = 0; v < nv; ++v)
         if (deg[v] > 255) return fail(NSFV1_ERR_ARGUMENT, "a node has more than 255 neighbours");
-    std::vector<int32_t> colidx;
+    std::vector<int32_t> colidx, adjv;
     std::vector<uint64_t> ipos;
     try {
         colidx.resize(nnz);
+        adjv.resize(nnz / (nc * nc));
         ipos.resize(ninc);
     } catch (...) {
         return fail(NSFV1_ERR_ARGUMENT, "pattern too large for host memory");
@@ -456,6 +458,7 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
                 }
                 ipos[q] = pk;
             }
+            for (size_t t = 0; t < nb.size(); ++t) adjv[rowptr[v * nc] / (nc * nc) + t] = nb[t];
             for (int a = 0; a < nc; ++a) {
                 int32_t* out = colidx.data() + rowptr[v * nc + a];
                 for (size_t t = 0; t < nb.size(); ++t)
@@ -472,6 +475,7 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
     bool ok = up(h->xyz, xyz, sizeof(double) * nv * h->dim) && up(h->conn, conn, sizeof(int32_t) * ne * nsh) &&
               up(h->rowptr, rowptr.data(), sizeof(int64_t) * rowptr.size()) &&
               up(h->colidx, colidx.data(), sizeof(int32_t) * nnz) &&
+              up(h->adj, adjv.data(), sizeof(int32_t) * std::max<size_t>(adjv.size(), 1)) &&
               up(h->inc_elem, ie.data(), sizeof(int32_t) * ninc) && up(h->inc_node, in.data(), sizeof(int32_t) * ninc) &&
               up(h->inc_info, ii.data(), sizeof(int32_t) * ninc) && up(h->inc_pos, ipos.data(), sizeof(uint64_t) * ninc) &&
               up(h->node_inc, ni32.data(), sizeof(int32_t) * ni32.size()) &&
@@ -1064,8 +1068,8 @@ int nsfv1_spmv(nsfv1_handle* h, const void* jvals, int jprec, const double* x, d
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
     if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
-    hipError_t e = launch_spmv(jprec, h->rowptr.as<int64_t>(), h->colidx.as<int32_t>(), jvals, x, y, h->ndof,
-                               (hipStream_t)stream);
+    hipError_t e = launch_spmv(jprec, h->rowptr.as<int64_t>(), h->colidx.as<int32_t>(), h->adj.as<int32_t>(),
+                               h->dim + 1, jvals, x, y, h->ndof, (hipStream_t)stream);
     if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("spmv: ") + hipGetErrorString(e));
     return NSFV1_OK;
 }
@@ -1291,11 +1295,12 @@ static hipError_t mg_cycle(nsfv1_handle* h, size_t l, const void* J0, const doub
     double* r = fine ? h->mg_r0.as<double>() : h->mg[l - 1]->r.as<double>();
     const int64_t* rp = c->rowptr.as<int64_t>();
     const int32_t* ci = c->colidx.as<int32_t>();
+    const int32_t* ad = c->adj.as<int32_t>();
     const int64_t n = c->ndof, nv = c->nv;
     const double w = h->mg_omega;
     hipError_t e = launch_bj_update(nc, Bi, b, x, w, 1, nv, s);
     for (int k = 1; k < h->mg_pre && e == hipSuccess; ++k) {
-        e = launch_residual(jp, rp, ci, J, x, b, r, n, s);
+        e = launch_residual(jp, rp, ci, ad, nc, J, x, b, r, n, s);
         if (e == hipSuccess) e = launch_bj_update(nc, Bi, r, x, w, 0, nv, s);
     }
     if (e != hipSuccess) return e;
@@ -1303,7 +1308,7 @@ static hipError_t mg_cycle(nsfv1_handle* h, size_t l, const void* J0, const doub
     nsfv1_handle* cc = C.h;
     // Dirichlet rows neither send residual to nor take corrections from the
     // coarse level (their equations are decoupled identities)
-    e = launch_residual(jp, rp, ci, J, x, b, r, n, s);
+    e = launch_residual(jp, rp, ci, ad, nc, J, x, b, r, n, s);
     if (e == hipSuccess) e = launch_zero_rows(c->dir_rows.as<int64_t>(), c->ndir, r, s);
     if (e == hipSuccess) e = launch_node_transfer(C.R_rp.as<int64_t>(), C.R_ci.as<int32_t>(), C.R_w.as<double>(), r, C.b.as<double>(), cc->nv, nc, 0, s);
     if (e == hipSuccess) e = launch_zero_rows(cc->dir_rows.as<int64_t>(), cc->ndir, C.b.as<double>(), s);
@@ -1312,7 +1317,7 @@ static hipError_t mg_cycle(nsfv1_handle* h, size_t l, const void* J0, const doub
     if (e == hipSuccess) e = launch_zero_rows(c->dir_rows.as<int64_t>(), c->ndir, r, s);
     if (e == hipSuccess) e = launch_axpby(1.0, r, 1.0, x, n, s);
     for (int k = 0; k < h->mg_post && e == hipSuccess; ++k) {
-        e = launch_residual(jp, rp, ci, J, x, b, r, n, s);
+        e = launch_residual(jp, rp, ci, ad, nc, J, x, b, r, n, s);
         if (e == hipSuccess) e = launch_bj_update(nc, Bi, r, x, w, 0, nv, s);
     }
     return e;
@@ -1459,6 +1464,7 @@ int nsfv1_linear_solve(nsfv1_handle* h, const void* jvals, int jprec, const doub
     double* dsm = h->gm_small.as<double>();
     const int64_t* rp = h->rowptr.as<int64_t>();
     const int32_t* ci = h->colidx.as<int32_t>();
+    const int32_t* ad = h->adj.as<int32_t>();
     auto chk = [&](hipError_t e) { return e == hipSuccess; };
     auto dot = [&](const double* Vb, int k, const double* vec, double* out) -> bool {
         if (!chk(launch_multi_dot(Vb, n, k, vec, n, part, dsm, s))) return false;
@@ -1487,7 +1493,7 @@ int nsfv1_linear_solve(nsfv1_handle* h, const void* jvals, int jprec, const doub
     std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
     while (it < maxit) {
         // r = b - J x  -> V0
-        if (!chk(launch_spmv(jprec, rp, ci, jvals, x, w, n, s)) || !chk(launch_axpby(1.0, b, -1.0, w, n, s)))
+        if (!chk(launch_spmv(jprec, rp, ci, ad, h->dim + 1, jvals, x, w, n, s)) || !chk(launch_axpby(1.0, b, -1.0, w, n, s)))
             return fail(NSFV1_ERR_DEVICE, "residual");
         double rn2;
         if (!dot(w, 1, w, &rn2)) return fail(NSFV1_ERR_DEVICE, "dot");
@@ -1499,7 +1505,7 @@ int nsfv1_linear_solve(nsfv1_handle* h, const void* jvals, int jprec, const doub
         g[0] = beta;
         int j = 0;
         for (; j < m && it < maxit; ++j, ++it) {
-            if (!chk(precond(V + (int64_t)j * n, z)) || !chk(launch_spmv(jprec, rp, ci, jvals, z, w, n, s)))
+            if (!chk(precond(V + (int64_t)j * n, z)) || !chk(launch_spmv(jprec, rp, ci, ad, h->dim + 1, jvals, z, w, n, s)))
                 return fail(NSFV1_ERR_DEVICE, "arnoldi");
             // classical Gram-Schmidt, twice
             double hcol[GMRES_MAX_RESTART + 1], h2[GMRES_MAX_RESTART + 1];

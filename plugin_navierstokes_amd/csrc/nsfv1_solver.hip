@@ -216,6 +216,54 @@ __global__ void multi_add(const double* __restrict__ V, int64_t ld, int k, const
 // ---------------------------------------------------------------------
 
 // r = b - A x, 16 lanes per row
+// Node-block variants: the pattern consists of (dim+1)x(dim+1) node blocks,
+// so the column of entry k of row (v, a) is adj[off(v) + s] * NC + b with
+// k - rowptr[row] = s * NC + b and off(v) = rowptr[v NC] / NC^2.  One
+// neighbour index per NC entries, shared by the node's NC rows, replaces a
+// column index per entry (4 of the 8 bytes an FP32 entry costs); the entries
+// are summed by the same lanes in the same order as the plain CSR kernels.
+template <typename JT, int NC>
+__global__ void __launch_bounds__(256)
+spmv_bcsr(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ adj, const JT* __restrict__ A,
+          const double* __restrict__ x, double* __restrict__ y, int64_t n)
+{
+    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int sub = threadIdx.x & 15;
+    double s = 0.0;
+    if (row < n) {
+        const int64_t a = rowptr[row], b = rowptr[row + 1];
+        const int32_t* ad = adj + rowptr[(row / NC) * NC] / (NC * NC);
+        for (int64_t k = a + sub; k < b; k += 16) {
+            const int rel = (int)(k - a), sl = rel / NC;
+            s += (double)A[k] * x[(int64_t)ad[sl] * NC + (rel - sl * NC)];
+        }
+    }
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
+    if (row < n && sub == 0) y[row] = s;
+}
+
+template <typename JT, int NC>
+__global__ void __launch_bounds__(256)
+residual_bcsr(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ adj, const JT* __restrict__ A,
+              const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ r, int64_t n)
+{
+    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+    const int sub = threadIdx.x & 15;
+    double s = 0.0;
+    if (row < n) {
+        const int64_t a = rowptr[row], e = rowptr[row + 1];
+        const int32_t* ad = adj + rowptr[(row / NC) * NC] / (NC * NC);
+        for (int64_t k = a + sub; k < e; k += 16) {
+            const int rel = (int)(k - a), sl = rel / NC;
+            s += (double)A[k] * x[(int64_t)ad[sl] * NC + (rel - sl * NC)];
+        }
+    }
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
+    if (row < n && sub == 0) r[row] = b[row] - s;
+}
+
 template <typename JT>
 __global__ void __launch_bounds__(256)
 residual_csr(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colidx, const JT* __restrict__ A,
@@ -313,12 +361,22 @@ hipError_t launch_dirichlet(int jprec, const int64_t* rowptr, const int32_t* col
     return hipGetLastError();
 }
 
-hipError_t launch_spmv(int jprec, const int64_t* rowptr, const int32_t* colidx, const void* A, const double* x,
+hipError_t launch_spmv(int jprec, const int64_t* rowptr, const int32_t* colidx, const int32_t* adj, int nc,
+                       const void* A, const double* x,
                        double* y, int64_t n, hipStream_t s)
 {
     const int bl = blocks_for(n * 16, 256);
-    if (jprec == 64) spmv_csr<double><<<bl, 256, 0, s>>>(rowptr, colidx, (const double*)A, x, y, n);
-    else spmv_csr<float><<<bl, 256, 0, s>>>(rowptr, colidx, (const float*)A, x, y, n);
+    if (adj && nc == 4) {
+        if (jprec == 64) spmv_bcsr<double, 4><<<bl, 256, 0, s>>>(rowptr, adj, (const double*)A, x, y, n);
+        else spmv_bcsr<float, 4><<<bl, 256, 0, s>>>(rowptr, adj, (const float*)A, x, y, n);
+    } else if (adj && nc == 3) {
+        if (jprec == 64) spmv_bcsr<double, 3><<<bl, 256, 0, s>>>(rowptr, adj, (const double*)A, x, y, n);
+        else spmv_bcsr<float, 3><<<bl, 256, 0, s>>>(rowptr, adj, (const float*)A, x, y, n);
+    } else if (jprec == 64) {
+        spmv_csr<double><<<bl, 256, 0, s>>>(rowptr, colidx, (const double*)A, x, y, n);
+    } else {
+        spmv_csr<float><<<bl, 256, 0, s>>>(rowptr, colidx, (const float*)A, x, y, n);
+    }
     return hipGetLastError();
 }
 
@@ -370,12 +428,22 @@ hipError_t launch_axpby(double a, const double* x, double b, double* y, int64_t 
     return hipGetLastError();
 }
 
-hipError_t launch_residual(int jprec, const int64_t* rowptr, const int32_t* colidx, const void* A, const double* x,
+hipError_t launch_residual(int jprec, const int64_t* rowptr, const int32_t* colidx, const int32_t* adj, int nc,
+                           const void* A, const double* x,
                            const double* b, double* r, int64_t n, hipStream_t s)
 {
     const int bl = blocks_for(n * 16, 256);
-    if (jprec == 64) residual_csr<double><<<bl, 256, 0, s>>>(rowptr, colidx, (const double*)A, x, b, r, n);
-    else residual_csr<float><<<bl, 256, 0, s>>>(rowptr, colidx, (const float*)A, x, b, r, n);
+    if (adj && nc == 4) {
+        if (jprec == 64) residual_bcsr<double, 4><<<bl, 256, 0, s>>>(rowptr, adj, (const double*)A, x, b, r, n);
+        else residual_bcsr<float, 4><<<bl, 256, 0, s>>>(rowptr, adj, (const float*)A, x, b, r, n);
+    } else if (adj && nc == 3) {
+        if (jprec == 64) residual_bcsr<double, 3><<<bl, 256, 0, s>>>(rowptr, adj, (const double*)A, x, b, r, n);
+        else residual_bcsr<float, 3><<<bl, 256, 0, s>>>(rowptr, adj, (const float*)A, x, b, r, n);
+    } else if (jprec == 64) {
+        residual_csr<double><<<bl, 256, 0, s>>>(rowptr, colidx, (const double*)A, x, b, r, n);
+    } else {
+        residual_csr<float><<<bl, 256, 0, s>>>(rowptr, colidx, (const float*)A, x, b, r, n);
+    }
     return hipGetLastError();
 }
 
