@@ -233,6 +233,8 @@ spmv_bcsr(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ adj, c
     if (row < n) {
         const int64_t a = rowptr[row], b = rowptr[row + 1];
         const int32_t* ad = adj + rowptr[(row / NC) * NC] / (NC * NC);
+        // unrolled so that several index -> value load chains are in flight
+#pragma unroll 4
         for (int64_t k = a + sub; k < b; k += 16) {
             const int rel = (int)(k - a), sl = rel / NC;
             s += (double)A[k] * x[(int64_t)ad[sl] * NC + (rel - sl * NC)];
@@ -254,6 +256,8 @@ residual_bcsr(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ ad
     if (row < n) {
         const int64_t a = rowptr[row], e = rowptr[row + 1];
         const int32_t* ad = adj + rowptr[(row / NC) * NC] / (NC * NC);
+        // unrolled so that several index -> value load chains are in flight
+#pragma unroll 4
         for (int64_t k = a + sub; k < e; k += 16) {
             const int rel = (int)(k - a), sl = rel / NC;
             s += (double)A[k] * x[(int64_t)ad[sl] * NC + (rel - sl * NC)];
