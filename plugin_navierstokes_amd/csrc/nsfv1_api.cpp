@@ -1069,7 +1069,7 @@ int nsfv1_spmv(nsfv1_handle* h, const void* jvals, int jprec, const double* x, d
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
     if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
     hipError_t e = launch_spmv(jprec, h->rowptr.as<int64_t>(), h->colidx.as<int32_t>(), h->adj.as<int32_t>(),
-                               h->dim + 1, jvals, x, y, h->ndof, (hipStream_t)stream);
+                               h->dim + 1, jvals, x, y, h->ndof, h->nnz, (hipStream_t)stream);
     if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("spmv: ") + hipGetErrorString(e));
     return NSFV1_OK;
 }
@@ -1300,7 +1300,7 @@ static hipError_t mg_cycle(nsfv1_handle* h, size_t l, const void* J0, const doub
     const double w = h->mg_omega;
     hipError_t e = launch_bj_update(nc, Bi, b, x, w, 1, nv, s);
     for (int k = 1; k < h->mg_pre && e == hipSuccess; ++k) {
-        e = launch_residual(jp, rp, ci, ad, nc, J, x, b, r, n, s);
+        e = launch_residual(jp, rp, ci, ad, nc, J, x, b, r, n, c->nnz, s);
         if (e == hipSuccess) e = launch_bj_update(nc, Bi, r, x, w, 0, nv, s);
     }
     if (e != hipSuccess) return e;
@@ -1308,7 +1308,7 @@ static hipError_t mg_cycle(nsfv1_handle* h, size_t l, const void* J0, const doub
     nsfv1_handle* cc = C.h;
     // Dirichlet rows neither send residual to nor take corrections from the
     // coarse level (their equations are decoupled identities)
-    e = launch_residual(jp, rp, ci, ad, nc, J, x, b, r, n, s);
+    e = launch_residual(jp, rp, ci, ad, nc, J, x, b, r, n, c->nnz, s);
     if (e == hipSuccess) e = launch_zero_rows(c->dir_rows.as<int64_t>(), c->ndir, r, s);
     if (e == hipSuccess) e = launch_node_transfer(C.R_rp.as<int64_t>(), C.R_ci.as<int32_t>(), C.R_w.as<double>(), r, C.b.as<double>(), cc->nv, nc, 0, s);
     if (e == hipSuccess) e = launch_zero_rows(cc->dir_rows.as<int64_t>(), cc->ndir, C.b.as<double>(), s);
@@ -1317,7 +1317,7 @@ static hipError_t mg_cycle(nsfv1_handle* h, size_t l, const void* J0, const doub
     if (e == hipSuccess) e = launch_zero_rows(c->dir_rows.as<int64_t>(), c->ndir, r, s);
     if (e == hipSuccess) e = launch_axpby(1.0, r, 1.0, x, n, s);
     for (int k = 0; k < h->mg_post && e == hipSuccess; ++k) {
-        e = launch_residual(jp, rp, ci, ad, nc, J, x, b, r, n, s);
+        e = launch_residual(jp, rp, ci, ad, nc, J, x, b, r, n, c->nnz, s);
         if (e == hipSuccess) e = launch_bj_update(nc, Bi, r, x, w, 0, nv, s);
     }
     return e;
@@ -1493,7 +1493,7 @@ int nsfv1_linear_solve(nsfv1_handle* h, const void* jvals, int jprec, const doub
     std::vector<double> H((m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
     while (it < maxit) {
         // r = b - J x  -> V0
-        if (!chk(launch_spmv(jprec, rp, ci, ad, h->dim + 1, jvals, x, w, n, s)) || !chk(launch_axpby(1.0, b, -1.0, w, n, s)))
+        if (!chk(launch_spmv(jprec, rp, ci, ad, h->dim + 1, jvals, x, w, n, h->nnz, s)) || !chk(launch_axpby(1.0, b, -1.0, w, n, s)))
             return fail(NSFV1_ERR_DEVICE, "residual");
         double rn2;
         if (!dot(w, 1, w, &rn2)) return fail(NSFV1_ERR_DEVICE, "dot");
@@ -1505,7 +1505,7 @@ int nsfv1_linear_solve(nsfv1_handle* h, const void* jvals, int jprec, const doub
         g[0] = beta;
         int j = 0;
         for (; j < m && it < maxit; ++j, ++it) {
-            if (!chk(precond(V + (int64_t)j * n, z)) || !chk(launch_spmv(jprec, rp, ci, ad, h->dim + 1, jvals, z, w, n, s)))
+            if (!chk(precond(V + (int64_t)j * n, z)) || !chk(launch_spmv(jprec, rp, ci, ad, h->dim + 1, jvals, z, w, n, h->nnz, s)))
                 return fail(NSFV1_ERR_DEVICE, "arnoldi");
             // classical Gram-Schmidt, twice
             double hcol[GMRES_MAX_RESTART + 1], h2[GMRES_MAX_RESTART + 1];

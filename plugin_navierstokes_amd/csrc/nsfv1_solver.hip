@@ -215,57 +215,90 @@ __global__ void multi_add(const double* __restrict__ V, int64_t ld, int k, const
 // the dense coarsest-level solve
 // ---------------------------------------------------------------------
 
-// r = b - A x, 16 lanes per row
-// Node-block variants: the pattern consists of (dim+1)x(dim+1) node blocks,
-// so the column of entry k of row (v, a) is adj[off(v) + s] * NC + b with
-// k - rowptr[row] = s * NC + b and off(v) = rowptr[v NC] / NC^2.  One
-// neighbour index per NC entries, shared by the node's NC rows, replaces a
-// column index per entry (4 of the 8 bytes an FP32 entry costs); the entries
-// are summed by the same lanes in the same order as the plain CSR kernels.
+// Node-block products.  The pattern consists of (dim+1)x(dim+1) node
+// blocks: node v has nb neighbours adj[off(v) + s] (off(v) = rowptr[v NC] /
+// NC^2), and its NC rows all have nb NC entries, so entry (v a, s NC + c)
+// sits at rowptr[v NC] + a nb NC + s NC + c.  A group of G lanes handles one
+// node: lane j takes the slots s = j, j + G, ..., reads the neighbour index
+// once and the NC x NC block as NC row segments (one 16-byte load per FP32
+// row when NC = 4), and keeps NC FP64 partial sums.  The G partials are then
+// combined by a reduce-scatter across the group (two exchange steps that
+// halve the set of rows a lane carries, then a butterfly), in a fixed order.
+// y = A x (RES = 0) or y = b - A x (RES = 1).
 template <typename JT, int NC>
-__global__ void __launch_bounds__(256)
-spmv_bcsr(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ adj, const JT* __restrict__ A,
-          const double* __restrict__ x, double* __restrict__ y, int64_t n)
+__device__ __forceinline__ void block_row(const JT* __restrict__ p, double (&q)[NC])
 {
-    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
-    const int sub = threadIdx.x & 15;
-    double s = 0.0;
-    if (row < n) {
-        const int64_t a = rowptr[row], b = rowptr[row + 1];
-        const int32_t* ad = adj + rowptr[(row / NC) * NC] / (NC * NC);
-        // unrolled so that several index -> value load chains are in flight
-#pragma unroll 4
-        for (int64_t k = a + sub; k < b; k += 16) {
-            const int rel = (int)(k - a), sl = rel / NC;
-            s += (double)A[k] * x[(int64_t)ad[sl] * NC + (rel - sl * NC)];
-        }
-    }
 #pragma unroll
-    for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
-    if (row < n && sub == 0) y[row] = s;
+    for (int c = 0; c < NC; ++c) q[c] = (double)p[c];
+}
+template <>
+__device__ __forceinline__ void block_row<float, 4>(const float* __restrict__ p, double (&q)[4])
+{
+    const float4 t = *(const float4*)p;
+    q[0] = t.x, q[1] = t.y, q[2] = t.z, q[3] = t.w;
+}
+template <>
+__device__ __forceinline__ void block_row<double, 4>(const double* __restrict__ p, double (&q)[4])
+{
+    const double2 t0 = *(const double2*)p, t1 = *(const double2*)(p + 2);
+    q[0] = t0.x, q[1] = t0.y, q[2] = t1.x, q[3] = t1.y;
 }
 
-template <typename JT, int NC>
+template <typename JT, int NC, int G, int RES>
 __global__ void __launch_bounds__(256)
-residual_bcsr(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ adj, const JT* __restrict__ A,
-              const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ r, int64_t n)
+nodeblock_mv(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ adj, const JT* __restrict__ A,
+             const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ y, int64_t nv)
 {
-    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;
-    const int sub = threadIdx.x & 15;
-    double s = 0.0;
-    if (row < n) {
-        const int64_t a = rowptr[row], e = rowptr[row + 1];
-        const int32_t* ad = adj + rowptr[(row / NC) * NC] / (NC * NC);
-        // unrolled so that several index -> value load chains are in flight
-#pragma unroll 4
-        for (int64_t k = a + sub; k < e; k += 16) {
-            const int rel = (int)(k - a), sl = rel / NC;
-            s += (double)A[k] * x[(int64_t)ad[sl] * NC + (rel - sl * NC)];
+    static_assert(NC == 3 || NC == 4, "node blocks of 3 or 4 unknowns");
+    static_assert(G >= 4 && G <= 64 && (G & (G - 1)) == 0, "group size");
+    const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const int sub = threadIdx.x & (G - 1);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (v < nv) {
+        const int64_t base = rowptr[v * NC];
+        const int nb = (int)((rowptr[v * NC + 1] - base) / NC);
+        const int32_t* ad = adj + base / (NC * NC);
+        const JT* Av = A + base;
+#pragma unroll 2
+        for (int sl = sub; sl < nb; sl += G) {
+            const int64_t col = ad[sl];
+            double xv[NC];
+            if constexpr (NC == 4) {
+                const double2 x0 = *(const double2*)(x + col * 4), x1 = *(const double2*)(x + col * 4 + 2);
+                xv[0] = x0.x, xv[1] = x0.y, xv[2] = x1.x, xv[3] = x1.y;
+            } else {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) xv[c] = x[col * NC + c];
+            }
+#pragma unroll
+            for (int a = 0; a < NC; ++a) {
+                double q[NC];
+                block_row<JT, NC>(Av + (int64_t)a * nb * NC + sl * NC, q);
+                double t = acc[a];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) t += q[c] * xv[c];
+                acc[a] = t;
+            }
         }
     }
+    // reduce-scatter over the G lanes of the group: after the first step a
+    // lane carries rows {0,1} or {2,3}, after the second one row
+    const bool h1 = sub & (G / 2), h2 = sub & (G / 4);
+    double w0, w1;
+    {
+        const double s0 = h1 ? acc[0] : acc[2], s1 = h1 ? acc[1] : acc[3];
+        const double r0 = __shfl_xor(s0, G / 2, G), r1 = __shfl_xor(s1, G / 2, G);
+        w0 = (h1 ? acc[2] : acc[0]) + r0;
+        w1 = (h1 ? acc[3] : acc[1]) + r1;
+    }
+    double z = (h2 ? w1 : w0) + __shfl_xor(h2 ? w0 : w1, G / 4, G);
 #pragma unroll
-    for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
-    if (row < n && sub == 0) r[row] = b[row] - s;
+    for (int o = G / 8; o > 0; o >>= 1) z += __shfl_xor(z, o, G);
+    const int a = (h1 ? 2 : 0) + (h2 ? 1 : 0);
+    if (v < nv && (sub & (G / 4 - 1)) == 0 && a < NC) {
+        const int64_t row = v * NC + a;
+        y[row] = RES ? b[row] - z : z;
+    }
 }
 
 template <typename JT>
@@ -365,21 +398,51 @@ hipError_t launch_dirichlet(int jprec, const int64_t* rowptr, const int32_t* col
     return hipGetLastError();
 }
 
-hipError_t launch_spmv(int jprec, const int64_t* rowptr, const int32_t* colidx, const int32_t* adj, int nc,
-                       const void* A, const double* x,
-                       double* y, int64_t n, hipStream_t s)
+// node-block launch: G lanes per node, G the power of two (8, 16 or 32) that
+// covers the mean number of neighbour blocks per node
+template <typename JT, int NC, int RES>
+static void launch_nodeblock(const int64_t* rowptr, const int32_t* adj, const JT* A, const double* x, const double* b,
+                             double* y, int64_t nv, int64_t nnz, hipStream_t s)
 {
-    const int bl = blocks_for(n * 16, 256);
-    if (adj && nc == 4) {
-        if (jprec == 64) spmv_bcsr<double, 4><<<bl, 256, 0, s>>>(rowptr, adj, (const double*)A, x, y, n);
-        else spmv_bcsr<float, 4><<<bl, 256, 0, s>>>(rowptr, adj, (const float*)A, x, y, n);
-    } else if (adj && nc == 3) {
-        if (jprec == 64) spmv_bcsr<double, 3><<<bl, 256, 0, s>>>(rowptr, adj, (const double*)A, x, y, n);
-        else spmv_bcsr<float, 3><<<bl, 256, 0, s>>>(rowptr, adj, (const float*)A, x, y, n);
-    } else if (jprec == 64) {
-        spmv_csr<double><<<bl, 256, 0, s>>>(rowptr, colidx, (const double*)A, x, y, n);
+    const int64_t slots = nv > 0 ? nnz / ((int64_t)NC * NC * nv) : 0;
+    if (slots > 16)
+        nodeblock_mv<JT, NC, 32, RES><<<blocks_for(nv * 32, 256), 256, 0, s>>>(rowptr, adj, A, x, b, y, nv);
+    else if (slots > 8)
+        nodeblock_mv<JT, NC, 16, RES><<<blocks_for(nv * 16, 256), 256, 0, s>>>(rowptr, adj, A, x, b, y, nv);
+    else
+        nodeblock_mv<JT, NC, 8, RES><<<blocks_for(nv * 8, 256), 256, 0, s>>>(rowptr, adj, A, x, b, y, nv);
+}
+
+// with 4 unknowns per node the node-block kernels take 16-byte vector loads
+// of A and x
+static bool nodeblock_ok(const int32_t* adj, int nc, const void* A, const double* x)
+{
+    return adj && (nc == 3 || (nc == 4 && (((uintptr_t)A | (uintptr_t)x) & 15) == 0));
+}
+
+template <int RES>
+static void launch_mv(int jprec, const int64_t* rowptr, const int32_t* adj, int nc, const void* A, const double* x,
+                      const double* b, double* y, int64_t n, int64_t nnz, hipStream_t s)
+{
+    const int64_t nv = n / nc;
+    if (nc == 4) {
+        if (jprec == 64) launch_nodeblock<double, 4, RES>(rowptr, adj, (const double*)A, x, b, y, nv, nnz, s);
+        else launch_nodeblock<float, 4, RES>(rowptr, adj, (const float*)A, x, b, y, nv, nnz, s);
     } else {
-        spmv_csr<float><<<bl, 256, 0, s>>>(rowptr, colidx, (const float*)A, x, y, n);
+        if (jprec == 64) launch_nodeblock<double, 3, RES>(rowptr, adj, (const double*)A, x, b, y, nv, nnz, s);
+        else launch_nodeblock<float, 3, RES>(rowptr, adj, (const float*)A, x, b, y, nv, nnz, s);
+    }
+}
+
+hipError_t launch_spmv(int jprec, const int64_t* rowptr, const int32_t* colidx, const int32_t* adj, int nc,
+                       const void* A, const double* x, double* y, int64_t n, int64_t nnz, hipStream_t s)
+{
+    if (nodeblock_ok(adj, nc, A, x)) {
+        launch_mv<0>(jprec, rowptr, adj, nc, A, x, nullptr, y, n, nnz, s);
+    } else {
+        const int bl = blocks_for(n * 16, 256);
+        if (jprec == 64) spmv_csr<double><<<bl, 256, 0, s>>>(rowptr, colidx, (const double*)A, x, y, n);
+        else spmv_csr<float><<<bl, 256, 0, s>>>(rowptr, colidx, (const float*)A, x, y, n);
     }
     return hipGetLastError();
 }
@@ -433,20 +496,15 @@ hipError_t launch_axpby(double a, const double* x, double b, double* y, int64_t 
 }
 
 hipError_t launch_residual(int jprec, const int64_t* rowptr, const int32_t* colidx, const int32_t* adj, int nc,
-                           const void* A, const double* x,
-                           const double* b, double* r, int64_t n, hipStream_t s)
+                           const void* A, const double* x, const double* b, double* r, int64_t n, int64_t nnz,
+                           hipStream_t s)
 {
-    const int bl = blocks_for(n * 16, 256);
-    if (adj && nc == 4) {
-        if (jprec == 64) residual_bcsr<double, 4><<<bl, 256, 0, s>>>(rowptr, adj, (const double*)A, x, b, r, n);
-        else residual_bcsr<float, 4><<<bl, 256, 0, s>>>(rowptr, adj, (const float*)A, x, b, r, n);
-    } else if (adj && nc == 3) {
-        if (jprec == 64) residual_bcsr<double, 3><<<bl, 256, 0, s>>>(rowptr, adj, (const double*)A, x, b, r, n);
-        else residual_bcsr<float, 3><<<bl, 256, 0, s>>>(rowptr, adj, (const float*)A, x, b, r, n);
-    } else if (jprec == 64) {
-        residual_csr<double><<<bl, 256, 0, s>>>(rowptr, colidx, (const double*)A, x, b, r, n);
+    if (nodeblock_ok(adj, nc, A, x)) {
+        launch_mv<1>(jprec, rowptr, adj, nc, A, x, b, r, n, nnz, s);
     } else {
-        residual_csr<float><<<bl, 256, 0, s>>>(rowptr, colidx, (const float*)A, x, b, r, n);
+        const int bl = blocks_for(n * 16, 256);
+        if (jprec == 64) residual_csr<double><<<bl, 256, 0, s>>>(rowptr, colidx, (const double*)A, x, b, r, n);
+        else residual_csr<float><<<bl, 256, 0, s>>>(rowptr, colidx, (const float*)A, x, b, r, n);
     }
     return hipGetLastError();
 }

@@ -14,10 +14,10 @@ constexpr int MG_DENSE_MAX = 1200;  // largest coarsest-level system inverted de
 
 hipError_t launch_dirichlet(int jprec, const int64_t* rowptr, const int32_t* colidx, const int64_t* rows,
                             int64_t nrows, void* J, double* d, const double* u, const double* g, hipStream_t s);
-// adj: node-block neighbour lists (nullptr: plain CSR column indices); nc = dim+1
+// adj: node-block neighbour lists (nullptr: plain CSR column indices); nc = dim+1;
+// n rows, nnz entries
 hipError_t launch_spmv(int jprec, const int64_t* rowptr, const int32_t* colidx, const int32_t* adj, int nc,
-                       const void* A, const double* x,
-                       double* y, int64_t n, hipStream_t s);
+                       const void* A, const double* x, double* y, int64_t n, int64_t nnz, hipStream_t s);
 hipError_t launch_block_jacobi_setup(int jprec, int nc, const int64_t* rowptr, const int32_t* colidx, const void* A,
                                      int64_t nv, double* Binv, int* err, hipStream_t s);
 hipError_t launch_block_jacobi_apply(int nc, const double* Binv, const double* x, double* y, int64_t nv, hipStream_t s);
@@ -27,8 +27,8 @@ hipError_t launch_multi_axpy(const double* V, int64_t ld, int k, const double* h
 hipError_t launch_multi_add(const double* V, int64_t ld, int k, const double* c, double* x, int64_t n, hipStream_t s);
 hipError_t launch_axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t s);
 hipError_t launch_residual(int jprec, const int64_t* rowptr, const int32_t* colidx, const int32_t* adj, int nc,
-                           const void* A, const double* x,
-                           const double* b, double* r, int64_t n, hipStream_t s);
+                           const void* A, const double* x, const double* b, double* r, int64_t n, int64_t nnz,
+                           hipStream_t s);
 hipError_t launch_bj_update(int nc, const double* Binv, const double* r, double* x, double omega, int first,
                             int64_t nv, hipStream_t s);
 hipError_t launch_node_transfer(const int64_t* rowptr, const int32_t* colidx, const double* w, const double* x,
