@@ -136,11 +136,21 @@ def autotune(disc, run, candidates=None, reps: int = 5, log=None):
     if candidates is None:
         candidates = [(p, t, l) for p in (0, 1) for t in (64, 128, 256) for l in (16384, 32768, 65536)]
         candidates += [(2, 64, l) for l in (16384, 65536)]
+    from . import NSFV1Error
+
     table = []
     for p, t, l in candidates:
         disc.set_assembly_path(p)
         disc.set_launch(t, l)
-        run()  # builds the plan for this shape
+        try:
+            run()  # builds the plan for this shape
+        except NSFV1Error as err:
+            # a shape whose budget cannot hold one node row of this mesh
+            if err.name != "ARGUMENT":
+                raise
+            if log:
+                log(f"launch path={p} threads={t} lds={l}: not possible ({err})")
+            continue
         torch.cuda.synchronize()
         s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         s.record()
@@ -152,6 +162,8 @@ def autotune(disc, run, candidates=None, reps: int = 5, log=None):
         table.append((p, t, l, ms))
         if log:
             log(f"launch path={p} threads={t} lds={l}: {ms:.3f} ms")
+    if not table:
+        raise ValueError("no launch shape fits this mesh")
     best = min(table, key=lambda r: r[3])
     disc.set_assembly_path(best[0])
     disc.set_launch(best[1], best[2])

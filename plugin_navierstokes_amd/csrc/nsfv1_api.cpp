@@ -144,7 +144,7 @@ struct nsfv1_handle {
     bool mg_ready = false, use_mg = false;
     int mg_jprec = 0;
     int mg_ncoarse = 0;
-    DevBuf mg_r0, mg_dense;
+    DevBuf mg_r0, mg_dense, mg_gj;  // mg_gj: Gauss-Jordan tableaux of the coarsest level
     bool geo_valid = false;
     int geo_difflen = -1;
     // split assembly path (per-SCVF records)
@@ -1159,7 +1159,8 @@ int nsfv1_mg_set_hierarchy(nsfv1_handle* h, int nlev, nsfv1_handle* const* coars
         return fail(NSFV1_ERR_ARGUMENT, "coarsest level has " + std::to_string(ncoarse) + " DoFs; at most " +
                                             std::to_string(MG_DENSE_MAX) + " are solved densely");
     if (h->mg_r0.alloc(sizeof(double) * h->ndof) != hipSuccess ||
-        h->mg_dense.alloc(sizeof(double) * ncoarse * ncoarse) != hipSuccess)
+        h->mg_dense.alloc(sizeof(double) * ncoarse * ncoarse) != hipSuccess ||
+        h->mg_gj.alloc(dense_inverse_work_bytes((int)ncoarse)) != hipSuccess)
         return fail(NSFV1_ERR_DEVICE, "multigrid workspace");
     // coarse levels carry the configuration of the fine discretisation
     for (auto& L : lv) {
@@ -1187,40 +1188,6 @@ int nsfv1_mg_set_hierarchy(nsfv1_handle* h, int nlev, nsfv1_handle* const* coars
     h->mg_ncoarse = (int)ncoarse;
     h->mg_ready = false;
     return NSFV1_OK;
-}
-
-// dense Gauss-Jordan inverse with partial pivoting (coarsest level only)
-static bool dense_inverse(std::vector<double>& A, int n, std::vector<double>& inv)
-{
-    inv.assign((size_t)n * n, 0.0);
-    for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
-    for (int c = 0; c < n; ++c) {
-        int p = c;
-        double best = std::fabs(A[(size_t)c * n + c]);
-        for (int r = c + 1; r < n; ++r)
-            if (std::fabs(A[(size_t)r * n + c]) > best) { best = std::fabs(A[(size_t)r * n + c]); p = r; }
-        if (best == 0.0) return false;
-        if (p != c)
-            for (int k = 0; k < n; ++k) {
-                std::swap(A[(size_t)c * n + k], A[(size_t)p * n + k]);
-                std::swap(inv[(size_t)c * n + k], inv[(size_t)p * n + k]);
-            }
-        const double ip = 1.0 / A[(size_t)c * n + c];
-        for (int k = 0; k < n; ++k) {
-            A[(size_t)c * n + k] *= ip;
-            inv[(size_t)c * n + k] *= ip;
-        }
-        for (int r = 0; r < n; ++r) {
-            if (r == c) continue;
-            const double f = A[(size_t)r * n + c];
-            if (f == 0.0) continue;
-            for (int k = 0; k < n; ++k) {
-                A[(size_t)r * n + k] -= f * A[(size_t)c * n + k];
-                inv[(size_t)r * n + k] -= f * inv[(size_t)c * n + k];
-            }
-        }
-    }
-    return true;
 }
 
 int nsfv1_mg_setup(nsfv1_handle* h, int jprec, const double* u, const double* u_old, double dt, double s_a,
@@ -1257,25 +1224,19 @@ int nsfv1_mg_setup(nsfv1_handle* h, int jprec, const double* u, const double* u_
         uf = L.u.as<double>();
         uof = uof ? L.uold.as<double>() : nullptr;
     }
-    // coarsest level: dense inverse on the host
+    // coarsest level: dense inverse, Gauss-Jordan on the device
     MGLevel& C = *h->mg.back();
     nsfv1_handle* c = C.h;
     const int n = (int)c->ndof;
-    std::vector<char> jh(js * c->nnz);
-    if (hipMemcpyAsync(jh.data(), C.J.p, jh.size(), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
-        return fail(NSFV1_ERR_DEVICE, "coarse Jacobian download");
-    std::vector<int32_t> cci(c->nnz);
-    if (hipMemcpy(cci.data(), c->colidx.p, sizeof(int32_t) * c->nnz, hipMemcpyDeviceToHost) != hipSuccess)
-        return fail(NSFV1_ERR_DEVICE, "coarse pattern download");
-    std::vector<double> A((size_t)n * n, 0.0), inv;
-    for (int r = 0; r < n; ++r)
-        for (int64_t k = c->h_rowptr[r]; k < c->h_rowptr[r + 1]; ++k)
-            A[(size_t)r * n + cci[k]] = jprec == NSFV1_F64 ? reinterpret_cast<const double*>(jh.data())[k]
-                                                           : (double)reinterpret_cast<const float*>(jh.data())[k];
-    if (!dense_inverse(A, n, inv)) return fail(NSFV1_ERR_ARGUMENT, "coarsest multigrid level is singular");
-    if (hipMemcpyAsync(h->mg_dense.p, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+    int* gj_err = (int*)((char*)h->mg_gj.p + dense_inverse_work_bytes(n)) - 1;
+    if (launch_dense_inverse(jprec, c->rowptr.as<int64_t>(), c->colidx.as<int32_t>(), C.J.p, n, h->mg_gj.p,
+                             h->mg_dense.as<double>(), gj_err, s) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "coarse inverse");
+    int herr = 0;
+    if (hipMemcpyAsync(&herr, gj_err, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
-        return fail(NSFV1_ERR_DEVICE, "coarse inverse upload");
+        return fail(NSFV1_ERR_DEVICE, "coarse inverse");
+    if (herr) return fail(NSFV1_ERR_ARGUMENT, "coarsest multigrid level is singular");
     h->mg_jprec = jprec;
     h->mg_ready = true;
     return NSFV1_OK;

@@ -383,6 +383,99 @@ dense_matvec(const double* __restrict__ A, const double* __restrict__ x, double*
 }
 
 // ---------------------------------------------------------------------
+// coarsest-level dense inverse: Gauss-Jordan in exchange form with partial
+// pivoting and no row swaps.  The tableau y = M x starts as the coarse
+// matrix; step k takes the unpivoted row p with the largest |M[p][k]| and
+// exchanges y_p with x_k:
+//   M'[p][k] = 1/a, M'[p][j] = -M[p][j]/a,
+//   M'[i][k] = M[i][k]/a, M'[i][j] = M[i][j] - M'[i][k] M[p][j]   (i != p),
+// a = M[p][k].  After n steps x_k = sum_j M[p_k][j] y_{p_j}, so
+// inv[k][p_j] = M[p_k][j].  One launch per step, ping-pong tableaux; each
+// step also writes |M'[i][k+1]| of the unpivoted rows (-1 for pivoted ones)
+// for the next step's pivot search, which every workgroup repeats with
+// the same fixed-order reduction (ties go to the smaller row).
+// ---------------------------------------------------------------------
+template <typename JT>
+__global__ void __launch_bounds__(256)
+gj_load(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colidx, const JT* __restrict__ J, int n,
+        double* __restrict__ M, double* __restrict__ cabs, int* __restrict__ done, int* __restrict__ err)
+{
+    const int r = blockIdx.x;
+    double* row = M + (int64_t)r * n;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) row[j] = 0.0;
+    __syncthreads();
+    for (int64_t k = rowptr[r] + threadIdx.x; k < rowptr[r + 1]; k += blockDim.x) row[colidx[k]] = (double)J[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        cabs[r] = fabs(row[0]);
+        done[r] = 0;
+        if (r == 0) *err = 0;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+gj_step(const double* __restrict__ Min, double* __restrict__ Mout, const double* __restrict__ cin,
+        double* __restrict__ cout, int* __restrict__ done, int* __restrict__ piv, int* __restrict__ err, int n, int k)
+{
+    __shared__ double sv[4];
+    __shared__ int si[4];
+    const int i = blockIdx.x;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    // pivot search over the unpivoted rows
+    double bv = -2.0;
+    int bi = 0;
+    for (int r = t; r < n; r += 256) {
+        const double v = cin[r];
+        if (v > bv) bv = v, bi = r;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ov > bv || (ov == bv && oi < bi)) bv = ov, bi = oi;
+    }
+    if (lane == 0) sv[w] = bv, si[w] = bi;
+    __syncthreads();
+    bv = sv[0], bi = si[0];
+#pragma unroll
+    for (int q = 1; q < 4; ++q)
+        if (sv[q] > bv || (sv[q] == bv && si[q] < bi)) bv = sv[q], bi = si[q];
+    const int p = bi;
+    const bool was_done = done[i] != 0;
+    if (t == 0) {
+        if (i == 0 && !(bv > 0.0)) *err = 1;
+        if (i == p) done[p] = 1, piv[k] = p;
+    }
+    const double* rp = Min + (int64_t)p * n;
+    const double* ri = Min + (int64_t)i * n;
+    double* ro = Mout + (int64_t)i * n;
+    const double ia = 1.0 / rp[k];
+    const bool pivoted = was_done || i == p;
+    if (i == p) {
+        for (int j = t; j < n; j += 256) {
+            const double v = j == k ? ia : -rp[j] * ia;
+            ro[j] = v;
+            if (j == k + 1) cout[i] = -1.0;
+        }
+    } else {
+        const double f = ri[k] * ia;
+        for (int j = t; j < n; j += 256) {
+            const double v = j == k ? f : ri[j] - f * rp[j];
+            ro[j] = v;
+            if (j == k + 1) cout[i] = pivoted ? -1.0 : fabs(v);
+        }
+    }
+}
+
+__global__ void gj_unpermute(const double* __restrict__ M, const int* __restrict__ piv, int n, double* __restrict__ inv)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * n) return;
+    const int k = (int)(e / n), j = (int)(e - (int64_t)k * n);
+    inv[(int64_t)k * n + piv[j]] = M[(int64_t)piv[k] * n + j];
+}
+
+// ---------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------
 static inline int blocks_for(int64_t n, int t) { return (int)((n + t - 1) / t); }
@@ -535,6 +628,27 @@ hipError_t launch_zero_rows(const int64_t* rows, int64_t nrows, double* y, hipSt
 hipError_t launch_inject(const int64_t* inj, const double* uf, double* uc, int64_t nvc, int nc, hipStream_t s)
 {
     inject_state<<<blocks_for(nvc * nc, 256), 256, 0, s>>>(inj, uf, uc, nvc, nc);
+    return hipGetLastError();
+}
+
+size_t dense_inverse_work_bytes(int n)
+{
+    return sizeof(double) * (2 * (size_t)n * n + 2 * (size_t)n) + sizeof(int) * (2 * (size_t)n + 4);
+}
+
+hipError_t launch_dense_inverse(int jprec, const int64_t* rowptr, const int32_t* colidx, const void* J, int n,
+                                void* work, double* inv, int* err, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    double* M[2] = {(double*)work, (double*)work + (size_t)n * n};
+    double* c[2] = {M[1] + (size_t)n * n, M[1] + (size_t)n * n + n};
+    int* done = (int*)(c[1] + n);
+    int* piv = done + n;
+    if (jprec == 64) gj_load<double><<<n, 256, 0, s>>>(rowptr, colidx, (const double*)J, n, M[0], c[0], done, err);
+    else gj_load<float><<<n, 256, 0, s>>>(rowptr, colidx, (const float*)J, n, M[0], c[0], done, err);
+    for (int k = 0; k < n; ++k)
+        gj_step<<<n, 256, 0, s>>>(M[k & 1], M[(k + 1) & 1], c[k & 1], c[(k + 1) & 1], done, piv, err, n, k);
+    gj_unpermute<<<blocks_for((int64_t)n * n, 256), 256, 0, s>>>(M[n & 1], piv, n, inv);
     return hipGetLastError();
 }
 

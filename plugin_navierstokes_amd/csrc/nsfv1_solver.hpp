@@ -36,5 +36,10 @@ hipError_t launch_node_transfer(const int64_t* rowptr, const int32_t* colidx, co
 hipError_t launch_zero_rows(const int64_t* rows, int64_t nrows, double* y, hipStream_t s);
 hipError_t launch_inject(const int64_t* inj, const double* uf, double* uc, int64_t nvc, int nc, hipStream_t s);
 hipError_t launch_dense_matvec(const double* A, const double* x, double* y, int n, hipStream_t s);
+// inverse of the n x n CSR matrix (n <= MG_DENSE_MAX) into the dense row-major
+// inv; *err (device) = 1 when a pivot is zero (singular matrix)
+size_t dense_inverse_work_bytes(int n);
+hipError_t launch_dense_inverse(int jprec, const int64_t* rowptr, const int32_t* colidx, const void* J, int n,
+                                void* work, double* inv, int* err, hipStream_t s);
 
 }  // namespace nsfv1

@@ -163,11 +163,15 @@ def load(path):
 # ---------------------------------------------------------------------
 # the channel workload: plain solver, trajectories, training
 # ---------------------------------------------------------------------
-def channel_case(seed: int, cells=(96, 48, 48), dt: float = 0.005):
+def channel_case(seed: int, cells=(96, 48, 48), dt: float = 0.005, tune: bool = False):
     """Seeded channel: Re in [100, 500], 1-4 obstacle boxes.  Returns
-    (mesh, disc, newton, stepper, u0, info)."""
-    from . import F32
+    (mesh, disc, newton, stepper, u0, info).  tune: pick the assembly path
+    and launch shape of the fine level with the autotuner first."""
+    import torch
+
+    from . import F32, PART_A, PART_M
     from . import meshes as M
+    from .batch import autotune
     from .solver import BDFStepper, NewtonSolver, build_multigrid
 
     rng = np.random.default_rng(1000 + seed)
@@ -185,7 +189,13 @@ def channel_case(seed: int, cells=(96, 48, 48), dt: float = 0.005):
                       restart=30, multigrid=True)
     stepper = BDFStepper(nt, dt, reference="old", mg_lag=True)
     u0 = M.channel_initial_state(mesh, (dofs, vals))
-    return mesh, disc, nt, stepper, u0, {"seed": seed, "Re": re, "boxes": nbox}
+    info = {"seed": seed, "Re": re, "boxes": nbox}
+    if tune:
+        ud = torch.from_numpy(u0).to("cuda:0")
+        best, _ = autotune(disc, lambda: disc.assemble(ud, ud, dt, s_m=1.0, parts=PART_A | PART_M, jprec=F32),
+                           reps=3)
+        info["launch"] = {"path": best[0], "threads": best[1], "lds_bytes": best[2], "ms": best[3]}
+    return mesh, disc, nt, stepper, u0, info
 
 
 def run_trajectory(case, nsteps, predictor=None, record=None, log=None, warm=0):
@@ -294,7 +304,8 @@ def compare(weights, seeds=(100, 101), steps=8, cells=(96, 48, 48), log=print, w
     res = {"plain": [], "learned": [], "plain_newton": [], "learned_newton": [], "learned_choice": []}
     for sd in seeds:
         for mode in ("plain", "learned"):
-            case = channel_case(sd, cells)
+            case = channel_case(sd, cells, tune=True)
+            res["launch"] = case[5].get("launch")
             pred = LearnedPredictor(case[1], mlp, in_scale, out_scale) if mode == "learned" else None
             out = run_trajectory(case, steps, predictor=pred, log=None, warm=warm)
             res[mode] += [o[0] for o in out]
@@ -310,7 +321,7 @@ def compare(weights, seeds=(100, 101), steps=8, cells=(96, 48, 48), log=print, w
             "prediction_used_fraction": float(np.mean(np.asarray(res["learned_choice"]) == 0)),
             "plain_initial_residual": float(np.mean(res["plain_r0"])),
             "learned_initial_residual": float(np.mean(res["learned_r0"])),
-            "steps_timed": len(res["plain"]), "warm_steps": warm}
+            "steps_timed": len(res["plain"]), "warm_steps": warm, "launch": res.get("launch")}
 
 
 if __name__ == "__main__":
