@@ -1468,19 +1468,22 @@ int nsfv1_linear_solve(nsfv1_handle* h, const void* jvals, int jprec, const doub
         for (; j < m && it < maxit; ++j, ++it) {
             if (!chk(precond(V + (int64_t)j * n, z)) || !chk(launch_spmv(jprec, rp, ci, ad, h->dim + 1, jvals, z, w, n, h->nnz, s)))
                 return fail(NSFV1_ERR_DEVICE, "arnoldi");
-            // classical Gram-Schmidt, twice
-            double hcol[GMRES_MAX_RESTART + 1], h2[GMRES_MAX_RESTART + 1];
-            for (int pass = 0; pass < 2; ++pass) {
-                double* hc = pass == 0 ? hcol : h2;
-                if (!dot(V, j + 1, w, hc)) return fail(NSFV1_ERR_DEVICE, "dot");
-                if (hipMemcpyAsync(dsm + 128, hc, sizeof(double) * (j + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
-                    !chk(launch_multi_axpy(V, n, j + 1, dsm + 128, w, n, s)))
+            // classical Gram-Schmidt, twice; the projections stay on the
+            // device, and one download brings both and ||w||^2 to the host
+            constexpr int HS = GMRES_MAX_RESTART + 1;
+            static_assert(128 + 2 * HS + 1 <= 256, "GMRES device scratch holds 256 doubles");
+            double* hd = dsm + 128;  // [h1 (HS) | h2 (HS) | ||w||^2]
+            for (int pass = 0; pass < 2; ++pass)
+                if (!chk(launch_multi_dot(V, n, j + 1, w, n, part, hd + pass * HS, s)) ||
+                    !chk(launch_multi_axpy(V, n, j + 1, hd + pass * HS, w, n, s)))
                     return fail(NSFV1_ERR_DEVICE, "orthogonalise");
-            }
-            for (int i = 0; i <= j; ++i) H[i * m + j] = hcol[i] + h2[i];
-            double wn2;
-            if (!dot(w, 1, w, &wn2)) return fail(NSFV1_ERR_DEVICE, "dot");
-            const double hn = std::sqrt(wn2);
+            double hh[2 * HS + 1];
+            if (!chk(launch_multi_dot(w, n, 1, w, n, part, hd + 2 * HS, s)) ||
+                hipMemcpyAsync(hh, hd, sizeof(hh), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess)
+                return fail(NSFV1_ERR_DEVICE, "dot");
+            for (int i = 0; i <= j; ++i) H[i * m + j] = hh[i] + hh[HS + i];
+            const double hn = std::sqrt(hh[2 * HS]);
             H[(j + 1) * m + j] = hn;
             if (hn > 0 && !chk(launch_axpby(1.0 / hn, w, 0.0, V + (int64_t)(j + 1) * n, n, s)))
                 return fail(NSFV1_ERR_DEVICE, "scale");
