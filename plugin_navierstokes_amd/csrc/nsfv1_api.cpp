@@ -88,7 +88,7 @@ struct MGLevel {
     DevBuf P_rp, P_ci, P_w;          // prolongation: rows = finer nodes, cols = this level's nodes
     DevBuf R_rp, R_ci, R_w;          // restriction = transpose of P
     DevBuf inj;                      // this level's node -> finer node (state injection)
-    DevBuf J, Binv, u, uold, b, x, r;
+    DevBuf J, Binv, u, uold, b, x, r, t;  // t: second iterate of the smoothing sweeps
     int jprec = 0;
 };
 
@@ -144,7 +144,7 @@ struct nsfv1_handle {
     bool mg_ready = false, use_mg = false;
     int mg_jprec = 0;
     int mg_ncoarse = 0;
-    DevBuf mg_r0, mg_dense, mg_gj;  // mg_gj: Gauss-Jordan tableaux of the coarsest level
+    DevBuf mg_r0, mg_t0, mg_dense, mg_gj;  // mg_gj: Gauss-Jordan tableaux of the coarsest level
     bool geo_valid = false;
     int geo_difflen = -1;
     // split assembly path (per-SCVF records)
@@ -1149,7 +1149,7 @@ int nsfv1_mg_set_hierarchy(nsfv1_handle* h, int nlev, nsfv1_handle* const* coars
         if (L->Binv.alloc(sizeof(double) * c->nv * (c->dim + 1) * (c->dim + 1)) != hipSuccess ||
             L->u.alloc(sizeof(double) * n) != hipSuccess || L->uold.alloc(sizeof(double) * n) != hipSuccess ||
             L->b.alloc(sizeof(double) * n) != hipSuccess || L->x.alloc(sizeof(double) * n) != hipSuccess ||
-            L->r.alloc(sizeof(double) * n) != hipSuccess)
+            L->r.alloc(sizeof(double) * n) != hipSuccess || L->t.alloc(sizeof(double) * n) != hipSuccess)
             return fail(NSFV1_ERR_DEVICE, "multigrid workspace");
         lv.push_back(std::move(L));
         nv_f = nv_c;
@@ -1158,7 +1158,7 @@ int nsfv1_mg_set_hierarchy(nsfv1_handle* h, int nlev, nsfv1_handle* const* coars
     if (ncoarse > MG_DENSE_MAX)
         return fail(NSFV1_ERR_ARGUMENT, "coarsest level has " + std::to_string(ncoarse) + " DoFs; at most " +
                                             std::to_string(MG_DENSE_MAX) + " are solved densely");
-    if (h->mg_r0.alloc(sizeof(double) * h->ndof) != hipSuccess ||
+    if (h->mg_r0.alloc(sizeof(double) * h->ndof) != hipSuccess || h->mg_t0.alloc(sizeof(double) * h->ndof) != hipSuccess ||
         h->mg_dense.alloc(sizeof(double) * ncoarse * ncoarse) != hipSuccess ||
         h->mg_gj.alloc(dense_inverse_work_bytes((int)ncoarse)) != hipSuccess)
         return fail(NSFV1_ERR_DEVICE, "multigrid workspace");
@@ -1254,15 +1254,22 @@ static hipError_t mg_cycle(nsfv1_handle* h, size_t l, const void* J0, const doub
     const void* J = fine ? J0 : h->mg[l - 1]->J.p;
     const double* Bi = fine ? Binv0 : h->mg[l - 1]->Binv.as<double>();
     double* r = fine ? h->mg_r0.as<double>() : h->mg[l - 1]->r.as<double>();
+    double* t = fine ? h->mg_t0.as<double>() : h->mg[l - 1]->t.as<double>();
     const int64_t* rp = c->rowptr.as<int64_t>();
     const int32_t* ci = c->colidx.as<int32_t>();
     const int32_t* ad = c->adj.as<int32_t>();
     const int64_t n = c->ndof, nv = c->nv;
     const double w = h->mg_omega;
-    hipError_t e = launch_bj_update(nc, Bi, b, x, w, 1, nv, s);
-    for (int k = 1; k < h->mg_pre && e == hipSuccess; ++k) {
-        e = launch_residual(jp, rp, ci, ad, nc, J, x, b, r, n, c->nnz, s);
-        if (e == hipSuccess) e = launch_bj_update(nc, Bi, r, x, w, 0, nv, s);
+    // pre-smoothing: x = w Binv b, then mg_pre - 1 sweeps alternating
+    // between x and t; the first iterate goes to t when that count is odd,
+    // so that the last one lands in x
+    const int pre_sweeps = std::max(h->mg_pre, 1) - 1;
+    double* cur = (pre_sweeps & 1) ? t : x;
+    double* alt = (pre_sweeps & 1) ? x : t;
+    hipError_t e = launch_bj_update(nc, Bi, b, cur, w, 1, nv, s);
+    for (int k = 0; k < pre_sweeps && e == hipSuccess; ++k) {
+        e = launch_smooth(jp, rp, ci, ad, nc, J, cur, b, Bi, w, r, alt, n, c->nnz, s);
+        std::swap(cur, alt);
     }
     if (e != hipSuccess) return e;
     MGLevel& C = *h->mg[l];
@@ -1277,10 +1284,12 @@ static hipError_t mg_cycle(nsfv1_handle* h, size_t l, const void* J0, const doub
     if (e == hipSuccess) e = launch_node_transfer(C.P_rp.as<int64_t>(), C.P_ci.as<int32_t>(), C.P_w.as<double>(), C.x.as<double>(), r, nv, nc, 0, s);
     if (e == hipSuccess) e = launch_zero_rows(c->dir_rows.as<int64_t>(), c->ndir, r, s);
     if (e == hipSuccess) e = launch_axpby(1.0, r, 1.0, x, n, s);
+    cur = x, alt = t;
     for (int k = 0; k < h->mg_post && e == hipSuccess; ++k) {
-        e = launch_residual(jp, rp, ci, ad, nc, J, x, b, r, n, c->nnz, s);
-        if (e == hipSuccess) e = launch_bj_update(nc, Bi, r, x, w, 0, nv, s);
+        e = launch_smooth(jp, rp, ci, ad, nc, J, cur, b, Bi, w, r, alt, n, c->nnz, s);
+        std::swap(cur, alt);
     }
+    if (e == hipSuccess && cur != x) e = hipMemcpyAsync(x, cur, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
     return e;
 }
 

@@ -224,7 +224,8 @@ __global__ void multi_add(const double* __restrict__ V, int64_t ld, int k, const
 // row when NC = 4), and keeps NC FP64 partial sums.  The G partials are then
 // combined by a reduce-scatter across the group (two exchange steps that
 // halve the set of rows a lane carries, then a butterfly), in a fixed order.
-// y = A x (RES = 0) or y = b - A x (RES = 1).
+// y = A x (RES = 0), y = b - A x (RES = 1), or one damped point-block
+// Jacobi sweep y = x + omega Binv (b - A x) (RES = 2; y must not alias x).
 template <typename JT, int NC>
 __device__ __forceinline__ void block_row(const JT* __restrict__ p, double (&q)[NC])
 {
@@ -247,7 +248,8 @@ __device__ __forceinline__ void block_row<double, 4>(const double* __restrict__ 
 template <typename JT, int NC, int G, int RES>
 __global__ void __launch_bounds__(256)
 nodeblock_mv(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ adj, const JT* __restrict__ A,
-             const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ y, int64_t nv)
+             const double* __restrict__ x, const double* __restrict__ b, double* __restrict__ y, int64_t nv,
+             const double* __restrict__ Binv, double omega)
 {
     static_assert(NC == 3 || NC == 4, "node blocks of 3 or 4 unknowns");
     static_assert(G >= 4 && G <= 64 && (G & (G - 1)) == 0, "group size");
@@ -295,7 +297,21 @@ nodeblock_mv(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ adj
 #pragma unroll
     for (int o = G / 8; o > 0; o >>= 1) z += __shfl_xor(z, o, G);
     const int a = (h1 ? 2 : 0) + (h2 ? 1 : 0);
-    if (v < nv && (sub & (G / 4 - 1)) == 0 && a < NC) {
+    if constexpr (RES == 2) {
+        // row c's sum sits on lane (c >> 1) G/2 + (c & 1) G/4 of the group
+        double rr[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) rr[c] = __shfl(z, (c >> 1) * (G / 2) + (c & 1) * (G / 4), G);
+        if (v < nv && (sub & (G / 4 - 1)) == 0 && a < NC) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) rr[c] = b[v * NC + c] - rr[c];
+            const int64_t row = v * NC + a;
+            double t = 0;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) t += Binv[row * NC + c] * rr[c];
+            y[row] = x[row] + omega * t;
+        }
+    } else if (v < nv && (sub & (G / 4 - 1)) == 0 && a < NC) {
         const int64_t row = v * NC + a;
         y[row] = RES ? b[row] - z : z;
     }
@@ -495,15 +511,15 @@ hipError_t launch_dirichlet(int jprec, const int64_t* rowptr, const int32_t* col
 // covers the mean number of neighbour blocks per node
 template <typename JT, int NC, int RES>
 static void launch_nodeblock(const int64_t* rowptr, const int32_t* adj, const JT* A, const double* x, const double* b,
-                             double* y, int64_t nv, int64_t nnz, hipStream_t s)
+                             double* y, int64_t nv, int64_t nnz, const double* Binv, double omega, hipStream_t s)
 {
     const int64_t slots = nv > 0 ? nnz / ((int64_t)NC * NC * nv) : 0;
     if (slots > 16)
-        nodeblock_mv<JT, NC, 32, RES><<<blocks_for(nv * 32, 256), 256, 0, s>>>(rowptr, adj, A, x, b, y, nv);
+        nodeblock_mv<JT, NC, 32, RES><<<blocks_for(nv * 32, 256), 256, 0, s>>>(rowptr, adj, A, x, b, y, nv, Binv, omega);
     else if (slots > 8)
-        nodeblock_mv<JT, NC, 16, RES><<<blocks_for(nv * 16, 256), 256, 0, s>>>(rowptr, adj, A, x, b, y, nv);
+        nodeblock_mv<JT, NC, 16, RES><<<blocks_for(nv * 16, 256), 256, 0, s>>>(rowptr, adj, A, x, b, y, nv, Binv, omega);
     else
-        nodeblock_mv<JT, NC, 8, RES><<<blocks_for(nv * 8, 256), 256, 0, s>>>(rowptr, adj, A, x, b, y, nv);
+        nodeblock_mv<JT, NC, 8, RES><<<blocks_for(nv * 8, 256), 256, 0, s>>>(rowptr, adj, A, x, b, y, nv, Binv, omega);
 }
 
 // with 4 unknowns per node the node-block kernels take 16-byte vector loads
@@ -515,15 +531,16 @@ static bool nodeblock_ok(const int32_t* adj, int nc, const void* A, const double
 
 template <int RES>
 static void launch_mv(int jprec, const int64_t* rowptr, const int32_t* adj, int nc, const void* A, const double* x,
-                      const double* b, double* y, int64_t n, int64_t nnz, hipStream_t s)
+                      const double* b, double* y, int64_t n, int64_t nnz, hipStream_t s,
+                      const double* Binv = nullptr, double omega = 0.0)
 {
     const int64_t nv = n / nc;
     if (nc == 4) {
-        if (jprec == 64) launch_nodeblock<double, 4, RES>(rowptr, adj, (const double*)A, x, b, y, nv, nnz, s);
-        else launch_nodeblock<float, 4, RES>(rowptr, adj, (const float*)A, x, b, y, nv, nnz, s);
+        if (jprec == 64) launch_nodeblock<double, 4, RES>(rowptr, adj, (const double*)A, x, b, y, nv, nnz, Binv, omega, s);
+        else launch_nodeblock<float, 4, RES>(rowptr, adj, (const float*)A, x, b, y, nv, nnz, Binv, omega, s);
     } else {
-        if (jprec == 64) launch_nodeblock<double, 3, RES>(rowptr, adj, (const double*)A, x, b, y, nv, nnz, s);
-        else launch_nodeblock<float, 3, RES>(rowptr, adj, (const float*)A, x, b, y, nv, nnz, s);
+        if (jprec == 64) launch_nodeblock<double, 3, RES>(rowptr, adj, (const double*)A, x, b, y, nv, nnz, Binv, omega, s);
+        else launch_nodeblock<float, 3, RES>(rowptr, adj, (const float*)A, x, b, y, nv, nnz, Binv, omega, s);
     }
 }
 
@@ -600,6 +617,20 @@ hipError_t launch_residual(int jprec, const int64_t* rowptr, const int32_t* coli
         else residual_csr<float><<<bl, 256, 0, s>>>(rowptr, colidx, (const float*)A, x, b, r, n);
     }
     return hipGetLastError();
+}
+
+hipError_t launch_smooth(int jprec, const int64_t* rowptr, const int32_t* colidx, const int32_t* adj, int nc,
+                         const void* A, const double* x, const double* b, const double* Binv, double omega, double* r,
+                         double* y, int64_t n, int64_t nnz, hipStream_t s)
+{
+    if (nodeblock_ok(adj, nc, A, x)) {
+        launch_mv<2>(jprec, rowptr, adj, nc, A, x, b, y, n, nnz, s, Binv, omega);
+        return hipGetLastError();
+    }
+    hipError_t e = launch_residual(jprec, rowptr, colidx, adj, nc, A, x, b, r, n, nnz, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(y, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = launch_bj_update(nc, Binv, r, y, omega, 0, n / nc, s);
+    return e;
 }
 
 hipError_t launch_bj_update(int nc, const double* Binv, const double* r, double* x, double omega, int first,

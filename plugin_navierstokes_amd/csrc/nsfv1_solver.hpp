@@ -29,6 +29,11 @@ hipError_t launch_axpby(double a, const double* x, double b, double* y, int64_t 
 hipError_t launch_residual(int jprec, const int64_t* rowptr, const int32_t* colidx, const int32_t* adj, int nc,
                            const void* A, const double* x, const double* b, double* r, int64_t n, int64_t nnz,
                            hipStream_t s);
+// one damped point-block Jacobi sweep y = x + omega Binv (b - A x) (y must not
+// alias x; r: scratch of n doubles, used only without node blocks)
+hipError_t launch_smooth(int jprec, const int64_t* rowptr, const int32_t* colidx, const int32_t* adj, int nc,
+                         const void* A, const double* x, const double* b, const double* Binv, double omega, double* r,
+                         double* y, int64_t n, int64_t nnz, hipStream_t s);
 hipError_t launch_bj_update(int nc, const double* Binv, const double* r, double* x, double omega, int first,
                             int64_t nv, hipStream_t s);
 hipError_t launch_node_transfer(const int64_t* rowptr, const int32_t* colidx, const double* w, const double* x,
