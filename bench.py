@@ -71,7 +71,11 @@ def timestep_leg(mesh, disc, steps: int, dt: float = 0.01):
 
     dofs, vals = cavity_dirichlet(mesh)
     levels = build_multigrid(disc, mesh, cavity_dirichlet)
-    nt = NewtonSolver(disc, dofs, vals, jprec=F32, rtol=1e-8, atol=0.0, maxit=30, lin_rtol=1e-3, lin_maxit=200,
+    # the Newton iteration contracts linearly (about 10x per iteration: the
+    # FV1 Jacobian omits the derivatives of the upwind and stabilisation
+    # weights), so correction solves to 3e-2 take as many Newton iterations
+    # as tighter ones and fewer GMRES iterations in total
+    nt = NewtonSolver(disc, dofs, vals, jprec=F32, rtol=1e-8, atol=0.0, maxit=30, lin_rtol=3e-2, lin_maxit=200,
                       restart=30, multigrid=True, freeze_upwind_after=3)
     stepper = BDFStepper(nt, dt, reference="old", mg_lag=True)
     u = torch.from_numpy(M.cavity_lid_state(mesh, 1.0, M.divergence_free_perturbation(mesh, 1e-3, seed=0))).cuda()
@@ -92,7 +96,7 @@ def timestep_leg(mesh, disc, steps: int, dt: float = 0.01):
               file=sys.stderr, flush=True)
     return {"method": "plain (linear extrapolation guess)", "s_per_step": float(np.mean(times)),
             "newton_per_step": float(np.mean(its)), "gmres_per_step": float(np.mean(lin)), "steps": steps,
-            "levels": len(levels) + 1, "residual": "1e-8 x ||d(u^n)||, FP64",
+            "levels": len(levels) + 1, "residual": "1e-8 x ||d(u^n)||, FP64", "correction_rtol": nt.lin_rtol,
             "upwind_decisions": "skewed corners decided anew for 2 Newton iterations, then replayed",
             "max_fresh_residual_ratio": float(np.max(fresh)),
             "workload": f"{mesh.shape[0]}^3 hex cavity, Re=400, skewed+FIELDS, BDF2 dt={dt}, FP32 J + FP64 defect"}
