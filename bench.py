@@ -175,6 +175,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timestep-steps", type=int, default=4, help="timed BDF2 steps of the time-stepping leg (0: skip)")
     ap.add_argument("--no-batched", dest="batched", action="store_false", help="skip the batched-instances leg")
+    ap.add_argument("--no-channel", dest="channel", action="store_false",
+                    help="skip the learned-vs-plain channel leg of the time stepping (profiling runs)")
     ap.add_argument("--launch", type=int, nargs=3, metavar=("PATH", "THREADS", "LDS"),
                     help="fixed assembly path / launch shape instead of the autotuner (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -279,7 +281,8 @@ def main():
         out["timestep"] = {"plain": timestep_leg(mesh, disc, args.timestep_steps)}
         del disc
         torch.cuda.empty_cache()
-        out["timestep"]["channel"] = learned_leg(args.timestep_steps)
+        if args.channel:
+            out["timestep"]["channel"] = learned_leg(args.timestep_steps)
     if args.batched:
         out["batched"] = batched_leg()
     if not args.no_cpu:

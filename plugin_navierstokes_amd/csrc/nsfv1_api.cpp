@@ -764,8 +764,11 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
             return fail(NSFV1_ERR_DEVICE, "coupled record buffer");
         e = launch_assemble_coupled(h->et, jprec, m, k, u, u_old, jvals, defect, h->errflag.as<int>(), P.nwg, P.lds,
                                     h->ne, h->cpl_recs.p, nullptr, (hipStream_t)stream);
-    } else if ((h->path == 0 || h->path == 2) && jprec == NSFV1_F32 && jvals && (parts & NSFV1_PART_A) &&
+    } else if ((h->path == 0 || h->path == 2) && jprec == NSFV1_F32 && (parts & NSFV1_PART_A) &&
                split_supported(h->et, k)) {
+        // also without a Jacobian (defect only): the gather then reads only
+        // the records' fluxes, and the defect is bitwise the one assembled
+        // together with the Jacobian
         const size_t need = sizeof(float) * (size_t)split_record_floats(h->et, h->ne);
         if (h->recs.n < need && h->recs.alloc(need) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "record buffer");
         e = launch_assemble_split(h->et, m, k, u, u_old, static_cast<float*>(jvals), defect, h->errflag.as<int>(), P.nwg,

@@ -223,3 +223,28 @@ def test_split_row_lanes(cuda, et, up):
     J2, d2, _, _ = res[2]
     assert np.max(np.abs(J2.astype(np.float64) - J0) / (2.0 ** -20 * (Ja + np.abs(J0)) + 1e-300)) <= 1.0
     assert fp64_violation(d2, d0, da) <= 1.0
+
+
+@pytest.mark.parametrize("path", [0, 2])
+@pytest.mark.parametrize("et", [M.QUAD, M.HEX], ids=lambda e: ET_NAMES[e])
+def test_split_defect_only(cuda, et, path):
+    """A defect-only FP32 assembly takes the split path too: its defect is
+    bitwise the one assembled together with the Jacobian, and matches the
+    oracle."""
+    import torch
+
+    mesh = small_mesh(et, n=6, seed=8)
+    cfg = {"upwind": "skewed"}
+    disc = NavierStokesFV1("u,v,p" if mesh.dim == 2 else "u,v,w,p", "Inner")
+    configure_disc(disc, cfg)
+    disc.set_mesh(mesh)
+    disc.set_assembly_path(path)
+    uh, uoh = M.random_state(mesh, 5), M.random_state(mesh, 6)
+    u, uo = torch.from_numpy(uh).to(cuda), torch.from_numpy(uoh).to(cuda)
+    _, d_with = disc.assemble(u, uo, 0.02, s_m=1.0, parts=PART_A | PART_M, jprec=F32)
+    _, d_only = disc.assemble(u, uo, 0.02, s_m=1.0, parts=PART_A | PART_M, jprec=F32, want_J=False)
+    assert torch.equal(d_with, d_only)
+    p = oracle_params(mesh.dim, cfg)
+    _, dr = O.assemble(p, et, mesh.xyz, mesh.conn, uh, uoh, 0.02, s_m=1.0, parts=PART_A | PART_M, want_J=False)
+    _, da = O.assemble_abs(p, et, mesh.xyz, mesh.conn, uh, uoh, 0.02, s_m=1.0, parts=PART_A | PART_M)
+    assert fp64_violation(d_only.cpu().numpy(), dr, da) <= 1.0, worst(d_only.cpu().numpy(), dr, da)
