@@ -93,9 +93,13 @@ __global__ void block_jacobi_setup(const int64_t* __restrict__ rowptr, const int
         for (int r = c + 1; r < NC; ++r)
             if (fabs(M[r][c]) > best) { best = fabs(M[r][c]); p = r; }
         if (best == 0.0) { ok = false; break; }
-        if (p != c)
+        // swap rows c and p with compile-time indices only, so that M
+        // stays in registers
 #pragma unroll
-            for (int k = 0; k < 2 * NC; ++k) { const double t = M[c][k]; M[c][k] = M[p][k]; M[p][k] = t; }
+        for (int r = c + 1; r < NC; ++r)
+            if (r == p)
+#pragma unroll
+                for (int k = 0; k < 2 * NC; ++k) { const double t = M[c][k]; M[c][k] = M[r][k]; M[r][k] = t; }
         const double ip = 1.0 / M[c][c];
 #pragma unroll
         for (int k = 0; k < 2 * NC; ++k) M[c][k] *= ip;
