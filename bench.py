@@ -70,7 +70,10 @@ def timestep_leg(mesh, disc, steps: int, dt: float = 0.01):
     from plugin_navierstokes_amd.solver import BDFStepper, NewtonSolver, build_multigrid, cavity_dirichlet
 
     dofs, vals = cavity_dirichlet(mesh)
-    levels = build_multigrid(disc, mesh, cavity_dirichlet)
+    # one undamped block-Jacobi sweep before and after the coarse correction:
+    # as a GMRES preconditioner this is cheaper per iteration than 2 + 2
+    # damped sweeps and takes few more iterations
+    levels = build_multigrid(disc, mesh, cavity_dirichlet, pre=1, post=1, omega=1.0)
     # the Newton iteration contracts linearly (about 10x per iteration: the
     # FV1 Jacobian omits the derivatives of the upwind and stabilisation
     # weights), so correction solves to 3e-2 take as many Newton iterations
@@ -97,6 +100,7 @@ def timestep_leg(mesh, disc, steps: int, dt: float = 0.01):
     return {"method": "plain (linear extrapolation guess)", "s_per_step": float(np.mean(times)),
             "newton_per_step": float(np.mean(its)), "gmres_per_step": float(np.mean(lin)), "steps": steps,
             "levels": len(levels) + 1, "residual": "1e-8 x ||d(u^n)||, FP64", "correction_rtol": nt.lin_rtol,
+            "smoothing": "1 + 1 block-Jacobi sweeps, omega 1.0",
             "upwind_decisions": "skewed corners decided anew for 2 Newton iterations, then replayed",
             "max_fresh_residual_ratio": float(np.max(fresh)),
             "workload": f"{mesh.shape[0]}^3 hex cavity, Re=400, skewed+FIELDS, BDF2 dt={dt}, FP32 J + FP64 defect"}
