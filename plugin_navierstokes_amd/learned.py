@@ -163,10 +163,11 @@ def load(path):
 # ---------------------------------------------------------------------
 # the channel workload: plain solver, trajectories, training
 # ---------------------------------------------------------------------
-def channel_case(seed: int, cells=(96, 48, 48), dt: float = 0.005, tune: bool = False):
+def channel_case(seed: int, cells=(96, 48, 48), dt: float = 0.005, tune: bool = False, smoothing=(2, 2, 0.7)):
     """Seeded channel: Re in [100, 500], 1-4 obstacle boxes.  Returns
     (mesh, disc, newton, stepper, u0, info).  tune: pick the assembly path
-    and launch shape of the fine level with the autotuner first."""
+    and launch shape of the fine level with the autotuner first;
+    smoothing: (pre, post, omega) of the multigrid's block-Jacobi sweeps."""
     import torch
 
     from . import F32, PART_A, PART_M
@@ -184,7 +185,7 @@ def channel_case(seed: int, cells=(96, 48, 48), dt: float = 0.005, tune: bool = 
     disc.set_kinematic_viscosity(1.0 / re)
     disc.set_mesh(mesh)
     dofs, vals = M.channel_dirichlet(mesh)
-    build_multigrid(disc, mesh, M.channel_dirichlet)
+    build_multigrid(disc, mesh, M.channel_dirichlet, pre=smoothing[0], post=smoothing[1], omega=smoothing[2])
     nt = NewtonSolver(disc, dofs, vals, jprec=F32, rtol=1e-8, atol=0.0, maxit=30, lin_rtol=1e-3, lin_maxit=200,
                       restart=30, multigrid=True)
     stepper = BDFStepper(nt, dt, reference="old", mg_lag=True)
@@ -304,7 +305,9 @@ def compare(weights, seeds=(100, 101), steps=8, cells=(96, 48, 48), log=print, w
     res = {"plain": [], "learned": [], "plain_newton": [], "learned_newton": [], "learned_choice": []}
     for sd in seeds:
         for mode in ("plain", "learned"):
-            case = channel_case(sd, cells, tune=True)
+            # 1 + 1 block-Jacobi sweeps at omega 0.9: the fastest measured on
+            # this channel (undamped sweeps take 1.5x as long here)
+            case = channel_case(sd, cells, tune=True, smoothing=(1, 1, 0.9))
             res["launch"] = case[5].get("launch")
             pred = LearnedPredictor(case[1], mlp, in_scale, out_scale) if mode == "learned" else None
             out = run_trajectory(case, steps, predictor=pred, log=None, warm=warm)
