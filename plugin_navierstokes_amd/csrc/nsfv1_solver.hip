@@ -367,7 +367,10 @@ __global__ void node_transfer(const int64_t* __restrict__ rowptr, const int32_t*
     const int64_t row = t / nc;
     const int c = (int)(t - row * nc);
     double s = 0.0;
-    for (int64_t k = rowptr[row]; k < rowptr[row + 1]; ++k) s += w[k] * x[(int64_t)colidx[k] * nc + c];
+    const int64_t a = rowptr[row], b = rowptr[row + 1];
+    // unrolled so that several index -> value load chains are in flight
+#pragma unroll 4
+    for (int64_t k = a; k < b; ++k) s += w[k] * x[(int64_t)colidx[k] * nc + c];
     y[t] = accumulate ? y[t] + s : s;
 }
 
