@@ -187,8 +187,8 @@ def main():
                     help="also run a plain 1 GiB device copy (rocprof PMC byte calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU only; the multi-GPU sharding of the assembly is "
-              "described in DESIGN.md and is not run.", file=sys.stderr)
+        print("bench.py: measures one GPU only; sharding the assembly over several GPUs is not "
+              "implemented, so --gpus must be 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch
