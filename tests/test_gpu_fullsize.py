@@ -1,0 +1,126 @@
+"""The 128^3 hexahedral lid-driven cavity of the benchmark at its full size
+(2,097,152 elements, nnz 913,066,000), skewed upwind + FIELDS, time-dependent
+stabilisation (u_old, dt = 0.01), FP32 Jacobian + FP64 defect, checked
+through properties that do not need a full oracle assembly:
+
+- sampled rows against the oracle: every element around 300 seeded nodes
+  (interior, face, edge and corner nodes) is assembled by the oracle, which
+  makes those nodes' rows complete. The GPU defect rows must match to the
+  FP64 tolerance, |a-b| <= 1e-12|b| + 1e-13 S. The GPU row products J x (FP32
+  J, FP64 x) must match the oracle rows times x to the FP32 entry tolerance
+  summed over the row: sum_k (2^-17 S_k + 2^-20 |J_k|) |x_k|.
+- conservation: each SCVF flux enters its two nodes' rows with opposite
+  signs, and with no boundary discs nothing else enters the A part. So the
+  sum over all nodes of each defect component vanishes up to FP64 rounding,
+  and so does each component's row sum of J x for any x (the Jacobian's
+  column sums per component vanish) up to FP32 rounding.
+- bitwise repeatability of the whole assembly."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+from plugin_navierstokes_amd import F32, NavierStokesFV1, PART_A
+from plugin_navierstokes_amd import meshes as M
+from tests._parity import FP32_ABS, FP32_REL, fp64_violation, worst
+
+pytestmark = pytest.mark.gpu
+
+N = 128
+DT = 0.01
+
+
+@pytest.fixture(scope="module")
+def cavity(cuda):
+    import torch
+
+    mesh = M.cavity_3d(N, M.HEX)
+    u = M.cavity_lid_state(mesh, 1.0, base=M.divergence_free_perturbation(mesh, 1e-3, seed=0))
+    uold = 0.999 * u
+    disc = NavierStokesFV1("u,v,w,p", "Inner")
+    disc.set_stabilization("fields")
+    disc.set_upwind("skewed")
+    disc.set_kinematic_viscosity(1.0 / 400.0)
+    disc.set_mesh(mesh)
+    ut, uot = torch.from_numpy(u).to(cuda), torch.from_numpy(uold).to(cuda)
+    print(f"full-size cavity {N}^3: ne={mesh.ne} nnz={disc.nnz()}", flush=True)
+    J, d = disc.assemble(ut, uot, DT, parts=PART_A, jprec=F32)
+    return mesh, disc, u, uold, ut, uot, J, d
+
+
+def _sample_nodes(mesh, count=300, seed=5):
+    rng = np.random.default_rng(seed)
+    n1 = N + 1
+    picks = set(rng.choice(mesh.nv, size=count, replace=False).tolist())
+    # corners, edge midpoints and face centres of the unit cube
+    for i, j, k in [(0, 0, 0), (N, N, N), (N // 2, 0, 0), (0, N // 2, N), (N // 2, N // 2, 0), (N, N // 2, N // 2),
+                    (N // 2, N, N // 2)]:
+        for v in np.flatnonzero((np.abs(mesh.xyz - np.array([i, j, k]) / N) < 0.25 / N).all(axis=1)):
+            picks.add(int(v))
+    assert n1 ** 3 == mesh.nv
+    return np.array(sorted(picks), dtype=np.int64)
+
+
+def test_fullsize_sampled_rows_vs_oracle(cuda, cavity):
+    import torch
+
+    mesh, disc, u, uold, _, _, J, d = cavity
+    nodes = _sample_nodes(mesh)
+    conn_sub = mesh.conn[np.isin(mesh.conn, nodes).any(axis=1)]
+    p = O.Params(dim=3, upwind="skewed", nu=1.0 / 400.0)
+    rp, ci = O.pattern(O.HEX, mesh.nv, conn_sub)
+    Jr, dr = O.assemble(p, O.HEX, mesh.xyz, conn_sub, u, uold, DT, parts=PART_A, rowptr=rp, colidx=ci)
+    Ja, da = O.assemble_abs(p, O.HEX, mesh.xyz, conn_sub, u, uold, DT, parts=PART_A, rowptr=rp, colidx=ci)
+    rows = (nodes[:, None] * 4 + np.arange(4)[None, :]).ravel()
+    rows_t = torch.from_numpy(rows).to(cuda)
+
+    dg = d[rows_t].cpu().numpy()
+    assert fp64_violation(dg, dr[rows], da[rows]) <= 1.0, worst(dg, dr[rows], da[rows])
+
+    gen = torch.Generator(cuda).manual_seed(17)
+    x = torch.randn(disc.num_dofs(), dtype=torch.float64, device=cuda, generator=gen)
+    y = torch.empty_like(x)
+    disc.spmv(J, x, y, jprec=F32)
+    yg = y[rows_t].cpu().numpy()
+    xh = x.cpu().numpy()
+    yr = np.empty(len(rows))
+    tol = np.empty(len(rows))
+    for i, r in enumerate(rows):
+        k0, k1 = rp[r], rp[r + 1]
+        xs = xh[ci[k0:k1]]
+        yr[i] = np.dot(Jr[k0:k1], xs)
+        tol[i] = np.dot(FP32_ABS * Ja[k0:k1] + FP32_REL * np.abs(Jr[k0:k1]), np.abs(xs)) + 1e-300
+    v = np.abs(yg - yr) / tol
+    i = int(np.argmax(v))
+    assert v[i] <= 1.0, f"row {rows[i]}: got {yg[i]!r} want {yr[i]!r} tol {tol[i]:.3g}"
+
+
+def test_fullsize_conservation(cuda, cavity):
+    import torch
+
+    _, disc, _, _, _, _, J, d = cavity
+    dd = d.view(-1, 4)
+    s = dd.sum(dim=0).cpu().numpy()
+    a = dd.abs().sum(dim=0).cpu().numpy()
+    print(f"defect component sums {s} of {a}", flush=True)
+    assert np.all(np.abs(s) <= 1e-12 * a)
+
+    gen = torch.Generator(cuda).manual_seed(23)
+    x = torch.randn(disc.num_dofs(), dtype=torch.float64, device=cuda, generator=gen)
+    y = torch.empty_like(x)
+    disc.spmv(J, x, y, jprec=F32)
+    z = torch.empty_like(x)
+    disc.spmv(J.abs(), x.abs(), z, jprec=F32)
+    s = y.view(-1, 4).sum(dim=0).cpu().numpy()
+    a = z.view(-1, 4).sum(dim=0).cpu().numpy()
+    print(f"row sums of J x {s} of {a}", flush=True)
+    assert np.all(np.abs(s) <= 1e-6 * a)
+
+
+def test_fullsize_repeatable(cuda, cavity):
+    import torch
+
+    _, disc, _, _, ut, uot, J, d = cavity
+    J2, d2 = disc.assemble(ut, uot, DT, parts=PART_A, jprec=F32)
+    assert torch.equal(J, J2) and torch.equal(d, d2)
