@@ -84,3 +84,24 @@ def test_autotune_keeps_results(cuda):
     scale = J0.abs().max()
     assert float((J1 - J0).abs().max()) <= 1e-6 * float(scale)
     assert float((d1 - d0).abs().max()) <= 1e-12 * float(d0.abs().max())
+
+
+def test_autotune_skips_shapes_a_mesh_cannot_use(cuda):
+    """A tetrahedral node has more incidences than the 16 that one 64-lane
+    workgroup of the row-lane gather holds: the autotuner skips that shape
+    (the library refuses it with NSFV1_ERR_ARGUMENT) and keeps a usable one."""
+    import torch
+
+    mesh = M.cavity_3d(4, M.TET, 0.05, seed=2)
+    disc = NavierStokesFV1("u,v,w,p", "Inner")
+    disc.set_stabilization("fields")
+    disc.set_upwind("lps")
+    disc.set_kinematic_viscosity(0.01)
+    disc.set_mesh(mesh)
+    u = torch.from_numpy(M.random_state(mesh, 3)).to(cuda)
+    run = lambda: disc.assemble(u, jprec=F32)  # noqa: E731
+    best, table = autotune(disc, run, candidates=[(2, 64, 16384), (0, 64, 16384), (0, 128, 32768)], reps=1)
+    assert [r[:3] for r in table] == [(0, 64, 16384), (0, 128, 32768)]
+    assert best in table
+    J, d = run()
+    assert torch.isfinite(J).all() and torch.isfinite(d).all()
