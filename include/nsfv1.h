@@ -297,8 +297,9 @@ int nsfv1_mg_set_hierarchy(nsfv1_handle* h, int nlev, nsfv1_handle* const* coars
 /* Re-assembles every coarse level's Jacobian at the injected state (same
    time-series, scalings and parts as the fine Jacobian), applies the coarse
    Dirichlet rows, factors the point blocks and inverts the coarsest level on
-   the host.  Call after each fine assembly whose Jacobian the next
-   nsfv1_linear_solve uses.  Synchronises the stream. */
+   the device (Gauss-Jordan, one launch per pivot).  Call after each fine
+   assembly whose Jacobian the next nsfv1_linear_solve uses.  Synchronises
+   the stream once, to report a singular coarsest level. */
 int nsfv1_mg_setup(nsfv1_handle* h, int jprec, const double* u, const double* u_old, double dt, double s_a,
                    double s_m, int parts, void* stream);
 
