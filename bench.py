@@ -232,8 +232,9 @@ def main():
         disc.set_launch(args.launch[1], args.launch[2])
         launch = {"path": args.launch[0], "threads": args.launch[1], "lds_bytes": args.launch[2], "autotuned": False}
     else:
-        best, _ = autotune(disc, step, reps=2)
-        launch = {"path": best[0], "threads": best[1], "lds_bytes": best[2], "autotuned": True}
+        best, table = autotune(disc, step, reps=2)
+        launch = {"path": best[0], "threads": best[1], "lds_bytes": best[2], "autotuned": True,
+                  "candidates_ms": [[p, t, l, round(m, 3)] for p, t, l, m in table]}
     for _ in range(args.warmup):
         step()
     disc.sync_errors(stream)
