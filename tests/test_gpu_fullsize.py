@@ -1,10 +1,15 @@
-"""The 128^3 hexahedral lid-driven cavity of the benchmark at its full size
-(2,097,152 elements, nnz 913,066,000), skewed upwind + FIELDS, time-dependent
-stabilisation (u_old, dt = 0.01), FP32 Jacobian + FP64 defect, checked
-through properties that do not need a full oracle assembly:
+"""Full-size workloads, FP32 Jacobian + FP64 defect, time-dependent
+stabilisation (u_old, dt = 0.01), checked through properties that do not
+need a full oracle assembly:
+- "cavity": the 128^3 hexahedral lid-driven cavity of the benchmark
+  (2,097,152 elements, nnz 913,066,000), skewed upwind + FIELDS (split path);
+- "channel": a 96x48x48 x 6-tetrahedra obstacle channel like the learned
+  predictor's (two obstacle boxes, 1,308,672 elements), LPS upwind + FIELDS
+  (fused path), at a perturbed parabolic state.
 
 - sampled rows against the oracle: every element around 300 seeded nodes
-  (interior, face, edge and corner nodes) is assembled by the oracle, which
+  (plus corner, edge and face nodes of the cavity, and inflow, outflow, wall
+  and obstacle nodes of the channel) is assembled by the oracle, which
   makes those nodes' rows complete. The GPU defect rows must match to the
   FP64 tolerance, |a-b| <= 1e-12|b| + 1e-13 S. The GPU row products J x (FP32
   J, FP64 x) must match the oracle rows times x to the FP32 entry tolerance
@@ -31,47 +36,58 @@ N = 128
 DT = 0.01
 
 
-@pytest.fixture(scope="module")
-def cavity(cuda):
+@pytest.fixture(scope="module", params=["cavity", "channel"])
+def workload(request, cuda):
     import torch
 
-    mesh = M.cavity_3d(N, M.HEX)
-    u = M.cavity_lid_state(mesh, 1.0, base=M.divergence_free_perturbation(mesh, 1e-3, seed=0))
+    if request.param == "cavity":
+        et, up, nu = M.HEX, "skewed", 1.0 / 400.0
+        mesh = M.cavity_3d(N, M.HEX)
+        u = M.cavity_lid_state(mesh, 1.0, base=M.divergence_free_perturbation(mesh, 1e-3, seed=0))
+    else:
+        et, up, nu = M.TET, "lps", 1.0 / 300.0
+        mesh = M.channel_3d((96, 48, 48), n_boxes=2, seed=100)
+        u = M.channel_initial_state(mesh, M.channel_dirichlet(mesh)) + 0.05 * M.random_state(mesh, 7)
     uold = 0.999 * u
     disc = NavierStokesFV1("u,v,w,p", "Inner")
     disc.set_stabilization("fields")
-    disc.set_upwind("skewed")
-    disc.set_kinematic_viscosity(1.0 / 400.0)
+    disc.set_upwind(up)
+    disc.set_kinematic_viscosity(nu)
     disc.set_mesh(mesh)
     ut, uot = torch.from_numpy(u).to(cuda), torch.from_numpy(uold).to(cuda)
-    print(f"full-size cavity {N}^3: ne={mesh.ne} nnz={disc.nnz()}", flush=True)
+    print(f"full-size {request.param}: ne={mesh.ne} nnz={disc.nnz()}", flush=True)
     J, d = disc.assemble(ut, uot, DT, parts=PART_A, jprec=F32)
-    return mesh, disc, u, uold, ut, uot, J, d
+    params = O.Params(dim=3, upwind=up, nu=nu)
+    return request.param, et, params, mesh, disc, u, uold, ut, uot, J, d
 
 
-def _sample_nodes(mesh, count=300, seed=5):
+def _sample_nodes(name, mesh, count=300, seed=5):
     rng = np.random.default_rng(seed)
-    n1 = N + 1
-    picks = set(rng.choice(mesh.nv, size=count, replace=False).tolist())
-    # corners, edge midpoints and face centres of the unit cube
-    for i, j, k in [(0, 0, 0), (N, N, N), (N // 2, 0, 0), (0, N // 2, N), (N // 2, N // 2, 0), (N, N // 2, N // 2),
-                    (N // 2, N, N // 2)]:
-        for v in np.flatnonzero((np.abs(mesh.xyz - np.array([i, j, k]) / N) < 0.25 / N).all(axis=1)):
-            picks.add(int(v))
-    assert n1 ** 3 == mesh.nv
+    used = np.setdiff1d(np.arange(mesh.nv), mesh.boundary.get("unused", np.zeros(0, np.int64)))
+    picks = set(rng.choice(used, size=count, replace=False).tolist())
+    if name == "cavity":
+        # corners, edge midpoints and face centres of the unit cube
+        assert (N + 1) ** 3 == mesh.nv
+        for i, j, k in [(0, 0, 0), (N, N, N), (N // 2, 0, 0), (0, N // 2, N), (N // 2, N // 2, 0),
+                        (N, N // 2, N // 2), (N // 2, N, N // 2)]:
+            for v in np.flatnonzero((np.abs(mesh.xyz - np.array([i, j, k]) / N) < 0.25 / N).all(axis=1)):
+                picks.add(int(v))
+    else:
+        for group in ("inflow", "outflow", "wall", "obstacle"):
+            nodes = mesh.boundary[group]
+            picks.update(rng.choice(nodes, size=min(8, len(nodes)), replace=False).tolist())
     return np.array(sorted(picks), dtype=np.int64)
 
 
-def test_fullsize_sampled_rows_vs_oracle(cuda, cavity):
+def test_fullsize_sampled_rows_vs_oracle(cuda, workload):
     import torch
 
-    mesh, disc, u, uold, _, _, J, d = cavity
-    nodes = _sample_nodes(mesh)
+    name, et, p, mesh, disc, u, uold, _, _, J, d = workload
+    nodes = _sample_nodes(name, mesh)
     conn_sub = mesh.conn[np.isin(mesh.conn, nodes).any(axis=1)]
-    p = O.Params(dim=3, upwind="skewed", nu=1.0 / 400.0)
-    rp, ci = O.pattern(O.HEX, mesh.nv, conn_sub)
-    Jr, dr = O.assemble(p, O.HEX, mesh.xyz, conn_sub, u, uold, DT, parts=PART_A, rowptr=rp, colidx=ci)
-    Ja, da = O.assemble_abs(p, O.HEX, mesh.xyz, conn_sub, u, uold, DT, parts=PART_A, rowptr=rp, colidx=ci)
+    rp, ci = O.pattern(et, mesh.nv, conn_sub)
+    Jr, dr = O.assemble(p, et, mesh.xyz, conn_sub, u, uold, DT, parts=PART_A, rowptr=rp, colidx=ci)
+    Ja, da = O.assemble_abs(p, et, mesh.xyz, conn_sub, u, uold, DT, parts=PART_A, rowptr=rp, colidx=ci)
     rows = (nodes[:, None] * 4 + np.arange(4)[None, :]).ravel()
     rows_t = torch.from_numpy(rows).to(cuda)
 
@@ -96,10 +112,10 @@ def test_fullsize_sampled_rows_vs_oracle(cuda, cavity):
     assert v[i] <= 1.0, f"row {rows[i]}: got {yg[i]!r} want {yr[i]!r} tol {tol[i]:.3g}"
 
 
-def test_fullsize_conservation(cuda, cavity):
+def test_fullsize_conservation(cuda, workload):
     import torch
 
-    _, disc, _, _, _, _, J, d = cavity
+    disc, J, d = workload[4], workload[9], workload[10]
     dd = d.view(-1, 4)
     s = dd.sum(dim=0).cpu().numpy()
     a = dd.abs().sum(dim=0).cpu().numpy()
@@ -118,9 +134,9 @@ def test_fullsize_conservation(cuda, cavity):
     assert np.all(np.abs(s) <= 1e-6 * a)
 
 
-def test_fullsize_repeatable(cuda, cavity):
+def test_fullsize_repeatable(cuda, workload):
     import torch
 
-    _, disc, _, _, ut, uot, J, d = cavity
+    disc, ut, uot, J, d = workload[4], workload[7], workload[8], workload[9], workload[10]
     J2, d2 = disc.assemble(ut, uot, DT, parts=PART_A, jprec=F32)
     assert torch.equal(J, J2) and torch.equal(d, d2)
