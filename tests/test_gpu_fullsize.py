@@ -5,7 +5,9 @@ need a full oracle assembly:
   (2,097,152 elements, nnz 913,066,000), skewed upwind + FIELDS (split path);
 - "channel": a 96x48x48 x 6-tetrahedra obstacle channel like the learned
   predictor's (two obstacle boxes, 1,308,672 elements), LPS upwind + FIELDS
-  (fused path), at a perturbed parabolic state.
+  (fused path), at a perturbed parabolic state;
+- the batched leg's union of 256 cavities (3,958,533 quadrilaterals, full
+  upwind + FIELDS, per-instance viscosity), steady.
 
 - sampled rows against the oracle: every element around 300 seeded nodes
   (plus corner, edge and face nodes of the cavity, and inflow, outflow, wall
@@ -140,3 +142,71 @@ def test_fullsize_repeatable(cuda, workload):
     disc, ut, uot, J, d = workload[4], workload[7], workload[8], workload[9], workload[10]
     J2, d2 = disc.assemble(ut, uot, DT, parts=PART_A, jprec=F32)
     assert torch.equal(J, J2) and torch.equal(d, d2)
+
+
+@pytest.fixture(scope="module")
+def batch(cuda):
+    from plugin_navierstokes_amd.batch import BatchedCavities, draw_instances
+
+    b = BatchedCavities(draw_instances(256, seed=0))
+    u = b.state(seed=1)
+    print(f"full-size batch: 256 cavities, ne={b.elements} nnz={b.disc.nnz()}", flush=True)
+    J, d = b.disc.assemble(u, jprec=F32)
+    return b, u, J, d
+
+
+def test_fullsize_batch_sampled_rows_vs_oracle(cuda, batch):
+    """The 256-cavity union (one launch, per-instance viscosity): rows of 48
+    seeded nodes, each checked against the oracle run on its own instance's
+    elements with that instance's viscosity."""
+    import torch
+
+    b, u, J, d = batch
+    mesh = b.mesh
+    uh = u.cpu().numpy()
+    rng = np.random.default_rng(9)
+    nodes = np.sort(rng.choice(mesh.nv, size=48, replace=False))
+    inst = np.searchsorted(b.noff, nodes, side="right") - 1
+    gen = torch.Generator(cuda).manual_seed(31)
+    x = torch.randn(b.disc.num_dofs(), dtype=torch.float64, device=cuda, generator=gen)
+    y = torch.empty_like(x)
+    b.disc.spmv(J, x, y, jprec=F32)
+    xh, yh, dh = x.cpu().numpy(), y.cpu().numpy(), d.cpu().numpy()
+    for k in np.unique(inst):
+        vs = nodes[inst == k]
+        e0, e1 = b.eoff[k], b.eoff[k + 1]
+        conn_k = mesh.conn[e0:e1]
+        conn_sub = conn_k[np.isin(conn_k, vs).any(axis=1)]
+        p = O.Params(dim=2, upwind="full", nu=b.instances[k].nu)
+        rp, ci = O.pattern(O.QUAD, mesh.nv, conn_sub)
+        Jr, dr = O.assemble(p, O.QUAD, mesh.xyz, conn_sub, uh, parts=PART_A, rowptr=rp, colidx=ci)
+        Ja, da = O.assemble_abs(p, O.QUAD, mesh.xyz, conn_sub, uh, parts=PART_A, rowptr=rp, colidx=ci)
+        rows = (vs[:, None] * 3 + np.arange(3)[None, :]).ravel()
+        assert fp64_violation(dh[rows], dr[rows], da[rows]) <= 1.0, (k, worst(dh[rows], dr[rows], da[rows]))
+        for r in rows:
+            k0, k1 = rp[r], rp[r + 1]
+            xs = xh[ci[k0:k1]]
+            want = np.dot(Jr[k0:k1], xs)
+            tol = np.dot(FP32_ABS * Ja[k0:k1] + FP32_REL * np.abs(Jr[k0:k1]), np.abs(xs)) + 1e-300
+            assert abs(yh[r] - want) <= tol, (k, r, yh[r], want, tol)
+
+
+def test_fullsize_batch_conservation(cuda, batch):
+    import torch
+
+    b, _, J, d = batch
+    dd = d.view(-1, 3)
+    s = dd.sum(dim=0).cpu().numpy()
+    a = dd.abs().sum(dim=0).cpu().numpy()
+    print(f"defect component sums {s} of {a}", flush=True)
+    assert np.all(np.abs(s) <= 1e-12 * a)
+    gen = torch.Generator(cuda).manual_seed(37)
+    x = torch.randn(b.disc.num_dofs(), dtype=torch.float64, device=cuda, generator=gen)
+    y = torch.empty_like(x)
+    b.disc.spmv(J, x, y, jprec=F32)
+    z = torch.empty_like(x)
+    b.disc.spmv(J.abs(), x.abs(), z, jprec=F32)
+    s = y.view(-1, 3).sum(dim=0).cpu().numpy()
+    a = z.view(-1, 3).sum(dim=0).cpu().numpy()
+    print(f"row sums of J x {s} of {a}", flush=True)
+    assert np.all(np.abs(s) <= 1e-6 * a)
