@@ -4,8 +4,9 @@ Workload: 3D lid-driven cavity on the unit cube, 128^3 Q1 hexahedra
 (2,097,152 elements, 2,146,689 nodes), Re = 400 (nu = 2.5e-3), skewed upwind
 + FIELDS stabilisation (RAW diffusion length), time-dependent stabilisation
 with the previous time level (dt = 0.01), FP32 Jacobian + FP64 defect.  One
-step = one whole-mesh Jacobian + defect assembly with the state resident in
-HBM.  Prints one JSON line.
+step = one whole-mesh Jacobian + defect assembly of a BDF2 Newton iterate:
+stiffness/convection part plus the mass part weighted 1.5/dt, with the state
+resident in HBM.  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--no-cpu]
 """
@@ -32,29 +33,92 @@ def algorithmic_bytes(nv, ne, nnz, dim, nsh, jbytes, T):
     return nv * dim * 8 + T * nv * (dim + 1) * 8 + ne * nsh * 4 + nnz * jbytes + nv * (dim + 1) * 8
 
 
-def cpu_baseline(n_full: int, target_s: float = 10.0):
-    """Time the CPU oracle (1 thread) on a sub-block of the same cavity: the
-    same element shape, state construction and configuration."""
+DT = 0.01
+S_M = 1.5 / DT  # BDF2 weight of the mass part
+
+
+def host_cores() -> int:
+    """Host cores this process may use, capped at 16 (one GPU's share of the
+    host on the GPU machines)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(m: int = 48, repeats: int = 5):
+    """The CPU oracle (C, -O3 -ffp-contract=off, FP64, scalar CSR scatter) on
+    an m^3 sub-cavity of the same workload (same element size class, state
+    construction and configuration, A + M parts): 1 thread, as UG4 assembles
+    within a process, and every host core with the element-coloured OpenMP
+    restatement (standing in for UG4's one MPI rank per core).  Median of
+    `repeats` runs after one warm-up run each."""
     import oracle as O
     from plugin_navierstokes_amd import meshes as M
 
-    def run(m):
-        mesh = M.cavity_3d(m, M.HEX)
-        u = M.cavity_lid_state(mesh, 1.0, base=M.divergence_free_perturbation(mesh, 1e-3, seed=0))
-        uold = 0.999 * u
-        p = O.Params(dim=3, upwind="skewed", nu=1.0 / 400.0)
-        rp, ci = O.pattern(O.HEX, mesh.nv, mesh.conn)
-        t0 = time.perf_counter()
-        O.assemble(p, O.HEX, mesh.xyz, mesh.conn, u, uold, 0.01, parts=1, rowptr=rp, colidx=ci)
-        return time.perf_counter() - t0, mesh.ne
+    mesh = M.cavity_3d(m, M.HEX)
+    u = M.cavity_lid_state(mesh, 1.0, base=M.divergence_free_perturbation(mesh, 1e-3, seed=0))
+    uold = 0.999 * u
+    p = O.Params(dim=3, upwind="skewed", nu=1.0 / 400.0)
+    rp, ci = O.pattern(O.HEX, mesh.nv, mesh.conn)
+    order, cptr = O.structured_colors(mesh.shape)
+    cores = host_cores()
 
-    t, ne = run(8)
-    rate = ne / t
-    m = int(max(8, min(64, round((target_s * rate) ** (1.0 / 3.0)))))
-    t, ne = run(m)
-    return {"value": ne / t, "unit": "elem/s", "cores": 1, "kind": "port",
-            "sample": f"CPU oracle (C, -O2, 1 thread), full FP64 Jacobian+defect assembly of a {m}^3-hexahedron "
-                      f"cavity ({ne} elements, same element size class, state and configuration), {t:.2f} s"}
+    def timed(fn):
+        fn()
+        ts = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    t1 = timed(lambda: O.assemble(p, O.HEX, mesh.xyz, mesh.conn, u, uold, DT, s_m=S_M, parts=3, rowptr=rp,
+                                  colidx=ci))
+    tn = timed(lambda: O.assemble_colored(p, O.HEX, mesh.xyz, mesh.conn, u, order, cptr, uold, DT, s_m=S_M, parts=3,
+                                          rowptr=rp, colidx=ci, threads=cores))
+    sample = (f"CPU oracle (C, -O3 -ffp-contract=off, FP64), Jacobian+defect assembly (A + M parts) of a {m}^3-hexahedron "
+              f"cavity ({mesh.ne} elements, same element size class, state and configuration); median of {repeats} "
+              f"runs after a warm-up")
+    return {"value": mesh.ne / t1, "unit": "elem/s", "cores": 1, "kind": "port", "sample": sample,
+            "seconds_per_run": t1,
+            "all_cores": {"value": mesh.ne / tn, "unit": "elem/s", "cores": cores, "seconds_per_run": tn,
+                          "method": "element colouring (8 colours), OpenMP over the elements of a colour"}}
+
+
+def parity_sample(mesh, u, uold, J, d, rowptr_full, nodes=64, seed=5):
+    """Every Jacobian and defect entry of the rows of `nodes` seeded nodes
+    against the FP64 oracle (all elements around them assembled): the
+    largest per-entry error relative to the entry's scale S (sum of the
+    magnitudes of its element contributions) and the largest ratio to the
+    stated bound (<= 1 passes)."""
+    import torch
+
+    import oracle as O
+
+    rng = np.random.default_rng(seed)
+    vs = np.sort(rng.choice(mesh.nv, size=nodes, replace=False))
+    conn_sub = mesh.conn[np.isin(mesh.conn, vs).any(axis=1)]
+    p = O.Params(dim=3, upwind="skewed", nu=1.0 / 400.0)
+    rp, ci = O.pattern(O.HEX, mesh.nv, conn_sub)
+    Jr, dr = O.assemble(p, O.HEX, mesh.xyz, conn_sub, u, uold, DT, s_m=S_M, parts=3, rowptr=rp, colidx=ci)
+    Ja, da = O.assemble_abs(p, O.HEX, mesh.xyz, conn_sub, u, uold, DT, s_m=S_M, parts=3, rowptr=rp, colidx=ci)
+    rows = (vs[:, None] * 4 + np.arange(4)[None, :]).ravel()
+    idx = np.concatenate([np.arange(rowptr_full[r], rowptr_full[r + 1]) for r in rows])
+    idx_o = np.concatenate([np.arange(rp[r], rp[r + 1]) for r in rows])
+    Jg = J[torch.from_numpy(idx).to(J.device)].cpu().numpy().astype(np.float64)
+    dg = d[torch.from_numpy(rows).to(d.device)].cpu().numpy()
+    Jo, So, do, Sd = Jr[idx_o], Ja[idx_o], dr[rows], da[rows]
+    ej = np.abs(Jg - Jo)
+    ed = np.abs(dg - do)
+    return {"rows": int(len(rows)), "entries_J": int(len(idx)),
+            "max_err_J": float(np.max(ej / np.maximum(So, 1e-300))),
+            "max_err_d": float(np.max(ed / np.maximum(Sd, 1e-300))),
+            "bound": "J (FP32): |a-b| <= 2^-17 S + 2^-20 |b|; d (FP64): |a-b| <= 1e-12 |b| + 1e-13 S",
+            "max_violation_J": float(np.max(ej / (2.0 ** -17 * So + 2.0 ** -20 * np.abs(Jo) + 1e-300))),
+            "max_violation_d": float(np.max(ed / (1e-12 * np.abs(do) + 1e-13 * Sd + 1e-300))),
+            "reference": "FP64 CPU oracle, same state"}
 
 
 def timestep_leg(mesh, disc, steps: int, dt: float = 0.01):
@@ -102,6 +166,9 @@ def timestep_leg(mesh, disc, steps: int, dt: float = 0.01):
             "levels": len(levels) + 1, "residual": "1e-8 x ||d(u^n)||, FP64", "correction_rtol": nt.lin_rtol,
             "smoothing": "1 + 1 block-Jacobi sweeps, omega 1.0",
             "upwind_decisions": "skewed corners decided anew for 2 Newton iterations, then replayed",
+            "residual_kind": ("1e-8 is reached on the frozen-decision system (the skewed corner choices of the third "
+                              "iterate replayed); with the corners decided anew at the converged state the defect "
+                              "is max_fresh_residual_ratio of the reference norm"),
             "max_fresh_residual_ratio": float(np.max(fresh)),
             "workload": f"{mesh.shape[0]}^3 hex cavity, Re=400, skewed+FIELDS, BDF2 dt={dt}, FP32 J + FP64 defect"}
 
@@ -110,7 +177,8 @@ def learned_leg(steps: int):
     """Plain vs learned initial guess on a held-out 96x48x48x6-tetrahedra
     channel (Re, obstacles and seed not in the training set): BDF2 dt=0.005,
     LPS + FIELDS, FP32 J + FP64 defect, 1e-8 relative FP64 residual; 6
-    untimed warm-up steps, then `steps` timed steps per method."""
+    untimed warm-up steps, then `steps` timed steps per method.  The channel
+    assembly's roofline uses its autotuned launch time."""
     from plugin_navierstokes_amd.learned import compare
 
     w = os.path.join(ROOT, "plugin_navierstokes_amd", "data", "predictor_channel.npz")
@@ -118,6 +186,14 @@ def learned_leg(steps: int):
     r["workload"] = ("channel 96x48x48 hexahedra x 6 tetrahedra (1.26M elements after obstacles), held-out seed 100, "
                      "BDF2 dt=0.005, LPS+FIELDS, FP32 J + FP64 defect, residual 1e-8 x ||d(u^n)||")
     r["learned_vs_plain"] = r["learned_s_per_step"] / r["plain_s_per_step"]
+    launch = r.get("launch") or {}
+    geo = r.pop("mesh_sizes", None)
+    if geo and launch.get("ms"):
+        B = algorithmic_bytes(geo["nv"], geo["ne"], geo["nnz"], 3, 4, 4, 2)
+        a = B / (launch["ms"] * 1e-3) / 1e9
+        r["roofline"] = {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": a / HBM_PEAK_GBS,
+                         "algorithmic_bytes": B, "kernel_ms": launch["ms"],
+                         "kernel": "one channel assembly (A + M parts, FP32 J), autotuned launch"}
     return r
 
 
@@ -146,11 +222,17 @@ def batched_leg(steps: int = 20, count: int = 256):
     e.record()
     torch.cuda.synchronize()
     ms = s.elapsed_time(e) / steps
+    mesh = b.mesh
+    B = algorithmic_bytes(mesh.nv, mesh.ne, b.disc.nnz(), 2, 4, 4, 1)
+    a = B / (ms * 1e-3) / 1e9
     return {"workload": f"{count} 2D cavities in one launch (side U{{32..192}}, Re log-U[10,2000], lid U[0.5,1.5], "
                         f"seed 0), full upwind + FIELDS, FP32 J + FP64 defect, assembly + Dirichlet rows per HIP-graph "
                         f"replay",
             "elements": int(b.elements), "ms_per_step": ms, "elem_per_s": b.elements / (ms * 1e-3),
             "launch": {"path": best[0], "threads": best[1], "lds_bytes": best[2]},
+            "roofline": {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": a / HBM_PEAK_GBS,
+                         "algorithmic_bytes": B, "kernel_ms": ms,
+                         "kernel": "one graph replay: union assembly (steady, FP32 J) + Dirichlet rows"},
             "autotune": [{"path": p, "threads": t, "lds": l, "ms": m} for p, t, l, m in table]}
 
 
@@ -177,7 +259,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--timestep-steps", type=int, default=4, help="timed BDF2 steps of the time-stepping leg (0: skip)")
+    ap.add_argument("--timestep-steps", type=int, default=50,
+                    help="timed BDF2 steps of the 128^3 cavity time stepping (0: skip)")
+    ap.add_argument("--channel-steps", type=int, default=200,
+                    help="timed BDF2 steps per method (plain, learned) of the channel (0: skip)")
     ap.add_argument("--no-batched", dest="batched", action="store_false", help="skip the batched-instances leg")
     ap.add_argument("--no-channel", dest="channel", action="store_false",
                     help="skip the learned-vs-plain channel leg of the time stepping (profiling runs)")
@@ -193,7 +278,7 @@ def main():
 
     import torch
 
-    from plugin_navierstokes_amd import F32, NavierStokesFV1, PART_A
+    from plugin_navierstokes_amd import F32, NavierStokesFV1, PART_A, PART_M
     from plugin_navierstokes_amd import meshes as M
 
     dev = torch.device("cuda:0")
@@ -222,7 +307,7 @@ def main():
         torch.cuda.synchronize()
 
     def step():
-        disc.assemble(ut, uot, 0.01, parts=PART_A, jprec=F32, J=J, d=d, stream=stream, check=False)
+        disc.assemble(ut, uot, DT, s_m=S_M, parts=PART_A | PART_M, jprec=F32, J=J, d=d, stream=stream, check=False)
 
     # launch shape (incidences per workgroup, LDS budget) from the autotuner
     from plugin_navierstokes_amd.batch import autotune
@@ -270,7 +355,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"3D lid-driven cavity, {n}^3 Q1 hexahedra, Re=400, skewed upwind + FIELDS (RAW), "
-                        f"time-dependent stabilisation (u_old, dt=0.01), FP32 Jacobian + FP64 defect",
+                        f"time-dependent stabilisation (u_old, dt=0.01), BDF2 Newton iterate (A part + mass part "
+                        f"x 1.5/dt), FP32 Jacobian + FP64 defect",
             "elements": mesh.ne, "nodes": mesh.nv, "nnz": nnz, "parallelism": "1 GPU",
         },
         "roofline": {
@@ -280,14 +366,17 @@ def main():
             "kernel_ms": kavg, "algorithmic_bytes": B, "launch": launch,
         },
     }
+    out["parity"] = parity_sample(mesh, u, uold, J, d, disc.rowptr())
+    print(f"bench: parity {out['parity']}", file=sys.stderr, flush=True)
+    del J, d
+    torch.cuda.empty_cache()
+    out["timestep"] = {}
     if args.timestep_steps > 0:
-        del J, d
-        torch.cuda.empty_cache()
-        out["timestep"] = {"plain": timestep_leg(mesh, disc, args.timestep_steps)}
-        del disc
-        torch.cuda.empty_cache()
-        if args.channel:
-            out["timestep"]["channel"] = learned_leg(args.timestep_steps)
+        out["timestep"]["plain"] = timestep_leg(mesh, disc, args.timestep_steps)
+    del disc
+    torch.cuda.empty_cache()
+    if args.channel and args.channel_steps > 0:
+        out["timestep"]["channel"] = learned_leg(args.channel_steps)
     if args.batched:
         out["batched"] = batched_leg()
     if not args.no_cpu:

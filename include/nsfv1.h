@@ -70,6 +70,7 @@ enum { NSFV1_F32 = 32, NSFV1_F64 = 64 };
 #define NSFV1_ERR_NO_MESH        -12  /* assemble before set_mesh */
 #define NSFV1_ERR_DEVICE         -13  /* HIP runtime error */
 #define NSFV1_ERR_DEGENERATE     -14  /* zero Jacobian determinant in an element */
+#define NSFV1_ERR_SINGULAR_BLOCK -15  /* singular diagonal block in the block-Jacobi setup */
 
 /* Thread-local message of the last failing call. */
 const char* nsfv1_last_error(void);
@@ -244,7 +245,11 @@ int nsfv1_export(nsfv1_handle* h, int what, int at, const double* local_ips, int
 /* Synchronises `stream` and reports (then clears) errors raised inside the
    kernels since the last call: NSFV1_ERR_CUT_SIDE when a skewed/LPS upwind
    ray found no side (upwind.cpp:354, 422, 550), NSFV1_ERR_DEGENERATE for an
-   element with a zero Jacobian determinant. */
+   element with a zero Jacobian determinant, NSFV1_ERR_SINGULAR_BLOCK for a
+   singular diagonal block met by the block-Jacobi setup (linear solver and
+   multigrid smoother).  Each kind has its own device flag word, so one kind
+   cannot hide another; when several were raised, the first in this order is
+   reported. */
 int nsfv1_sync_errors(nsfv1_handle* h, void* stream);
 
 /* ---- Newton-loop services (ugcore DirichletBoundary / linear solver, which

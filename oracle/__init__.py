@@ -123,6 +123,9 @@ def lib():
         L.or_assemble.argtypes = [C.POINTER(_Params), C.c_int, dp, C.c_int64, ip32, C.c_int64, C.c_int64,
                                   dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                   ip64, ip32, dp, dp]
+        L.or_assemble_colored.argtypes = [C.POINTER(_Params), C.c_int, dp, C.c_int64, ip32, ip64, ip64, C.c_int,
+                                          dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                          ip64, ip32, dp, dp, C.c_int]
         L.or_assemble_abs.argtypes = [C.POINTER(_Params), C.c_int, dp, C.c_int64, ip32, C.c_int64,
                                       dp, dp, C.c_double, C.c_double, C.c_double, C.c_int,
                                       ip64, ip32, dp, dp]
@@ -261,6 +264,42 @@ def assemble(p: Params, et: int, xyz, conn, u, uold=None, dt=0.0, s_a=1.0, s_m=0
                            _dp(u), _dp(uold), float(dt), float(s_a), float(s_m), float(s_f), int(parts),
                            rowptr.ctypes.data_as(C.POINTER(C.c_int64)),
                            colidx.ctypes.data_as(C.POINTER(C.c_int32)), _dp(J), _dp(d)))
+    return J, d
+
+
+def structured_colors(shape):
+    """Element colouring of a lexicographic structured mesh (x fastest):
+    colour = parity of each cell index, so elements of one colour share no
+    node.  Returns (order, color_ptr)."""
+    idx = np.indices(tuple(shape)[::-1]).reshape(len(shape), -1)[::-1]  # idx[0] = i (fastest)
+    col = np.zeros(idx.shape[1], dtype=np.int64)
+    for a in range(len(shape)):
+        col += (idx[a] % 2) << a
+    order = np.argsort(col, kind="stable").astype(np.int64)
+    color_ptr = np.searchsorted(col[order], np.arange((1 << len(shape)) + 1)).astype(np.int64)
+    return order, color_ptr
+
+
+def assemble_colored(p: Params, et: int, xyz, conn, u, order, color_ptr, uold=None, dt=0.0, s_a=1.0, s_m=0.0,
+                     s_f=0.0, parts=1, rowptr=None, colidx=None, threads=1):
+    """assemble() with the elements of each colour spread over `threads`
+    OpenMP threads (the all-cores CPU baseline)."""
+    xyz, u, uold = _f64(xyz), _f64(u), _f64(uold)
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    order = np.ascontiguousarray(order, dtype=np.int64)
+    color_ptr = np.ascontiguousarray(color_ptr, dtype=np.int64)
+    nv = xyz.shape[0]
+    if rowptr is None:
+        rowptr, colidx = pattern(et, nv, conn)
+    J = np.zeros(int(rowptr[-1]))
+    d = np.zeros(nv * (DIM[et] + 1))
+    pc = p.c()
+    ip64 = C.POINTER(C.c_int64)
+    _chk(lib().or_assemble_colored(C.byref(pc), et, _dp(xyz), nv, conn.ctypes.data_as(C.POINTER(C.c_int32)),
+                                   order.ctypes.data_as(ip64), color_ptr.ctypes.data_as(ip64), len(color_ptr) - 1,
+                                   _dp(u), _dp(uold), float(dt), float(s_a), float(s_m), float(s_f), int(parts),
+                                   rowptr.ctypes.data_as(ip64), colidx.ctypes.data_as(C.POINTER(C.c_int32)),
+                                   _dp(J), _dp(d), int(threads)))
     return J, d
 
 

@@ -1314,6 +1314,79 @@ static int64_t find_col(const int64_t* rowptr, const int32_t* colidx, int64_t ro
     return -1;
 }
 
+/* one element's local defect/Jacobian (or their magnitudes, absmode) added
+   into the global vector and CSR values */
+static int assemble_elem(const or_params* p, int et, const double* xyz, const int32_t* conn, int64_t e,
+                         const double* u, const double* uold, double dt,
+                         double s_a, double s_m, double s_f, int parts,
+                         const int64_t* rowptr, const int32_t* colidx,
+                         double* J, double* d, int absmode)
+{
+    const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1, nl = nsh * nc;
+    double co[24], ul[32], uo[32], Jl[1024], Ja[1024], dl[32], da[32], fl[32];
+    const int32_t* c = conn + e * nsh;
+    for (int k = 0; k < nsh; ++k) {
+        for (int i = 0; i < dim; ++i) co[k * dim + i] = xyz[(int64_t)c[k] * dim + i];
+        for (int a = 0; a < nc; ++a) {
+            ul[k * nc + a] = u[(int64_t)c[k] * nc + a];
+            if (uold) uo[k * nc + a] = uold[(int64_t)c[k] * nc + a];
+        }
+    }
+    memset(Jl, 0, sizeof(double) * nl * nl);
+    memset(dl, 0, sizeof(double) * nl);
+    memset(Ja, 0, sizeof(double) * nl * nl);
+    memset(da, 0, sizeof(double) * nl);
+    int rc;
+    if (parts & 1) {
+        double Jt[1024], dt_[32];
+        memset(Jt, 0, sizeof(double) * nl * nl);
+        memset(dt_, 0, sizeof(double) * nl);
+        rc = or_elem_A(p, et, co, ul, uold ? uo : NULL, dt, J ? Jt : NULL, d ? dt_ : NULL);
+        if (rc) return rc;
+        if (absmode) {
+            double Jb[1024], db[32];
+            memset(Jb, 0, sizeof(double) * nl * nl);
+            memset(db, 0, sizeof(double) * nl);
+            rc = or_elem_A_abs(p, et, co, ul, uold ? uo : NULL, dt, J ? Jb : NULL, d ? db : NULL);
+            if (rc) return rc;
+            for (int i = 0; i < nl * nl; ++i) Ja[i] += fabs(s_a) * Jb[i];
+            for (int i = 0; i < nl; ++i) da[i] += fabs(s_a) * db[i];
+        }
+        for (int i = 0; i < nl * nl; ++i) Jl[i] += s_a * Jt[i];
+        for (int i = 0; i < nl; ++i) dl[i] += s_a * dt_[i];
+    }
+    if (parts & 2) {
+        double Jt[1024], dt_[32];
+        memset(Jt, 0, sizeof(double) * nl * nl);
+        memset(dt_, 0, sizeof(double) * nl);
+        rc = or_elem_M(p, et, co, ul, J ? Jt : NULL, d ? dt_ : NULL);
+        if (rc) return rc;
+        for (int i = 0; i < nl * nl; ++i) { Jl[i] += s_m * Jt[i]; Ja[i] += fabs(s_m * Jt[i]); }
+        for (int i = 0; i < nl; ++i) { dl[i] += s_m * dt_[i]; da[i] += fabs(s_m * dt_[i]); }
+    }
+    if ((parts & 4) && d) {
+        memset(fl, 0, sizeof(double) * nl);
+        rc = or_elem_rhs(p, et, co, fl);
+        if (rc) return rc;
+        for (int i = 0; i < nl; ++i) { dl[i] += s_f * fl[i]; da[i] += fabs(s_f * fl[i]); }
+    }
+    const double* Jsrc = absmode ? Ja : Jl;
+    const double* dsrc = absmode ? da : dl;
+    for (int i = 0; i < nsh; ++i)
+        for (int a = 0; a < nc; ++a) {
+            const int64_t row = (int64_t)c[i] * nc + a;
+            if (d) d[row] += dsrc[i * nc + a];
+            if (J)
+                for (int j = 0; j < nsh; ++j)
+                    for (int b = 0; b < nc; ++b) {
+                        const int64_t pos = find_col(rowptr, colidx, row, (int32_t)((int64_t)c[j] * nc + b));
+                        if (pos < 0) { set_err("pattern misses an entry"); return -6; }
+                        J[pos] += Jsrc[(i * nc + a) * nl + j * nc + b];
+                    }
+        }
+    return 0;
+}
+
 static int assemble_impl(const or_params* p, int et, const double* xyz, int64_t nv,
                          const int32_t* conn, int64_t e0, int64_t e1,
                          const double* u, const double* uold, double dt,
@@ -1322,75 +1395,49 @@ static int assemble_impl(const or_params* p, int et, const double* xyz, int64_t 
                          double* J, double* d, int absmode)
 {
     if (et < 0 || et > 3) { set_err("unknown element type"); return -1; }
-    const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1, nl = nsh * nc;
+    const int dim = REF[et].dim, nc = dim + 1;
     if (p->dim != dim) { set_err("Wrong number of functions"); return -5; }
     const int64_t ndof = nv * nc;
     if (J) memset(J, 0, sizeof(double) * rowptr[ndof]);
     if (d) memset(d, 0, sizeof(double) * ndof);
-    double co[24], ul[32], uo[32], Jl[1024], Ja[1024], dl[32], da[32], fl[32];
     for (int64_t e = e0; e < e1; ++e) {
-        const int32_t* c = conn + e * nsh;
-        for (int k = 0; k < nsh; ++k) {
-            for (int i = 0; i < dim; ++i) co[k * dim + i] = xyz[(int64_t)c[k] * dim + i];
-            for (int a = 0; a < nc; ++a) {
-                ul[k * nc + a] = u[(int64_t)c[k] * nc + a];
-                if (uold) uo[k * nc + a] = uold[(int64_t)c[k] * nc + a];
-            }
-        }
-        memset(Jl, 0, sizeof(double) * nl * nl);
-        memset(dl, 0, sizeof(double) * nl);
-        memset(Ja, 0, sizeof(double) * nl * nl);
-        memset(da, 0, sizeof(double) * nl);
-        int rc;
-        if (parts & 1) {
-            double Jt[1024], dt_[32];
-            memset(Jt, 0, sizeof(double) * nl * nl);
-            memset(dt_, 0, sizeof(double) * nl);
-            rc = or_elem_A(p, et, co, ul, uold ? uo : NULL, dt, J ? Jt : NULL, d ? dt_ : NULL);
-            if (rc) return rc;
-            if (absmode) {
-                double Jb[1024], db[32];
-                memset(Jb, 0, sizeof(double) * nl * nl);
-                memset(db, 0, sizeof(double) * nl);
-                rc = or_elem_A_abs(p, et, co, ul, uold ? uo : NULL, dt, J ? Jb : NULL, d ? db : NULL);
-                if (rc) return rc;
-                for (int i = 0; i < nl * nl; ++i) Ja[i] += fabs(s_a) * Jb[i];
-                for (int i = 0; i < nl; ++i) da[i] += fabs(s_a) * db[i];
-            }
-            for (int i = 0; i < nl * nl; ++i) Jl[i] += s_a * Jt[i];
-            for (int i = 0; i < nl; ++i) dl[i] += s_a * dt_[i];
-        }
-        if (parts & 2) {
-            double Jt[1024], dt_[32];
-            memset(Jt, 0, sizeof(double) * nl * nl);
-            memset(dt_, 0, sizeof(double) * nl);
-            rc = or_elem_M(p, et, co, ul, J ? Jt : NULL, d ? dt_ : NULL);
-            if (rc) return rc;
-            for (int i = 0; i < nl * nl; ++i) { Jl[i] += s_m * Jt[i]; Ja[i] += fabs(s_m * Jt[i]); }
-            for (int i = 0; i < nl; ++i) { dl[i] += s_m * dt_[i]; da[i] += fabs(s_m * dt_[i]); }
-        }
-        if ((parts & 4) && d) {
-            memset(fl, 0, sizeof(double) * nl);
-            rc = or_elem_rhs(p, et, co, fl);
-            if (rc) return rc;
-            for (int i = 0; i < nl; ++i) { dl[i] += s_f * fl[i]; da[i] += fabs(s_f * fl[i]); }
-        }
-        const double* Jsrc = absmode ? Ja : Jl;
-        const double* dsrc = absmode ? da : dl;
-        for (int i = 0; i < nsh; ++i)
-            for (int a = 0; a < nc; ++a) {
-                const int64_t row = (int64_t)c[i] * nc + a;
-                if (d) d[row] += dsrc[i * nc + a];
-                if (J)
-                    for (int j = 0; j < nsh; ++j)
-                        for (int b = 0; b < nc; ++b) {
-                            const int64_t pos = find_col(rowptr, colidx, row, (int32_t)((int64_t)c[j] * nc + b));
-                            if (pos < 0) { set_err("pattern misses an entry"); return -6; }
-                            J[pos] += Jsrc[(i * nc + a) * nl + j * nc + b];
-                        }
-            }
+        const int rc = assemble_elem(p, et, xyz, conn, e, u, uold, dt, s_a, s_m, s_f, parts, rowptr, colidx, J, d,
+                                     absmode);
+        if (rc) return rc;
     }
     return 0;
+}
+
+/* The same assembly with the elements taken colour by colour
+   (order[color_ptr[c] .. color_ptr[c+1]) holds colour c) and the elements of
+   one colour spread over OpenMP threads: no two elements of a colour may
+   share a node, so no two threads add into the same row.  The CPU baseline's
+   all-cores restatement of UG4's one-rank-per-core element loop; the sums
+   per entry then follow the colour order, not the element order. */
+int or_assemble_colored(const or_params* p, int et, const double* xyz, int64_t nv,
+                        const int32_t* conn, const int64_t* order, const int64_t* color_ptr, int ncolors,
+                        const double* u, const double* uold, double dt,
+                        double s_a, double s_m, double s_f, int parts,
+                        const int64_t* rowptr, const int32_t* colidx,
+                        double* J, double* d, int nthreads)
+{
+    if (et < 0 || et > 3) { set_err("unknown element type"); return -1; }
+    const int dim = REF[et].dim, nc = dim + 1;
+    if (p->dim != dim) { set_err("Wrong number of functions"); return -5; }
+    const int64_t ndof = nv * nc;
+    if (J) memset(J, 0, sizeof(double) * rowptr[ndof]);
+    if (d) memset(d, 0, sizeof(double) * ndof);
+    int err = 0;
+    for (int c = 0; c < ncolors && !err; ++c) {
+        const int64_t a = color_ptr[c], b = color_ptr[c + 1];
+#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(min : err)
+        for (int64_t i = a; i < b; ++i) {
+            const int rc = assemble_elem(p, et, xyz, conn, order[i], u, uold, dt, s_a, s_m, s_f, parts, rowptr,
+                                         colidx, J, d, 0);
+            if (rc < err) err = rc;
+        }
+    }
+    return err;
 }
 
 int or_assemble(const or_params* p, int et, const double* xyz, int64_t nv,

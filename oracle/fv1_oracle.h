@@ -102,6 +102,12 @@ int or_assemble(const or_params* p, int et, const double* xyz, int64_t nv,
                 double* J, double* d);
 
 /* Same, but returns the term-magnitude scale per entry (see or_elem_A_abs). */
+int or_assemble_colored(const or_params* p, int et, const double* xyz, int64_t nv,
+                        const int32_t* conn, const int64_t* order, const int64_t* color_ptr, int ncolors,
+                        const double* u, const double* uold, double dt,
+                        double s_a, double s_m, double s_f, int parts,
+                        const int64_t* rowptr, const int32_t* colidx,
+                        double* J, double* d, int nthreads);
 int or_assemble_abs(const or_params* p, int et, const double* xyz, int64_t nv,
                     const int32_t* conn, int64_t ne,
                     const double* u, const double* uold, double dt,

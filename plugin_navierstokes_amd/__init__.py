@@ -29,7 +29,7 @@ F32, F64 = 32, 64
 ERRORS = {
     -1: "ARGUMENT", -2: "FUNCTION_COUNT", -3: "TRIAL_SPACE", -4: "NO_STABILIZATION", -5: "NO_UPWIND",
     -6: "NO_VISCOSITY", -7: "HANGING_NODES", -8: "TIME_SERIES", -9: "CUT_SIDE", -10: "UNKNOWN_NAME",
-    -11: "NOT_IMPLEMENTED", -12: "NO_MESH", -13: "DEVICE", -14: "DEGENERATE",
+    -11: "NOT_IMPLEMENTED", -12: "NO_MESH", -13: "DEVICE", -14: "DEGENERATE", -15: "SINGULAR_BLOCK",
 }
 
 # every symbol the C ABI exports (include/nsfv1.h)
@@ -214,8 +214,12 @@ class NavierStokesFV1:
         _check(_lib.nsfv1_set_pac_upwind(self._h, int(bool(on))))
 
     def prepare_setting(self, lfe_types, orders, non_regular_grid=False):
-        n = len(lfe_types)
-        t = (C.c_int * n)(*lfe_types)
+        """lfe_types: LFEID types per function, as codes (0 = Lagrange) or
+        names ("lagrange", "crouzeix-raviart", "piecewise-constant")."""
+        names = {"lagrange": 0, "crouzeix-raviart": 1, "piecewise-constant": 2}
+        codes = [names.get(str(x).strip().lower(), -1) if isinstance(x, str) else int(x) for x in lfe_types]
+        n = len(codes)
+        t = (C.c_int * n)(*codes)
         o = (C.c_int * n)(*orders)
         _check(_lib.nsfv1_prepare_setting(self._h, t, o, n, int(bool(non_regular_grid))))
 
@@ -246,6 +250,12 @@ class NavierStokesFV1:
         colidx = np.zeros(self.nnz(), dtype=np.int32)
         _check(_lib.nsfv1_get_pattern(self._h, rowptr.ctypes.data, colidx.ctypes.data))
         return rowptr, colidx
+
+    def rowptr(self):
+        """Row pointers only (the column indices stay on the device)."""
+        rowptr = np.zeros(self.num_dofs() + 1, dtype=np.int64)
+        _check(_lib.nsfv1_get_pattern(self._h, rowptr.ctypes.data, None))
+        return rowptr
 
     def pattern_device(self):
         r, c = C.c_void_p(), C.c_void_p()

@@ -102,7 +102,8 @@ struct nsfv1_handle {
     std::vector<std::string> functions, subsets;
     // NavierStokesBase / IncompressibleNavierStokesBase state
     double nu = 0.0;
-    bool nu_set = false;
+    bool nu_set = false;         // a viscosity (global or per instance) is available
+    bool nu_global_set = false;  // set_kinematic_viscosity was called
     double rho = 1.0;  // navier_stokes_fv1.cpp:82
     bool has_source = false;
     double f[3] = {0, 0, 0};
@@ -193,8 +194,8 @@ int nsfv1_create(int dim, const char* functions, const char* subsets, nsfv1_hand
     h->device = dev;
     h->functions = fct;
     h->subsets.push_back(subsets ? subsets : "");
-    if (h->errflag.alloc(sizeof(int)) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "hipMalloc");
-    if (hipMemset(h->errflag.p, 0, sizeof(int)) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "hipMemset");
+    if (h->errflag.alloc(sizeof(int) * KERR_WORDS) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "hipMalloc");
+    if (hipMemset(h->errflag.p, 0, sizeof(int) * KERR_WORDS) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "hipMemset");
     *out = h.release();
     return NSFV1_OK;
 }
@@ -228,6 +229,7 @@ int nsfv1_set_kinematic_viscosity(nsfv1_handle* h, double nu)
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
     h->nu = nu;
     h->nu_set = true;
+    h->nu_global_set = true;
     return NSFV1_OK;
 }
 
@@ -709,6 +711,11 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
     if (parts & NSFV1_PART_A) {
         int rc = check_setup(h);
         if (rc) return rc;
+        if (h->nu_elem.p)
+            for (int kind = 0; kind < BND_KINDS; ++kind)
+                if (h->bnd[kind].nbn)
+                    return fail(NSFV1_ERR_NOT_IMPLEMENTED,
+                                "boundary discs with per-instance viscosity (batched instances) are not provided");
     }
     Plan& P = jprec == NSFV1_F64 ? h->plan64 : h->plan32;
     if (!P.valid) {
@@ -799,17 +806,21 @@ int nsfv1_batch_set_instances(nsfv1_handle* h, int64_t ninst, const int64_t* ele
     if (ninst == 0) {
         h->nu_elem.release();
         h->ninst = 0;
+        h->nu_set = h->nu_global_set;  // back to the global viscosity, if one was set
         return NSFV1_OK;
     }
     if (ninst < 0 || !elem_offset || !nu) return fail(NSFV1_ERR_ARGUMENT, "bad instance arrays");
     if (elem_offset[0] != 0 || elem_offset[ninst] != h->ne)
         return fail(NSFV1_ERR_ARGUMENT, "instance element ranges must cover the mesh: offset[0] = 0, offset[n] = ne");
-    std::vector<double> ne_nu((size_t)h->ne);
+    // validate every range and viscosity before anything is written
     for (int64_t i = 0; i < ninst; ++i) {
-        if (elem_offset[i + 1] < elem_offset[i]) return fail(NSFV1_ERR_ARGUMENT, "instance element ranges not monotone");
+        if (elem_offset[i + 1] < elem_offset[i] || elem_offset[i + 1] > h->ne)
+            return fail(NSFV1_ERR_ARGUMENT, "instance element ranges not monotone within [0, ne]");
         if (!(nu[i] > 0.0)) return fail(NSFV1_ERR_ARGUMENT, "instance viscosity must be positive");
-        for (int64_t e = elem_offset[i]; e < elem_offset[i + 1]; ++e) ne_nu[e] = nu[i];
     }
+    std::vector<double> ne_nu((size_t)h->ne);
+    for (int64_t i = 0; i < ninst; ++i)
+        for (int64_t e = elem_offset[i]; e < elem_offset[i + 1]; ++e) ne_nu[e] = nu[i];
     if (h->nu_elem.alloc(sizeof(double) * h->ne) != hipSuccess ||
         hipMemcpy(h->nu_elem.p, ne_nu.data(), sizeof(double) * h->ne, hipMemcpyHostToDevice) != hipSuccess)
         return fail(NSFV1_ERR_DEVICE, "instance viscosity upload");
@@ -1026,14 +1037,17 @@ int nsfv1_set_upwind_decisions(nsfv1_handle* h, int mode)
 int nsfv1_sync_errors(nsfv1_handle* h, void* stream)
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
-    int flag = 0;
-    if (hipMemcpyAsync(&flag, h->errflag.p, sizeof(int), hipMemcpyDeviceToHost, (hipStream_t)stream) != hipSuccess ||
+    int flag[KERR_WORDS] = {0, 0, 0, 0};
+    if (hipMemcpyAsync(flag, h->errflag.p, sizeof(flag), hipMemcpyDeviceToHost, (hipStream_t)stream) != hipSuccess ||
         hipStreamSynchronize((hipStream_t)stream) != hipSuccess)
         return fail(NSFV1_ERR_DEVICE, "error-flag copy");
-    if (flag) {
-        (void)hipMemset(h->errflag.p, 0, sizeof(int));
-        if (flag & KERR_CUT_SIDE) return fail(NSFV1_ERR_CUT_SIDE, "GetNodeNextToCut: Cannot find cut side.");
-        return fail(NSFV1_ERR_DEGENERATE, "degenerate element");
+    if (flag[KERR_CUT_SIDE] || flag[KERR_DEGENERATE] || flag[KERR_SINGULAR_BLOCK]) {
+        if (hipMemsetAsync(h->errflag.p, 0, sizeof(flag), (hipStream_t)stream) != hipSuccess ||
+            hipStreamSynchronize((hipStream_t)stream) != hipSuccess)
+            return fail(NSFV1_ERR_DEVICE, "error-flag reset");
+        if (flag[KERR_CUT_SIDE]) return fail(NSFV1_ERR_CUT_SIDE, "GetNodeNextToCut: Cannot find cut side.");
+        if (flag[KERR_DEGENERATE]) return fail(NSFV1_ERR_DEGENERATE, "degenerate element (zero Jacobian determinant)");
+        return fail(NSFV1_ERR_SINGULAR_BLOCK, "block-Jacobi setup: singular diagonal block in the Jacobian");
     }
     return NSFV1_OK;
 }
@@ -1091,6 +1105,30 @@ int nsfv1_set_preconditioner(nsfv1_handle* h, const char* name)
     else if (n == "gmg" || n == "multigrid") h->use_mg = true;
     else return fail(NSFV1_ERR_UNKNOWN_NAME, "preconditioner '" + std::string(name ? name : "") + "' not found. Options are: block-jacobi, gmg");
     return NSFV1_OK;
+}
+
+// coarse levels carry the configuration of the fine discretisation
+static void mg_sync_settings(nsfv1_handle* h)
+{
+    for (auto& L : h->mg) {
+        nsfv1_handle* c = L->h;
+        c->nu = h->nu;
+        c->nu_set = h->nu_set;
+        c->nu_global_set = h->nu_global_set;
+        c->rho = h->rho;
+        c->has_source = h->has_source;
+        for (int i = 0; i < 3; ++i) c->f[i] = h->f[i];
+        c->exact = h->exact;
+        c->stokes = h->stokes;
+        c->laplace = h->laplace;
+        c->peclet = h->peclet;
+        c->grad_div = h->grad_div;
+        c->stab_type = h->stab_type;
+        c->stab_upwind = h->stab_upwind;
+        c->difflen = h->difflen;
+        c->conv_upwind = h->conv_upwind;
+        c->pac = h->pac;
+    }
 }
 
 int nsfv1_mg_set_hierarchy(nsfv1_handle* h, int nlev, nsfv1_handle* const* coarse, const int64_t* const* prol_rowptr,
@@ -1165,26 +1203,8 @@ int nsfv1_mg_set_hierarchy(nsfv1_handle* h, int nlev, nsfv1_handle* const* coars
         h->mg_dense.alloc(sizeof(double) * ncoarse * ncoarse) != hipSuccess ||
         h->mg_gj.alloc(dense_inverse_work_bytes((int)ncoarse)) != hipSuccess)
         return fail(NSFV1_ERR_DEVICE, "multigrid workspace");
-    // coarse levels carry the configuration of the fine discretisation
-    for (auto& L : lv) {
-        nsfv1_handle* c = L->h;
-        c->nu = h->nu;
-        c->nu_set = h->nu_set;
-        c->rho = h->rho;
-        c->has_source = h->has_source;
-        for (int i = 0; i < 3; ++i) c->f[i] = h->f[i];
-        c->exact = h->exact;
-        c->stokes = h->stokes;
-        c->laplace = h->laplace;
-        c->peclet = h->peclet;
-        c->grad_div = h->grad_div;
-        c->stab_type = h->stab_type;
-        c->stab_upwind = h->stab_upwind;
-        c->difflen = h->difflen;
-        c->conv_upwind = h->conv_upwind;
-        c->pac = h->pac;
-    }
     h->mg.swap(lv);
+    mg_sync_settings(h);
     h->mg_pre = pre;
     h->mg_post = post;
     h->mg_omega = omega;
@@ -1198,6 +1218,10 @@ int nsfv1_mg_setup(nsfv1_handle* h, int jprec, const double* u, const double* u_
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
     if (h->mg.empty()) return fail(NSFV1_ERR_ARGUMENT, "no multigrid hierarchy set");
+    if (h->nu_elem.p)
+        return fail(NSFV1_ERR_NOT_IMPLEMENTED, "multigrid with per-instance viscosity (batched instances) is not provided");
+    // the coarse levels take the fine level's current configuration
+    mg_sync_settings(h);
     if (!u) return fail(NSFV1_ERR_ARGUMENT, "u is NULL");
     if (jprec != NSFV1_F32 && jprec != NSFV1_F64) return fail(NSFV1_ERR_ARGUMENT, "jprec must be 32 or 64");
     hipStream_t s = (hipStream_t)stream;

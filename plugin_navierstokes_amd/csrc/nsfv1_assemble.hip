@@ -388,7 +388,7 @@ geometry_cache(const double* __restrict__ xyz, const int32_t* __restrict__ conn,
         ok &= (nn[ip] > 0.0) && (a2 > 0.0);
         l2inv[e * NIP + ip] = L;
     }
-    if (!ok) err[0] = KERR_DEGENERATE;
+    if (!ok) kernel_error(err, KERR_DEGENERATE);
 }
 
 // ---------------------------------------------------------------------
@@ -1030,7 +1030,8 @@ assemble_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const double
 #pragma unroll
                 for (int d = 0; d < D; ++d) L.dacc[d] += P.s_f * (P.f[d] * volk * P.rho);
         }
-        if (!okg || !oku) err[0] = (oku ? 0 : KERR_CUT_SIDE) | (okg ? 0 : KERR_DEGENERATE);
+        if (!oku) kernel_error(err, KERR_CUT_SIDE);
+    if (!okg) kernel_error(err, KERR_DEGENERATE);
         // column slots of the (canonically ordered) corners in v's block row
         const uint64_t pw = M.inc_pos[inc];
 #pragma unroll
@@ -1272,7 +1273,8 @@ scvf_records(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
     bool okg = true, oku = true;
     record_one<ET, FAST, Q>(C, co, M.xyz, M.conn + e * N, ul, pk, ccn, c, uold, l2, kv, rec, ne_pad, okg, oku, decp,
                             affine);
-    if (!okg || !oku) err[0] = (oku ? 0 : KERR_CUT_SIDE) | (okg ? 0 : KERR_DEGENERATE);
+    if (!oku) kernel_error(err, KERR_CUT_SIDE);
+    if (!okg) kernel_error(err, KERR_DEGENERATE);
 }
 
 template <int ET, int FAST, int Q>
@@ -1908,7 +1910,8 @@ export_stab_vel(KMeshPtrs M, KParams P, const double* __restrict__ u, const doub
     const Cfg<-1> C{P, M.nu_elem ? M.nu_elem[e] : P.nu};
     stab_vel_all<ET, 0>(C, co, M.xyz, M.conn + e * N, ul, pk, ccn, c, uold, M.l2inv + e * NIP, kv,
                         out + e * NIP * D, okg, oku, (ET == 0 || ET == 2) || (M.affine != nullptr && M.affine[e] != 0));
-    if (!okg || !oku) err[0] = (oku ? 0 : KERR_CUT_SIDE) | (okg ? 0 : KERR_DEGENERATE);
+    if (!oku) kernel_error(err, KERR_CUT_SIDE);
+    if (!okg) kernel_error(err, KERR_DEGENERATE);
 }
 
 hipError_t launch_export_stab_vel(int et, const KMeshPtrs& m, const KParams& p, const double* u, const double* uold,

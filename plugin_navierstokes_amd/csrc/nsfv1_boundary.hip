@@ -39,6 +39,7 @@
 #include <stdint.h>
 
 #include "nsfv1_boundary.hpp"
+#include "nsfv1_device.hpp"
 #include "nsfv1_topo.hpp"
 
 namespace nsfv1 {
@@ -201,14 +202,14 @@ outflow_rows(OutflowArgs A, const double* __restrict__ u, JT* __restrict__ J, do
         double Ji[3][3];
         if constexpr (D == 2) {
             const double det = Jm[0][0] * Jm[1][1] - Jm[0][1] * Jm[1][0];
-            if (det == 0.0) { err[0] = 2; continue; }
+            if (det == 0.0) { kernel_error(err, KERR_DEGENERATE); continue; }
             Ji[0][0] = Jm[1][1] / det;  Ji[0][1] = -Jm[0][1] / det;
             Ji[1][0] = -Jm[1][0] / det; Ji[1][1] = Jm[0][0] / det;
         } else {
             const double det = Jm[0][0] * (Jm[1][1] * Jm[2][2] - Jm[1][2] * Jm[2][1]) -
                                Jm[0][1] * (Jm[1][0] * Jm[2][2] - Jm[1][2] * Jm[2][0]) +
                                Jm[0][2] * (Jm[1][0] * Jm[2][1] - Jm[1][1] * Jm[2][0]);
-            if (det == 0.0) { err[0] = 2; continue; }
+            if (det == 0.0) { kernel_error(err, KERR_DEGENERATE); continue; }
             Ji[0][0] = (Jm[1][1] * Jm[2][2] - Jm[1][2] * Jm[2][1]) / det;
             Ji[0][1] = (Jm[0][2] * Jm[2][1] - Jm[0][1] * Jm[2][2]) / det;
             Ji[0][2] = (Jm[0][1] * Jm[1][2] - Jm[0][2] * Jm[1][1]) / det;

@@ -511,7 +511,8 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
             svel[d] = x;
         }
     }
-    if (!ok_geo || !ok_cut) err[0] = (ok_cut ? 0 : KERR_CUT_SIDE) | (ok_geo ? 0 : KERR_DEGENERATE);
+    if (!ok_cut) kernel_error(err, KERR_CUT_SIDE);
+    if (!ok_geo) kernel_error(err, KERR_DEGENERATE);
     if (vel_out) {
 #pragma unroll
         for (int d = 0; d < D; ++d) vel_out[(e * NIP + ip) * D + d] = svel[d];
@@ -530,7 +531,7 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
         if (P.conv_upwind != P.stab_upwind || P.stab_type == STAB_WITHOUT)
             ok_cut &= upwind_lane<ET>(P.conv_upwind, 1.0, ip, co, n, S, Uc, Wc, Lc);
         conv_ip = P.conv_upwind == UPW_POSITIVE;
-        if (!ok_cut) err[0] = KERR_CUT_SIDE;
+        if (!ok_cut) kernel_error(err, KERR_CUT_SIDE);
     }
     (void)Lc;
     // upwind velocity: upwind_interface.h:335-358, or the stabilised one

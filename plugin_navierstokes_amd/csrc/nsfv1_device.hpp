@@ -12,8 +12,12 @@ enum { UPW_NO = 0, UPW_FULL = 1, UPW_SKEWED = 2, UPW_LPS = 3, UPW_POSITIVE = 4 }
 enum { STAB_FIELDS = 0, STAB_WITHOUT = 1, STAB_FLOW = 2 };
 enum { DL_RAW = 0, DL_FIVEPOINT = 1, DL_COR = 2 };
 
-// error bits reported by kernels
-enum { KERR_CUT_SIDE = 1, KERR_DEGENERATE = 2 };
+// error flags reported by kernels: one int word per kind, set by a plain
+// store of 1 (any lane may set it; no lane ever clears it), so concurrent
+// writers of different kinds cannot overwrite each other
+enum { KERR_CUT_SIDE = 0, KERR_DEGENERATE = 1, KERR_SINGULAR_BLOCK = 2, KERR_WORDS = 4 };
+
+__device__ inline void kernel_error(int* err, int kind) { err[kind] = 1; }
 
 struct KParams {
     int conv_upwind, stab_upwind, stab_type, pac, difflen;

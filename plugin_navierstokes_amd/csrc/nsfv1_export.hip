@@ -90,14 +90,14 @@ export_points(ExportArgs A, const double* __restrict__ u, double* __restrict__ o
         }
     if constexpr (D == 2) {
         const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-        if (det == 0.0) { err[0] = KERR_DEGENERATE; return; }
+        if (det == 0.0) { kernel_error(err, KERR_DEGENERATE); return; }
         Ji[0][0] = J[1][1] / det;  Ji[0][1] = -J[0][1] / det;
         Ji[1][0] = -J[1][0] / det; Ji[1][1] = J[0][0] / det;
     } else {
         const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
                            J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
                            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-        if (det == 0.0) { err[0] = KERR_DEGENERATE; return; }
+        if (det == 0.0) { kernel_error(err, KERR_DEGENERATE); return; }
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "nsfv1_device.hpp"
 #include "nsfv1_solver.hpp"
 
 namespace nsfv1 {
@@ -115,7 +116,7 @@ __global__ void block_jacobi_setup(const int64_t* __restrict__ rowptr, const int
     for (int a = 0; a < NC; ++a)
 #pragma unroll
         for (int b = 0; b < NC; ++b) Binv[(v * NC + a) * NC + b] = ok ? M[a][NC + b] : (a == b ? 1.0 : 0.0);
-    if (!ok) err[0] = 4;
+    if (!ok) kernel_error(err, KERR_SINGULAR_BLOCK);
 }
 
 template <int NC>

@@ -309,6 +309,7 @@ def compare(weights, seeds=(100, 101), steps=8, cells=(96, 48, 48), log=print, w
             # this channel (undamped sweeps take 1.5x as long here)
             case = channel_case(sd, cells, tune=True, smoothing=(1, 1, 0.9))
             res["launch"] = case[5].get("launch")
+            res["mesh_sizes"] = {"nv": int(case[0].nv), "ne": int(case[0].ne), "nnz": int(case[1].nnz())}
             pred = LearnedPredictor(case[1], mlp, in_scale, out_scale) if mode == "learned" else None
             out = run_trajectory(case, steps, predictor=pred, log=None, warm=warm)
             res[mode] += [o[0] for o in out]
@@ -324,7 +325,8 @@ def compare(weights, seeds=(100, 101), steps=8, cells=(96, 48, 48), log=print, w
             "prediction_used_fraction": float(np.mean(np.asarray(res["learned_choice"]) == 0)),
             "plain_initial_residual": float(np.mean(res["plain_r0"])),
             "learned_initial_residual": float(np.mean(res["learned_r0"])),
-            "steps_timed": len(res["plain"]), "warm_steps": warm, "launch": res.get("launch")}
+            "steps_timed": len(res["plain"]), "warm_steps": warm, "launch": res.get("launch"),
+            "mesh_sizes": res.get("mesh_sizes")}
 
 
 if __name__ == "__main__":

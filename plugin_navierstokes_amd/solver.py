@@ -124,11 +124,13 @@ class NewtonSolver:
 
         return float(torch.linalg.vector_norm(self.defect(u, u_old, dt, s_m, const, want_J=False)))
 
-    def solve(self, u, u_old=None, dt=0.0, s_m=0.0, const=None, ref_norm=None, mg_lag=False) -> NewtonStats:
+    def solve(self, u, u_old=None, dt=0.0, s_m=0.0, const=None, ref_norm=None, mg_lag=False,
+              callback=None) -> NewtonStats:
         """Iterate until ||d(u)|| <= max(rtol * ref, atol), ref = ref_norm or
         the defect norm of the initial guess.  mg_lag keeps the multigrid
         levels of the first iteration for the whole solve (the Krylov
-        operator is always the current Jacobian)."""
+        operator is always the current Jacobian).  callback(it, u), if
+        given, sees every iterate before its defect is assembled."""
         import torch
 
         st = NewtonStats()
@@ -136,7 +138,7 @@ class NewtonSolver:
         self.impose(u)
         uf = u.view(-1)
         try:
-            self._iterate(st, u, uf, u_old, dt, s_m, const, ref_norm, mg_lag)
+            self._iterate(st, u, uf, u_old, dt, s_m, const, ref_norm, mg_lag, callback)
         finally:
             if self.freeze_after is not None:
                 self.disc.set_upwind_decisions(UPWIND_FRESH)
@@ -147,11 +149,13 @@ class NewtonSolver:
         st.seconds = time.perf_counter() - t0
         return st
 
-    def _iterate(self, st, u, uf, u_old, dt, s_m, const, ref_norm, mg_lag):
+    def _iterate(self, st, u, uf, u_old, dt, s_m, const, ref_norm, mg_lag, callback=None):
         import torch
 
         k = self.freeze_after
         for it in range(self.maxit + 1):
+            if callback is not None:
+                callback(it, u)
             if k is not None:
                 self.disc.set_upwind_decisions(UPWIND_FRESH if it < k - 1 else
                                                (UPWIND_RECORD if it == k - 1 else UPWIND_REPLAY))

@@ -35,3 +35,18 @@ def test_upwind_and_stabilization_keys_documented():
     txt = open(os.path.join(ROOT, "include", "nsfv1.h")).read()
     for key in ["no", "full", "skewed", "lps", "fields", "flow", "raw", "fivepoint", "cor"]:
         assert key in txt
+
+
+def test_function_count_checked_before_any_device_call():
+    """navier_stokes_fv1.cpp:66-68: the constructor refuses a function list
+    of the wrong length; the check precedes every HIP call, so it also runs
+    without a GPU."""
+    if not os.path.exists(P.LIB_PATH):
+        pytest.fail("libnsfv1.so not built (run __graft_entry__.build())")
+    L = P.load_library()
+    h = ctypes.c_void_p()
+    rc = L.nsfv1_create(2, b"u,v,w,p", b"Inner", ctypes.byref(h))
+    assert rc == -2 and not h.value
+    assert b"needs exactly 3 symbolic function" in L.nsfv1_last_error()
+    rc = L.nsfv1_create(4, b"u,v,w,p", b"Inner", ctypes.byref(h))
+    assert rc == -1 and b"dim must be 2 or 3" in L.nsfv1_last_error()

@@ -1,6 +1,6 @@
 """Full-size workloads, FP32 Jacobian + FP64 defect, time-dependent
-stabilisation (u_old, dt = 0.01), checked through properties that do not
-need a full oracle assembly:
+stabilisation (u_old, dt = 0.01) and the BDF2 mass part (s_m = 1.5/dt), as
+in one Newton iterate of a time step:
 - "cavity": the 128^3 hexahedral lid-driven cavity of the benchmark
   (2,097,152 elements, nnz 913,066,000), skewed upwind + FIELDS (split path);
 - "channel": a 96x48x48 x 6-tetrahedra obstacle channel like the learned
@@ -9,13 +9,18 @@ need a full oracle assembly:
 - the batched leg's union of 256 cavities (3,958,533 quadrilaterals, full
   upwind + FIELDS, per-instance viscosity), steady.
 
+- every entry of the 128^3 cavity against the oracle: the oracle assembles
+  the mesh in slabs of node planes (each slab with the element layers around
+  it, on a thread pool), and every Jacobian entry and defect entry of each
+  slab's rows must match to the FP32 / FP64 entry tolerance below.
 - sampled rows against the oracle: every element around 300 seeded nodes
   (plus corner, edge and face nodes of the cavity, and inflow, outflow, wall
   and obstacle nodes of the channel) is assembled by the oracle, which
   makes those nodes' rows complete. The GPU defect rows must match to the
-  FP64 tolerance, |a-b| <= 1e-12|b| + 1e-13 S. The GPU row products J x (FP32
-  J, FP64 x) must match the oracle rows times x to the FP32 entry tolerance
-  summed over the row: sum_k (2^-17 S_k + 2^-20 |J_k|) |x_k|.
+  FP64 tolerance, |a-b| <= 1e-12|b| + 1e-13 S; every Jacobian entry of those
+  rows to the FP32 entry tolerance |a-b| <= 2^-17 S + 2^-20 |b|; and the GPU
+  row products J x (FP32 J, FP64 x) to that tolerance summed over the row:
+  sum_k (2^-17 S_k + 2^-20 |J_k|) |x_k|.
 - conservation: each SCVF flux enters its two nodes' rows with opposite
   signs, and with no boundary discs nothing else enters the A part. So the
   sum over all nodes of each defect component vanishes up to FP64 rounding,
@@ -28,14 +33,16 @@ import numpy as np
 import pytest
 
 import oracle as O
-from plugin_navierstokes_amd import F32, NavierStokesFV1, PART_A
+from plugin_navierstokes_amd import F32, NavierStokesFV1, PART_A, PART_M
 from plugin_navierstokes_amd import meshes as M
-from tests._parity import FP32_ABS, FP32_REL, fp64_violation, worst
+from tests._parity import FP32_ABS, FP32_REL, fp32_violation, fp64_violation, worst
 
 pytestmark = pytest.mark.gpu
 
 N = 128
 DT = 0.01
+S_M = 1.5 / DT
+PARTS = PART_A | PART_M
 
 
 @pytest.fixture(scope="module", params=["cavity", "channel"])
@@ -58,7 +65,7 @@ def workload(request, cuda):
     disc.set_mesh(mesh)
     ut, uot = torch.from_numpy(u).to(cuda), torch.from_numpy(uold).to(cuda)
     print(f"full-size {request.param}: ne={mesh.ne} nnz={disc.nnz()}", flush=True)
-    J, d = disc.assemble(ut, uot, DT, parts=PART_A, jprec=F32)
+    J, d = disc.assemble(ut, uot, DT, s_m=S_M, parts=PARTS, jprec=F32)
     params = O.Params(dim=3, upwind=up, nu=nu)
     return request.param, et, params, mesh, disc, u, uold, ut, uot, J, d
 
@@ -88,13 +95,24 @@ def test_fullsize_sampled_rows_vs_oracle(cuda, workload):
     nodes = _sample_nodes(name, mesh)
     conn_sub = mesh.conn[np.isin(mesh.conn, nodes).any(axis=1)]
     rp, ci = O.pattern(et, mesh.nv, conn_sub)
-    Jr, dr = O.assemble(p, et, mesh.xyz, conn_sub, u, uold, DT, parts=PART_A, rowptr=rp, colidx=ci)
-    Ja, da = O.assemble_abs(p, et, mesh.xyz, conn_sub, u, uold, DT, parts=PART_A, rowptr=rp, colidx=ci)
+    Jr, dr = O.assemble(p, et, mesh.xyz, conn_sub, u, uold, DT, s_m=S_M, parts=PARTS, rowptr=rp, colidx=ci)
+    Ja, da = O.assemble_abs(p, et, mesh.xyz, conn_sub, u, uold, DT, s_m=S_M, parts=PARTS, rowptr=rp, colidx=ci)
     rows = (nodes[:, None] * 4 + np.arange(4)[None, :]).ravel()
     rows_t = torch.from_numpy(rows).to(cuda)
 
     dg = d[rows_t].cpu().numpy()
     assert fp64_violation(dg, dr[rows], da[rows]) <= 1.0, worst(dg, dr[rows], da[rows])
+
+    # every Jacobian entry of the sampled rows: the oracle rows are complete
+    # (all incident elements assembled), so their columns are the GPU's
+    rpg = disc.rowptr()
+    idx = np.concatenate([np.arange(rpg[r], rpg[r + 1]) for r in rows])
+    idx_o = np.concatenate([np.arange(rp[r], rp[r + 1]) for r in rows])
+    assert len(idx) == len(idx_o)
+    Jg = J[torch.from_numpy(idx).to(cuda)].cpu().numpy().astype(np.float64)
+    v = fp32_violation(Jg, Jr[idx_o], Ja[idx_o])
+    print(f"{name}: {len(idx)} Jacobian entries of {len(rows)} sampled rows, max violation {v:.3g}", flush=True)
+    assert v <= 1.0, worst(Jg, Jr[idx_o], Ja[idx_o], FP32_REL, FP32_ABS)
 
     gen = torch.Generator(cuda).manual_seed(17)
     x = torch.randn(disc.num_dofs(), dtype=torch.float64, device=cuda, generator=gen)
@@ -114,10 +132,58 @@ def test_fullsize_sampled_rows_vs_oracle(cuda, workload):
     assert v[i] <= 1.0, f"row {rows[i]}: got {yg[i]!r} want {yr[i]!r} tol {tol[i]:.3g}"
 
 
+def _slab(args):
+    """Oracle rows of the node planes [k0, k1) of the cavity: the element
+    layers [k0-1, k1) make those rows complete."""
+    p, mesh, u, uold, k0, k1 = args
+    plane, layer = (N + 1) ** 2, N * N
+    conn_sub = mesh.conn[max(k0 - 1, 0) * layer:min(k1, N) * layer]
+    rp, ci = O.pattern(O.HEX, mesh.nv, conn_sub)
+    Jr, dr = O.assemble(p, O.HEX, mesh.xyz, conn_sub, u, uold, DT, s_m=S_M, parts=PARTS, rowptr=rp, colidx=ci)
+    Ja, da = O.assemble_abs(p, O.HEX, mesh.xyz, conn_sub, u, uold, DT, s_m=S_M, parts=PARTS, rowptr=rp, colidx=ci)
+    r0, r1 = k0 * plane * 4, k1 * plane * 4
+    a, b = rp[r0], rp[r1]
+    return k0, k1, ci[a:b].copy(), Jr[a:b].copy(), Ja[a:b].copy(), dr[r0:r1].copy(), da[r0:r1].copy()
+
+
+def test_fullsize_cavity_every_entry(cuda, workload):
+    """All 913,066,000 Jacobian entries and 8,586,756 defect entries of the
+    benchmark cavity against the oracle, slab by slab."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    name, _, p, mesh, disc, u, uold, _, _, J, d = workload
+    if name != "cavity":
+        pytest.skip("every-entry check covers the benchmark cavity (the channel is checked on sampled rows)")
+    rpg, cig = disc.pattern()
+    dg_all = d.cpu().numpy()
+    slabs = [(k, min(k + 8, N + 1)) for k in range(0, N + 1, 8)]
+    workers = max(1, min(16, os.cpu_count() or 1))
+    worst_j = worst_d = 0.0
+    nj = 0
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        for k0, k1, ci, Jr, Ja, dr, da in ex.map(_slab, [(p, mesh, u, uold, a, b) for a, b in slabs]):
+            r0, r1 = k0 * (N + 1) ** 2 * 4, k1 * (N + 1) ** 2 * 4
+            a, b = int(rpg[r0]), int(rpg[r1])
+            assert b - a == len(Jr) and np.array_equal(cig[a:b], ci), (k0, k1)
+            Jg = J[a:b].cpu().numpy().astype(np.float64)
+            vj = fp32_violation(Jg, Jr, Ja)
+            vd = fp64_violation(dg_all[r0:r1], dr, da)
+            assert vj <= 1.0, (k0, k1, worst(Jg, Jr, Ja, FP32_REL, FP32_ABS))
+            assert vd <= 1.0, (k0, k1, worst(dg_all[r0:r1], dr, da))
+            worst_j, worst_d = max(worst_j, vj), max(worst_d, vd)
+            nj += b - a
+            print(f"cavity planes [{k0}, {k1}): {b - a} J entries, violations J {vj:.3g} d {vd:.3g}", flush=True)
+    assert nj == disc.nnz() == 913_066_000
+    print(f"cavity: every entry checked, worst violations J {worst_j:.3g} d {worst_d:.3g}", flush=True)
+
+
 def test_fullsize_conservation(cuda, workload):
+    """On the A part alone (the mass part is not a flux)."""
     import torch
 
-    disc, J, d = workload[4], workload[9], workload[10]
+    disc, ut, uot = workload[4], workload[7], workload[8]
+    J, d = disc.assemble(ut, uot, DT, parts=PART_A, jprec=F32)
     dd = d.view(-1, 4)
     s = dd.sum(dim=0).cpu().numpy()
     a = dd.abs().sum(dim=0).cpu().numpy()
@@ -140,7 +206,7 @@ def test_fullsize_repeatable(cuda, workload):
     import torch
 
     disc, ut, uot, J, d = workload[4], workload[7], workload[8], workload[9], workload[10]
-    J2, d2 = disc.assemble(ut, uot, DT, parts=PART_A, jprec=F32)
+    J2, d2 = disc.assemble(ut, uot, DT, s_m=S_M, parts=PARTS, jprec=F32)
     assert torch.equal(J, J2) and torch.equal(d, d2)
 
 
@@ -172,6 +238,7 @@ def test_fullsize_batch_sampled_rows_vs_oracle(cuda, batch):
     y = torch.empty_like(x)
     b.disc.spmv(J, x, y, jprec=F32)
     xh, yh, dh = x.cpu().numpy(), y.cpu().numpy(), d.cpu().numpy()
+    rpg = b.disc.rowptr()
     for k in np.unique(inst):
         vs = nodes[inst == k]
         e0, e1 = b.eoff[k], b.eoff[k + 1]
@@ -183,6 +250,12 @@ def test_fullsize_batch_sampled_rows_vs_oracle(cuda, batch):
         Ja, da = O.assemble_abs(p, O.QUAD, mesh.xyz, conn_sub, uh, parts=PART_A, rowptr=rp, colidx=ci)
         rows = (vs[:, None] * 3 + np.arange(3)[None, :]).ravel()
         assert fp64_violation(dh[rows], dr[rows], da[rows]) <= 1.0, (k, worst(dh[rows], dr[rows], da[rows]))
+        # every Jacobian entry of these rows
+        idx = np.concatenate([np.arange(rpg[r], rpg[r + 1]) for r in rows])
+        idx_o = np.concatenate([np.arange(rp[r], rp[r + 1]) for r in rows])
+        assert len(idx) == len(idx_o)
+        Jg = J[torch.from_numpy(idx).to(cuda)].cpu().numpy().astype(np.float64)
+        assert fp32_violation(Jg, Jr[idx_o], Ja[idx_o]) <= 1.0, (k, worst(Jg, Jr[idx_o], Ja[idx_o], FP32_REL, FP32_ABS))
         for r in rows:
             k0, k1 = rp[r], rp[r + 1]
             xs = xh[ci[k0:k1]]

@@ -402,3 +402,24 @@ def test_ghia_re100_centrelines():
     ev = np.abs(vx - np.array(ref["horiz_v"])).max()
     print(f"Ghia Re=100: max |u - ref| = {eu:.4f}, max |v - ref| = {ev:.4f}, Newton its {len(hist)}")
     assert eu < 0.03 and ev < 0.03
+
+
+def test_colored_assembly_matches_serial():
+    """The all-cores (coloured OpenMP) oracle assembly equals the serial one
+    up to the order of the sums: elements of one colour share no node."""
+    mesh = M.cavity_3d(6, M.HEX, 0.1, 4)
+    u = M.random_state(mesh, 2)
+    uold = M.random_state(mesh, 3)
+    p = O.Params(dim=3, upwind="skewed", nu=1.0 / 400.0)
+    rp, ci = O.pattern(O.HEX, mesh.nv, mesh.conn)
+    order, cptr = O.structured_colors(mesh.shape)
+    assert len(cptr) == 9 and cptr[-1] == mesh.ne
+    for c in range(8):
+        nodes = mesh.conn[order[cptr[c]:cptr[c + 1]]].ravel()
+        assert len(np.unique(nodes)) == len(nodes)
+    Js, ds = O.assemble(p, O.HEX, mesh.xyz, mesh.conn, u, uold, 0.01, s_m=150.0, parts=3, rowptr=rp, colidx=ci)
+    Jc, dc = O.assemble_colored(p, O.HEX, mesh.xyz, mesh.conn, u, order, cptr, uold, 0.01, s_m=150.0, parts=3,
+                                rowptr=rp, colidx=ci, threads=4)
+    Ja, da = O.assemble_abs(p, O.HEX, mesh.xyz, mesh.conn, u, uold, 0.01, s_m=150.0, parts=3, rowptr=rp, colidx=ci)
+    assert np.all(np.abs(Jc - Js) <= 1e-13 * Ja + 1e-300)
+    assert np.all(np.abs(dc - ds) <= 1e-13 * da + 1e-300)
