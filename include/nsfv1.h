@@ -344,19 +344,31 @@ int nsfv1_predict_guess(nsfv1_handle* h, const double* u, const double* u_prev, 
 int nsfv1_predict_target(nsfv1_handle* h, const double* u_next, const double* u, const double* u_prev,
                          const float* out_scale, float* t, void* stream);
 
-/* Assembly path: 0 (default) evaluates every SCVF once into a per-SCVF
-   record (FP32 Jacobian, common configuration: FIELDS, one upwind object,
-   no Stokes/PAC/Peclet/exact terms) and gathers rows from the records; 1
+/* Assembly path: 0 evaluates every SCVF once into a per-SCVF record (FP32
+   Jacobian, common configuration: FIELDS, one upwind object, no
+   Stokes/PAC/Peclet/exact terms) and gathers rows from the records; 1
    always runs the fused kernel (each SCVF evaluated by both of its rows);
    2 is path 0 with dim+1 gather lanes per (node, element), one per row,
-   when the launch has 64 threads (nsfv1_set_launch).  All are
+   when the launch has 64 threads (nsfv1_set_launch); 3 (default) marches
+   through the element layers of a structured grid of parallelepipeds
+   (a lexicographic hexahedral grid, detected by nsfv1_set_mesh) with the
+   SCVF records held on chip -- FP32 Jacobian, no/full/skewed upwind, RAW
+   diffusion length, one viscosity -- and otherwise behaves as 0.  All are
    deterministic.  The FLOW stabilisation and the positive upwind always
    use the coupled path.  No counterpart in the plugin. */
 int nsfv1_set_assembly_path(nsfv1_handle* h, int path);
 
+/* Kernel family that ran the last element assembly of the handle: 3 plane
+   march, 0 split (records + row gather), 2 split with row lanes, 1 fused,
+   4 coupled (FLOW / positive upwind); -1 before the first assembly.  No
+   counterpart in the plugin. */
+int nsfv1_assembly_path_used(nsfv1_handle* h, int* path);
+
 /* Launch shape of the assembly: incidences (threads) per workgroup (64..256,
    a multiple of 64) and the LDS budget per workgroup in bytes (4..64 KiB);
-   0 keeps the current value.  Used by the autotuner. */
+   0 keeps the current value.  Set while path 3 is selected, the budget in
+   KiB is instead the number of node planes per work unit of the plane
+   march.  Used by the autotuner. */
 int nsfv1_set_launch(nsfv1_handle* h, int threads, int lds_bytes);
 
 #ifdef __cplusplus

@@ -44,7 +44,7 @@ EXPORTS = [
     "nsfv1_set_preconditioner", "nsfv1_mg_set_hierarchy", "nsfv1_mg_setup", "nsfv1_set_upwind_decisions",
     "nsfv1_mlp_sizes", "nsfv1_mlp_pack", "nsfv1_mlp_forward", "nsfv1_mlp_mse", "nsfv1_mlp_backward", "nsfv1_mlp_adam",
     "nsfv1_predict_features", "nsfv1_predict_guess", "nsfv1_predict_target", "nsfv1_batch_set_instances",
-    "nsfv1_set_assembly_path", "nsfv1_set_outflow", "nsfv1_set_symmetry", "nsfv1_set_sliding_wall", "nsfv1_export_size",
+    "nsfv1_set_assembly_path", "nsfv1_assembly_path_used", "nsfv1_set_outflow", "nsfv1_set_symmetry", "nsfv1_set_sliding_wall", "nsfv1_export_size",
     "nsfv1_export",
 ]
 # data exports (navier_stokes_fv1.cpp:1573-1577) and their points
@@ -105,6 +105,7 @@ def load_library(path: str | None = None):
     L.nsfv1_set_upwind_decisions.argtypes = [H, C.c_int]
     L.nsfv1_batch_set_instances.argtypes = [H, C.c_int64, vp, vp]
     L.nsfv1_set_assembly_path.argtypes = [H, C.c_int]
+    L.nsfv1_assembly_path_used.argtypes = [H, C.POINTER(C.c_int)]
     L.nsfv1_set_outflow.argtypes = [H, vp, vp, C.c_int64]
     L.nsfv1_set_symmetry.argtypes = [H, vp, vp, C.c_int64]
     L.nsfv1_set_sliding_wall.argtypes = [H, vp, vp, C.c_int64, C.c_double, C.c_double]
@@ -347,6 +348,13 @@ class NavierStokesFV1:
 
     def set_assembly_path(self, path: int):
         _check(_lib.nsfv1_set_assembly_path(self._h, int(path)))
+
+    def assembly_path_used(self) -> int:
+        """Kernel family of the last assembly: 3 plane march, 0 split, 2 split
+        with row lanes, 1 fused, 4 coupled, -1 none yet."""
+        p = C.c_int(-1)
+        _check(_lib.nsfv1_assembly_path_used(self._h, C.byref(p)))
+        return p.value
 
     def batch_set_instances(self, elem_offset, nu):
         off = np.ascontiguousarray(elem_offset, dtype=np.int64)

@@ -130,12 +130,16 @@ def autotune(disc, run, candidates=None, reps: int = 5, log=None):
     shape, keep the fastest on disc and return (best, table of (path,
     threads, lds, ms)).  Path 0 evaluates each SCVF once into records, path 1
     is the fused kernel, path 2 is path 0 with dim+1 gather lanes per
-    (node, element) (64 threads only)."""
+    (node, element) (64 threads only), path 3 the plane march of structured
+    hexahedral grids."""
     import torch
 
     if candidates is None:
         candidates = [(p, t, l) for p in (0, 1) for t in (64, 128, 256) for l in (16384, 32768, 65536)]
         candidates += [(2, 64, l) for l in (16384, 65536)]
+        # the plane march (structured grids of parallelepipeds; elsewhere path
+        # 0): the budget in KiB is its node planes per work unit
+        candidates += [(3, 256, l) for l in (8192, 16384, 32768)]
     from . import NSFV1Error
 
     table = []

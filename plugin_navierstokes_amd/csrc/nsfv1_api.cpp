@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/nsfv1.h"
+#include "nsfv1_march.hpp"
 #include "nsfv1_internal.hpp"
 #include "nsfv1_solver.hpp"
 #include "nsfv1_mlp.hpp"
@@ -150,7 +151,14 @@ struct nsfv1_handle {
     int geo_difflen = -1;
     // split assembly path (per-SCVF records)
     DevBuf recs;
-    int path = 0;  // 0: split where supported, 1: fused kernel only
+    // 0: split where supported, 1: fused kernel only, 2: split with row lanes,
+    // 3: plane march on structured grids of parallelepipeds, otherwise 0
+    int path = 3;
+    // structured grid of parallelepipeds (plane-marching assembly path)
+    bool march_ok = false;
+    int march_n[3] = {0, 0, 0};
+    int march_zc = 0;  // node planes per work unit (0: chosen from the grid)
+    int last_path = -1;  // kernel family of the last element assembly (nsfv1_assembly_path_used)
     // batched instances: per-element viscosity
     DevBuf nu_elem;
     int64_t ninst = 0;
@@ -377,6 +385,59 @@ int nsfv1_set_pac_upwind(nsfv1_handle* h, int on)
 static int nsh_of(int et) { return et == ET_TRI ? 3 : (et == ET_QUAD || et == ET_TET) ? 4 : 8; }
 static int dim_of(int et) { return et <= ET_QUAD ? 2 : 3; }
 
+// A hexahedral mesh that is a lexicographic nx x ny x nz grid (node (i,j,k)
+// = i + (nx+1)(j + (ny+1)k), element (i,j,k) = i + nx(j + ny k) with the UG4
+// corner order) of parallelepipeds (the geometry cache's test, tolerance
+// 1e-12 of the element size), whose CSR rows are the grid's closed form.
+// Fills n[3] and returns true when all of it holds.
+static bool detect_structured_hex(const double* xyz, int64_t nv, const int32_t* conn, int64_t ne,
+                                  const std::vector<int64_t>& rowptr, int* n)
+{
+    if (ne < 1 || conn[0] != 0) return false;
+    const int64_t NX = conn[3], NXY = conn[4];
+    if (NX < 2 || NXY < 2 * NX || NXY % NX) return false;
+    const int64_t nx = NX - 1, ny = NXY / NX - 1;
+    if (ne % (nx * ny)) return false;
+    const int64_t nz = ne / (nx * ny);
+    if (nx > (1 << 20) || ny > (1 << 20) || nz > (1 << 20) || nv != NXY * (nz + 1)) return false;
+    const int64_t off[8] = {0, 1, NX + 1, NX, NXY, NXY + 1, NXY + NX + 1, NXY + NX};
+    for (int64_t k = 0, e = 0; k < nz; ++k)
+        for (int64_t j = 0; j < ny; ++j)
+            for (int64_t i = 0; i < nx; ++i, ++e) {
+                const int64_t v0 = i + NX * j + NXY * k;
+                const int32_t* c = conn + e * 8;
+                for (int q = 0; q < 8; ++q)
+                    if (c[q] != v0 + off[q]) return false;
+                double scale = 0.0, dev = 0.0;
+                const double* x[8];
+                for (int q = 0; q < 8; ++q) x[q] = xyz + 3 * (v0 + off[q]);
+                for (int q = 1; q < 8; ++q)
+                    for (int a = 0; a < 3; ++a) scale = std::max(scale, std::fabs(x[q][a] - x[0][a]));
+                for (int a = 0; a < 3; ++a) {
+                    dev = std::max(dev, std::fabs(x[1][a] + x[3][a] - x[0][a] - x[2][a]));
+                    dev = std::max(dev, std::fabs(x[1][a] + x[4][a] - x[0][a] - x[5][a]));
+                    dev = std::max(dev, std::fabs(x[3][a] + x[4][a] - x[0][a] - x[7][a]));
+                    dev = std::max(dev, std::fabs(x[1][a] + x[3][a] + x[4][a] - 2.0 * x[0][a] - x[6][a]));
+                }
+                if (!(dev <= 1e-12 * scale)) return false;
+            }
+    // rows: 4 per node, 4 ax ay az values each, nodes in order
+    const int64_t SX = 3 * nx + 1, SY = 3 * ny + 1;
+    for (int64_t k = 0, v = 0; k <= nz; ++k)
+        for (int64_t j = 0; j <= ny; ++j)
+            for (int64_t i = 0; i <= nx; ++i, ++v) {
+                const int64_t ax = 1 + (i > 0) + (i < nx), ay = 1 + (j > 0) + (j < ny), az = 1 + (k > 0) + (k < nz);
+                const int64_t sx = i > 0 ? 3 * i - 1 : 0, sy = j > 0 ? 3 * j - 1 : 0, sz = k > 0 ? 3 * k - 1 : 0;
+                const int64_t rb = 16 * (SX * SY * sz + SX * sy * az + sx * ay * az);
+                for (int r = 0; r <= 4; ++r)
+                    if (rowptr[v * 4 + r] != rb + r * 4 * ax * ay * az) return false;
+            }
+    n[0] = (int)nx;
+    n[1] = (int)ny;
+    n[2] = (int)nz;
+    return true;
+}
+
 int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const int32_t* conn, int64_t ne)
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
@@ -432,6 +493,8 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
     const int64_t nnz = rowptr[nv * nc];
     for (int64_t v = 0; v < nv; ++v)
         if (deg[v] > 255) return fail(NSFV1_ERR_ARGUMENT, "a node has more than 255 neighbours");
+    int mn[3] = {0, 0, 0};
+    const bool structured = et == ET_HEX && detect_structured_hex(xyz, nv, conn, ne, rowptr, mn);
     std::vector<int32_t> colidx, adjv;
     std::vector<uint64_t> ipos;
     try {
@@ -488,6 +551,8 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
     h->nv = nv;
     h->ne = ne;
     h->ndof = nv * nc;
+    h->march_ok = structured;
+    for (int a = 0; a < 3; ++a) h->march_n[a] = mn[a];
     h->nnz = nnz;
     h->ninc = ninc;
     h->h_rowptr.swap(rowptr);
@@ -553,7 +618,11 @@ int nsfv1_set_launch(nsfv1_handle* h, int threads, int lds_bytes)
         if (threads < 64 || threads > 256 || threads % 64) return fail(NSFV1_ERR_ARGUMENT, "threads must be 64..256, a multiple of 64");
         h->max_threads = threads;
     }
-    if (lds_bytes) {
+    if (lds_bytes && h->path == 3) {
+        // the plane march reads the budget in KiB as node planes per work unit
+        if (lds_bytes < 2048 || lds_bytes > 65536) return fail(NSFV1_ERR_ARGUMENT, "plane march: 2..64 node planes per work unit");
+        h->march_zc = lds_bytes / 1024;
+    } else if (lds_bytes) {
         if (lds_bytes < 4096 || lds_bytes > 65536) return fail(NSFV1_ERR_ARGUMENT, "LDS budget must be 4..64 KiB");
         h->lds_cap = (size_t)lds_bytes;
     }
@@ -762,7 +831,28 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
                 P.perm.as<int32_t>(), h->nu_elem.as<double>(), P.block, P.lane_a.as<int4>(),
                 P.lane_pos.as<uint64_t>(), P.wg_a.as<int4>()};
     hipError_t e;
-    if (coupled_config(h)) {
+    int mfast = -1;
+    if (h->path == 3 && h->march_ok && jprec == NSFV1_F32 && (parts & NSFV1_PART_A) && h->difflen == DL_RAW &&
+        !h->nu_elem.p && k.stab_type == STAB_FIELDS && !k.stokes && !k.pac && !k.peclet && k.exact == 0.0 &&
+        k.conv_upwind == k.stab_upwind && (k.conv_upwind == UPW_NO || k.conv_upwind == UPW_FULL || k.conv_upwind == UPW_SKEWED) &&
+        ((uintptr_t)jvals & 15) == 0 && ((uintptr_t)defect & 15) == 0)
+        mfast = k.conv_upwind;
+    if (mfast >= 0) {
+        MarchGrid g{};
+        g.nx = h->march_n[0];
+        g.ny = h->march_n[1];
+        g.nz = h->march_n[2];
+        g.NX = g.nx + 1;
+        g.NXY = g.NX * (g.ny + 1);
+        g.tiles_x = (g.nx + 1 + MARCH_TX - 1) / MARCH_TX;
+        g.tiles_y = (g.ny + 1 + MARCH_TY - 1) / MARCH_TY;
+        g.zc = h->march_zc > 0 ? std::min(std::max(h->march_zc, 2), g.nz + 1) : std::min(16, g.nz + 1);
+        g.nzc = (g.nz + 1 + g.zc - 1) / g.zc;
+        g.units = (int64_t)g.tiles_x * g.tiles_y * g.nzc;
+        e = launch_assemble_march(mfast, g, h->xyz.as<double>(), k, u, u_old, h->upw_dec.as<uint8_t>(),
+                                  static_cast<float*>(jvals), defect, h->errflag.as<int>(), (hipStream_t)stream);
+        h->last_path = 3;
+    } else if (coupled_config(h)) {
         if (h->upw_mode != NSFV1_UPWIND_FRESH)
             return fail(NSFV1_ERR_NOT_IMPLEMENTED, "recorded upwind decisions are not available with the FLOW "
                                                    "stabilization or the positive upwind");
@@ -771,7 +861,8 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
             return fail(NSFV1_ERR_DEVICE, "coupled record buffer");
         e = launch_assemble_coupled(h->et, jprec, m, k, u, u_old, jvals, defect, h->errflag.as<int>(), P.nwg, P.lds,
                                     h->ne, h->cpl_recs.p, nullptr, (hipStream_t)stream);
-    } else if ((h->path == 0 || h->path == 2) && jprec == NSFV1_F32 && (parts & NSFV1_PART_A) &&
+        h->last_path = 4;
+    } else if ((h->path == 0 || h->path == 2 || h->path == 3) && jprec == NSFV1_F32 && (parts & NSFV1_PART_A) &&
                split_supported(h->et, k)) {
         // also without a Jacobian (defect only): the gather then reads only
         // the records' fluxes, and the defect is bitwise the one assembled
@@ -780,9 +871,11 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
         if (h->recs.n < need && h->recs.alloc(need) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "record buffer");
         e = launch_assemble_split(h->et, m, k, u, u_old, static_cast<float*>(jvals), defect, h->errflag.as<int>(), P.nwg,
                                   P.lds, h->ne, h->recs.as<float>(), h->path == 2 && P.rowlane, (hipStream_t)stream);
+        h->last_path = (h->path == 2 && P.rowlane) ? 2 : 0;
     } else {
         e = launch_assemble(h->et, jprec, m, k, u, u_old, jvals, defect, h->errflag.as<int>(), P.nwg, P.lds,
                             (hipStream_t)stream);
+        h->last_path = 1;
     }
     if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("assemble launch: ") + hipGetErrorString(e));
     for (int kind = 0; kind < BND_KINDS; ++kind) {
@@ -1015,10 +1108,18 @@ int nsfv1_export(nsfv1_handle* h, int what, int at, const double* local_ips, int
 int nsfv1_set_assembly_path(nsfv1_handle* h, int path)
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
-    if (path < 0 || path > 2)
-        return fail(NSFV1_ERR_ARGUMENT, "path must be 0 (split where supported), 1 (fused) or 2 (split, row lanes)");
+    if (path < 0 || path > 3)
+        return fail(NSFV1_ERR_ARGUMENT, "path must be 0 (split where supported), 1 (fused), 2 (split, row lanes) or "
+                                        "3 (plane march where supported)");
     if (path != h->path) h->plan32.valid = h->plan64.valid = false;  // the row-lane plan packs fewer incidences
     h->path = path;
+    return NSFV1_OK;
+}
+
+int nsfv1_assembly_path_used(nsfv1_handle* h, int* path)
+{
+    if (!h || !path) return fail(NSFV1_ERR_ARGUMENT, "NULL argument");
+    *path = h->last_path;
     return NSFV1_OK;
 }
 

@@ -61,6 +61,8 @@ __device__ inline void elem_centre(const double (&co)[Topo<ET>::nsh][Topo<ET>::d
 __device__ inline bool ray_tri_nc(const double* p0, const double* p1, const double* p2, const double* from,
                                   const double* dir, double& b1o, double& b2o)
 {
+    // no FMA contraction: the oracle's rounding, operation for operation
+#pragma clang fp contract(off)
     double e1[3], e2[3], pv[3], tv[3], qv[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) { e1[i] = p1[i] - p0[i]; e2[i] = p2[i] - p0[i]; }
@@ -85,6 +87,7 @@ template <int ET, int S, int TR>
 __device__ inline void try_side(const double (&co)[Topo<ET>::nsh][Topo<ET>::dim], const double* from,
                                 const double* dir, int& side, double* gi, double* li)
 {
+#pragma clang fp contract(off)
     using T = Topo<ET>;
     if (side >= 0) return;
     if constexpr (T::dim == 2) {

@@ -53,6 +53,7 @@ def test_unknown_names(cuda):
     _raises("UNKNOWN_NAME", disc.set_stabilization, "supg", match="not a valid name of a Schneider-Raw stabilization")
     _raises("UNKNOWN_NAME", disc.set_stabilization, "fields", "cubic",
             match="Diffusion Length calculation method not found")
+    disc.set_stabilization("fields")
     _raises("UNKNOWN_NAME", disc.set_stabilization_upwind, "weighted", match="not found")
     # the accepted spellings: trimmed and case-insensitive
     disc.set_upwind("  LinearProfileSkewed ")

@@ -1,0 +1,181 @@
+"""The plane-march assembly of structured hexahedral grids (assembly path 3)
+against the FP64 oracle, through the C ABI.
+
+The march serves lexicographic grids of parallelepipeds with FP32 Jacobian,
+FIELDS + one no/full/skewed upwind object and the RAW diffusion length.
+The grids here have cell counts that are not multiples of the 7 x 8 node
+tile and node-plane chunks of 2 to 16 planes, so tiles and chunks end inside
+the grid and at its faces; a sheared grid has non-orthogonal parallelepipeds.
+Every entry of J (FP32: |a-b| <= 2^-17 S + 2^-20 |b|) and d (FP64:
+|a-b| <= 1e-12 |b| + 1e-13 S) is checked against the oracle, S being the
+sum of the magnitudes of the element contributions to the entry."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+from plugin_navierstokes_amd import F32, PART_A, PART_M, PART_RHS, NavierStokesFV1, UPWIND_FRESH, UPWIND_RECORD, \
+    UPWIND_REPLAY
+from plugin_navierstokes_amd import meshes as M
+from tests._parity import FP32_ABS, FP32_REL, configure_disc, fp32_violation, fp64_violation, oracle_params, worst
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid(cells, shear=False):
+    mesh = M.box_3d(cells, (1.0, 0.8, 1.3))
+    if shear:
+        A = np.array([[1.0, 0.3, 0.1], [0.0, 1.1, -0.2], [0.05, 0.0, 0.9]])
+        mesh.xyz = mesh.xyz @ A.T
+    return mesh
+
+
+def _run(cuda, mesh, cfg, seed, time_dep, parts, s, zc=0, path=3):
+    import torch
+
+    disc = NavierStokesFV1("u,v,w,p", "Inner")
+    configure_disc(disc, cfg)
+    disc.set_mesh(mesh)
+    disc.set_assembly_path(path)
+    if zc:
+        disc.set_launch(0, zc * 1024)
+    u = M.random_state(mesh, seed)
+    uold = M.random_state(mesh, seed + 100) if time_dep else None
+    dt = 0.01 if time_dep else 0.0
+    ut = torch.from_numpy(u).to(cuda)
+    uot = torch.from_numpy(uold).to(cuda) if time_dep else None
+    J, d = disc.assemble(ut, uot, dt, s_a=s[0], s_m=s[1], s_f=s[2], parts=parts, jprec=F32)
+    used = disc.assembly_path_used()
+    rowptr, colidx = disc.pattern()
+    p = oracle_params(3, cfg)
+    Jr, dr = O.assemble(p, O.HEX, mesh.xyz, mesh.conn, u, uold, dt, s_a=s[0], s_m=s[1], s_f=s[2], parts=parts,
+                        rowptr=rowptr, colidx=colidx)
+    Ja, da = O.assemble_abs(p, O.HEX, mesh.xyz, mesh.conn, u, uold, dt, s_a=s[0], s_m=s[1], parts=parts,
+                            rowptr=rowptr, colidx=colidx)
+    if parts & PART_RHS and cfg.get("source") is not None:
+        da = da + np.abs(dr)
+    return J.cpu().numpy(), d.cpu().numpy(), Jr, dr, Ja, da, used, disc
+
+
+CASES = [
+    # (cells, shear, cfg, time_dep, parts, (s_a, s_m, s_f), planes per unit)
+    ((3, 3, 3), False, {"upwind": "skewed"}, True, PART_A, (1.0, 0.0, 0.0), 0),
+    ((9, 10, 11), False, {"upwind": "skewed"}, True, PART_A | PART_M, (1.0, 150.0, 0.0), 4),
+    ((9, 10, 11), True, {"upwind": "skewed"}, True, PART_A | PART_M, (0.8, 150.0, 0.0), 16),
+    ((15, 4, 7), False, {"upwind": "full"}, False, PART_A, (1.0, 0.0, 0.0), 3),
+    ((15, 4, 7), True, {"upwind": "full", "nu": 0.002}, True, PART_A | PART_M, (1.0, 2.0, 0.0), 2),
+    ((8, 17, 6), False, {"upwind": "no"}, True, PART_A | PART_M, (1.0, 2.0, 0.0), 5),
+    ((8, 17, 6), True, {"upwind": "skewed", "laplace": True}, True, PART_A, (1.0, 0.0, 0.0), 0),
+    ((7, 8, 9), False, {"upwind": "skewed", "source": (0.2, -0.1, 0.3)}, True, PART_A | PART_M | PART_RHS,
+     (0.8, 1.5, -0.5), 7),
+    ((1, 1, 1), False, {"upwind": "skewed"}, True, PART_A, (1.0, 0.0, 0.0), 0),
+    ((2, 1, 3), True, {"upwind": "full"}, False, PART_A | PART_M, (1.0, 3.0, 0.0), 2),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_march_vs_oracle(cuda, case):
+    cells, shear, cfg, td, parts, s, zc = CASES[case]
+    print(f"case {case}: cells={cells} shear={shear} cfg={cfg} time_dep={td} parts={parts} s={s} zc={zc}",
+          flush=True)
+    mesh = _grid(cells, shear)
+    J, d, Jr, dr, Ja, da, used, _ = _run(cuda, mesh, cfg, 5 + case, td, parts, s, zc)
+    assert used == 3, f"the plane march did not run (path {used})"
+    assert fp32_violation(J, Jr, Ja) <= 1.0, worst(J, Jr, Ja, FP32_REL, FP32_ABS)
+    assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+
+
+def test_march_defect_only_and_repeatable(cuda):
+    """The defect of a defect-only assembly is bitwise the one assembled with
+    the Jacobian, and every assembly repeats bitwise."""
+    import torch
+
+    mesh = _grid((9, 10, 11), True)
+    disc = NavierStokesFV1("u,v,w,p", "Inner")
+    configure_disc(disc, {"upwind": "skewed"})
+    disc.set_mesh(mesh)
+    u = torch.from_numpy(M.random_state(mesh, 3)).to(cuda)
+    uo = 0.9 * u
+    J1, d1 = disc.assemble(u, uo, 0.02, s_m=1.0, parts=PART_A | PART_M, jprec=F32)
+    assert disc.assembly_path_used() == 3
+    J2, d2 = disc.assemble(u, uo, 0.02, s_m=1.0, parts=PART_A | PART_M, jprec=F32)
+    _, d3 = disc.assemble(u, uo, 0.02, s_m=1.0, parts=PART_A | PART_M, jprec=F32, want_J=False)
+    assert torch.equal(J1, J2) and torch.equal(d1, d2) and torch.equal(d1, d3)
+
+
+def test_march_matches_split_path(cuda):
+    """Path 3 and path 0 (SCVF records in HBM, row gather) compute the same
+    defect expressions (the diffusion length's volumes come from |det J|/8
+    here and from the geometry cache there) and agree to FP64 rounding; the
+    Jacobians agree to FP32 rounding."""
+    mesh = _grid((10, 9, 8), True)
+    cfg = {"upwind": "skewed"}
+    out = {}
+    for path in (3, 0):
+        J, d, Jr, dr, Ja, da, used, _ = _run(cuda, mesh, cfg, 21, True, PART_A | PART_M, (1.0, 150.0, 0.0), 0, path)
+        assert used == path
+        out[path] = (J, d)
+    J3, d3 = out[3]
+    J0, d0 = out[0]
+    assert fp64_violation(d3, d0, da) <= 1.0, worst(d3, d0, da)
+    assert np.max(np.abs(J3.astype(np.float64) - J0) / (2.0 ** -20 * (Ja + np.abs(J0)) + 1e-300)) <= 1.0
+
+
+def test_march_upwind_record_replay(cuda):
+    """Skewed-upwind decisions recorded by the march replay bitwise at the same
+    state, and replayed at another state they equal the split path's replay
+    of its own recording (the same ray casts)."""
+    import torch
+
+    mesh = _grid((9, 10, 11), False)
+    res = {}
+    for path in (3, 0):
+        disc = NavierStokesFV1("u,v,w,p", "Inner")
+        configure_disc(disc, {"upwind": "skewed", "nu": 0.01})
+        disc.set_mesh(mesh)
+        disc.set_assembly_path(path)
+        u = torch.from_numpy(M.random_state(mesh, 7)).to(cuda)
+        u2 = torch.from_numpy(M.random_state(mesh, 8)).to(cuda)
+        J0, d0 = disc.assemble(u, 0.9 * u, 0.01, jprec=F32)
+        disc.set_upwind_decisions(UPWIND_RECORD)
+        J1, d1 = disc.assemble(u, 0.9 * u, 0.01, jprec=F32)
+        disc.set_upwind_decisions(UPWIND_REPLAY)
+        J2, d2 = disc.assemble(u, 0.9 * u, 0.01, jprec=F32)
+        assert disc.assembly_path_used() == path
+        assert torch.equal(J0, J1) and torch.equal(d0, d1) and torch.equal(J0, J2) and torch.equal(d0, d2)
+        J3, d3 = disc.assemble(u2, 0.9 * u2, 0.01, jprec=F32)
+        disc.set_upwind_decisions(UPWIND_FRESH)
+        J4, d4 = disc.assemble(u2, 0.9 * u2, 0.01, jprec=F32)
+        assert not torch.equal(d3, d4)
+        disc.sync_errors()
+        res[path] = (J3.cpu().numpy(), d3.cpu().numpy(), np.abs(d3.cpu().numpy()))
+    J3, d3, a3 = res[3]
+    J0, d0, a0 = res[0]
+    assert np.max(np.abs(d3 - d0) / (1e-12 * (a3 + a0) + 1e-300)) <= 1.0
+    # entries that cancel to (nearly) zero carry rounding noise of the size of
+    # their terms: an absolute floor at FP32 rounding of the matrix scale
+    tol = 2.0 ** -17 * (np.abs(J3) + np.abs(J0)) + 2.0 ** -22 * np.max(np.abs(J0))
+    assert np.max(np.abs(J3.astype(np.float64) - J0) / tol) <= 1.0
+
+
+def test_march_not_taken_outside_its_scope(cuda):
+    """Jittered grids, LPS, FP64 Jacobians and other diffusion lengths take
+    the other paths."""
+    import torch
+
+    from plugin_navierstokes_amd import F64
+
+    for mesh, cfg, jprec, want in [
+        (M.cavity_3d(4, M.HEX, 0.1, 2), {"upwind": "skewed"}, F32, 0),
+        (_grid((4, 5, 6)), {"upwind": "lps"}, F32, 0),
+        (_grid((4, 5, 6)), {"upwind": "skewed"}, F64, 1),
+        (_grid((4, 5, 6)), {"upwind": "skewed", "difflen": "cor"}, F32, 0),
+        (_grid((4, 5, 6)), {"upwind": "skewed"}, F32, 3),
+    ]:
+        disc = NavierStokesFV1("u,v,w,p", "Inner")
+        configure_disc(disc, cfg)
+        disc.set_mesh(mesh)
+        u = torch.from_numpy(M.random_state(mesh, 1)).to(cuda)
+        disc.assemble(u, jprec=jprec)
+        assert disc.assembly_path_used() == want, (cfg, jprec)
