@@ -19,7 +19,9 @@ entry against the FP64 oracle on the same state:
 
 Tolerances: FP64 |a-b| <= 1e-12|b| + 1e-13 S; FP32 J |a-b| <= 2^-17 S +
 2^-20 |b| (S = sum of the magnitudes of the element contributions to the
-entry, from the oracle's assemble_abs)."""
+entry, from the oracle's assemble_abs).  The FP64 LPS Jacobian of the
+triangle workload adds 8 kappa, kappa being the oracle's own change under
+1-ulp perturbations of its inputs (ill-conditioned ray cuts)."""
 from __future__ import annotations
 
 import json
@@ -141,6 +143,21 @@ def test_c1_fp64_and_fp32_every_entry(cuda, tri512):
     p = O.Params(dim=2, upwind="lps", nu=nu)
     Jr, dr = O.assemble(p, O.TRI, mesh.xyz, mesh.conn, u, rowptr=rp, colidx=ci)
     Ja, da = O.assemble_abs(p, O.TRI, mesh.xyz, mesh.conn, u, rowptr=rp, colidx=ci)
+    # LPS places its upwind point where the velocity ray cuts the element
+    # boundary; near a corner that cut is ill-conditioned, and flipping the
+    # last bit of the state and the coordinates moves some oracle entries by
+    # up to twice the FP64 tolerance.  The oracle's own spread under three
+    # such 1-ulp perturbations, kappa, is added to the FP64 Jacobian
+    # tolerance with a factor 8 (GPU and oracle round their intermediates
+    # differently).  kappa is ~1e-13 S where an entry is ill-conditioned and
+    # far below for most entries, so a wrong term still fails by orders.
+    rng = np.random.default_rng(0)
+    kappa = np.zeros_like(Jr)
+    for _ in range(3):
+        up = u * (1.0 + rng.choice([-1.0, 1.0], size=u.shape) * 2.0 ** -52)
+        xp = mesh.xyz * (1.0 + rng.choice([-1.0, 1.0], size=mesh.xyz.shape) * 2.0 ** -52)
+        Jp, _ = O.assemble(p, O.TRI, xp, mesh.conn, up, rowptr=rp, colidx=ci)
+        kappa = np.maximum(kappa, np.abs(Jp - Jr))
     for jprec in (F64, F32):
         J, d = disc.assemble(ut, jprec=jprec)
         Jg, dg = J.cpu().numpy().astype(np.float64), d.cpu().numpy()
@@ -153,7 +170,10 @@ def test_c1_fp64_and_fp32_every_entry(cuda, tri512):
               f"violations J {vj:.3g} d {vd:.3g}", flush=True)
         assert vd <= 1.0, worst(dg, dr, da)
         if jprec == F64:
-            assert vj <= 1.0, worst(Jg, Jr, Ja)
+            tol = 1e-12 * np.abs(Jr) + 1e-13 * Ja + 8.0 * kappa + 1e-300
+            vk = float(np.max(np.abs(Jg - Jr) / tol))
+            print(f"c1 FP64 J against tolerance + 8 kappa: violation {vk:.3g}", flush=True)
+            assert vk <= 1.0, worst(Jg, Jr, Ja)
         else:
             assert vj <= 1.0, worst(Jg, Jr, Ja, 2.0 ** -20, 2.0 ** -17)
 
