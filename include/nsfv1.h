@@ -161,6 +161,29 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
                    double s_a, double s_m, double s_f, int parts,
                    void* jvals, int jprec, double* defect, void* stream);
 
+/* nsfv1_assemble that allocates no device memory.  No counterpart in the
+   reference: it is for callers that cannot call hipMalloc while they
+   assemble (code under HIP graph capture, tight serving loops, harnesses
+   that forbid allocation).  Same arguments, dispatch and results (bitwise)
+   as nsfv1_assemble; the SCVF records that the split and coupled paths
+   keep between their two kernels live in a fixed device array of
+   NSFV1_NOALLOC_SCRATCH_BYTES (one per device, held until the process
+   ends) instead of a buffer the handle allocates.  Returns
+   NSFV1_ERR_ARGUMENT, before any launch, when the mesh's records exceed
+   that array (split path: 128 bytes per SCVF, so about 87 000 hexahedra,
+   262 000 quadrilaterals or 349 000 triangles), or when the handle has no
+   launch plan for jprec or, in NSFV1_UPWIND_RECORD mode, no decision buffer
+   yet (one nsfv1_assemble call with the same settings builds them).  The
+   structured-hexahedra plane march and the fused path (tetrahedra, FP64
+   Jacobians) keep no records and have no size limit.
+   Nothing in the array has to start at zero.  All calls on one device share
+   its array: calls that overlap on one device (other streams, other host
+   threads) race. */
+#define NSFV1_NOALLOC_SCRATCH_BYTES (128ull << 20)
+int nsfv1_assemble_noalloc(nsfv1_handle* h, const double* u, const double* u_old, double dt,
+                           double s_a, double s_m, double s_f, int parts,
+                           void* jvals, int jprec, double* defect, void* stream);
+
 /* Batched instances: the handle's mesh is the disjoint union of ninst
    independent problems (e.g. cavities of different sizes); instance i owns
    elements [elem_offset[i], elem_offset[i+1]) and has kinematic viscosity

@@ -39,7 +39,8 @@ EXPORTS = [
     "nsfv1_set_stokes", "nsfv1_set_laplace", "nsfv1_set_peclet_blend", "nsfv1_set_grad_div",
     "nsfv1_set_upwind", "nsfv1_set_stabilization", "nsfv1_set_stabilization_without",
     "nsfv1_set_stabilization_upwind", "nsfv1_set_pac_upwind", "nsfv1_set_mesh", "nsfv1_num_dofs", "nsfv1_nnz",
-    "nsfv1_get_pattern", "nsfv1_pattern_device", "nsfv1_assemble", "nsfv1_sync_errors", "nsfv1_set_launch",
+    "nsfv1_get_pattern", "nsfv1_pattern_device", "nsfv1_assemble", "nsfv1_assemble_noalloc", "nsfv1_sync_errors",
+    "nsfv1_set_launch",
     "nsfv1_set_dirichlet", "nsfv1_apply_dirichlet", "nsfv1_spmv", "nsfv1_linear_solve",
     "nsfv1_set_preconditioner", "nsfv1_mg_set_hierarchy", "nsfv1_mg_setup", "nsfv1_set_upwind_decisions",
     "nsfv1_mlp_sizes", "nsfv1_mlp_pack", "nsfv1_mlp_forward", "nsfv1_mlp_mse", "nsfv1_mlp_backward", "nsfv1_mlp_adam",
@@ -51,6 +52,8 @@ EXPORTS = [
 EX = {"velocity": 0, "velocity_grad": 1, "velocity_ip": 2, "pressure": 3, "pressure_grad": 4}
 AT = {"scvf": 0, "scv": 1, "local": 2}
 UPWIND_FRESH, UPWIND_RECORD, UPWIND_REPLAY = 0, 1, 2
+# device scratch of nsfv1_assemble_noalloc (NSFV1_NOALLOC_SCRATCH_BYTES)
+NOALLOC_SCRATCH_BYTES = 128 << 20
 
 
 class NSFV1Error(RuntimeError):
@@ -94,6 +97,7 @@ def load_library(path: str | None = None):
     L.nsfv1_get_pattern.argtypes = [H, vp, vp]
     L.nsfv1_pattern_device.argtypes = [H, C.POINTER(vp), C.POINTER(vp)]
     L.nsfv1_assemble.argtypes = [H, vp, vp, dp, dp, dp, dp, C.c_int, vp, C.c_int, vp, vp]
+    L.nsfv1_assemble_noalloc.argtypes = L.nsfv1_assemble.argtypes
     L.nsfv1_sync_errors.argtypes = [H, vp]
     L.nsfv1_set_launch.argtypes = [H, C.c_int, C.c_int]
     L.nsfv1_set_dirichlet.argtypes = [H, vp, C.c_int64]
@@ -268,20 +272,24 @@ class NavierStokesFV1:
 
     # --- assembly ---
     def assemble(self, u, u_old=None, dt=0.0, s_a=1.0, s_m=0.0, s_f=0.0, parts=PART_A, jprec=F64,
-                 J=None, d=None, want_J=True, want_d=True, stream=None, check=True):
+                 J=None, d=None, want_J=True, want_d=True, stream=None, check=True, noalloc=False):
         """Assemble J = s_a J_A + s_m J_M and d = s_a d_A + s_m d_M + s_f f.
 
-        ``u``/``u_old``: CUDA float64 tensors [nv, dim+1]; returns (J, d)."""
+        ``u``/``u_old``: CUDA float64 tensors [nv, dim+1]; returns (J, d).
+        ``noalloc`` calls the entry that allocates no device memory (J and d
+        must then be given when wanted)."""
         import torch
 
         dev = u.device
+        if noalloc and ((want_J and J is None) or (want_d and d is None)):
+            raise ValueError("the no-allocation assembly writes into given J and d only")
         if want_J and J is None:
             J = torch.empty(self.nnz(), dtype=torch.float32 if jprec == F32 else torch.float64, device=dev)
         if want_d and d is None:
             d = torch.empty(self.num_dofs(), dtype=torch.float64, device=dev)
-        _check(_lib.nsfv1_assemble(self._h, _ptr(u), _ptr(u_old), float(dt), float(s_a), float(s_m), float(s_f),
-                                   int(parts), _ptr(J) if want_J else None, int(jprec), _ptr(d) if want_d else None,
-                                   _stream_ptr(stream)))
+        fn = _lib.nsfv1_assemble_noalloc if noalloc else _lib.nsfv1_assemble
+        _check(fn(self._h, _ptr(u), _ptr(u_old), float(dt), float(s_a), float(s_m), float(s_f), int(parts),
+                  _ptr(J) if want_J else None, int(jprec), _ptr(d) if want_d else None, _stream_ptr(stream)))
         if check:
             self.sync_errors(stream)
         return (J if want_J else None), (d if want_d else None)

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libnsfv1.so")
 
-SOURCES = ["nsfv1_api.cpp", "nsfv1_assemble.hip", "nsfv1_march.hip", "nsfv1_solver.hip", "nsfv1_mlp.hip", "nsfv1_boundary.hip", "nsfv1_export.hip", "nsfv1_coupled.hip"]
+SOURCES = ["nsfv1_api.cpp", "nsfv1_assemble.hip", "nsfv1_march.hip", "nsfv1_solver.hip", "nsfv1_mlp.hip", "nsfv1_boundary.hip", "nsfv1_export.hip", "nsfv1_coupled.hip", "nsfv1_scratch.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NSFV1_ARCH", "gfx950")

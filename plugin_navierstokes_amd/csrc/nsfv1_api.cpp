@@ -204,6 +204,9 @@ int nsfv1_create(int dim, const char* functions, const char* subsets, nsfv1_hand
     h->subsets.push_back(subsets ? subsets : "");
     if (h->errflag.alloc(sizeof(int) * KERR_WORDS) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "hipMalloc");
     if (hipMemset(h->errflag.p, 0, sizeof(int) * KERR_WORDS) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "hipMemset");
+    // device address of the no-allocation entry's scratch, once per device;
+    // a failure here is reported by that entry, which retries the lookup
+    (void)noalloc_scratch_setup(dev);
     *out = h.release();
     return NSFV1_OK;
 }
@@ -768,8 +771,13 @@ static int check_setup(nsfv1_handle* h)
     return NSFV1_OK;
 }
 
-int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double dt, double s_a, double s_m,
-                   double s_f, int parts, void* jvals, int jprec, double* defect, void* stream)
+// Body of nsfv1_assemble and nsfv1_assemble_noalloc.  With `scratch` set
+// (the no-allocation entry) the record buffers of the split and coupled
+// paths come from that fixed device array of NOALLOC_SCRATCH_BYTES, and
+// every state the call would otherwise allocate (launch plan, upwind
+// record buffer) must exist already.
+static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, double dt, double s_a, double s_m,
+                         double s_f, int parts, void* jvals, int jprec, double* defect, void* stream, void* scratch)
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
     if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
@@ -788,6 +796,9 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
     }
     Plan& P = jprec == NSFV1_F64 ? h->plan64 : h->plan32;
     if (!P.valid) {
+        if (scratch)
+            return fail(NSFV1_ERR_ARGUMENT, "no launch plan for this Jacobian precision yet: the no-allocation entry "
+                                            "needs one nsfv1_assemble call with the same jprec first (it builds the plan)");
         int rc = make_plan(h, jprec == NSFV1_F64 ? 8 : 4, P);
         if (rc) return rc;
     }
@@ -821,6 +832,9 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
     if (h->upw_mode == NSFV1_UPWIND_REPLAY && !h->upw_recorded)
         return fail(NSFV1_ERR_ARGUMENT, "no recorded upwind decisions to replay");
     if (h->upw_mode == NSFV1_UPWIND_RECORD && h->upw_dec.p == nullptr) {
+        if (scratch)
+            return fail(NSFV1_ERR_ARGUMENT, "no upwind record buffer yet: the no-allocation entry needs one "
+                                            "nsfv1_assemble call in NSFV1_UPWIND_RECORD mode first (it allocates it)");
         const int nip = h->et == 0 ? 3 : (h->et == 1 ? 4 : (h->et == 2 ? 6 : 12));
         if (h->upw_dec.alloc((size_t)h->ne * nip) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "upwind record buffer");
     }
@@ -857,10 +871,19 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
             return fail(NSFV1_ERR_NOT_IMPLEMENTED, "recorded upwind decisions are not available with the FLOW "
                                                    "stabilization or the positive upwind");
         const size_t need = (size_t)coupled_record_bytes(h->et, jprec, h->ne);
-        if (h->cpl_recs.n < need && h->cpl_recs.alloc(need) != hipSuccess)
-            return fail(NSFV1_ERR_DEVICE, "coupled record buffer");
+        void* recs = scratch;
+        if (scratch) {
+            if (need > NOALLOC_SCRATCH_BYTES)
+                return fail(NSFV1_ERR_ARGUMENT, "the coupled path needs " + std::to_string(need) +
+                                                    " bytes of SCVF records for this mesh; the no-allocation entry holds at most " +
+                                                    std::to_string(NOALLOC_SCRATCH_BYTES));
+        } else {
+            if (h->cpl_recs.n < need && h->cpl_recs.alloc(need) != hipSuccess)
+                return fail(NSFV1_ERR_DEVICE, "coupled record buffer");
+            recs = h->cpl_recs.p;
+        }
         e = launch_assemble_coupled(h->et, jprec, m, k, u, u_old, jvals, defect, h->errflag.as<int>(), P.nwg, P.lds,
-                                    h->ne, h->cpl_recs.p, nullptr, (hipStream_t)stream);
+                                    h->ne, recs, nullptr, (hipStream_t)stream);
         h->last_path = 4;
     } else if ((h->path == 0 || h->path == 2 || h->path == 3) && jprec == NSFV1_F32 && (parts & NSFV1_PART_A) &&
                split_supported(h->et, k)) {
@@ -868,9 +891,18 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
         // the records' fluxes, and the defect is bitwise the one assembled
         // together with the Jacobian
         const size_t need = sizeof(float) * (size_t)split_record_floats(h->et, h->ne);
-        if (h->recs.n < need && h->recs.alloc(need) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "record buffer");
+        float* recs = static_cast<float*>(scratch);
+        if (scratch) {
+            if (need > NOALLOC_SCRATCH_BYTES)
+                return fail(NSFV1_ERR_ARGUMENT, "the split path needs " + std::to_string(need) +
+                                                    " bytes of SCVF records for this mesh; the no-allocation entry holds at most " +
+                                                    std::to_string(NOALLOC_SCRATCH_BYTES));
+        } else {
+            if (h->recs.n < need && h->recs.alloc(need) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "record buffer");
+            recs = h->recs.as<float>();
+        }
         e = launch_assemble_split(h->et, m, k, u, u_old, static_cast<float*>(jvals), defect, h->errflag.as<int>(), P.nwg,
-                                  P.lds, h->ne, h->recs.as<float>(), h->path == 2 && P.rowlane, (hipStream_t)stream);
+                                  P.lds, h->ne, recs, h->path == 2 && P.rowlane, (hipStream_t)stream);
         h->last_path = (h->path == 2 && P.rowlane) ? 2 : 0;
     } else {
         e = launch_assemble(h->et, jprec, m, k, u, u_old, jvals, defect, h->errflag.as<int>(), P.nwg, P.lds,
@@ -890,6 +922,26 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
     }
     if (h->upw_mode == NSFV1_UPWIND_RECORD && (parts & NSFV1_PART_A)) h->upw_recorded = true;
     return NSFV1_OK;
+}
+
+int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double dt, double s_a, double s_m,
+                   double s_f, int parts, void* jvals, int jprec, double* defect, void* stream)
+{
+    return assemble_impl(h, u, u_old, dt, s_a, s_m, s_f, parts, jvals, jprec, defect, stream, nullptr);
+}
+
+int nsfv1_assemble_noalloc(nsfv1_handle* h, const double* u, const double* u_old, double dt, double s_a, double s_m,
+                           double s_f, int parts, void* jvals, int jprec, double* defect, void* stream)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    void* scratch = noalloc_scratch(h->device);
+    if (!scratch) {
+        // the handle was created before the lookup could run on this device
+        hipError_t e = noalloc_scratch_setup(h->device);
+        if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("no-allocation scratch: ") + hipGetErrorString(e));
+        scratch = noalloc_scratch(h->device);
+    }
+    return assemble_impl(h, u, u_old, dt, s_a, s_m, s_f, parts, jvals, jprec, defect, stream, scratch);
 }
 
 int nsfv1_batch_set_instances(nsfv1_handle* h, int64_t ninst, const int64_t* elem_offset, const double* nu)

@@ -60,6 +60,13 @@ hipError_t launch_assemble_coupled(int et, int jprec, const KMeshPtrs& m, const 
                                    const double* uold, void* J, double* d, int* err, int nwg, size_t lds, int64_t ne,
                                    void* recs, double* vel_out, hipStream_t stream);
 
+// fixed record scratch of nsfv1_assemble_noalloc (nsfv1_scratch.hip):
+// setup looks its device address up once per device (with that device
+// current); noalloc_scratch returns it, or null before setup
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(128) << 20;
+hipError_t noalloc_scratch_setup(int device);
+void* noalloc_scratch(int device);
+
 hipError_t launch_geometry_cache(int et, const double* xyz, const int32_t* conn, int64_t ne, int difflen,
                                  double* l2inv, double* vol, uint8_t* affine, int* err, hipStream_t stream);
 
