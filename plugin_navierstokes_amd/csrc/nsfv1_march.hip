@@ -563,6 +563,8 @@ struct Pair {
     float sn[3];          // the normal N_q points away from the row node (+1) or toward it (-1)
     float gl[3][3];       // reference gradient of shape kw at each SCVF's ip
     float phi[3];         // shape kw at each SCVF's ip
+    float phs[3];         // phi sn
+    float phg[3];         // phi sg
 };
 struct PairTabs {
     Pair p[8][8];
@@ -604,6 +606,8 @@ constexpr PairTabs make_pairs()
                             P.sn[q] = pq[q] ? -1.f : 1.f;
                             for (int r = 0; r < 3; ++r) P.gl[q][r] = float(TabH<ET_HEX>::t.gl[ip][P.kw][r]);
                             P.phi[q] = float(TabH<ET_HEX>::t.phi[ip][P.kw]);
+                            P.phs[q] = P.phi[q] * P.sn[q];
+                            P.phg[q] = P.phi[q] * P.sg[q];
                         }
                         P.first = 0;
                         P.last = 0;
@@ -624,21 +628,98 @@ struct RowCtx {
     int ax, ay, x0, y0, z0;
 };
 
-// The family t of blocks of the node's rows (navier_stokes_fv1.cpp:317-593
-// for FIELDS with one upwind object): per SCVF at the node, diffusion,
-// pressure, convection through the upwind corner and continuity through the
-// stabilisation's shapes; the mass part on the centre block's diagonal.
-template <int FAST>
-__device__ inline void row_family(int t, const Smem& S, const MarchGrid& G, const RowCtx& X, const KParams& P,
-                                  float nr, float rho, float* __restrict__ J)
+// LDS data of one (block, element) pair of a node row: the element's
+// inverse Jacobian and SCVF normals, the records and upwind corners of its
+// three SCVFs at the node.  Every index stays inside the LDS arrays also for
+// elements outside the grid (their values are then unused).
+struct PairData {
+    float Ji[3][3];
+    float N[3][3];
+    float4 R[3];
+    int id[3];
+};
+
+__device__ inline void load_pair(const Smem& S, const Pair& Q, const RowCtx& X, PairData& D)
 {
-    const bool lap = P.laplace != 0;
-    const float sa = float(P.s_a);
+    const int layer = X.k + (Q.sz < 0 ? -1 : 0);
+    const int col = X.li + (Q.sx < 0 ? 0 : 1) + CX * (X.lj + (Q.sy < 0 ? 0 : 1));
+    const int sl = layer & 1;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            D.Ji[r][c] = S.ejn[sl][3 * r + c][col];
+            D.N[r][c] = S.ejn[sl][9 + 3 * r + c][col];
+        }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        D.R[q] = S.rec[sl][Q.ip[q]][col];
+        D.id[q] = S.idx[sl][Q.ip[q]][col];
+    }
+}
+
+// Pair contributions to the open block B (navier_stokes_fv1.cpp:317-593 for
+// FIELDS with one upwind object): per SCVF at the node, diffusion,
+// pressure, convection through the upwind corner and continuity through
+// the stabilisation's shapes.  The normal of the SCVF is sn N_q (sn = +-1
+// orients it away from the node); sn and the edge sign sg are folded into
+// the scalar factors, and row 3 is accumulated without the density, which
+// is applied once per block when it is stored.
+template <int FAST, bool LAP>
+__device__ inline void add_pair(const PairData& D, const Pair& Q, float nr, float (&B)[4][4])
+{
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        float g[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) g[c] = Q.gl[q][0] * D.Ji[0][c] + Q.gl[q][1] * D.Ji[1][c] + Q.gl[q][2] * D.Ji[2][c];
+        const float gN = g[0] * D.N[q][0] + g[1] * D.N[q][1] + g[2] * D.N[q][2];
+        float Usg, Usn;
+        if constexpr (FAST == UPW_NO) {
+            Usg = Q.phg[q];
+            Usn = Q.phs[q];
+        } else {
+            const bool up = D.id[q] == Q.kw;
+            Usg = up ? Q.sg[q] : 0.f;
+            Usn = up ? Q.sn[q] : 0.f;
+        }
+        const float nrs = nr * Q.sn[q];
+        const float dg = nrs * gN + Usg * D.R[q].x;
+#pragma unroll
+        for (int d1 = 0; d1 < 3; ++d1) {
+            B[d1][d1] += dg;
+            if constexpr (!LAP) {
+                const float gd = nrs * g[d1];
+#pragma unroll
+                for (int d2 = 0; d2 < 3; ++d2) B[d1][d2] += gd * D.N[q][d2];
+            }
+            B[d1][3] += Q.phs[q] * D.N[q][d1];
+        }
+        const float Svs = D.R[q].y * Q.phs[q] + D.R[q].z * Usn;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) B[3][d] += Svs * D.N[q][d];
+        B[3][3] += (Q.sn[q] * D.R[q].w) * gN;
+    }
+}
+
+// The family t of blocks of the node's rows: its eight (block, element)
+// pairs in table order, each pair's LDS data loaded one pair ahead of its
+// arithmetic; a block is stored (with s_a, the density of row 3 and the
+// mass part on the centre block's diagonal) after its last pair.
+template <int FAST, bool LAP>
+__device__ inline void row_family(int t, const Smem& S, const MarchGrid& G, const RowCtx& X, const KParams& P,
+                                  float nr, float* __restrict__ J)
+{
+    const float sa = float(P.s_a), sr = float(P.s_a * P.rho);
     const float msc = (P.parts & 2) ? float(P.s_m * P.rho) : 0.f;
     float B[4][4];
     float mvol = 0.f;
+    PairData D[2];
+    load_pair(S, kPairs.p[t][0], X, D[0]);
+#pragma unroll 2
     for (int m = 0; m < 8; ++m) {
         const Pair& Q = kPairs.p[t][m];
+        if (m < 7) load_pair(S, kPairs.p[t][m + 1], X, D[(m + 1) & 1]);
         if (Q.first) {
 #pragma unroll
             for (int r = 0; r < 4; ++r)
@@ -649,43 +730,11 @@ __device__ inline void row_family(int t, const Smem& S, const MarchGrid& G, cons
         const int layer = X.k + (Q.sz < 0 ? -1 : 0);
         const int ci = X.i + (Q.sx < 0 ? -1 : 0), cj = X.j + (Q.sy < 0 ? -1 : 0);
         if (layer >= 0 && layer < G.nz && ci >= 0 && ci < G.nx && cj >= 0 && cj < G.ny) {
-            const int col = X.li + (Q.sx < 0 ? 0 : 1) + CX * (X.lj + (Q.sy < 0 ? 0 : 1));
-            const int sl = layer & 1;
-            float Ji[3][3];
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) Ji[r][c] = S.ejn[sl][3 * r + c][col];
-            if (t == 0) mvol += float(S.evol[sl][col]);
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const int ip = Q.ip[q];
-                const float4 R = S.rec[sl][ip][col];
-                const float pw = Q.sg[q] * R.x;
-                float n[3], g[3];
-#pragma unroll
-                for (int i = 0; i < 3; ++i) n[i] = Q.sn[q] * S.ejn[sl][9 + 3 * q + i][col];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) g[c] = Q.gl[q][0] * Ji[0][c] + Q.gl[q][1] * Ji[1][c] + Q.gl[q][2] * Ji[2][c];
-                const float ph = Q.phi[q];
-                float U;
-                if constexpr (FAST == UPW_NO) U = ph;
-                else U = S.idx[sl][ip][col] == Q.kw ? 1.f : 0.f;
-                const float gn = g[0] * n[0] + g[1] * n[1] + g[2] * n[2];
-#pragma unroll
-                for (int d1 = 0; d1 < 3; ++d1) {
-                    B[d1][d1] += nr * gn;
-                    if (!lap)
-#pragma unroll
-                        for (int d2 = 0; d2 < 3; ++d2) B[d1][d2] += nr * g[d1] * n[d2];
-                    B[d1][3] += ph * n[d1];
-                    B[d1][d1] += U * pw;
-                }
-                const float Sv = R.y * ph + R.z * U;
-#pragma unroll
-                for (int d = 0; d < 3; ++d) B[3][d] += Sv * n[d] * rho;
-                B[3][3] += R.w * gn * rho;
+            if (t == 0) {
+                const int col = X.li + (Q.sx < 0 ? 0 : 1) + CX * (X.lj + (Q.sy < 0 ? 0 : 1));
+                mvol += float(S.evol[layer & 1][col]);
             }
+            add_pair<FAST, LAP>(D[m & 1], Q, nr, B);
         }
         if (Q.last) {
             const int dx = Q.dx, dy = Q.dy, dz = Q.dz;
@@ -696,7 +745,8 @@ __device__ inline void row_family(int t, const Smem& S, const MarchGrid& G, cons
                 float* base = J + X.rb + (int64_t)slot * 4;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    float4 v = make_float4(sa * B[r][0], sa * B[r][1], sa * B[r][2], sa * B[r][3]);
+                    const float f = r == 3 ? sr : sa;
+                    float4 v = make_float4(f * B[r][0], f * B[r][1], f * B[r][2], f * B[r][3]);
                     if (r == 0) v.x += mass;
                     if (r == 1) v.y += mass;
                     if (r == 2) v.z += mass;
@@ -773,15 +823,17 @@ __device__ inline void store_plane(Smem& S, const Stage& r, int p)
     }
 }
 
-template <int FAST>
+template <int FAST, bool LAP>
 __global__ void __launch_bounds__(NT, 2)
 march_hex(MarchGrid G, const double* __restrict__ xyz, KParams P, const double* __restrict__ u,
           const double* __restrict__ uold, uint8_t* __restrict__ dec, float* __restrict__ J, double* __restrict__ dout,
           int* __restrict__ err)
 {
     __shared__ Smem S;
-    const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
-    const float nr = float(-1.0 * P.nu * P.rho), rho = float(P.rho);
+    // the wave index is uniform: in an SGPR, the per-wave tables (edge of
+    // phase 1, block family of phase 2) are scalar loads and offsets
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = (int)(threadIdx.x & 63);
+    const float nr = float(-1.0 * P.nu * P.rho);
     const int64_t per = (int64_t)G.tiles_x * G.tiles_y;
     const int li = lane % TX, lj = lane / TX;
     for (int64_t unit = blockIdx.x; unit < G.units; unit += gridDim.x) {
@@ -833,7 +885,7 @@ march_hex(MarchGrid G, const double* __restrict__ xyz, KParams P, const double* 
             }
             if (emit && J != nullptr && node) {
                 const RowCtx X = row_ctx(G, ni, nj, k, li, lj);
-                for (int pass = 0; pass < 2; ++pass) row_family<FAST>(wave + 4 * pass, S, G, X, P, nr, rho, J);
+                for (int pass = 0; pass < 2; ++pass) row_family<FAST, LAP>(wave + 4 * pass, S, G, X, P, nr, J);
             }
             __syncthreads();
         }
@@ -848,15 +900,25 @@ hipError_t launch_assemble_march(int fast, const MarchGrid& g, const double* xyz
 {
     if (g.units <= 0 || g.units > 0x7fffffff) return hipErrorInvalidValue;
     const dim3 grid((unsigned)g.units), block(march::NT);
-    switch (fast) {
-    case UPW_NO:
-        march::march_hex<UPW_NO><<<grid, block, 0, stream>>>(g, xyz, p, u, uold, upw_dec, J, d, err);
+    const bool lap = p.laplace != 0;
+    switch (fast * 2 + (lap ? 1 : 0)) {
+    case UPW_NO * 2:
+        march::march_hex<UPW_NO, false><<<grid, block, 0, stream>>>(g, xyz, p, u, uold, upw_dec, J, d, err);
         break;
-    case UPW_FULL:
-        march::march_hex<UPW_FULL><<<grid, block, 0, stream>>>(g, xyz, p, u, uold, upw_dec, J, d, err);
+    case UPW_NO * 2 + 1:
+        march::march_hex<UPW_NO, true><<<grid, block, 0, stream>>>(g, xyz, p, u, uold, upw_dec, J, d, err);
         break;
-    case UPW_SKEWED:
-        march::march_hex<UPW_SKEWED><<<grid, block, 0, stream>>>(g, xyz, p, u, uold, upw_dec, J, d, err);
+    case UPW_FULL * 2:
+        march::march_hex<UPW_FULL, false><<<grid, block, 0, stream>>>(g, xyz, p, u, uold, upw_dec, J, d, err);
+        break;
+    case UPW_FULL * 2 + 1:
+        march::march_hex<UPW_FULL, true><<<grid, block, 0, stream>>>(g, xyz, p, u, uold, upw_dec, J, d, err);
+        break;
+    case UPW_SKEWED * 2:
+        march::march_hex<UPW_SKEWED, false><<<grid, block, 0, stream>>>(g, xyz, p, u, uold, upw_dec, J, d, err);
+        break;
+    case UPW_SKEWED * 2 + 1:
+        march::march_hex<UPW_SKEWED, true><<<grid, block, 0, stream>>>(g, xyz, p, u, uold, upw_dec, J, d, err);
         break;
     default:
         return hipErrorInvalidValue;
