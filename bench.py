@@ -197,6 +197,68 @@ def learned_leg(steps: int):
     return r
 
 
+def channel_assembly_leg(steps: int, launch=None):
+    """The assembly alone on the held-out channel of the learned-guess
+    comparison (96x48x48 hexahedra x 6 tetrahedra minus obstacles, seed 100,
+    its Re and obstacle count drawn as there): LPS + FIELDS, time-dependent
+    stabilisation (dt = 0.005), A + M parts, FP32 J + FP64 defect, on the
+    initial channel state.  Launch shape from the autotuner unless given;
+    `steps` timed assemblies (HIP events on the launch stream)."""
+    import torch
+
+    from plugin_navierstokes_amd import F32, NavierStokesFV1, PART_A, PART_M
+    from plugin_navierstokes_amd import meshes as M
+    from plugin_navierstokes_amd.batch import autotune
+
+    seed, dt = 100, 0.005
+    rng = np.random.default_rng(1000 + seed)
+    re = float(rng.uniform(100.0, 500.0))
+    nbox = int(rng.integers(1, 5))
+    mesh = M.channel_3d((96, 48, 48), n_boxes=nbox, seed=seed)
+    disc = NavierStokesFV1("u,v,w,p", "Inner")
+    disc.set_stabilization("fields")
+    disc.set_upwind("lps")
+    disc.set_kinematic_viscosity(1.0 / re)
+    disc.set_mesh(mesh)
+    dofs, vals = M.channel_dirichlet(mesh)
+    u = torch.from_numpy(M.channel_initial_state(mesh, (dofs, vals))).to("cuda:0")
+    J = torch.empty(disc.nnz(), dtype=torch.float32, device="cuda:0")
+    d = torch.empty(disc.num_dofs(), dtype=torch.float64, device="cuda:0")
+    stream = torch.cuda.current_stream()
+
+    def step():
+        disc.assemble(u, u, dt, s_m=1.5 / dt, parts=PART_A | PART_M, jprec=F32, J=J, d=d, stream=stream, check=False)
+
+    if launch:
+        disc.set_assembly_path(launch[0])
+        disc.set_launch(launch[1], launch[2])
+        shape = {"path": launch[0], "threads": launch[1], "lds_bytes": launch[2], "autotuned": False}
+    else:
+        best, _ = autotune(disc, step, reps=2)
+        shape = {"path": best[0], "threads": best[1], "lds_bytes": best[2], "autotuned": True}
+    for _ in range(3):
+        step()
+    disc.sync_errors(stream)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for a, b in ev:
+        a.record(stream)
+        step()
+        b.record(stream)
+    torch.cuda.synchronize()
+    disc.sync_errors(stream)
+    ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    B = algorithmic_bytes(mesh.nv, mesh.ne, disc.nnz(), 3, 4, 4, 2)
+    a = B / (ms * 1e-3) / 1e9
+    return {"workload": f"held-out channel (seed {seed}, Re {re:.1f}, {nbox} obstacles): {mesh.ne} tetrahedra, "
+                        f"{mesh.nv} nodes, nnz {disc.nnz()}, LPS + FIELDS, dt {dt}, A + M parts, FP32 J + FP64 defect",
+            "ms_per_assembly": ms, "elem_per_s": mesh.ne / (ms * 1e-3), "path_used": disc.assembly_path_used(),
+            "launch": shape,
+            "roofline": {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": a / HBM_PEAK_GBS,
+                         "algorithmic_bytes": B, "kernel_ms": ms,
+                         "kernel": "nsfv1::assemble_rows<2,float,3> (fused tetrahedral assembly, one launch)"}}
+
+
 def batched_leg(steps: int = 20, count: int = 256):
     """256 cavity instances (side ~ U{32..192}, Re log-uniform [10, 2000],
     lid ~ U[0.5, 1.5], one seed) assembled in one launch: autotuned launch
@@ -236,6 +298,15 @@ def batched_leg(steps: int = 20, count: int = 256):
             "autotune": [{"path": p, "threads": t, "lds": l, "ms": m} for p, t, l, m in table]}
 
 
+# the kernels of one 128^3 hex assembly per assembly path that ran
+ASSEMBLY_KERNELS = {
+    3: "nsfv1::march::march_hex<2, false> (plane march: one launch per assembly)",
+    0: "one assembly = nsfv1::scvf_records<3,2,Q> (Q=0,1,2) + nsfv1::assemble_rows_split<3,2>",
+    2: "one assembly = nsfv1::scvf_records<3,2,Q> (Q=0,1,2) + nsfv1::assemble_rows_rowlane<3,2>",
+    1: "nsfv1::assemble_rows<3,float,2> (fused, one launch per assembly)",
+}
+
+
 def measured_traffic(n: int):
     """HBM bytes per assembly launch from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE passes scaled by a plain-copy calibration);
@@ -268,6 +339,10 @@ def main():
                     help="skip the learned-vs-plain channel leg of the time stepping (profiling runs)")
     ap.add_argument("--launch", type=int, nargs=3, metavar=("PATH", "THREADS", "LDS"),
                     help="fixed assembly path / launch shape instead of the autotuner (profiling runs)")
+    ap.add_argument("--channel-assembly", type=int, default=20,
+                    help="timed assemblies of the held-out channel alone (0: skip)")
+    ap.add_argument("--channel-launch", type=int, nargs=3, metavar=("PATH", "THREADS", "LDS"),
+                    help="fixed launch shape of the channel assembly leg (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (rocprof PMC byte calibration)")
     args = ap.parse_args()
@@ -362,7 +437,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n),
-            "kernel": "one assembly = nsfv1::scvf_records<3,2,Q> (Q=0,1,2) + nsfv1::assemble_rows_split<3,2>",
+            "kernel": ASSEMBLY_KERNELS.get(disc.assembly_path_used(), "one assembly"),
             "kernel_ms": kavg, "algorithmic_bytes": B, "launch": launch,
         },
     }
@@ -377,10 +452,15 @@ def main():
     torch.cuda.empty_cache()
     if args.channel and args.channel_steps > 0:
         out["timestep"]["channel"] = learned_leg(args.channel_steps)
+    if args.channel_assembly > 0:
+        out["channel_assembly"] = channel_assembly_leg(args.channel_assembly, args.channel_launch)
+        print(f"bench: channel assembly {out['channel_assembly']['ms_per_assembly']:.3f} ms", file=sys.stderr,
+              flush=True)
     if args.batched:
         out["batched"] = batched_leg()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(n)
+        # a bounded sample: the 64^3 sub-cavity (262k elements) of the same workload
+        out["cpu_baseline"] = cpu_baseline(min(n, 64))
     print(json.dumps(out), flush=True)
 
 
