@@ -171,11 +171,12 @@ int nsfv1_assemble(nsfv1_handle* h, const double* u, const double* u_old, double
    ends) instead of a buffer the handle allocates.  Returns
    NSFV1_ERR_ARGUMENT, before any launch, when the mesh's records exceed
    that array (split path: 128 bytes per SCVF, so about 87 000 hexahedra,
-   262 000 quadrilaterals or 349 000 triangles), or when the handle has no
-   launch plan for jprec or, in NSFV1_UPWIND_RECORD mode, no decision buffer
-   yet (one nsfv1_assemble call with the same settings builds them).  The
-   structured-hexahedra plane march and the fused path (tetrahedra, FP64
-   Jacobians) keep no records and have no size limit.
+   174 000 tetrahedra, 262 000 quadrilaterals or 349 000 triangles), or
+   when the handle has no launch plan for jprec or, in NSFV1_UPWIND_RECORD
+   mode, no decision buffer yet (one nsfv1_assemble call with the same
+   settings builds them).  The structured-hexahedra plane march and the
+   fused path (FP64 Jacobians, the generic configuration) keep no records
+   and have no size limit.
    Nothing in the array has to start at zero.  All calls on one device share
    its array: calls that overlap on one device (other streams, other host
    threads) race. */

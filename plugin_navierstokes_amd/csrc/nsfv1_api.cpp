@@ -1416,7 +1416,7 @@ int nsfv1_mg_setup(nsfv1_handle* h, int jprec, const double* u, const double* u_
     if (hipMemcpyAsync(&herr, gj_err, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return fail(NSFV1_ERR_DEVICE, "coarse inverse");
-    if (herr) return fail(NSFV1_ERR_ARGUMENT, "coarsest multigrid level is singular");
+    if (herr) return fail(NSFV1_ERR_ARGUMENT, "coarsest multigrid level: a column without finite entries (non-finite state or Jacobian)");
     h->mg_jprec = jprec;
     h->mg_ready = true;
     return NSFV1_OK;

@@ -369,9 +369,37 @@ __device__ inline bool upwind_canon(int type, bool kvfrom, const double (&co)[To
     return true;
 }
 
+// original number of the edge between original corners a and b
+template <int ET>
+__device__ inline int edge_between(int a, int b)
+{
+    int r = 0;
+#pragma unroll
+    for (int e = 0; e < Topo<ET>::nip; ++e)
+        if ((Topo<ET>::edge[e][0] == a && Topo<ET>::edge[e][1] == b) ||
+            (Topo<ET>::edge[e][0] == b && Topo<ET>::edge[e][1] == a))
+            r = e;
+    return r;
+}
+
+// scvf_frame of a runtime SCVF number (uniform over a wave where it is used)
+template <int ET, int E = 0>
+__device__ inline void scvf_frame_rt(int ip, const double (&co)[Topo<ET>::nsh][Topo<ET>::dim], const double* c,
+                                     double* ipg, double* nn, double* me)
+{
+    if constexpr (E < Topo<ET>::nip) {
+        if (ip == E) scvf_frame<ET, E>(co, c, ipg, nn, me);
+        else scvf_frame_rt<ET, E + 1>(ip, co, c, ipg, nn, me);
+    }
+}
+
 // FP64 part of canonical SCVF Q at corner 0 (= the row node): geometry,
 // upwind, stabilisation, defect; emits the record for the Jacobian.
-template <int ET, int FAST, int Q, typename JT>
+// ORIGF: the SCVF's ip, normal and edge midpoint are formed from the corners
+// in the element's own order (the reference's sums: element centre, face
+// centres), so they do not depend on the frame; the records of the split
+// path, which serve both rows of an SCVF, take this form for tetrahedra.
+template <int ET, int FAST, int Q, typename JT, bool ORIGF = false>
 __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET>::nsh][Topo<ET>::dim],
                                   const double* __restrict__ xyz, const int32_t* __restrict__ econn,
                                   const double (&ul)[Topo<ET>::nsh][Topo<ET>::dim + 1], const int* pk,
@@ -392,7 +420,17 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
         // frame of canonical SCVF IP with the normal pointing away from
         // corner 0 (the row node): row contributions are then all `+`
         double nn_[D];
-        scvf_frame<ET, IP>(co, c, ipg, nn_, me);
+        if constexpr (ORIGF) {
+            double cor[N][D], cc[D];
+#pragma unroll
+            for (int k = 0; k < N; ++k)
+#pragma unroll
+                for (int i = 0; i < D; ++i) cor[k][i] = xyz[(int64_t)econn[k] * D + i];
+            elem_centre<ET>(cor, cc);
+            scvf_frame_rt<ET>(edge_between<ET>(pk[0], pk[OT]), cor, cc, ipg, nn_, me);
+        } else {
+            scvf_frame<ET, IP>(co, c, ipg, nn_, me);
+        }
         double e[D];
 #pragma unroll
         for (int i = 0; i < D; ++i) e[i] = co[OT][i] - co[0][i];
@@ -869,8 +907,8 @@ __device__ inline void record_one(const Cfg<FAST>& C, const double (&co)[Topo<ET
     bool have_cut = false;
     int side = -1;
     double gi[3] = {0, 0, 0}, li[3] = {0, 0, 0};
-    scvf_canon<ET, FAST, Q, float>(C, co, xyz, econn, ul, pk, ccn, c, uold, l2, true, true, L, R, okg, oku, have_cut,
-                                   side, gi, li, decp, affine);
+    scvf_canon<ET, FAST, Q, float, ET == ET_TET>(C, co, xyz, econn, ul, pk, ccn, c, uold, l2, true, true, L, R, okg,
+                                                 oku, have_cut, side, gi, li, decp, affine);
     using RL = RecLayout<ET>;
     float rv[RL::F];
 #pragma unroll
@@ -889,7 +927,10 @@ __device__ inline void record_one(const Cfg<FAST>& C, const double (&co)[Topo<ET
     } else {
         // physical gradients of every corner, computed once in the original
         // frame in FP64 (constant on a simplex): J = sum_k x_k (x) grad_ref
-        // phi_k, cofactor inverse, grad phi_c = J^-T grad_ref phi_c
+        // phi_k, cofactor inverse, grad phi_c = J^-T grad_ref phi_c; every
+        // product rounded on its own (the oracle's arithmetic), so a
+        // gradient component that vanishes there is exactly zero here too
+#pragma clang fp contract(off)
         double J[D][D], Ki[D][D];
 #pragma unroll
         for (int i = 0; i < D; ++i)
@@ -1556,7 +1597,7 @@ static hipError_t launch_split_et(const KMeshPtrs& m, const KParams& p, const do
 // frame rounds some exactly-zero normal/gradient products of axis-aligned
 // tetrahedra differently from a row's own frame (O(1e-19) where the
 // reference arithmetic gives an exact 0).
-bool split_supported(int et, const KParams& p) { return et != ET_TET && fast_path(p) >= 0; }
+bool split_supported(int et, const KParams& p) { (void)et; return fast_path(p) >= 0; }
 
 int64_t split_record_floats(int et, int64_t ne)
 {
@@ -1573,6 +1614,7 @@ hipError_t launch_assemble_split(int et, const KMeshPtrs& m, const KParams& p, c
     switch (et) {
     case ET_TRI:  return launch_split_et<ET_TRI>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, rowlane, s);
     case ET_QUAD: return launch_split_et<ET_QUAD>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, rowlane, s);
+    case ET_TET:  return launch_split_et<ET_TET>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, rowlane, s);
     case ET_HEX:  return launch_split_et<ET_HEX>(m, p, u, uold, J, d, err, nwg, lds, ne, recs, rowlane, s);
     }
     return hipErrorInvalidValue;

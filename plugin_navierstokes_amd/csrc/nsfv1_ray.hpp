@@ -11,9 +11,13 @@ namespace nsfv1 {
 
 #define NSFV1_DBL_MAX 1.7976931348623157e308
 
+// The _nc helpers round every product and sum separately (no FMA
+// contraction), as the oracle's arithmetic does: a cross-product component
+// a*b - c*d whose products are equal is then exactly zero.
 template <int D>
 __device__ inline double dot_nc(const double* a, const double* b)
 {
+#pragma clang fp contract(off)
     double s = 0;
 #pragma unroll
     for (int i = 0; i < D; ++i) s += a[i] * b[i];
@@ -23,6 +27,7 @@ __device__ inline double dot_nc(const double* a, const double* b)
 template <int D>
 __device__ inline double dist2_nc(const double* a, const double* b)
 {
+#pragma clang fp contract(off)
     double s = 0;
 #pragma unroll
     for (int i = 0; i < D; ++i) s += (a[i] - b[i]) * (a[i] - b[i]);
@@ -31,6 +36,7 @@ __device__ inline double dist2_nc(const double* a, const double* b)
 
 __device__ inline void cross_nc(const double* a, const double* b, double* c)
 {
+#pragma clang fp contract(off)
     c[0] = a[1] * b[2] - a[2] * b[1];
     c[1] = a[2] * b[0] - a[0] * b[2];
     c[2] = a[0] * b[1] - a[1] * b[0];

@@ -196,10 +196,12 @@ __global__ void multi_axpy(const double* __restrict__ V, int64_t ld, int k, cons
 }
 
 // y = a x + b y
+// y = a x + b y; with b = 0, y is not read (it may hold anything, NaN
+// bit patterns of a fresh allocation included)
 __global__ void axpby(double a, const double* __restrict__ x, double b, double* __restrict__ y, int64_t n)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) y[t] = a * x[t] + b * y[t];
+    if (t < n) y[t] = b == 0.0 ? a * x[t] : a * x[t] + b * y[t];
 }
 
 // x += sum_i c_i V_i
@@ -466,14 +468,20 @@ gj_step(const double* __restrict__ Min, double* __restrict__ Mout, const double*
         if (sv[q] > bv || (sv[q] == bv && si[q] < bi)) bv = sv[q], bi = si[q];
     const int p = bi;
     const bool was_done = done[i] != 0;
+    // column k vanishes on every unpivoted row (a null mode of the coarse
+    // operator, e.g. the constant pressure of a domain with Dirichlet
+    // velocity everywhere): unknown k is pinned to zero in the coarse
+    // correction and row p dropped, instead of dividing by zero.  A column
+    // with no finite entry left is reported.
+    const bool pin = bv == 0.0;
     if (t == 0) {
-        if (i == 0 && !(bv > 0.0)) *err = 1;
+        if (i == 0 && !(bv >= 0.0)) *err = 1;
         if (i == p) done[p] = 1, piv[k] = p;
     }
     const double* rp = Min + (int64_t)p * n;
     const double* ri = Min + (int64_t)i * n;
     double* ro = Mout + (int64_t)i * n;
-    const double ia = 1.0 / rp[k];
+    const double ia = pin ? 0.0 : 1.0 / rp[k];
     const bool pivoted = was_done || i == p;
     if (i == p) {
         for (int j = t; j < n; j += 256) {

@@ -5,7 +5,7 @@ in one Newton iterate of a time step:
   (2,097,152 elements, nnz 913,066,000), skewed upwind + FIELDS (split path);
 - "channel": a 96x48x48 x 6-tetrahedra obstacle channel like the learned
   predictor's (two obstacle boxes, 1,308,672 elements), LPS upwind + FIELDS
-  (fused path), at a perturbed parabolic state;
+  (split path), at a perturbed parabolic state;
 - the batched leg's union of 256 cavities (3,958,533 quadrilaterals, full
   upwind + FIELDS, per-instance viscosity), steady.
 

@@ -70,6 +70,24 @@ def test_block_jacobi_gmres(cuda, jprec):
     assert rr <= 1e-8 and true <= 1e-7
 
 
+def test_gmres_does_not_read_its_fresh_workspace(cuda):
+    """The GMRES workspace comes from hipMalloc and may hold any bit pattern.
+    Device memory filled with NaN is handed back to the driver first, so the
+    fresh workspace likely reuses it: the basis vectors are written as
+    y = a x (b = 0, y not read), and the solve stays finite and converges."""
+    import torch
+
+    junk = torch.full((1 << 25,), float("nan"), dtype=torch.float64, device=cuda)
+    del junk
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    mesh = M.cavity_2d(8, M.QUAD, 0.1, 2)
+    disc = _disc(mesh, "full", 0.01)
+    _, _, _, J = _system(cuda, mesh, disc, F32)
+    it, rr, true = _check_solve(cuda, disc, J, F32, 1e-8, 4000, 40)
+    assert np.isfinite(rr) and rr <= 1e-8 and true <= 1e-7, (it, rr, true)
+
+
 @pytest.mark.parametrize("case", ["quad32-steady", "tri32-steady", "hex16-bdf2", "tet8-bdf2"])
 @pytest.mark.parametrize("jprec", [F64, F32])
 def test_multigrid_gmres(cuda, case, jprec):
