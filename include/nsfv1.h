@@ -106,10 +106,15 @@ int nsfv1_set_grad_div(nsfv1_handle* h, double factor);
 
 /* NavierStokesFV1::set_upwind(name): navier_stokes_fv1.h:213-215 with the
    factory upwind_interface.cpp:43-62 (keys trimmed, case-insensitive: no,
-   full, skewed, linearprofileskewed|lps, positive|pos; regular|reg, 2D only,
-   is recognised but not provided by this build: it needs ugcore's SCVF-of-SCV
-   ray intersection).  The positive upwind has ip shapes: the assembly then
-   solves the stabilisation's numIp x numIp system per element. */
+   full, skewed, linearprofileskewed|lps, positive|pos, regular|reg).  The
+   positive and regular upwinds have ip shapes: the assembly then solves the
+   stabilisation's numIp x numIp system per element.  The regular upwind is
+   2D only (upwind.cpp:806: assembly of a 3D mesh returns
+   NSFV1_ERR_NOT_IMPLEMENTED); its ray/SCV intersection, ugcore's
+   SCVFofSCVRayIntersection, follows a stated convention: the SCV of corner
+   k is [x_k, m_k, c, m_{k-1}], the ray leaves it through the other SCVF,
+   side [x_k, m_k] or side [m_{k-1}, x_k] (in that order), with lambda from
+   the SCVF's edge midpoint to the centre on an SCVF. */
 int nsfv1_set_upwind(nsfv1_handle* h, const char* name);
 
 /* set_stabilization(name[, diffLength]): navier_stokes_fv1.h:189-200 with

@@ -22,7 +22,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "lib", "libfv1oracle.so")
 
 TRI, QUAD, TET, HEX = 0, 1, 2, 3
-UPW = {"no": 0, "full": 1, "skewed": 2, "lps": 3, "linearprofileskewed": 3, "positive": 4}
+UPW = {"no": 0, "full": 1, "skewed": 2, "lps": 3, "linearprofileskewed": 3, "positive": 4, "regular": 5}
 STAB_FIELDS, STAB_WITHOUT, STAB_FLOW = 0, 1, 2
 DIFFLEN = {"raw": 0, "fivepoint": 1, "cor": 2}
 NSH = {TRI: 3, QUAD: 4, TET: 4, HEX: 8}

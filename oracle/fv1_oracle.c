@@ -470,6 +470,95 @@ static void positive_upwind(const or_geo* G, const double (*vel)[3], double (*U)
 }
 
 /* ------------------------------------------------------------------ */
+/* regular upwind (2D): upwind.cpp:792-964.  Its ray/SCV intersection   */
+/* is ugcore's SCVFofSCVRayIntersection, which the reference tree does  */
+/* not hold; it is restated with this convention:                       */
+/*   - the SCV of corner k is the quad [x_k, m_k, c, m_{k-1}] (m_j the  */
+/*     midpoint of edge j = (j, j+1), c the element centre); side 0 =   */
+/*     [x_k, m_k] and side 3 = [m_{k-1}, x_k] lie on the element's      */
+/*     edges, the other two sides are the SCVFs of edges k and k-1;     */
+/*   - the ray x_ip - t v (t > 0) leaves the upwind SCV through the     */
+/*     SCVF other than ip, side 0 or side 3, tried in that order; the   */
+/*     first hit with 0 <= lambda <= 1 (tolerance 1e-10) wins;          */
+/*   - lambda runs on side 0 from x_k to m_k, on side 3 from m_{k-1}   */
+/*     to x_k (corner order), and on the SCVF from its edge midpoint    */
+/*     (0) to c (1): the reading under which the reference's branch     */
+/*     a.1 reproduces the intersection point exactly for either sign    */
+/*     of the flux.                                                     */
+/* ------------------------------------------------------------------ */
+static int ray_segment_2d(const double* o, const double* v, const double* a, const double* b, double* lam)
+{
+    const double d0 = b[0] - a[0], d1 = b[1] - a[1];
+    const double r0 = o[0] - a[0], r1 = o[1] - a[1];
+    const double det = d0 * v[1] - d1 * v[0];
+    if (det == 0.0) return 0;
+    const double l = (r0 * v[1] - r1 * v[0]) / det;
+    const double t = (d0 * r1 - d1 * r0) / det;
+    if (!(t > 0.0) || l < -1e-10 || l > 1.0 + 1e-10) return 0;
+    *lam = l < 0.0 ? 0.0 : (l > 1.0 ? 1.0 : l);
+    return 1;
+}
+
+static int regular_upwind(const or_geo* G, const double (*vel)[3], double (*U)[8], double (*W)[12],
+                          double* convlen)
+{
+    if (G->dim != 2) { set_err("RegularUpwind only implemented for 2d."); return -11; }
+    const int nc = G->nsh, nip = G->nip;
+    const double eps = DBL_EPSILON * 10;
+    for (int ip = 0; ip < nip; ++ip) {
+        for (int sh = 0; sh < nc; ++sh) U[ip][sh] = 0.0;
+        for (int j = 0; j < nip; ++j) W[ip][j] = 0.0;
+        const double* v = vel[ip];
+        const double* n = G->n[ip];
+        if (fabs(dot(2, v, v)) <= eps) continue;
+        double flux = dot(2, v, n);
+        if (fabs(flux) <= 100 * eps) {
+            flux = v[0] * n[1] - v[1] * n[0];
+            if (flux > 0) W[ip][(ip + nc - 1) % nc] = W[ip][(ip + 1) % nc] = 0.5;
+            else U[ip][ip] = U[ip][(ip + 1) % nc] = 0.5;
+            continue;
+        }
+        const int k = flux > 0.0 ? G->from[ip] : G->to[ip];
+        const int oth = flux > 0.0 ? (ip + nc - 1) % nc : (ip + 1) % nc;
+        double lam = 0.0;
+        if (ray_segment_2d(G->ipg[ip], v, G->c0[oth], G->c2[oth], &lam)) {
+            int ipI = (ip + 1) % nc, ipO = (ip + nc - 1) % nc;
+            if (flux > 0.0) { ipI = (ip + nc - 1) % nc; ipO = (ip + 1) % nc; }
+            if (lam <= 0.5) {
+                W[ip][ipO] = lam - 0.5;
+                W[ip][ipI] = 1.0 - (lam - 0.5);
+            } else {
+                U[ip][G->from[ip]] = 0.5 * 2 * (lam - 0.5);
+                U[ip][G->to[ip]] = 0.5 * 2 * (lam - 0.5);
+                W[ip][ipI] = 1.0 - 2 * (lam - 0.5);
+            }
+            continue;
+        }
+        int side = -1;
+        if (ray_segment_2d(G->ipg[ip], v, G->co[k], G->c0[k], &lam)) side = 0;
+        else if (ray_segment_2d(G->ipg[ip], v, G->c0[(k + nc - 1) % nc], G->co[k], &lam)) side = 3;
+        if (side < 0) { set_err("RegularUpwind: the ray leaves the SCV through no side."); return -10; }
+        if (flux > 0.0) {
+            if (side == 0) { U[ip][ip] = 1.0 - 0.5 * lam; U[ip][(ip + 1) % nc] = 0.5 * lam; }
+            else { U[ip][(ip + nc - 1) % nc] = 1.0 - 0.5 * (lam + 1.0); U[ip][ip] = 0.5 * (lam + 1.0); }
+        } else {
+            if (side == 0) { U[ip][(ip + 1) % nc] = 1.0 - 0.5 * lam; U[ip][(ip + 2) % nc] = 0.5 * lam; }
+            else { U[ip][ip] = 1.0 - 0.5 * (lam + 1.0); U[ip][(ip + 1) % nc] = 0.5 * (lam + 1.0); }
+        }
+    }
+    /* convection length to the upwind point (upwind.cpp:945-962) */
+    for (int ip = 0; ip < nip; ++ip) {
+        double up[3] = {0, 0, 0};
+        for (int sh = 0; sh < nc; ++sh)
+            for (int i = 0; i < 2; ++i) up[i] += U[ip][sh] * G->co[sh][i];
+        for (int j = 0; j < nip; ++j)
+            for (int i = 0; i < 2; ++i) up[i] += W[ip][j] * G->ipg[j][i];
+        convlen[ip] = sqrt(dist2(2, G->ipg[ip], up));
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
 /* upwind shapes: upwind.cpp:52-80 (no), 133-172 (full),                */
 /* 337-430 (skewed + GetNodeNextToCut), 505-575 (LPS), 643-786          */
 /* (positive).  W (ip shapes) may be NULL for the corner-only upwinds.  */
@@ -482,6 +571,10 @@ static int upwind_geo(int type, const or_geo* G, const double (*vel)[3], double 
         if (!W) { set_err("positive upwind needs ip shapes"); return -4; }
         positive_upwind(G, vel, U, W, convlen);
         return 0;
+    }
+    if (type == OR_UPW_REGULAR) {
+        if (!W) { set_err("regular upwind needs ip shapes"); return -4; }
+        return regular_upwind(G, vel, U, W, convlen);
     }
     if (W)
         for (int ip = 0; ip < nip; ++ip)
@@ -877,7 +970,7 @@ static int stab_prologue(const or_params* p, const or_geo* G, const double* uloc
             if (rc) return rc;
         }
     }
-    return stab_update(p, G, uloc, uold, dt, stdvel, st, &dn, p->stab_upwind == OR_UPW_POSITIVE || p->force_matrix, S);
+    return stab_update(p, G, uloc, uold, dt, stdvel, st, &dn, p->stab_upwind == OR_UPW_POSITIVE || p->stab_upwind == OR_UPW_REGULAR || p->force_matrix, S);
 }
 
 static int prologue(const or_params* p, const or_geo* G, const double* uloc, const double* uold, double dt, prolog* W)
@@ -907,7 +1000,7 @@ static int prologue(const or_params* p, const or_geo* G, const double* uloc, con
             rc = upwind_geo(p->conv_upwind, G, (const double (*)[3])W->stdvel, W->Uc, W->Wc, W->lc);
             if (rc) return rc;
         }
-        W->conv_ip = p->conv_upwind == OR_UPW_POSITIVE;
+        W->conv_ip = p->conv_upwind == OR_UPW_POSITIVE || p->conv_upwind == OR_UPW_REGULAR;
     }
     return 0;
 }

@@ -307,10 +307,7 @@ static int parse_upwind(const char* name, int* type)
     if (n == "skewed") { *type = UPW_SKEWED; return NSFV1_OK; }
     if (n == "linearprofileskewed" || n == "lps") { *type = UPW_LPS; return NSFV1_OK; }
     if (n == "positive" || n == "pos") { *type = UPW_POSITIVE; return NSFV1_OK; }
-    if (n == "regular" || n == "reg")
-        return fail(NSFV1_ERR_NOT_IMPLEMENTED, "NavierStokes: upwind type '" + std::string(name) +
-                                                   "' (2D only) needs ugcore's SCVF-of-SCV ray intersection, "
-                                                   "which this build does not provide");
+    if (n == "regular" || n == "reg") { *type = UPW_REGULAR; return NSFV1_OK; }
     return fail(NSFV1_ERR_UNKNOWN_NAME, "NavierStokes: upwind type '" + std::string(name ? name : "") +
                                             "' not found. Options are: no, full, weighted, skewed, "
                                             "LinearProfileSkewed (lps), positive (pos), regular (reg)");
@@ -738,8 +735,8 @@ static bool coupled_config(const nsfv1_handle* h)
 {
     if (h->stab_type == STAB_FLOW) return true;
     if (h->stokes) return false;
-    if (h->stab_type != STAB_WITHOUT && h->stab_upwind == UPW_POSITIVE) return true;
-    return !h->pac && h->conv_upwind == UPW_POSITIVE;
+    if (h->stab_type != STAB_WITHOUT && (h->stab_upwind == UPW_POSITIVE || h->stab_upwind == UPW_REGULAR)) return true;
+    return !h->pac && (h->conv_upwind == UPW_POSITIVE || h->conv_upwind == UPW_REGULAR);
 }
 
 // per-element SCV volumes and diffusion-length factors, built on first use
@@ -768,6 +765,10 @@ static int check_setup(nsfv1_handle* h)
     }
     if (!h->nu_set)
         return fail(NSFV1_ERR_NO_VISCOSITY, "NavierStokes::prep_elem_loop: Kinematic Viscosity has not been set, but is required.");
+    // upwind.cpp:806 (thrown when the upwind is computed)
+    const bool reg = (!h->stokes && h->stab_type != STAB_WITHOUT && h->stab_upwind == UPW_REGULAR) ||
+                     (!h->stokes && !h->pac && h->conv_upwind == UPW_REGULAR);
+    if (reg && h->dim != 2) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "RegularUpwind only implemented for 2d.");
     return NSFV1_OK;
 }
 

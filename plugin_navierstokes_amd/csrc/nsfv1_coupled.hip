@@ -53,6 +53,113 @@ struct CplCfg {
     static constexpr int EPW = 64 / NIP;  // elements per workgroup
 };
 
+// The ray o - t v (t > 0) against the segment [a, b] (2D): true with the
+// segment parameter lambda (from a, clamped to [0, 1], tolerance 1e-10) when
+// it crosses; every product rounded on its own, as in the oracle.
+__device__ inline bool ray_segment_2d(const double* o, const double* v, const double* a, const double* b,
+                                      double& lam)
+{
+#pragma clang fp contract(off)
+    const double d0 = b[0] - a[0], d1 = b[1] - a[1];
+    const double r0 = o[0] - a[0], r1 = o[1] - a[1];
+    const double det = d0 * v[1] - d1 * v[0];
+    if (det == 0.0) return false;
+    const double l = (r0 * v[1] - r1 * v[0]) / det;
+    const double t = (d0 * r1 - d1 * r0) / det;
+    if (!(t > 0.0) || l < -1e-10 || l > 1.0 + 1e-10) return false;
+    lam = l < 0.0 ? 0.0 : (l > 1.0 ? 1.0 : l);
+    return true;
+}
+
+// Regular upwind of one SCVF (upwind.cpp:792-964, 2D).  ugcore's
+// SCVFofSCVRayIntersection is taken with this convention: the SCV of corner
+// k is the quad [x_k, m_k, c, m_{k-1}] (m_j the midpoint of edge j, c the
+// element centre); the ray ip - t v leaves it through the other SCVF (than
+// the lane's), side 0 = [x_k, m_k] or side 3 = [m_{k-1}, x_k], tried in that
+// order; lambda runs from x_k to m_k on side 0, from m_{k-1} to x_k on side
+// 3, and from the SCVF's edge midpoint to c on the SCVF.
+template <int ET>
+__device__ inline bool regular_lane(int ip, const double (&co)[Topo<ET>::nsh][Topo<ET>::dim], const double* n,
+                                    const double* ipg, const double* v, double (&U)[Topo<ET>::nsh],
+                                    double (&W)[Topo<ET>::nip])
+{
+    using T = Topo<ET>;
+    constexpr int N = T::nsh;
+    const double eps = 2.220446049250313e-16 * 10;
+    if (fabs(dot_nc<2>(v, v)) <= eps) return true;  // no convection: no upwind shapes
+    double flux = dot_nc<2>(v, n);
+    if (fabs(flux) <= 100 * eps) {
+        // flow along the SCVF: the neighbouring ips (toward the element
+        // boundary) or the SCVF's corners (toward the centre)
+        {
+#pragma clang fp contract(off)
+            flux = v[0] * n[1] - v[1] * n[0];
+        }
+        if (flux > 0) W[(ip + N - 1) % N] = W[(ip + 1) % N] = 0.5;
+        else U[ip] = U[(ip + 1) % N] = 0.5;
+        return true;
+    }
+    const int k = flux > 0.0 ? T::edge[ip][0] : T::edge[ip][1];
+    const int oth = flux > 0.0 ? (ip + N - 1) % N : (ip + 1) % N;
+    double c[2], xk[2], mo[2], mk[2], mkm[2];
+    elem_centre<ET>(co, c);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        double a = 0.0, b = 0.0, x = 0.0, bb = 0.0, cc = 0.0;
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            x = q == k ? co[q][i] : x;
+            a = q == T::edge[oth][0] ? co[q][i] : a;
+            b = q == T::edge[oth][1] ? co[q][i] : b;
+            bb = q == T::edge[k][1] ? co[q][i] : bb;
+            cc = q == T::edge[(k + N - 1) % N][0] ? co[q][i] : cc;
+        }
+        xk[i] = x;
+        mo[i] = 0.5 * (a + b);
+        mk[i] = 0.5 * (x + bb);
+        mkm[i] = 0.5 * (cc + x);
+    }
+    double lam = 0.0;
+    if (ray_segment_2d(ipg, v, mo, c, lam)) {
+        int ipI = (ip + 1) % N, ipO = (ip + N - 1) % N;
+        if (flux > 0.0) {
+            ipI = (ip + N - 1) % N;
+            ipO = (ip + 1) % N;
+        }
+        if (lam <= 0.5) {
+            W[ipO] = lam - 0.5;
+            W[ipI] = 1.0 - (lam - 0.5);
+        } else {
+            U[T::edge[ip][0]] = 0.5 * 2 * (lam - 0.5);
+            U[T::edge[ip][1]] = 0.5 * 2 * (lam - 0.5);
+            W[ipI] = 1.0 - 2 * (lam - 0.5);
+        }
+        return true;
+    }
+    int side = -1;
+    if (ray_segment_2d(ipg, v, xk, mk, lam)) side = 0;
+    else if (ray_segment_2d(ipg, v, mkm, xk, lam)) side = 3;
+    if (side < 0) return false;
+    if (flux > 0.0) {
+        if (side == 0) {
+            U[ip] = 1.0 - 0.5 * lam;
+            U[(ip + 1) % N] = 0.5 * lam;
+        } else {
+            U[(ip + N - 1) % N] = 1.0 - 0.5 * (lam + 1.0);
+            U[ip] = 0.5 * (lam + 1.0);
+        }
+    } else {
+        if (side == 0) {
+            U[(ip + 1) % N] = 1.0 - 0.5 * lam;
+            U[(ip + 2) % N] = 0.5 * lam;
+        } else {
+            U[ip] = 1.0 - 0.5 * (lam + 1.0);
+            U[(ip + 1) % N] = 0.5 * (lam + 1.0);
+        }
+    }
+    return true;
+}
+
 // upwind shapes of the lane's SCVF for velocity sign * v (sign = -1: the
 // downwind of upwind_interface.h:157-165): corner shapes U, ip shapes W,
 // convection length.  Returns false when a ray finds no side.
@@ -83,6 +190,27 @@ __device__ inline bool upwind_lane(int type, double sign, int ip, const double (
 #pragma unroll
         for (int k = 0; k < N; ++k) U[k] = k == c ? 1.0 : 0.0;
         len = sqrt(dist2_nc<D>(ipg, co[c]));
+        return true;
+    }
+    if (type == UPW_REGULAR) {
+        if constexpr (D == 2) {
+            if (!regular_lane<ET>(ip, co, n, ipg, v, U, W)) return false;
+        } else {
+            return false;  // refused on the host for 3D meshes
+        }
+        // convection length to the upwind point (upwind.cpp:945-962)
+        double up[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) up[i] = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+#pragma unroll
+            for (int i = 0; i < D; ++i) up[i] += U[k] * co[k][i];
+#pragma unroll
+        for (int j = 0; j < NIP; ++j)
+#pragma unroll
+            for (int i = 0; i < D; ++i) up[i] += W[j] * S.ipg[j][i];
+        len = sqrt(dist2_nc<D>(ipg, up));
         return true;
     }
     if (type == UPW_POSITIVE) {  // upwind.cpp:643-786
@@ -172,7 +300,8 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
     const int fr = T::edge[ip][0], to = T::edge[ip][1];
     const bool flow = P.stab_type == STAB_FLOW;
     const bool stokes = P.stokes != 0;
-    const bool matrix = !stokes && P.stab_type != STAB_WITHOUT && P.stab_upwind == UPW_POSITIVE;
+    const bool matrix = !stokes && P.stab_type != STAB_WITHOUT &&
+                        (P.stab_upwind == UPW_POSITIVE || P.stab_upwind == UPW_REGULAR);
     bool ok_geo = true, ok_cut = true;
 
     // ---- geometry of the lane's SCVF (restated ugcore FV1Geometry)
@@ -530,7 +659,7 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
     if (!stokes && !pac) {
         if (P.conv_upwind != P.stab_upwind || P.stab_type == STAB_WITHOUT)
             ok_cut &= upwind_lane<ET>(P.conv_upwind, 1.0, ip, co, n, S, Uc, Wc, Lc);
-        conv_ip = P.conv_upwind == UPW_POSITIVE;
+        conv_ip = P.conv_upwind == UPW_POSITIVE || P.conv_upwind == UPW_REGULAR;
         if (!ok_cut) kernel_error(err, KERR_CUT_SIDE);
     }
     (void)Lc;

@@ -1,6 +1,8 @@
 """GPU parity of the coupled path -- the FLOW stabilisation
 (incompressible/fv1/stabilization.cpp:436-772), the positive upwind
-(upwind.cpp:643-786) with the matrix case of the stabilisation
+(upwind.cpp:643-786) and the regular upwind (upwind.cpp:792-964, 2D; its
+ray/SCV intersection under the oracle's stated convention) with the matrix
+case of the stabilisation
 (stabilization.cpp:244-403) and ip shapes in the convective term
 (navier_stokes_fv1.cpp:430-457) -- through the C ABI against the CPU oracle,
 every element type, FP64 and FP32 Jacobians, steady and time-dependent,
@@ -113,8 +115,59 @@ def test_coupled_stabilised_velocity_export(cuda, et, cfg):
     assert np.max(np.abs(got - want) / tol) <= 1.0
 
 
-def test_regular_upwind_not_provided(cuda):
-    disc = NavierStokesFV1("u,v,p", "Inner")
+REGULAR_CASES = [
+    {"upwind": "regular"},
+    {"upwind": "regular", "stab": "flow"},
+    {"upwind": "regular", "difflen": "cor", "nu": 0.01},
+    {"upwind": "full", "stab_upwind": "regular"},
+    {"upwind": "regular", "stab_upwind": "lps"},
+    {"upwind": "regular", "pac": True},
+    {"upwind": "regular", "exact": 0.5, "peclet": True},
+]
+
+
+@pytest.mark.parametrize("cfg", REGULAR_CASES, ids=_cid)
+@pytest.mark.parametrize("et", [M.TRI, M.QUAD], ids=lambda e: ET_NAMES[e])
+def test_regular_upwind_vs_oracle(cuda, et, cfg):
+    """The regular upwind (upwind.cpp:792-964, 2D) on the coupled path against
+    the oracle's restatement, steady (FP64) and time-dependent with the mass
+    and source parts (FP64 and FP32 Jacobians), on jittered meshes."""
+    print(f"case et={ET_NAMES[et]} cfg={cfg}", flush=True)
+    mesh = small_mesh(et, n=6, seed=9, jitter=0.12)
+    J, d, Jr, dr, Ja, da, disc = run_case(cuda, et, cfg, mesh=mesh, seed=13)
+    assert disc.assembly_path_used() == 4
+    # with ip shapes the convective term carries the stabilised ip
+    # velocities' cross-component derivatives; where those cancel exactly
+    # inside an element the oracle's entry and its S are 0 while a different
+    # summation order keeps rounding noise of the terms: an absolute floor
+    # at FP64 rounding of the matrix scale
+    floor = 1e-15 * np.abs(Jr).max()
+    tol = 1e-12 * np.abs(Jr) + 1e-13 * Ja + floor
+    print(f"entries on the floor: {int(np.sum((np.abs(J - Jr) > tol - floor) & (np.abs(J - Jr) <= tol)))}")
+    assert np.max(np.abs(J - Jr) / tol) <= 1.0, worst(J, Jr, Ja)
+    assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+    for jprec in (F64, F32):
+        J, d, Jr, dr, Ja, da, _ = run_case(cuda, et, dict(cfg, source=(0.2, -0.3, 0.0)), mesh=mesh, seed=14,
+                                           time_dep=True, parts=PART_A | PART_M | PART_RHS, s=(0.9, 1.2, -0.4),
+                                           jprec=jprec)
+        if jprec == F64:
+            tol = 1e-12 * np.abs(Jr) + 1e-13 * Ja + 1e-15 * np.abs(Jr).max()
+            assert np.max(np.abs(J - Jr) / tol) <= 1.0, worst(J, Jr, Ja)
+        else:
+            tol = FP32_ABS * Ja + FP32_REL * np.abs(Jr) + 2.0 ** -24 * np.abs(Jr).max()
+            assert np.max(np.abs(J.astype(np.float64) - Jr) / tol) <= 1.0, worst(J, Jr, Ja, FP32_REL, FP32_ABS)
+        assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+
+
+def test_regular_upwind_is_2d_only(cuda):
+    """upwind.cpp:806: the regular upwind throws for 3D when it is computed."""
+    import torch
+
+    mesh = small_mesh(M.TET, 4, seed=2)
+    disc = NavierStokesFV1("u,v,w,p", "Inner")
+    configure_disc(disc, {"upwind": "regular"})
+    disc.set_mesh(mesh)
+    u = torch.from_numpy(M.random_state(mesh, 1)).to(cuda)
     with pytest.raises(NSFV1Error) as e:
-        disc.set_upwind("regular")
-    assert e.value.name == "NOT_IMPLEMENTED"
+        disc.assemble(u)
+    assert e.value.name == "NOT_IMPLEMENTED" and "only implemented for 2d" in str(e.value)
