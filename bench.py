@@ -261,18 +261,20 @@ def channel_assembly_leg(steps: int, launch=None):
 
 def batched_leg(steps: int = 20, count: int = 256):
     """256 cavity instances (side ~ U{32..192}, Re log-uniform [10, 2000],
-    lid ~ U[0.5, 1.5], one seed) assembled in one launch: autotuned launch
-    shape, assembly + Dirichlet rows captured in a HIP graph and replayed."""
+    lid ~ U[0.5, 1.5], one seed) assembled in one launch: launch shape from
+    the C++ autotuner, assembly + Dirichlet rows captured in a HIP graph and
+    replayed, and a whole Newton step in the library's own graph."""
     import torch
 
-    from plugin_navierstokes_amd.batch import BatchedCavities, autotune, draw_instances
+    from plugin_navierstokes_amd.batch import BatchedCavities, draw_instances
 
     t0 = time.perf_counter()
     b = BatchedCavities(draw_instances(count, seed=0))
     u = b.state(seed=1)
     print(f"bench: batch of {count} cavities, {b.elements} quads, setup {time.perf_counter() - t0:.1f} s",
           file=sys.stderr, flush=True)
-    best, table = autotune(b.disc, lambda: b.step(u), reps=3)
+    best, table = b.disc.autotune_native(u, reps=3)
+    table = [(p, t, l, m) for p, t, l, m in table]
     b.capture(u)
     for _ in range(3):
         b.replay()
@@ -284,6 +286,23 @@ def batched_leg(steps: int = 20, count: int = 256):
     e.record()
     torch.cuda.synchronize()
     ms = s.elapsed_time(e) / steps
+    # a whole Newton step in the library's own HIP graph (C ABI): assembly,
+    # Dirichlet rows, 20 block-Jacobi GMRES iterations with device-side
+    # reductions, update
+    un = u.clone()
+    r2 = torch.zeros(1, dtype=torch.float64, device="cuda:0")
+    b.capture_newton(un, krylov=20, res2=r2)
+    for _ in range(2):
+        b.replay_newton()
+    torch.cuda.synchronize()
+    s2, e2 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s2.record(b.nt_stream)
+    for _ in range(steps):
+        b.replay_newton()
+    e2.record(b.nt_stream)
+    torch.cuda.synchronize()
+    b.disc.sync_errors()
+    nms = s2.elapsed_time(e2) / steps
     mesh = b.mesh
     B = algorithmic_bytes(mesh.nv, mesh.ne, b.disc.nnz(), 2, 4, 4, 1)
     a = B / (ms * 1e-3) / 1e9
@@ -295,7 +314,11 @@ def batched_leg(steps: int = 20, count: int = 256):
             "roofline": {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": a / HBM_PEAK_GBS,
                          "algorithmic_bytes": B, "kernel_ms": ms,
                          "kernel": "one graph replay: union assembly (steady, FP32 J) + Dirichlet rows"},
-            "autotune": [{"path": p, "threads": t, "lds": l, "ms": m} for p, t, l, m in table]}
+            "autotune": [{"path": p, "threads": t, "lds": l, "ms": m} for p, t, l, m in table],
+            "newton_step": {"ms_per_step": nms, "krylov": 20, "preconditioner": "point-block Jacobi",
+                            "graph": "nsfv1_newton_graph_capture / nsfv1_newton_graph_replay (C ABI)",
+                            "contents": "assembly, Dirichlet rows, 20 GMRES iterations with device-side reductions "
+                                        "and least squares, FP64 update; no host synchronisation"}}
 
 
 # the kernels of one 128^3 hex assembly per assembly path that ran

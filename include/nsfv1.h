@@ -310,6 +310,50 @@ int nsfv1_spmv(nsfv1_handle* h, const void* jvals, int jprec, const double* x, d
 int nsfv1_linear_solve(nsfv1_handle* h, const void* jvals, int jprec, const double* b, double* x, double rtol,
                        int maxit, int restart, int* iters, double* relres, void* stream);
 
+/* One Newton step for d(u) = 0 that runs entirely on the device, with no
+   host synchronisation: assemble J (jprec) and d (A part, plus the mass part
+   with weight s_m when s_m != 0, time-dependent stabilisation when u_old is
+   given), d += dconst (device, or NULL: e.g. the BDF history term), the
+   Dirichlet rows of nsfv1_set_dirichlet (values g, device), then `krylov`
+   iterations of right-preconditioned GMRES from zero with every reduction
+   and the Hessenberg least-squares solve on the device, and u -= J^-1 d so
+   obtained.  res2 (device double, or NULL) receives ||d||^2 of this step
+   (before the update).  The preconditioner is nsfv1_linear_solve's (block
+   Jacobi; the multigrid hierarchy as last set up).  Allocates its
+   workspace on the first call.  No counterpart in the plugin: the Newton
+   iteration that drives NavierStokesFV1 is ugcore's NewtonSolver, outside
+   the reference tree; this is one of its steps with a fixed linear effort,
+   for callers that want a step without host round trips or a graph. */
+int nsfv1_newton_step(nsfv1_handle* h, double* u, const double* u_old, double dt, double s_m,
+                      const double* dconst, const double* g, void* jvals, int jprec, double* defect,
+                      int krylov, double* res2, void* stream);
+
+/* Captures nsfv1_newton_step with these arguments into a HIP graph owned
+   by the handle (replacing a previous one); stream must not be the null
+   stream.  Allocations and the geometry happen eagerly first (one
+   assembly, which writes J and d but not u), so the graph holds kernels
+   and device-to-device copies only; every pointer must stay valid while
+   the graph is replayed.  No counterpart in the plugin. */
+int nsfv1_newton_graph_capture(nsfv1_handle* h, double* u, const double* u_old, double dt, double s_m,
+                               const double* dconst, const double* g, void* jvals, int jprec, double* defect,
+                               int krylov, double* res2, void* stream);
+
+/* Launches the captured Newton step on stream.  No counterpart in the
+   plugin. */
+int nsfv1_newton_graph_replay(nsfv1_handle* h, void* stream);
+
+/* Launch-shape autotuner: times `reps` assemblies (HIP events on stream)
+   of every assembly path and launch shape the handle accepts (paths 0 and
+   1 x 64/128/256 threads x 16/32/64 KiB, path 2 x 64 threads, path 3 on
+   structured grids), keeps the fastest on the handle and returns it in
+   best[3] = {path, threads, lds_bytes}; table (host, 4 doubles per
+   candidate: path, threads, lds, ms), with capacity and count in *ntable,
+   may be NULL.  Shapes the mesh cannot use are skipped.  No counterpart in
+   the plugin (the CPU assembly has no launch shape). */
+int nsfv1_autotune(nsfv1_handle* h, const double* u, const double* u_old, double dt, double s_m, int parts,
+                   void* jvals, int jprec, double* defect, int reps, double* table, int* ntable, int* best,
+                   void* stream);
+
 /* Preconditioner of nsfv1_linear_solve: "block-jacobi" (default) or "gmg"
    (geometric multigrid, needs nsfv1_mg_set_hierarchy and nsfv1_mg_setup).
    No counterpart in the plugin: ugcore's GMG plays this role for it. */

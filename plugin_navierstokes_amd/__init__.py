@@ -46,7 +46,7 @@ EXPORTS = [
     "nsfv1_mlp_sizes", "nsfv1_mlp_pack", "nsfv1_mlp_forward", "nsfv1_mlp_mse", "nsfv1_mlp_backward", "nsfv1_mlp_adam",
     "nsfv1_predict_features", "nsfv1_predict_guess", "nsfv1_predict_target", "nsfv1_batch_set_instances",
     "nsfv1_set_assembly_path", "nsfv1_assembly_path_used", "nsfv1_set_outflow", "nsfv1_set_symmetry", "nsfv1_set_sliding_wall", "nsfv1_export_size",
-    "nsfv1_export",
+    "nsfv1_export", "nsfv1_newton_step", "nsfv1_newton_graph_capture", "nsfv1_newton_graph_replay", "nsfv1_autotune",
 ]
 # data exports (navier_stokes_fv1.cpp:1573-1577) and their points
 EX = {"velocity": 0, "velocity_grad": 1, "velocity_ip": 2, "pressure": 3, "pressure_grad": 4}
@@ -103,6 +103,11 @@ def load_library(path: str | None = None):
     L.nsfv1_set_dirichlet.argtypes = [H, vp, C.c_int64]
     L.nsfv1_apply_dirichlet.argtypes = [H, vp, C.c_int, vp, vp, vp, vp]
     L.nsfv1_spmv.argtypes = [H, vp, C.c_int, vp, vp, vp]
+    L.nsfv1_newton_step.argtypes = [H, vp, vp, dp, dp, vp, vp, vp, C.c_int, vp, C.c_int, vp, vp]
+    L.nsfv1_newton_graph_capture.argtypes = L.nsfv1_newton_step.argtypes
+    L.nsfv1_newton_graph_replay.argtypes = [H, vp]
+    L.nsfv1_autotune.argtypes = [H, vp, vp, dp, dp, C.c_int, vp, C.c_int, vp, C.c_int, C.POINTER(dp),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_int), vp]
     L.nsfv1_linear_solve.argtypes = [H, vp, C.c_int, vp, vp, dp, C.c_int, C.c_int, C.POINTER(C.c_int),
                                      C.POINTER(C.c_double), vp]
     L.nsfv1_set_preconditioner.argtypes = [H, C.c_char_p]
@@ -306,6 +311,61 @@ class NavierStokesFV1:
 
     def spmv(self, J, x, y, jprec=F64, stream=None):
         _check(_lib.nsfv1_spmv(self._h, _ptr(J), int(jprec), _ptr(x), _ptr(y), _stream_ptr(stream)))
+
+    # --- Newton step on the device, its HIP graph, the native autotuner ---
+    def _newton_args(self, u, u_old, dt, s_m, dconst, g, J, d, krylov, res2, jprec):
+        import torch
+
+        dev = u.device
+        if J is None:
+            J = torch.empty(self.nnz(), dtype=torch.float32 if jprec == F32 else torch.float64, device=dev)
+        if d is None:
+            d = torch.empty(self.num_dofs(), dtype=torch.float64, device=dev)
+        return (self._h, _ptr(u), _ptr(u_old), float(dt), float(s_m), _ptr(dconst), _ptr(g), _ptr(J), int(jprec),
+                _ptr(d), int(krylov), _ptr(res2)), J, d
+
+    def newton_step(self, u, u_old=None, dt=0.0, s_m=0.0, dconst=None, g=None, J=None, d=None, krylov=20,
+                    res2=None, jprec=F32, stream=None):
+        """One device Newton step (nsfv1_newton_step): u is updated in place;
+        returns (J, d).  g: Dirichlet values (device, sorted like the rows
+        given to set_dirichlet); res2: optional device float64 scalar."""
+        args, J, d = self._newton_args(u, u_old, dt, s_m, dconst, g, J, d, krylov, res2, jprec)
+        _check(_lib.nsfv1_newton_step(*args, _stream_ptr(stream)))
+        return J, d
+
+    def newton_graph_capture(self, u, u_old=None, dt=0.0, s_m=0.0, dconst=None, g=None, J=None, d=None,
+                             krylov=20, res2=None, jprec=F32, stream=None):
+        """Capture the Newton step into the handle's HIP graph (stream must
+        not be the null stream).  The tensors are kept referenced for the
+        replays."""
+        args, J, d = self._newton_args(u, u_old, dt, s_m, dconst, g, J, d, krylov, res2, jprec)
+        _check(_lib.nsfv1_newton_graph_capture(*args, _stream_ptr(stream)))
+        self._graph_keep = (u, u_old, dconst, g, J, d, res2)
+        return J, d
+
+    def newton_graph_replay(self, stream=None):
+        _check(_lib.nsfv1_newton_graph_replay(self._h, _stream_ptr(stream)))
+
+    def autotune_native(self, u, u_old=None, dt=0.0, s_m=0.0, parts=PART_A, jprec=F32, J=None, d=None, reps=3,
+                        stream=None):
+        """The C++ launch-shape autotuner (nsfv1_autotune): returns ((path,
+        threads, lds), [(path, threads, lds, ms), ...]); the best shape is
+        kept on the handle."""
+        import torch
+
+        if J is None:
+            J = torch.empty(self.nnz(), dtype=torch.float32 if jprec == F32 else torch.float64, device=u.device)
+        if d is None:
+            d = torch.empty(self.num_dofs(), dtype=torch.float64, device=u.device)
+        cap = 64
+        table = (C.c_double * (4 * cap))()
+        nt = C.c_int(cap)
+        best = (C.c_int * 3)()
+        _check(_lib.nsfv1_autotune(self._h, _ptr(u), _ptr(u_old), float(dt), float(s_m), int(parts), _ptr(J),
+                                   int(jprec), _ptr(d), int(reps), table, C.byref(nt), best, _stream_ptr(stream)))
+        rows = [(int(table[4 * i]), int(table[4 * i + 1]), int(table[4 * i + 2]), float(table[4 * i + 3]))
+                for i in range(min(nt.value, cap))]
+        return (best[0], best[1], best[2]), rows
 
     def linear_solve(self, J, b, x, jprec=F64, rtol=1e-6, maxit=500, restart=30, stream=None):
         it = C.c_int(0)

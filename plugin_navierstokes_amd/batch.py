@@ -7,9 +7,13 @@ each element's viscosity from the instance table
 (``nsfv1_batch_set_instances``), so one launch assembles every instance;
 Dirichlet rows of every instance go through the union's Dirichlet list.
 The assembly + Dirichlet step is captured once into a HIP graph (through
-torch's graph API on a side stream) and replayed.  ``autotune`` times the
-launch shapes (incidences per workgroup, LDS budget) on the actual batch
-and keeps the fastest.  No counterpart in the reference plugin.
+torch's graph API on a side stream) and replayed; a whole Newton step
+(assembly, Dirichlet rows, device-side GMRES, update) is captured by the
+library itself (``nsfv1_newton_graph_capture``), so a C++ caller gets the
+same graph.  ``autotune`` times the launch shapes (incidences per
+workgroup, LDS budget) on the actual batch and keeps the fastest
+(``nsfv1_autotune`` is the same search in C++).  No counterpart in the
+reference plugin.
 """
 from __future__ import annotations
 
@@ -123,6 +127,27 @@ class BatchedCavities:
 
     def replay(self):
         self.graph.replay()
+
+    def newton_step(self, u, krylov: int = 20, res2=None, stream=None):
+        """One Newton step of every instance on the device
+        (nsfv1_newton_step): assembly, Dirichlet rows, `krylov` block-Jacobi
+        GMRES iterations with device-side reductions, u -= correction."""
+        self.disc.newton_step(u, g=self.g, J=self.J, d=self.d, krylov=krylov, res2=res2, jprec=self.jprec,
+                              stream=stream)
+
+    def capture_newton(self, u, krylov: int = 20, res2=None, stream=None):
+        """Capture the Newton step into the library's HIP graph
+        (nsfv1_newton_graph_capture, through the C ABI); replay with
+        replay_newton.  u, res2 must stay at their addresses."""
+        import torch
+
+        self.nt_stream = stream or torch.cuda.Stream()
+        self.disc.newton_graph_capture(u, g=self.g, J=self.J, d=self.d, krylov=krylov, res2=res2,
+                                       jprec=self.jprec, stream=self.nt_stream)
+        torch.cuda.synchronize()
+
+    def replay_newton(self):
+        self.disc.newton_graph_replay(self.nt_stream)
 
 
 def autotune(disc, run, candidates=None, reps: int = 5, log=None):

@@ -166,6 +166,18 @@ struct nsfv1_handle {
     DevBuf upw_dec;
     int upw_mode = 0;
     bool upw_recorded = false;
+    // capturable Newton step: its own Krylov workspace (V: (m+1) ndof, w: 2
+    // ndof, dev: Gram-Schmidt projections | ||r||^2 | beta | y | Hessenberg)
+    // and the HIP graph of the step
+    DevBuf nt_V, nt_w, nt_Binv, nt_part, nt_dev;
+    int nt_m = 0;
+    hipGraph_t nt_graph = nullptr;
+    hipGraphExec_t nt_exec = nullptr;
+    ~nsfv1_handle()
+    {
+        if (nt_exec) (void)hipGraphExecDestroy(nt_exec);
+        if (nt_graph) (void)hipGraphDestroy(nt_graph);
+    }
 };
 
 extern "C" {
@@ -1709,6 +1721,218 @@ int nsfv1_linear_solve(nsfv1_handle* h, const void* jvals, int jprec, const doub
     if (hipStreamSynchronize(s) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "sync");
     if (iters) *iters = it;
     if (relres) *relres = res / bnorm;
+    return NSFV1_OK;
+}
+
+
+// ---------------------------------------------------------------------
+// capturable Newton step, its HIP graph, and the launch-shape autotuner
+// ---------------------------------------------------------------------
+static constexpr int NT_HS = GMRES_MAX_RESTART + 1;
+static constexpr size_t NT_DEV_DOUBLES = 2 * NT_HS + 1 + 2 + GMRES_MAX_RESTART + (size_t)NT_HS * GMRES_MAX_RESTART;
+
+static int newton_check(nsfv1_handle* h, double* u, const double* g, void* jvals, int jprec, double* defect, int m)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    if (!u || !jvals || !defect) return fail(NSFV1_ERR_ARGUMENT, "u, J and d are required");
+    if (jprec != NSFV1_F32 && jprec != NSFV1_F64) return fail(NSFV1_ERR_ARGUMENT, "jprec must be 32 or 64");
+    if (m < 1 || m > GMRES_MAX_RESTART) return fail(NSFV1_ERR_ARGUMENT, "krylov must be 1..60");
+    if (h->ndir && !g) return fail(NSFV1_ERR_ARGUMENT, "Dirichlet values g are required (nsfv1_set_dirichlet rows)");
+    if (h->use_mg && (!h->mg_ready || h->mg_jprec != jprec))
+        return fail(NSFV1_ERR_ARGUMENT, "multigrid preconditioner selected but not set up for this Jacobian precision (call nsfv1_mg_setup)");
+    return NSFV1_OK;
+}
+
+// allocations of the step (workspace; plan, records and geometry through one
+// assembly, which only writes J and d); never called while a graph captures
+static int newton_prepare(nsfv1_handle* h, const double* u, const double* u_old, double dt, double s_m, void* jvals,
+                          int jprec, double* defect, int m, hipStream_t s)
+{
+    const int64_t n = h->ndof, nv = h->nv;
+    const int nc = h->dim + 1;
+    if (h->nt_m < m) {
+        if (h->nt_V.alloc(sizeof(double) * n * (m + 1)) != hipSuccess || h->nt_w.alloc(sizeof(double) * n * 2) != hipSuccess ||
+            h->nt_Binv.alloc(sizeof(double) * nv * nc * nc) != hipSuccess ||
+            h->nt_part.alloc(sizeof(double) * RED_PARTIALS) != hipSuccess ||
+            h->nt_dev.alloc(sizeof(double) * NT_DEV_DOUBLES) != hipSuccess)
+            return fail(NSFV1_ERR_DEVICE, "Newton step workspace");
+        h->nt_m = m;
+    }
+    const int parts = NSFV1_PART_A | (s_m != 0.0 ? NSFV1_PART_M : 0);
+    return assemble_impl(h, u, u_old, dt, 1.0, s_m, 0.0, parts, jvals, jprec, defect, s, nullptr);
+}
+
+// the step: kernels only, on stream s, in a fixed sequence
+static int newton_body(nsfv1_handle* h, double* u, const double* u_old, double dt, double s_m, const double* dconst,
+                       const double* g, void* jvals, int jprec, double* d, int m, double* res2, hipStream_t s)
+{
+    const int parts = NSFV1_PART_A | (s_m != 0.0 ? NSFV1_PART_M : 0);
+    int rc = assemble_impl(h, u, u_old, dt, 1.0, s_m, 0.0, parts, jvals, jprec, d, s, nullptr);
+    if (rc) return rc;
+    const int64_t n = h->ndof, nv = h->nv;
+    const int nc = h->dim + 1;
+    const int64_t* rp = h->rowptr.as<int64_t>();
+    const int32_t* ci = h->colidx.as<int32_t>();
+    const int32_t* ad = h->adj.as<int32_t>();
+    double* V = h->nt_V.as<double>();
+    double* w = h->nt_w.as<double>();
+    double* z = w + n;
+    double* Binv = h->nt_Binv.as<double>();
+    double* part = h->nt_part.as<double>();
+    double* hd = h->nt_dev.as<double>();  // [h1 (HS) | h2 (HS) | ||w||^2]
+    double* n2 = hd + 2 * NT_HS + 1;
+    double* beta = n2 + 1;
+    double* y = beta + 1;
+    double* H = y + GMRES_MAX_RESTART;  // (m+1) x m, row-major
+    auto chk = [&](hipError_t e) { return e == hipSuccess; };
+    auto precond = [&](const double* in, double* out) -> hipError_t {
+        if (h->use_mg) return mg_cycle(h, 0, jvals, Binv, in, out, s);
+        return launch_block_jacobi_apply(nc, Binv, in, out, nv, s);
+    };
+    bool ok = true;
+    if (dconst) ok = ok && chk(launch_axpby(1.0, dconst, 1.0, d, n, s));
+    if (h->ndir)
+        ok = ok && chk(launch_dirichlet(jprec, rp, ci, h->dir_rows.as<int64_t>(), h->ndir, jvals, d, u, g, s));
+    ok = ok && chk(launch_block_jacobi_setup(jprec, nc, rp, ci, jvals, nv, Binv, h->errflag.as<int>(), s));
+    // GMRES(m) from x = 0: r = d, beta = ||d||, V0 = d / beta
+    ok = ok && chk(launch_multi_dot(d, n, 1, d, n, part, n2, s)) && chk(launch_gm_beta(n2, beta, res2, s)) &&
+         chk(launch_gm_scale(d, beta, V, n, s));
+    for (int j = 0; ok && j < m; ++j) {
+        ok = chk(precond(V + (int64_t)j * n, z)) && chk(launch_spmv(jprec, rp, ci, ad, nc, jvals, z, w, n, h->nnz, s));
+        for (int pass = 0; ok && pass < 2; ++pass)
+            ok = chk(launch_multi_dot(V, n, j + 1, w, n, part, hd + pass * NT_HS, s)) &&
+                 chk(launch_multi_axpy(V, n, j + 1, hd + pass * NT_HS, w, n, s));
+        ok = ok && chk(launch_multi_dot(w, n, 1, w, n, part, hd + 2 * NT_HS, s)) &&
+             chk(launch_gm_hcol(hd, NT_HS, H, m, j, s)) &&
+             chk(launch_gm_scale(w, H + (int64_t)(j + 1) * m + j, V + (int64_t)(j + 1) * n, n, s));
+    }
+    // u -= M^-1 V y
+    ok = ok && chk(launch_gm_lsq(H, beta, m, y, s)) && chk(launch_fill(w, 0.0, n, s)) &&
+         chk(launch_multi_add(V, n, m, y, w, n, s)) && chk(precond(w, z)) && chk(launch_axpby(-1.0, z, 1.0, u, n, s));
+    if (!ok) return fail(NSFV1_ERR_DEVICE, std::string("Newton step launch: ") + hipGetErrorString(hipGetLastError()));
+    return NSFV1_OK;
+}
+
+int nsfv1_newton_step(nsfv1_handle* h, double* u, const double* u_old, double dt, double s_m, const double* dconst,
+                      const double* g, void* jvals, int jprec, double* defect, int krylov, double* res2, void* stream)
+{
+    int rc = newton_check(h, u, g, jvals, jprec, defect, krylov);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    rc = newton_prepare(h, u, u_old, dt, s_m, jvals, jprec, defect, krylov, s);
+    if (rc) return rc;
+    return newton_body(h, u, u_old, dt, s_m, dconst, g, jvals, jprec, defect, krylov, res2, s);
+}
+
+int nsfv1_newton_graph_capture(nsfv1_handle* h, double* u, const double* u_old, double dt, double s_m,
+                               const double* dconst, const double* g, void* jvals, int jprec, double* defect,
+                               int krylov, double* res2, void* stream)
+{
+    int rc = newton_check(h, u, g, jvals, jprec, defect, krylov);
+    if (rc) return rc;
+    if (!stream) return fail(NSFV1_ERR_ARGUMENT, "graph capture needs a stream other than the null stream");
+    hipStream_t s = (hipStream_t)stream;
+    rc = newton_prepare(h, u, u_old, dt, s_m, jvals, jprec, defect, krylov, s);
+    if (rc) return rc;
+    if (hipStreamSynchronize(s) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "sync before capture");
+    if (h->nt_exec) (void)hipGraphExecDestroy(h->nt_exec);
+    if (h->nt_graph) (void)hipGraphDestroy(h->nt_graph);
+    h->nt_exec = nullptr;
+    h->nt_graph = nullptr;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "hipStreamBeginCapture");
+    rc = newton_body(h, u, u_old, dt, s_m, dconst, g, jvals, jprec, defect, krylov, res2, s);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess || !graph) return fail(NSFV1_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return fail(NSFV1_ERR_DEVICE, "hipGraphInstantiate");
+    }
+    h->nt_graph = graph;
+    h->nt_exec = exec;
+    return NSFV1_OK;
+}
+
+int nsfv1_newton_graph_replay(nsfv1_handle* h, void* stream)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (!h->nt_exec) return fail(NSFV1_ERR_ARGUMENT, "no captured Newton step (nsfv1_newton_graph_capture)");
+    const hipError_t e = hipGraphLaunch(h->nt_exec, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+    return NSFV1_OK;
+}
+
+int nsfv1_autotune(nsfv1_handle* h, const double* u, const double* u_old, double dt, double s_m, int parts,
+                   void* jvals, int jprec, double* defect, int reps, double* table, int* ntable, int* best,
+                   void* stream)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    if (reps < 1) return fail(NSFV1_ERR_ARGUMENT, "reps must be positive");
+    hipStream_t s = (hipStream_t)stream;
+    struct Cand { int path, threads, lds; };
+    std::vector<Cand> cand;
+    for (int p : {0, 1})
+        for (int t : {64, 128, 256})
+            for (int l : {16384, 32768, 65536}) cand.push_back({p, t, l});
+    cand.push_back({2, 64, 16384});
+    cand.push_back({2, 64, 65536});
+    if (h->march_ok)
+        for (int l : {8192, 16384, 32768}) cand.push_back({3, 256, l});
+    const int keep_path = h->path, keep_threads = h->max_threads;
+    const size_t keep_lds = h->lds_cap;
+    const int keep_zc = h->march_zc;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "hipEventCreate");
+    double best_ms = 1e300;
+    Cand bc{-1, 0, 0};
+    int nt = 0;
+    const int cap = ntable ? *ntable : 0;
+    for (const Cand& c : cand) {
+        if (nsfv1_set_assembly_path(h, c.path) || nsfv1_set_launch(h, c.threads, c.lds)) continue;
+        // a shape the mesh cannot use is refused by the library: skipped
+        if (nsfv1_assemble(h, u, u_old, dt, 1.0, s_m, 0.0, parts, jvals, jprec, defect, stream)) continue;
+        if (hipEventRecord(e0, s) != hipSuccess) break;
+        int rc = 0;
+        for (int r = 0; r < reps && !rc; ++r)
+            rc = nsfv1_assemble(h, u, u_old, dt, 1.0, s_m, 0.0, parts, jvals, jprec, defect, stream);
+        float ms = 0.f;
+        if (rc || hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            continue;
+        const double avg = (double)ms / reps;
+        if (table && nt < cap) {
+            table[4 * nt] = c.path;
+            table[4 * nt + 1] = c.threads;
+            table[4 * nt + 2] = c.lds;
+            table[4 * nt + 3] = avg;
+        }
+        ++nt;
+        if (avg < best_ms) best_ms = avg, bc = c;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ntable) *ntable = nt;
+    if (bc.path < 0) {
+        h->path = keep_path;
+        h->max_threads = keep_threads;
+        h->lds_cap = keep_lds;
+        h->march_zc = keep_zc;
+        h->plan32.valid = h->plan64.valid = false;
+        return fail(NSFV1_ERR_ARGUMENT, "no launch shape could assemble this mesh");
+    }
+    int rc = nsfv1_set_assembly_path(h, bc.path);
+    if (!rc) rc = nsfv1_set_launch(h, bc.threads, bc.lds);
+    if (rc) return rc;
+    if (best) best[0] = bc.path, best[1] = bc.threads, best[2] = bc.lds;
     return NSFV1_OK;
 }
 

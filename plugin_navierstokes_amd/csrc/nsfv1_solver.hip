@@ -195,13 +195,82 @@ __global__ void multi_axpy(const double* __restrict__ V, int64_t ld, int k, cons
     w[t] = s;
 }
 
-// y = a x + b y
 // y = a x + b y; with b = 0, y is not read (it may hold anything, NaN
 // bit patterns of a fresh allocation included)
 __global__ void axpby(double a, const double* __restrict__ x, double b, double* __restrict__ y, int64_t n)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) y[t] = b == 0.0 ? a * x[t] : a * x[t] + b * y[t];
+}
+
+// ---------------------------------------------------------------------
+// device-side GMRES of the capturable Newton step: the Hessenberg matrix,
+// the Givens rotations and the back substitution stay on the device, so a
+// fixed number of iterations runs without host synchronisation
+// ---------------------------------------------------------------------
+__global__ void fill_value(double* __restrict__ x, double v, int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) x[t] = v;
+}
+
+// beta = sqrt(||r||^2); the squared norm is also copied to res2 if given
+__global__ void gm_beta(const double* __restrict__ n2, double* __restrict__ beta, double* __restrict__ res2)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *beta = sqrt(*n2);
+        if (res2) *res2 = *n2;
+    }
+}
+
+// column j of the (m+1) x m Hessenberg matrix H (row-major) from the two
+// classical Gram-Schmidt passes hd[0..j], hd[hs..hs+j] and ||w||^2 = hd[2 hs]
+__global__ void gm_hcol(const double* __restrict__ hd, int hs, double* __restrict__ H, int m, int j)
+{
+    const int i = threadIdx.x;
+    if (i <= j) H[i * m + j] = hd[i] + hd[hs + i];
+    if (i == 0) H[(j + 1) * m + j] = sqrt(hd[2 * hs]);
+}
+
+// y = x / h, or 0 when h = 0 (a breakdown leaves a zero basis vector, whose
+// Hessenberg column then vanishes)
+__global__ void gm_scale(const double* __restrict__ x, const double* __restrict__ h, double* __restrict__ y, int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double hv = *h;
+    y[t] = hv > 0.0 ? x[t] / hv : 0.0;
+}
+
+// min ||beta e1 - H y|| over y (H overwritten by its rotated triangle):
+// the Givens rotations of the host GMRES, then back substitution; a zero
+// diagonal gives y_i = 0.  One thread (m <= GMRES_MAX_RESTART).
+__global__ void gm_lsq(double* __restrict__ H, const double* __restrict__ beta, int m, double* __restrict__ y)
+{
+    __shared__ double g[GMRES_MAX_RESTART + 1], cs[GMRES_MAX_RESTART], sn[GMRES_MAX_RESTART];
+    if (threadIdx.x != 0) return;
+    for (int i = 0; i <= m; ++i) g[i] = 0.0;
+    g[0] = *beta;
+    for (int j = 0; j < m; ++j) {
+        for (int i = 0; i < j; ++i) {
+            const double t = cs[i] * H[i * m + j] + sn[i] * H[(i + 1) * m + j];
+            H[(i + 1) * m + j] = -sn[i] * H[i * m + j] + cs[i] * H[(i + 1) * m + j];
+            H[i * m + j] = t;
+        }
+        const double a = H[j * m + j], c = H[(j + 1) * m + j];
+        const double r = hypot(a, c);
+        cs[j] = r > 0 ? a / r : 1.0;
+        sn[j] = r > 0 ? c / r : 0.0;
+        H[j * m + j] = r;
+        H[(j + 1) * m + j] = 0.0;
+        g[j + 1] = -sn[j] * g[j];
+        g[j] = cs[j] * g[j];
+    }
+    for (int i = m - 1; i >= 0; --i) {
+        double t = g[i];
+        for (int k = i + 1; k < m; ++k) t -= H[i * m + k] * y[k];
+        y[i] = H[i * m + i] != 0.0 ? t / H[i * m + i] : 0.0;
+    }
 }
 
 // x += sum_i c_i V_i
@@ -618,6 +687,39 @@ hipError_t launch_multi_add(const double* V, int64_t ld, int k, const double* c,
 hipError_t launch_axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t s)
 {
     axpby<<<blocks_for(n, 256), 256, 0, s>>>(a, x, b, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(double* x, double v, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    fill_value<<<blocks_for(n, 256), 256, 0, s>>>(x, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_gm_beta(const double* n2, double* beta, double* res2, hipStream_t s)
+{
+    gm_beta<<<1, 64, 0, s>>>(n2, beta, res2);
+    return hipGetLastError();
+}
+
+hipError_t launch_gm_hcol(const double* hd, int hs, double* H, int m, int j, hipStream_t s)
+{
+    if (j < 0 || j >= m || m > GMRES_MAX_RESTART || hs < m) return hipErrorInvalidValue;
+    gm_hcol<<<1, 64, 0, s>>>(hd, hs, H, m, j);
+    return hipGetLastError();
+}
+
+hipError_t launch_gm_scale(const double* x, const double* h, double* y, int64_t n, hipStream_t s)
+{
+    gm_scale<<<blocks_for(n, 256), 256, 0, s>>>(x, h, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_gm_lsq(double* H, const double* beta, int m, double* y, hipStream_t s)
+{
+    if (m < 1 || m > GMRES_MAX_RESTART) return hipErrorInvalidValue;
+    gm_lsq<<<1, 64, 0, s>>>(H, beta, m, y);
     return hipGetLastError();
 }
 

@@ -26,6 +26,15 @@ hipError_t launch_multi_dot(const double* V, int64_t ld, int k, const double* w,
 hipError_t launch_multi_axpy(const double* V, int64_t ld, int k, const double* h, double* w, int64_t n, hipStream_t s);
 hipError_t launch_multi_add(const double* V, int64_t ld, int k, const double* c, double* x, int64_t n, hipStream_t s);
 hipError_t launch_axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t s);
+// device-side pieces of the fixed-iteration GMRES of the capturable Newton
+// step (no host synchronisation): x = v; beta = sqrt(*n2) (and *res2 = *n2);
+// Hessenberg column j (m <= hs <= ...); y = x / *h (0 if *h = 0); Givens
+// least squares of the (m+1) x m Hessenberg H with right-hand side beta e1
+hipError_t launch_fill(double* x, double v, int64_t n, hipStream_t s);
+hipError_t launch_gm_beta(const double* n2, double* beta, double* res2, hipStream_t s);
+hipError_t launch_gm_hcol(const double* hd, int hs, double* H, int m, int j, hipStream_t s);
+hipError_t launch_gm_scale(const double* x, const double* h, double* y, int64_t n, hipStream_t s);
+hipError_t launch_gm_lsq(double* H, const double* beta, int m, double* y, hipStream_t s);
 hipError_t launch_residual(int jprec, const int64_t* rowptr, const int32_t* colidx, const int32_t* adj, int nc,
                            const void* A, const double* x, const double* b, double* r, int64_t n, int64_t nnz,
                            hipStream_t s);
