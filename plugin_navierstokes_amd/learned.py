@@ -257,7 +257,8 @@ def train(out_path, n_traj=16, steps=24, skip=2, epochs=12, batch=65536, lr=1e-3
         run_trajectory(case, steps, record=record)
         # scales from the first trajectory: unit-variance features and targets
         if in_scale is None:
-            Xs = torch.cat([pred.features(u, up)[:, :12] for _, u, up in rec[skip:]])
+            # features() fills one buffer: each step's slice is copied
+            Xs = torch.cat([pred.features(u, up)[:, :12].clone() for _, u, up in rec[skip:]])
             in_scale = (1.0 / Xs.std(0).clamp_min(1e-12)).cpu().numpy()
             Ts = torch.cat([pred.target(un, u, up) for un, u, up in rec[skip:]])
             out_scale = Ts.std(0).clamp_min(1e-12).cpu().numpy()
