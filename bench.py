@@ -174,11 +174,13 @@ def timestep_leg(mesh, disc, steps: int, dt: float = 0.01):
 
 
 def learned_leg(steps: int):
-    """Plain vs learned initial guess on a held-out 96x48x48x6-tetrahedra
-    channel (Re, obstacles and seed not in the training set): BDF2 dt=0.005,
-    LPS + FIELDS, FP32 J + FP64 defect, 1e-8 relative FP64 residual; 6
-    untimed warm-up steps, then `steps` timed steps per method.  The channel
-    assembly's roofline uses its autotuned launch time."""
+    """Plain (linear extrapolation) vs learned initial guess on a held-out
+    96x48x48x6-tetrahedra channel (Re, obstacles and seed not in the
+    training set), with the quadratic extrapolation alone (no network) as
+    the ablation: BDF2 dt=0.005, LPS + FIELDS, FP32 J + FP64 defect, 1e-8
+    relative FP64 residual; 6 untimed warm-up steps, then `steps` timed
+    steps per method.  The channel assembly's roofline uses its autotuned
+    launch time."""
     from plugin_navierstokes_amd.learned import compare
 
     w = os.path.join(ROOT, "plugin_navierstokes_amd", "data", "predictor_channel.npz")
@@ -186,6 +188,7 @@ def learned_leg(steps: int):
     r["workload"] = ("channel 96x48x48 hexahedra x 6 tetrahedra (1.26M elements after obstacles), held-out seed 100, "
                      "BDF2 dt=0.005, LPS+FIELDS, FP32 J + FP64 defect, residual 1e-8 x ||d(u^n)||")
     r["learned_vs_plain"] = r["learned_s_per_step"] / r["plain_s_per_step"]
+    r["quadratic_vs_plain"] = r["quadratic_s_per_step"] / r["plain_s_per_step"]
     launch = r.get("launch") or {}
     geo = r.pop("mesh_sizes", None)
     if geo and launch.get("ms"):
@@ -256,7 +259,7 @@ def channel_assembly_leg(steps: int, launch=None):
             "launch": shape,
             "roofline": {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": a / HBM_PEAK_GBS,
                          "algorithmic_bytes": B, "kernel_ms": ms,
-                         "kernel": "nsfv1::assemble_rows<2,float,3> (fused tetrahedral assembly, one launch)"}}
+                         "kernel": CHANNEL_KERNELS.get(disc.assembly_path_used(), "one assembly")}}
 
 
 def batched_leg(steps: int = 20, count: int = 256):
@@ -327,6 +330,13 @@ ASSEMBLY_KERNELS = {
     0: "one assembly = nsfv1::scvf_records<3,2,Q> (Q=0,1,2) + nsfv1::assemble_rows_split<3,2>",
     2: "one assembly = nsfv1::scvf_records<3,2,Q> (Q=0,1,2) + nsfv1::assemble_rows_rowlane<3,2>",
     1: "nsfv1::assemble_rows<3,float,2> (fused, one launch per assembly)",
+}
+
+# the tetrahedral channel (LPS upwind: template argument 3)
+CHANNEL_KERNELS = {
+    0: "one assembly = nsfv1::scvf_records<2,3,Q> + nsfv1::assemble_rows_split<2,3> (split path)",
+    2: "one assembly = nsfv1::scvf_records<2,3,Q> + nsfv1::assemble_rows_rowlane<2,3>",
+    1: "nsfv1::assemble_rows<2,float,3> (fused, one launch per assembly)",
 }
 
 

@@ -382,12 +382,12 @@ int nsfv1_mg_setup(nsfv1_handle* h, int jprec, const double* u, const double* u_
                    double s_m, int parts, void* stream);
 
 /* ---- Learned Newton initial guess (no counterpart in the plugin) ----
-   A per-node MLP 12 -> 128 -> 128 -> 4 with ReLU, BF16 weights and
+   A per-node MLP 24 -> 128 -> 128 -> 4 with ReLU, BF16 weights and
    activations, FP32 accumulation (MFMA).  params: FP32 blob of nparam
-   floats, layout W1[128][16] (input columns 12..15 zero), b1[128],
+   (22800) floats, layout W1[128][32] (input columns 24..31 zero), b1[128],
    W2[128][128], b2[128], W3[16][128] (output rows 4..15 zero), b3[16];
    wbf16: the BF16 copy (same offsets) made by nsfv1_mlp_pack or
-   nsfv1_mlp_adam.  All buffers are device pointers; x [n][16] and y [n][4]
+   nsfv1_mlp_adam.  All buffers are device pointers; x [n][32] and y [n][4]
    FP32, 16-byte aligned. */
 int nsfv1_mlp_sizes(int64_t n, int64_t* nparam, int64_t* work_floats);
 int nsfv1_mlp_pack(const float* params, uint16_t* wbf16, void* stream);
@@ -405,17 +405,27 @@ int nsfv1_mlp_backward(const uint16_t* wbf16, const float* x, const float* z1, c
 /* One Adam step (bias-corrected) on params with moments m, v; refreshes wbf16. */
 int nsfv1_mlp_adam(float* params, float* m, float* v, const float* grad, int step, double lr, double beta1,
                    double beta2, double eps, uint16_t* wbf16, void* stream);
-/* Features of every node of h's mesh (3D, [nv][4] states):
-   x[v] = (u^n, u^n - u^{n-1}, 1-ring mean of u^n - u^n) * scale[12]. */
-int nsfv1_predict_features(nsfv1_handle* h, const double* u, const double* u_prev, const float* scale, float* x,
-                           void* stream);
-/* guess = 2 u^n - u^{n-1} + y * out_scale[4] (linear extrapolation plus the
-   learned correction). */
-int nsfv1_predict_guess(nsfv1_handle* h, const double* u, const double* u_prev, const float* y, const float* out_scale,
-                        double* guess, void* stream);
-/* training target t = (u^{n+1} - 2 u^n + u^{n-1}) / out_scale. */
+/* Learned initial guess, 3D ([nv][4] states, FP64, device pointers).  With
+   u1 = u^{n-1}, u2 = u^{n-2}, u3 = u^{n-3}, the backward differences
+   d1 = u - u1, d2 = u - 2 u1 + u2, d3 = u - 3 u1 + 3 u2 - u3 (evaluated
+   as nested differences of neighbouring states: exactly 0 for equal
+   states) and the
+   step's scale sigma = max(rms of d3 over all velocity components, 1e-30)
+   (a deterministic device reduction, written to sigma[0]):
+     x[v] = clamp((u, d1/sigma, d2/sigma, d3/sigma, ring(u) - u,
+                   (ring(d1) - d1)/sigma) * scale[24], +-64), padded to 32,
+   ring() the mean over the node's 1-ring (its CSR column blocks).  x is
+   16-byte aligned, nv * 32 floats. */
+int nsfv1_predict_features(nsfv1_handle* h, const double* u, const double* u_prev, const double* u_prev2,
+                           const double* u_prev3, const float* scale, float* x, double* sigma, void* stream);
+/* guess = 3 u - 3 u1 + u2 + y * out_scale[4] * sigma (quadratic
+   extrapolation plus the learned correction). */
+int nsfv1_predict_guess(nsfv1_handle* h, const double* u, const double* u_prev, const double* u_prev2, const float* y,
+                        const float* out_scale, const double* sigma, double* guess, void* stream);
+/* training target t = clamp((u^{n+1} - (3 u - 3 u1 + u2)) / (sigma * out_scale), +-64),
+   0 where out_scale is 0. */
 int nsfv1_predict_target(nsfv1_handle* h, const double* u_next, const double* u, const double* u_prev,
-                         const float* out_scale, float* t, void* stream);
+                         const double* u_prev2, const float* out_scale, const double* sigma, float* t, void* stream);
 
 /* Assembly path: 0 evaluates every SCVF once into a per-SCVF record (FP32
    Jacobian, common configuration: FIELDS, one upwind object, no

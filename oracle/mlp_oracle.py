@@ -2,10 +2,11 @@
 
 TEST INFRASTRUCTURE ONLY: the checker of the HIP kernels in
 plugin_navierstokes_amd/csrc/nsfv1_mlp.hip (forward, backward, Adam,
-features, guess and target).  Never imported by the product path.
+features, per-step scale, guess and target).  Never imported by the
+product path.
 
 The network has no counterpart in the reference plugin.  Its arithmetic:
-    x  = bf16(features)                    [n, 16]  (12 used, 4 zero)
+    x  = bf16(features)                    [n, 32]  (24 used, 8 zero)
     z1 = W1 x + b1,  h1 = bf16(relu(z1))   [n, 128]
     z2 = W2 h1 + b2, h2 = bf16(relu(z2))   [n, 128]
     y  = W3 h2 + b3                        [n, 4]
@@ -19,7 +20,8 @@ from __future__ import annotations
 
 import numpy as np
 
-IN, KIN, H, OUT, KOUT = 12, 16, 128, 4, 16
+IN, KIN, H, OUT, KOUT = 24, 32, 128, 4, 16
+CLAMP = 64.0
 OFF_W1 = 0
 OFF_B1 = OFF_W1 + H * KIN
 OFF_W2 = OFF_B1 + H
@@ -121,14 +123,65 @@ def adam(P, M, V, G, step, lr, b1=0.9, b2=0.999, eps=1e-8):
     return P, M, V
 
 
-def features(rowptr, colidx, u, up, scale):
-    """Per-node features (u^n, u^n - u^{n-1}, 1-ring mean of u^n - u^n) * scale,
-    the ring taken from the CSR pattern's column blocks of component 0."""
+def differences(u, u1, u2, u3):
+    """Backward differences d1, d2, d3 of the last four states, as nested
+    differences: d1 = u - u1, e1 = u1 - u2, d2 = d1 - e1, d3 = d2 - (e1 - (u2 - u3))."""
+    d1, e1 = u - u1, u1 - u2
+    d2 = d1 - e1
+    return d1, d2, d2 - (e1 - (u2 - u3))
+
+
+def step_scale(u, u1, u2, u3) -> float:
+    """sigma = max(rms of d3 over the velocity components of all nodes, 1e-30)."""
+    d3 = differences(u, u1, u2, u3)[2][:, :3]
+    r = float(np.sqrt(np.sum(d3 * d3) / (3.0 * u.shape[0]))) if u.shape[0] else 0.0
+    return max(r, 1e-30)
+
+
+def _f32_scaled(a, s):
+    """clamp(float32(a) * float32(s), +-CLAMP) in FP32 arithmetic."""
+    p = np.asarray(a, dtype=np.float64).astype(np.float32) * np.asarray(s, dtype=np.float32)
+    return np.clip(p, np.float32(-CLAMP), np.float32(CLAMP)).astype(np.float64)
+
+
+def features(rowptr, colidx, u, u1, u2, u3, scale, sigma=None):
+    """Per-node features
+        (u, d1/sigma, d2/sigma, d3/sigma, ring(u) - u, (ring(d1) - d1)/sigma) * scale,
+    each clamped to +-CLAMP, ring() the mean over the node's 1-ring taken
+    from the CSR pattern's column blocks of component 0.  Returns (X, sigma)."""
+    if sigma is None:
+        sigma = step_scale(u, u1, u2, u3)
+    d1, d2, d3 = differences(u, u1, u2, u3)
+    isg = 1.0 / sigma
+    scale = np.asarray(scale, dtype=np.float32)
     nv = u.shape[0]
     X = np.zeros((nv, KIN))
     for v in range(nv):
         cols = colidx[rowptr[4 * v]:rowptr[4 * v + 1]:4] // 4
         ring = [w for w in cols if w != v]
-        m = u[ring].mean(0) - u[v] if ring else np.zeros(4)
-        X[v, :12] = np.concatenate([u[v], u[v] - up[v], m]) * scale
-    return X
+        m0 = u[ring].mean(0) - u[v] if ring else np.zeros(4)
+        m1 = (d1[ring].mean(0) - d1[v]) * isg if ring else np.zeros(4)
+        raw = [u[v], d1[v] * isg, d2[v] * isg, d3[v] * isg, m0, m1]
+        for g, a in enumerate(raw):
+            X[v, 4 * g:4 * g + 4] = _f32_scaled(a, scale[4 * g:4 * g + 4])
+    return X, sigma
+
+
+def quadratic(u, u1, u2):
+    """Quadratic extrapolation u + d1 + d2 (= 3 u - 3 u1 + u2, the base of the guess)."""
+    d1 = u - u1
+    return u + d1 + (d1 - (u1 - u2))
+
+
+def guess(u, u1, u2, y, out_scale, sigma):
+    return quadratic(u, u1, u2) + np.asarray(y, dtype=np.float64) * np.asarray(out_scale, dtype=np.float32) * sigma
+
+
+def target(u_next, u, u1, u2, out_scale, sigma):
+    """clamp(((u^{n+1} - u) - d1 - d2) / (sigma * out_scale)) as FP32; 0 where out_scale is 0."""
+    os = np.asarray(out_scale, dtype=np.float32).astype(np.float64)
+    d1 = u - u1
+    r = ((u_next - u) - d1) - (d1 - (u1 - u2))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        t = np.where(os > 0, r / (sigma * np.where(os > 0, os, 1.0)), 0.0)
+    return np.clip(t.astype(np.float32), np.float32(-CLAMP), np.float32(CLAMP)).astype(np.float64)

@@ -126,9 +126,9 @@ def load_library(path: str | None = None):
     L.nsfv1_mlp_mse.argtypes = [vp, vp, C.c_int64, vp, vp, C.POINTER(C.c_double), vp]
     L.nsfv1_mlp_backward.argtypes = [vp, vp, vp, vp, vp, C.c_int64, vp, vp, vp]
     L.nsfv1_mlp_adam.argtypes = [vp, vp, vp, vp, C.c_int, dp, dp, dp, dp, vp, vp]
-    L.nsfv1_predict_features.argtypes = [H, vp, vp, vp, vp, vp]
-    L.nsfv1_predict_guess.argtypes = [H, vp, vp, vp, vp, vp, vp]
-    L.nsfv1_predict_target.argtypes = [H, vp, vp, vp, vp, vp, vp]
+    L.nsfv1_predict_features.argtypes = [H, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.nsfv1_predict_guess.argtypes = [H, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.nsfv1_predict_target.argtypes = [H, vp, vp, vp, vp, vp, vp, vp, vp]
     L.nsfv1_mg_set_hierarchy.argtypes = [H, C.c_int, C.POINTER(H), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
                                          C.POINTER(vp), C.c_int, C.c_int, dp]
     L.nsfv1_mg_setup.argtypes = [H, C.c_int, vp, vp, dp, dp, dp, C.c_int, vp]

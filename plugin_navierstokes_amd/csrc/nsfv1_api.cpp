@@ -170,6 +170,7 @@ struct nsfv1_handle {
     // ndof, dev: Gram-Schmidt projections | ||r||^2 | beta | y | Hessenberg)
     // and the HIP graph of the step
     DevBuf nt_V, nt_w, nt_Binv, nt_part, nt_dev;
+    DevBuf pred_part;  // learned predictor: partial sums of the per-step scale
     int nt_m = 0;
     hipGraph_t nt_graph = nullptr;
     hipGraphExec_t nt_exec = nullptr;
@@ -1563,37 +1564,45 @@ static int predict_check(nsfv1_handle* h)
     return NSFV1_OK;
 }
 
-int nsfv1_predict_features(nsfv1_handle* h, const double* u, const double* u_prev, const float* scale, float* x,
-                           void* stream)
+int nsfv1_predict_features(nsfv1_handle* h, const double* u, const double* u_prev, const double* u_prev2,
+                           const double* u_prev3, const float* scale, float* x, double* sigma, void* stream)
 {
     int rc = predict_check(h);
     if (rc) return rc;
-    if (!u || !u_prev || !scale || !x) return fail(NSFV1_ERR_ARGUMENT, "NULL buffer");
+    if (!u || !u_prev || !u_prev2 || !u_prev3 || !scale || !x || !sigma) return fail(NSFV1_ERR_ARGUMENT, "NULL buffer");
     if ((uintptr_t)x & 15) return fail(NSFV1_ERR_ARGUMENT, "feature buffer must be 16-byte aligned");
-    if (launch_predict_features(h->rowptr.as<int64_t>(), h->colidx.as<int32_t>(), u, u_prev, h->nv, scale, x,
-                                (hipStream_t)stream) != hipSuccess)
+    if (h->pred_part.n < sizeof(double) * PRED_SCALE_BLOCKS &&
+        h->pred_part.alloc(sizeof(double) * PRED_SCALE_BLOCKS) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "predictor scale buffer");
+    hipStream_t s = (hipStream_t)stream;
+    if (launch_predict_scale(u, u_prev, u_prev2, u_prev3, h->nv, h->pred_part.as<double>(), sigma, s) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "predict scale");
+    if (launch_predict_features(h->rowptr.as<int64_t>(), h->colidx.as<int32_t>(), u, u_prev, u_prev2, u_prev3, h->nv,
+                                scale, sigma, x, s) != hipSuccess)
         return fail(NSFV1_ERR_DEVICE, "predict features");
     return NSFV1_OK;
 }
 
-int nsfv1_predict_guess(nsfv1_handle* h, const double* u, const double* u_prev, const float* y, const float* out_scale,
-                        double* guess, void* stream)
+int nsfv1_predict_guess(nsfv1_handle* h, const double* u, const double* u_prev, const double* u_prev2, const float* y,
+                        const float* out_scale, const double* sigma, double* guess, void* stream)
 {
     int rc = predict_check(h);
     if (rc) return rc;
-    if (!u || !u_prev || !y || !out_scale || !guess) return fail(NSFV1_ERR_ARGUMENT, "NULL buffer");
-    if (launch_predict_guess(u, u_prev, y, out_scale, h->nv, guess, (hipStream_t)stream) != hipSuccess)
+    if (!u || !u_prev || !u_prev2 || !y || !out_scale || !sigma || !guess) return fail(NSFV1_ERR_ARGUMENT, "NULL buffer");
+    if (launch_predict_guess(u, u_prev, u_prev2, y, out_scale, sigma, h->nv, guess, (hipStream_t)stream) != hipSuccess)
         return fail(NSFV1_ERR_DEVICE, "predict guess");
     return NSFV1_OK;
 }
 
 int nsfv1_predict_target(nsfv1_handle* h, const double* u_next, const double* u, const double* u_prev,
-                         const float* out_scale, float* t, void* stream)
+                         const double* u_prev2, const float* out_scale, const double* sigma, float* t, void* stream)
 {
     int rc = predict_check(h);
     if (rc) return rc;
-    if (!u_next || !u || !u_prev || !out_scale || !t) return fail(NSFV1_ERR_ARGUMENT, "NULL buffer");
-    if (launch_predict_target(u_next, u, u_prev, out_scale, h->nv, t, (hipStream_t)stream) != hipSuccess)
+    if (!u_next || !u || !u_prev || !u_prev2 || !out_scale || !sigma || !t)
+        return fail(NSFV1_ERR_ARGUMENT, "NULL buffer");
+    if (launch_predict_target(u_next, u, u_prev, u_prev2, out_scale, sigma, h->nv, t, (hipStream_t)stream) !=
+        hipSuccess)
         return fail(NSFV1_ERR_DEVICE, "predict target");
     return NSFV1_OK;
 }

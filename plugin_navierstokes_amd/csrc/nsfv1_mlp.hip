@@ -1,12 +1,12 @@
 // nsfv1_mlp.hip -- learned Newton initial guess: a per-node MLP
-// 12 -> 128 -> 128 -> 4 (ReLU), BF16 weights and activations with FP32
+// 24 -> 128 -> 128 -> 4 (ReLU), BF16 weights and activations with FP32
 // accumulation on the MFMA units (v_mfma_f32_16x16x16_bf16), plus the FP32
 // backward pass, deterministic gradient reduction and Adam used to train it
 // on trajectories of the plain solver, and the feature / guess kernels that
 // connect it to the discretisation.  No counterpart in the reference.
 //
 // Parameter blob (FP32 master copy, MLP_NPARAM floats):
-//   W1 [128][16] (inputs padded 12 -> 16 with zero columns), b1 [128],
+//   W1 [128][32] (inputs padded 24 -> 32 with zero columns), b1 [128],
 //   W2 [128][128], b2 [128], W3 [16][128] (outputs padded 4 -> 16 with zero
 //   rows), b3 [16].
 // The BF16 copy (MLP_NPARAM uint16) uses the same offsets; only its weight
@@ -65,23 +65,27 @@ mlp_forward(const float* __restrict__ P, const uint16_t* __restrict__ WB, const 
     const int64_t node = ((int64_t)blockIdx.x * 4 + wave) * 16 + col;
     const bool valid = node < n;
 
-    // layer 1: B = X^T tile (k = input feature, column = node)
-    bf16x4 xb;
-    {
+    // layer 1: B = X^T tiles (k = input feature, column = node), two k-blocks
+    bf16x4 xb[MLP_KIN / 16];
+#pragma unroll
+    for (int ks = 0; ks < MLP_KIN / 16; ++ks) {
         float xv[4] = {0.f, 0.f, 0.f, 0.f};
         if (valid) {
-            const float4 t = *reinterpret_cast<const float4*>(X + node * MLP_KIN + kq);
+            const float4 t = *reinterpret_cast<const float4*>(X + node * MLP_KIN + ks * 16 + kq);
             xv[0] = t.x; xv[1] = t.y; xv[2] = t.z; xv[3] = t.w;
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) xb[j] = (short)f2bf(xv[j]);
+        for (int j = 0; j < 4; ++j) xb[ks][j] = (short)f2bf(xv[j]);
     }
     bf16x4 h1[MLP_H / 16];
 #pragma unroll
     for (int ot = 0; ot < MLP_H / 16; ++ot) {
-        const bf16x4 a = *reinterpret_cast<const bf16x4*>(&w1[(ot * 16 + col) * MLP_KIN + kq]);
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, xb, acc, 0, 0, 0);
+#pragma unroll
+        for (int ks = 0; ks < MLP_KIN / 16; ++ks) {
+            const bf16x4 a = *reinterpret_cast<const bf16x4*>(&w1[(ot * 16 + col) * MLP_KIN + ks * 16 + kq]);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, xb[ks], acc, 0, 0, 0);
+        }
         float z[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -302,22 +306,86 @@ __global__ void mlp_pack(const float* __restrict__ P, uint16_t* __restrict__ WB)
 }
 
 // ---------------------------------------------------------------------
-// features and guess (node-major states [nv][4], FP64)
+// features, guess and target (node-major states [nv][4], FP64)
+//
+// With the three previous states u1 = u^{n-1}, u2 = u^{n-2}, u3 = u^{n-3},
+// the backward differences are evaluated as nested differences,
+//   d1 = u - u1,  e1 = u1 - u2,  f1 = u2 - u3,
+//   d2 = d1 - e1 (= u - 2 u1 + u2),  d3 = d2 - (e1 - f1) (= u - 3 u1 + 3 u2 - u3),
+// (exactly 0 for equal states, no products to contract), the quadratic
+// extrapolation is q = u + d1 + d2 (= 3 u - 3 u1 + u2),
+// and the step's scale sigma = max(rms of d3 over the velocity components
+// of all nodes, 1e-30).  Differences enter the features divided by sigma,
+// so one network sees slow and fast phases of a trajectory on one scale.
 // ---------------------------------------------------------------------
-// x[v] = ( u^n, u^n - u^{n-1}, mean over the 1-ring of u^n - u^n ) * scale,
-// padded to 16.  The 1-ring is the node's column blocks in the CSR pattern
-// (rows of component 0), the node itself excluded.
+__device__ inline float clampf(float x) { return fminf(fmaxf(x, -PRED_CLAMP), PRED_CLAMP); }
+
+// partial sums of d3^2 over the velocity components, one per workgroup
+// (grid-stride over the nodes, fixed grid: the sum order is fixed)
+__global__ void __launch_bounds__(256)
+predict_scale_partial(const double* __restrict__ u, const double* __restrict__ u1, const double* __restrict__ u2,
+                      const double* __restrict__ u3, int64_t nv, double* __restrict__ part)
+{
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int64_t i = v * 4 + c;
+            const double a = u1[i], b = u2[i], e1 = a - b;
+            const double d3 = ((u[i] - a) - e1) - (e1 - (b - u3[i]));
+            acc += d3 * d3;
+        }
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(256)
+predict_scale_final(const double* __restrict__ part, int nparts, int64_t nv, double* __restrict__ sigma)
+{
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) acc += part[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double r = nv > 0 ? sqrt(sh[0] / (3.0 * (double)nv)) : 0.0;
+        *sigma = r > 1e-30 ? r : 1e-30;
+    }
+}
+
+// x[v] = clamp(( u, d1 / sigma, d2 / sigma, d3 / sigma,
+//                1-ring mean of u - u, (1-ring mean of d1 - d1) / sigma ) * scale),
+// 24 values padded to 32.  The 1-ring is the node's column blocks in the
+// CSR pattern (rows of component 0), the node itself excluded.
 __global__ void predict_features(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colidx,
-                                 const double* __restrict__ u, const double* __restrict__ up, int64_t nv,
-                                 const float* __restrict__ scale, float* __restrict__ X)
+                                 const double* __restrict__ u, const double* __restrict__ u1,
+                                 const double* __restrict__ u2, const double* __restrict__ u3, int64_t nv,
+                                 const float* __restrict__ scale, const double* __restrict__ sigma,
+                                 float* __restrict__ X)
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nv) return;
-    double un[4], d[4], m[4] = {0, 0, 0, 0};
+    const double is = 1.0 / *sigma;
+    double un[4], d1[4], d2[4], d3[4], m0[4] = {0, 0, 0, 0}, m1[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        un[c] = u[v * 4 + c];
-        d[c] = un[c] - up[v * 4 + c];
+        const int64_t i = v * 4 + c;
+        un[c] = u[i];
+        const double a = u1[i], b = u2[i], e1 = a - b;
+        d1[c] = un[c] - a;
+        d2[c] = d1[c] - e1;
+        d3[c] = d2[c] - (e1 - (b - u3[i]));
     }
     const int64_t r0 = rowptr[v * 4], r1 = rowptr[v * 4 + 1];
     int cnt = 0;
@@ -325,16 +393,23 @@ __global__ void predict_features(const int64_t* __restrict__ rowptr, const int32
         const int64_t w = colidx[k] / 4;
         if (w == v) continue;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) m[c] += u[w * 4 + c];
+        for (int c = 0; c < 4; ++c) {
+            const double x = u[w * 4 + c];
+            m0[c] += x;
+            m1[c] += x - u1[w * 4 + c];
+        }
         ++cnt;
     }
     const double ic = cnt > 0 ? 1.0 / cnt : 0.0;
     float out[MLP_KIN];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        out[c] = (float)(un[c]) * scale[c];
-        out[4 + c] = (float)(d[c]) * scale[4 + c];
-        out[8 + c] = (float)(cnt > 0 ? m[c] * ic - un[c] : 0.0) * scale[8 + c];
+        out[c] = clampf((float)un[c] * scale[c]);
+        out[4 + c] = clampf((float)(d1[c] * is) * scale[4 + c]);
+        out[8 + c] = clampf((float)(d2[c] * is) * scale[8 + c]);
+        out[12 + c] = clampf((float)(d3[c] * is) * scale[12 + c]);
+        out[16 + c] = clampf((float)(cnt > 0 ? m0[c] * ic - un[c] : 0.0) * scale[16 + c]);
+        out[20 + c] = clampf((float)(cnt > 0 ? (m1[c] * ic - d1[c]) * is : 0.0) * scale[20 + c]);
     }
 #pragma unroll
     for (int c = MLP_IN; c < MLP_KIN; ++c) out[c] = 0.f;
@@ -343,25 +418,35 @@ __global__ void predict_features(const int64_t* __restrict__ rowptr, const int32
         *reinterpret_cast<float4*>(X + v * MLP_KIN + q) = make_float4(out[q], out[q + 1], out[q + 2], out[q + 3]);
 }
 
-// guess = 2 u^n - u^{n-1} + y * out_scale (per component)
-__global__ void predict_guess(const double* __restrict__ u, const double* __restrict__ up, const float* __restrict__ Y,
-                              const float* __restrict__ oscale, int64_t nv, double* __restrict__ g)
+// guess = u + d1 + d2 + y * out_scale * sigma (per component)
+__global__ void predict_guess(const double* __restrict__ u, const double* __restrict__ u1,
+                              const double* __restrict__ u2, const float* __restrict__ Y,
+                              const float* __restrict__ oscale, const double* __restrict__ sigma, int64_t nv,
+                              double* __restrict__ g)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nv * 4) return;
     const int c = (int)(t & 3);
-    g[t] = 2.0 * u[t] - up[t] + (double)Y[t] * (double)oscale[c];
+    const double d1 = u[t] - u1[t];
+    const double d2 = d1 - (u1[t] - u2[t]);
+    g[t] = u[t] + d1 + d2 + (double)Y[t] * (double)oscale[c] * *sigma;
 }
 
-// training target: t = ((u^{n+1} - u^n) - (u^n - u^{n-1})) / out_scale
+// training target: t = clamp(((u^{n+1} - u) - d1 - d2) / (sigma * out_scale)),
+// 0 for a component whose out_scale is 0 (not corrected)
 __global__ void predict_target(const double* __restrict__ un1, const double* __restrict__ u,
-                               const double* __restrict__ up, const float* __restrict__ oscale, int64_t nv,
+                               const double* __restrict__ u1, const double* __restrict__ u2,
+                               const float* __restrict__ oscale, const double* __restrict__ sigma, int64_t nv,
                                float* __restrict__ T)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nv * 4) return;
     const int c = (int)(t & 3);
-    T[t] = (float)((un1[t] - 2.0 * u[t] + up[t]) / (double)oscale[c]);
+    const double os = (double)oscale[c];
+    const double d1 = u[t] - u1[t];
+    const double d2 = d1 - (u1[t] - u2[t]);
+    const double r = ((un1[t] - u[t]) - d1) - d2;
+    T[t] = os > 0.0 ? clampf((float)(r / (*sigma * os))) : 0.f;
 }
 
 // ---------------------------------------------------------------------
@@ -416,24 +501,36 @@ hipError_t launch_mlp_pack(const float* P, uint16_t* WB, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_predict_features(const int64_t* rowptr, const int32_t* colidx, const double* u, const double* up,
-                                   int64_t nv, const float* scale, float* X, hipStream_t s)
+hipError_t launch_predict_scale(const double* u, const double* u1, const double* u2, const double* u3, int64_t nv,
+                                double* part, double* sigma, hipStream_t s)
 {
-    predict_features<<<nblocks(nv, 256), 256, 0, s>>>(rowptr, colidx, u, up, nv, scale, X);
+    predict_scale_partial<<<PRED_SCALE_BLOCKS, 256, 0, s>>>(u, u1, u2, u3, nv, part);
+    predict_scale_final<<<1, 256, 0, s>>>(part, PRED_SCALE_BLOCKS, nv, sigma);
     return hipGetLastError();
 }
 
-hipError_t launch_predict_guess(const double* u, const double* up, const float* Y, const float* oscale, int64_t nv,
-                                double* g, hipStream_t s)
+hipError_t launch_predict_features(const int64_t* rowptr, const int32_t* colidx, const double* u, const double* u1,
+                                   const double* u2, const double* u3, int64_t nv, const float* scale,
+                                   const double* sigma, float* X, hipStream_t s)
 {
-    predict_guess<<<nblocks(nv * 4, 256), 256, 0, s>>>(u, up, Y, oscale, nv, g);
+    if (nv <= 0) return hipSuccess;
+    predict_features<<<nblocks(nv, 256), 256, 0, s>>>(rowptr, colidx, u, u1, u2, u3, nv, scale, sigma, X);
     return hipGetLastError();
 }
 
-hipError_t launch_predict_target(const double* un1, const double* u, const double* up, const float* oscale, int64_t nv,
-                                 float* T, hipStream_t s)
+hipError_t launch_predict_guess(const double* u, const double* u1, const double* u2, const float* Y,
+                                const float* oscale, const double* sigma, int64_t nv, double* g, hipStream_t s)
 {
-    predict_target<<<nblocks(nv * 4, 256), 256, 0, s>>>(un1, u, up, oscale, nv, T);
+    if (nv <= 0) return hipSuccess;
+    predict_guess<<<nblocks(nv * 4, 256), 256, 0, s>>>(u, u1, u2, Y, oscale, sigma, nv, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_predict_target(const double* un1, const double* u, const double* u1, const double* u2,
+                                 const float* oscale, const double* sigma, int64_t nv, float* T, hipStream_t s)
+{
+    if (nv <= 0) return hipSuccess;
+    predict_target<<<nblocks(nv * 4, 256), 256, 0, s>>>(un1, u, u1, u2, oscale, sigma, nv, T);
     return hipGetLastError();
 }
 

@@ -125,12 +125,15 @@ class NewtonSolver:
         return float(torch.linalg.vector_norm(self.defect(u, u_old, dt, s_m, const, want_J=False)))
 
     def solve(self, u, u_old=None, dt=0.0, s_m=0.0, const=None, ref_norm=None, mg_lag=False,
-              callback=None) -> NewtonStats:
+              callback=None, assembled=None) -> NewtonStats:
         """Iterate until ||d(u)|| <= max(rtol * ref, atol), ref = ref_norm or
         the defect norm of the initial guess.  mg_lag keeps the multigrid
         levels of the first iteration for the whole solve (the Krylov
         operator is always the current Jacobian).  callback(it, u), if
-        given, sees every iterate before its defect is assembled."""
+        given, sees every iterate before its defect is assembled.
+        assembled: ||d(u)|| when self.J, self.d (and the multigrid levels)
+        already hold the full assembly at this u; the first iteration then
+        uses them instead of assembling again."""
         import torch
 
         st = NewtonStats()
@@ -138,7 +141,7 @@ class NewtonSolver:
         self.impose(u)
         uf = u.view(-1)
         try:
-            self._iterate(st, u, uf, u_old, dt, s_m, const, ref_norm, mg_lag, callback)
+            self._iterate(st, u, uf, u_old, dt, s_m, const, ref_norm, mg_lag, callback, assembled)
         finally:
             if self.freeze_after is not None:
                 self.disc.set_upwind_decisions(UPWIND_FRESH)
@@ -149,18 +152,23 @@ class NewtonSolver:
         st.seconds = time.perf_counter() - t0
         return st
 
-    def _iterate(self, st, u, uf, u_old, dt, s_m, const, ref_norm, mg_lag, callback=None):
+    def _iterate(self, st, u, uf, u_old, dt, s_m, const, ref_norm, mg_lag, callback=None, assembled=None):
         import torch
 
         k = self.freeze_after
+        if assembled is not None and k is not None:
+            raise ValueError("a reused first assembly cannot be combined with frozen upwind decisions")
         for it in range(self.maxit + 1):
             if callback is not None:
                 callback(it, u)
             if k is not None:
                 self.disc.set_upwind_decisions(UPWIND_FRESH if it < k - 1 else
                                                (UPWIND_RECORD if it == k - 1 else UPWIND_REPLAY))
-            d = self.defect(u, u_old, dt, s_m, const, mg_setup=(it == 0 or not mg_lag))
-            r = float(torch.linalg.vector_norm(d))
+            if it == 0 and assembled is not None:
+                d, r = self.d, float(assembled)
+            else:
+                d = self.defect(u, u_old, dt, s_m, const, mg_setup=(it == 0 or not mg_lag))
+                r = float(torch.linalg.vector_norm(d))
             st.residuals.append(r)
             ref = ref_norm if ref_norm is not None else st.residuals[0]
             st.reference = ref
@@ -202,10 +210,20 @@ class BDFStepper:
                               want_J=False, check=False)
         return c
 
-    def step(self, u_n, guess=None, candidates=None):
+    def step(self, u_n, guess=None, candidates=None, reuse_last=False, fallback=None):
         """Returns (u^{n+1}, NewtonStats); u_n is not modified.  candidates:
         optional list of initial guesses; the one with the smallest FP64
-        defect starts the Newton iteration (index kept in last_choice)."""
+        defect starts the Newton iteration (index kept in last_choice; a tie
+        goes to the earlier one).  reuse_last: the last candidate is judged
+        with the full assembly (Jacobian, defect, multigrid levels) instead
+        of the defect alone; when it wins, Newton's first iteration reuses
+        that assembly, so judging it costs no extra assembly.  fallback
+        (with guess): the guess is judged against the old state instead, by
+        its full assembly (reused when it passes): the step starts from the
+        fallback when ||d(guess)|| exceeds ||d(u_n)|| of the step's system
+        (last_choice 0), else from the guess (last_choice 1)."""
+        import torch
+
         dt = self.dt
         if self.u_prev is None:
             s_m, const = 1.0 / dt, self.mass_const(u_n, -1.0 / dt)
@@ -215,17 +233,39 @@ class BDFStepper:
         ref = None
         if self.reference == "old":
             ref = self.nt.defect_norm(u_n, u_n, dt, s_m, const)
+        assembled = None
         if candidates:
+            reuse = reuse_last and self.nt.freeze_after is None
             norms = []
-            for c in candidates:
+            for i, c in enumerate(candidates):
+                full = reuse and i == len(candidates) - 1
                 self.nt.impose(c)
-                norms.append(self.nt.defect_norm(c, u_n, dt, s_m, const))
+                d = self.nt.defect(c, u_n, dt, s_m, const, want_J=full, mg_setup=full)
+                norms.append(float(torch.linalg.vector_norm(d)))
             k = int(np.argmin(norms))
             self.last_choice = k
             u = candidates[k].clone()
+            if reuse and k == len(candidates) - 1:
+                assembled = norms[k]
+        elif guess is not None and fallback is not None:
+            if ref is None:
+                ref = self.nt.defect_norm(u_n, u_n, dt, s_m, const)
+            full = self.nt.freeze_after is None
+            u = guess.clone()
+            self.nt.impose(u)
+            r = float(torch.linalg.vector_norm(self.nt.defect(u, u_n, dt, s_m, const, want_J=full, mg_setup=full)))
+            if r <= ref:
+                self.last_choice = 1
+                assembled = r if full else None
+            else:
+                self.last_choice = 0
+                u = fallback.clone()
+            if self.reference != "old":
+                ref = None
         else:
             u = (guess if guess is not None else u_n).clone()
-        st = self.nt.solve(u, u_old=u_n, dt=dt, s_m=s_m, const=const, ref_norm=ref, mg_lag=self.mg_lag)
+        st = self.nt.solve(u, u_old=u_n, dt=dt, s_m=s_m, const=const, ref_norm=ref, mg_lag=self.mg_lag,
+                           assembled=assembled)
         self.u_prev = u_n.clone()
         self.steps += 1
         return u, st

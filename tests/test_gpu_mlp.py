@@ -22,8 +22,8 @@ def _params(seed):
 
 def _inputs(n, seed):
     rng = np.random.default_rng(seed)
-    X = np.zeros((n, 16), np.float32)
-    X[:, :12] = rng.normal(size=(n, 12))
+    X = np.zeros((n, MO.KIN), np.float32)
+    X[:, :MO.IN] = rng.normal(size=(n, MO.IN))
     return X
 
 
@@ -67,7 +67,7 @@ def test_backward_matches_oracle(cuda):
         print(f"grad {name}: relative error {rel:.2e}")
         assert rel <= 2e-3, name
     # padding carries no gradient
-    assert np.all(g[MO.OFF_W1:MO.OFF_B1].reshape(128, 16)[:, 12:] == 0)
+    assert np.all(g[MO.OFF_W1:MO.OFF_B1].reshape(MO.H, MO.KIN)[:, MO.IN:] == 0)
     assert np.all(g[MO.OFF_W3 + 4 * 128:MO.OFF_B3] == 0) and np.all(g[MO.OFF_B3 + 4:] == 0)
 
 
@@ -101,21 +101,32 @@ def test_features_guess_target(cuda):
     disc = NavierStokesFV1("u,v,w,p", "Inner")
     disc.set_mesh(mesh)
     rng = np.random.default_rng(8)
-    u, up, un = (rng.normal(size=(mesh.nv, 4)) for _ in range(3))
-    scale = rng.uniform(0.5, 2, 12).astype(np.float32)
+    u, u1, u2, u3, un = (rng.normal(size=(mesh.nv, 4)) for _ in range(5))
+    scale = rng.uniform(0.5, 2, 24).astype(np.float32)
     osc = rng.uniform(0.5, 2, 4).astype(np.float32)
+    osc[3] = 0.0  # a component without correction
     mlp = MLP(_params(9))
     pr = LearnedPredictor(disc, mlp, scale, osc)
     dev = lambda a: torch.from_numpy(a).to(cuda)
-    X = pr.features(dev(u), dev(up)).cpu().numpy()
+    du, du1, du2, du3 = dev(u), dev(u1), dev(u2), dev(u3)
+    X, sig = pr.features(du, du1, du2, du3)
+    X = X.cpu().numpy()
     rp, ci = O.pattern(O.TET, mesh.nv, mesh.conn)
-    Xr = MO.features(rp, ci, u, up, scale.astype(np.float64))
+    Xr, sr = MO.features(rp, ci, u, u1, u2, u3, scale)
+    s = float(sig)
+    assert abs(s - sr) <= 1e-13 * sr
     np.testing.assert_allclose(X, Xr, rtol=2e-7, atol=1e-6)
-    T = pr.target(dev(un), dev(u), dev(up)).cpu().numpy()
-    np.testing.assert_allclose(T, (un - 2 * u + up) / osc, rtol=1e-6, atol=1e-6)
-    g = pr.guess(dev(u), dev(up)).cpu().numpy()
+    T = pr.target(dev(un), du, du1, du2, sig).cpu().numpy()
+    np.testing.assert_allclose(T, MO.target(un, u, u1, u2, osc, s), rtol=1e-6, atol=1e-6)
+    g = pr.guess(du, du1, du2, du3).cpu().numpy()
     Y = mlp.forward(torch.from_numpy(X).to(cuda)).cpu().numpy().astype(np.float64)
-    np.testing.assert_allclose(g, 2 * u - up + Y * osc, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(g, MO.guess(u, u1, u2, Y, osc, s), rtol=1e-12, atol=1e-12)
+    # clamped features: a state far outside the scale
+    Xc, _ = pr.features(du * 1e3, du1, du2, du3)
+    assert float(Xc.abs().max()) == MO.CLAMP
+    # a stationary state: sigma at its floor, features of the differences 0
+    Xs, ss = pr.features(du, du, du, du)
+    assert float(ss) == 1e-30 and float(Xs[:, 4:16].abs().max()) == 0.0
 
 
 def test_fit_reduces_loss(cuda):

@@ -1,7 +1,9 @@
 """Pins the FP64 MLP oracle (the checker of the learned-guess kernels):
 bf16 rounding on known values, gradients by central finite differences of
 the unrounded network, Adam's first step in closed form, features by brute
-force on a small mesh, and the library's parameter layout."""
+force on a small mesh, the per-step scale, the quadratic base and the
+target on states sampled from polynomials in time (where the extrapolation
+error is known in closed form), and the library's parameter layout."""
 from __future__ import annotations
 
 import numpy as np
@@ -22,8 +24,8 @@ def test_bf16_rounding_known_values():
 
 
 def test_layout_matches_library_constants():
-    assert MO.NPARAM == 20752
-    assert (MO.OFF_B1, MO.OFF_W2, MO.OFF_B2, MO.OFF_W3, MO.OFF_B3) == (2048, 2176, 18560, 18688, 20736)
+    assert MO.NPARAM == 22800
+    assert (MO.OFF_B1, MO.OFF_W2, MO.OFF_B2, MO.OFF_W3, MO.OFF_B3) == (4096, 4224, 20608, 20736, 22784)
 
 
 def test_gradient_finite_differences():
@@ -32,10 +34,10 @@ def test_gradient_finite_differences():
     P[MO.OFF_B1:MO.OFF_W2] = rng.normal(0, 0.1, MO.H)
     P[MO.OFF_B2:MO.OFF_W3] = rng.normal(0, 0.1, MO.H)
     X = np.zeros((7, MO.KIN))
-    X[:, :12] = rng.normal(size=(7, 12))
+    X[:, :MO.IN] = rng.normal(size=(7, MO.IN))
     T = rng.normal(size=(7, 4))
     _, g, _ = MO.loss_and_grad(P, X, T, rounding=False)
-    idx = np.concatenate([rng.choice(np.arange(MO.OFF_W1, MO.OFF_B1).reshape(MO.H, MO.KIN)[:, :12].ravel(), 6),
+    idx = np.concatenate([rng.choice(np.arange(MO.OFF_W1, MO.OFF_B1).reshape(MO.H, MO.KIN)[:, :MO.IN].ravel(), 6),
                           rng.integers(MO.OFF_B1, MO.OFF_W2, 3), rng.integers(MO.OFF_W2, MO.OFF_B2, 6),
                           rng.integers(MO.OFF_B2, MO.OFF_W3, 3),
                           rng.choice(np.arange(MO.OFF_W3, MO.OFF_W3 + 4 * MO.H), 6), MO.OFF_B3 + np.arange(4)])
@@ -59,19 +61,70 @@ def test_adam_first_step_closed_form():
     assert np.all(P1[MO.OFF_B3 + 4:] == 0) and np.all(P1[MO.OFF_W3 + 4 * MO.H:MO.OFF_B3] == 0)
 
 
+def _pattern(mesh):
+    import oracle as O
+
+    return O.pattern(O.TET, mesh.nv, mesh.conn)
+
+
 def test_features_bruteforce():
     mesh = M.cavity_3d(2, M.TET)
     rng = np.random.default_rng(5)
-    u = rng.normal(size=(mesh.nv, 4))
-    up = rng.normal(size=(mesh.nv, 4))
-    scale = rng.uniform(0.5, 2, 12)
-    import oracle as O
-
-    rp, ci = O.pattern(O.TET, mesh.nv, mesh.conn)
-    X = MO.features(rp, ci, u, up, scale)
+    u, u1, u2, u3 = (rng.normal(size=(mesh.nv, 4)) for _ in range(4))
+    scale = rng.uniform(0.5, 2, 24)
+    rp, ci = _pattern(mesh)
+    X, sigma = MO.features(rp, ci, u, u1, u2, u3, scale)
+    # sigma by an explicit loop over nodes and velocity components
+    acc = 0.0
+    for v in range(mesh.nv):
+        for c in range(3):
+            acc += (u[v, c] - 3 * u1[v, c] + 3 * u2[v, c] - u3[v, c]) ** 2
+    assert abs(sigma - np.sqrt(acc / (3 * mesh.nv))) <= 1e-14 * sigma
     # brute force: neighbours = nodes sharing an element
     for v in [0, 13, 26]:
         nb = sorted(set(mesh.conn[np.any(mesh.conn == v, axis=1)].ravel()) - {v})
-        want = np.concatenate([u[v], u[v] - up[v], u[nb].mean(0) - u[v]]) * scale
-        np.testing.assert_allclose(X[v, :12], want, rtol=1e-14)
-        assert np.all(X[v, 12:] == 0)
+        d1 = u - u1
+        want = np.concatenate([u[v], d1[v] / sigma, (u[v] - 2 * u1[v] + u2[v]) / sigma,
+                               (u[v] - 3 * u1[v] + 3 * u2[v] - u3[v]) / sigma, u[nb].mean(0) - u[v],
+                               (d1[nb].mean(0) - d1[v]) / sigma]) * scale
+        np.testing.assert_allclose(X[v, :24], want, rtol=2e-7)  # FP32 features
+        assert np.all(X[v, 24:] == 0)
+    # the clamp bounds every feature
+    Xc, _ = MO.features(rp, ci, u * 1e3, u1, u2, u3, scale)
+    assert np.max(np.abs(Xc)) == MO.CLAMP
+
+
+def _poly_states(nv, coef, t):
+    """States u(t_k) = sum_j coef[j] * t_k^j (per node and component)."""
+    return [sum(c * tk ** j for j, c in enumerate(coef)) for tk in t]
+
+
+def test_quadratic_base_and_target_on_polynomials():
+    rng = np.random.default_rng(11)
+    nv = 9
+    t = [3.0, 2.0, 1.0, 0.0]  # u, u1, u2, u3 at t = 3, 2, 1, 0; u^{n+1} at t = 4
+    # a quadratic trajectory: d3 = 0 (sigma at its floor), the base is exact
+    coef = [rng.normal(size=(nv, 4)) for _ in range(3)]
+    u, u1, u2, u3 = _poly_states(nv, coef, t)
+    un = _poly_states(nv, coef, [4.0])[0]
+    assert MO.step_scale(u, u1, u2, u3) < 1e-13  # rounding of the samples only
+    assert MO.step_scale(u, u, u, u) == 1e-30  # the floor
+    np.testing.assert_allclose(MO.quadratic(u, u1, u2), un, rtol=1e-13, atol=1e-12)
+    # a cubic trajectory c3 t^3: d3 = 6 c3 at every node, and the error of the
+    # quadratic extrapolation u(4) - q equals d3 = 6 c3
+    c3 = np.zeros((nv, 4))
+    c3[:, :3] = 0.5
+    c3[:, 3] = 2.0
+    coef = [rng.normal(size=(nv, 4)), rng.normal(size=(nv, 4)), rng.normal(size=(nv, 4)), c3]
+    u, u1, u2, u3 = _poly_states(nv, coef, t)
+    un = _poly_states(nv, coef, [4.0])[0]
+    sigma = MO.step_scale(u, u1, u2, u3)
+    assert abs(sigma - 3.0) < 1e-12  # rms of 6 * 0.5 over the velocity components
+    np.testing.assert_allclose(un - MO.quadratic(u, u1, u2), 6 * c3, atol=1e-11)
+    osc = np.array([1.0, 0.5, 2.0, 0.0])
+    T = MO.target(un, u, u1, u2, osc, sigma)
+    np.testing.assert_allclose(T[:, :3], (6 * c3[:, :3]) / (3.0 * osc[:3]), rtol=1e-6)
+    assert np.all(T[:, 3] == 0)  # out_scale 0: component not corrected
+    # guess with y = target reproduces u^{n+1} (to the FP32 rounding of t)
+    g = MO.guess(u, u1, u2, T, np.where(osc > 0, osc, 0.0), sigma)
+    np.testing.assert_allclose(g[:, :3], un[:, :3], rtol=1e-6, atol=1e-9)
