@@ -1,7 +1,8 @@
 // nsfv1_mlp.hip -- learned Newton initial guess: a per-node MLP
 // 24 -> 128 -> 128 -> 4 (ReLU), BF16 weights and activations with FP32
-// accumulation on the MFMA units (v_mfma_f32_16x16x16_bf16), plus the FP32
-// backward pass, deterministic gradient reduction and Adam used to train it
+// accumulation on the MFMA units (v_mfma_f32_16x16x16_bf16), plus the
+// backward pass on the MFMA units with FP32 operands
+// (v_mfma_f32_16x16x4_f32), deterministic gradient reduction and Adam used to train it
 // on trajectories of the plain solver, and the feature / guess kernels that
 // connect it to the discretisation.  No counterpart in the reference.
 //
@@ -126,65 +127,99 @@ mlp_forward(const float* __restrict__ P, const uint16_t* __restrict__ WB, const 
 }
 
 // ---------------------------------------------------------------------
-// backward through the activations, one thread per node (FP32):
-//   dz2 = (W3^T dy) * [z2 > 0],  dz1 = (W2^T dz2) * [z1 > 0]
+// backward through the activations on the MFMA units (FP32 operands,
+// v_mfma_f32_16x16x4_f32), one wavefront per 16 nodes, transposed form:
+//   dZ2^T = (W3^T dY^T) * [z2 > 0],  dZ1^T = (W2^T dZ2^T) * [z1 > 0]
 // with the BF16-rounded weights the forward used (straight-through for the
-// activation rounding)
+// activation rounding).  16x16x4 layouts: A[i][k] in lane (i + 16 k),
+// B[k][j] in lane (j + 16 k), D[4 g + r][j] in lane (j + 16 g), element r.
+// The dZ2 accumulators feed the W2^T product as B operands without leaving
+// registers: MFMA (ot, r) sums over the hidden rows k = 16 ot + 4 g + r
+// held by lane group g, and its A operand is gathered for the same k.
 // ---------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 mlp_backward_act(const uint16_t* __restrict__ WB, const float* __restrict__ Z1, const float* __restrict__ Z2,
                  const float* __restrict__ DY, int64_t n, float* __restrict__ DZ1, float* __restrict__ DZ2)
 {
     __shared__ __align__(16) uint16_t w2[MLP_H * MLP_H];
-    __shared__ float w3[MLP_OUT * MLP_H];
+    __shared__ __align__(16) uint16_t w3[MLP_OUT * MLP_H];
     for (int i = threadIdx.x; i < MLP_H * MLP_H; i += blockDim.x) w2[i] = WB[MLP_OFF_W2 + i];
-    for (int i = threadIdx.x; i < MLP_OUT * MLP_H; i += blockDim.x) w3[i] = bf2f(WB[MLP_OFF_W3 + i]);
+    for (int i = threadIdx.x; i < MLP_OUT * MLP_H; i += blockDim.x) w3[i] = WB[MLP_OFF_W3 + i];
     __syncthreads();
-    const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (node >= n) return;
-    float dy[MLP_OUT];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, g = lane >> 4;
+    const int64_t node = ((int64_t)blockIdx.x * 4 + wave) * 16 + col;
+    const bool valid = node < n;
+    // B of the first product: dY^T (k = output 0..3 = g, column = node)
+    const float dyv = valid ? DY[node * MLP_OUT + g] : 0.f;
+    f32x4 dz2[MLP_H / 16];
 #pragma unroll
-    for (int r = 0; r < MLP_OUT; ++r) dy[r] = DY[node * MLP_OUT + r];
-    float dz2[MLP_H];
-#pragma unroll
-    for (int o = 0; o < MLP_H; ++o) {
-        float s = 0.f;
-#pragma unroll
-        for (int r = 0; r < MLP_OUT; ++r) s += w3[r * MLP_H + o] * dy[r];
-        const float z = Z2[node * MLP_H + o];
-        dz2[o] = z > 0.f ? s : 0.f;
-        DZ2[node * MLP_H + o] = dz2[o];
+    for (int ot = 0; ot < MLP_H / 16; ++ot) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(bf2f(w3[g * MLP_H + ot * 16 + col]), dyv, acc, 0, 0, 0);
+        float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (valid) z = *reinterpret_cast<const float4*>(Z2 + node * MLP_H + ot * 16 + 4 * g);
+        dz2[ot][0] = z.x > 0.f ? acc[0] : 0.f;
+        dz2[ot][1] = z.y > 0.f ? acc[1] : 0.f;
+        dz2[ot][2] = z.z > 0.f ? acc[2] : 0.f;
+        dz2[ot][3] = z.w > 0.f ? acc[3] : 0.f;
+        if (valid)
+            *reinterpret_cast<float4*>(DZ2 + node * MLP_H + ot * 16 + 4 * g) =
+                make_float4(dz2[ot][0], dz2[ot][1], dz2[ot][2], dz2[ot][3]);
     }
-    for (int i = 0; i < MLP_H; ++i) {
-        float s = 0.f;
 #pragma unroll
-        for (int o = 0; o < MLP_H; ++o) s += bf2f(w2[o * MLP_H + i]) * dz2[o];
-        const float z = Z1[node * MLP_H + i];
-        DZ1[node * MLP_H + i] = z > 0.f ? s : 0.f;
+    for (int o1 = 0; o1 < MLP_H / 16; ++o1) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ot = 0; ot < MLP_H / 16; ++ot) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float a = bf2f(w2[(ot * 16 + 4 * g + r) * MLP_H + o1 * 16 + col]);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, dz2[ot][r], acc, 0, 0, 0);
+            }
+        }
+        float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (valid) z = *reinterpret_cast<const float4*>(Z1 + node * MLP_H + o1 * 16 + 4 * g);
+        if (valid)
+            *reinterpret_cast<float4*>(DZ1 + node * MLP_H + o1 * 16 + 4 * g) =
+                make_float4(z.x > 0.f ? acc[0] : 0.f, z.y > 0.f ? acc[1] : 0.f, z.z > 0.f ? acc[2] : 0.f,
+                            z.w > 0.f ? acc[3] : 0.f);
     }
 }
 
 // ---------------------------------------------------------------------
-// weight gradients: partial sums over a chunk of nodes per workgroup,
-// written to part[block][MLP_NPARAM]; reduced in block order afterwards
+// weight gradients on the MFMA units: for a chunk of nodes per workgroup,
+//   dW1 = dZ1^T X,  dW2 = dZ2^T H1,  dW3 = dY^T H2
+// as 16x16 output tiles accumulated over the nodes (k = node, 4 per MFMA):
+// W1 8x2 tiles, W2 8x8, W3 1x8 (rows 4..15 zero): 88 tiles, 22 per
+// wavefront.  Nodes are staged in LDS 16 at a time; the biases' sums are
+// taken by 144 threads on the VALU.  Partials go to part[block][MLP_NPARAM]
+// and are reduced in block order afterwards (deterministic).
 // ---------------------------------------------------------------------
-constexpr int WG_SUB = 16;  // nodes staged in LDS at a time
+constexpr int WG_SUB = 16;                      // nodes staged in LDS at a time
+constexpr int WG_TILES = 16 + 64 + 8;           // output tiles of W1, W2, W3
+constexpr int WG_TPW = WG_TILES / 4;            // tiles per wavefront
+constexpr int WG_LD = MLP_H + 4;                // padded LDS row (floats)
 
 __global__ void __launch_bounds__(256)
 mlp_wgrad_partial(const float* __restrict__ X, const float* __restrict__ Z1, const float* __restrict__ Z2,
                   const float* __restrict__ DY, const float* __restrict__ DZ1, const float* __restrict__ DZ2,
                   int64_t n, int64_t chunk, float* __restrict__ part)
 {
-    __shared__ float sx[WG_SUB][MLP_KIN];
-    __shared__ float sh1[WG_SUB][MLP_H];
-    __shared__ float sh2[WG_SUB][MLP_H];
-    __shared__ float sdz1[WG_SUB][MLP_H];
-    __shared__ float sdz2[WG_SUB][MLP_H];
-    __shared__ float sdy[WG_SUB][MLP_KOUT];
-    constexpr int PER = (MLP_NPARAM + 255) / 256;
-    float acc[PER];
+    __shared__ float sx[WG_SUB][MLP_KIN + 4];
+    __shared__ float sh1[WG_SUB][WG_LD];
+    __shared__ float sh2[WG_SUB][WG_LD];
+    __shared__ float sdz1[WG_SUB][WG_LD];
+    __shared__ float sdz2[WG_SUB][WG_LD];
+    __shared__ float sdy[WG_SUB][MLP_KOUT + 4];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int col = lane & 15, g = lane >> 4;
+    f32x4 acc[WG_TPW];
 #pragma unroll
-    for (int q = 0; q < PER; ++q) acc[q] = 0.f;
+    for (int q = 0; q < WG_TPW; ++q) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float bsum = 0.f;   // b1 (threads 0..127) and b2 (threads 128..255) gradient sums
+    float b3sum = 0.f;  // b3 (threads 0..15)
     const int64_t a = (int64_t)blockIdx.x * chunk;
     const int64_t b = a + chunk < n ? a + chunk : n;
     for (int64_t s0 = a; s0 < b; s0 += WG_SUB) {
@@ -208,38 +243,54 @@ mlp_wgrad_partial(const float* __restrict__ X, const float* __restrict__ Z1, con
             sdy[t][f] = (t < cnt && f < MLP_OUT) ? DY[(s0 + t) * MLP_OUT + f] : 0.f;
         }
         __syncthreads();
+        if (threadIdx.x < MLP_H) {
+            for (int t = 0; t < WG_SUB; ++t) bsum += sdz1[t][threadIdx.x];
+        } else if (threadIdx.x < 2 * MLP_H) {
+            for (int t = 0; t < WG_SUB; ++t) bsum += sdz2[t][threadIdx.x - MLP_H];
+        }
+        if (threadIdx.x < MLP_KOUT) {
+            for (int t = 0; t < WG_SUB; ++t) b3sum += sdy[t][threadIdx.x];
+        }
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int p = threadIdx.x + q * 256;
-            if (p >= MLP_NPARAM) continue;
-            float s = 0.f;
-            if (p < MLP_OFF_B1) {
-                const int o = p / MLP_KIN, k = p % MLP_KIN;
-                for (int t = 0; t < WG_SUB; ++t) s += sdz1[t][o] * sx[t][k];
-            } else if (p < MLP_OFF_W2) {
-                const int o = p - MLP_OFF_B1;
-                for (int t = 0; t < WG_SUB; ++t) s += sdz1[t][o];
-            } else if (p < MLP_OFF_B2) {
-                const int o = (p - MLP_OFF_W2) / MLP_H, i = (p - MLP_OFF_W2) % MLP_H;
-                for (int t = 0; t < WG_SUB; ++t) s += sdz2[t][o] * sh1[t][i];
-            } else if (p < MLP_OFF_W3) {
-                const int o = p - MLP_OFF_B2;
-                for (int t = 0; t < WG_SUB; ++t) s += sdz2[t][o];
-            } else if (p < MLP_OFF_B3) {
-                const int r = (p - MLP_OFF_W3) / MLP_H, i = (p - MLP_OFF_W3) % MLP_H;
-                for (int t = 0; t < WG_SUB; ++t) s += sdy[t][r] * sh2[t][i];
-            } else {
-                const int r = p - MLP_OFF_B3;
-                for (int t = 0; t < WG_SUB; ++t) s += sdy[t][r];
+        for (int q = 0; q < WG_TPW; ++q) {
+            const int tile = wave * WG_TPW + q;
+#pragma unroll
+            for (int ks = 0; ks < WG_SUB / 4; ++ks) {
+                const int nd = ks * 4 + g;
+                float av, bv;
+                if (tile < 16) {  // W1: rows o (128), columns = inputs (32)
+                    av = sdz1[nd][(tile >> 1) * 16 + col];
+                    bv = sx[nd][(tile & 1) * 16 + col];
+                } else if (tile < 80) {  // W2
+                    av = sdz2[nd][((tile - 16) >> 3) * 16 + col];
+                    bv = sh1[nd][((tile - 16) & 7) * 16 + col];
+                } else {  // W3 (rows = outputs, padded to 16)
+                    av = sdy[nd][col];
+                    bv = sh2[nd][(tile - 80) * 16 + col];
+                }
+                acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[q], 0, 0, 0);
             }
-            acc[q] += s;
         }
     }
+    float* out = part + (int64_t)blockIdx.x * MLP_NPARAM;
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int p = threadIdx.x + q * 256;
-        if (p < MLP_NPARAM) part[(int64_t)blockIdx.x * MLP_NPARAM + p] = acc[q];
+    for (int q = 0; q < WG_TPW; ++q) {
+        const int tile = wave * WG_TPW + q;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int64_t p;
+            if (tile < 16)
+                p = MLP_OFF_W1 + (int64_t)((tile >> 1) * 16 + 4 * g + r) * MLP_KIN + (tile & 1) * 16 + col;
+            else if (tile < 80)
+                p = MLP_OFF_W2 + (int64_t)(((tile - 16) >> 3) * 16 + 4 * g + r) * MLP_H + ((tile - 16) & 7) * 16 + col;
+            else
+                p = MLP_OFF_W3 + (int64_t)(4 * g + r) * MLP_H + (tile - 80) * 16 + col;
+            out[p] = acc[q][r];
+        }
     }
+    if (threadIdx.x < MLP_H) out[MLP_OFF_B1 + threadIdx.x] = bsum;
+    else if (threadIdx.x < 2 * MLP_H) out[MLP_OFF_B2 + threadIdx.x - MLP_H] = bsum;
+    if (threadIdx.x < MLP_KOUT) out[MLP_OFF_B3 + threadIdx.x] = b3sum;
 }
 
 __global__ void mlp_reduce(const float* __restrict__ part, int nblk, float* __restrict__ grad)
@@ -471,7 +522,7 @@ hipError_t launch_mlp_backward(const uint16_t* WB, const float* X, const float* 
     float* dz1 = work;
     float* dz2 = dz1 + n * MLP_H;
     float* part = dz2 + n * MLP_H;
-    mlp_backward_act<<<nblocks(n, 256), 256, 0, s>>>(WB, Z1, Z2, DY, n, dz1, dz2);
+    mlp_backward_act<<<nblocks(n, 64), 256, 0, s>>>(WB, Z1, Z2, DY, n, dz1, dz2);
     const int64_t chunk = (n + MLP_WG_BLOCKS - 1) / MLP_WG_BLOCKS;
     mlp_wgrad_partial<<<MLP_WG_BLOCKS, 256, 0, s>>>(X, Z1, Z2, DY, dz1, dz2, n, chunk, part);
     mlp_reduce<<<nblocks(MLP_NPARAM, 256), 256, 0, s>>>(part, MLP_WG_BLOCKS, grad);
