@@ -91,10 +91,35 @@ int nsfv1_prepare_setting(nsfv1_handle* h, const int* lfe_type, const int* order
 int nsfv1_disc_type(nsfv1_handle* h, char* buf, size_t n);
 
 /* NavierStokesBase setters: navier_stokes_base.h:170-196,
-   navier_stokes_base.cpp:72-77 (constant user data only). */
+   navier_stokes_base.cpp:72-77, constant user data.  A constant replaces
+   an import set below (and the other way round), as a later set_* call
+   replaces the user data in the reference. */
 int nsfv1_set_kinematic_viscosity(nsfv1_handle* h, double nu);
 int nsfv1_set_source(nsfv1_handle* h, const double* f, int n);
 int nsfv1_set_exact_jacobian(nsfv1_handle* h, double factor);
+
+/* Function-valued kinematic viscosity and source: the
+   set_kinematic_viscosity(CplUserData) / set_source(CplUserData) overloads
+   (register_navier_stokes.cpp:109-118, navier_stokes_base.h:170-189).  The
+   reference imports them at the integration points
+   (navier_stokes_fv1.cpp:188-247: viscosity and source at the SCVF ips,
+   source at the SCV ips); here the caller evaluates its function at the
+   positions nsfv1_ip_positions returns and hands the values over (host
+   arrays, FP64, uploaded once):
+     nu    [ne][nip]       at the SCVF ips (ip = the element's edge number);
+     f_scv [ne][nsh][dim]  at the SCV ips (the corners): the rhs part;
+     f_scvf[ne][nip][dim]  at the SCVF ips: the stabilisation's source.
+   NULL clears (the source needs both arrays).  A new mesh clears them.
+   These run on the generic path (nsfv1_set_assembly_path 1 behaviour);
+   with the FLOW stabilization, the positive or regular upwind, boundary
+   discs or multigrid they are refused with NSFV1_ERR_NOT_IMPLEMENTED. */
+int nsfv1_set_kinematic_viscosity_ip(nsfv1_handle* h, const double* nu, int64_t n);
+int nsfv1_set_source_ip(nsfv1_handle* h, const double* f_scv, int64_t n_scv, const double* f_scvf, int64_t n_scvf);
+/* Global positions of the integration points (host array): kind 0 = SCVF
+   ips [ne][nip][dim] (FV1Geometry: the mean of edge midpoint, element
+   centre and, in 3D, the centres of the edge's two faces), kind 1 = SCV ips
+   [ne][nsh][dim] (the corners). */
+int nsfv1_ip_positions(nsfv1_handle* h, int kind, double* out, int64_t n);
 
 /* IncompressibleNavierStokesBase setters:
    incompressible_navier_stokes_base.h:170-227 */

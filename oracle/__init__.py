@@ -45,6 +45,7 @@ class _Params(C.Structure):
         ("stokes", C.c_int), ("laplace", C.c_int), ("peclet", C.c_int),
         ("exact", C.c_double), ("nu", C.c_double), ("rho", C.c_double),
         ("has_source", C.c_int), ("f", C.c_double * 3), ("force_matrix", C.c_int),
+        ("nu_ip", C.POINTER(C.c_double)), ("f_scv", C.POINTER(C.c_double)), ("f_scvf", C.POINTER(C.c_double)),
     ]
 
 
@@ -83,6 +84,13 @@ class Params:
     rho: float = 1.0
     source: tuple | None = None
     force_matrix: bool = False           # pinning knob: matrix solve of the ip system without ip shapes
+    # imports at the integration points (None: the constants above), indexed
+    # by element for mesh-level calls: nu_ip [ne, nip] (SCVF ips), f_scv
+    # [ne, nsh, dim] (SCV ips), f_scvf [ne, nip, dim] (SCVF ips); the source
+    # arrays need `source` set (it switches the source term on)
+    nu_ip: object = None
+    f_scv: object = None
+    f_scvf: object = None
 
     def c(self) -> _Params:
         p = _Params()
@@ -100,6 +108,14 @@ class Params:
         if self.source is not None:
             for i, v in enumerate(self.source):
                 p.f[i] = float(v)
+        keep = []
+        for name in ("nu_ip", "f_scv", "f_scvf"):
+            a = getattr(self, name)
+            if a is not None:
+                a = np.ascontiguousarray(a, dtype=np.float64)
+                keep.append(a)
+                setattr(p, name, a.ctypes.data_as(C.POINTER(C.c_double)))
+        p._keep = keep  # the arrays live as long as the structure
         return p
 
 

@@ -36,6 +36,13 @@ static char g_err[256];
 static void set_err(const char* m) { snprintf(g_err, sizeof g_err, "%s", m); }
 const char* or_last_error(void) { return g_err; }
 
+/* the imports at the integration points: the kinematic viscosity and the
+   source at the SCVF ips, the source at the SCV ips
+   (navier_stokes_fv1.cpp:188-247; used at :296, 336, 351, 390, 855-868) */
+static double nu_at(const or_params* p, int ip) { return p->nu_ip ? p->nu_ip[ip] : p->nu; }
+static double src_scvf(const or_params* p, int ip, int d, int dim) { return p->f_scvf ? p->f_scvf[ip * dim + d] : p->f[d]; }
+static double src_scv(const or_params* p, int sh, int d, int dim) { return p->f_scv ? p->f_scv[sh * dim + d] : p->f[d]; }
+
 /* ------------------------------------------------------------------ */
 /* reference elements (UG4 corner / edge / side numbering)            */
 /* ------------------------------------------------------------------ */
@@ -790,16 +797,17 @@ static int stab_update(const or_params* p, const or_geo* G, const double* uloc, 
     double L2inv[12];
     diff_length(p->difflen, G, L2inv);
     double vvisc[12], vconv[12] = {0}, vdown[12] = {0};
-    for (int ip = 0; ip < nip; ++ip) vvisc[ip] = p->nu * L2inv[ip];
+    for (int ip = 0; ip < nip; ++ip) vvisc[ip] = nu_at(p, ip) * L2inv[ip];
     if (!p->stokes)
         for (int ip = 0; ip < nip; ++ip) {
             const double nrm = sqrt(dot(dim, stdvel[ip], stdvel[ip]));
             vconv[ip] = nrm / up->len[ip];
             if (flow) vdown[ip] = nrm / (dn->len[ip] + up->len[ip]);
         }
-    double src[3] = {0, 0, 0};
+    double src[12][3] = {{0}};
     if (p->has_source)
-        for (int d = 0; d < dim; ++d) src[d] = p->f[d];
+        for (int ip = 0; ip < nip; ++ip)
+            for (int d = 0; d < dim; ++d) src[ip][d] = src_scvf(p, ip, d, dim);
     double old[12][3] = {{0}};
     if (uold)
         for (int ip = 0; ip < nip; ++ip)
@@ -818,7 +826,7 @@ static int stab_update(const or_params* p, const or_geo* G, const double* uloc, 
             S->diag[ip] = diag;
             const double* sv = stdvel[ip];
             for (int d = 0; d < dim; ++d) {
-                double rhs = src[d];
+                double rhs = src[ip][d];
                 if (uold) rhs += old[ip][d] / dt;
                 S->cst[ip][d] = rhs / diag;
                 for (int k = 0; k < nsh; ++k) {
@@ -892,7 +900,7 @@ static int stab_update(const or_params* p, const or_geo* G, const double* uloc, 
             }
         }
         for (int ip = 0; ip < nip; ++ip) {
-            f0[ip] = src[d];
+            f0[ip] = src[ip][d];
             if (uold) f0[ip] += old[ip][d] / dt;
         }
         if (!flow) {
@@ -912,7 +920,7 @@ static int stab_update(const or_params* p, const or_geo* G, const double* uloc, 
                     f[ip] = f[ip] + UL(dim, k) * contP[k][ip];
                 }
             for (int ip = 0; ip < nip; ++ip) {
-                if (p->has_source) f[ip] += src[d];
+                if (p->has_source) f[ip] += src[ip][d];
                 if (uold) f[ip] += old[ip][d] / dt;
             }
         }
@@ -931,7 +939,7 @@ static double peclet_blend(const or_params* p, const or_geo* G, int ip, const do
 {
     const int dim = G->dim;
     const double Pe = dot(dim, stdvel, G->n[ip]) / dot(dim, G->n[ip], G->n[ip])
-                    * sqrt(dist2(dim, G->co[G->to[ip]], G->co[G->from[ip]])) / p->nu;
+                    * sqrt(dist2(dim, G->co[G->to[ip]], G->co[G->from[ip]])) / nu_at(p, ip);
     const double Pe2 = Pe * Pe;
     const double w = Pe2 / (5.0 + Pe2);
     for (int i = 0; i < dim; ++i) upvel[i] = w * upvel[i] + (1.0 - w) * stdvel[i];
@@ -1021,9 +1029,10 @@ static void conv_upwind_vel(const or_geo* G, const double* uloc, const prolog* W
 static int jac_A(const or_params* p, const or_geo* G, const double* uloc, const prolog* W, double* Jl)
 {
     const int dim = G->dim, nip = G->nip, nsh = G->nsh, nc = dim + 1, P = dim, nl = nsh * nc;
-    const double nu = p->nu, rho = p->rho;
+    const double rho = p->rho;
 #define JL(a, i, b, j) Jl[((i) * nc + (a)) * nl + (j) * nc + (b)]
     for (int ip = 0; ip < nip; ++ip) {
+        const double nu = nu_at(p, ip);
         const int fr = G->from[ip], to = G->to[ip];
         const double* n = G->n[ip];
         for (int sh = 0; sh < nsh; ++sh) {
@@ -1135,9 +1144,10 @@ static int jac_A(const or_params* p, const or_geo* G, const double* uloc, const 
 static int def_A(const or_params* p, const or_geo* G, const double* uloc, const prolog* W, double* dl)
 {
     const int dim = G->dim, nip = G->nip, nsh = G->nsh, nc = dim + 1, P = dim;
-    const double nu = p->nu, rho = p->rho;
+    const double rho = p->rho;
 #define DL(a, i) dl[(i) * nc + (a)]
     for (int ip = 0; ip < nip; ++ip) {
+        const double nu = nu_at(p, ip);
         const int fr = G->from[ip], to = G->to[ip];
         const double* n = G->n[ip];
         double gv[3][3];
@@ -1189,12 +1199,13 @@ static void elem_A_abs(const or_params* p, const or_geo* G, const double* uloc, 
                        const prolog* W, double* Ja, double* da)
 {
     const int dim = G->dim, nip = G->nip, nsh = G->nsh, nc = dim + 1, P = dim, nl = nsh * nc;
-    const double nu = fabs(p->nu), rho = fabs(p->rho);
+    const double rho = fabs(p->rho);
 #define JA(a, i, b, j) Ja[((i) * nc + (a)) * nl + (j) * nc + (b)]
 #define DA(a, i) da[(i) * nc + (a)]
 #define ADDJ(a, b, j, X) do { const double _x = fabs(X); if (Ja) { JA(a, fr, b, j) += _x; JA(a, to, b, j) += _x; } } while (0)
 #define ADDD(a, X) do { const double _x = fabs(X); if (da) { DA(a, fr) += _x; DA(a, to) += _x; } } while (0)
     for (int ip = 0; ip < nip; ++ip) {
+        const double nu = fabs(nu_at(p, ip));
         const int fr = G->from[ip], to = G->to[ip];
         double n[3] = {0, 0, 0};
         for (int i = 0; i < dim; ++i) n[i] = fabs(G->n[ip][i]);
@@ -1228,7 +1239,7 @@ static void elem_A_abs(const or_params* p, const or_geo* G, const double* uloc, 
         } else {
             const double diag = W->S.diag[ip];
             for (int d = 0; d < dim; ++d) {
-                double a = p->has_source ? fabs(p->f[d]) : 0.0;
+                double a = p->has_source ? fabs(src_scvf(p, ip, d, dim)) : 0.0;
                 if (uold) {
                     double o = 0;
                     for (int sh = 0; sh < nsh; ++sh) o += fabs(G->phi[ip][sh] * UO(d, sh));
@@ -1389,7 +1400,7 @@ int or_elem_rhs(const or_params* p, int et, const double* corners, double* fl)
     if (rc) return rc;
     const int dim = G.dim, nsh = G.nsh, nc = dim + 1;
     for (int sh = 0; sh < nsh; ++sh)
-        for (int d1 = 0; d1 < dim; ++d1) fl[sh * nc + d1] += p->f[d1] * G.vol[sh] * p->rho;
+        for (int d1 = 0; d1 < dim; ++d1) fl[sh * nc + d1] += src_scv(p, sh, d1, dim) * G.vol[sh] * p->rho;
     return 0;
 }
 
@@ -1415,9 +1426,15 @@ static int assemble_elem(const or_params* p, int et, const double* xyz, const in
                          const int64_t* rowptr, const int32_t* colidx,
                          double* J, double* d, int absmode)
 {
-    const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1, nl = nsh * nc;
+    const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1, nl = nsh * nc, nip = REF[et].nip;
     double co[24], ul[32], uo[32], Jl[1024], Ja[1024], dl[32], da[32], fl[32];
     const int32_t* c = conn + e * nsh;
+    /* this element's rows of the integration-point imports */
+    or_params q = *p;
+    if (q.nu_ip) q.nu_ip += e * nip;
+    if (q.f_scv) q.f_scv += e * nsh * dim;
+    if (q.f_scvf) q.f_scvf += e * nip * dim;
+    p = &q;
     for (int k = 0; k < nsh; ++k) {
         for (int i = 0; i < dim; ++i) co[k * dim + i] = xyz[(int64_t)c[k] * dim + i];
         for (int a = 0; a < nc; ++a) {

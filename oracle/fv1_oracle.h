@@ -48,6 +48,14 @@ typedef struct {
     int has_source;
     double f[3];
     int force_matrix;  /* pinning knob: solve the ip system as a matrix even without ip shapes */
+    /* imports evaluated at the integration points (NULL: the constants
+       above).  Mesh-level calls index them by element: nu_ip [ne][nip] at
+       the SCVF ips, f_scv [ne][nsh][dim] at the SCV ips (the corners),
+       f_scvf [ne][nip][dim] at the SCVF ips; per-element calls read the
+       element's own rows. */
+    const double* nu_ip;
+    const double* f_scv;
+    const double* f_scvf;
 } or_params;
 
 /* FV1 geometry of one element (restated ugcore FV1Geometry). */

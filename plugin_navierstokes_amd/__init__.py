@@ -36,6 +36,7 @@ ERRORS = {
 EXPORTS = [
     "nsfv1_last_error", "nsfv1_create", "nsfv1_destroy", "nsfv1_prepare_setting", "nsfv1_disc_type",
     "nsfv1_set_kinematic_viscosity", "nsfv1_set_source", "nsfv1_set_exact_jacobian", "nsfv1_set_density",
+    "nsfv1_set_kinematic_viscosity_ip", "nsfv1_set_source_ip", "nsfv1_ip_positions",
     "nsfv1_set_stokes", "nsfv1_set_laplace", "nsfv1_set_peclet_blend", "nsfv1_set_grad_div",
     "nsfv1_set_upwind", "nsfv1_set_stabilization", "nsfv1_set_stabilization_without",
     "nsfv1_set_stabilization_upwind", "nsfv1_set_pac_upwind", "nsfv1_set_mesh", "nsfv1_num_dofs", "nsfv1_nnz",
@@ -87,6 +88,9 @@ def load_library(path: str | None = None):
     for n in ["nsfv1_set_stokes", "nsfv1_set_laplace", "nsfv1_set_peclet_blend", "nsfv1_set_pac_upwind"]:
         getattr(L, n).argtypes = [H, C.c_int]
     L.nsfv1_set_source.argtypes = [H, C.POINTER(dp), C.c_int]
+    L.nsfv1_set_kinematic_viscosity_ip.argtypes = [H, C.POINTER(dp), C.c_int64]
+    L.nsfv1_set_source_ip.argtypes = [H, C.POINTER(dp), C.c_int64, C.POINTER(dp), C.c_int64]
+    L.nsfv1_ip_positions.argtypes = [H, C.c_int, C.POINTER(dp), C.c_int64]
     L.nsfv1_set_upwind.argtypes = [H, C.c_char_p]
     L.nsfv1_set_stabilization.argtypes = [H, C.c_char_p, C.c_char_p]
     L.nsfv1_set_stabilization_without.argtypes = [H]
@@ -176,12 +180,62 @@ class NavierStokesFV1:
             self._h = None
 
     # --- NavierStokesBase ---
-    def set_kinematic_viscosity(self, nu: float):
-        _check(_lib.nsfv1_set_kinematic_viscosity(self._h, float(nu)))
+    def set_kinematic_viscosity(self, nu):
+        """A number (constant user data), or a function of position: a
+        callable x [npts, dim] -> [npts], evaluated at the SCVF integration
+        points (navier_stokes_fv1.cpp:192, 243), or the values there
+        ([ne, nip]).  A function is evaluated again after a new mesh."""
+        self._nu_fn = None
+        if callable(nu) or isinstance(nu, np.ndarray):
+            self._nu_fn = nu
+            self._apply_nu_fn()
+        else:
+            _check(_lib.nsfv1_set_kinematic_viscosity(self._h, float(nu)))
 
     def set_source(self, f):
+        """A vector (constant user data), None (no source), a function of
+        position x [npts, dim] -> [npts, dim] evaluated at the SCV and SCVF
+        integration points (navier_stokes_fv1.cpp:195-196, 246-247), or the
+        pair of values there ([ne, nsh, dim], [ne, nip, dim])."""
+        self._f_fn = None
+        if callable(f) or (isinstance(f, tuple) and len(f) == 2 and isinstance(f[0], np.ndarray)):
+            self._f_fn = f
+            self._apply_f_fn()
+            return
         arr = (C.c_double * len(f))(*[float(x) for x in f]) if f is not None else None
         _check(_lib.nsfv1_set_source(self._h, arr, 0 if f is None else len(f)))
+
+    def ip_positions(self, kind: str = "scvf") -> np.ndarray:
+        """Global integration points: "scvf" [ne, nip, dim] or "scv" [ne, nsh, dim] (the corners)."""
+        k = {"scvf": 0, "scv": 1}[kind]
+        per = {0: (3, 3), 1: (4, 4), 2: (6, 4), 3: (12, 8)}[self._et][k]
+        out = np.zeros((self.ne, per, self.dim))
+        _check(_lib.nsfv1_ip_positions(self._h, k, out.ctypes.data_as(C.POINTER(C.c_double)), out.size))
+        return out
+
+    def _apply_nu_fn(self):
+        fn = getattr(self, "_nu_fn", None)
+        if fn is None:
+            return
+        if isinstance(fn, np.ndarray):
+            vals = np.ascontiguousarray(fn, dtype=np.float64)
+        else:
+            x = self.ip_positions("scvf")
+            vals = np.ascontiguousarray(np.asarray(fn(x.reshape(-1, self.dim)), dtype=np.float64).reshape(x.shape[:2]))
+        _check(_lib.nsfv1_set_kinematic_viscosity_ip(self._h, vals.ctypes.data_as(C.POINTER(C.c_double)), vals.size))
+
+    def _apply_f_fn(self):
+        fn = getattr(self, "_f_fn", None)
+        if fn is None:
+            return
+        if isinstance(fn, tuple):
+            a, b = (np.ascontiguousarray(v, dtype=np.float64) for v in fn)
+        else:
+            xs, xf = self.ip_positions("scv"), self.ip_positions("scvf")
+            a = np.ascontiguousarray(np.asarray(fn(xs.reshape(-1, self.dim)), dtype=np.float64).reshape(xs.shape))
+            b = np.ascontiguousarray(np.asarray(fn(xf.reshape(-1, self.dim)), dtype=np.float64).reshape(xf.shape))
+        dp = C.POINTER(C.c_double)
+        _check(_lib.nsfv1_set_source_ip(self._h, a.ctypes.data_as(dp), a.size, b.ctypes.data_as(dp), b.size))
 
     def set_exact_jacobian(self, factor):
         _check(_lib.nsfv1_set_exact_jacobian(self._h, (1.0 if factor else 0.0) if isinstance(factor, bool) else float(factor)))
@@ -244,6 +298,12 @@ class NavierStokesFV1:
         self.nv = xyz.shape[0]
         self.ne = conn.shape[0]
         self.dim = 2 if int(et) in (0, 1) else 3
+        self._et = int(et)
+        # functions of position are imports: evaluated again on the new mesh
+        if callable(getattr(self, "_nu_fn", None)):
+            self._apply_nu_fn()
+        if callable(getattr(self, "_f_fn", None)):
+            self._apply_f_fn()
 
     def num_dofs(self) -> int:
         n = C.c_int64()

@@ -171,6 +171,10 @@ struct nsfv1_handle {
     // and the HIP graph of the step
     DevBuf nt_V, nt_w, nt_Binv, nt_part, nt_dev;
     DevBuf pred_part;  // learned predictor: partial sums of the per-step scale
+    // imports at the integration points (function-valued viscosity and
+    // source, evaluated by the caller): [ne][nip], [ne][nsh][dim], [ne][nip][dim]
+    DevBuf nu_ip, f_scv, f_scvf;
+    bool ip_imports() const { return nu_ip.p != nullptr || f_scv.p != nullptr; }
     int nt_m = 0;
     hipGraph_t nt_graph = nullptr;
     hipGraphExec_t nt_exec = nullptr;
@@ -180,6 +184,57 @@ struct nsfv1_handle {
         if (nt_graph) (void)hipGraphDestroy(nt_graph);
     }
 };
+
+// ---- imports at the integration points (helpers of the entry points below) ----
+static int nsh_count(int et) { return et == 0 ? 3 : (et == 1 ? 4 : (et == 2 ? 4 : 8)); }
+static int nip_of(int et) { return et == 0 ? 3 : (et == 1 ? 4 : (et == 2 ? 6 : 12)); }
+
+static int upload(DevBuf& b, const double* x, int64_t n)
+{
+    if (b.alloc(sizeof(double) * (size_t)n) != hipSuccess ||
+        hipMemcpy(b.p, x, sizeof(double) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "import upload");
+    return NSFV1_OK;
+}
+
+template <int ET>
+static void ip_positions_et(const std::vector<double>& xyz, const std::vector<int32_t>& conn, int64_t ne, int kind,
+                            double* out)
+{
+    using T = Topo<ET>;
+    constexpr int D = T::dim, N = T::nsh;
+    for (int64_t e = 0; e < ne; ++e) {
+        double co[N][3] = {}, c[3] = {0, 0, 0};
+        for (int k = 0; k < N; ++k)
+            for (int i = 0; i < D; ++i) {
+                co[k][i] = xyz[(size_t)conn[(size_t)e * N + k] * D + i];
+                c[i] += co[k][i];
+            }
+        for (int i = 0; i < D; ++i) c[i] /= N;
+        if (kind == 1) {
+            for (int k = 0; k < N; ++k)
+                for (int i = 0; i < D; ++i) out[((size_t)e * N + k) * D + i] = co[k][i];
+            continue;
+        }
+        for (int ip = 0; ip < T::nip; ++ip) {
+            double me[3];
+            for (int i = 0; i < D; ++i) me[i] = 0.5 * (co[T::edge[ip][0]][i] + co[T::edge[ip][1]][i]);
+            double* o = out + ((size_t)e * T::nip + ip) * D;
+            if constexpr (D == 2) {
+                for (int i = 0; i < 2; ++i) o[i] = 0.5 * (me[i] + c[i]);
+            } else {
+                double fc[2][3] = {};
+                for (int q = 0; q < 2; ++q) {
+                    const int s = edge_face<ET>(ip, q);
+                    for (int j = 0; j < T::side_n[s]; ++j)
+                        for (int i = 0; i < 3; ++i) fc[q][i] += co[T::side[s][j]][i];
+                    for (int i = 0; i < 3; ++i) fc[q][i] /= T::side_n[s];
+                }
+                for (int i = 0; i < 3; ++i) o[i] = 0.25 * (me[i] + fc[0][i] + c[i] + fc[1][i]);
+            }
+        }
+    }
+}
 
 extern "C" {
 
@@ -251,6 +306,7 @@ int nsfv1_disc_type(nsfv1_handle* h, char* buf, size_t n)
 int nsfv1_set_kinematic_viscosity(nsfv1_handle* h, double nu)
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    h->nu_ip.release();
     h->nu = nu;
     h->nu_set = true;
     h->nu_global_set = true;
@@ -260,12 +316,76 @@ int nsfv1_set_kinematic_viscosity(nsfv1_handle* h, double nu)
 int nsfv1_set_source(nsfv1_handle* h, const double* f, int n)
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    h->f_scv.release();
+    h->f_scvf.release();
     if (!f || n <= 0) {
         h->has_source = false;
         return NSFV1_OK;
     }
     for (int i = 0; i < 3; ++i) h->f[i] = i < n && i < h->dim ? f[i] : 0.0;
     h->has_source = true;
+    return NSFV1_OK;
+}
+
+int nsfv1_set_kinematic_viscosity_ip(nsfv1_handle* h, const double* nu, int64_t n)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (!nu) {
+        h->nu_ip.release();
+        return NSFV1_OK;
+    }
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    if (n != h->ne * nip_of(h->et))
+        return fail(NSFV1_ERR_ARGUMENT, "viscosity at the SCVF ips: expected ne * nip = " +
+                                            std::to_string(h->ne * nip_of(h->et)) + " values");
+    int rc = upload(h->nu_ip, nu, n);
+    if (rc) return rc;
+    h->nu_set = true;
+    h->nu_global_set = true;
+    return NSFV1_OK;
+}
+
+int nsfv1_set_source_ip(nsfv1_handle* h, const double* f_scv, int64_t n_scv, const double* f_scvf, int64_t n_scvf)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (!f_scv && !f_scvf) {
+        h->f_scv.release();
+        h->f_scvf.release();
+        h->has_source = false;
+        return NSFV1_OK;
+    }
+    if (!f_scv || !f_scvf) return fail(NSFV1_ERR_ARGUMENT, "the source import needs its values at both the SCV and the SCVF ips");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    const int64_t a = h->ne * nsh_count(h->et) * h->dim, b = h->ne * nip_of(h->et) * h->dim;
+    if (n_scv != a || n_scvf != b)
+        return fail(NSFV1_ERR_ARGUMENT, "source at the ips: expected ne * nsh * dim = " + std::to_string(a) +
+                                            " and ne * nip * dim = " + std::to_string(b) + " values");
+    int rc = upload(h->f_scv, f_scv, n_scv);
+    if (!rc) rc = upload(h->f_scvf, f_scvf, n_scvf);
+    if (rc) return rc;
+    for (int i = 0; i < 3; ++i) h->f[i] = 0.0;
+    h->has_source = true;
+    return NSFV1_OK;
+}
+
+int nsfv1_ip_positions(nsfv1_handle* h, int kind, double* out, int64_t n)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    if (kind != 0 && kind != 1) return fail(NSFV1_ERR_ARGUMENT, "kind: 0 = SCVF ips, 1 = SCV ips");
+    const int64_t want = h->ne * (kind == 0 ? nip_of(h->et) : nsh_count(h->et)) * h->dim;
+    if (!out || n != want) return fail(NSFV1_ERR_ARGUMENT, "expected " + std::to_string(want) + " values");
+    std::vector<double> xyz((size_t)h->nv * h->dim);
+    std::vector<int32_t> conn((size_t)h->ne * nsh_count(h->et));
+    if (hipMemcpy(xyz.data(), h->xyz.p, sizeof(double) * xyz.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(conn.data(), h->conn.p, sizeof(int32_t) * conn.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "mesh download");
+    switch (h->et) {
+    case 0: ip_positions_et<0>(xyz, conn, h->ne, kind, out); break;
+    case 1: ip_positions_et<1>(xyz, conn, h->ne, kind, out); break;
+    case 2: ip_positions_et<2>(xyz, conn, h->ne, kind, out); break;
+    default: ip_positions_et<3>(xyz, conn, h->ne, kind, out); break;
+    }
     return NSFV1_OK;
 }
 
@@ -583,6 +703,12 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
     h->upw_dec.release();
     h->upw_recorded = false;
     h->nu_elem.release();
+    h->nu_ip.release();
+    if (h->f_scv.p) {
+        h->f_scv.release();
+        h->f_scvf.release();
+        h->has_source = false;
+    }
     h->ninst = 0;
     h->recs.release();
     h->cpl_recs.release();
@@ -807,6 +933,15 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
                 if (h->bnd[kind].nbn)
                     return fail(NSFV1_ERR_NOT_IMPLEMENTED,
                                 "boundary discs with per-instance viscosity (batched instances) are not provided");
+        if (h->ip_imports()) {
+            for (int kind = 0; kind < BND_KINDS; ++kind)
+                if (h->bnd[kind].nbn)
+                    return fail(NSFV1_ERR_NOT_IMPLEMENTED,
+                                "boundary discs with imports at the integration points are not provided");
+            if (coupled_config(h))
+                return fail(NSFV1_ERR_NOT_IMPLEMENTED, "imports at the integration points are not provided with the "
+                                                       "FLOW stabilization or the positive/regular upwinds");
+        }
     }
     Plan& P = jprec == NSFV1_F64 ? h->plan64 : h->plan32;
     if (!P.valid) {
@@ -856,12 +991,13 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
                 h->node_inc.as<int32_t>(), h->inc_elem.as<int32_t>(), h->inc_node.as<int32_t>(),
                 h->inc_info.as<int32_t>(), P.wg_node.as<int32_t>(), P.wg_rounds.as<int32_t>(), h->l2inv.as<double>(),
                 h->vol.as<double>(), h->upw_dec.as<uint8_t>(), h->affine.as<uint8_t>(), h->inc_pos.as<uint64_t>(),
-                P.perm.as<int32_t>(), h->nu_elem.as<double>(), P.block, P.lane_a.as<int4>(),
-                P.lane_pos.as<uint64_t>(), P.wg_a.as<int4>()};
+                P.perm.as<int32_t>(), h->nu_elem.as<double>(), h->nu_ip.as<double>(), h->f_scv.as<double>(),
+                h->f_scvf.as<double>(), P.block, P.lane_a.as<int4>(), P.lane_pos.as<uint64_t>(), P.wg_a.as<int4>()};
     hipError_t e;
     int mfast = -1;
     if (h->path == 3 && h->march_ok && jprec == NSFV1_F32 && (parts & NSFV1_PART_A) && h->difflen == DL_RAW &&
-        !h->nu_elem.p && k.stab_type == STAB_FIELDS && !k.stokes && !k.pac && !k.peclet && k.exact == 0.0 &&
+        !h->nu_elem.p && !h->ip_imports() && k.stab_type == STAB_FIELDS && !k.stokes && !k.pac && !k.peclet &&
+        k.exact == 0.0 &&
         k.conv_upwind == k.stab_upwind && (k.conv_upwind == UPW_NO || k.conv_upwind == UPW_FULL || k.conv_upwind == UPW_SKEWED) &&
         ((uintptr_t)jvals & 15) == 0 && ((uintptr_t)defect & 15) == 0)
         mfast = k.conv_upwind;
@@ -900,7 +1036,7 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
                                     h->ne, recs, nullptr, (hipStream_t)stream);
         h->last_path = 4;
     } else if ((h->path == 0 || h->path == 2 || h->path == 3) && jprec == NSFV1_F32 && (parts & NSFV1_PART_A) &&
-               split_supported(h->et, k)) {
+               !h->ip_imports() && split_supported(h->et, k)) {
         // also without a Jacobian (defect only): the gather then reads only
         // the records' fluxes, and the defect is bitwise the one assembled
         // together with the Jacobian
@@ -1142,7 +1278,13 @@ int nsfv1_export(nsfv1_handle* h, int what, int at, const double* local_ips, int
         m.vol = h->vol.as<double>();
         m.affine = h->affine.as<uint8_t>();
         m.nu_elem = h->nu_elem.as<double>();
+        m.nu_ip = h->nu_ip.as<double>();
+        m.f_scv = h->f_scv.as<double>();
+        m.f_scvf = h->f_scvf.as<double>();
         hipError_t e;
+        if (coupled_config(h) && h->ip_imports())
+            return fail(NSFV1_ERR_NOT_IMPLEMENTED, "imports at the integration points are not provided with the "
+                                                   "FLOW stabilization or the positive/regular upwinds");
         if (coupled_config(h)) {
             k.conv_upwind = h->conv_upwind;
             k.pac = h->pac;
@@ -1387,6 +1529,8 @@ int nsfv1_mg_setup(nsfv1_handle* h, int jprec, const double* u, const double* u_
     if (h->mg.empty()) return fail(NSFV1_ERR_ARGUMENT, "no multigrid hierarchy set");
     if (h->nu_elem.p)
         return fail(NSFV1_ERR_NOT_IMPLEMENTED, "multigrid with per-instance viscosity (batched instances) is not provided");
+    if (h->ip_imports())
+        return fail(NSFV1_ERR_NOT_IMPLEMENTED, "multigrid with imports at the integration points is not provided");
     // the coarse levels take the fine level's current configuration
     mg_sync_settings(h);
     if (!u) return fail(NSFV1_ERR_ARGUMENT, "u is NULL");

@@ -206,6 +206,11 @@ template <int FAST>
 struct Cfg {
     const KParams& P;
     double nu;  // kinematic viscosity of the lane's element (batched instances carry their own)
+    // the element's rows of the imports at the integration points
+    // (navier_stokes_fv1.cpp:188-247), or null: [nip] viscosity, [nip][dim]
+    // source at the SCVF ips, indexed by the original SCVF (= edge) number
+    const double* nu_ip = nullptr;
+    const double* f_scvf = nullptr;
     __device__ bool stokes() const { return FAST >= 0 ? false : P.stokes != 0; }
     __device__ bool pac() const { return FAST >= 0 ? false : P.pac != 0; }
     __device__ bool peclet() const { return FAST >= 0 ? false : P.peclet != 0; }
@@ -221,6 +226,7 @@ template <int ET, typename JT>
 struct Rec {
     static constexpr int D = Topo<ET>::dim, N = Topo<ET>::nsh;
     JT n[D], Ji[D][D], pw, pw1, vv, vc, sp, w, up[D];
+    JT nr;  // -nu rho at the SCVF
     JT Us[N], Uc[N];
 };
 
@@ -413,7 +419,14 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
     constexpr int IP = corner_ip<ET>(0, Q);
     constexpr int OT = T::edge[IP][0] == 0 ? T::edge[IP][1] : T::edge[IP][0];
     const KParams& P = C.P;
-    const double nu = C.nu, rho = P.rho;
+    double nu = C.nu;
+    const double rho = P.rho;
+    const double* fip = nullptr;  // source at this SCVF's ip (import), or the constant
+    if (C.nu_ip || C.f_scvf) {
+        const int oip = edge_between<ET>(pk[0], pk[OT]);
+        if (C.nu_ip) nu = C.nu_ip[oip];
+        if (C.f_scvf) fip = C.f_scvf + oip * D;
+    }
 
     double ipg[D], n[D], me[D];
     {
@@ -520,7 +533,7 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
         idg = 1.0 / diag;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            double rhs = P.has_source ? P.f[d] : 0.0;
+            double rhs = P.has_source ? (fip ? fip[d] : P.f[d]) : 0.0;
             if (P.has_old) {
                 double o = 0.0;
 #pragma unroll
@@ -572,6 +585,7 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
     L.dacc[D] += sgn * (dot_nc<D>(svel, n) * rho);
 
     if (!wantJ) return;
+    R.nr = JT(-1.0 * nu * rho);
 #pragma unroll
     for (int i = 0; i < D; ++i) {
         R.n[i] = JT(n[i]);
@@ -598,7 +612,7 @@ __device__ inline void scvf_jac(const Cfg<FAST>& C, const Rec<ET, JT>& R, Lane<E
     constexpr int IP = corner_ip<ET>(0, Q);
     const KParams& P = C.P;
     const bool fields = C.fields();
-    const JT s_ = JT(P.s_a), rh = JT(P.rho), nr = JT(-1.0 * C.nu * P.rho), ex = JT(P.exact * P.rho);
+    const JT s_ = JT(P.s_a), rh = JT(P.rho), nr = R.nr, ex = JT(P.exact * P.rho);
 #pragma unroll
     for (int sh = 0; sh < N; ++sh) {
         JT g[D];
@@ -788,7 +802,8 @@ assemble_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const double
             double c[D];
             elem_centre<ET>(co, c);
             Rec<ET, JT> R[T::cip];
-            const Cfg<FAST> C{P, M.nu_elem ? M.nu_elem[e] : P.nu};
+            const Cfg<FAST> C{P, M.nu_elem ? M.nu_elem[e] : P.nu, M.nu_ip ? M.nu_ip + (int64_t)e * NIP : nullptr,
+                              M.f_scvf ? M.f_scvf + (int64_t)e * NIP * D : nullptr};
             scvf_all<ET, FAST, 0, JT>(C, co, M.xyz, M.conn + (int64_t)e * N, ul, pk, ccn, c, uold, M.l2inv + (int64_t)e * NIP,
                                       (P.upw_mode != 0 && M.upw_dec) ? M.upw_dec + (int64_t)e * NIP : nullptr, kv, wantJ, L, R,
                                       okg, oku, (ET == 0 || ET == 2) || (M.affine != nullptr && M.affine[e] != 0));
@@ -807,7 +822,10 @@ assemble_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const double
             }
             if ((P.parts & 4) && P.has_source)
 #pragma unroll
-                for (int d = 0; d < D; ++d) L.dacc[d] += P.s_f * (P.f[d] * volk * P.rho);
+                for (int d = 0; d < D; ++d) {
+                    const double fk = M.f_scv ? M.f_scv[((int64_t)e * N + kv) * D + d] : P.f[d];
+                    L.dacc[d] += P.s_f * (fk * volk * P.rho);
+                }
         }
         if (!oku) kernel_error(err, KERR_CUT_SIDE);
     if (!okg) kernel_error(err, KERR_DEGENERATE);
@@ -1117,6 +1135,7 @@ __device__ inline void gather_jac(const Cfg<FAST>& C, const float* __restrict__ 
             }
             R.pw = sg * rv[RL::S + 0];
             R.pw1 = 0.f;
+            R.nr = float(-1.0 * C.nu * C.P.rho);
             R.vv = rv[RL::S + 1];
             R.vc = rv[RL::S + 2];
             R.sp = rv[RL::S + 3];
@@ -1223,7 +1242,10 @@ assemble_rows_split(KMeshPtrs M, KParams P, const double* __restrict__ u, const 
             }
             if ((P.parts & 4) && P.has_source)
 #pragma unroll
-                for (int d = 0; d < D; ++d) L.dacc[d] += P.s_f * (P.f[d] * volk * P.rho);
+                for (int d = 0; d < D; ++d) {
+                    const double fk = M.f_scv ? M.f_scv[((int64_t)e * N + kv) * D + d] : P.f[d];
+                    L.dacc[d] += P.s_f * (fk * volk * P.rho);
+                }
         }
 #pragma unroll
         for (int k = 0; k < N; ++k) pos[k] = (int)((pw >> (8 * pk[k])) & 0xffu) * NC;
@@ -1690,7 +1712,8 @@ export_stab_vel(KMeshPtrs M, KParams P, const double* __restrict__ u, const doub
     double c[D];
     elem_centre<ET>(co, c);
     bool okg = true, oku = true;
-    const Cfg<-1> C{P, M.nu_elem ? M.nu_elem[e] : P.nu};
+    const Cfg<-1> C{P, M.nu_elem ? M.nu_elem[e] : P.nu, M.nu_ip ? M.nu_ip + e * NIP : nullptr,
+                    M.f_scvf ? M.f_scvf + e * NIP * D : nullptr};
     stab_vel_all<ET, 0>(C, co, M.xyz, M.conn + e * N, ul, pk, ccn, c, uold, M.l2inv + e * NIP, kv,
                         out + e * NIP * D, okg, oku, (ET == 0 || ET == 2) || (M.affine != nullptr && M.affine[e] != 0));
     if (!oku) kernel_error(err, KERR_CUT_SIDE);

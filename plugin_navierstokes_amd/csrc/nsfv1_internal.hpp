@@ -28,6 +28,9 @@ struct KMeshPtrs {
     const uint64_t* inc_pos;  // [ninc] byte c = column block of corner c in the node's block row
     const int32_t* perm;      // [ninc] lane order of the plan: incidences of a workgroup sorted by slot
     const double* nu_elem;    // [ne] per-element kinematic viscosity (batched instances), or null
+    const double* nu_ip;      // [ne][nip] kinematic viscosity at the SCVF ips (function-valued import), or null
+    const double* f_scv;      // [ne][nsh][dim] source at the SCV ips (the corners), or null
+    const double* f_scvf;     // [ne][nip][dim] source at the SCVF ips (stabilisation), or null
     int block;                // threads per workgroup of the launch (64..256)
     const int4* lane_a;       // [ninc] in lane order: {element, node, corner | slot << 8 | row length << 16, LDS row base}
     const uint64_t* lane_pos; // [ninc] in lane order: inc_pos of the lane's incidence
