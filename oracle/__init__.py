@@ -21,13 +21,13 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "lib", "libfv1oracle.so")
 
-TRI, QUAD, TET, HEX = 0, 1, 2, 3
+TRI, QUAD, TET, HEX, PRISM = 0, 1, 2, 3, 4
 UPW = {"no": 0, "full": 1, "skewed": 2, "lps": 3, "linearprofileskewed": 3, "positive": 4, "regular": 5}
 STAB_FIELDS, STAB_WITHOUT, STAB_FLOW = 0, 1, 2
 DIFFLEN = {"raw": 0, "fivepoint": 1, "cor": 2}
-NSH = {TRI: 3, QUAD: 4, TET: 4, HEX: 8}
-NIP = {TRI: 3, QUAD: 4, TET: 6, HEX: 12}
-DIM = {TRI: 2, QUAD: 2, TET: 3, HEX: 3}
+NSH = {TRI: 3, QUAD: 4, TET: 4, HEX: 8, PRISM: 6}
+NIP = {TRI: 3, QUAD: 4, TET: 6, HEX: 12, PRISM: 9}
+DIM = {TRI: 2, QUAD: 2, TET: 3, HEX: 3, PRISM: 3}
 
 
 def build(force: bool = False) -> str:

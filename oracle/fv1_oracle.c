@@ -54,7 +54,7 @@ typedef struct {
     int side[6][4];
 } refelem;
 
-static const refelem REF[4] = {
+static const refelem REF[5] = {
     /* triangle */
     {2, 3, 3, 3,
      {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}},
@@ -79,11 +79,17 @@ static const refelem REF[4] = {
      {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 4}, {1, 5}, {2, 6}, {3, 7}, {4, 5}, {5, 6}, {6, 7}, {7, 4}},
      {4, 4, 4, 4, 4, 4},
      {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {0, 4, 7, 3}, {4, 5, 6, 7}}},
+    /* prism: triangle 0-1-2 at zeta = 0, 3-4-5 above it (UG4 numbering) */
+    {3, 6, 9, 5,
+     {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}},
+     {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 4}, {2, 5}, {3, 4}, {4, 5}, {5, 3}},
+     {3, 4, 4, 4, 3},
+     {{0, 2, 1}, {0, 1, 4, 3}, {1, 2, 5, 4}, {2, 0, 3, 5}, {3, 4, 5}}},
 };
 
-int or_nsh(int et) { return (et >= 0 && et < 4) ? REF[et].nsh : -1; }
-int or_nip(int et) { return (et >= 0 && et < 4) ? REF[et].nip : -1; }
-int or_dim(int et) { return (et >= 0 && et < 4) ? REF[et].dim : -1; }
+int or_nsh(int et) { return (et >= 0 && et < 5) ? REF[et].nsh : -1; }
+int or_nip(int et) { return (et >= 0 && et < 5) ? REF[et].nip : -1; }
+int or_dim(int et) { return (et >= 0 && et < 5) ? REF[et].dim : -1; }
 
 static int side_has(const refelem* R, int s, int c)
 {
@@ -128,6 +134,19 @@ static void shapes(int et, const double* x, double* phi, double (*gl)[3])
         for (int k = 0; k < 4; ++k) for (int j = 0; j < 3; ++j) gl[k][j] = 0;
         gl[0][0] = gl[0][1] = gl[0][2] = -1;
         gl[1][0] = 1; gl[2][1] = 1; gl[3][2] = 1;
+        break;
+    case OR_PRISM:
+        /* P1 triangle (barycentric lambda) times P1 segment in zeta */
+        for (int k = 0; k < 6; ++k) {
+            const int t = k % 3;
+            const double lam = t == 0 ? 1.0 - x[0] - x[1] : (t == 1 ? x[0] : x[1]);
+            const double dl0 = t == 0 ? -1.0 : (t == 1 ? 1.0 : 0.0), dl1 = t == 0 ? -1.0 : (t == 2 ? 1.0 : 0.0);
+            const double w = k < 3 ? 1.0 - x[2] : x[2], dw = k < 3 ? -1.0 : 1.0;
+            phi[k] = lam * w;
+            gl[k][0] = dl0 * w;
+            gl[k][1] = dl1 * w;
+            gl[k][2] = lam * dw;
+        }
         break;
     case OR_HEX:
         for (int k = 0; k < 8; ++k) {
@@ -193,7 +212,7 @@ static double trilinear_hex_volume(const double p[8][3])
 /* ------------------------------------------------------------------ */
 int or_geometry(int et, const double* corners, or_geo* G)
 {
-    if (et < 0 || et > 3) { set_err("unknown element type"); return -1; }
+    if (et < 0 || et > 4) { set_err("unknown element type"); return -1; }
     const refelem* R = &REF[et];
     const int dim = R->dim, nsh = R->nsh, nip = R->nip;
     memset(G, 0, sizeof *G);
@@ -1504,7 +1523,7 @@ static int assemble_impl(const or_params* p, int et, const double* xyz, int64_t 
                          const int64_t* rowptr, const int32_t* colidx,
                          double* J, double* d, int absmode)
 {
-    if (et < 0 || et > 3) { set_err("unknown element type"); return -1; }
+    if (et < 0 || et > 4) { set_err("unknown element type"); return -1; }
     const int dim = REF[et].dim, nc = dim + 1;
     if (p->dim != dim) { set_err("Wrong number of functions"); return -5; }
     const int64_t ndof = nv * nc;
@@ -1531,7 +1550,7 @@ int or_assemble_colored(const or_params* p, int et, const double* xyz, int64_t n
                         const int64_t* rowptr, const int32_t* colidx,
                         double* J, double* d, int nthreads)
 {
-    if (et < 0 || et > 3) { set_err("unknown element type"); return -1; }
+    if (et < 0 || et > 4) { set_err("unknown element type"); return -1; }
     const int dim = REF[et].dim, nc = dim + 1;
     if (p->dim != dim) { set_err("Wrong number of functions"); return -5; }
     const int64_t ndof = nv * nc;

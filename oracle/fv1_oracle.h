@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-enum { OR_TRI = 0, OR_QUAD = 1, OR_TET = 2, OR_HEX = 3 };
+enum { OR_TRI = 0, OR_QUAD = 1, OR_TET = 2, OR_HEX = 3, OR_PRISM = 4 };
 enum { OR_UPW_NONE_SET = -1, OR_UPW_NO = 0, OR_UPW_FULL = 1, OR_UPW_SKEWED = 2, OR_UPW_LPS = 3, OR_UPW_POSITIVE = 4,
        OR_UPW_REGULAR = 5 };
 enum { OR_STAB_NONE_SET = -1, OR_STAB_FIELDS = 0, OR_STAB_WITHOUT = 1, OR_STAB_FLOW = 2 };

@@ -208,7 +208,7 @@ class NavierStokesFV1:
     def ip_positions(self, kind: str = "scvf") -> np.ndarray:
         """Global integration points: "scvf" [ne, nip, dim] or "scv" [ne, nsh, dim] (the corners)."""
         k = {"scvf": 0, "scv": 1}[kind]
-        per = {0: (3, 3), 1: (4, 4), 2: (6, 4), 3: (12, 8)}[self._et][k]
+        per = {0: (3, 3), 1: (4, 4), 2: (6, 4), 3: (12, 8), 4: (9, 6)}[self._et][k]
         out = np.zeros((self.ne, per, self.dim))
         _check(_lib.nsfv1_ip_positions(self._h, k, out.ctypes.data_as(C.POINTER(C.c_double)), out.size))
         return out

@@ -186,8 +186,10 @@ struct nsfv1_handle {
 };
 
 // ---- imports at the integration points (helpers of the entry points below) ----
-static int nsh_count(int et) { return et == 0 ? 3 : (et == 1 ? 4 : (et == 2 ? 4 : 8)); }
-static int nip_of(int et) { return et == 0 ? 3 : (et == 1 ? 4 : (et == 2 ? 6 : 12)); }
+// corners and SCVFs (= edges) per element type: triangle, quadrilateral,
+// tetrahedron, hexahedron, prism
+static int nsh_count(int et) { static const int n[5] = {3, 4, 4, 8, 6}; return n[et]; }
+static int nip_of(int et) { static const int n[5] = {3, 4, 6, 12, 9}; return n[et]; }
 
 static int upload(DevBuf& b, const double* x, int64_t n)
 {
@@ -384,7 +386,8 @@ int nsfv1_ip_positions(nsfv1_handle* h, int kind, double* out, int64_t n)
     case 0: ip_positions_et<0>(xyz, conn, h->ne, kind, out); break;
     case 1: ip_positions_et<1>(xyz, conn, h->ne, kind, out); break;
     case 2: ip_positions_et<2>(xyz, conn, h->ne, kind, out); break;
-    default: ip_positions_et<3>(xyz, conn, h->ne, kind, out); break;
+    case 3: ip_positions_et<3>(xyz, conn, h->ne, kind, out); break;
+    default: ip_positions_et<4>(xyz, conn, h->ne, kind, out); break;
     }
     return NSFV1_OK;
 }
@@ -515,7 +518,7 @@ int nsfv1_set_pac_upwind(nsfv1_handle* h, int on)
     return NSFV1_OK;
 }
 
-static int nsh_of(int et) { return et == ET_TRI ? 3 : (et == ET_QUAD || et == ET_TET) ? 4 : 8; }
+static int nsh_of(int et) { return nsh_count(et); }
 static int dim_of(int et) { return et <= ET_QUAD ? 2 : 3; }
 
 // A hexahedral mesh that is a lexicographic nx x ny x nz grid (node (i,j,k)
@@ -574,7 +577,10 @@ static bool detect_structured_hex(const double* xyz, int64_t nv, const int32_t* 
 int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const int32_t* conn, int64_t ne)
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
-    if (et < 0 || et > 3) return fail(NSFV1_ERR_ARGUMENT, "unknown element type");
+    if (et == NSFV1_PYRAMID)
+        return fail(NSFV1_ERR_NOT_IMPLEMENTED, "pyramids are not provided: the apex corner has four edges, so its "
+                                               "FV1 sub-control volume is not the hexahedron the other elements use");
+    if (et < 0 || et > 4) return fail(NSFV1_ERR_ARGUMENT, "unknown element type");
     if (dim_of(et) != h->dim) return fail(NSFV1_ERR_ARGUMENT, "element type does not match the domain dimension");
     if (nv <= 0 || ne <= 0 || !xyz || !conn) return fail(NSFV1_ERR_ARGUMENT, "empty mesh");
     if (nv >= (int64_t(1) << 31) / (h->dim + 1)) return fail(NSFV1_ERR_ARGUMENT, "too many nodes for int32 column indices");
@@ -677,7 +683,7 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
               up(h->inc_elem, ie.data(), sizeof(int32_t) * ninc) && up(h->inc_node, in.data(), sizeof(int32_t) * ninc) &&
               up(h->inc_info, ii.data(), sizeof(int32_t) * ninc) && up(h->inc_pos, ipos.data(), sizeof(uint64_t) * ninc) &&
               up(h->node_inc, ni32.data(), sizeof(int32_t) * ni32.size()) &&
-              h->l2inv.alloc(sizeof(double) * ne * (et == ET_HEX ? 12 : et == ET_TET ? 6 : et == ET_QUAD ? 4 : 3)) == hipSuccess &&
+              h->l2inv.alloc(sizeof(double) * ne * nip_of(et)) == hipSuccess &&
               h->vol.alloc(sizeof(double) * ne * nsh) == hipSuccess && h->affine.alloc((size_t)ne) == hipSuccess;
     if (!ok) return fail(NSFV1_ERR_DEVICE, std::string("mesh upload: ") + hipGetErrorString(hipGetLastError()));
     h->et = et;
@@ -933,6 +939,9 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
                 if (h->bnd[kind].nbn)
                     return fail(NSFV1_ERR_NOT_IMPLEMENTED,
                                 "boundary discs with per-instance viscosity (batched instances) are not provided");
+        if (h->et == ET_PRISM && coupled_config(h))
+            return fail(NSFV1_ERR_NOT_IMPLEMENTED, "prisms are not provided with the FLOW stabilization or the "
+                                                   "positive/regular upwinds");
         if (h->ip_imports()) {
             for (int kind = 0; kind < BND_KINDS; ++kind)
                 if (h->bnd[kind].nbn)
@@ -984,7 +993,7 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
         if (scratch)
             return fail(NSFV1_ERR_ARGUMENT, "no upwind record buffer yet: the no-allocation entry needs one "
                                             "nsfv1_assemble call in NSFV1_UPWIND_RECORD mode first (it allocates it)");
-        const int nip = h->et == 0 ? 3 : (h->et == 1 ? 4 : (h->et == 2 ? 6 : 12));
+        const int nip = nip_of(h->et);
         if (h->upw_dec.alloc((size_t)h->ne * nip) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "upwind record buffer");
     }
     KMeshPtrs m{h->xyz.as<double>(), h->conn.as<int32_t>(), h->rowptr.as<int64_t>(), h->colidx.as<int32_t>(),
@@ -1142,8 +1151,9 @@ static int set_boundary_list(nsfv1_handle* h, int kind, const int32_t* elem, con
     auto& B = h->bnd[kind];
     B.nbn = 0;
     if (n == 0) return NSFV1_OK;
+    if (h->et == ET_PRISM) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "boundary discs on prisms are not provided");
     if (n < 0 || !elem || !side) return fail(NSFV1_ERR_ARGUMENT, "bad boundary side arrays");
-    const int nsh = h->et == ET_HEX ? 8 : (h->et == ET_TRI ? 3 : 4);
+    const int nsh = nsh_of(h->et);
     const int nside = h->et == ET_HEX ? 6 : (h->et == ET_QUAD ? 4 : (h->et == ET_TET ? 4 : 3));
     std::vector<int32_t> cn;  // conn rows are needed: download once
     cn.resize((size_t)h->ne * nsh);
@@ -1227,8 +1237,8 @@ int nsfv1_export_size(nsfv1_handle* h, int what, int at, int nip, int64_t* count
     if (at == NSFV1_AT_LOCAL && (nip < 1 || nip > EX_MAX_LOCAL))
         return fail(NSFV1_ERR_ARGUMENT, "local points per call must be 1..32");
     const int dim = (h->et == ET_TRI || h->et == ET_QUAD) ? 2 : 3;
-    const int nsh = h->et == ET_HEX ? 8 : (h->et == ET_TRI ? 3 : 4);
-    const int nscvf = h->et == ET_TRI ? 3 : (h->et == ET_QUAD ? 4 : (h->et == ET_TET ? 6 : 12));
+    const int nsh = nsh_of(h->et);
+    const int nscvf = nip_of(h->et);
     const int npt = at == NSFV1_AT_SCVF ? nscvf : (at == NSFV1_AT_SCV ? nsh : nip);
     const int ncomp = what == NSFV1_EX_VELOCITY_GRAD ? dim * dim : (what == NSFV1_EX_PRESSURE ? 1 : dim);
     *count = h->ne * npt * ncomp;
@@ -1241,6 +1251,7 @@ int nsfv1_export(nsfv1_handle* h, int what, int at, const double* local_ips, int
     int64_t count = 0;
     int rc = nsfv1_export_size(h, what, at, nip, &count);
     if (rc) return rc;
+    if (h->et == ET_PRISM) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "data exports on prisms are not provided");
     if (!u || !out) return fail(NSFV1_ERR_ARGUMENT, "NULL buffer");
     if (at == NSFV1_AT_LOCAL && !local_ips) return fail(NSFV1_ERR_ARGUMENT, "local points are NULL");
     const int dim = (h->et == ET_TRI || h->et == ET_QUAD) ? 2 : 3;

@@ -71,12 +71,14 @@ geometry_cache(const double* __restrict__ xyz, const int32_t* __restrict__ conn,
         for (int i = 0; i < D; ++i) co[k][i] = xyz[v * D + i];
     }
     double vol[N];
-    scv_volumes<ET>(co, vol);
+    if constexpr (ET == ET_PRISM) scv_volumes_prism(co, vol);
+    else scv_volumes<ET>(co, vol);
 #pragma unroll
     for (int k = 0; k < N; ++k) volc[e * N + k] = vol[k];
     {
-        // parallelogram / parallelepiped test (simplices are always affine)
-        bool aff = true;
+        // parallelogram / parallelepiped test (simplices are always affine;
+        // prisms take the general ray test)
+        bool aff = ET != ET_PRISM;
         if constexpr (ET == 1 || ET == 3) {
             double scale = 0.0, dev = 0.0;
 #pragma unroll
@@ -355,7 +357,11 @@ __device__ inline bool upwind_canon(int type, bool kvfrom, const double (&co)[To
                 const double a = li[0], b = li[1];
                 v = c == 0 ? (1 - a) * (1 - b) : (c == 1 ? a * (1 - b) : (c == 2 ? a * b : (1 - a) * b));
             } else if constexpr (ET == 2) v = c == 0 ? 1.0 - li[0] - li[1] - li[2] : li[c - 1];
-            else {
+            else if constexpr (ET == ET_PRISM) {
+                const int t = c % 3;
+                const double lam = t == 0 ? 1.0 - li[0] - li[1] : (t == 1 ? li[0] : li[1]);
+                v = lam * (c < 3 ? 1.0 - li[2] : li[2]);
+            } else {
                 const double f0 = T::lc[c][0] > 0.5 ? li[0] : 1.0 - li[0];
                 const double f1 = T::lc[c][1] > 0.5 ? li[1] : 1.0 - li[1];
                 const double f2 = T::lc[c][2] > 0.5 ? li[2] : 1.0 - li[2];
@@ -1572,6 +1578,7 @@ hipError_t launch_assemble(int et, int jprec, const KMeshPtrs& m, const KParams&
     case ET_QUAD: return launch_et<ET_QUAD>(jprec, m, p, u, uold, J, d, err, nwg, lds, s);
     case ET_TET:  return launch_et<ET_TET>(jprec, m, p, u, uold, J, d, err, nwg, lds, s);
     case ET_HEX:  return launch_et<ET_HEX>(jprec, m, p, u, uold, J, d, err, nwg, lds, s);
+    case ET_PRISM: return launch_et<ET_PRISM>(jprec, m, p, u, uold, J, d, err, nwg, lds, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1619,7 +1626,9 @@ static hipError_t launch_split_et(const KMeshPtrs& m, const KParams& p, const do
 // frame rounds some exactly-zero normal/gradient products of axis-aligned
 // tetrahedra differently from a row's own frame (O(1e-19) where the
 // reference arithmetic gives an exact 0).
-bool split_supported(int et, const KParams& p) { (void)et; return fast_path(p) >= 0; }
+// prisms keep the fused path: the record layout and the canonical record
+// frames cover the four element types of the reference's 2D/3D workloads
+bool split_supported(int et, const KParams& p) { return et != ET_PRISM && fast_path(p) >= 0; }
 
 int64_t split_record_floats(int et, int64_t ne)
 {
@@ -1651,6 +1660,7 @@ hipError_t launch_geometry_cache(int et, const double* xyz, const int32_t* conn,
     case ET_QUAD: geometry_cache<ET_QUAD><<<blocks, 256, 0, s>>>(xyz, conn, ne, difflen, l2inv, vol, affine, err); break;
     case ET_TET:  geometry_cache<ET_TET><<<blocks, 256, 0, s>>>(xyz, conn, ne, difflen, l2inv, vol, affine, err); break;
     case ET_HEX:  geometry_cache<ET_HEX><<<blocks, 256, 0, s>>>(xyz, conn, ne, difflen, l2inv, vol, affine, err); break;
+    case ET_PRISM: geometry_cache<ET_PRISM><<<blocks, 256, 0, s>>>(xyz, conn, ne, difflen, l2inv, vol, affine, err); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

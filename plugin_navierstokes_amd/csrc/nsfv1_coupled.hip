@@ -966,8 +966,8 @@ coupled_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const JT* __r
 
 int64_t coupled_record_bytes(int et, int jprec, int64_t ne)
 {
-    const int nsh = et == ET_HEX ? 8 : (et == ET_TRI ? 3 : 4);
-    const int nip = et == ET_TRI ? 3 : (et == ET_QUAD ? 4 : (et == ET_TET ? 6 : 12));
+    const int nsh = et == ET_HEX ? 8 : (et == ET_PRISM ? 6 : (et == ET_TRI ? 3 : 4));
+    const int nip = et == ET_TRI ? 3 : (et == ET_QUAD ? 4 : (et == ET_TET ? 6 : (et == ET_PRISM ? 9 : 12)));
     const int nc = (et == ET_TRI || et == ET_QUAD) ? 3 : 4;
     const int64_t jb = ne * nip * nsh * nc * nc * (jprec == 64 ? 8 : 4);
     return ((jb + 255) / 256) * 256 + ne * nip * nc * 8;

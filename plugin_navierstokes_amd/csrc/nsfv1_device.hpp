@@ -7,7 +7,7 @@
 
 namespace nsfv1 {
 
-enum { ET_TRI = 0, ET_QUAD = 1, ET_TET = 2, ET_HEX = 3 };
+enum { ET_TRI = 0, ET_QUAD = 1, ET_TET = 2, ET_HEX = 3, ET_PRISM = 4 };
 enum { UPW_NO = 0, UPW_FULL = 1, UPW_SKEWED = 2, UPW_LPS = 3, UPW_POSITIVE = 4, UPW_REGULAR = 5 };
 enum { STAB_FIELDS = 0, STAB_WITHOUT = 1, STAB_FLOW = 2 };
 enum { DL_RAW = 0, DL_FIVEPOINT = 1, DL_COR = 2 };

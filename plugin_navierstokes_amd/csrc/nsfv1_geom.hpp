@@ -1,8 +1,8 @@
 // nsfv1_geom.hpp -- FV1 geometry shared by the assembly kernels: compile-time
 // shape tables at the SCVF integration points, the SCVF frame (ip, normal,
-// edge midpoint; restated ugcore FV1Geometry), the inverse element
-// Jacobian at an ip, and the reference-coordinate ray cast of affine
-// elements used by the skewed upwinds.
+// edge midpoint; restated ugcore FV1Geometry), the prism's SCV volumes,
+// the inverse element Jacobian at an ip, and the reference-coordinate ray
+// cast of affine elements used by the skewed upwinds.
 #pragma once
 
 #include "nsfv1_device.hpp"
@@ -92,6 +92,88 @@ __device__ inline void scvf_frame(const double (&co)[Topo<ET>::nsh][Topo<ET>::di
     const double s = dot_nc<D>(nn, e);
 #pragma unroll
     for (int i = 0; i < D; ++i) n[i] = s < 0 ? -nn[i] : nn[i];
+}
+
+// SCV volumes of a prism.  Every corner has three edges, so its SCV is the
+// hexahedron [corner, midpoints of edges e0 < e1 < e2, the face centres of
+// the edge pairs, the element centre]; its volume is that of the trilinear
+// hexahedron through the eight points (2x2x2 Gauss on det J, exact for a
+// trilinear map), as FV1Geometry's hexahedral SCVs.
+__device__ inline double trilinear_hex_volume(const double (&p)[8][3])
+{
+    const double g[2] = {0.5 - 0.5 / sqrt(3.0), 0.5 + 0.5 / sqrt(3.0)};
+    double v = 0.0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const double x[3] = {g[a], g[b], g[c]};
+                double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    double f[3], df[3];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        const bool hi = Topo<3>::lc[k][j] > 0.5;
+                        f[j] = hi ? x[j] : 1.0 - x[j];
+                        df[j] = hi ? 1.0 : -1.0;
+                    }
+                    const double gl[3] = {df[0] * f[1] * f[2], f[0] * df[1] * f[2], f[0] * f[1] * df[2]};
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) J[i][j] += p[k][i] * gl[j];
+                }
+                const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                                   J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                                   J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+                v += 0.125 * det;
+            }
+    return fabs(v);
+}
+
+__device__ inline void scv_volumes_prism(const double (&co)[6][3], double* vol)
+{
+    using T = Topo<ET_PRISM>;
+    double c[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) c[i] += co[k][i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) c[i] /= 6;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        double p[8][3];
+        const int e[3] = {corner_ip<ET_PRISM>(k, 0), corner_ip<ET_PRISM>(k, 1), corner_ip<ET_PRISM>(k, 2)};
+        const int f[3] = {pair_face<ET_PRISM>(k, 0), pair_face<ET_PRISM>(k, 1), pair_face<ET_PRISM>(k, 2)};
+        double me[3][3], fc[3][3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                me[q][i] = 0.5 * (co[T::edge[e[q]][0]][i] + co[T::edge[e[q]][1]][i]);
+                double s = 0.0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (j < T::side_n[f[q]]) s += co[T::side[f[q]][j]][i];
+                fc[q][i] = s / T::side_n[f[q]];
+            }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            p[0][i] = co[k][i];
+            p[1][i] = me[0][i];
+            p[2][i] = fc[0][i];
+            p[3][i] = me[1][i];
+            p[4][i] = me[2][i];
+            p[5][i] = fc[1][i];
+            p[6][i] = c[i];
+            p[7][i] = fc[2][i];
+        }
+        vol[k] = trilinear_hex_volume(p);
+    }
 }
 
 // J^-1 of the element map at SCVF IP (J_ij = dx_i/dxi_j); false if singular

@@ -1,7 +1,7 @@
 // nsfv1_topo.hpp -- compile-time reference-element topology and FV1 tables.
 //
 // UG4 corner/edge/side numbering of the triangle, quadrilateral,
-// tetrahedron and hexahedron; the FV1 sub-control-volume-face (SCVF) of
+// tetrahedron, hexahedron and prism; the FV1 sub-control-volume-face (SCVF) of
 // edge i has from/to = the edge's corners and, in 3D, spans the edge
 // midpoint, the centres of the two faces containing the edge and the
 // element centre.  Local integration points, shape values and reference
@@ -44,6 +44,14 @@ template <> struct Topo<3> {  // hexahedron
                                        {2, 3, 7, 6}, {0, 4, 7, 3}, {4, 5, 6, 7}};
     static constexpr double lc[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
                                         {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+};
+
+template <> struct Topo<4> {  // prism: triangle 0-1-2 at zeta = 0, 3-4-5 above it
+    static constexpr int dim = 3, nsh = 6, nip = 9, nside = 5, cip = 3;
+    static constexpr int edge[9][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 4}, {2, 5}, {3, 4}, {4, 5}, {5, 3}};
+    static constexpr int side_n[5] = {3, 4, 4, 4, 3};
+    static constexpr int side[5][4] = {{0, 2, 1, 0}, {0, 1, 4, 3}, {1, 2, 5, 4}, {2, 0, 3, 5}, {3, 4, 5, 0}};
+    static constexpr double lc[6][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}};
 };
 
 template <int ET>
@@ -119,6 +127,15 @@ constexpr double shape_at(int k, int j, double x0, double x1, double x2)
         const double g[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
         return j < 0 ? v[k] : g[k][j];
     }
+    if (ET == 4) {
+        // P1 triangle (barycentric lambda) times P1 segment in zeta
+        const double lam[3] = {1.0 - x0 - x1, x0, x1};
+        const double dl[3][2] = {{-1, -1}, {1, 0}, {0, 1}};
+        const int t = k % 3;
+        const double w = k < 3 ? 1.0 - x2 : x2, dw = k < 3 ? -1.0 : 1.0;
+        if (j < 0) return lam[t] * w;
+        return j < 2 ? dl[t][j] * w : lam[t] * dw;
+    }
     const double* c = Topo<3>::lc[k];
     const double x[3] = {x0, x1, x2};
     double f[3] = {0, 0, 0}, df[3] = {0, 0, 0};
@@ -163,7 +180,8 @@ constexpr int pair_face(int k, int p)  // p: 0 -> (e0,e1), 1 -> (e0,e2), 2 -> (e
 // corner permutation) that maps reference corner 0 to corner kv.
 // canonical corner k' is original corner perm(kv, k').  Hexahedra and
 // quadrilaterals use the reflections x -> x xor c(kv); tetrahedra (any
-// permutation of the corners is affine) and triangles list kv first.
+// permutation of the corners is affine) and triangles list kv first;
+// prisms rotate their triangles and swap levels.
 // ---------------------------------------------------------------------
 template <int ET>
 constexpr int corner_of_coords(int x, int y, int z)
@@ -184,6 +202,9 @@ constexpr int canon_perm(int kv, int kp)
         return corner_of_coords<ET>(x, y, z);
     }
     if (ET == 0) return (kv + kp) % 3;
+    // prism: rotate the triangles (any corner permutation of a triangle is
+    // affine) and swap the two levels
+    if (ET == 4) return (kp % 3 + kv % 3) % 3 + 3 * ((kp / 3) ^ (kv / 3));
     // tetrahedron: kv first, the others ascending
     if (kp == 0) return kv;
     int m = 0;

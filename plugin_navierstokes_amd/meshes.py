@@ -3,7 +3,8 @@ workloads (input generation; the assembly itself lives in the HIP library).
 
 Element corner order follows the UG4 reference elements: triangles and quads
 counter-clockwise from the lower-left corner, tetrahedra (0,0,0),(1,0,0),
-(0,1,0),(0,0,1), hexahedra bottom face counter-clockwise then the top face.
+(0,1,0),(0,0,1), hexahedra bottom face counter-clockwise then the top face,
+prisms the bottom triangle counter-clockwise then the triangle above it.
 Node numbering is lexicographic (x fastest); DoF = node*(dim+1)+component.
 """
 from __future__ import annotations
@@ -12,16 +13,17 @@ from dataclasses import dataclass
 
 import numpy as np
 
-TRI, QUAD, TET, HEX = 0, 1, 2, 3
-ELEM_NAMES = {TRI: "triangle", QUAD: "quadrilateral", TET: "tetrahedron", HEX: "hexahedron"}
-DIM = {TRI: 2, QUAD: 2, TET: 3, HEX: 3}
-NSH = {TRI: 3, QUAD: 4, TET: 4, HEX: 8}
+TRI, QUAD, TET, HEX, PRISM = 0, 1, 2, 3, 4
+ELEM_NAMES = {TRI: "triangle", QUAD: "quadrilateral", TET: "tetrahedron", HEX: "hexahedron", PRISM: "prism"}
+DIM = {TRI: 2, QUAD: 2, TET: 3, HEX: 3, PRISM: 3}
+NSH = {TRI: 3, QUAD: 4, TET: 4, HEX: 8, PRISM: 6}
 # element sides in UG4 reference-element numbering (corners of each side)
 SIDES = {
     TRI: [(0, 1), (1, 2), (2, 0)],
     QUAD: [(0, 1), (1, 2), (2, 3), (3, 0)],
     TET: [(0, 2, 1), (1, 2, 3), (0, 3, 2), (0, 1, 3)],
     HEX: [(0, 3, 2, 1), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (0, 4, 7, 3), (4, 5, 6, 7)],
+    PRISM: [(0, 2, 1), (0, 1, 4, 3), (1, 2, 5, 4), (2, 0, 3, 5), (3, 4, 5)],
 }
 
 
@@ -114,10 +116,15 @@ _KUHN = [
 ]
 
 
+# a hexahedron split along its bottom/top diagonal 0-2 / 4-6 into two
+# prisms (triangle at z = 0 counter-clockwise from +z, then the one above)
+_PRISMS = [(0, 1, 2, 4, 5, 6), (0, 2, 3, 4, 6, 7)]
+
+
 def box_3d(cells, lengths, et: int = HEX, jitter: float = 0.0, seed: int = 0, kind: str = "cavity",
            keep_cell=None, solid=None) -> Mesh:
-    """Box [0,lx]x[0,ly]x[0,lz] of nx*ny*nz hexahedra (or 6 Kuhn tetrahedra
-    each).  keep_cell(ci, cj, ck) or solid(x, y, z) (evaluated at cell
+    """Box [0,lx]x[0,ly]x[0,lz] of nx*ny*nz hexahedra (or 6 Kuhn tetrahedra,
+    or 2 prisms, each).  keep_cell(ci, cj, ck) or solid(x, y, z) (evaluated at cell
     centres; True = inside an obstacle) drop cells; their nodes stay in the
     numbering and are listed in boundary["unused"]."""
     nx, ny, nz = cells
@@ -141,8 +148,10 @@ def box_3d(cells, lengths, et: int = HEX, jitter: float = 0.0, seed: int = 0, ki
         conn = hexc
     elif et == TET:
         conn = np.stack([hexc[:, list(t)] for t in _KUHN], axis=1).reshape(-1, 4)
+    elif et == PRISM:
+        conn = np.stack([hexc[:, list(t)] for t in _PRISMS], axis=1).reshape(-1, 6)
     else:
-        raise ValueError("3D box needs TET or HEX")
+        raise ValueError("3D box needs TET, HEX or PRISM")
     interior = (ii > 0) & (ii < nx) & (jj > 0) & (jj < ny) & (kk > 0) & (kk < nz)
     xyz = _jitter(xyz, interior, min(hx, hy, hz), jitter, seed)
     if kind == "cavity":
