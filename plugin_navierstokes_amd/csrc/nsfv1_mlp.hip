@@ -63,7 +63,12 @@ mlp_forward(const float* __restrict__ P, const uint16_t* __restrict__ WB, const 
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, kq = (lane >> 4) * 4;
-    const int64_t node = ((int64_t)blockIdx.x * 4 + wave) * 16 + col;
+    // the weights are staged once per workgroup (the compiler keeps each
+    // lane's A operands in registers across the loop); the workgroup then
+    // walks 64-node blocks (grid-stride), so the staging is amortised
+    const int64_t nblk = (n + 63) / 64;
+    for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const int64_t node = (blk * 4 + wave) * 16 + col;
     const bool valid = node < n;
 
     // layer 1: B = X^T tiles (k = input feature, column = node), two k-blocks
@@ -124,6 +129,7 @@ mlp_forward(const float* __restrict__ P, const uint16_t* __restrict__ WB, const 
         *reinterpret_cast<float4*>(Y + node * MLP_OUT) =
             make_float4(acc[0] + bias[2 * MLP_H + 0], acc[1] + bias[2 * MLP_H + 1], acc[2] + bias[2 * MLP_H + 2],
                         acc[3] + bias[2 * MLP_H + 3]);
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -509,7 +515,10 @@ hipError_t launch_mlp_forward(const float* P, const uint16_t* WB, const float* X
                               float* Z2, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    mlp_forward<<<nblocks(n, 64), 256, 0, s>>>(P, WB, X, n, Y, Z1, Z2);
+    // one workgroup per CU (256 CUs): each holds the weights in registers
+    // (one wave per SIMD) and walks its share of the 64-node blocks
+    const unsigned g = nblocks(n, 64) < 256u ? nblocks(n, 64) : 256u;
+    mlp_forward<<<g, 256, 0, s>>>(P, WB, X, n, Y, Z1, Z2);
     return hipGetLastError();
 }
 
