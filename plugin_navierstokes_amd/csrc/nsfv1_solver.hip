@@ -485,10 +485,13 @@ dense_matvec(const double* __restrict__ A, const double* __restrict__ x, double*
 //   M'[p][k] = 1/a, M'[p][j] = -M[p][j]/a,
 //   M'[i][k] = M[i][k]/a, M'[i][j] = M[i][j] - M'[i][k] M[p][j]   (i != p),
 // a = M[p][k].  After n steps x_k = sum_j M[p_k][j] y_{p_j}, so
-// inv[k][p_j] = M[p_k][j].  One launch per step, ping-pong tableaux; each
-// step also writes |M'[i][k+1]| of the unpivoted rows (-1 for pivoted ones)
-// for the next step's pivot search, which every workgroup repeats with
-// the same fixed-order reduction (ties go to the smaller row).
+// inv[k][p_j] = M[p_k][j].  Ping-pong tableaux; every workgroup repeats the
+// pivot search with the same fixed-order reduction (ties go to the smaller
+// row).  A column that vanishes on every unpivoted row (a null mode of the
+// coarse operator, e.g. the constant pressure of a domain with Dirichlet
+// velocity everywhere) pins its unknown to zero in the coarse correction
+// (a = 0 is replaced by 1/a := 0), instead of dividing by zero; a column
+// with no finite entry left is reported.
 // ---------------------------------------------------------------------
 template <typename JT>
 __global__ void __launch_bounds__(256)
@@ -508,63 +511,109 @@ gj_load(const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colidx, 
     }
 }
 
+// GJ_BLOCK exchange steps per launch (same pivots, same arithmetic as one
+// step per launch, with every update an explicit fma).  Each workgroup
+// loads columns k0 .. k0+nb-1 of every row (the panel) and replays the
+// steps on it: the pivot search of step s reads the panel after steps
+// 0 .. s-1, the pivot row of step s is row p_s of the input evolved through
+// those steps, and the multiplier of row r at step s (its column-k value
+// after the step, f = M[r][k] / a) is kept for the row updates.  The panel
+// and the full rows take the same operations in the same order, so the
+// pivots are those the full rows give.  Then each workgroup applies the nb
+// steps to its rows.  Pivoted flags ping-pong between launches (din ->
+// dout), so no workgroup reads a flag that another one writes.
+constexpr int GJ_BLOCK = 4;
+
 __global__ void __launch_bounds__(256)
-gj_step(const double* __restrict__ Min, double* __restrict__ Mout, const double* __restrict__ cin,
-        double* __restrict__ cout, int* __restrict__ done, int* __restrict__ piv, int* __restrict__ err, int n, int k)
+gj_block(const double* __restrict__ Min, double* __restrict__ Mout, const int* __restrict__ din,
+         int* __restrict__ dout, int* __restrict__ piv, int* __restrict__ err, int n, int k0, int nb)
 {
-    __shared__ double sv[4];
-    __shared__ int si[4];
-    const int i = blockIdx.x;
+    __shared__ double pan[MG_DENSE_MAX][GJ_BLOCK];   // panel columns of every row
+    __shared__ double prow[GJ_BLOCK][MG_DENSE_MAX];  // pivot row of step s, before step s
+    __shared__ double fm[GJ_BLOCK][MG_DENSE_MAX];    // multiplier of every row at step s
+    __shared__ int sdone[MG_DENSE_MAX];
+    __shared__ double sv[4], sia[GJ_BLOCK];
+    __shared__ int si[4], sp[GJ_BLOCK];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    // pivot search over the unpivoted rows
-    double bv = -2.0;
-    int bi = 0;
     for (int r = t; r < n; r += 256) {
-        const double v = cin[r];
-        if (v > bv) bv = v, bi = r;
-    }
+        sdone[r] = din[r];
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const double ov = __shfl_xor(bv, o, 64);
-        const int oi = __shfl_xor(bi, o, 64);
-        if (ov > bv || (ov == bv && oi < bi)) bv = ov, bi = oi;
+        for (int q = 0; q < GJ_BLOCK; ++q) pan[r][q] = q < nb ? Min[(int64_t)r * n + k0 + q] : 0.0;
     }
-    if (lane == 0) sv[w] = bv, si[w] = bi;
     __syncthreads();
-    bv = sv[0], bi = si[0];
+    for (int q = 0; q < nb; ++q) {
+        const int k = k0 + q;
+        // pivot search over the unpivoted rows (ties to the smaller row)
+        double bv = -2.0;
+        int bi = 0;
+        for (int r = t; r < n; r += 256) {
+            const double v = sdone[r] ? -1.0 : fabs(pan[r][q]);
+            if (v > bv) bv = v, bi = r;
+        }
 #pragma unroll
-    for (int q = 1; q < 4; ++q)
-        if (sv[q] > bv || (sv[q] == bv && si[q] < bi)) bv = sv[q], bi = si[q];
-    const int p = bi;
-    const bool was_done = done[i] != 0;
-    // column k vanishes on every unpivoted row (a null mode of the coarse
-    // operator, e.g. the constant pressure of a domain with Dirichlet
-    // velocity everywhere): unknown k is pinned to zero in the coarse
-    // correction and row p dropped, instead of dividing by zero.  A column
-    // with no finite entry left is reported.
-    const bool pin = bv == 0.0;
-    if (t == 0) {
-        if (i == 0 && !(bv >= 0.0)) *err = 1;
-        if (i == p) done[p] = 1, piv[k] = p;
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ov = __shfl_xor(bv, o, 64);
+            const int oi = __shfl_xor(bi, o, 64);
+            if (ov > bv || (ov == bv && oi < bi)) bv = ov, bi = oi;
+        }
+        if (lane == 0) sv[w] = bv, si[w] = bi;
+        __syncthreads();
+        bv = sv[0], bi = si[0];
+#pragma unroll
+        for (int x = 1; x < 4; ++x)
+            if (sv[x] > bv || (sv[x] == bv && si[x] < bi)) bv = sv[x], bi = si[x];
+        const int p = bi;
+        const bool pin = bv == 0.0;
+        if (t == 0 && blockIdx.x == 0 && !(bv >= 0.0)) *err = 1;
+        // the pivot row before this step: row p evolved through steps 0 .. q-1
+        for (int j = t; j < n; j += 256) {
+            double v = Min[(int64_t)p * n + j];
+            for (int x = 0; x < q; ++x) {
+                const double f = fm[x][p];
+                v = j == k0 + x ? f : fma(-f, prow[x][j], v);
+            }
+            prow[q][j] = v;
+        }
+        double pv[GJ_BLOCK];
+#pragma unroll
+        for (int x = 0; x < GJ_BLOCK; ++x) pv[x] = pan[p][x];
+        __syncthreads();
+        const double ia = pin ? 0.0 : 1.0 / prow[q][k];
+        if (t == 0) sp[q] = p, sia[q] = ia;
+        for (int r = t; r < n; r += 256) {
+            if (r == p) {
+                fm[q][r] = ia;
+#pragma unroll
+                for (int x = 0; x < GJ_BLOCK; ++x)
+                    if (x > q) pan[r][x] = -pv[x] * ia;
+            } else {
+                const double f = pan[r][q] * ia;
+                fm[q][r] = f;
+#pragma unroll
+                for (int x = 0; x < GJ_BLOCK; ++x)
+                    if (x > q) pan[r][x] = fma(-f, pv[x], pan[r][x]);
+            }
+        }
+        if (t == 0) sdone[p] = 1;
+        __syncthreads();
     }
-    const double* rp = Min + (int64_t)p * n;
-    const double* ri = Min + (int64_t)i * n;
-    double* ro = Mout + (int64_t)i * n;
-    const double ia = pin ? 0.0 : 1.0 / rp[k];
-    const bool pivoted = was_done || i == p;
-    if (i == p) {
+    if (blockIdx.x == 0 && t < nb) piv[k0 + t] = sp[t];
+    // the workgroup's rows through the nb steps
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
         for (int j = t; j < n; j += 256) {
-            const double v = j == k ? ia : -rp[j] * ia;
-            ro[j] = v;
-            if (j == k + 1) cout[i] = -1.0;
+            double v = Min[(int64_t)i * n + j];
+            for (int x = 0; x < nb; ++x) {
+                const int kx = k0 + x;
+                if (i == sp[x]) {
+                    v = j == kx ? sia[x] : -v * sia[x];
+                } else {
+                    const double f = fm[x][i];
+                    v = j == kx ? f : fma(-f, prow[x][j], v);
+                }
+            }
+            Mout[(int64_t)i * n + j] = v;
         }
-    } else {
-        const double f = ri[k] * ia;
-        for (int j = t; j < n; j += 256) {
-            const double v = j == k ? f : ri[j] - f * rp[j];
-            ro[j] = v;
-            if (j == k + 1) cout[i] = pivoted ? -1.0 : fabs(v);
-        }
+        if (t == 0) dout[i] = sdone[i];
     }
 }
 
@@ -795,9 +844,17 @@ hipError_t launch_dense_inverse(int jprec, const int64_t* rowptr, const int32_t*
     int* piv = done + n;
     if (jprec == 64) gj_load<double><<<n, 256, 0, s>>>(rowptr, colidx, (const double*)J, n, M[0], c[0], done, err);
     else gj_load<float><<<n, 256, 0, s>>>(rowptr, colidx, (const float*)J, n, M[0], c[0], done, err);
-    for (int k = 0; k < n; ++k)
-        gj_step<<<n, 256, 0, s>>>(M[k & 1], M[(k + 1) & 1], c[k & 1], c[(k + 1) & 1], done, piv, err, n, k);
-    gj_unpermute<<<blocks_for((int64_t)n * n, 256), 256, 0, s>>>(M[n & 1], piv, n, inv);
+    if (n > MG_DENSE_MAX) return hipErrorInvalidValue;
+    // GJ_BLOCK steps per launch; the pivoted flags alternate between `done`
+    // and the (unused here) second pivot-magnitude array
+    int* dflag[2] = {done, reinterpret_cast<int*>(c[1])};
+    const int nl = (n + GJ_BLOCK - 1) / GJ_BLOCK;
+    const unsigned g = n < 256 ? (unsigned)n : 256u;
+    for (int l = 0; l < nl; ++l) {
+        const int k0 = l * GJ_BLOCK, nb = n - k0 < GJ_BLOCK ? n - k0 : GJ_BLOCK;
+        gj_block<<<g, 256, 0, s>>>(M[l & 1], M[(l + 1) & 1], dflag[l & 1], dflag[(l + 1) & 1], piv, err, n, k0, nb);
+    }
+    gj_unpermute<<<blocks_for((int64_t)n * n, 256), 256, 0, s>>>(M[nl & 1], piv, n, inv);
     return hipGetLastError();
 }
 
