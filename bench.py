@@ -262,6 +262,53 @@ def channel_assembly_leg(steps: int, launch=None):
                          "kernel": CHANNEL_KERNELS.get(disc.assembly_path_used(), "one assembly")}}
 
 
+def coupled_leg(steps: int, n: int = 64):
+    """The coupled path (csrc/nsfv1_coupled.hip: SCVFs of an element solved
+    together) on an n^3 hexahedral cavity with the 128^3 leg's state
+    construction, Re = 400, FP32 J + FP64 defect, A part: the FLOW
+    stabilisation with the LPS upwind, and FIELDS with the positive upwind.
+    `steps` timed assemblies each (HIP events), after one untimed one."""
+    import torch
+
+    from plugin_navierstokes_amd import F32, NavierStokesFV1, PART_A
+    from plugin_navierstokes_amd import meshes as M
+
+    mesh = M.cavity_3d(n, M.HEX)
+    u = torch.from_numpy(M.cavity_lid_state(mesh, 1.0, base=M.divergence_free_perturbation(mesh, 1e-3, seed=0))).to(
+        "cuda:0")
+    out = {}
+    for name, stab, upw in (("flow_lps", "flow", "lps"), ("fields_positive", "fields", "positive")):
+        disc = NavierStokesFV1("u,v,w,p", "Inner")
+        disc.set_stabilization(stab)
+        disc.set_upwind(upw)
+        disc.set_kinematic_viscosity(1.0 / 400.0)
+        disc.set_mesh(mesh)
+        J = torch.empty(disc.nnz(), dtype=torch.float32, device="cuda:0")
+        d = torch.empty(disc.num_dofs(), dtype=torch.float64, device="cuda:0")
+        stream = torch.cuda.current_stream()
+        disc.assemble(u, parts=PART_A, jprec=F32, J=J, d=d, stream=stream)
+        disc.sync_errors(stream)
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        for a, b in ev:
+            a.record(stream)
+            disc.assemble(u, parts=PART_A, jprec=F32, J=J, d=d, stream=stream, check=False)
+            b.record(stream)
+        torch.cuda.synchronize()
+        disc.sync_errors(stream)
+        ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+        B = algorithmic_bytes(mesh.nv, mesh.ne, disc.nnz(), 3, 8, 4, 1)
+        ach = B / (ms * 1e-3) / 1e9
+        out[name] = {"ms_per_assembly": ms, "elem_per_s": mesh.ne / (ms * 1e-3), "path_used": disc.assembly_path_used(),
+                     "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": B, "kernel_ms": ms,
+                                  "kernel": "one assembly = nsfv1::coupled_scvf<3,float> + nsfv1::coupled_rows<3,float>"}}
+        del disc, J, d
+        torch.cuda.empty_cache()
+    out["workload"] = (f"{n}^3 hex cavity ({mesh.ne} elements), Re = 400, FP32 J + FP64 defect, A part; FLOW + LPS "
+                       "and FIELDS + positive upwind (coupled path)")
+    return out
+
+
 def batched_leg(steps: int = 20, count: int = 256):
     """256 cavity instances (side ~ U{32..192}, Re log-uniform [10, 2000],
     lid ~ U[0.5, 1.5], one seed) assembled in one launch: launch shape from
@@ -374,6 +421,8 @@ def main():
                     help="fixed assembly path / launch shape instead of the autotuner (profiling runs)")
     ap.add_argument("--channel-assembly", type=int, default=20,
                     help="timed assemblies of the held-out channel alone (0: skip)")
+    ap.add_argument("--coupled", type=int, default=5,
+                    help="timed assemblies of the coupled path (FLOW, positive upwind) on a 64^3 cavity (0: skip)")
     ap.add_argument("--channel-launch", type=int, nargs=3, metavar=("PATH", "THREADS", "LDS"),
                     help="fixed launch shape of the channel assembly leg (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -488,6 +537,11 @@ def main():
     if args.channel_assembly > 0:
         out["channel_assembly"] = channel_assembly_leg(args.channel_assembly, args.channel_launch)
         print(f"bench: channel assembly {out['channel_assembly']['ms_per_assembly']:.3f} ms", file=sys.stderr,
+              flush=True)
+    if args.coupled > 0:
+        out["coupled"] = coupled_leg(args.coupled)
+        print(f"bench: coupled path {out['coupled']['flow_lps']['ms_per_assembly']:.2f} ms (FLOW + LPS), "
+              f"{out['coupled']['fields_positive']['ms_per_assembly']:.2f} ms (FIELDS + positive)", file=sys.stderr,
               flush=True)
     if args.batched:
         out["batched"] = batched_leg()
