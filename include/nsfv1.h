@@ -175,6 +175,20 @@ int nsfv1_set_pac_upwind(nsfv1_handle* h, int on);
 int nsfv1_set_mesh(nsfv1_handle* h, int elem_type, const double* xyz, int64_t nv,
                    const int32_t* conn, int64_t ne);
 
+/* Mixed-element mesh: ngroups (1..8) groups of elements of one type each
+   (elem_type[g], conn[g] = [ne[g]][corners], UG4 corner order) over the
+   same nodes; elements are numbered group after group.  Replaces ugcore's
+   element loop over a subset that holds several element types, which calls
+   the disc once per type (prep_elem_loop, navier_stokes_fv1.cpp:136-182,
+   for each type registered at :1515-1551).  One CSR pattern covers all
+   groups; the assembly runs the generic kernel once per group on the
+   stream, the first group writing every row and the others adding theirs.
+   The FLOW stabilization, the positive and regular upwinds, boundary
+   discs, data exports, integration-point imports and batched instances
+   refuse mixed meshes with NSFV1_ERR_NOT_IMPLEMENTED. */
+int nsfv1_set_mesh_mixed(nsfv1_handle* h, const double* xyz, int64_t nv, int ngroups, const int* elem_type,
+                         const int32_t* const* conn, const int64_t* ne);
+
 int nsfv1_num_dofs(nsfv1_handle* h, int64_t* ndof);
 int nsfv1_nnz(nsfv1_handle* h, int64_t* nnz);
 

@@ -337,6 +337,56 @@ def assemble_abs(p: Params, et: int, xyz, conn, u, uold=None, dt=0.0, s_a=1.0, s
     return J, d
 
 
+def pattern_mixed(nv: int, groups):
+    """CSR pattern of a mesh of several element types (groups of (type,
+    connectivity) over nv nodes): node v's row holds every node that shares
+    an element with it, of any group, sorted, in (dim+1)^2 blocks (an
+    isolated node keeps its own block)."""
+    dim = DIM[groups[0][0]]
+    nc = dim + 1
+    pairs = [np.stack([np.arange(nv), np.arange(nv)], 1)]
+    for et, conn in groups:
+        conn = np.asarray(conn, dtype=np.int64)
+        n = conn.shape[1]
+        a = np.repeat(conn, n, axis=1).ravel()
+        b = np.tile(conn, (1, n)).ravel()
+        pairs.append(np.stack([a, b], 1))
+    pr = np.unique(np.concatenate(pairs), axis=0)  # sorted by row node, then column node
+    deg = np.bincount(pr[:, 0], minlength=nv)
+    rowptr = np.zeros(nv * nc + 1, dtype=np.int64)
+    rowptr[1:] = np.cumsum(np.repeat(deg * nc, nc))
+    colidx = np.empty(int(rowptr[-1]), dtype=np.int32)
+    start = np.concatenate([[0], np.cumsum(deg)])
+    for v in range(nv):
+        nb = pr[start[v]:start[v + 1], 1]
+        cols = (nb[:, None] * nc + np.arange(nc)[None, :]).ravel()
+        for a in range(nc):
+            colidx[rowptr[v * nc + a]:rowptr[v * nc + a + 1]] = cols
+    return rowptr, colidx
+
+
+def assemble_mixed(p: Params, groups, xyz, u, rowptr, colidx, **kw):
+    """A mesh of several element types: the sum of assemble() over its
+    groups in one pattern (the element loop of each type adds into the same
+    matrix and defect)."""
+    J = d = None
+    for et, conn in groups:
+        Jg, dg = assemble(p, et, xyz, conn, u, rowptr=rowptr, colidx=colidx, **kw)
+        J = Jg if J is None else J + Jg
+        d = dg if d is None else d + dg
+    return J, d
+
+
+def assemble_abs_mixed(p: Params, groups, xyz, u, rowptr, colidx, **kw):
+    """assemble_abs() summed over the groups of a mixed mesh."""
+    J = d = None
+    for et, conn in groups:
+        Jg, dg = assemble_abs(p, et, xyz, conn, u, rowptr=rowptr, colidx=colidx, **kw)
+        J = Jg if J is None else J + Jg
+        d = dg if d is None else d + dg
+    return J, d
+
+
 def bnd_faces(et: int, corners, side: int) -> dict:
     """FV1 boundary faces of one element side (restated ugcore FV1Geometry BF)."""
     co = _f64(corners)

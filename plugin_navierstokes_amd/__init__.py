@@ -39,7 +39,7 @@ EXPORTS = [
     "nsfv1_set_kinematic_viscosity_ip", "nsfv1_set_source_ip", "nsfv1_ip_positions",
     "nsfv1_set_stokes", "nsfv1_set_laplace", "nsfv1_set_peclet_blend", "nsfv1_set_grad_div",
     "nsfv1_set_upwind", "nsfv1_set_stabilization", "nsfv1_set_stabilization_without",
-    "nsfv1_set_stabilization_upwind", "nsfv1_set_pac_upwind", "nsfv1_set_mesh", "nsfv1_num_dofs", "nsfv1_nnz",
+    "nsfv1_set_stabilization_upwind", "nsfv1_set_pac_upwind", "nsfv1_set_mesh", "nsfv1_set_mesh_mixed", "nsfv1_num_dofs", "nsfv1_nnz",
     "nsfv1_get_pattern", "nsfv1_pattern_device", "nsfv1_assemble", "nsfv1_assemble_noalloc", "nsfv1_sync_errors",
     "nsfv1_set_launch",
     "nsfv1_set_dirichlet", "nsfv1_apply_dirichlet", "nsfv1_spmv", "nsfv1_linear_solve",
@@ -96,6 +96,8 @@ def load_library(path: str | None = None):
     L.nsfv1_set_stabilization_without.argtypes = [H]
     L.nsfv1_set_stabilization_upwind.argtypes = [H, C.c_char_p]
     L.nsfv1_set_mesh.argtypes = [H, C.c_int, vp, C.c_int64, vp, C.c_int64]
+    L.nsfv1_set_mesh_mixed.argtypes = [H, vp, C.c_int64, C.c_int, C.POINTER(C.c_int), C.POINTER(vp),
+                                       C.POINTER(C.c_int64)]
     L.nsfv1_num_dofs.argtypes = [H, C.POINTER(C.c_int64)]
     L.nsfv1_nnz.argtypes = [H, C.POINTER(C.c_int64)]
     L.nsfv1_get_pattern.argtypes = [H, vp, vp]
@@ -289,6 +291,8 @@ class NavierStokesFV1:
 
     # --- mesh / pattern ---
     def set_mesh(self, mesh=None, et=None, xyz=None, conn=None):
+        if mesh is not None and getattr(mesh, "groups", None) is not None:
+            return self._set_mesh_mixed(mesh)
         if mesh is not None:
             et, xyz, conn = mesh.et, mesh.xyz, mesh.conn
             self.mesh = mesh
@@ -300,6 +304,27 @@ class NavierStokesFV1:
         self.dim = 2 if int(et) in (0, 1) else 3
         self._et = int(et)
         # functions of position are imports: evaluated again on the new mesh
+        if callable(getattr(self, "_nu_fn", None)):
+            self._apply_nu_fn()
+        if callable(getattr(self, "_f_fn", None)):
+            self._apply_f_fn()
+
+    def _set_mesh_mixed(self, mesh):
+        """A mesh of several element types (meshes.MixedMesh): groups of
+        (element type, connectivity) over the same nodes."""
+        xyz = np.ascontiguousarray(mesh.xyz, dtype=np.float64)
+        conns = [np.ascontiguousarray(c, dtype=np.int32) for _, c in mesh.groups]
+        ng = len(conns)
+        ets = (C.c_int * ng)(*[int(et) for et, _ in mesh.groups])
+        ptrs = (C.c_void_p * ng)(*[c.ctypes.data for c in conns])
+        nes = (C.c_int64 * ng)(*[c.shape[0] for c in conns])
+        _check(_lib.nsfv1_set_mesh_mixed(self._h, xyz.ctypes.data, xyz.shape[0], ng, ets, ptrs, nes))
+        self.mesh = mesh
+        self.nv = xyz.shape[0]
+        self.ne = sum(c.shape[0] for c in conns)
+        self.dim = xyz.shape[1]
+        self._et = int(mesh.groups[0][0])
+        # function imports are refused on mixed meshes (raises)
         if callable(getattr(self, "_nu_fn", None)):
             self._apply_nu_fn()
         if callable(getattr(self, "_f_fn", None)):

@@ -93,6 +93,18 @@ struct MGLevel {
     int jprec = 0;
 };
 
+// A further element type of a mixed mesh (the first type lives in the
+// handle itself): its elements, their node incidences with column slots in
+// the mesh's common pattern, per-element geometry and launch plans.
+struct ElemGroup {
+    int et = -1;
+    int64_t ne = 0, ninc = 0;
+    DevBuf conn, node_inc, inc_elem, inc_node, inc_info, inc_pos, l2inv, vol, affine, upw_dec;
+    std::vector<int64_t> h_node_inc;
+    std::vector<int32_t> h_inc_info, h_inc_elem;
+    Plan plan32, plan64;
+};
+
 // enum values of the configuration state
 enum { UPW_UNSET = -1 };
 enum { STAB_UNSET = -1 };
@@ -123,6 +135,9 @@ struct nsfv1_handle {
     DevBuf adj;  // node-block neighbour lists: adj[rowptr[v nc] / nc^2 + s] = s-th neighbour of v
     DevBuf xyz, conn, rowptr, colidx, errflag, l2inv, vol, affine, node_inc, inc_elem, inc_node, inc_info, inc_pos;
     std::vector<int32_t> h_inc_info, h_inc_elem;
+    // element types after the first of a mixed mesh (empty: one type)
+    std::vector<std::unique_ptr<ElemGroup>> groups;
+    bool mixed() const { return !groups.empty(); }
     // boundary discs (no-normal-stress outflow, symmetry, sliding wall): per
     // boundary node, its boundary faces as (incidence, side) items
     struct BndList {
@@ -190,6 +205,64 @@ struct nsfv1_handle {
 // tetrahedron, hexahedron, prism
 static int nsh_count(int et) { static const int n[5] = {3, 4, 4, 8, 6}; return n[et]; }
 static int nip_of(int et) { static const int n[5] = {3, 4, 6, 12, 9}; return n[et]; }
+
+// ---- mesh set-up (helpers of nsfv1_set_mesh / nsfv1_set_mesh_mixed) ----
+struct GroupIn {
+    int et;
+    const int32_t* conn;
+    int64_t ne;
+};
+
+// node -> (element, corner) incidences of one element group, sorted by node
+// then element: cnt = node offsets, ie/in = element and node, ii = corner |
+// slot << 8 (slot = rank among the node's incidences of the group)
+struct IncList {
+    std::vector<int64_t> cnt;
+    std::vector<int32_t> ie, in, ii;
+};
+
+static int build_incidences(const GroupIn& g, int64_t nv, int max_threads, IncList& L)
+{
+    const int nsh = nsh_count(g.et);
+    L.cnt.assign((size_t)nv + 1, 0);
+    for (int64_t i = 0; i < g.ne * nsh; ++i) L.cnt[g.conn[i] + 1]++;
+    for (int64_t v = 0; v < nv; ++v) L.cnt[v + 1] += L.cnt[v];
+    const int64_t ninc = L.cnt[nv];
+    if (ninc >= (int64_t(1) << 31)) return fail(NSFV1_ERR_ARGUMENT, "too many incidences");
+    L.ie.resize(ninc);
+    L.in.resize(ninc);
+    L.ii.resize(ninc);
+    std::vector<int64_t> fill(L.cnt.begin(), L.cnt.end() - 1);
+    for (int64_t e = 0; e < g.ne; ++e)
+        for (int k = 0; k < nsh; ++k) {
+            const int32_t v = g.conn[e * nsh + k];
+            const int64_t q = fill[v]++;
+            L.ie[q] = (int32_t)e;
+            L.in[q] = v;
+            L.ii[q] = k | (int32_t)((q - L.cnt[v]) << 8);
+        }
+    int maxinc = 0;
+    for (int64_t v = 0; v < nv; ++v) maxinc = std::max<int>(maxinc, (int)(L.cnt[v + 1] - L.cnt[v]));
+    if (maxinc > 255 || maxinc > max_threads)
+        return fail(NSFV1_ERR_ARGUMENT, "a node has more incident elements than one workgroup can hold");
+    return NSFV1_OK;
+}
+
+// the incidences a launch plan is built from (the handle's own element
+// group or a further group of a mixed mesh)
+struct IncView {
+    const std::vector<int64_t>& node_inc;
+    const std::vector<int32_t>& info;
+    const std::vector<int32_t>& elem;
+    int64_t ninc;
+    const DevBuf& pos;
+};
+
+static void invalidate_plans(nsfv1_handle* h)
+{
+    h->plan32.valid = h->plan64.valid = false;
+    for (auto& g : h->groups) g->plan32.valid = g->plan64.valid = false;
+}
 
 static int upload(DevBuf& b, const double* x, int64_t n)
 {
@@ -337,6 +410,7 @@ int nsfv1_set_kinematic_viscosity_ip(nsfv1_handle* h, const double* nu, int64_t 
         return NSFV1_OK;
     }
     if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    if (h->mixed()) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "imports at the integration points are not provided on mixed-element meshes");
     if (n != h->ne * nip_of(h->et))
         return fail(NSFV1_ERR_ARGUMENT, "viscosity at the SCVF ips: expected ne * nip = " +
                                             std::to_string(h->ne * nip_of(h->et)) + " values");
@@ -358,6 +432,7 @@ int nsfv1_set_source_ip(nsfv1_handle* h, const double* f_scv, int64_t n_scv, con
     }
     if (!f_scv || !f_scvf) return fail(NSFV1_ERR_ARGUMENT, "the source import needs its values at both the SCV and the SCVF ips");
     if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    if (h->mixed()) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "imports at the integration points are not provided on mixed-element meshes");
     const int64_t a = h->ne * nsh_count(h->et) * h->dim, b = h->ne * nip_of(h->et) * h->dim;
     if (n_scv != a || n_scvf != b)
         return fail(NSFV1_ERR_ARGUMENT, "source at the ips: expected ne * nsh * dim = " + std::to_string(a) +
@@ -374,6 +449,7 @@ int nsfv1_ip_positions(nsfv1_handle* h, int kind, double* out, int64_t n)
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
     if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    if (h->mixed()) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "integration-point positions are not provided on mixed-element meshes");
     if (kind != 0 && kind != 1) return fail(NSFV1_ERR_ARGUMENT, "kind: 0 = SCVF ips, 1 = SCV ips");
     const int64_t want = h->ne * (kind == 0 ? nip_of(h->et) : nsh_count(h->et)) * h->dim;
     if (!out || n != want) return fail(NSFV1_ERR_ARGUMENT, "expected " + std::to_string(want) + " values");
@@ -574,72 +650,69 @@ static bool detect_structured_hex(const double* xyz, int64_t nv, const int32_t* 
     return true;
 }
 
-int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const int32_t* conn, int64_t ne)
+// The mesh as groups of elements of one type each (a mixed mesh numbers its
+// elements group after group): one common CSR pattern over all groups, the
+// node incidences of each group with their column slots in it.
+static int set_mesh_groups(nsfv1_handle* h, const double* xyz, int64_t nv, const std::vector<GroupIn>& G)
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
-    if (et == NSFV1_PYRAMID)
-        return fail(NSFV1_ERR_NOT_IMPLEMENTED, "pyramids are not provided: the apex corner has four edges, so its "
-                                               "FV1 sub-control volume is not the hexahedron the other elements use");
-    if (et < 0 || et > 4) return fail(NSFV1_ERR_ARGUMENT, "unknown element type");
-    if (dim_of(et) != h->dim) return fail(NSFV1_ERR_ARGUMENT, "element type does not match the domain dimension");
-    if (nv <= 0 || ne <= 0 || !xyz || !conn) return fail(NSFV1_ERR_ARGUMENT, "empty mesh");
-    if (nv >= (int64_t(1) << 31) / (h->dim + 1)) return fail(NSFV1_ERR_ARGUMENT, "too many nodes for int32 column indices");
-    const int nsh = nsh_of(et), nc = h->dim + 1;
-    for (int64_t i = 0; i < ne * nsh; ++i)
-        if (conn[i] < 0 || conn[i] >= nv) return fail(NSFV1_ERR_ARGUMENT, "connectivity index out of range");
-
-    // node -> (element, corner) incidences, sorted by node then element
-    std::vector<int64_t> cnt(nv + 1, 0);
-    for (int64_t i = 0; i < ne * nsh; ++i) cnt[conn[i] + 1]++;
-    for (int64_t v = 0; v < nv; ++v) cnt[v + 1] += cnt[v];
-    const int64_t ninc = cnt[nv];
-    if (ninc >= (int64_t(1) << 31)) return fail(NSFV1_ERR_ARGUMENT, "too many incidences");
-    std::vector<int32_t> ie(ninc), in(ninc), ii(ninc);
-    {
-        std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
-        for (int64_t e = 0; e < ne; ++e)
-            for (int k = 0; k < nsh; ++k) {
-                const int32_t v = conn[e * nsh + k];
-                const int64_t q = fill[v]++;
-                ie[q] = (int32_t)e;
-                in[q] = v;
-                ii[q] = k | (int32_t)((q - cnt[v]) << 8);
-            }
+    if (G.empty()) return fail(NSFV1_ERR_ARGUMENT, "no element groups");
+    for (const GroupIn& g : G) {
+        if (g.et == NSFV1_PYRAMID)
+            return fail(NSFV1_ERR_NOT_IMPLEMENTED, "pyramids are not provided: the apex corner has four edges, so its "
+                                                   "FV1 sub-control volume is not the hexahedron the other elements use");
+        if (g.et < 0 || g.et > 4) return fail(NSFV1_ERR_ARGUMENT, "unknown element type");
+        if (dim_of(g.et) != h->dim) return fail(NSFV1_ERR_ARGUMENT, "element type does not match the domain dimension");
+        if (g.ne <= 0 || !g.conn) return fail(NSFV1_ERR_ARGUMENT, "empty mesh");
     }
-    int maxinc = 0;
-    for (int64_t v = 0; v < nv; ++v) maxinc = std::max<int>(maxinc, (int)(cnt[v + 1] - cnt[v]));
-    if (maxinc > 255 || maxinc > h->max_threads)
-        return fail(NSFV1_ERR_ARGUMENT, "a node has more incident elements than one workgroup can hold");
+    if (nv <= 0 || !xyz) return fail(NSFV1_ERR_ARGUMENT, "empty mesh");
+    if (nv >= (int64_t(1) << 31) / (h->dim + 1)) return fail(NSFV1_ERR_ARGUMENT, "too many nodes for int32 column indices");
+    const int nc = h->dim + 1;
+    const size_t ng = G.size();
+    for (const GroupIn& g : G)
+        for (int64_t i = 0; i < g.ne * nsh_of(g.et); ++i)
+            if (g.conn[i] < 0 || g.conn[i] >= nv) return fail(NSFV1_ERR_ARGUMENT, "connectivity index out of range");
+
+    std::vector<IncList> inc(ng);
+    for (size_t g = 0; g < ng; ++g) {
+        int rc = build_incidences(G[g], nv, h->max_threads, inc[g]);
+        if (rc) return rc;
+    }
+    // the neighbours of node v: every corner of its incident elements (of
+    // all groups), sorted, without repeats; an isolated node keeps itself
+    auto neighbours = [&](int64_t v, std::vector<int32_t>& nb) {
+        nb.clear();
+        for (size_t g = 0; g < ng; ++g) {
+            const int nsh = nsh_of(G[g].et);
+            for (int64_t q = inc[g].cnt[v]; q < inc[g].cnt[v + 1]; ++q)
+                for (int k = 0; k < nsh; ++k) nb.push_back(G[g].conn[(int64_t)inc[g].ie[q] * nsh + k]);
+        }
+        if (nb.empty()) nb.push_back((int32_t)v);
+        std::sort(nb.begin(), nb.end());
+        nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+    };
 
     // CSR pattern: node adjacency closure (x) (dim+1)^2, sorted columns
     std::vector<int64_t> rowptr(nv * nc + 1);
-    std::vector<int32_t> deg(nv);
     {
         std::vector<int32_t> nb;
         nb.reserve(256);
-        for (int64_t v = 0; v < nv; ++v) {
-            nb.clear();
-            for (int64_t q = cnt[v]; q < cnt[v + 1]; ++q)
-                for (int k = 0; k < nsh; ++k) nb.push_back(conn[(int64_t)ie[q] * nsh + k]);
-            if (nb.empty()) nb.push_back((int32_t)v);
-            std::sort(nb.begin(), nb.end());
-            deg[v] = (int32_t)(std::unique(nb.begin(), nb.end()) - nb.begin());
-        }
         rowptr[0] = 0;
-        for (int64_t v = 0; v < nv; ++v)
-            for (int a = 0; a < nc; ++a) rowptr[v * nc + a + 1] = rowptr[v * nc + a] + (int64_t)deg[v] * nc;
+        for (int64_t v = 0; v < nv; ++v) {
+            neighbours(v, nb);
+            if (nb.size() > 255) return fail(NSFV1_ERR_ARGUMENT, "a node has more than 255 neighbours");
+            for (int a = 0; a < nc; ++a) rowptr[v * nc + a + 1] = rowptr[v * nc + a] + (int64_t)nb.size() * nc;
+        }
     }
     const int64_t nnz = rowptr[nv * nc];
-    for (int64_t v = 0; v < nv; ++v)
-        if (deg[v] > 255) return fail(NSFV1_ERR_ARGUMENT, "a node has more than 255 neighbours");
     int mn[3] = {0, 0, 0};
-    const bool structured = et == ET_HEX && detect_structured_hex(xyz, nv, conn, ne, rowptr, mn);
+    const bool structured = ng == 1 && G[0].et == ET_HEX && detect_structured_hex(xyz, nv, G[0].conn, G[0].ne, rowptr, mn);
     std::vector<int32_t> colidx, adjv;
-    std::vector<uint64_t> ipos;
+    std::vector<std::vector<uint64_t>> ipos(ng);
     try {
         colidx.resize(nnz);
         adjv.resize(nnz / (nc * nc));
-        ipos.resize(ninc);
+        for (size_t g = 0; g < ng; ++g) ipos[g].resize(inc[g].ie.size());
     } catch (...) {
         return fail(NSFV1_ERR_ARGUMENT, "pattern too large for host memory");
     }
@@ -647,20 +720,19 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
         std::vector<int32_t> nb;
         nb.reserve(256);
         for (int64_t v = 0; v < nv; ++v) {
-            nb.clear();
-            for (int64_t q = cnt[v]; q < cnt[v + 1]; ++q)
-                for (int k = 0; k < nsh; ++k) nb.push_back(conn[(int64_t)ie[q] * nsh + k]);
-            if (nb.empty()) nb.push_back((int32_t)v);
-            std::sort(nb.begin(), nb.end());
-            nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
-            for (int64_t q = cnt[v]; q < cnt[v + 1]; ++q) {
-                uint64_t pk = 0;
-                for (int k = 0; k < nsh; ++k) {
-                    const int32_t w = conn[(int64_t)ie[q] * nsh + k];
-                    const uint64_t slot = (uint64_t)(std::lower_bound(nb.begin(), nb.end(), w) - nb.begin());
-                    pk |= slot << (8 * k);
+            neighbours(v, nb);
+            // column slots of each incident element's corners in v's row
+            for (size_t g = 0; g < ng; ++g) {
+                const int nsh = nsh_of(G[g].et);
+                for (int64_t q = inc[g].cnt[v]; q < inc[g].cnt[v + 1]; ++q) {
+                    uint64_t pk = 0;
+                    for (int k = 0; k < nsh; ++k) {
+                        const int32_t w = G[g].conn[(int64_t)inc[g].ie[q] * nsh + k];
+                        const uint64_t slot = (uint64_t)(std::lower_bound(nb.begin(), nb.end(), w) - nb.begin());
+                        pk |= slot << (8 * k);
+                    }
+                    ipos[g][q] = pk;
                 }
-                ipos[q] = pk;
             }
             for (size_t t = 0; t < nb.size(); ++t) adjv[rowptr[v * nc] / (nc * nc) + t] = nb[t];
             for (int a = 0; a < nc; ++a) {
@@ -671,35 +743,62 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
         }
     }
 
-    std::vector<int32_t> ni32(cnt.begin(), cnt.end());
     auto up = [](DevBuf& b, const void* src, size_t bytes) -> bool {
         if (b.alloc(bytes) != hipSuccess) return false;
         return hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
     };
-    bool ok = up(h->xyz, xyz, sizeof(double) * nv * h->dim) && up(h->conn, conn, sizeof(int32_t) * ne * nsh) &&
+    // the device arrays of group g: its connectivity, incidences and
+    // per-element geometry buffers
+    auto up_group = [&](size_t g, DevBuf& conn, DevBuf& node_inc, DevBuf& inc_elem, DevBuf& inc_node, DevBuf& inc_info,
+                        DevBuf& inc_pos, DevBuf& l2inv, DevBuf& vol, DevBuf& affine) -> bool {
+        const GroupIn& gi = G[g];
+        const IncList& L = inc[g];
+        const size_t ninc = L.ie.size();
+        const std::vector<int32_t> ni32(L.cnt.begin(), L.cnt.end());
+        return up(conn, gi.conn, sizeof(int32_t) * gi.ne * nsh_of(gi.et)) &&
+               up(inc_elem, L.ie.data(), sizeof(int32_t) * std::max<size_t>(ninc, 1)) &&
+               up(inc_node, L.in.data(), sizeof(int32_t) * std::max<size_t>(ninc, 1)) &&
+               up(inc_info, L.ii.data(), sizeof(int32_t) * std::max<size_t>(ninc, 1)) &&
+               up(inc_pos, ipos[g].data(), sizeof(uint64_t) * std::max<size_t>(ninc, 1)) &&
+               up(node_inc, ni32.data(), sizeof(int32_t) * ni32.size()) &&
+               l2inv.alloc(sizeof(double) * gi.ne * nip_of(gi.et)) == hipSuccess &&
+               vol.alloc(sizeof(double) * gi.ne * nsh_of(gi.et)) == hipSuccess && affine.alloc((size_t)gi.ne) == hipSuccess;
+    };
+    std::vector<std::unique_ptr<ElemGroup>> more;
+    bool ok = up(h->xyz, xyz, sizeof(double) * nv * h->dim) &&
               up(h->rowptr, rowptr.data(), sizeof(int64_t) * rowptr.size()) &&
               up(h->colidx, colidx.data(), sizeof(int32_t) * nnz) &&
               up(h->adj, adjv.data(), sizeof(int32_t) * std::max<size_t>(adjv.size(), 1)) &&
-              up(h->inc_elem, ie.data(), sizeof(int32_t) * ninc) && up(h->inc_node, in.data(), sizeof(int32_t) * ninc) &&
-              up(h->inc_info, ii.data(), sizeof(int32_t) * ninc) && up(h->inc_pos, ipos.data(), sizeof(uint64_t) * ninc) &&
-              up(h->node_inc, ni32.data(), sizeof(int32_t) * ni32.size()) &&
-              h->l2inv.alloc(sizeof(double) * ne * nip_of(et)) == hipSuccess &&
-              h->vol.alloc(sizeof(double) * ne * nsh) == hipSuccess && h->affine.alloc((size_t)ne) == hipSuccess;
+              up_group(0, h->conn, h->node_inc, h->inc_elem, h->inc_node, h->inc_info, h->inc_pos, h->l2inv, h->vol,
+                       h->affine);
+    for (size_t g = 1; ok && g < ng; ++g) {
+        auto eg = std::make_unique<ElemGroup>();
+        ok = up_group(g, eg->conn, eg->node_inc, eg->inc_elem, eg->inc_node, eg->inc_info, eg->inc_pos, eg->l2inv,
+                      eg->vol, eg->affine);
+        eg->et = G[g].et;
+        eg->ne = G[g].ne;
+        eg->ninc = (int64_t)inc[g].ie.size();
+        eg->h_node_inc.swap(inc[g].cnt);
+        eg->h_inc_info.swap(inc[g].ii);
+        eg->h_inc_elem.swap(inc[g].ie);
+        more.push_back(std::move(eg));
+    }
     if (!ok) return fail(NSFV1_ERR_DEVICE, std::string("mesh upload: ") + hipGetErrorString(hipGetLastError()));
-    h->et = et;
+    h->et = G[0].et;
     h->nv = nv;
-    h->ne = ne;
+    h->ne = G[0].ne;
     h->ndof = nv * nc;
     h->march_ok = structured;
     for (int a = 0; a < 3; ++a) h->march_n[a] = mn[a];
     h->nnz = nnz;
-    h->ninc = ninc;
+    h->ninc = (int64_t)inc[0].ie.size();
     h->h_rowptr.swap(rowptr);
-    h->h_node_inc.swap(cnt);
-    h->h_inc_info.swap(ii);
-    h->h_inc_elem.swap(ie);
+    h->h_node_inc.swap(inc[0].cnt);
+    h->h_inc_info.swap(inc[0].ii);
+    h->h_inc_elem.swap(inc[0].ie);
+    h->groups.swap(more);
     for (auto& b : h->bnd) b.nbn = 0;
-    h->plan32.valid = h->plan64.valid = false;
+    invalidate_plans(h);
     h->geo_valid = false;
     // solver state sized for the previous mesh
     h->gm_restart = 0;
@@ -719,6 +818,21 @@ int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const
     h->recs.release();
     h->cpl_recs.release();
     return NSFV1_OK;
+}
+
+int nsfv1_set_mesh(nsfv1_handle* h, int et, const double* xyz, int64_t nv, const int32_t* conn, int64_t ne)
+{
+    return set_mesh_groups(h, xyz, nv, {GroupIn{et, conn, ne}});
+}
+
+int nsfv1_set_mesh_mixed(nsfv1_handle* h, const double* xyz, int64_t nv, int ngroups, const int* et,
+                         const int32_t* const* conn, const int64_t* ne)
+{
+    if (ngroups <= 0 || ngroups > 8 || !et || !conn || !ne)
+        return fail(NSFV1_ERR_ARGUMENT, "mixed mesh: 1..8 element groups with their types, connectivities and counts");
+    std::vector<GroupIn> G;
+    for (int g = 0; g < ngroups; ++g) G.push_back(GroupIn{et[g], conn[g], ne[g]});
+    return set_mesh_groups(h, xyz, nv, G);
 }
 
 int nsfv1_num_dofs(nsfv1_handle* h, int64_t* ndof)
@@ -771,13 +885,13 @@ int nsfv1_set_launch(nsfv1_handle* h, int threads, int lds_bytes)
         if (lds_bytes < 4096 || lds_bytes > 65536) return fail(NSFV1_ERR_ARGUMENT, "LDS budget must be 4..64 KiB");
         h->lds_cap = (size_t)lds_bytes;
     }
-    h->plan32.valid = h->plan64.valid = false;
+    invalidate_plans(h);
     return NSFV1_OK;
 }
 
 // Workgroup plan: consecutive nodes while their incidences fit the thread
 // budget and their rows (plus defect) fit the LDS budget (<= 64 KiB).
-static int make_plan(nsfv1_handle* h, int jbytes, Plan& P)
+static int make_plan(nsfv1_handle* h, const IncView& g, int jbytes, Plan& P)
 {
     const int nc = h->dim + 1;
     std::vector<int32_t> wn, wr;
@@ -792,14 +906,14 @@ static int make_plan(nsfv1_handle* h, int jbytes, Plan& P)
         const int64_t v0 = v;
         int rounds = 0;
         while (v < h->nv) {
-            const int64_t incs = h->h_node_inc[v + 1] - h->h_node_inc[v0];
+            const int64_t incs = g.node_inc[v + 1] - g.node_inc[v0];
             const int64_t vals = h->h_rowptr[(v + 1) * nc] - h->h_rowptr[v0 * nc];
             const size_t lds = ((size_t)vals * jbytes + (size_t)(v + 1 - v0) * 16 + 15) / 16 * 16 + (size_t)(v + 1 - v0) * nc * 8;
             if (incs > inc_budget || lds > h->lds_cap) {
                 if (v == v0) return fail(NSFV1_ERR_ARGUMENT, "one node row exceeds the workgroup budget");
                 break;
             }
-            rounds = std::max<int>(rounds, (int)(h->h_node_inc[v + 1] - h->h_node_inc[v]));
+            rounds = std::max<int>(rounds, (int)(g.node_inc[v + 1] - g.node_inc[v]));
             maxlds = std::max(maxlds, lds);
             ++v;
         }
@@ -812,12 +926,12 @@ static int make_plan(nsfv1_handle* h, int jbytes, Plan& P)
     P.rowlane = rowlane;
     // lane order inside each workgroup: by incidence slot, then node, so a
     // wavefront takes part in as few accumulation rounds as possible
-    std::vector<int32_t> perm((size_t)h->ninc);
+    std::vector<int32_t> perm((size_t)g.ninc);
     for (int w = 0; w < P.nwg; ++w) {
-        const int64_t i0 = h->h_node_inc[wn[w]], i1 = h->h_node_inc[wn[w + 1]];
+        const int64_t i0 = g.node_inc[wn[w]], i1 = g.node_inc[wn[w + 1]];
         std::vector<std::pair<int32_t, int32_t>> key;
         key.reserve((size_t)(i1 - i0));
-        for (int64_t q = i0; q < i1; ++q) key.push_back({h->h_inc_info[q] >> 8, (int32_t)q});
+        for (int64_t q = i0; q < i1; ++q) key.push_back({g.info[q] >> 8, (int32_t)q});
         std::stable_sort(key.begin(), key.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
         for (int64_t t = 0; t < i1 - i0; ++t) perm[i0 + t] = key[t].second;
     }
@@ -828,24 +942,24 @@ static int make_plan(nsfv1_handle* h, int jbytes, Plan& P)
     if (!up(P.wg_node, wn) || !up(P.wg_rounds, wr) || !up(P.perm, perm)) return fail(NSFV1_ERR_DEVICE, "plan upload");
     {
         const int pade = nc == 4 ? 16 / jbytes : 0;  // the row kernels pad 3D node rows only
-        std::vector<int32_t> la((size_t)h->ninc * 4), wa((size_t)P.nwg * 8);
-        std::vector<uint64_t> lp((size_t)h->ninc);
-        std::vector<uint64_t> hpos((size_t)h->ninc);
-        if (h->ninc > 0 &&
-            hipMemcpy(hpos.data(), h->inc_pos.p, sizeof(uint64_t) * hpos.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        std::vector<int32_t> la((size_t)g.ninc * 4), wa((size_t)P.nwg * 8);
+        std::vector<uint64_t> lp((size_t)g.ninc);
+        std::vector<uint64_t> hpos((size_t)g.ninc);
+        if (g.ninc > 0 &&
+            hipMemcpy(hpos.data(), g.pos.p, sizeof(uint64_t) * hpos.size(), hipMemcpyDeviceToHost) != hipSuccess)
             return fail(NSFV1_ERR_DEVICE, "plan download");
         for (int w = 0; w < P.nwg; ++w) {
             const int64_t v0 = wn[w], v1 = wn[w + 1];
-            const int64_t i0 = h->h_node_inc[v0], i1 = h->h_node_inc[v1];
+            const int64_t i0 = g.node_inc[v0], i1 = g.node_inc[v1];
             const int64_t jbase = h->h_rowptr[v0 * nc];
             for (int64_t t = i0; t < i1; ++t) {
                 const int32_t q = perm[t];
                 int64_t v = v0;
-                while (h->h_node_inc[v + 1] <= q) ++v;
-                const int32_t info = h->h_inc_info[q];
+                while (g.node_inc[v + 1] <= q) ++v;
+                const int32_t info = g.info[q];
                 const int64_t rlen = h->h_rowptr[v * nc + 1] - h->h_rowptr[v * nc];
                 if (rlen >= (1 << 15)) return fail(NSFV1_ERR_ARGUMENT, "node row too long for the plan");
-                la[t * 4 + 0] = h->h_inc_elem[q];
+                la[t * 4 + 0] = g.elem[q];
                 la[t * 4 + 1] = (int32_t)v;
                 la[t * 4 + 2] = (info & 0xffff) | (int32_t)(rlen << 16);
                 la[t * 4 + 3] = (int32_t)(h->h_rowptr[v * nc] - jbase + (v - v0) * pade);
@@ -892,6 +1006,11 @@ static int ensure_geometry(nsfv1_handle* h, void* stream)
     hipError_t ge = launch_geometry_cache(h->et, h->xyz.as<double>(), h->conn.as<int32_t>(), h->ne, h->difflen,
                                           h->l2inv.as<double>(), h->vol.as<double>(), h->affine.as<uint8_t>(),
                                           h->errflag.as<int>(), (hipStream_t)stream);
+    for (size_t g = 0; ge == hipSuccess && g < h->groups.size(); ++g) {
+        ElemGroup& G = *h->groups[g];
+        ge = launch_geometry_cache(G.et, h->xyz.as<double>(), G.conn.as<int32_t>(), G.ne, h->difflen, G.l2inv.as<double>(),
+                                   G.vol.as<double>(), G.affine.as<uint8_t>(), h->errflag.as<int>(), (hipStream_t)stream);
+    }
     if (ge != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("geometry cache: ") + hipGetErrorString(ge));
     h->geo_valid = true;
     h->geo_difflen = h->difflen;
@@ -942,6 +1061,9 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
         if (h->et == ET_PRISM && coupled_config(h))
             return fail(NSFV1_ERR_NOT_IMPLEMENTED, "prisms are not provided with the FLOW stabilization or the "
                                                    "positive/regular upwinds");
+        if (h->mixed() && coupled_config(h))
+            return fail(NSFV1_ERR_NOT_IMPLEMENTED, "mixed-element meshes are not provided with the FLOW stabilization "
+                                                   "or the positive/regular upwinds");
         if (h->ip_imports()) {
             for (int kind = 0; kind < BND_KINDS; ++kind)
                 if (h->bnd[kind].nbn)
@@ -957,7 +1079,19 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
         if (scratch)
             return fail(NSFV1_ERR_ARGUMENT, "no launch plan for this Jacobian precision yet: the no-allocation entry "
                                             "needs one nsfv1_assemble call with the same jprec first (it builds the plan)");
-        int rc = make_plan(h, jprec == NSFV1_F64 ? 8 : 4, P);
+        int rc = make_plan(h, IncView{h->h_node_inc, h->h_inc_info, h->h_inc_elem, h->ninc, h->inc_pos},
+                           jprec == NSFV1_F64 ? 8 : 4, P);
+        if (rc) return rc;
+    }
+    for (auto& gp : h->groups) {
+        ElemGroup& G = *gp;
+        Plan& PG = jprec == NSFV1_F64 ? G.plan64 : G.plan32;
+        if (PG.valid) continue;
+        if (scratch)
+            return fail(NSFV1_ERR_ARGUMENT, "no launch plan for this Jacobian precision yet: the no-allocation entry "
+                                            "needs one nsfv1_assemble call with the same jprec first (it builds the plan)");
+        int rc = make_plan(h, IncView{G.h_node_inc, G.h_inc_info, G.h_inc_elem, G.ninc, G.inc_pos},
+                           jprec == NSFV1_F64 ? 8 : 4, PG);
         if (rc) return rc;
     }
     KParams k{};
@@ -995,6 +1129,9 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
                                             "nsfv1_assemble call in NSFV1_UPWIND_RECORD mode first (it allocates it)");
         const int nip = nip_of(h->et);
         if (h->upw_dec.alloc((size_t)h->ne * nip) != hipSuccess) return fail(NSFV1_ERR_DEVICE, "upwind record buffer");
+        for (auto& gp : h->groups)
+            if (gp->upw_dec.alloc((size_t)gp->ne * nip_of(gp->et)) != hipSuccess)
+                return fail(NSFV1_ERR_DEVICE, "upwind record buffer");
     }
     KMeshPtrs m{h->xyz.as<double>(), h->conn.as<int32_t>(), h->rowptr.as<int64_t>(), h->colidx.as<int32_t>(),
                 h->node_inc.as<int32_t>(), h->inc_elem.as<int32_t>(), h->inc_node.as<int32_t>(),
@@ -1010,7 +1147,40 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
         k.conv_upwind == k.stab_upwind && (k.conv_upwind == UPW_NO || k.conv_upwind == UPW_FULL || k.conv_upwind == UPW_SKEWED) &&
         ((uintptr_t)jvals & 15) == 0 && ((uintptr_t)defect & 15) == 0)
         mfast = k.conv_upwind;
-    if (mfast >= 0) {
+    if (h->mixed()) {
+        // a mixed mesh: every element group on the fused kernel, in group
+        // order on the one stream; the first group assembles every row from
+        // zero, each further group adds its elements to the rows it touches
+        e = launch_assemble(h->et, jprec, m, k, u, u_old, jvals, defect, h->errflag.as<int>(), P.nwg, P.lds,
+                            (hipStream_t)stream);
+        for (size_t g = 0; e == hipSuccess && g < h->groups.size(); ++g) {
+            ElemGroup& G = *h->groups[g];
+            const Plan& PG = jprec == NSFV1_F64 ? G.plan64 : G.plan32;
+            KMeshPtrs mg = m;
+            mg.conn = G.conn.as<int32_t>();
+            mg.node_inc = G.node_inc.as<int32_t>();
+            mg.inc_elem = G.inc_elem.as<int32_t>();
+            mg.inc_node = G.inc_node.as<int32_t>();
+            mg.inc_info = G.inc_info.as<int32_t>();
+            mg.wg_node = PG.wg_node.as<int32_t>();
+            mg.wg_rounds = PG.wg_rounds.as<int32_t>();
+            mg.l2inv = G.l2inv.as<double>();
+            mg.vol = G.vol.as<double>();
+            mg.upw_dec = G.upw_dec.as<uint8_t>();
+            mg.affine = G.affine.as<uint8_t>();
+            mg.inc_pos = G.inc_pos.as<uint64_t>();
+            mg.perm = PG.perm.as<int32_t>();
+            mg.block = PG.block;
+            mg.lane_a = PG.lane_a.as<int4>();
+            mg.lane_pos = PG.lane_pos.as<uint64_t>();
+            mg.wg_a = PG.wg_a.as<int4>();
+            KParams kg = k;
+            kg.accum = 1;
+            e = launch_assemble(G.et, jprec, mg, kg, u, u_old, jvals, defect, h->errflag.as<int>(), PG.nwg, PG.lds,
+                                (hipStream_t)stream);
+        }
+        h->last_path = 1;
+    } else if (mfast >= 0) {
         MarchGrid g{};
         g.nx = h->march_n[0];
         g.ny = h->march_n[1];
@@ -1107,6 +1277,7 @@ int nsfv1_batch_set_instances(nsfv1_handle* h, int64_t ninst, const int64_t* ele
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
     if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    if (h->mixed() && ninst != 0) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "batched instances of mixed-element meshes are not provided");
     if (ninst == 0) {
         h->nu_elem.release();
         h->ninst = 0;
@@ -1152,6 +1323,7 @@ static int set_boundary_list(nsfv1_handle* h, int kind, const int32_t* elem, con
     B.nbn = 0;
     if (n == 0) return NSFV1_OK;
     if (h->et == ET_PRISM) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "boundary discs on prisms are not provided");
+    if (h->mixed()) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "boundary discs on mixed-element meshes are not provided");
     if (n < 0 || !elem || !side) return fail(NSFV1_ERR_ARGUMENT, "bad boundary side arrays");
     const int nsh = nsh_of(h->et);
     const int nside = h->et == ET_HEX ? 6 : (h->et == ET_QUAD ? 4 : (h->et == ET_TET ? 4 : 3));
@@ -1252,6 +1424,7 @@ int nsfv1_export(nsfv1_handle* h, int what, int at, const double* local_ips, int
     int rc = nsfv1_export_size(h, what, at, nip, &count);
     if (rc) return rc;
     if (h->et == ET_PRISM) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "data exports on prisms are not provided");
+    if (h->mixed()) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "data exports on mixed-element meshes are not provided");
     if (!u || !out) return fail(NSFV1_ERR_ARGUMENT, "NULL buffer");
     if (at == NSFV1_AT_LOCAL && !local_ips) return fail(NSFV1_ERR_ARGUMENT, "local points are NULL");
     const int dim = (h->et == ET_TRI || h->et == ET_QUAD) ? 2 : 3;
@@ -1330,7 +1503,7 @@ int nsfv1_set_assembly_path(nsfv1_handle* h, int path)
     if (path < 0 || path > 3)
         return fail(NSFV1_ERR_ARGUMENT, "path must be 0 (split where supported), 1 (fused), 2 (split, row lanes) or "
                                         "3 (plane march where supported)");
-    if (path != h->path) h->plan32.valid = h->plan64.valid = false;  // the row-lane plan packs fewer incidences
+    if (path != h->path) invalidate_plans(h);  // the row-lane plan packs fewer incidences
     h->path = path;
     return NSFV1_OK;
 }
@@ -2090,7 +2263,7 @@ int nsfv1_autotune(nsfv1_handle* h, const double* u, const double* u_old, double
         h->max_threads = keep_threads;
         h->lds_cap = keep_lds;
         h->march_zc = keep_zc;
-        h->plan32.valid = h->plan64.valid = false;
+        invalidate_plans(h);
         return fail(NSFV1_ERR_ARGUMENT, "no launch shape could assemble this mesh");
     }
     int rc = nsfv1_set_assembly_path(h, bc.path);

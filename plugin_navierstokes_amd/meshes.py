@@ -171,6 +171,84 @@ def box_3d(cells, lengths, et: int = HEX, jitter: float = 0.0, seed: int = 0, ki
     return Mesh(et, xyz, conn.astype(np.int32), bnd, (nx, ny, nz), recipe)
 
 
+@dataclass
+class MixedMesh:
+    """A mesh of several element types over one node set: groups of
+    (element type, connectivity [ne, nsh]); elements are numbered group after
+    group."""
+    groups: list
+    xyz: np.ndarray
+    boundary: dict
+    shape: tuple = ()
+
+    @property
+    def dim(self) -> int:
+        return self.xyz.shape[1]
+
+    @property
+    def nv(self) -> int:
+        return self.xyz.shape[0]
+
+    @property
+    def ne(self) -> int:
+        return sum(c.shape[0] for _, c in self.groups)
+
+
+def mixed_2d(n: int, jitter: float = 0.0, seed: int = 0, first: int = QUAD) -> MixedMesh:
+    """The n x n cavity of cavity_2d with every third cell ((i + 2j) % 3 ==
+    0) split into two triangles along its (i,j)-(i+1,j+1) diagonal, the
+    others quadrilaterals; `first` names the group listed first."""
+    m = cavity_2d(n, QUAD, jitter, seed)
+    ci, cj = np.arange(m.ne) % n, np.arange(m.ne) // n
+    split = (ci + 2 * cj) % 3 == 0
+    quads = m.conn[~split]
+    q = m.conn[split]
+    tris = np.stack([q[:, [0, 1, 2]], q[:, [0, 2, 3]]], axis=1).reshape(-1, 3)
+    groups = [(QUAD, quads), (TRI, tris.astype(np.int32))]
+    if first == TRI:
+        groups = groups[::-1]
+    return MixedMesh(groups, m.xyz, m.boundary, (n, n))
+
+
+def mixed_3d(n: int, kind: str = "hex_prism", jitter: float = 0.0, seed: int = 0) -> MixedMesh:
+    """The n^3 cavity of cavity_3d with conforming element types:
+    "hex_prism": the cells with i < n/2 split into two prisms each (their
+      quadrilateral sides meet the hexahedra whole), the others hexahedra;
+    "prism_tet": the cells with k < n/2 split into two prisms, the cells
+      above into six Kuhn tetrahedra (both split the horizontal faces along
+      the same diagonal).
+    A hexahedron and a tetrahedron cannot share a face (a quadrilateral
+    against two triangles): that needs a pyramid between them."""
+    m = box_3d((n, n, n), (1.0, 1.0, 1.0), et=HEX, jitter=jitter, seed=seed)
+    c = np.arange(m.ne)
+    ci, ck = c % n, c // (n * n)
+    low_i, low_k = ci < n // 2, ck < n // 2
+    if kind == "hex_prism":
+        sel = {HEX: ~low_i, PRISM: low_i}
+        order = [HEX, PRISM]
+    elif kind == "prism_tet":
+        sel = {PRISM: low_k, TET: ~low_k}
+        order = [PRISM, TET]
+    else:
+        raise ValueError(kind)
+    groups = []
+    for et in order:
+        h = m.conn[sel[et]]
+        if et == HEX:
+            conn = h
+        elif et == PRISM:
+            conn = np.stack([h[:, list(t)] for t in _PRISMS], axis=1).reshape(-1, 6)
+        else:
+            conn = np.stack([h[:, list(t)] for t in _KUHN], axis=1).reshape(-1, 4)
+        groups.append((et, conn.astype(np.int32)))
+    return MixedMesh(groups, m.xyz, m.boundary, (n, n, n))
+
+
+def split_groups(mesh: Mesh, cut: int) -> MixedMesh:
+    """One element type as two groups: elements [0, cut) and [cut, ne)."""
+    return MixedMesh([(mesh.et, mesh.conn[:cut]), (mesh.et, mesh.conn[cut:])], mesh.xyz, mesh.boundary, mesh.shape)
+
+
 def _grid_index(shape):
     """Structured node coordinates (i, j[, k]) of a lexicographic numbering."""
     dims = [s + 1 for s in shape]

@@ -1,0 +1,202 @@
+"""Mixed-element meshes through the C ABI (nsfv1_set_mesh_mixed): several
+element types over one node set, as ugcore's element loop runs the disc
+once per type of a subset (navier_stokes_fv1.cpp:136-182, :1515-1551).
+Checked against the oracle's sum over the element groups in the same
+pattern (quadrilaterals + triangles in both group orders, hexahedra +
+prisms, prisms + tetrahedra; four upwinds; FP64 and FP32 Jacobians; A + M +
+rhs parts, time-dependent stabilisation); the pattern against the union of
+the types' adjacencies; one element type given as two groups against the
+single group bitwise (the same sums in the same order); the skewed-upwind
+record and replay across groups; a Newton solve; and the refusals.
+
+Tolerance: the parity rule (|a - b| <= 1e-12 |b| + 1e-13 S in FP64,
+2^-17 S + 2^-20 |b| for FP32 Jacobians); on meshes with prisms plus the
+prisms' absolute floor of 1e-15 max|b| (an entry that cancels exactly in the
+oracle's corner order can be ~1e-19 in the kernel's canonical order)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+from plugin_navierstokes_amd import (F32, F64, PART_A, PART_M, PART_RHS, UPWIND_FRESH, UPWIND_RECORD,
+                                     UPWIND_REPLAY, NavierStokesFV1, NSFV1Error)
+from plugin_navierstokes_amd import meshes as M
+from tests._parity import FP32_ABS, FP32_REL, FP64_ABS, FP64_REL, configure_disc, oracle_params
+
+PRISM_FLOOR = 1e-15
+
+
+def _viol(a, b, scale, rel, ab, floor):
+    a, b, scale = np.asarray(a, dtype=np.float64), np.asarray(b), np.asarray(scale)
+    tol = rel * np.abs(b) + ab * scale + floor * np.abs(b).max() + 1e-300
+    i = int(np.argmax(np.abs(a - b) / tol))
+    return float(np.abs(a - b)[i] / tol[i]), f"idx={i} got={a[i]!r} want={b[i]!r} scale={scale[i]!r}"
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "quad_tri": lambda: M.mixed_2d(6, 0.1, 1),
+    "tri_quad": lambda: M.mixed_2d(5, 0.1, 2, first=M.TRI),
+    "hex_prism": lambda: M.mixed_3d(3, "hex_prism", 0.1, 1),
+    "prism_tet": lambda: M.mixed_3d(4, "prism_tet", 0.1, 2),
+}
+
+
+def _disc(mm, cfg):
+    disc = NavierStokesFV1("u,v,p" if mm.dim == 2 else "u,v,w,p", "Inner")
+    configure_disc(disc, cfg)
+    disc.set_mesh(mm)
+    return disc
+
+
+@pytest.mark.parametrize("jprec", [F64, F32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("up", ["no", "full", "skewed", "lps"])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_mixed_mesh_parity(cuda, name, up, jprec):
+    import torch
+
+    print(f"case mixed={name} upwind={up} jprec={jprec}", flush=True)
+    mm = CASES[name]()
+    dim = mm.dim
+    cfg = {"upwind": up, "source": (0.3, -0.2, 0.1)[:dim]}
+    disc = _disc(mm, cfg)
+    rowptr, colidx = disc.pattern()
+    rp, ci = O.pattern_mixed(mm.nv, mm.groups)
+    assert np.array_equal(rowptr, rp) and np.array_equal(colidx, ci)
+    u = M.random_state(mm, 5)
+    uold = M.random_state(mm, 105)
+    dt, s = 0.01, (0.7, 1.3, -0.7)
+    parts = PART_A | PART_M | PART_RHS
+    J, d = disc.assemble(torch.from_numpy(u).to(cuda), torch.from_numpy(uold).to(cuda), dt, s_a=s[0], s_m=s[1],
+                         s_f=s[2], parts=parts, jprec=jprec)
+    assert disc.assembly_path_used() == 1
+    J, d = J.cpu().numpy(), d.cpu().numpy()
+    p = oracle_params(dim, cfg)
+    kw = dict(uold=uold, dt=dt, s_a=s[0], s_m=s[1], parts=parts)
+    Jr, dr = O.assemble_mixed(p, mm.groups, mm.xyz, u, rowptr, colidx, s_f=s[2], **kw)
+    Ja, da = O.assemble_abs_mixed(p, mm.groups, mm.xyz, u, rowptr, colidx, **kw)
+    da = da + np.abs(dr)
+    floor = PRISM_FLOOR if any(et == M.PRISM for et, _ in mm.groups) else 0.0
+    v, where = _viol(d, dr, da, FP64_REL, FP64_ABS, floor)
+    assert v <= 1.0, where
+    if jprec == F64:
+        v, where = _viol(J, Jr, Ja, FP64_REL, FP64_ABS, floor)
+    else:
+        v, where = _viol(J, Jr, Ja, FP32_REL, FP32_ABS, floor)
+    assert v <= 1.0, where
+
+
+@pytest.mark.parametrize("et", [M.TRI, M.TET, M.HEX, M.PRISM], ids=["tri", "tet", "hex", "prism"])
+def test_one_type_as_two_groups_is_bitwise_the_single_group(cuda, et):
+    import torch
+
+    mesh = M.cavity_2d(6, et, 0.1, 3) if et == M.TRI else M.cavity_3d(3, et, 0.1, 3)
+    print(f"case split groups et={et} ne={mesh.ne}", flush=True)
+    cfg = {"upwind": "lps"}
+    a = NavierStokesFV1("u,v,p" if mesh.dim == 2 else "u,v,w,p", "Inner")
+    configure_disc(a, cfg)
+    a.set_mesh(mesh)
+    a.set_assembly_path(1)
+    b = _disc(M.split_groups(mesh, mesh.ne // 3), cfg)
+    u = torch.from_numpy(M.random_state(mesh, 2)).to(cuda)
+    uo = torch.from_numpy(M.random_state(mesh, 3)).to(cuda)
+    for jprec in (F64, F32):
+        Ja, da = a.assemble(u, uo, 0.01, s_m=1.0, parts=PART_A | PART_M, jprec=jprec)
+        Jb, db = b.assemble(u, uo, 0.01, s_m=1.0, parts=PART_A | PART_M, jprec=jprec)
+        assert a.assembly_path_used() == b.assembly_path_used() == 1
+        assert torch.equal(Ja, Jb) and torch.equal(da, db)
+        # defect only (no Jacobian) through the same groups
+        _, dd = b.assemble(u, uo, 0.01, s_m=1.0, parts=PART_A | PART_M, jprec=jprec, want_J=False)
+        assert torch.equal(dd, db)
+
+
+def test_skewed_record_and_replay_across_groups(cuda):
+    import torch
+
+    mm = CASES["hex_prism"]()
+    disc = _disc(mm, {"upwind": "skewed"})
+    u = torch.from_numpy(M.random_state(mm, 7)).to(cuda)
+    u2 = torch.from_numpy(M.random_state(mm, 8)).to(cuda)
+    disc.set_upwind_decisions(UPWIND_RECORD)
+    J1, d1 = disc.assemble(u, jprec=F64)
+    disc.set_upwind_decisions(UPWIND_REPLAY)
+    J2, d2 = disc.assemble(u, jprec=F64)
+    assert torch.equal(J1, J2) and torch.equal(d1, d2)
+    # replayed decisions at another state differ from fresh ones there
+    J3, _ = disc.assemble(u2, jprec=F64)
+    disc.set_upwind_decisions(UPWIND_FRESH)
+    J4, _ = disc.assemble(u2, jprec=F64)
+    assert not torch.equal(J3, J4)
+
+
+def test_newton_on_a_mixed_mesh(cuda):
+    """A Re = 20 cavity of quadrilaterals and triangles solved by the device
+    Newton/GMRES loop: the converged state's oracle defect on the free rows
+    is below 1e-8 of the initial one."""
+    import torch
+
+    from plugin_navierstokes_amd.solver import NewtonSolver, cavity_dirichlet
+
+    mm = M.mixed_2d(8, 0.1, 4)
+    cfg = {"upwind": "full", "nu": 0.05}
+    disc = _disc(mm, cfg)
+    dofs, vals = cavity_dirichlet(mm)
+    o = np.argsort(dofs)
+    dofs, vals = dofs[o], vals[o]
+    nt = NewtonSolver(disc, dofs, vals, jprec=F64, rtol=1e-10, maxit=40, lin_rtol=1e-6, lin_maxit=400, restart=30)
+    u = torch.zeros(mm.nv, 3, dtype=torch.float64, device=cuda)
+    st = nt.solve(u)
+    print(f"mixed Newton: its={st.iterations} res {st.residuals[0]:.3e} -> {st.residuals[-1]:.3e}", flush=True)
+    assert st.converged
+    rowptr, colidx = disc.pattern()
+    p = oracle_params(2, cfg)
+    free = np.ones(mm.nv * 3, bool)
+    free[dofs] = False
+    u0 = np.zeros((mm.nv, 3))
+    u0.reshape(-1)[dofs] = vals
+    _, d0 = O.assemble_mixed(p, mm.groups, mm.xyz, u0, rowptr, colidx)
+    _, d1 = O.assemble_mixed(p, mm.groups, mm.xyz, u.cpu().numpy(), rowptr, colidx)
+    assert np.linalg.norm(d1[free]) <= 1e-8 * np.linalg.norm(d0[free])
+
+
+def test_refusals(cuda):
+    import torch
+
+    mm = CASES["hex_prism"]()
+    disc = _disc(mm, {"upwind": "full"})
+    u = torch.from_numpy(M.random_state(mm, 2)).to(cuda)
+    disc.assemble(u, jprec=F32)
+    disc.set_stabilization("flow")
+    with pytest.raises(NSFV1Error) as e:
+        disc.assemble(u, jprec=F64)
+    assert e.value.name == "NOT_IMPLEMENTED"
+    disc.set_stabilization("fields")
+    disc.set_upwind("positive")
+    with pytest.raises(NSFV1Error) as e:
+        disc.assemble(u, jprec=F64)
+    assert e.value.name == "NOT_IMPLEMENTED"
+    disc.set_upwind("full")
+    with pytest.raises(NSFV1Error) as e:
+        disc.set_outflow(np.array([0], np.int32), np.array([0], np.int32))
+    assert e.value.name == "NOT_IMPLEMENTED"
+    with pytest.raises(NSFV1Error) as e:
+        disc.set_kinematic_viscosity(lambda x: 0.02 + 0 * x[:, 0])
+    assert e.value.name == "NOT_IMPLEMENTED"
+    with pytest.raises(NSFV1Error) as e:
+        disc.export(u, "velocity", "scvf")
+    assert e.value.name == "NOT_IMPLEMENTED"
+    # a pyramid group is refused like a pyramid mesh
+    with pytest.raises(NSFV1Error) as e:
+        disc.set_mesh(M.MixedMesh([(M.HEX, mm.groups[0][1]), (5, np.zeros((1, 5), np.int32))], mm.xyz, mm.boundary))
+    assert e.value.name == "NOT_IMPLEMENTED"
+    # a group of the other dimension
+    with pytest.raises(NSFV1Error) as e:
+        disc.set_mesh(M.MixedMesh([(M.HEX, mm.groups[0][1]), (M.TRI, np.zeros((1, 3), np.int32))], mm.xyz,
+                                  mm.boundary))
+    assert e.value.name == "ARGUMENT"
+    # the handle still works after a refused mesh: a single-type mesh
+    mesh = M.cavity_3d(2, M.HEX, 0.1, 1)
+    disc.set_mesh(mesh)
+    disc.set_kinematic_viscosity(0.05)
+    disc.assemble(torch.from_numpy(M.random_state(mesh, 2)).to(cuda), jprec=F32)
