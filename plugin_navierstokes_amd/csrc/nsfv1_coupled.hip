@@ -47,6 +47,20 @@ struct CplShared {
     int piv;
 };
 
+// corner c (runtime) of a per-lane coordinate array, by selects over the
+// compile-time corners, so the array stays in registers
+template <int N, int D>
+__device__ inline void pick_corner(const double (&co)[N][D], int c, double (&x)[D])
+{
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) v = k == c ? co[k][i] : v;
+        x[i] = v;
+    }
+}
+
 template <int ET>
 struct CplCfg {
     static constexpr int NIP = Topo<ET>::nip;
@@ -189,7 +203,9 @@ __device__ inline bool upwind_lane(int type, double sign, int ip, const double (
         const int c = dot_nc<D>(n, v) > 0.0 ? fr : to;
 #pragma unroll
         for (int k = 0; k < N; ++k) U[k] = k == c ? 1.0 : 0.0;
-        len = sqrt(dist2_nc<D>(ipg, co[c]));
+        double xc[D];
+        pick_corner(co, c, xc);
+        len = sqrt(dist2_nc<D>(ipg, xc));
         return true;
     }
     if (type == UPW_REGULAR) {
@@ -215,8 +231,8 @@ __device__ inline bool upwind_lane(int type, double sign, int ip, const double (
     }
     if (type == UPW_POSITIVE) {  // upwind.cpp:643-786
         if (!S.has[ip]) {
-            U[fr] = 0.5;
-            U[to] = 0.5;
+#pragma unroll
+            for (int k = 0; k < N; ++k) U[k] = (k == fr || k == to) ? 0.5 : 0.0;
         } else {
             const int sc = sign * S.mflux[ip] > 0 ? fr : to;  // the SCV the SCVF leaves
             double m_in = 0.0, m_out = 0.0;
@@ -270,12 +286,16 @@ __device__ inline bool upwind_lane(int type, double sign, int ip, const double (
         int best = 0;
         for (int j = 0; j < T::side_n[side]; ++j) {
             const int c = T::side[side][j];
-            const double dd = dist2_nc<D>(gi, co[c]);
+            double xc[D];
+            pick_corner(co, c, xc);
+            const double dd = dist2_nc<D>(gi, xc);
             if (dd < mn) { mn = dd; best = c; }
         }
 #pragma unroll
         for (int k = 0; k < N; ++k) U[k] = k == best ? 1.0 : 0.0;
-        len = sqrt(dist2_nc<D>(ipg, co[best]));
+        double xb[D];
+        pick_corner(co, best, xb);
+        len = sqrt(dist2_nc<D>(ipg, xb));
     } else {
 #pragma unroll
         for (int k = 0; k < N; ++k) U[k] = side_has<ET>(side, k) ? shape_at<ET>(k, -1, li[0], li[1], li[2]) : 0.0;
@@ -318,8 +338,11 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
         if (M.nu_elem) nu = M.nu_elem[e];
         double c[D], me[D], ipg[D], nn[3] = {0, 0, 0};
         elem_centre<ET>(co, c);
+        double cfr[D], cto[D];
+        pick_corner(co, fr, cfr);
+        pick_corner(co, to, cto);
 #pragma unroll
-        for (int i = 0; i < D; ++i) me[i] = 0.5 * (co[fr][i] + co[to][i]);
+        for (int i = 0; i < D; ++i) me[i] = 0.5 * (cfr[i] + cto[i]);
         if constexpr (D == 2) {
 #pragma unroll
             for (int i = 0; i < 2; ++i) ipg[i] = 0.5 * (me[i] + c[i]);
@@ -328,12 +351,18 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
         } else {
             const int A = edge_face<ET>(ip, 0), B = edge_face<ET>(ip, 1);
             double fa[3] = {0, 0, 0}, fb[3] = {0, 0, 0}, d1[3], d2[3];
-            for (int j = 0; j < T::side_n[A]; ++j)
 #pragma unroll
-                for (int i = 0; i < 3; ++i) fa[i] += co[T::side[A][j]][i];
-            for (int j = 0; j < T::side_n[B]; ++j)
+            for (int j = 0; j < 4; ++j) {
+                double xa[3], xb[3];
+                pick_corner(co, T::side[A][j], xa);
+                pick_corner(co, T::side[B][j], xb);
+                if (j < T::side_n[A])
 #pragma unroll
-                for (int i = 0; i < 3; ++i) fb[i] += co[T::side[B][j]][i];
+                    for (int i = 0; i < 3; ++i) fa[i] += xa[i];
+                if (j < T::side_n[B])
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) fb[i] += xb[i];
+            }
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 fa[i] /= T::side_n[A];
@@ -349,7 +378,7 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
         {
             double ed[D];
 #pragma unroll
-            for (int i = 0; i < D; ++i) ed[i] = co[to][i] - co[fr][i];
+            for (int i = 0; i < D; ++i) ed[i] = cto[i] - cfr[i];
             const double s = dot_nc<D>(nn, ed);
 #pragma unroll
             for (int i = 0; i < D; ++i) n[i] = s < 0 ? -nn[i] : nn[i];
@@ -503,27 +532,16 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
     }
     if (!act) return;
 
-    // ---- stabilised velocity and shapes of the lane's ip
-    const double rho = P.rho, irho = P.inv_rho;
-    double Sv[D][D][N], Sp[D][N], svel[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d)
-#pragma unroll
-        for (int d2 = 0; d2 < D; ++d2)
-#pragma unroll
-            for (int k = 0; k < N; ++k) Sv[d][d2][k] = 0.0;
+    // ---- stabilised velocity of the lane's ip.  Its shapes (the derivatives
+    //      by each corner's unknowns) are formed corner by corner in the
+    //      Jacobian loop below, from the same expressions.
+    const double rho = P.rho;
+    double svel[D], diag = 1.0;
     if (P.stab_type == STAB_WITHOUT) {  // stabilization.cpp:805-850
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            svel[d] = S.sv[ip][d];
-#pragma unroll
-            for (int k = 0; k < N; ++k) {
-                Sv[d][d][k] = S.phi[ip][k];
-                Sp[d][k] = 0.0;
-            }
-        }
+        for (int d = 0; d < D; ++d) svel[d] = S.sv[ip][d];
     } else if (!matrix) {  // diagonal case: stabilization.cpp:163-242, 486-602
-        double diag = vv;
+        diag = vv;
         if (P.has_old) diag += P.inv_dt;
         if (!stokes) diag += vc;
 #pragma unroll
@@ -547,92 +565,74 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
                     for (int d2 = 0; d2 < D; ++d2)
                         if (d2 != d) sumVel -= S.sv[ip][d2] * S.g[ip][k][d2];
                 rhs += sumVel * ul[k][d];
-                Sv[d][d][k] = sumVel / diag;
                 if (flow)
 #pragma unroll
                     for (int d2 = 0; d2 < D; ++d2) {
                         if (d2 == d) continue;
                         const double sumVel2 = S.sv[ip][d] * S.g[ip][k][d2];
                         rhs += sumVel2 * ul[k][d2];
-                        Sv[d][d2][k] = sumVel2 / diag;
                     }
                 const double sumP = -1.0 * S.g[ip][k][d] / rho;
                 rhs += sumP * ul[k][D];
-                Sp[d][k] = sumP / diag;
             }
             svel[d] = rhs / diag;
         }
     } else {  // matrix case: stabilization.cpp:244-403, 604-772
         const double* iv = &S.aug[ip][NIP];
-        double oldip[NIP][D];
-#pragma unroll
-        for (int j = 0; j < NIP; ++j)
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                double o = 0.0;
-                if (P.has_old)
-#pragma unroll
-                    for (int k = 0; k < N; ++k) o += S.phi[j][k] * uold[(int64_t)M.conn[e * N + k] * NC + d];
-                oldip[j][d] = o;
-            }
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            // corner contributions cv(d2, k, j) of the velocity component d
-            // and cp(k, j), each mapped through row ip of the inverse
+            // corner contributions of the velocity component d and the
+            // pressure to each ip's right-hand side, mapped through row ip of
+            // the inverse
             double f[NIP];
 #pragma unroll
             for (int j = 0; j < NIP; ++j) {
                 f[j] = 0.0;
                 if (!flow) {
                     if (P.has_source) f[j] = P.f[d];
-                    if (P.has_old) f[j] += oldip[j][d] * P.inv_dt;
+                    if (P.has_old) {
+                        double o = 0.0;
+#pragma unroll
+                        for (int k = 0; k < N; ++k) o += S.phi[j][k] * uold[(int64_t)M.conn[e * N + k] * NC + d];
+                        f[j] += o * P.inv_dt;
+                    }
                 }
             }
 #pragma unroll
             for (int k = 0; k < N; ++k) {
-                double xp = 0.0, xv[D];
-#pragma unroll
-                for (int d2 = 0; d2 < D; ++d2) xv[d2] = 0.0;
 #pragma unroll
                 for (int j = 0; j < NIP; ++j) {
                     const double cp = -1.0 * S.g[j][k][d] / rho;
                     double cd = S.vv[j] * S.phi[j][k];
                     cd += S.vc[j] * S.Us[j][k];
-                    double cvo[D];
                     if (flow) {
                         cd += S.vd[j] * (S.Ud[j][k] - S.Us[j][k]);
 #pragma unroll
                         for (int d2 = 0; d2 < D; ++d2) {
                             if (d2 == d) continue;
                             cd -= S.sv[j][d2] * S.g[j][k][d2];
-                            cvo[d2] = S.sv[j][d] * S.g[j][k][d2];
                         }
-                    }
-                    xp += iv[j] * cp;
-                    xv[d] += iv[j] * cd;
-                    if (flow) {
-#pragma unroll
-                        for (int d2 = 0; d2 < D; ++d2)
-                            if (d2 != d) xv[d2] += iv[j] * cvo[d2];
                         // f: corner contributions of every component, then pressure
 #pragma unroll
-                        for (int d2 = 0; d2 < D; ++d2) f[j] = f[j] + ul[k][d2] * (d2 == d ? cd : cvo[d2]);
+                        for (int d2 = 0; d2 < D; ++d2)
+                            f[j] = f[j] + ul[k][d2] * (d2 == d ? cd : S.sv[j][d] * S.g[j][k][d2]);
                         f[j] = f[j] + ul[k][D] * cp;
                     } else {
                         f[j] = f[j] + ul[k][d] * cd;
                         f[j] = f[j] + ul[k][D] * cp;
                     }
                 }
-                Sp[d][k] = xp;
-#pragma unroll
-                for (int d2 = 0; d2 < D; ++d2)
-                    if (flow || d2 == d) Sv[d][d2][k] = xv[d2];
             }
             if (flow)
 #pragma unroll
                 for (int j = 0; j < NIP; ++j) {
                     if (P.has_source) f[j] += P.f[d];
-                    if (P.has_old) f[j] += oldip[j][d] * P.inv_dt;
+                    if (P.has_old) {
+                        double o = 0.0;
+#pragma unroll
+                        for (int k = 0; k < N; ++k) o += S.phi[j][k] * uold[(int64_t)M.conn[e * N + k] * NC + d];
+                        f[j] += o * P.inv_dt;
+                    }
                 }
             double x = 0.0;
 #pragma unroll
@@ -681,7 +681,10 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
     const double* sv = S.sv[ip];
     double w = 1.0;
     if (!stokes && P.peclet) {  // navier_stokes_fv1.cpp:871-892
-        const double Pe = dot_nc<D>(sv, n) / dot_nc<D>(n, n) * sqrt(dist2_nc<D>(co[to], co[fr])) / nu;
+        double cfr[D], cto[D];
+        pick_corner(co, fr, cfr);
+        pick_corner(co, to, cto);
+        const double Pe = dot_nc<D>(sv, n) / dot_nc<D>(n, n) * sqrt(dist2_nc<D>(cto, cfr)) / nu;
         w = Pe * Pe / (5.0 + Pe * Pe);
 #pragma unroll
         for (int d = 0; d < D; ++d) upv[d] = w * upv[d] + (1.0 - w) * sv[d];
@@ -724,7 +727,78 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
 
     // ---- Jacobian block of the `from` row, one column block per corner:
     //      navier_stokes_fv1.cpp:317-593
+#pragma unroll
     for (int sh = 0; sh < N; ++sh) {
+        // stabilised-velocity shapes of corner sh: Sv[d][d2] = d svel_d /
+        // d u_{sh,d2}, Sp[d] = d svel_d / d p_sh
+        double Sv[D][D], Sp[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            Sp[d] = 0.0;
+#pragma unroll
+            for (int d2 = 0; d2 < D; ++d2) Sv[d][d2] = 0.0;
+        }
+        if (P.stab_type == STAB_WITHOUT) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) Sv[d][d] = S.phi[ip][sh];
+        } else if (!matrix) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                double sumVel = vv * S.phi[ip][sh];
+                if (!stokes) {
+                    sumVel += vc * Us[sh];
+                    if (flow) sumVel += vd * (Ud[sh] - Us[sh]);
+                }
+                if (flow)
+#pragma unroll
+                    for (int d2 = 0; d2 < D; ++d2)
+                        if (d2 != d) sumVel -= S.sv[ip][d2] * S.g[ip][sh][d2];
+                Sv[d][d] = sumVel / diag;
+                if (flow)
+#pragma unroll
+                    for (int d2 = 0; d2 < D; ++d2) {
+                        if (d2 == d) continue;
+                        const double sumVel2 = S.sv[ip][d] * S.g[ip][sh][d2];
+                        Sv[d][d2] = sumVel2 / diag;
+                    }
+                const double sumP = -1.0 * S.g[ip][sh][d] / rho;
+                Sp[d] = sumP / diag;
+            }
+        } else {
+            const double* iv = &S.aug[ip][NIP];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                double xp = 0.0, xv[D];
+#pragma unroll
+                for (int d2 = 0; d2 < D; ++d2) xv[d2] = 0.0;
+#pragma unroll
+                for (int j = 0; j < NIP; ++j) {
+                    const double cp = -1.0 * S.g[j][sh][d] / rho;
+                    double cd = S.vv[j] * S.phi[j][sh];
+                    cd += S.vc[j] * S.Us[j][sh];
+                    double cvo[D];
+                    if (flow) {
+                        cd += S.vd[j] * (S.Ud[j][sh] - S.Us[j][sh]);
+#pragma unroll
+                        for (int d2 = 0; d2 < D; ++d2) {
+                            if (d2 == d) continue;
+                            cd -= S.sv[j][d2] * S.g[j][sh][d2];
+                            cvo[d2] = S.sv[j][d] * S.g[j][sh][d2];
+                        }
+                    }
+                    xp += iv[j] * cp;
+                    xv[d] += iv[j] * cd;
+                    if (flow)
+#pragma unroll
+                        for (int d2 = 0; d2 < D; ++d2)
+                            if (d2 != d) xv[d2] += iv[j] * cvo[d2];
+                }
+                Sp[d] = xp;
+#pragma unroll
+                for (int d2 = 0; d2 < D; ++d2)
+                    if (flow || d2 == d) Sv[d][d2] = xv[d2];
+            }
+        }
         double Jb[NC][NC];
 #pragma unroll
         for (int a = 0; a < NC; ++a)
@@ -747,11 +821,11 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
                 for (int d1 = 0; d1 < D; ++d1) {
                     if (conn) {
 #pragma unroll
-                        for (int d2 = 0; d2 < D; ++d2) Jb[d1][d2] += prod * w * Sv[d1][d2][sh];
+                        for (int d2 = 0; d2 < D; ++d2) Jb[d1][d2] += prod * w * Sv[d1][d2];
                     } else {
-                        Jb[d1][d1] += prod * w * Sv[d1][d1][sh];
+                        Jb[d1][d1] += prod * w * Sv[d1][d1];
                     }
-                    Jb[d1][D] += prod * w * Sp[d1][sh];
+                    Jb[d1][D] += prod * w * Sp[d1];
                 }
             } else {
                 double cv = Uc[sh];
@@ -776,16 +850,16 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
                             double pv = 0.0;
                             if (conn) {
 #pragma unroll
-                                for (int k = 0; k < D; ++k) pv += w * Sv[k][d2][sh] * n[k];
+                                for (int k = 0; k < D; ++k) pv += w * Sv[k][d2] * n[k];
                             } else {
-                                pv = Sv[d1][d1][sh] * n[d1];
+                                pv = Sv[d1][d1] * n[d1];
                             }
                             pv *= P.exact * rho;
                             Jb[d1][d2] += pv * upv[d1];
                         }
                         double pp = 0.0;
 #pragma unroll
-                        for (int k = 0; k < D; ++k) pp += Sp[k][sh] * n[k];
+                        for (int k = 0; k < D; ++k) pp += Sp[k] * n[k];
                         pp *= P.exact * rho;
                         Jb[d1][D] += pp * upv[d1];
                     } else {
@@ -804,16 +878,16 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
             double cv = 0.0;
             if (conn) {
 #pragma unroll
-                for (int d2 = 0; d2 < D; ++d2) cv += Sv[d2][d1][sh] * n[d2] * rho;
+                for (int d2 = 0; d2 < D; ++d2) cv += Sv[d2][d1] * n[d2] * rho;
             } else {
-                cv = Sv[d1][d1][sh] * n[d1] * rho;
+                cv = Sv[d1][d1] * n[d1] * rho;
             }
             Jb[D][d1] += cv;
         }
         {
             double cpp = 0.0;
 #pragma unroll
-            for (int d1 = 0; d1 < D; ++d1) cpp += Sp[d1][sh] * n[d1] * rho;
+            for (int d1 = 0; d1 < D; ++d1) cpp += Sp[d1] * n[d1] * rho;
             Jb[D][D] += cpp;
         }
         if (recJ) {
