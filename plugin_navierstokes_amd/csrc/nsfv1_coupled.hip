@@ -765,38 +765,56 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
                 Sp[d] = sumP / diag;
             }
         } else {
+            // every accumulator takes its terms in ascending j, as in the
+            // stabilised velocity above; the ip data of j is read once for
+            // all components
             const double* iv = &S.aug[ip][NIP];
+            double xp[D], xv[D][D];
 #pragma unroll
             for (int d = 0; d < D; ++d) {
-                double xp = 0.0, xv[D];
+                xp[d] = 0.0;
 #pragma unroll
-                for (int d2 = 0; d2 < D; ++d2) xv[d2] = 0.0;
+                for (int d2 = 0; d2 < D; ++d2) xv[d][d2] = 0.0;
+            }
 #pragma unroll
-                for (int j = 0; j < NIP; ++j) {
-                    const double cp = -1.0 * S.g[j][sh][d] / rho;
-                    double cd = S.vv[j] * S.phi[j][sh];
-                    cd += S.vc[j] * S.Us[j][sh];
+            for (int j = 0; j < NIP; ++j) {
+                const double ivj = iv[j];
+                double gj[D], svj[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    gj[d] = S.g[j][sh][d];
+                    svj[d] = S.sv[j][d];
+                }
+                double cd0 = S.vv[j] * S.phi[j][sh];
+                cd0 += S.vc[j] * S.Us[j][sh];
+                if (flow) cd0 += S.vd[j] * (S.Ud[j][sh] - S.Us[j][sh]);
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const double cp = -1.0 * gj[d] / rho;
+                    double cd = cd0;
                     double cvo[D];
                     if (flow) {
-                        cd += S.vd[j] * (S.Ud[j][sh] - S.Us[j][sh]);
 #pragma unroll
                         for (int d2 = 0; d2 < D; ++d2) {
                             if (d2 == d) continue;
-                            cd -= S.sv[j][d2] * S.g[j][sh][d2];
-                            cvo[d2] = S.sv[j][d] * S.g[j][sh][d2];
+                            cd -= svj[d2] * gj[d2];
+                            cvo[d2] = svj[d] * gj[d2];
                         }
                     }
-                    xp += iv[j] * cp;
-                    xv[d] += iv[j] * cd;
+                    xp[d] += ivj * cp;
+                    xv[d][d] += ivj * cd;
                     if (flow)
 #pragma unroll
                         for (int d2 = 0; d2 < D; ++d2)
-                            if (d2 != d) xv[d2] += iv[j] * cvo[d2];
+                            if (d2 != d) xv[d][d2] += ivj * cvo[d2];
                 }
-                Sp[d] = xp;
+            }
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                Sp[d] = xp[d];
 #pragma unroll
                 for (int d2 = 0; d2 < D; ++d2)
-                    if (flow || d2 == d) Sv[d][d2] = xv[d2];
+                    if (flow || d2 == d) Sv[d][d2] = xv[d][d2];
             }
         }
         double Jb[NC][NC];
