@@ -578,65 +578,78 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
             svel[d] = rhs / diag;
         }
     } else {  // matrix case: stabilization.cpp:244-403, 604-772
+        // corner contributions of every velocity component and the pressure
+        // to each ip's right-hand side f[d][j], then mapped through row ip of
+        // the inverse; each f[d][j] takes its corners in ascending order, and
+        // the ip data of (j, k) is read once for all components
         const double* iv = &S.aug[ip][NIP];
+        double f[D][NIP];
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            // corner contributions of the velocity component d and the
-            // pressure to each ip's right-hand side, mapped through row ip of
-            // the inverse
-            double f[NIP];
+        for (int d = 0; d < D; ++d)
 #pragma unroll
             for (int j = 0; j < NIP; ++j) {
-                f[j] = 0.0;
+                f[d][j] = 0.0;
                 if (!flow) {
-                    if (P.has_source) f[j] = P.f[d];
+                    if (P.has_source) f[d][j] = P.f[d];
                     if (P.has_old) {
                         double o = 0.0;
 #pragma unroll
                         for (int k = 0; k < N; ++k) o += S.phi[j][k] * uold[(int64_t)M.conn[e * N + k] * NC + d];
-                        f[j] += o * P.inv_dt;
+                        f[d][j] += o * P.inv_dt;
                     }
                 }
             }
 #pragma unroll
-            for (int k = 0; k < N; ++k) {
+        for (int k = 0; k < N; ++k) {
 #pragma unroll
-                for (int j = 0; j < NIP; ++j) {
-                    const double cp = -1.0 * S.g[j][k][d] / rho;
-                    double cd = S.vv[j] * S.phi[j][k];
-                    cd += S.vc[j] * S.Us[j][k];
+            for (int j = 0; j < NIP; ++j) {
+                double gj[D], svj[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    gj[d] = S.g[j][k][d];
+                    svj[d] = S.sv[j][d];
+                }
+                double cd0 = S.vv[j] * S.phi[j][k];
+                cd0 += S.vc[j] * S.Us[j][k];
+                if (flow) cd0 += S.vd[j] * (S.Ud[j][k] - S.Us[j][k]);
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const double cp = -1.0 * gj[d] / rho;
+                    double cd = cd0;
                     if (flow) {
-                        cd += S.vd[j] * (S.Ud[j][k] - S.Us[j][k]);
 #pragma unroll
                         for (int d2 = 0; d2 < D; ++d2) {
                             if (d2 == d) continue;
-                            cd -= S.sv[j][d2] * S.g[j][k][d2];
+                            cd -= svj[d2] * gj[d2];
                         }
                         // f: corner contributions of every component, then pressure
 #pragma unroll
                         for (int d2 = 0; d2 < D; ++d2)
-                            f[j] = f[j] + ul[k][d2] * (d2 == d ? cd : S.sv[j][d] * S.g[j][k][d2]);
-                        f[j] = f[j] + ul[k][D] * cp;
+                            f[d][j] = f[d][j] + ul[k][d2] * (d2 == d ? cd : svj[d] * gj[d2]);
+                        f[d][j] = f[d][j] + ul[k][D] * cp;
                     } else {
-                        f[j] = f[j] + ul[k][d] * cd;
-                        f[j] = f[j] + ul[k][D] * cp;
+                        f[d][j] = f[d][j] + ul[k][d] * cd;
+                        f[d][j] = f[d][j] + ul[k][D] * cp;
                     }
                 }
             }
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
             if (flow)
 #pragma unroll
                 for (int j = 0; j < NIP; ++j) {
-                    if (P.has_source) f[j] += P.f[d];
+                    if (P.has_source) f[d][j] += P.f[d];
                     if (P.has_old) {
                         double o = 0.0;
 #pragma unroll
                         for (int k = 0; k < N; ++k) o += S.phi[j][k] * uold[(int64_t)M.conn[e * N + k] * NC + d];
-                        f[j] += o * P.inv_dt;
+                        f[d][j] += o * P.inv_dt;
                     }
                 }
             double x = 0.0;
 #pragma unroll
-            for (int j = 0; j < NIP; ++j) x += iv[j] * f[j];
+            for (int j = 0; j < NIP; ++j) x += iv[j] * f[d][j];
             svel[d] = x;
         }
     }
