@@ -118,8 +118,12 @@ int nsfv1_set_exact_jacobian(nsfv1_handle* h, double factor);
      f_scvf[ne][nip][dim]  at the SCVF ips: the stabilisation's source.
    NULL clears (the source needs both arrays).  A new mesh clears them.
    These run on the generic path (nsfv1_set_assembly_path 1 behaviour);
-   with the FLOW stabilization, the positive or regular upwind, boundary
-   discs or multigrid they are refused with NSFV1_ERR_NOT_IMPLEMENTED. */
+   with the FLOW stabilization, the positive or regular upwind or boundary
+   discs they are refused with NSFV1_ERR_NOT_IMPLEMENTED.  With multigrid
+   every coarse handle needs its own imports of the same quantities (the
+   function evaluated at its ips); nsfv1_mg_setup refuses a hierarchy that
+   lacks them with NSFV1_ERR_ARGUMENT and drops coarse imports the fine
+   level no longer has. */
 int nsfv1_set_kinematic_viscosity_ip(nsfv1_handle* h, const double* nu, int64_t n);
 int nsfv1_set_source_ip(nsfv1_handle* h, const double* f_scv, int64_t n_scv, const double* f_scvf, int64_t n_scvf);
 /* Global positions of the integration points (host array): kind 0 = SCVF
@@ -421,7 +425,7 @@ int nsfv1_mg_set_hierarchy(nsfv1_handle* h, int nlev, nsfv1_handle* const* coars
 /* Re-assembles every coarse level's Jacobian at the injected state (same
    time-series, scalings and parts as the fine Jacobian), applies the coarse
    Dirichlet rows, factors the point blocks and inverts the coarsest level on
-   the device (Gauss-Jordan, one launch per pivot).  Call after each fine
+   the device (Gauss-Jordan, four pivots per launch).  Call after each fine
    assembly whose Jacobian the next nsfv1_linear_solve uses.  Synchronises
    the stream once, to report a singular coarsest level. */
 int nsfv1_mg_setup(nsfv1_handle* h, int jprec, const double* u, const double* u_old, double dt, double s_a,

@@ -225,6 +225,9 @@ class NavierStokesFV1:
             x = self.ip_positions("scvf")
             vals = np.ascontiguousarray(np.asarray(fn(x.reshape(-1, self.dim)), dtype=np.float64).reshape(x.shape[:2]))
         _check(_lib.nsfv1_set_kinematic_viscosity_ip(self._h, vals.ctypes.data_as(C.POINTER(C.c_double)), vals.size))
+        if callable(fn):
+            for c in getattr(self, "_mg_levels", []):
+                c.set_kinematic_viscosity(fn)
 
     def _apply_f_fn(self):
         fn = getattr(self, "_f_fn", None)
@@ -238,6 +241,9 @@ class NavierStokesFV1:
             b = np.ascontiguousarray(np.asarray(fn(xf.reshape(-1, self.dim)), dtype=np.float64).reshape(xf.shape))
         dp = C.POINTER(C.c_double)
         _check(_lib.nsfv1_set_source_ip(self._h, a.ctypes.data_as(dp), a.size, b.ctypes.data_as(dp), b.size))
+        if callable(fn):
+            for c in getattr(self, "_mg_levels", []):
+                c.set_source(fn)
 
     def set_exact_jacobian(self, factor):
         _check(_lib.nsfv1_set_exact_jacobian(self._h, (1.0 if factor else 0.0) if isinstance(factor, bool) else float(factor)))
@@ -303,6 +309,7 @@ class NavierStokesFV1:
         self.ne = conn.shape[0]
         self.dim = 2 if int(et) in (0, 1) else 3
         self._et = int(et)
+        self._mg_levels = []  # a new mesh drops the multigrid hierarchy
         # functions of position are imports: evaluated again on the new mesh
         if callable(getattr(self, "_nu_fn", None)):
             self._apply_nu_fn()
@@ -324,6 +331,7 @@ class NavierStokesFV1:
         self.ne = sum(c.shape[0] for c in conns)
         self.dim = xyz.shape[1]
         self._et = int(mesh.groups[0][0])
+        self._mg_levels = []  # a new mesh drops the multigrid hierarchy
         # function imports are refused on mixed meshes (raises)
         if callable(getattr(self, "_nu_fn", None)):
             self._apply_nu_fn()
@@ -540,6 +548,18 @@ class NavierStokesFV1:
         ij = arr(injections, np.int64)
         _check(_lib.nsfv1_mg_set_hierarchy(self._h, nl, Hs, rp, ci, w, ij, int(pre), int(post), float(omega)))
         self._mg_levels = list(coarse)  # the library borrows the coarse handles
+        # functions of position are imported on every level
+        for c in self._mg_levels:
+            c._inherit_functions(self)
+
+    def _inherit_functions(self, fine):
+        """Evaluate the fine level's viscosity / source functions on this
+        (coarse) level's integration points."""
+        nu_fn, f_fn = getattr(fine, "_nu_fn", None), getattr(fine, "_f_fn", None)
+        if callable(nu_fn):
+            self.set_kinematic_viscosity(nu_fn)
+        if callable(f_fn):
+            self.set_source(f_fn)
 
     def mg_setup(self, jprec, u, u_old=None, dt=0.0, s_a=1.0, s_m=0.0, parts=PART_A, stream=None):
         _check(_lib.nsfv1_mg_setup(self._h, int(jprec), _ptr(u), _ptr(u_old), float(dt), float(s_a), float(s_m),

@@ -1621,6 +1621,13 @@ static void mg_sync_settings(nsfv1_handle* h)
         c->difflen = h->difflen;
         c->conv_upwind = h->conv_upwind;
         c->pac = h->pac;
+        // imports at the integration points live per level (each level has
+        // its own elements); a fine level without them clears the coarse ones
+        if (!h->nu_ip.p) c->nu_ip.release();
+        if (!h->f_scv.p) {
+            c->f_scv.release();
+            c->f_scvf.release();
+        }
     }
 }
 
@@ -1713,10 +1720,17 @@ int nsfv1_mg_setup(nsfv1_handle* h, int jprec, const double* u, const double* u_
     if (h->mg.empty()) return fail(NSFV1_ERR_ARGUMENT, "no multigrid hierarchy set");
     if (h->nu_elem.p)
         return fail(NSFV1_ERR_NOT_IMPLEMENTED, "multigrid with per-instance viscosity (batched instances) is not provided");
-    if (h->ip_imports())
-        return fail(NSFV1_ERR_NOT_IMPLEMENTED, "multigrid with imports at the integration points is not provided");
-    // the coarse levels take the fine level's current configuration
+    // the coarse levels take the fine level's current configuration; imports
+    // at the integration points must be given on every coarse level too
+    // (the caller evaluates its function there), for the same quantities
     mg_sync_settings(h);
+    for (auto& Lp : h->mg) {
+        const nsfv1_handle* c = Lp->h;
+        if ((h->nu_ip.p && !c->nu_ip.p) || (h->f_scv.p && !c->f_scv.p))
+            return fail(NSFV1_ERR_ARGUMENT, "multigrid with imports at the integration points: every coarse level "
+                                            "needs its own imports of the same quantities (nsfv1_set_kinematic_"
+                                            "viscosity_ip / nsfv1_set_source_ip on the coarse handles)");
+    }
     if (!u) return fail(NSFV1_ERR_ARGUMENT, "u is NULL");
     if (jprec != NSFV1_F32 && jprec != NSFV1_F64) return fail(NSFV1_ERR_ARGUMENT, "jprec must be 32 or 64");
     hipStream_t s = (hipStream_t)stream;

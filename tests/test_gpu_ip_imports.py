@@ -161,3 +161,58 @@ def test_refusals_and_clearing(cuda):
     u2 = torch.from_numpy(M.random_state(mesh2, 2)).to(cuda)
     disc.assemble(u2, jprec=F32)
     assert disc.assembly_path_used() == 1
+
+
+def test_multigrid_with_function_imports(cuda):
+    """Newton with the multigrid preconditioner on a cavity with a
+    function-valued viscosity and source: the facade evaluates the functions
+    on every coarse level too.  The converged state equals the block-Jacobi
+    one (the same discrete system) and multigrid needs fewer Krylov
+    iterations; an array import given on the fine level only is refused by
+    the multigrid set-up."""
+    import torch
+
+    from plugin_navierstokes_amd.solver import NewtonSolver, build_multigrid, cavity_dirichlet
+
+    def dirichlet(m):
+        dofs, vals = cavity_dirichlet(m)
+        o = np.argsort(dofs)
+        return dofs[o], vals[o]
+
+    mesh = M.cavity_2d(16, M.QUAD, 0.1, 3)
+    dofs, vals = dirichlet(mesh)
+    out = {}
+    for mg in (False, True):
+        print(f"case multigrid={mg}", flush=True)
+        disc = _disc(mesh, {"upwind": "full"})
+        disc.set_kinematic_viscosity(NU)
+        disc.set_source(F)
+        if mg:
+            build_multigrid(disc, mesh, dirichlet, max_coarse_dofs=300)
+        nt = NewtonSolver(disc, dofs, vals, jprec=F64, rtol=1e-11, maxit=40, lin_rtol=1e-6, lin_maxit=600,
+                          restart=30, multigrid=mg)
+        u = torch.zeros(mesh.nv, 3, dtype=torch.float64, device=cuda)
+        st = nt.solve(u)
+        print(f"multigrid={mg}: newton {st.iterations}, krylov {st.linear_iterations}", flush=True)
+        assert st.converged
+        out[mg] = (u.cpu().numpy(), st.linear_iterations)
+    scale = np.abs(out[False][0]).max()
+    assert np.abs(out[True][0] - out[False][0]).max() <= 1e-7 * scale
+    assert out[True][1] < out[False][1]
+    # an array import is not evaluated on the coarse levels: refused
+    disc = _disc(mesh, {"upwind": "full"})
+    build_multigrid(disc, mesh, dirichlet, max_coarse_dofs=300)
+    disc.set_kinematic_viscosity(np.full((mesh.ne, 4), 0.02))
+    u = torch.zeros(mesh.nv, 3, dtype=torch.float64, device=cuda)
+    with pytest.raises(NSFV1Error) as e:
+        disc.mg_setup(F64, u)
+    assert e.value.name == "ARGUMENT"
+    # back to a constant: the set-up runs
+    disc.set_kinematic_viscosity(0.02)
+    disc.mg_setup(F64, u)
+    # a function on the fine level reaches the coarse levels; a constant
+    # afterwards drops their imports again at the next set-up
+    disc.set_kinematic_viscosity(NU)
+    disc.mg_setup(F64, u)
+    disc.set_kinematic_viscosity(0.02)
+    disc.mg_setup(F64, u)
