@@ -117,9 +117,10 @@ int nsfv1_set_exact_jacobian(nsfv1_handle* h, double factor);
      f_scv [ne][nsh][dim]  at the SCV ips (the corners): the rhs part;
      f_scvf[ne][nip][dim]  at the SCVF ips: the stabilisation's source.
    NULL clears (the source needs both arrays).  A new mesh clears them.
-   These run on the generic path (nsfv1_set_assembly_path 1 behaviour);
-   with the FLOW stabilization, the positive or regular upwind or boundary
-   discs they are refused with NSFV1_ERR_NOT_IMPLEMENTED.  With multigrid
+   These run on the generic path (nsfv1_set_assembly_path 1 behaviour), or
+   on the coupled path with the FLOW stabilization and the positive or
+   regular upwind; with boundary discs they are refused with
+   NSFV1_ERR_NOT_IMPLEMENTED.  With multigrid
    every coarse handle needs its own imports of the same quantities (the
    function evaluated at its ips); nsfv1_mg_setup refuses a hierarchy that
    lacks them with NSFV1_ERR_ARGUMENT and drops coarse imports the fine

@@ -1069,9 +1069,6 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
                 if (h->bnd[kind].nbn)
                     return fail(NSFV1_ERR_NOT_IMPLEMENTED,
                                 "boundary discs with imports at the integration points are not provided");
-            if (coupled_config(h))
-                return fail(NSFV1_ERR_NOT_IMPLEMENTED, "imports at the integration points are not provided with the "
-                                                       "FLOW stabilization or the positive/regular upwinds");
         }
     }
     Plan& P = jprec == NSFV1_F64 ? h->plan64 : h->plan32;
@@ -1466,9 +1463,6 @@ int nsfv1_export(nsfv1_handle* h, int what, int at, const double* local_ips, int
         m.f_scv = h->f_scv.as<double>();
         m.f_scvf = h->f_scvf.as<double>();
         hipError_t e;
-        if (coupled_config(h) && h->ip_imports())
-            return fail(NSFV1_ERR_NOT_IMPLEMENTED, "imports at the integration points are not provided with the "
-                                                   "FLOW stabilization or the positive/regular upwinds");
         if (coupled_config(h)) {
             k.conv_upwind = h->conv_upwind;
             k.pac = h->pac;

@@ -336,6 +336,7 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
             for (int a = 0; a < NC; ++a) ul[k][a] = u[w * NC + a];
         }
         if (M.nu_elem) nu = M.nu_elem[e];
+        if (M.nu_ip) nu = M.nu_ip[e * NIP + ip];  // function-valued viscosity at the lane's ip
         double c[D], me[D], ipg[D], nn[3] = {0, 0, 0};
         elem_centre<ET>(co, c);
         double cfr[D], cto[D];
@@ -546,7 +547,7 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
         if (!stokes) diag += vc;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            double rhs = P.has_source ? P.f[d] : 0.0;
+            double rhs = P.has_source ? (M.f_scvf ? M.f_scvf[(e * NIP + ip) * D + d] : P.f[d]) : 0.0;
             if (P.has_old) {
                 double o = 0.0;
 #pragma unroll
@@ -590,7 +591,7 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
             for (int j = 0; j < NIP; ++j) {
                 f[d][j] = 0.0;
                 if (!flow) {
-                    if (P.has_source) f[d][j] = P.f[d];
+                    if (P.has_source) f[d][j] = M.f_scvf ? M.f_scvf[(e * NIP + j) * D + d] : P.f[d];
                     if (P.has_old) {
                         double o = 0.0;
 #pragma unroll
@@ -639,7 +640,7 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
             if (flow)
 #pragma unroll
                 for (int j = 0; j < NIP; ++j) {
-                    if (P.has_source) f[d][j] += P.f[d];
+                    if (P.has_source) f[d][j] += M.f_scvf ? M.f_scvf[(e * NIP + j) * D + d] : P.f[d];
                     if (P.has_old) {
                         double o = 0.0;
 #pragma unroll
@@ -1025,7 +1026,10 @@ coupled_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const JT* __r
             }
             if ((P.parts & 4) && P.has_source)
 #pragma unroll
-                for (int d = 0; d < D; ++d) dacc[d] += P.s_f * (P.f[d] * volk * P.rho);
+                for (int d = 0; d < D; ++d) {
+                    const double fk = M.f_scv ? M.f_scv[((int64_t)e * N + kv) * D + d] : P.f[d];
+                    dacc[d] += P.s_f * (fk * volk * P.rho);
+                }
         }
         const uint64_t pw = M.inc_pos[inc];
 #pragma unroll

@@ -7,7 +7,9 @@ function is evaluated at the integration points the library reports
 imports (every element type, three upwinds, FP64 and FP32 Jacobians, A + M
 + rhs parts, time-dependent stabilisation); the positions against the
 oracle's FV1 geometry; constant functions against the constants (bitwise,
-same kernel); the refused combinations and clearing."""
+same kernel); the coupled path (positive upwind, FLOW) with the same
+imports; multigrid with the functions on every level; the refused
+combinations and clearing."""
 from __future__ import annotations
 
 import numpy as np
@@ -44,16 +46,25 @@ def _oracle_ips(et, mesh):
     return np.stack([O.geometry(et, mesh.xyz[mesh.conn[e]])["ipg"] for e in range(mesh.ne)])
 
 
+CFGS = [{"upwind": "full"}, {"upwind": "skewed"}, {"upwind": "lps"},
+        # the coupled path: the matrix case of the positive upwind, FLOW
+        {"upwind": "positive"}, {"upwind": "lps", "stab": "flow"}, {"upwind": "positive", "stab": "flow"}]
+
+
+def _cid(c):
+    return "-".join(f"{k}={v}" for k, v in c.items())
+
+
 @pytest.mark.parametrize("jprec", [F64, F32], ids=["fp64", "fp32"])
-@pytest.mark.parametrize("up", ["full", "skewed", "lps"])
+@pytest.mark.parametrize("cfg", CFGS, ids=_cid)
 @pytest.mark.parametrize("et", ETS, ids=lambda e: NAMES[e])
-def test_function_valued_viscosity_and_source(cuda, et, up, jprec):
+def test_function_valued_viscosity_and_source(cuda, et, cfg, jprec):
     import torch
 
-    print(f"case et={NAMES[et]} upwind={up} jprec={jprec}", flush=True)
+    print(f"case et={NAMES[et]} cfg={cfg} jprec={jprec}", flush=True)
     mesh = small_mesh(et)
     dim, ne = mesh.dim, mesh.ne
-    cfg = {"upwind": up}
+    coupled = cfg.get("stab") == "flow" or cfg["upwind"] == "positive"
     disc = _disc(mesh, cfg)
     disc.set_kinematic_viscosity(NU)
     disc.set_source(F)
@@ -68,7 +79,7 @@ def test_function_valued_viscosity_and_source(cuda, et, up, jprec):
     parts = PART_A | PART_M | PART_RHS
     J, d = disc.assemble(torch.from_numpy(u).to(cuda), torch.from_numpy(uold).to(cuda), dt, s_a=s[0], s_m=s[1],
                          s_f=s[2], parts=parts, jprec=jprec)
-    assert disc.assembly_path_used() == 1
+    assert disc.assembly_path_used() == (4 if coupled else 1)
     J, d = J.cpu().numpy(), d.cpu().numpy()
     p = oracle_params(dim, cfg)
     p.source = (0.0,) * dim
@@ -129,16 +140,14 @@ def test_refusals_and_clearing(cuda):
     disc.set_source(F)
     disc.assemble(u, jprec=F32)
     assert disc.assembly_path_used() == 1
-    # FLOW and the positive upwind run on the coupled path: refused
+    # FLOW and the positive upwind take the imports on the coupled path
     disc.set_stabilization("flow")
-    with pytest.raises(NSFV1Error) as e:
-        disc.assemble(u, jprec=F64)
-    assert e.value.name == "NOT_IMPLEMENTED"
+    disc.assemble(u, jprec=F64)
+    assert disc.assembly_path_used() == 4
     disc.set_stabilization("fields")
     disc.set_upwind("positive")
-    with pytest.raises(NSFV1Error) as e:
-        disc.assemble(u, jprec=F64)
-    assert e.value.name == "NOT_IMPLEMENTED"
+    disc.assemble(u, jprec=F64)
+    assert disc.assembly_path_used() == 4
     disc.set_upwind("full")
     # boundary discs: refused
     belem, bside = M.boundary_sides(mesh, mesh.boundary["lid"])
