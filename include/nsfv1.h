@@ -186,11 +186,12 @@ int nsfv1_set_mesh(nsfv1_handle* h, int elem_type, const double* xyz, int64_t nv
    element loop over a subset that holds several element types, which calls
    the disc once per type (prep_elem_loop, navier_stokes_fv1.cpp:136-182,
    for each type registered at :1515-1551).  One CSR pattern covers all
-   groups; the assembly runs the generic kernel once per group on the
-   stream, the first group writing every row and the others adding theirs.
-   The FLOW stabilization, the positive and regular upwinds, boundary
-   discs, data exports, integration-point imports and batched instances
-   refuse mixed meshes with NSFV1_ERR_NOT_IMPLEMENTED. */
+   groups; the assembly runs the generic kernel (the coupled path for the
+   FLOW stabilization and the positive/regular upwinds) once per group on
+   the stream, the first group writing every row and the others adding
+   theirs.  Boundary discs, data exports, integration-point imports and
+   batched instances refuse mixed meshes with NSFV1_ERR_NOT_IMPLEMENTED, as
+   does the coupled path for a mesh with prisms. */
 int nsfv1_set_mesh_mixed(nsfv1_handle* h, const double* xyz, int64_t nv, int ngroups, const int* elem_type,
                          const int32_t* const* conn, const int64_t* ne);
 

@@ -1061,9 +1061,11 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
         if (h->et == ET_PRISM && coupled_config(h))
             return fail(NSFV1_ERR_NOT_IMPLEMENTED, "prisms are not provided with the FLOW stabilization or the "
                                                    "positive/regular upwinds");
-        if (h->mixed() && coupled_config(h))
-            return fail(NSFV1_ERR_NOT_IMPLEMENTED, "mixed-element meshes are not provided with the FLOW stabilization "
-                                                   "or the positive/regular upwinds");
+        if (coupled_config(h))
+            for (auto& gp : h->groups)
+                if (gp->et == ET_PRISM)
+                    return fail(NSFV1_ERR_NOT_IMPLEMENTED, "prisms are not provided with the FLOW stabilization or the "
+                                                           "positive/regular upwinds");
         if (h->ip_imports()) {
             for (int kind = 0; kind < BND_KINDS; ++kind)
                 if (h->bnd[kind].nbn)
@@ -1145,11 +1147,38 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
         ((uintptr_t)jvals & 15) == 0 && ((uintptr_t)defect & 15) == 0)
         mfast = k.conv_upwind;
     if (h->mixed()) {
-        // a mixed mesh: every element group on the fused kernel, in group
-        // order on the one stream; the first group assembles every row from
-        // zero, each further group adds its elements to the rows it touches
-        e = launch_assemble(h->et, jprec, m, k, u, u_old, jvals, defect, h->errflag.as<int>(), P.nwg, P.lds,
-                            (hipStream_t)stream);
+        // a mixed mesh: every element group on the fused kernel (or the
+        // coupled path), in group order on the one stream; the first group
+        // assembles every row from zero, each further group adds its
+        // elements to the rows it touches.  The coupled path's record
+        // buffer is reused group after group (stream order).
+        const bool cpl = coupled_config(h);
+        void* crecs = nullptr;
+        if (cpl) {
+            if (h->upw_mode != NSFV1_UPWIND_FRESH)
+                return fail(NSFV1_ERR_NOT_IMPLEMENTED, "recorded upwind decisions are not available with the FLOW "
+                                                       "stabilization or the positive upwind");
+            size_t need = (size_t)coupled_record_bytes(h->et, jprec, h->ne);
+            for (auto& gp : h->groups) need = std::max(need, (size_t)coupled_record_bytes(gp->et, jprec, gp->ne));
+            if (scratch) {
+                if (need > NOALLOC_SCRATCH_BYTES)
+                    return fail(NSFV1_ERR_ARGUMENT, "the coupled path needs " + std::to_string(need) +
+                                                        " bytes of SCVF records for this mesh; the no-allocation entry holds at most " +
+                                                        std::to_string(NOALLOC_SCRATCH_BYTES));
+                crecs = scratch;
+            } else {
+                if (h->cpl_recs.n < need && h->cpl_recs.alloc(need) != hipSuccess)
+                    return fail(NSFV1_ERR_DEVICE, "coupled record buffer");
+                crecs = h->cpl_recs.p;
+            }
+        }
+        auto launch = [&](int et, const KMeshPtrs& mm, const KParams& kk, const Plan& PP, int64_t ne) {
+            return cpl ? launch_assemble_coupled(et, jprec, mm, kk, u, u_old, jvals, defect, h->errflag.as<int>(), PP.nwg,
+                                                 PP.lds, ne, crecs, nullptr, (hipStream_t)stream)
+                       : launch_assemble(et, jprec, mm, kk, u, u_old, jvals, defect, h->errflag.as<int>(), PP.nwg, PP.lds,
+                                         (hipStream_t)stream);
+        };
+        e = launch(h->et, m, k, P, h->ne);
         for (size_t g = 0; e == hipSuccess && g < h->groups.size(); ++g) {
             ElemGroup& G = *h->groups[g];
             const Plan& PG = jprec == NSFV1_F64 ? G.plan64 : G.plan32;
@@ -1173,10 +1202,9 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
             mg.wg_a = PG.wg_a.as<int4>();
             KParams kg = k;
             kg.accum = 1;
-            e = launch_assemble(G.et, jprec, mg, kg, u, u_old, jvals, defect, h->errflag.as<int>(), PG.nwg, PG.lds,
-                                (hipStream_t)stream);
+            e = launch(G.et, mg, kg, PG, G.ne);
         }
-        h->last_path = 1;
+        h->last_path = cpl ? 4 : 1;
     } else if (mfast >= 0) {
         MarchGrid g{};
         g.nx = h->march_n[0];

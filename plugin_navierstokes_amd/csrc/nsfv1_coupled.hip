@@ -960,11 +960,25 @@ coupled_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const JT* __r
     constexpr int PADE = NC == 4 ? 16 / (int)sizeof(JT) : 0;  // 2D rows already have odd strides
     double* dbuf = reinterpret_cast<double*>(smem + (((jlen + (v1 - v0) * PADE) * (int)sizeof(JT) + 15) & ~15));
     const bool wantJ = J != nullptr;
-    if (wantJ)
-        for (int i = threadIdx.x; i < jlen + (v1 - v0) * PADE; i += blockDim.x) jbuf[i] = JT(0);
-    for (int i = threadIdx.x; i < dlen; i += blockDim.x) dbuf[i] = 0.0;
-
     const int i0 = M.node_inc[v0], i1 = M.node_inc[v1];
+    // a further element group of a mixed mesh adds to the rows the groups
+    // before it wrote (same stream, earlier launch); a workgroup none of
+    // whose nodes touch the group leaves its rows as they are
+    if (P.accum && i0 == i1) return;
+    if (P.accum) {
+        if (wantJ)
+            for (int j = 0; j < v1 - v0; ++j) {
+                const int off = (int)(M.rowptr[(int64_t)(v0 + j) * NC] - jbase);
+                const int len = (int)(M.rowptr[(int64_t)(v0 + j + 1) * NC] - jbase) - off;
+                for (int i = threadIdx.x; i < len; i += blockDim.x) jbuf[off + j * PADE + i] = J[jbase + off + i];
+            }
+        for (int i = threadIdx.x; i < dlen; i += blockDim.x) dbuf[i] = dout ? dout[(int64_t)v0 * NC + i] : 0.0;
+    } else {
+        if (wantJ)
+            for (int i = threadIdx.x; i < jlen + (v1 - v0) * PADE; i += blockDim.x) jbuf[i] = JT(0);
+        for (int i = threadIdx.x; i < dlen; i += blockDim.x) dbuf[i] = 0.0;
+    }
+
     const int lane_inc = i0 + (int)threadIdx.x;
     const bool active = lane_inc < i1;
     JT jacc[NC][N][NC];

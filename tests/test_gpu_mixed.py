@@ -6,8 +6,10 @@ pattern (quadrilaterals + triangles in both group orders, hexahedra +
 prisms, prisms + tetrahedra; four upwinds; FP64 and FP32 Jacobians; A + M +
 rhs parts, time-dependent stabilisation); the pattern against the union of
 the types' adjacencies; one element type given as two groups against the
-single group bitwise (the same sums in the same order); the skewed-upwind
-record and replay across groups; a Newton solve; and the refusals.
+single group bitwise (the same sums in the same order), on the generic and
+the coupled path; the coupled path (positive upwind, FLOW) on
+quadrilaterals + triangles; the skewed-upwind record and replay across
+groups; a Newton solve; and the refusals.
 
 Tolerance: the parity rule (|a - b| <= 1e-12 |b| + 1e-13 S in FP64,
 2^-17 S + 2^-20 |b| for FP32 Jacobians); on meshes with prisms plus the
@@ -85,6 +87,67 @@ def test_mixed_mesh_parity(cuda, name, up, jprec):
     else:
         v, where = _viol(J, Jr, Ja, FP32_REL, FP32_ABS, floor)
     assert v <= 1.0, where
+
+
+COUPLED = [{"upwind": "positive"}, {"upwind": "lps", "stab": "flow"}, {"upwind": "positive", "stab": "flow"}]
+
+
+def _cid(c):
+    return "-".join(f"{k}={v}" for k, v in c.items())
+
+
+@pytest.mark.parametrize("jprec", [F64, F32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("cfg", COUPLED, ids=_cid)
+@pytest.mark.parametrize("name", ["quad_tri", "tri_quad"])
+def test_mixed_mesh_coupled_parity(cuda, name, cfg, jprec):
+    """The coupled path (positive upwind, FLOW) group by group."""
+    import torch
+
+    print(f"case mixed={name} cfg={cfg} jprec={jprec}", flush=True)
+    mm = CASES[name]()
+    cfg = dict(cfg, source=(0.3, -0.2))
+    disc = _disc(mm, cfg)
+    rowptr, colidx = disc.pattern()
+    u = M.random_state(mm, 6)
+    uold = M.random_state(mm, 106)
+    dt, s = 0.01, (0.7, 1.3, -0.7)
+    parts = PART_A | PART_M | PART_RHS
+    J, d = disc.assemble(torch.from_numpy(u).to(cuda), torch.from_numpy(uold).to(cuda), dt, s_a=s[0], s_m=s[1],
+                         s_f=s[2], parts=parts, jprec=jprec)
+    assert disc.assembly_path_used() == 4
+    J, d = J.cpu().numpy(), d.cpu().numpy()
+    p = oracle_params(2, cfg)
+    kw = dict(uold=uold, dt=dt, s_a=s[0], s_m=s[1], parts=parts)
+    Jr, dr = O.assemble_mixed(p, mm.groups, mm.xyz, u, rowptr, colidx, s_f=s[2], **kw)
+    Ja, da = O.assemble_abs_mixed(p, mm.groups, mm.xyz, u, rowptr, colidx, **kw)
+    da = da + np.abs(dr)
+    v, where = _viol(d, dr, da, FP64_REL, FP64_ABS, 0.0)
+    assert v <= 1.0, where
+    if jprec == F64:
+        v, where = _viol(J, Jr, Ja, FP64_REL, FP64_ABS, 0.0)
+    else:
+        v, where = _viol(J, Jr, Ja, FP32_REL, FP32_ABS, 0.0)
+    assert v <= 1.0, where
+
+
+@pytest.mark.parametrize("cfg", COUPLED[:2], ids=_cid)
+@pytest.mark.parametrize("et", [M.QUAD, M.TET, M.HEX], ids=["quad", "tet", "hex"])
+def test_one_type_as_two_groups_coupled_is_bitwise(cuda, et, cfg):
+    import torch
+
+    mesh = M.cavity_2d(6, et, 0.1, 3) if et == M.QUAD else M.cavity_3d(3, et, 0.1, 3)
+    print(f"case split groups et={et} cfg={cfg}", flush=True)
+    a = NavierStokesFV1("u,v,p" if mesh.dim == 2 else "u,v,w,p", "Inner")
+    configure_disc(a, cfg)
+    a.set_mesh(mesh)
+    b = _disc(M.split_groups(mesh, mesh.ne // 3), cfg)
+    u = torch.from_numpy(M.random_state(mesh, 2)).to(cuda)
+    uo = torch.from_numpy(M.random_state(mesh, 3)).to(cuda)
+    for jprec in (F64, F32):
+        Ja, da = a.assemble(u, uo, 0.01, s_m=1.0, parts=PART_A | PART_M, jprec=jprec)
+        Jb, db = b.assemble(u, uo, 0.01, s_m=1.0, parts=PART_A | PART_M, jprec=jprec)
+        assert a.assembly_path_used() == b.assembly_path_used() == 4
+        assert torch.equal(Ja, Jb) and torch.equal(da, db)
 
 
 @pytest.mark.parametrize("et", [M.TRI, M.TET, M.HEX, M.PRISM], ids=["tri", "tet", "hex", "prism"])
@@ -167,6 +230,7 @@ def test_refusals(cuda):
     disc = _disc(mm, {"upwind": "full"})
     u = torch.from_numpy(M.random_state(mm, 2)).to(cuda)
     disc.assemble(u, jprec=F32)
+    # the coupled path has no prisms
     disc.set_stabilization("flow")
     with pytest.raises(NSFV1Error) as e:
         disc.assemble(u, jprec=F64)
