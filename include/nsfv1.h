@@ -189,9 +189,11 @@ int nsfv1_set_mesh(nsfv1_handle* h, int elem_type, const double* xyz, int64_t nv
    groups; the assembly runs the generic kernel (the coupled path for the
    FLOW stabilization and the positive/regular upwinds) once per group on
    the stream, the first group writing every row and the others adding
-   theirs.  Boundary discs, data exports, integration-point imports and
-   batched instances refuse mixed meshes with NSFV1_ERR_NOT_IMPLEMENTED, as
-   does the coupled path for a mesh with prisms. */
+   theirs.  Integration-point imports and nsfv1_ip_positions concatenate
+   the groups' blocks in group order ([ne_g][nip_g] ..., each with dim
+   where it applies).  Boundary discs, data exports and batched instances
+   refuse mixed meshes with NSFV1_ERR_NOT_IMPLEMENTED, as does the coupled
+   path for a mesh with prisms. */
 int nsfv1_set_mesh_mixed(nsfv1_handle* h, const double* xyz, int64_t nv, int ngroups, const int* elem_type,
                          const int32_t* const* conn, const int64_t* ne);
 

@@ -208,10 +208,15 @@ class NavierStokesFV1:
         _check(_lib.nsfv1_set_source(self._h, arr, 0 if f is None else len(f)))
 
     def ip_positions(self, kind: str = "scvf") -> np.ndarray:
-        """Global integration points: "scvf" [ne, nip, dim] or "scv" [ne, nsh, dim] (the corners)."""
+        """Global integration points: "scvf" [ne, nip, dim] or "scv" [ne, nsh, dim] (the corners);
+        on a mixed mesh [points, dim], group after group."""
         k = {"scvf": 0, "scv": 1}[kind]
-        per = {0: (3, 3), 1: (4, 4), 2: (6, 4), 3: (12, 8), 4: (9, 6)}[self._et][k]
-        out = np.zeros((self.ne, per, self.dim))
+        per = {0: (3, 3), 1: (4, 4), 2: (6, 4), 3: (12, 8), 4: (9, 6)}
+        groups = getattr(self, "_groups", [(self._et, self.ne)])
+        if len(groups) == 1:
+            out = np.zeros((self.ne, per[self._et][k], self.dim))
+        else:
+            out = np.zeros((sum(ne * per[et][k] for et, ne in groups), self.dim))
         _check(_lib.nsfv1_ip_positions(self._h, k, out.ctypes.data_as(C.POINTER(C.c_double)), out.size))
         return out
 
@@ -223,7 +228,7 @@ class NavierStokesFV1:
             vals = np.ascontiguousarray(fn, dtype=np.float64)
         else:
             x = self.ip_positions("scvf")
-            vals = np.ascontiguousarray(np.asarray(fn(x.reshape(-1, self.dim)), dtype=np.float64).reshape(x.shape[:2]))
+            vals = np.ascontiguousarray(np.asarray(fn(x.reshape(-1, self.dim)), dtype=np.float64).reshape(x.shape[:-1]))
         _check(_lib.nsfv1_set_kinematic_viscosity_ip(self._h, vals.ctypes.data_as(C.POINTER(C.c_double)), vals.size))
         if callable(fn):
             for c in getattr(self, "_mg_levels", []):
@@ -309,6 +314,7 @@ class NavierStokesFV1:
         self.ne = conn.shape[0]
         self.dim = 2 if int(et) in (0, 1) else 3
         self._et = int(et)
+        self._groups = [(self._et, self.ne)]
         self._mg_levels = []  # a new mesh drops the multigrid hierarchy
         # functions of position are imports: evaluated again on the new mesh
         if callable(getattr(self, "_nu_fn", None)):
@@ -331,8 +337,9 @@ class NavierStokesFV1:
         self.ne = sum(c.shape[0] for c in conns)
         self.dim = xyz.shape[1]
         self._et = int(mesh.groups[0][0])
+        self._groups = [(int(et), c.shape[0]) for et, c in mesh.groups]
         self._mg_levels = []  # a new mesh drops the multigrid hierarchy
-        # function imports are refused on mixed meshes (raises)
+        # functions of position are imports: evaluated again on the new mesh
         if callable(getattr(self, "_nu_fn", None)):
             self._apply_nu_fn()
         if callable(getattr(self, "_f_fn", None)):

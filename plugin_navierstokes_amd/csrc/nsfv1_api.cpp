@@ -264,6 +264,15 @@ static void invalidate_plans(nsfv1_handle* h)
     for (auto& g : h->groups) g->plan32.valid = g->plan64.valid = false;
 }
 
+// integration points over all element groups (kind 0: SCVF ips, 1: SCV
+// ips = corners); imports and positions run through the groups in order
+static int64_t group_points(const nsfv1_handle* h, int kind)
+{
+    int64_t n = h->ne * (kind == 0 ? nip_of(h->et) : nsh_count(h->et));
+    for (auto& g : h->groups) n += g->ne * (kind == 0 ? nip_of(g->et) : nsh_count(g->et));
+    return n;
+}
+
 static int upload(DevBuf& b, const double* x, int64_t n)
 {
     if (b.alloc(sizeof(double) * (size_t)n) != hipSuccess ||
@@ -410,10 +419,9 @@ int nsfv1_set_kinematic_viscosity_ip(nsfv1_handle* h, const double* nu, int64_t 
         return NSFV1_OK;
     }
     if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
-    if (h->mixed()) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "imports at the integration points are not provided on mixed-element meshes");
-    if (n != h->ne * nip_of(h->et))
+    if (n != group_points(h, 0))
         return fail(NSFV1_ERR_ARGUMENT, "viscosity at the SCVF ips: expected ne * nip = " +
-                                            std::to_string(h->ne * nip_of(h->et)) + " values");
+                                            std::to_string(group_points(h, 0)) + " values");
     int rc = upload(h->nu_ip, nu, n);
     if (rc) return rc;
     h->nu_set = true;
@@ -432,8 +440,7 @@ int nsfv1_set_source_ip(nsfv1_handle* h, const double* f_scv, int64_t n_scv, con
     }
     if (!f_scv || !f_scvf) return fail(NSFV1_ERR_ARGUMENT, "the source import needs its values at both the SCV and the SCVF ips");
     if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
-    if (h->mixed()) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "imports at the integration points are not provided on mixed-element meshes");
-    const int64_t a = h->ne * nsh_count(h->et) * h->dim, b = h->ne * nip_of(h->et) * h->dim;
+    const int64_t a = group_points(h, 1) * h->dim, b = group_points(h, 0) * h->dim;
     if (n_scv != a || n_scvf != b)
         return fail(NSFV1_ERR_ARGUMENT, "source at the ips: expected ne * nsh * dim = " + std::to_string(a) +
                                             " and ne * nip * dim = " + std::to_string(b) + " values");
@@ -449,21 +456,30 @@ int nsfv1_ip_positions(nsfv1_handle* h, int kind, double* out, int64_t n)
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
     if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
-    if (h->mixed()) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "integration-point positions are not provided on mixed-element meshes");
     if (kind != 0 && kind != 1) return fail(NSFV1_ERR_ARGUMENT, "kind: 0 = SCVF ips, 1 = SCV ips");
-    const int64_t want = h->ne * (kind == 0 ? nip_of(h->et) : nsh_count(h->et)) * h->dim;
+    const int64_t want = group_points(h, kind) * h->dim;
     if (!out || n != want) return fail(NSFV1_ERR_ARGUMENT, "expected " + std::to_string(want) + " values");
     std::vector<double> xyz((size_t)h->nv * h->dim);
-    std::vector<int32_t> conn((size_t)h->ne * nsh_count(h->et));
-    if (hipMemcpy(xyz.data(), h->xyz.p, sizeof(double) * xyz.size(), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(conn.data(), h->conn.p, sizeof(int32_t) * conn.size(), hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpy(xyz.data(), h->xyz.p, sizeof(double) * xyz.size(), hipMemcpyDeviceToHost) != hipSuccess)
         return fail(NSFV1_ERR_DEVICE, "mesh download");
-    switch (h->et) {
-    case 0: ip_positions_et<0>(xyz, conn, h->ne, kind, out); break;
-    case 1: ip_positions_et<1>(xyz, conn, h->ne, kind, out); break;
-    case 2: ip_positions_et<2>(xyz, conn, h->ne, kind, out); break;
-    case 3: ip_positions_et<3>(xyz, conn, h->ne, kind, out); break;
-    default: ip_positions_et<4>(xyz, conn, h->ne, kind, out); break;
+    // group after group (the handle's own elements first)
+    auto group = [&](int et, int64_t ne, const DevBuf& cb, double* o) -> bool {
+        std::vector<int32_t> conn((size_t)ne * nsh_count(et));
+        if (hipMemcpy(conn.data(), cb.p, sizeof(int32_t) * conn.size(), hipMemcpyDeviceToHost) != hipSuccess) return false;
+        switch (et) {
+        case 0: ip_positions_et<0>(xyz, conn, ne, kind, o); break;
+        case 1: ip_positions_et<1>(xyz, conn, ne, kind, o); break;
+        case 2: ip_positions_et<2>(xyz, conn, ne, kind, o); break;
+        case 3: ip_positions_et<3>(xyz, conn, ne, kind, o); break;
+        default: ip_positions_et<4>(xyz, conn, ne, kind, o); break;
+        }
+        return true;
+    };
+    if (!group(h->et, h->ne, h->conn, out)) return fail(NSFV1_ERR_DEVICE, "mesh download");
+    int64_t off = h->ne * (kind == 0 ? nip_of(h->et) : nsh_count(h->et)) * h->dim;
+    for (auto& g : h->groups) {
+        if (!group(g->et, g->ne, g->conn, out + off)) return fail(NSFV1_ERR_DEVICE, "mesh download");
+        off += g->ne * (kind == 0 ? nip_of(g->et) : nsh_count(g->et)) * h->dim;
     }
     return NSFV1_OK;
 }
@@ -1179,6 +1195,8 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
                                          (hipStream_t)stream);
         };
         e = launch(h->et, m, k, P, h->ne);
+        // imports at the integration points run through the groups in order
+        int64_t off_ip = h->ne * nip_of(h->et), off_sh = h->ne * nsh_count(h->et);
         for (size_t g = 0; e == hipSuccess && g < h->groups.size(); ++g) {
             ElemGroup& G = *h->groups[g];
             const Plan& PG = jprec == NSFV1_F64 ? G.plan64 : G.plan32;
@@ -1200,6 +1218,11 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
             mg.lane_a = PG.lane_a.as<int4>();
             mg.lane_pos = PG.lane_pos.as<uint64_t>();
             mg.wg_a = PG.wg_a.as<int4>();
+            mg.nu_ip = h->nu_ip.p ? h->nu_ip.as<double>() + off_ip : nullptr;
+            mg.f_scv = h->f_scv.p ? h->f_scv.as<double>() + off_sh * h->dim : nullptr;
+            mg.f_scvf = h->f_scvf.p ? h->f_scvf.as<double>() + off_ip * h->dim : nullptr;
+            off_ip += G.ne * nip_of(G.et);
+            off_sh += G.ne * nsh_count(G.et);
             KParams kg = k;
             kg.accum = 1;
             e = launch(G.et, mg, kg, PG, G.ne);

@@ -8,8 +8,9 @@ rhs parts, time-dependent stabilisation); the pattern against the union of
 the types' adjacencies; one element type given as two groups against the
 single group bitwise (the same sums in the same order), on the generic and
 the coupled path; the coupled path (positive upwind, FLOW) on
-quadrilaterals + triangles; the skewed-upwind record and replay across
-groups; a Newton solve; and the refusals.
+quadrilaterals + triangles; function-valued viscosity and source, group
+by group; the skewed-upwind record and replay across groups; a Newton
+solve; and the refusals.
 
 Tolerance: the parity rule (|a - b| <= 1e-12 |b| + 1e-13 S in FP64,
 2^-17 S + 2^-20 |b| for FP32 Jacobians); on meshes with prisms plus the
@@ -223,6 +224,73 @@ def test_newton_on_a_mixed_mesh(cuda):
     assert np.linalg.norm(d1[free]) <= 1e-8 * np.linalg.norm(d0[free])
 
 
+def _nu(x):
+    return 0.02 * (1.0 + x[:, 0] + 0.5 * x[:, 1] ** 2)
+
+
+def _f(x):
+    cols = [np.sin(3.0 * x[:, 0]), np.cos(2.0 * x[:, 1]), x[:, 0] * x[:, 1] - 0.2]
+    return np.stack(cols[: x.shape[1]], 1)
+
+
+def _coupled(cfg):
+    return cfg["upwind"] == "positive" or cfg.get("stab") == "flow"
+
+
+# the coupled path has no prisms (refused, see test_refusals)
+IMPORT_CASES = [(n, c) for n in sorted(CASES)
+                for c in ({"upwind": "full"}, {"upwind": "lps"}, {"upwind": "positive"}, {"upwind": "lps", "stab": "flow"})
+                if not (_coupled(c) and "prism" in n)]
+
+
+@pytest.mark.parametrize("name,cfg", IMPORT_CASES, ids=[f"{n}-{_cid(c)}" for n, c in IMPORT_CASES])
+def test_mixed_mesh_function_imports(cuda, name, cfg):
+    """Function-valued viscosity and source on a mixed mesh: positions and
+    imports run through the groups in order; each group's elements take
+    their own slice (generic and coupled path)."""
+    import torch
+
+    mm = CASES[name]()
+    coupled = _coupled(cfg)
+    print(f"case mixed={name} cfg={cfg} imports", flush=True)
+    dim = mm.dim
+    disc = _disc(mm, cfg)
+    disc.set_kinematic_viscosity(_nu)
+    disc.set_source(_f)
+    xf = disc.ip_positions("scvf")
+    xo = np.concatenate([np.stack([O.geometry(et, mm.xyz[c])["ipg"] for c in conn]).reshape(-1, dim)
+                         for et, conn in mm.groups])
+    assert xf.shape == xo.shape and np.max(np.abs(xf - xo)) <= 1e-14
+    xc = disc.ip_positions("scv")
+    assert np.array_equal(xc, np.concatenate([mm.xyz[conn].reshape(-1, dim) for _, conn in mm.groups]))
+    u = M.random_state(mm, 5)
+    uold = M.random_state(mm, 105)
+    dt, s = 0.01, (0.7, 1.3, -0.7)
+    parts = PART_A | PART_M | PART_RHS
+    J, d = disc.assemble(torch.from_numpy(u).to(cuda), torch.from_numpy(uold).to(cuda), dt, s_a=s[0], s_m=s[1],
+                         s_f=s[2], parts=parts, jprec=F64)
+    assert disc.assembly_path_used() == (4 if coupled else 1)
+    J, d = J.cpu().numpy(), d.cpu().numpy()
+    rowptr, colidx = disc.pattern()
+    Jr = dr = Ja = da = 0.0
+    for et, conn in mm.groups:
+        p = oracle_params(dim, cfg)
+        p.source = (0.0,) * dim
+        ipg = np.stack([O.geometry(et, mm.xyz[c])["ipg"] for c in conn])
+        p.nu_ip = _nu(ipg.reshape(-1, dim)).reshape(len(conn), -1)
+        p.f_scv = _f(mm.xyz[conn].reshape(-1, dim)).reshape(len(conn), -1, dim)
+        p.f_scvf = _f(ipg.reshape(-1, dim)).reshape(len(conn), -1, dim)
+        kw = dict(s_a=s[0], s_m=s[1], parts=parts, rowptr=rowptr, colidx=colidx)
+        Jg, dg = O.assemble(p, et, mm.xyz, conn, u, uold, dt, s_f=s[2], **kw)
+        Jag, dag = O.assemble_abs(p, et, mm.xyz, conn, u, uold, dt, **kw)
+        Jr, dr, Ja, da = Jr + Jg, dr + dg, Ja + Jag, da + dag
+    da = da + np.abs(dr)
+    floor = PRISM_FLOOR if any(et == M.PRISM for et, _ in mm.groups) else 0.0
+    for a, b, sc in ((d, dr, da), (J, Jr, Ja)):
+        v, where = _viol(a, b, sc, FP64_REL, FP64_ABS, floor)
+        assert v <= 1.0, where
+
+
 def test_refusals(cuda):
     import torch
 
@@ -243,9 +311,6 @@ def test_refusals(cuda):
     disc.set_upwind("full")
     with pytest.raises(NSFV1Error) as e:
         disc.set_outflow(np.array([0], np.int32), np.array([0], np.int32))
-    assert e.value.name == "NOT_IMPLEMENTED"
-    with pytest.raises(NSFV1Error) as e:
-        disc.set_kinematic_viscosity(lambda x: 0.02 + 0 * x[:, 0])
     assert e.value.name == "NOT_IMPLEMENTED"
     with pytest.raises(NSFV1Error) as e:
         disc.export(u, "velocity", "scvf")
