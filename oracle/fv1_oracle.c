@@ -2081,7 +2081,7 @@ static int point_grads(int et, const double (*co)[3], const double* x, double* p
 int or_export_elem(const or_params* p, int et, const double* corners, const double* uloc, const double* uold,
                    double dt, int what, int at, const double* lips, int nip, double* out)
 {
-    if (et < 0 || et > 3) { set_err("unknown element type"); return -1; }
+    if (et < 0 || et > 4) { set_err("unknown element type"); return -1; }
     const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1, P = dim;
     const int ncomp = or_export_ncomp(what, dim);
     if (ncomp < 0 || at < 0 || at > 2) { set_err("unknown export"); return -1; }
@@ -2151,7 +2151,7 @@ int or_export_elem(const or_params* p, int et, const double* corners, const doub
 int or_export(const or_params* p, int et, const double* xyz, const int32_t* conn, int64_t ne, const double* u,
               const double* uold, double dt, int what, int at, const double* lips, int nip, double* out)
 {
-    if (et < 0 || et > 3) { set_err("unknown element type"); return -1; }
+    if (et < 0 || et > 4) { set_err("unknown element type"); return -1; }
     const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1;
     const int npt = at == 0 ? REF[et].nip : (at == 1 ? nsh : nip);
     const int ncomp = or_export_ncomp(what, dim);

@@ -1471,7 +1471,7 @@ int nsfv1_export(nsfv1_handle* h, int what, int at, const double* local_ips, int
     int64_t count = 0;
     int rc = nsfv1_export_size(h, what, at, nip, &count);
     if (rc) return rc;
-    if (h->et == ET_PRISM) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "data exports on prisms are not provided");
+
     if (h->mixed()) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "data exports on mixed-element meshes are not provided");
     if (!u || !out) return fail(NSFV1_ERR_ARGUMENT, "NULL buffer");
     if (at == NSFV1_AT_LOCAL && !local_ips) return fail(NSFV1_ERR_ARGUMENT, "local points are NULL");
@@ -1515,6 +1515,9 @@ int nsfv1_export(nsfv1_handle* h, int what, int at, const double* local_ips, int
         m.f_scvf = h->f_scvf.as<double>();
         hipError_t e;
         if (coupled_config(h)) {
+            if (h->et == ET_PRISM)
+                return fail(NSFV1_ERR_NOT_IMPLEMENTED, "prisms are not provided with the FLOW stabilization or the "
+                                                       "positive/regular upwinds");
             k.conv_upwind = h->conv_upwind;
             k.pac = h->pac;
             e = launch_assemble_coupled(h->et, NSFV1_F64, m, k, u, u_old, nullptr, nullptr, h->errflag.as<int>(), 0, 0,

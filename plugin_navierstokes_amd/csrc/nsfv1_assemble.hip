@@ -1756,6 +1756,7 @@ hipError_t launch_export_stab_vel(int et, const KMeshPtrs& m, const KParams& p, 
     case ET_QUAD: export_stab_vel<ET_QUAD><<<blocks, 256, 0, s>>>(m, p, u, uold, ne, out, err); break;
     case ET_TET:  export_stab_vel<ET_TET><<<blocks, 256, 0, s>>>(m, p, u, uold, ne, out, err); break;
     case ET_HEX:  export_stab_vel<ET_HEX><<<blocks, 256, 0, s>>>(m, p, u, uold, ne, out, err); break;
+    case ET_PRISM: export_stab_vel<ET_PRISM><<<blocks, 256, 0, s>>>(m, p, u, uold, ne, out, err); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

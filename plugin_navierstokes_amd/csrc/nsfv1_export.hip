@@ -147,6 +147,7 @@ hipError_t launch_export_points(int et, const ExportArgs& a, const double* u, do
     case 1: export_points<1><<<blocks, 256, 0, s>>>(a, u, out, err); break;
     case 2: export_points<2><<<blocks, 256, 0, s>>>(a, u, out, err); break;
     case 3: export_points<3><<<blocks, 256, 0, s>>>(a, u, out, err); break;
+    case 4: export_points<4><<<blocks, 256, 0, s>>>(a, u, out, err); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

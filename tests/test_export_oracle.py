@@ -1,6 +1,6 @@
 """Pins the oracle's data exports (navier_stokes_fv1.cpp:894-1491):
-linear fields are reproduced exactly by the P1/Q1 interpolation (also on
-distorted hexahedra, where x(xi) itself is trilinear), so values and global
+linear fields are reproduced exactly by the P1/Q1 and prism interpolations
+(also on distorted hexahedra and prisms, where x(xi) is not affine), so values and global
 gradients at SCVF ips, corners and local points have closed forms; the
 "general case" velocity gradient is the reference-element gradient
 (:1072-1080), A B on an affine element x = B xi + x0; the stabilised ip
@@ -13,10 +13,11 @@ import pytest
 
 import oracle as O
 
-ETS = [O.TRI, O.QUAD, O.TET, O.HEX]
+ETS = [O.TRI, O.QUAD, O.TET, O.HEX, O.PRISM]
 LC = {O.TRI: [[0, 0], [1, 0], [0, 1]], O.QUAD: [[0, 0], [1, 0], [1, 1], [0, 1]],
       O.TET: [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]],
-      O.HEX: [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]]}
+      O.HEX: [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]],
+      O.PRISM: [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [0, 1, 1]]}
 
 
 def _distorted(et, seed):

@@ -16,7 +16,7 @@ from tests._parity import configure_disc, oracle_params, small_mesh
 
 pytestmark = pytest.mark.gpu
 
-ETS = [M.TRI, M.QUAD, M.TET, M.HEX]
+ETS = [M.TRI, M.QUAD, M.TET, M.HEX, M.PRISM]
 WHAT = ["velocity", "velocity_grad", "velocity_ip", "pressure", "pressure_grad"]
 
 

@@ -3,7 +3,8 @@ pattern bit-exactly, the A part in FP64 for every upwind, stabilisation and
 flag case of the tetrahedral/hexahedral parity suite, the time-dependent
 A + M + rhs assembly, the FP32 Jacobian bound, the no-allocation entry,
 function-valued imports, and the refused combinations (coupled path,
-boundary discs, data exports, pyramids).
+boundary discs, the coupled configurations' stabilised ip velocity,
+pyramids).
 
 Tolerance: the parity rule (|a - b| <= 1e-12 |b| + 1e-13 S) plus an
 absolute floor of 1e-15 max|b| over the array.  On prisms whose vertical
@@ -129,8 +130,12 @@ def test_refused_combinations(cuda):
     with pytest.raises(NSFV1Error) as e:
         disc.set_outflow(belem, bside)
     assert e.value.name == "NOT_IMPLEMENTED"
+    # exports run on prisms (tests/test_gpu_export.py); the stabilised ip
+    # velocity of the coupled configurations does not
+    assert disc.export(u, "velocity", "scv").shape == (mesh.ne, 6, 3)
+    disc.set_upwind("positive")
     with pytest.raises(NSFV1Error) as e:
-        disc.export(u, "velocity", "scv")
+        disc.export(u, "velocity_ip", "scvf")
     assert e.value.name == "NOT_IMPLEMENTED"
     # pyramids are refused when the mesh is set
     pyr = NavierStokesFV1("u,v,w,p", "Inner")
