@@ -43,8 +43,8 @@ typedef struct nsfv1_handle nsfv1_handle;
    navier_stokes_fv1.cpp:1515-1551 for the 2D and 3D domains).  Prisms
    (UG4 numbering: corners 0-1-2 at the bottom, 3-4-5 above them; edges
    01 12 20 03 14 25 34 45 53; sides 021, 0143, 1254, 2035, 345) assemble on
-   the generic path only and export data: the FLOW stabilization, the
-   positive and regular upwinds and boundary discs refuse them.  Pyramids are
+   the generic path, with boundary discs and data exports: the FLOW
+   stabilization and the positive and regular upwinds refuse them.  Pyramids are
    refused by nsfv1_set_mesh with NSFV1_ERR_NOT_IMPLEMENTED. */
 enum {
     NSFV1_TRIANGLE = 0,

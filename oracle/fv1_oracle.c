@@ -1652,7 +1652,7 @@ int or_pattern(int et, int64_t nv, const int32_t* conn, int64_t ne,
 /* ------------------------------------------------------------------ */
 int or_bnd_faces(int et, const double* corners, int side, or_bfgeo* B)
 {
-    if (et < 0 || et > 3) { set_err("unknown element type"); return -1; }
+    if (et < 0 || et > 4) { set_err("unknown element type"); return -1; }
     const refelem* R = &REF[et];
     if (side < 0 || side >= R->nside) { set_err("side index out of range"); return -1; }
     const int dim = R->dim, nsh = R->nsh, k = R->side_n[side];
@@ -1837,7 +1837,7 @@ static int bnd_scatter(int kind, const or_params* p, int et, const double* xyz, 
                        const int32_t* belem, const int32_t* bside, int64_t nb, const double* u, double s_a,
                        double k, double lim, const int64_t* rowptr, const int32_t* colidx, double* J, double* d)
 {
-    if (et < 0 || et > 3) { set_err("unknown element type"); return -1; }
+    if (et < 0 || et > 4) { set_err("unknown element type"); return -1; }
     const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1, nl = nsh * nc;
     double co[24], ul[32], Jl[1024], dl[32];
     for (int64_t b = 0; b < nb; ++b) {

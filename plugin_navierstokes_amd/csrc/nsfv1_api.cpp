@@ -1370,11 +1370,10 @@ static int set_boundary_list(nsfv1_handle* h, int kind, const int32_t* elem, con
     auto& B = h->bnd[kind];
     B.nbn = 0;
     if (n == 0) return NSFV1_OK;
-    if (h->et == ET_PRISM) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "boundary discs on prisms are not provided");
     if (h->mixed()) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "boundary discs on mixed-element meshes are not provided");
     if (n < 0 || !elem || !side) return fail(NSFV1_ERR_ARGUMENT, "bad boundary side arrays");
     const int nsh = nsh_of(h->et);
-    const int nside = h->et == ET_HEX ? 6 : (h->et == ET_QUAD ? 4 : (h->et == ET_TET ? 4 : 3));
+    const int nside = h->et == ET_HEX ? 6 : (h->et == ET_PRISM ? 5 : (h->et == ET_QUAD ? 4 : (h->et == ET_TET ? 4 : 3)));
     std::vector<int32_t> cn;  // conn rows are needed: download once
     cn.resize((size_t)h->ne * nsh);
     if (hipMemcpy(cn.data(), h->conn.p, sizeof(int32_t) * cn.size(), hipMemcpyDeviceToHost) != hipSuccess)
@@ -1391,7 +1390,8 @@ static int set_boundary_list(nsfv1_handle* h, int kind, const int32_t* elem, con
         case 0: side_corners<0>(s, &k, c); break;
         case 1: side_corners<1>(s, &k, c); break;
         case 2: side_corners<2>(s, &k, c); break;
-        default: side_corners<3>(s, &k, c); break;
+        case 3: side_corners<3>(s, &k, c); break;
+        default: side_corners<4>(s, &k, c); break;
         }
         for (int j = 0; j < k; ++j) {
             const int64_t v = cn[e * nsh + c[j]];

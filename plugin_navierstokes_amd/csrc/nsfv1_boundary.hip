@@ -44,7 +44,7 @@
 
 namespace nsfv1 {
 
-// P1/Q1 shapes and reference gradients at a run-time local point
+// P1/Q1 (prism: P1 x P1) shapes and reference gradients at a run-time local point
 template <int ET>
 __device__ inline void shapes_at_point(const double* x, double* phi, double (*gl)[3])
 {
@@ -64,6 +64,13 @@ __device__ inline void shapes_at_point(const double* x, double* phi, double (*gl
             for (int j = 0; j < 3; ++j) gl[k][j] = 0.0;
         gl[0][0] = gl[0][1] = gl[0][2] = -1;
         gl[1][0] = 1; gl[2][1] = 1; gl[3][2] = 1;
+    } else if constexpr (ET == 4) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            phi[k] = shape_at<4>(k, -1, x[0], x[1], x[2]);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) gl[k][j] = shape_at<4>(k, j, x[0], x[1], x[2]);
+        }
     } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -437,6 +444,7 @@ hipError_t launch_outflow(int et, int jprec, const OutflowArgs& a, const double*
     case 1: NSFV1_OUT(1) break;
     case 2: NSFV1_OUT(2) break;
     case 3: NSFV1_OUT(3) break;
+    case 4: NSFV1_OUT(4) break;
     default: return hipErrorInvalidValue;
     }
 #undef NSFV1_OUT

@@ -16,14 +16,16 @@ import pytest
 import oracle as O
 from plugin_navierstokes_amd import meshes as M
 
-ETS = [O.TRI, O.QUAD, O.TET, O.HEX]
+ETS = [O.TRI, O.QUAD, O.TET, O.HEX, O.PRISM]
+PRISM_LC = [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [0, 1, 1]]
 
 
 def _elem(et, seed, jitter=0.15):
     rng = np.random.default_rng(seed)
     lc = {O.TRI: [[0, 0], [1, 0], [0, 1]], O.QUAD: [[0, 0], [1, 0], [1, 1], [0, 1]],
           O.TET: [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]],
-          O.HEX: [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]]}[et]
+          O.HEX: [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]],
+          O.PRISM: PRISM_LC}[et]
     co = np.asarray(lc, dtype=float) * 1.3 + rng.uniform(-jitter, jitter, size=(len(lc), O.DIM[et]))
     return co
 
@@ -55,7 +57,7 @@ def test_boundary_face_areas_and_ips(et):
     lc = np.asarray({O.TRI: [[0, 0], [1, 0], [0, 1]], O.QUAD: [[0, 0], [1, 0], [1, 1], [0, 1]],
                      O.TET: [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]],
                      O.HEX: [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1],
-                             [0, 1, 1]]}[et], dtype=float)
+                             [0, 1, 1]], O.PRISM: PRISM_LC}[et], dtype=float)
     co = 2.0 * lc
     c = co.mean(0)
     for s, corners in enumerate(M.SIDES[et]):
