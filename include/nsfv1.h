@@ -43,8 +43,8 @@ typedef struct nsfv1_handle nsfv1_handle;
    navier_stokes_fv1.cpp:1515-1551 for the 2D and 3D domains).  Prisms
    (UG4 numbering: corners 0-1-2 at the bottom, 3-4-5 above them; edges
    01 12 20 03 14 25 34 45 53; sides 021, 0143, 1254, 2035, 345) assemble on
-   the generic path, with boundary discs and data exports: the FLOW
-   stabilization and the positive and regular upwinds refuse them.  Pyramids are
+   the generic path (and the coupled path for the FLOW stabilization and
+   the positive upwind), with boundary discs and data exports.  Pyramids are
    refused by nsfv1_set_mesh with NSFV1_ERR_NOT_IMPLEMENTED. */
 enum {
     NSFV1_TRIANGLE = 0,
@@ -192,8 +192,7 @@ int nsfv1_set_mesh(nsfv1_handle* h, int elem_type, const double* xyz, int64_t nv
    theirs.  Integration-point imports and nsfv1_ip_positions concatenate
    the groups' blocks in group order ([ne_g][nip_g] ..., each with dim
    where it applies).  Boundary discs, data exports and batched instances
-   refuse mixed meshes with NSFV1_ERR_NOT_IMPLEMENTED, as does the coupled
-   path for a mesh with prisms. */
+   refuse mixed meshes with NSFV1_ERR_NOT_IMPLEMENTED. */
 int nsfv1_set_mesh_mixed(nsfv1_handle* h, const double* xyz, int64_t nv, int ngroups, const int* elem_type,
                          const int32_t* const* conn, const int64_t* ne);
 

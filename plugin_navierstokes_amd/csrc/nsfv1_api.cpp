@@ -1074,14 +1074,6 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
                 if (h->bnd[kind].nbn)
                     return fail(NSFV1_ERR_NOT_IMPLEMENTED,
                                 "boundary discs with per-instance viscosity (batched instances) are not provided");
-        if (h->et == ET_PRISM && coupled_config(h))
-            return fail(NSFV1_ERR_NOT_IMPLEMENTED, "prisms are not provided with the FLOW stabilization or the "
-                                                   "positive/regular upwinds");
-        if (coupled_config(h))
-            for (auto& gp : h->groups)
-                if (gp->et == ET_PRISM)
-                    return fail(NSFV1_ERR_NOT_IMPLEMENTED, "prisms are not provided with the FLOW stabilization or the "
-                                                           "positive/regular upwinds");
         if (h->ip_imports()) {
             for (int kind = 0; kind < BND_KINDS; ++kind)
                 if (h->bnd[kind].nbn)
@@ -1515,9 +1507,6 @@ int nsfv1_export(nsfv1_handle* h, int what, int at, const double* local_ips, int
         m.f_scvf = h->f_scvf.as<double>();
         hipError_t e;
         if (coupled_config(h)) {
-            if (h->et == ET_PRISM)
-                return fail(NSFV1_ERR_NOT_IMPLEMENTED, "prisms are not provided with the FLOW stabilization or the "
-                                                       "positive/regular upwinds");
             k.conv_upwind = h->conv_upwind;
             k.pac = h->pac;
             e = launch_assemble_coupled(h->et, NSFV1_F64, m, k, u, u_old, nullptr, nullptr, h->errflag.as<int>(), 0, 0,

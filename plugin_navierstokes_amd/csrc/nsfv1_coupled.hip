@@ -1132,6 +1132,7 @@ hipError_t launch_assemble_coupled(int et, int jprec, const KMeshPtrs& m, const 
     case ET_QUAD: NSFV1_CPL(ET_QUAD)
     case ET_TET: NSFV1_CPL(ET_TET)
     case ET_HEX: NSFV1_CPL(ET_HEX)
+    case ET_PRISM: NSFV1_CPL(ET_PRISM)
     default: return hipErrorInvalidValue;
     }
 #undef NSFV1_CPL

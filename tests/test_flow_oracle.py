@@ -23,10 +23,11 @@ import scipy.sparse as sp
 import oracle as O
 from plugin_navierstokes_amd import meshes as M
 
-ETS = [O.TRI, O.QUAD, O.TET, O.HEX]
+ETS = [O.TRI, O.QUAD, O.TET, O.HEX, O.PRISM]
 LC = {O.TRI: [[0, 0], [1, 0], [0, 1]], O.QUAD: [[0, 0], [1, 0], [1, 1], [0, 1]],
       O.TET: [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]],
-      O.HEX: [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]]}
+      O.HEX: [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]],
+      O.PRISM: [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [0, 1, 1]]}
 
 
 def _elem(et, seed, jitter=0.12):

@@ -5,7 +5,7 @@ ray/SCV intersection under the oracle's stated convention) with the matrix
 case of the stabilisation
 (stabilization.cpp:244-403) and ip shapes in the convective term
 (navier_stokes_fv1.cpp:430-457) -- through the C ABI against the CPU oracle,
-every element type, FP64 and FP32 Jacobians, steady and time-dependent,
+every element type (prisms included), FP64 and FP32 Jacobians, steady and time-dependent,
 with the mass and source parts, and the stabilised-velocity export.
 
 Tolerances as in the other parity tests: FP64 |a-b| <= 1e-12|b| + 1e-13 S,
@@ -24,8 +24,17 @@ from tests._parity import (FP32_ABS, FP32_REL, configure_disc, fp32_violation, f
 
 pytestmark = pytest.mark.gpu
 
-ETS = [M.TRI, M.QUAD, M.TET, M.HEX]
-ET_NAMES = {M.TRI: "tri", M.QUAD: "quad", M.TET: "tet", M.HEX: "hex"}
+ETS = [M.TRI, M.QUAD, M.TET, M.HEX, M.PRISM]
+ET_NAMES = {M.TRI: "tri", M.QUAD: "quad", M.TET: "tet", M.HEX: "hex", M.PRISM: "prism"}
+# prisms: the absolute floor of their parity suite, 1e-15 max|b| (an entry
+# that cancels exactly in the oracle's corner order can be ~1e-19 in the
+# kernel's order)
+PRISM_FLOOR = 1e-15
+
+
+def _fl(et, b, scale, unit=1e-13):
+    """The scale S of the tolerance with the prisms' floor folded in."""
+    return scale + (PRISM_FLOOR / unit) * np.abs(b).max() if et == M.PRISM else scale
 
 CASES = [
     {"upwind": "positive"},
@@ -56,8 +65,8 @@ def _cid(c):
 def test_coupled_A_part_fp64(cuda, et, cfg):
     print(f"case et={ET_NAMES[et]} cfg={cfg}", flush=True)
     J, d, Jr, dr, Ja, da, _ = run_case(cuda, et, cfg)
-    assert fp64_violation(J, Jr, Ja) <= 1.0, worst(J, Jr, Ja)
-    assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+    assert fp64_violation(J, Jr, _fl(et, Jr, Ja)) <= 1.0, worst(J, Jr, Ja)
+    assert fp64_violation(d, dr, _fl(et, dr, da)) <= 1.0, worst(d, dr, da)
 
 
 @pytest.mark.parametrize("cfg", [{"upwind": "positive", "source": (0.1, 0.2, -0.3)},
@@ -68,8 +77,8 @@ def test_coupled_time_dependent(cuda, et, cfg):
     print(f"case et={ET_NAMES[et]} cfg={cfg}", flush=True)
     J, d, Jr, dr, Ja, da, _ = run_case(cuda, et, cfg, time_dep=True, parts=PART_A | PART_M | PART_RHS,
                                        s=(0.7, 1.3, -0.7))
-    assert fp64_violation(J, Jr, Ja) <= 1.0, worst(J, Jr, Ja)
-    assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+    assert fp64_violation(J, Jr, _fl(et, Jr, Ja)) <= 1.0, worst(J, Jr, Ja)
+    assert fp64_violation(d, dr, _fl(et, dr, da)) <= 1.0, worst(d, dr, da)
 
 
 @pytest.mark.parametrize("cfg", [{"upwind": "positive"}, {"upwind": "skewed", "stab": "flow"}], ids=_cid)
@@ -77,8 +86,8 @@ def test_coupled_time_dependent(cuda, et, cfg):
 def test_coupled_fp32_jacobian(cuda, et, cfg):
     J, d, Jr, dr, Ja, da, _ = run_case(cuda, et, cfg, jprec=F32)
     assert J.dtype == np.float32
-    assert fp32_violation(J, Jr, Ja) <= 1.0, worst(J, Jr, Ja, FP32_REL, FP32_ABS)
-    assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+    assert fp32_violation(J, Jr, _fl(et, Jr, Ja, FP32_ABS)) <= 1.0, worst(J, Jr, Ja, FP32_REL, FP32_ABS)
+    assert fp64_violation(d, dr, _fl(et, dr, da)) <= 1.0, worst(d, dr, da)
 
 
 @pytest.mark.parametrize("et", ETS, ids=lambda e: ET_NAMES[e])
@@ -88,8 +97,8 @@ def test_coupled_repeatable_and_larger_mesh(cuda, et):
     mesh = small_mesh(et, n=8, seed=11, jitter=0.08)
     cfg = {"upwind": "positive", "stab": "flow"}
     J, d, Jr, dr, Ja, da, disc = run_case(cuda, et, cfg, mesh=mesh, seed=12)
-    assert fp64_violation(J, Jr, Ja) <= 1.0, worst(J, Jr, Ja)
-    assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+    assert fp64_violation(J, Jr, _fl(et, Jr, Ja)) <= 1.0, worst(J, Jr, Ja)
+    assert fp64_violation(d, dr, _fl(et, dr, da)) <= 1.0, worst(d, dr, da)
     u = torch.from_numpy(M.random_state(mesh, 12)).to(cuda)
     J1, d1 = disc.assemble(u)
     J2, d2 = disc.assemble(u)
@@ -145,7 +154,7 @@ def test_regular_upwind_vs_oracle(cuda, et, cfg):
     tol = 1e-12 * np.abs(Jr) + 1e-13 * Ja + floor
     print(f"entries on the floor: {int(np.sum((np.abs(J - Jr) > tol - floor) & (np.abs(J - Jr) <= tol)))}")
     assert np.max(np.abs(J - Jr) / tol) <= 1.0, worst(J, Jr, Ja)
-    assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+    assert fp64_violation(d, dr, _fl(et, dr, da)) <= 1.0, worst(d, dr, da)
     for jprec in (F64, F32):
         J, d, Jr, dr, Ja, da, _ = run_case(cuda, et, dict(cfg, source=(0.2, -0.3, 0.0)), mesh=mesh, seed=14,
                                            time_dep=True, parts=PART_A | PART_M | PART_RHS, s=(0.9, 1.2, -0.4),
@@ -156,7 +165,7 @@ def test_regular_upwind_vs_oracle(cuda, et, cfg):
         else:
             tol = FP32_ABS * Ja + FP32_REL * np.abs(Jr) + 2.0 ** -24 * np.abs(Jr).max()
             assert np.max(np.abs(J.astype(np.float64) - Jr) / tol) <= 1.0, worst(J, Jr, Ja, FP32_REL, FP32_ABS)
-        assert fp64_violation(d, dr, da) <= 1.0, worst(d, dr, da)
+        assert fp64_violation(d, dr, _fl(et, dr, da)) <= 1.0, worst(d, dr, da)
 
 
 def test_regular_upwind_is_2d_only(cuda):

@@ -8,7 +8,7 @@ rhs parts, time-dependent stabilisation); the pattern against the union of
 the types' adjacencies; one element type given as two groups against the
 single group bitwise (the same sums in the same order), on the generic and
 the coupled path; the coupled path (positive upwind, FLOW) on
-quadrilaterals + triangles; function-valued viscosity and source, group
+every mixed mesh; function-valued viscosity and source, group
 by group; the skewed-upwind record and replay across groups; a Newton
 solve; and the refusals.
 
@@ -99,14 +99,15 @@ def _cid(c):
 
 @pytest.mark.parametrize("jprec", [F64, F32], ids=["fp64", "fp32"])
 @pytest.mark.parametrize("cfg", COUPLED, ids=_cid)
-@pytest.mark.parametrize("name", ["quad_tri", "tri_quad"])
+@pytest.mark.parametrize("name", sorted(CASES))
 def test_mixed_mesh_coupled_parity(cuda, name, cfg, jprec):
     """The coupled path (positive upwind, FLOW) group by group."""
     import torch
 
     print(f"case mixed={name} cfg={cfg} jprec={jprec}", flush=True)
     mm = CASES[name]()
-    cfg = dict(cfg, source=(0.3, -0.2))
+    dim = mm.dim
+    cfg = dict(cfg, source=(0.3, -0.2, 0.1)[:dim])
     disc = _disc(mm, cfg)
     rowptr, colidx = disc.pattern()
     u = M.random_state(mm, 6)
@@ -117,17 +118,18 @@ def test_mixed_mesh_coupled_parity(cuda, name, cfg, jprec):
                          s_f=s[2], parts=parts, jprec=jprec)
     assert disc.assembly_path_used() == 4
     J, d = J.cpu().numpy(), d.cpu().numpy()
-    p = oracle_params(2, cfg)
+    p = oracle_params(dim, cfg)
     kw = dict(uold=uold, dt=dt, s_a=s[0], s_m=s[1], parts=parts)
     Jr, dr = O.assemble_mixed(p, mm.groups, mm.xyz, u, rowptr, colidx, s_f=s[2], **kw)
     Ja, da = O.assemble_abs_mixed(p, mm.groups, mm.xyz, u, rowptr, colidx, **kw)
     da = da + np.abs(dr)
-    v, where = _viol(d, dr, da, FP64_REL, FP64_ABS, 0.0)
+    floor = PRISM_FLOOR if any(et == M.PRISM for et, _ in mm.groups) else 0.0
+    v, where = _viol(d, dr, da, FP64_REL, FP64_ABS, floor)
     assert v <= 1.0, where
     if jprec == F64:
-        v, where = _viol(J, Jr, Ja, FP64_REL, FP64_ABS, 0.0)
+        v, where = _viol(J, Jr, Ja, FP64_REL, FP64_ABS, floor)
     else:
-        v, where = _viol(J, Jr, Ja, FP32_REL, FP32_ABS, 0.0)
+        v, where = _viol(J, Jr, Ja, FP32_REL, FP32_ABS, floor)
     assert v <= 1.0, where
 
 
@@ -237,10 +239,8 @@ def _coupled(cfg):
     return cfg["upwind"] == "positive" or cfg.get("stab") == "flow"
 
 
-# the coupled path has no prisms (refused, see test_refusals)
 IMPORT_CASES = [(n, c) for n in sorted(CASES)
-                for c in ({"upwind": "full"}, {"upwind": "lps"}, {"upwind": "positive"}, {"upwind": "lps", "stab": "flow"})
-                if not (_coupled(c) and "prism" in n)]
+                for c in ({"upwind": "full"}, {"upwind": "lps"}, {"upwind": "positive"}, {"upwind": "lps", "stab": "flow"})]
 
 
 @pytest.mark.parametrize("name,cfg", IMPORT_CASES, ids=[f"{n}-{_cid(c)}" for n, c in IMPORT_CASES])
@@ -298,17 +298,6 @@ def test_refusals(cuda):
     disc = _disc(mm, {"upwind": "full"})
     u = torch.from_numpy(M.random_state(mm, 2)).to(cuda)
     disc.assemble(u, jprec=F32)
-    # the coupled path has no prisms
-    disc.set_stabilization("flow")
-    with pytest.raises(NSFV1Error) as e:
-        disc.assemble(u, jprec=F64)
-    assert e.value.name == "NOT_IMPLEMENTED"
-    disc.set_stabilization("fields")
-    disc.set_upwind("positive")
-    with pytest.raises(NSFV1Error) as e:
-        disc.assemble(u, jprec=F64)
-    assert e.value.name == "NOT_IMPLEMENTED"
-    disc.set_upwind("full")
     with pytest.raises(NSFV1Error) as e:
         disc.set_outflow(np.array([0], np.int32), np.array([0], np.int32))
     assert e.value.name == "NOT_IMPLEMENTED"

@@ -2,8 +2,8 @@
 pattern bit-exactly, the A part in FP64 for every upwind, stabilisation and
 flag case of the tetrahedral/hexahedral parity suite, the time-dependent
 A + M + rhs assembly, the FP32 Jacobian bound, the no-allocation entry,
-function-valued imports, and the refused combinations (coupled path,
-the coupled configurations' stabilised ip velocity, pyramids).
+function-valued imports, and the refused combinations (the 3D regular
+upwind, pyramids).
 
 Tolerance: the parity rule (|a - b| <= 1e-12 |b| + 1e-13 S) plus an
 absolute floor of 1e-15 max|b| over the array.  On prisms whose vertical
@@ -119,10 +119,10 @@ def test_refused_combinations(cuda):
     configure_disc(disc, {"upwind": "full"})
     disc.set_mesh(mesh)
     u = torch.from_numpy(M.random_state(mesh, 2)).to(cuda)
+    # the coupled path runs on prisms (tests/test_gpu_coupled.py)
     disc.set_stabilization("flow")
-    with pytest.raises(NSFV1Error) as e:
-        disc.assemble(u, jprec=F64)
-    assert e.value.name == "NOT_IMPLEMENTED"
+    disc.assemble(u, jprec=F64)
+    assert disc.assembly_path_used() == 4
     disc.set_stabilization("fields")
     # boundary discs run on prisms (tests/test_gpu_outflow.py)
     belem, bside = M.boundary_sides(mesh, mesh.boundary["lid"])
@@ -130,12 +130,15 @@ def test_refused_combinations(cuda):
     disc.set_outflow(belem, bside)
     disc.assemble(u, jprec=F64)
     disc.set_outflow([], [])
-    # exports run on prisms (tests/test_gpu_export.py); the stabilised ip
-    # velocity of the coupled configurations does not
+    # exports run on prisms (tests/test_gpu_export.py), the coupled
+    # configurations' stabilised ip velocity too
     assert disc.export(u, "velocity", "scv").shape == (mesh.ne, 6, 3)
     disc.set_upwind("positive")
+    assert disc.export(u, "velocity_ip", "scvf").shape == (mesh.ne, 9, 3)
+    # the regular upwind is 2D only (upwind.cpp:806)
+    disc.set_upwind("regular")
     with pytest.raises(NSFV1Error) as e:
-        disc.export(u, "velocity_ip", "scvf")
+        disc.assemble(u, jprec=F64)
     assert e.value.name == "NOT_IMPLEMENTED"
     # pyramids are refused when the mesh is set
     pyr = NavierStokesFV1("u,v,w,p", "Inner")
