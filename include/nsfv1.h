@@ -191,8 +191,9 @@ int nsfv1_set_mesh(nsfv1_handle* h, int elem_type, const double* xyz, int64_t nv
    the stream, the first group writing every row and the others adding
    theirs.  Integration-point imports and nsfv1_ip_positions concatenate
    the groups' blocks in group order ([ne_g][nip_g] ..., each with dim
-   where it applies).  Boundary discs, data exports and batched instances
-   refuse mixed meshes with NSFV1_ERR_NOT_IMPLEMENTED. */
+   where it applies).  Boundary discs take (element, side) pairs with the
+   global element numbers.  Data exports and batched instances refuse mixed
+   meshes with NSFV1_ERR_NOT_IMPLEMENTED. */
 int nsfv1_set_mesh_mixed(nsfv1_handle* h, const double* xyz, int64_t nv, int ngroups, const int* elem_type,
                          const int32_t* const* conn, const int64_t* ne);
 

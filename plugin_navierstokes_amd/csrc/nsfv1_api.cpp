@@ -93,9 +93,18 @@ struct MGLevel {
     int jprec = 0;
 };
 
+// boundary discs (no-normal-stress outflow, symmetry, sliding wall) of one
+// element group: per boundary node, its boundary faces as (incidence, side)
+// items
+struct BndList {
+    DevBuf ptr, node, inc, side;
+    int64_t nbn = 0;
+};
+
 // A further element type of a mixed mesh (the first type lives in the
 // handle itself): its elements, their node incidences with column slots in
-// the mesh's common pattern, per-element geometry and launch plans.
+// the mesh's common pattern, per-element geometry, launch plans and
+// boundary lists.
 struct ElemGroup {
     int et = -1;
     int64_t ne = 0, ninc = 0;
@@ -103,6 +112,7 @@ struct ElemGroup {
     std::vector<int64_t> h_node_inc;
     std::vector<int32_t> h_inc_info, h_inc_elem;
     Plan plan32, plan64;
+    BndList bnd[BND_KINDS];
 };
 
 // enum values of the configuration state
@@ -138,12 +148,8 @@ struct nsfv1_handle {
     // element types after the first of a mixed mesh (empty: one type)
     std::vector<std::unique_ptr<ElemGroup>> groups;
     bool mixed() const { return !groups.empty(); }
-    // boundary discs (no-normal-stress outflow, symmetry, sliding wall): per
-    // boundary node, its boundary faces as (incidence, side) items
-    struct BndList {
-        DevBuf ptr, node, inc, side;
-        int64_t nbn = 0;
-    } bnd[BND_KINDS];
+    // boundary discs of the handle's own element group
+    BndList bnd[BND_KINDS];
     double slide_k = 0.0, slide_lim = 0.0;
     std::vector<int64_t> h_rowptr, h_node_inc;
     int max_threads = 256;
@@ -1069,17 +1075,16 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
     if (parts & NSFV1_PART_A) {
         int rc = check_setup(h);
         if (rc) return rc;
-        if (h->nu_elem.p)
-            for (int kind = 0; kind < BND_KINDS; ++kind)
-                if (h->bnd[kind].nbn)
-                    return fail(NSFV1_ERR_NOT_IMPLEMENTED,
-                                "boundary discs with per-instance viscosity (batched instances) are not provided");
-        if (h->ip_imports()) {
-            for (int kind = 0; kind < BND_KINDS; ++kind)
-                if (h->bnd[kind].nbn)
-                    return fail(NSFV1_ERR_NOT_IMPLEMENTED,
-                                "boundary discs with imports at the integration points are not provided");
+        bool bnd = false;
+        for (int kind = 0; kind < BND_KINDS; ++kind) {
+            bnd |= h->bnd[kind].nbn > 0;
+            for (auto& g : h->groups) bnd |= g->bnd[kind].nbn > 0;
         }
+        if (bnd && h->nu_elem.p)
+            return fail(NSFV1_ERR_NOT_IMPLEMENTED,
+                        "boundary discs with per-instance viscosity (batched instances) are not provided");
+        if (bnd && h->ip_imports())
+            return fail(NSFV1_ERR_NOT_IMPLEMENTED, "boundary discs with imports at the integration points are not provided");
     }
     Plan& P = jprec == NSFV1_F64 ? h->plan64 : h->plan32;
     if (!P.valid) {
@@ -1279,16 +1284,22 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
         h->last_path = 1;
     }
     if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("assemble launch: ") + hipGetErrorString(e));
-    for (int kind = 0; kind < BND_KINDS; ++kind) {
-        const auto& B = h->bnd[kind];
-        if (B.nbn == 0 || !(parts & NSFV1_PART_A)) continue;
-        OutflowArgs oa{h->xyz.as<double>(), h->conn.as<int32_t>(), h->rowptr.as<int64_t>(), h->inc_elem.as<int32_t>(),
-                       h->inc_info.as<int32_t>(), h->inc_pos.as<uint64_t>(), B.ptr.as<int32_t>(), B.node.as<int64_t>(),
-                       B.inc.as<int32_t>(), B.side.as<int32_t>(), B.nbn, h->nu, h->rho, s_a, h->laplace ? 1 : 0,
-                       h->stokes ? 1 : 0, kind, h->slide_k, h->slide_lim};
-        e = launch_outflow(h->et, jprec, oa, u, jvals, defect, h->errflag.as<int>(), (hipStream_t)stream);
-        if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("boundary disc launch: ") + hipGetErrorString(e));
-    }
+    // boundary discs, group by group in order (each launch adds into the
+    // rows of its boundary nodes after the launches before it)
+    for (int kind = 0; kind < BND_KINDS && (parts & NSFV1_PART_A); ++kind)
+        for (size_t g = 0; g <= h->groups.size(); ++g) {
+            const bool own = g == 0;
+            const ElemGroup* G = own ? nullptr : h->groups[g - 1].get();
+            const BndList& B = own ? h->bnd[kind] : G->bnd[kind];
+            if (B.nbn == 0) continue;
+            OutflowArgs oa{h->xyz.as<double>(), (own ? h->conn : G->conn).as<int32_t>(), h->rowptr.as<int64_t>(),
+                           (own ? h->inc_elem : G->inc_elem).as<int32_t>(), (own ? h->inc_info : G->inc_info).as<int32_t>(),
+                           (own ? h->inc_pos : G->inc_pos).as<uint64_t>(), B.ptr.as<int32_t>(), B.node.as<int64_t>(),
+                           B.inc.as<int32_t>(), B.side.as<int32_t>(), B.nbn, h->nu, h->rho, s_a, h->laplace ? 1 : 0,
+                           h->stokes ? 1 : 0, kind, h->slide_k, h->slide_lim};
+            e = launch_outflow(own ? h->et : G->et, jprec, oa, u, jvals, defect, h->errflag.as<int>(), (hipStream_t)stream);
+            if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("boundary disc launch: ") + hipGetErrorString(e));
+        }
     if (h->upw_mode == NSFV1_UPWIND_RECORD && (parts & NSFV1_PART_A)) h->upw_recorded = true;
     return NSFV1_OK;
 }
@@ -1354,31 +1365,52 @@ static void side_corners(int s, int* n, int* c)
 }
 
 // Groups the corners of the listed (element, side) pairs by node, keeping the
-// input order within a node, and uploads them as boundary list `kind`.
+// input order within a node, and uploads them as boundary list `kind` of
+// each element group (a mixed mesh numbers its elements group after group).
 static int set_boundary_list(nsfv1_handle* h, int kind, const int32_t* elem, const int32_t* side, int64_t n)
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
     if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
-    auto& B = h->bnd[kind];
-    B.nbn = 0;
+    h->bnd[kind].nbn = 0;
+    for (auto& g : h->groups) g->bnd[kind].nbn = 0;
     if (n == 0) return NSFV1_OK;
-    if (h->mixed()) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "boundary discs on mixed-element meshes are not provided");
     if (n < 0 || !elem || !side) return fail(NSFV1_ERR_ARGUMENT, "bad boundary side arrays");
-    const int nsh = nsh_of(h->et);
-    const int nside = h->et == ET_HEX ? 6 : (h->et == ET_PRISM ? 5 : (h->et == ET_QUAD ? 4 : (h->et == ET_TET ? 4 : 3)));
-    std::vector<int32_t> cn;  // conn rows are needed: download once
-    cn.resize((size_t)h->ne * nsh);
-    if (hipMemcpy(cn.data(), h->conn.p, sizeof(int32_t) * cn.size(), hipMemcpyDeviceToHost) != hipSuccess)
-        return fail(NSFV1_ERR_DEVICE, "connectivity download");
-    // (node, item) pairs in input order, grouped by node
-    std::vector<std::pair<int64_t, std::pair<int32_t, int32_t>>> items;
+    // the element groups: type, element range, connectivity, incidences, list
+    struct GV {
+        int et;
+        int64_t e0, ne;
+        const DevBuf* conn;
+        const std::vector<int64_t>* node_inc;
+        const std::vector<int32_t>* inc_elem;
+        BndList* B;
+        std::vector<int32_t> cn;  // connectivity, downloaded when first needed
+        std::vector<std::pair<int64_t, std::pair<int32_t, int32_t>>> items;  // (node, (incidence, side))
+    };
+    std::vector<GV> gv;
+    gv.push_back({h->et, 0, h->ne, &h->conn, &h->h_node_inc, &h->h_inc_elem, &h->bnd[kind], {}, {}});
+    int64_t e0 = h->ne;
+    for (auto& g : h->groups) {
+        gv.push_back({g->et, e0, g->ne, &g->conn, &g->h_node_inc, &g->h_inc_elem, &g->bnd[kind], {}, {}});
+        e0 += g->ne;
+    }
     for (int64_t b = 0; b < n; ++b) {
-        const int64_t e = elem[b];
+        const int64_t eg = elem[b];
         const int s = side[b];
-        if (e < 0 || e >= h->ne) return fail(NSFV1_ERR_ARGUMENT, "boundary element out of range");
+        if (eg < 0 || eg >= e0) return fail(NSFV1_ERR_ARGUMENT, "boundary element out of range");
+        size_t g = 0;
+        while (eg >= gv[g].e0 + gv[g].ne) ++g;
+        GV& G = gv[g];
+        const int64_t e = eg - G.e0;
+        const int nsh = nsh_of(G.et);
+        const int nside = G.et == ET_HEX ? 6 : (G.et == ET_PRISM ? 5 : (G.et == ET_QUAD ? 4 : (G.et == ET_TET ? 4 : 3)));
         if (s < 0 || s >= nside) return fail(NSFV1_ERR_ARGUMENT, "boundary side out of range");
+        if (G.cn.empty()) {
+            G.cn.resize((size_t)G.ne * nsh);
+            if (hipMemcpy(G.cn.data(), G.conn->p, sizeof(int32_t) * G.cn.size(), hipMemcpyDeviceToHost) != hipSuccess)
+                return fail(NSFV1_ERR_DEVICE, "connectivity download");
+        }
         int k = 0, c[4];
-        switch (h->et) {
+        switch (G.et) {
         case 0: side_corners<0>(s, &k, c); break;
         case 1: side_corners<1>(s, &k, c); break;
         case 2: side_corners<2>(s, &k, c); break;
@@ -1386,33 +1418,38 @@ static int set_boundary_list(nsfv1_handle* h, int kind, const int32_t* elem, con
         default: side_corners<4>(s, &k, c); break;
         }
         for (int j = 0; j < k; ++j) {
-            const int64_t v = cn[e * nsh + c[j]];
+            const int64_t v = G.cn[e * nsh + c[j]];
             int64_t q = -1;
-            for (int64_t t = h->h_node_inc[v]; t < h->h_node_inc[v + 1]; ++t)
-                if (h->h_inc_elem[t] == e) { q = t; break; }
+            for (int64_t t = (*G.node_inc)[v]; t < (*G.node_inc)[v + 1]; ++t)
+                if ((*G.inc_elem)[t] == e) { q = t; break; }
             if (q < 0) return fail(NSFV1_ERR_ARGUMENT, "boundary side corner without incidence");
-            items.push_back({v, {(int32_t)q, (int32_t)s}});
+            G.items.push_back({v, {(int32_t)q, (int32_t)s}});
         }
     }
-    std::stable_sort(items.begin(), items.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
-    std::vector<int64_t> nodes;
-    std::vector<int32_t> ptr{0}, iq, is;
-    for (size_t i = 0; i < items.size(); ++i) {
-        if (nodes.empty() || nodes.back() != items[i].first) {
-            if (!nodes.empty()) ptr.push_back((int32_t)iq.size());
-            nodes.push_back(items[i].first);
-        }
-        iq.push_back(items[i].second.first);
-        is.push_back(items[i].second.second);
-    }
-    ptr.push_back((int32_t)iq.size());
-    auto up = [](DevBuf& b, const void* src, size_t bytes) {
-        return b.alloc(bytes) == hipSuccess && hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    auto up = [](DevBuf& d, const void* src, size_t bytes) {
+        return d.alloc(bytes) == hipSuccess && hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
     };
-    if (!up(B.ptr, ptr.data(), sizeof(int32_t) * ptr.size()) || !up(B.node, nodes.data(), sizeof(int64_t) * nodes.size()) ||
-        !up(B.inc, iq.data(), sizeof(int32_t) * iq.size()) || !up(B.side, is.data(), sizeof(int32_t) * is.size()))
-        return fail(NSFV1_ERR_DEVICE, "boundary list upload");
-    B.nbn = (int64_t)nodes.size();
+    for (GV& G : gv) {
+        if (G.items.empty()) continue;
+        std::stable_sort(G.items.begin(), G.items.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+        std::vector<int64_t> nodes;
+        std::vector<int32_t> ptr{0}, iq, is;
+        for (size_t i = 0; i < G.items.size(); ++i) {
+            if (nodes.empty() || nodes.back() != G.items[i].first) {
+                if (!nodes.empty()) ptr.push_back((int32_t)iq.size());
+                nodes.push_back(G.items[i].first);
+            }
+            iq.push_back(G.items[i].second.first);
+            is.push_back(G.items[i].second.second);
+        }
+        ptr.push_back((int32_t)iq.size());
+        BndList& B = *G.B;
+        if (!up(B.ptr, ptr.data(), sizeof(int32_t) * ptr.size()) ||
+            !up(B.node, nodes.data(), sizeof(int64_t) * nodes.size()) ||
+            !up(B.inc, iq.data(), sizeof(int32_t) * iq.size()) || !up(B.side, is.data(), sizeof(int32_t) * is.size()))
+            return fail(NSFV1_ERR_DEVICE, "boundary list upload");
+        B.nbn = (int64_t)nodes.size();
+    }
     return NSFV1_OK;
 }
 

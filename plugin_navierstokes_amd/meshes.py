@@ -29,15 +29,20 @@ SIDES = {
 
 def boundary_sides(mesh: "Mesh", nodes) -> tuple:
     """(element, side) pairs of every element side whose corners all lie in
-    `nodes` (e.g. the outflow plane).  Returns int32 arrays."""
+    `nodes` (e.g. the outflow plane).  Returns int32 arrays.  On a mixed
+    mesh the elements are numbered group after group."""
     mask = np.zeros(mesh.nv, dtype=bool)
     mask[np.asarray(nodes, dtype=np.int64)] = True
     belem, bside = [], []
-    for s, corners in enumerate(SIDES[mesh.et]):
-        on = np.all(mask[mesh.conn[:, list(corners)]], axis=1)
-        e = np.nonzero(on)[0]
-        belem.append(e)
-        bside.append(np.full(len(e), s))
+    groups = mesh.groups if getattr(mesh, "groups", None) is not None else [(mesh.et, mesh.conn)]
+    e0 = 0
+    for et, conn in groups:
+        for s, corners in enumerate(SIDES[et]):
+            on = np.all(mask[conn[:, list(corners)]], axis=1)
+            e = np.nonzero(on)[0]
+            belem.append(e + e0)
+            bside.append(np.full(len(e), s))
+        e0 += conn.shape[0]
     belem = np.concatenate(belem)
     bside = np.concatenate(bside)
     o = np.lexsort((bside, belem))

@@ -9,7 +9,7 @@ the types' adjacencies; one element type given as two groups against the
 single group bitwise (the same sums in the same order), on the generic and
 the coupled path; the coupled path (positive upwind, FLOW) on
 every mixed mesh; function-valued viscosity and source, group
-by group; the skewed-upwind record and replay across groups; a Newton
+by group; boundary discs on sides of both element types; the skewed-upwind record and replay across groups; a Newton
 solve; and the refusals.
 
 Tolerance: the parity rule (|a - b| <= 1e-12 |b| + 1e-13 S in FP64,
@@ -291,6 +291,69 @@ def test_mixed_mesh_function_imports(cuda, name, cfg):
         assert v <= 1.0, where
 
 
+def _split(mm, belem, bside):
+    """Global boundary (element, side) lists as per-group local lists."""
+    out, e0 = [], 0
+    for et, conn in mm.groups:
+        m = (belem >= e0) & (belem < e0 + conn.shape[0])
+        out.append((et, conn, (belem[m] - e0).astype(np.int32), bside[m].astype(np.int32)))
+        e0 += conn.shape[0]
+    return out
+
+
+@pytest.mark.parametrize("jprec", [F64, F32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("kind", ["outflow", "symmetry", "sliding"])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_mixed_mesh_boundary_discs(cuda, name, kind, jprec):
+    """Boundary discs on sides of both element types (the y = max side meets
+    every group): each group's boundary faces are added after the element
+    assembly, group by group, against the oracle's discs per group."""
+    import torch
+
+    print(f"case mixed={name} boundary={kind} jprec={jprec}", flush=True)
+    mm = CASES[name]()
+    dim = mm.dim
+    cfg = {"upwind": "full"}
+    disc = _disc(mm, cfg)
+    top = np.nonzero(np.isclose(mm.xyz[:, 1], mm.xyz[:, 1].max()))[0]  # y = max meets every group
+    belem, bside = M.boundary_sides(mm, top)
+    parts_g = _split(mm, belem, bside)
+    assert sum(len(b) > 0 for _, _, b, _ in parts_g) == 2  # both groups have sides on it
+    if kind == "outflow":
+        disc.set_outflow(belem, bside)
+    elif kind == "symmetry":
+        disc.set_symmetry(belem, bside)
+    else:
+        disc.set_sliding_wall(belem, bside, 1.3, 0.2)
+    u = M.random_state(mm, 21)
+    J, d = disc.assemble(torch.from_numpy(u).to(cuda), s_a=0.8, parts=PART_A, jprec=jprec)
+    J, d = J.cpu().numpy().astype(np.float64), d.cpu().numpy()
+    rp, ci = disc.pattern()
+    p = oracle_params(dim, cfg)
+    Jr, dr = O.assemble_mixed(p, mm.groups, mm.xyz, u, rp, ci, s_a=0.8, parts=1)
+    Ja, da = O.assemble_abs_mixed(p, mm.groups, mm.xyz, u, rp, ci, s_a=0.8, parts=1)
+    Jo = do = 0.0
+    for et, conn, be, bs in parts_g:
+        if len(be) == 0:
+            continue
+        kw = dict(kind=kind, k=1.3, lim=0.2) if kind == "sliding" else dict(kind=kind)
+        Jx, dx = O.outflow(p, et, mm.xyz, conn, be, bs, u, s_a=0.8, rowptr=rp, colidx=ci, **kw)
+        Jr, dr = Jr + Jx, dr + dx
+        Jo, do = Jo + np.abs(Jx), do + np.abs(dx)
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    rmax = np.zeros(len(rp) - 1)
+    np.maximum.at(rmax, rows, np.abs(Jo))
+    floor = PRISM_FLOOR if any(et == M.PRISM for et, _ in mm.groups) else 0.0
+    SJ = Ja + rmax[rows]
+    if jprec == F64:
+        v, where = _viol(J, Jr, SJ, FP64_REL, FP64_ABS, floor)
+    else:
+        v, where = _viol(J, Jr, SJ, FP32_REL, FP32_ABS, floor)
+    assert v <= 1.0, where
+    v, where = _viol(d, dr, da + np.abs(do), FP64_REL, FP64_ABS, floor)
+    assert v <= 1.0, where
+
+
 def test_refusals(cuda):
     import torch
 
@@ -298,9 +361,6 @@ def test_refusals(cuda):
     disc = _disc(mm, {"upwind": "full"})
     u = torch.from_numpy(M.random_state(mm, 2)).to(cuda)
     disc.assemble(u, jprec=F32)
-    with pytest.raises(NSFV1Error) as e:
-        disc.set_outflow(np.array([0], np.int32), np.array([0], np.int32))
-    assert e.value.name == "NOT_IMPLEMENTED"
     with pytest.raises(NSFV1Error) as e:
         disc.export(u, "velocity", "scvf")
     assert e.value.name == "NOT_IMPLEMENTED"
