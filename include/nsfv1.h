@@ -192,8 +192,10 @@ int nsfv1_set_mesh(nsfv1_handle* h, int elem_type, const double* xyz, int64_t nv
    theirs.  Integration-point imports and nsfv1_ip_positions concatenate
    the groups' blocks in group order ([ne_g][nip_g] ..., each with dim
    where it applies).  Boundary discs take (element, side) pairs with the
-   global element numbers.  Data exports and batched instances refuse mixed
-   meshes with NSFV1_ERR_NOT_IMPLEMENTED. */
+   global element numbers.  Data exports at SCVF ips and corners write the
+   groups' [ne_g][points_g][components] blocks in order; exports at local
+   points and batched instances refuse mixed meshes with
+   NSFV1_ERR_NOT_IMPLEMENTED. */
 int nsfv1_set_mesh_mixed(nsfv1_handle* h, const double* xyz, int64_t nv, int ngroups, const int* elem_type,
                          const int32_t* const* conn, const int64_t* ne);
 

@@ -498,7 +498,8 @@ class NavierStokesFV1:
         """Data export ("velocity", "velocity_grad", "velocity_ip", "pressure",
         "pressure_grad") at the SCVF ips, the corners ("scv") or the given
         local points of every element: a CUDA float64 tensor
-        [ne, points, components]."""
+        [ne, points, components] ([points, components] on a mixed mesh,
+        group after group)."""
         import torch
 
         lips = None if local_ips is None else np.ascontiguousarray(local_ips, dtype=np.float64)
@@ -512,6 +513,8 @@ class NavierStokesFV1:
         if check:
             self.sync_errors(stream)
         ncomp = self.dim * self.dim if what == "velocity_grad" else (1 if what == "pressure" else self.dim)
+        if len(getattr(self, "_groups", [])) > 1:
+            return out.view(-1, ncomp)  # mixed mesh: points group after group
         return out.view(self.ne, -1, ncomp)
 
     def set_assembly_path(self, path: int):

@@ -1485,12 +1485,13 @@ int nsfv1_export_size(nsfv1_handle* h, int what, int at, int nip, int64_t* count
     if (at < NSFV1_AT_SCVF || at > NSFV1_AT_LOCAL) return fail(NSFV1_ERR_ARGUMENT, "unknown export points");
     if (at == NSFV1_AT_LOCAL && (nip < 1 || nip > EX_MAX_LOCAL))
         return fail(NSFV1_ERR_ARGUMENT, "local points per call must be 1..32");
-    const int dim = (h->et == ET_TRI || h->et == ET_QUAD) ? 2 : 3;
-    const int nsh = nsh_of(h->et);
-    const int nscvf = nip_of(h->et);
-    const int npt = at == NSFV1_AT_SCVF ? nscvf : (at == NSFV1_AT_SCV ? nsh : nip);
+    if (at == NSFV1_AT_LOCAL && h->mixed())
+        return fail(NSFV1_ERR_NOT_IMPLEMENTED, "exports at local points on mixed-element meshes are not provided "
+                                               "(local coordinates belong to one reference element)");
+    const int dim = h->dim;
     const int ncomp = what == NSFV1_EX_VELOCITY_GRAD ? dim * dim : (what == NSFV1_EX_PRESSURE ? 1 : dim);
-    *count = h->ne * npt * ncomp;
+    // group after group on a mixed mesh: [ne_g][points_g][ncomp] blocks
+    *count = (at == NSFV1_AT_LOCAL ? h->ne * nip : group_points(h, at == NSFV1_AT_SCVF ? 0 : 1)) * ncomp;
     return NSFV1_OK;
 }
 
@@ -1501,7 +1502,6 @@ int nsfv1_export(nsfv1_handle* h, int what, int at, const double* local_ips, int
     int rc = nsfv1_export_size(h, what, at, nip, &count);
     if (rc) return rc;
 
-    if (h->mixed()) return fail(NSFV1_ERR_NOT_IMPLEMENTED, "data exports on mixed-element meshes are not provided");
     if (!u || !out) return fail(NSFV1_ERR_ARGUMENT, "NULL buffer");
     if (at == NSFV1_AT_LOCAL && !local_ips) return fail(NSFV1_ERR_ARGUMENT, "local points are NULL");
     const int dim = (h->et == ET_TRI || h->et == ET_QUAD) ? 2 : 3;
@@ -1542,31 +1542,58 @@ int nsfv1_export(nsfv1_handle* h, int what, int at, const double* local_ips, int
         m.nu_ip = h->nu_ip.as<double>();
         m.f_scv = h->f_scv.as<double>();
         m.f_scvf = h->f_scvf.as<double>();
-        hipError_t e;
         if (coupled_config(h)) {
             k.conv_upwind = h->conv_upwind;
             k.pac = h->pac;
-            e = launch_assemble_coupled(h->et, NSFV1_F64, m, k, u, u_old, nullptr, nullptr, h->errflag.as<int>(), 0, 0,
-                                        h->ne, nullptr, out, (hipStream_t)stream);
-        } else {
-            e = launch_export_stab_vel(h->et, m, k, u, u_old, h->ne, out, h->errflag.as<int>(), (hipStream_t)stream);
+        }
+        auto launch = [&](int et, const KMeshPtrs& mm, int64_t ne, double* o) {
+            return coupled_config(h)
+                       ? launch_assemble_coupled(et, NSFV1_F64, mm, k, u, u_old, nullptr, nullptr, h->errflag.as<int>(), 0,
+                                                 0, ne, nullptr, o, (hipStream_t)stream)
+                       : launch_export_stab_vel(et, mm, k, u, u_old, ne, o, h->errflag.as<int>(), (hipStream_t)stream);
+        };
+        hipError_t e = launch(h->et, m, h->ne, out);
+        // further element groups: their elements' blocks follow in order
+        int64_t off_ip = h->ne * nip_of(h->et), off_sh = h->ne * nsh_count(h->et);
+        for (size_t g = 0; e == hipSuccess && g < h->groups.size(); ++g) {
+            const ElemGroup& G = *h->groups[g];
+            KMeshPtrs mg = m;
+            mg.conn = G.conn.as<int32_t>();
+            mg.l2inv = G.l2inv.as<double>();
+            mg.vol = G.vol.as<double>();
+            mg.affine = G.affine.as<uint8_t>();
+            mg.nu_ip = h->nu_ip.p ? h->nu_ip.as<double>() + off_ip : nullptr;
+            mg.f_scv = h->f_scv.p ? h->f_scv.as<double>() + off_sh * dim : nullptr;
+            mg.f_scvf = h->f_scvf.p ? h->f_scvf.as<double>() + off_ip * dim : nullptr;
+            e = launch(G.et, mg, G.ne, out + off_ip * dim);
+            off_ip += G.ne * nip_of(G.et);
+            off_sh += G.ne * nsh_count(G.et);
         }
         if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("export launch: ") + hipGetErrorString(e));
         return NSFV1_OK;
     }
     ExportArgs a{};
     a.xyz = h->xyz.as<double>();
-    a.conn = h->conn.as<int32_t>();
-    a.ne = h->ne;
     a.what = what;
     a.at = at;
-    a.npt = (int)(count / std::max<int64_t>(h->ne, 1) /
-                  (what == NSFV1_EX_VELOCITY_GRAD ? dim * dim : (what == NSFV1_EX_PRESSURE ? 1 : dim)));
     a.ncomp = what == NSFV1_EX_VELOCITY_GRAD ? dim * dim : (what == NSFV1_EX_PRESSURE ? 1 : dim);
     if (at == NSFV1_AT_LOCAL)
         for (int q = 0; q < nip; ++q)
             for (int i = 0; i < dim; ++i) a.lips[q][i] = local_ips[q * dim + i];
-    hipError_t e = launch_export_points(h->et, a, u, out, h->errflag.as<int>(), (hipStream_t)stream);
+    // group after group (the handle's own elements first)
+    auto launch = [&](int et, const DevBuf& conn, int64_t ne, double* o) {
+        a.conn = conn.as<int32_t>();
+        a.ne = ne;
+        a.npt = at == NSFV1_AT_SCVF ? nip_of(et) : (at == NSFV1_AT_SCV ? nsh_count(et) : nip);
+        return launch_export_points(et, a, u, o, h->errflag.as<int>(), (hipStream_t)stream);
+    };
+    hipError_t e = launch(h->et, h->conn, h->ne, out);
+    int64_t off = h->ne * (at == NSFV1_AT_SCVF ? nip_of(h->et) : (at == NSFV1_AT_SCV ? nsh_count(h->et) : nip)) * a.ncomp;
+    for (size_t g = 0; e == hipSuccess && g < h->groups.size(); ++g) {
+        const ElemGroup& G = *h->groups[g];
+        e = launch(G.et, G.conn, G.ne, out + off);
+        off += G.ne * (at == NSFV1_AT_SCVF ? nip_of(G.et) : nsh_count(G.et)) * a.ncomp;
+    }
     if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("export launch: ") + hipGetErrorString(e));
     return NSFV1_OK;
 }

@@ -1747,7 +1747,7 @@ export_stab_vel(KMeshPtrs M, KParams P, const double* __restrict__ u, const doub
 hipError_t launch_export_stab_vel(int et, const KMeshPtrs& m, const KParams& p, const double* u, const double* uold,
                                   int64_t ne, double* out, int* err, hipStream_t s)
 {
-    const int nsh = et == ET_HEX ? 8 : (et == ET_TRI ? 3 : 4);
+    const int nsh = et == ET_HEX ? 8 : (et == ET_PRISM ? 6 : (et == ET_TRI ? 3 : 4));
     const int64_t lanes = ne * nsh;
     if (lanes == 0) return hipSuccess;
     const unsigned blocks = (unsigned)((lanes + 255) / 256);

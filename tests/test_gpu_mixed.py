@@ -9,7 +9,8 @@ the types' adjacencies; one element type given as two groups against the
 single group bitwise (the same sums in the same order), on the generic and
 the coupled path; the coupled path (positive upwind, FLOW) on
 every mixed mesh; function-valued viscosity and source, group
-by group; boundary discs on sides of both element types; the skewed-upwind record and replay across groups; a Newton
+by group; boundary discs on sides of both element types; data exports
+group by group; the skewed-upwind record and replay across groups; a Newton
 solve; and the refusals.
 
 Tolerance: the parity rule (|a - b| <= 1e-12 |b| + 1e-13 S in FP64,
@@ -354,6 +355,33 @@ def test_mixed_mesh_boundary_discs(cuda, name, kind, jprec):
     assert v <= 1.0, where
 
 
+@pytest.mark.parametrize("cfg", [{"upwind": "full"}, {"upwind": "positive", "stab": "flow"}], ids=_cid)
+@pytest.mark.parametrize("at", ["scvf", "scv"])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_mixed_mesh_exports(cuda, name, at, cfg):
+    """Data exports on a mixed mesh: each group's [ne][points][components]
+    block in group order, against the oracle's exports per group (the
+    stabilised ip velocity through the generic or the coupled path)."""
+    import torch
+
+    print(f"case mixed={name} exports at {at} cfg={cfg}", flush=True)
+    mm = CASES[name]()
+    dim = mm.dim
+    disc = _disc(mm, cfg)
+    u = M.random_state(mm, 31)
+    uold = M.random_state(mm, 32)
+    ut, uot = torch.from_numpy(u).to(cuda), torch.from_numpy(uold).to(cuda)
+    p = oracle_params(dim, cfg)
+    for what in ("velocity", "velocity_grad", "velocity_ip", "pressure", "pressure_grad"):
+        got = disc.export(ut, what, at, u_old=uot, dt=0.02).cpu().numpy()
+        ncomp = dim * dim if what == "velocity_grad" else (1 if what == "pressure" else dim)
+        want = np.concatenate([O.export(p, et, mm.xyz, conn, u, what, at, None, uold, 0.02).reshape(-1, ncomp)
+                               for et, conn in mm.groups])
+        assert got.shape == want.shape, (what, got.shape, want.shape)
+        tol = 1e-12 * np.abs(want) + 1e-12 * np.abs(want).max()
+        assert np.max(np.abs(got - want) / tol) <= 1.0, what
+
+
 def test_refusals(cuda):
     import torch
 
@@ -361,8 +389,9 @@ def test_refusals(cuda):
     disc = _disc(mm, {"upwind": "full"})
     u = torch.from_numpy(M.random_state(mm, 2)).to(cuda)
     disc.assemble(u, jprec=F32)
+    # local points belong to one reference element
     with pytest.raises(NSFV1Error) as e:
-        disc.export(u, "velocity", "scvf")
+        disc.export(u, "velocity", "local", np.full((2, 3), 0.2))
     assert e.value.name == "NOT_IMPLEMENTED"
     # a pyramid group is refused like a pyramid mesh
     with pytest.raises(NSFV1Error) as e:

@@ -20,7 +20,7 @@ import pytest
 import oracle as O
 from plugin_navierstokes_amd import F32, F64, PART_A, PART_M, PART_RHS, NavierStokesFV1, NSFV1Error
 from plugin_navierstokes_amd import meshes as M
-from tests._parity import configure_disc, fp64_violation, worst
+from tests._parity import configure_disc, fp64_violation, oracle_params, worst
 from tests._parity import run_case as _run
 from tests.test_gpu_parity import CASES, _case_id
 
@@ -146,3 +146,21 @@ def test_refused_combinations(cuda):
     with pytest.raises(NSFV1Error) as e:
         pyr.set_mesh(et=5, xyz=xyz, conn=np.arange(5, dtype=np.int32)[None])
     assert e.value.name == "NOT_IMPLEMENTED"
+
+
+def test_stabilised_velocity_export_covers_every_prism(cuda):
+    """The stabilised ip velocity of every SCVF of a 128-prism mesh (one
+    launch lane per (element, corner): six per prism) against the oracle."""
+    import torch
+
+    mesh = M.cavity_3d(4, M.PRISM, 0.1, 5)
+    cfg = {"upwind": "full"}
+    disc = NavierStokesFV1("u,v,w,p", "Inner")
+    configure_disc(disc, cfg)
+    disc.set_mesh(mesh)
+    u = M.random_state(mesh, 7)
+    got = disc.export(torch.from_numpy(u).to(cuda), "velocity_ip", "scvf").cpu().numpy()
+    want = O.export(oracle_params(3, cfg), M.PRISM, mesh.xyz, mesh.conn, u, "velocity_ip", "scvf")
+    assert got.shape == want.shape == (mesh.ne, 9, 3)
+    tol = 1e-12 * np.abs(want) + 1e-12 * np.abs(want).max()
+    assert np.max(np.abs(got - want) / tol) <= 1.0
