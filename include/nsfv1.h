@@ -302,6 +302,20 @@ int nsfv1_set_symmetry(nsfv1_handle* h, const int32_t* elem, const int32_t* side
 int nsfv1_set_sliding_wall(nsfv1_handle* h, const int32_t* elem, const int32_t* side, int64_t n,
                            double sliding_factor, double sliding_limit);
 
+/* A function-valued viscosity on the boundary discs: the reference's
+   outflow and symmetry discs evaluate the master disc's viscosity at their
+   boundary-face ips (no_normal_stress_outflow_fv1.cpp:188-426,
+   symmetric_boundary_fv1.cpp:224-386).  nsfv1_bnd_ip_positions returns
+   those ips of boundary list `kind` (0 outflow, 1 symmetry, 2 sliding wall)
+   in input order, out[b][j][dim] for pair b and the side's j-th corner
+   (4 slots per pair, unused ones 0); the caller evaluates its function
+   there and hands the values to nsfv1_set_boundary_viscosity (nu[b][j],
+   NULL clears).  With nsfv1_set_kinematic_viscosity_ip in force, the
+   assembly refuses outflow and symmetry lists without them
+   (NSFV1_ERR_ARGUMENT).  A new list clears them. */
+int nsfv1_bnd_ip_positions(nsfv1_handle* h, int kind, double* out, int64_t n);
+int nsfv1_set_boundary_viscosity(nsfv1_handle* h, int kind, const double* nu, int64_t n);
+
 /* Data exports of the disc (navier_stokes_fv1.cpp:894-1491; the export
    objects of navier_stokes_fv1.cpp:1573-1577): values at points of every
    element, written to the device array out[ne][points][components] (FP64)

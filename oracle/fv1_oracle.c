@@ -1773,8 +1773,9 @@ int or_outflow_elem(const or_params* p, int et, const double* corners, int side,
     int rc = or_bnd_faces(et, corners, side, &B);
     if (rc) return rc;
     const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1, nl = nsh * nc;
-    const double nu = p->nu, rho = p->rho;
+    const double rho = p->rho;
     for (int f = 0; f < B.nbf; ++f) {
+        const double nu = p->nu_bf ? p->nu_bf[f] : p->nu;  /* the import at the face's ip */
         const double* n = B.n[f];
         const int row = B.node[f];
         double vel[3] = {0, 0, 0};
@@ -1848,7 +1849,9 @@ static int bnd_scatter(int kind, const or_params* p, int et, const double* xyz, 
         }
         memset(Jl, 0, sizeof(double) * nl * nl);
         memset(dl, 0, sizeof(double) * nl);
-        int rc = bnd_elem(kind, p, et, co, bside[b], ul, k, lim, J ? Jl : NULL, d ? dl : NULL);
+        or_params q = *p;
+        if (p->nu_bf) q.nu_bf = p->nu_bf + b * 4;  /* the side's own faces */
+        int rc = bnd_elem(kind, &q, et, co, bside[b], ul, k, lim, J ? Jl : NULL, d ? dl : NULL);
         if (rc) return rc;
         for (int i = 0; i < nsh; ++i)
             for (int a = 0; a < nc; ++a) {
@@ -1890,8 +1893,8 @@ int or_symmetry_elem(const or_params* p, int et, const double* corners, int side
     int rc = or_bnd_faces(et, corners, side, &B);
     if (rc) return rc;
     const int dim = REF[et].dim, nsh = REF[et].nsh, nc = dim + 1, nl = nsh * nc;
-    const double nr = -1.0 * p->nu * p->rho;
     for (int f = 0; f < B.nbf; ++f) {
+        const double nr = -1.0 * (p->nu_bf ? p->nu_bf[f] : p->nu) * p->rho;  /* the import at the face's ip */
         const double* n = B.n[f];
         const int row = B.node[f];
         if (dl) {

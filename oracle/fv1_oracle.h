@@ -56,6 +56,10 @@ typedef struct {
     const double* nu_ip;
     const double* f_scv;
     const double* f_scvf;
+    /* viscosity imported at the boundary-face ips (NULL: nu): boundary
+       calls on a list of sides read [nb][4] (face j = the side's j-th
+       corner), element calls on one side read [4] */
+    const double* nu_bf;
 } or_params;
 
 /* FV1 geometry of one element (restated ugcore FV1Geometry). */

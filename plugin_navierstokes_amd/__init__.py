@@ -39,7 +39,8 @@ EXPORTS = [
     "nsfv1_set_kinematic_viscosity_ip", "nsfv1_set_source_ip", "nsfv1_ip_positions",
     "nsfv1_set_stokes", "nsfv1_set_laplace", "nsfv1_set_peclet_blend", "nsfv1_set_grad_div",
     "nsfv1_set_upwind", "nsfv1_set_stabilization", "nsfv1_set_stabilization_without",
-    "nsfv1_set_stabilization_upwind", "nsfv1_set_pac_upwind", "nsfv1_set_mesh", "nsfv1_set_mesh_mixed", "nsfv1_num_dofs", "nsfv1_nnz",
+    "nsfv1_set_stabilization_upwind", "nsfv1_set_pac_upwind", "nsfv1_set_mesh", "nsfv1_set_mesh_mixed", "nsfv1_bnd_ip_positions",
+    "nsfv1_set_boundary_viscosity", "nsfv1_num_dofs", "nsfv1_nnz",
     "nsfv1_get_pattern", "nsfv1_pattern_device", "nsfv1_assemble", "nsfv1_assemble_noalloc", "nsfv1_sync_errors",
     "nsfv1_set_launch",
     "nsfv1_set_dirichlet", "nsfv1_apply_dirichlet", "nsfv1_spmv", "nsfv1_linear_solve",
@@ -96,6 +97,8 @@ def load_library(path: str | None = None):
     L.nsfv1_set_stabilization_without.argtypes = [H]
     L.nsfv1_set_stabilization_upwind.argtypes = [H, C.c_char_p]
     L.nsfv1_set_mesh.argtypes = [H, C.c_int, vp, C.c_int64, vp, C.c_int64]
+    L.nsfv1_bnd_ip_positions.argtypes = [H, C.c_int, C.POINTER(dp), C.c_int64]
+    L.nsfv1_set_boundary_viscosity.argtypes = [H, C.c_int, C.POINTER(dp), C.c_int64]
     L.nsfv1_set_mesh_mixed.argtypes = [H, vp, C.c_int64, C.c_int, C.POINTER(C.c_int), C.POINTER(vp),
                                        C.POINTER(C.c_int64)]
     L.nsfv1_num_dofs.argtypes = [H, C.POINTER(C.c_int64)]
@@ -233,6 +236,8 @@ class NavierStokesFV1:
         if callable(fn):
             for c in getattr(self, "_mg_levels", []):
                 c.set_kinematic_viscosity(fn)
+            for kind in (0, 1):
+                self._apply_bnd_nu(kind)
 
     def _apply_f_fn(self):
         fn = getattr(self, "_f_fn", None)
@@ -316,6 +321,7 @@ class NavierStokesFV1:
         self._et = int(et)
         self._groups = [(self._et, self.ne)]
         self._mg_levels = []  # a new mesh drops the multigrid hierarchy
+        self._bnd_n = [0, 0, 0]  # and the boundary lists
         # functions of position are imports: evaluated again on the new mesh
         if callable(getattr(self, "_nu_fn", None)):
             self._apply_nu_fn()
@@ -339,6 +345,7 @@ class NavierStokesFV1:
         self._et = int(mesh.groups[0][0])
         self._groups = [(int(et), c.shape[0]) for et, c in mesh.groups]
         self._mg_levels = []  # a new mesh drops the multigrid hierarchy
+        self._bnd_n = [0, 0, 0]  # and the boundary lists
         # functions of position are imports: evaluated again on the new mesh
         if callable(getattr(self, "_nu_fn", None)):
             self._apply_nu_fn()
@@ -479,12 +486,45 @@ class NavierStokesFV1:
         e = np.ascontiguousarray(elem, dtype=np.int32)
         s = np.ascontiguousarray(side, dtype=np.int32)
         _check(_lib.nsfv1_set_outflow(self._h, e.ctypes.data, s.ctypes.data, len(e)))
+        self._bnd_count()[0] = len(e)
+        self._apply_bnd_nu(0)
 
     def set_symmetry(self, elem, side):
         """Symmetry condition (NavierStokesSymBCFV1) on the (element, local side) list."""
         e = np.ascontiguousarray(elem, dtype=np.int32)
         s = np.ascontiguousarray(side, dtype=np.int32)
         _check(_lib.nsfv1_set_symmetry(self._h, e.ctypes.data, s.ctypes.data, len(e)))
+        self._bnd_count()[1] = len(e)
+        self._apply_bnd_nu(1)
+
+    def _bnd_count(self):
+        if not hasattr(self, "_bnd_n"):
+            self._bnd_n = [0, 0, 0]
+        return self._bnd_n
+
+    def bnd_ip_positions(self, kind: int) -> np.ndarray:
+        """Boundary-face ips of list `kind` (0 outflow, 1 symmetry, 2 sliding
+        wall) in input order: [sides, 4, dim] (unused slots 0)."""
+        out = np.zeros((self._bnd_count()[kind], 4, self.dim))
+        _check(_lib.nsfv1_bnd_ip_positions(self._h, int(kind), out.ctypes.data_as(C.POINTER(C.c_double)), out.size))
+        return out
+
+    def set_boundary_viscosity(self, kind: int, values):
+        """The viscosity at the boundary-face ips of list `kind` ([sides, 4]);
+        None clears."""
+        if values is None:
+            _check(_lib.nsfv1_set_boundary_viscosity(self._h, int(kind), None, 0))
+            return
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        _check(_lib.nsfv1_set_boundary_viscosity(self._h, int(kind), v.ctypes.data_as(C.POINTER(C.c_double)), v.size))
+
+    def _apply_bnd_nu(self, kind: int):
+        """A viscosity function evaluated at the outflow / symmetry faces."""
+        fn = getattr(self, "_nu_fn", None)
+        if not callable(fn) or kind == 2 or self._bnd_count()[kind] == 0:
+            return
+        x = self.bnd_ip_positions(kind)
+        self.set_boundary_viscosity(kind, np.asarray(fn(x.reshape(-1, self.dim)), dtype=np.float64).reshape(x.shape[:2]))
 
     def set_sliding_wall(self, elem, side, sliding_factor: float, sliding_limit: float = 0.0):
         """Sliding wall (NavierStokesWSBCFV1) on the (element, local side) list."""
@@ -492,6 +532,7 @@ class NavierStokesFV1:
         s = np.ascontiguousarray(side, dtype=np.int32)
         _check(_lib.nsfv1_set_sliding_wall(self._h, e.ctypes.data, s.ctypes.data, len(e), float(sliding_factor),
                                            float(sliding_limit)))
+        self._bnd_count()[2] = len(e)
 
     def export(self, u, what: str, at: str = "scvf", local_ips=None, u_old=None, dt=0.0, out=None, stream=None,
                check=True):

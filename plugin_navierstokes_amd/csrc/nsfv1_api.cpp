@@ -99,6 +99,8 @@ struct MGLevel {
 struct BndList {
     DevBuf ptr, node, inc, side;
     int64_t nbn = 0;
+    std::vector<int32_t> src;  // per item: its face in the input list (pair b, side corner j) as 4 b + j
+    DevBuf nu;                 // per item: the viscosity at the face's ip (function-valued viscosity), or empty
 };
 
 // A further element type of a mixed mesh (the first type lives in the
@@ -150,6 +152,7 @@ struct nsfv1_handle {
     bool mixed() const { return !groups.empty(); }
     // boundary discs of the handle's own element group
     BndList bnd[BND_KINDS];
+    std::vector<int32_t> bnd_elem_in[BND_KINDS], bnd_side_in[BND_KINDS];  // the input (element, side) lists
     double slide_k = 0.0, slide_lim = 0.0;
     std::vector<int64_t> h_rowptr, h_node_inc;
     int max_threads = 256;
@@ -397,6 +400,11 @@ int nsfv1_set_kinematic_viscosity(nsfv1_handle* h, double nu)
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
     h->nu_ip.release();
+    // the constant replaces a function's values at the boundary faces too
+    for (int kind = 0; kind < BND_KINDS; ++kind) {
+        h->bnd[kind].nu.release();
+        for (auto& g : h->groups) g->bnd[kind].nu.release();
+    }
     h->nu = nu;
     h->nu_set = true;
     h->nu_global_set = true;
@@ -1083,8 +1091,17 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
         if (bnd && h->nu_elem.p)
             return fail(NSFV1_ERR_NOT_IMPLEMENTED,
                         "boundary discs with per-instance viscosity (batched instances) are not provided");
-        if (bnd && h->ip_imports())
-            return fail(NSFV1_ERR_NOT_IMPLEMENTED, "boundary discs with imports at the integration points are not provided");
+        // a function-valued viscosity reaches the outflow and symmetry discs
+        // through its values at their boundary faces
+        if (h->nu_ip.p)
+            for (int kind = 0; kind < BND_KINDS; ++kind) {
+                if (kind == BND_SLIDING) continue;  // the sliding wall has no viscous term
+                bool miss = h->bnd[kind].nbn > 0 && !h->bnd[kind].nu.p;
+                for (auto& g : h->groups) miss |= g->bnd[kind].nbn > 0 && !g->bnd[kind].nu.p;
+                if (miss)
+                    return fail(NSFV1_ERR_ARGUMENT, "a function-valued viscosity needs its values at the boundary faces "
+                                                    "of the outflow and symmetry discs (nsfv1_set_boundary_viscosity)");
+            }
     }
     Plan& P = jprec == NSFV1_F64 ? h->plan64 : h->plan32;
     if (!P.valid) {
@@ -1296,7 +1313,7 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
                            (own ? h->inc_elem : G->inc_elem).as<int32_t>(), (own ? h->inc_info : G->inc_info).as<int32_t>(),
                            (own ? h->inc_pos : G->inc_pos).as<uint64_t>(), B.ptr.as<int32_t>(), B.node.as<int64_t>(),
                            B.inc.as<int32_t>(), B.side.as<int32_t>(), B.nbn, h->nu, h->rho, s_a, h->laplace ? 1 : 0,
-                           h->stokes ? 1 : 0, kind, h->slide_k, h->slide_lim};
+                           h->stokes ? 1 : 0, kind, h->slide_k, h->slide_lim, B.nu.as<double>()};
             e = launch_outflow(own ? h->et : G->et, jprec, oa, u, jvals, defect, h->errflag.as<int>(), (hipStream_t)stream);
             if (e != hipSuccess) return fail(NSFV1_ERR_DEVICE, std::string("boundary disc launch: ") + hipGetErrorString(e));
         }
@@ -1372,7 +1389,13 @@ static int set_boundary_list(nsfv1_handle* h, int kind, const int32_t* elem, con
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
     if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
     h->bnd[kind].nbn = 0;
-    for (auto& g : h->groups) g->bnd[kind].nbn = 0;
+    h->bnd[kind].nu.release();
+    for (auto& g : h->groups) {
+        g->bnd[kind].nbn = 0;
+        g->bnd[kind].nu.release();
+    }
+    h->bnd_elem_in[kind].clear();
+    h->bnd_side_in[kind].clear();
     if (n == 0) return NSFV1_OK;
     if (n < 0 || !elem || !side) return fail(NSFV1_ERR_ARGUMENT, "bad boundary side arrays");
     // the element groups: type, element range, connectivity, incidences, list
@@ -1385,12 +1408,13 @@ static int set_boundary_list(nsfv1_handle* h, int kind, const int32_t* elem, con
         BndList* B;
         std::vector<int32_t> cn;  // connectivity, downloaded when first needed
         std::vector<std::pair<int64_t, std::pair<int32_t, int32_t>>> items;  // (node, (incidence, side))
+        std::vector<int32_t> src;                                           // 4 b + j of each item
     };
     std::vector<GV> gv;
-    gv.push_back({h->et, 0, h->ne, &h->conn, &h->h_node_inc, &h->h_inc_elem, &h->bnd[kind], {}, {}});
+    gv.push_back({h->et, 0, h->ne, &h->conn, &h->h_node_inc, &h->h_inc_elem, &h->bnd[kind], {}, {}, {}});
     int64_t e0 = h->ne;
     for (auto& g : h->groups) {
-        gv.push_back({g->et, e0, g->ne, &g->conn, &g->h_node_inc, &g->h_inc_elem, &g->bnd[kind], {}, {}});
+        gv.push_back({g->et, e0, g->ne, &g->conn, &g->h_node_inc, &g->h_inc_elem, &g->bnd[kind], {}, {}, {}});
         e0 += g->ne;
     }
     for (int64_t b = 0; b < n; ++b) {
@@ -1424,6 +1448,7 @@ static int set_boundary_list(nsfv1_handle* h, int kind, const int32_t* elem, con
                 if ((*G.inc_elem)[t] == e) { q = t; break; }
             if (q < 0) return fail(NSFV1_ERR_ARGUMENT, "boundary side corner without incidence");
             G.items.push_back({v, {(int32_t)q, (int32_t)s}});
+            G.src.push_back((int32_t)(4 * b + j));
         }
     }
     auto up = [](DevBuf& d, const void* src, size_t bytes) {
@@ -1431,16 +1456,22 @@ static int set_boundary_list(nsfv1_handle* h, int kind, const int32_t* elem, con
     };
     for (GV& G : gv) {
         if (G.items.empty()) continue;
-        std::stable_sort(G.items.begin(), G.items.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+        // stable order by node (input order within a node), the source of
+        // each item carried along
+        std::vector<size_t> ord(G.items.size());
+        for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return G.items[x].first < G.items[y].first; });
         std::vector<int64_t> nodes;
         std::vector<int32_t> ptr{0}, iq, is;
-        for (size_t i = 0; i < G.items.size(); ++i) {
-            if (nodes.empty() || nodes.back() != G.items[i].first) {
+        G.B->src.clear();
+        for (size_t o : ord) {
+            if (nodes.empty() || nodes.back() != G.items[o].first) {
                 if (!nodes.empty()) ptr.push_back((int32_t)iq.size());
-                nodes.push_back(G.items[i].first);
+                nodes.push_back(G.items[o].first);
             }
-            iq.push_back(G.items[i].second.first);
-            is.push_back(G.items[i].second.second);
+            iq.push_back(G.items[o].second.first);
+            is.push_back(G.items[o].second.second);
+            G.B->src.push_back(G.src[o]);
         }
         ptr.push_back((int32_t)iq.size());
         BndList& B = *G.B;
@@ -1450,6 +1481,8 @@ static int set_boundary_list(nsfv1_handle* h, int kind, const int32_t* elem, con
             return fail(NSFV1_ERR_DEVICE, "boundary list upload");
         B.nbn = (int64_t)nodes.size();
     }
+    h->bnd_elem_in[kind].assign(elem, elem + n);
+    h->bnd_side_in[kind].assign(side, side + n);
     return NSFV1_OK;
 }
 
@@ -1474,6 +1507,108 @@ int nsfv1_set_sliding_wall(nsfv1_handle* h, const int32_t* elem, const int32_t* 
     if (rc) return rc;
     h->slide_k = sliding_factor;
     h->slide_lim = sliding_limit;
+    return NSFV1_OK;
+}
+
+// Positions of the boundary-face ips of boundary list `kind`, in input
+// order: out[b][j][dim] for pair b and the side's j-th corner (j < 4; unused
+// slots of triangles and 2D sides are 0).  The FV1 boundary face of corner
+// c on a side: 2D [x_c, side midpoint], 3D [x_c, mid(c, next), side centre,
+// mid(prev, c)]; its ip is the mean of those points.
+int nsfv1_bnd_ip_positions(nsfv1_handle* h, int kind, double* out, int64_t n)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    if (kind < 0 || kind >= BND_KINDS) return fail(NSFV1_ERR_ARGUMENT, "kind: 0 outflow, 1 symmetry, 2 sliding wall");
+    const auto& E = h->bnd_elem_in[kind];
+    const auto& S = h->bnd_side_in[kind];
+    const int dim = h->dim;
+    if (!out || n != (int64_t)E.size() * 4 * dim)
+        return fail(NSFV1_ERR_ARGUMENT, "expected " + std::to_string(E.size() * 4 * dim) + " values");
+    std::fill(out, out + n, 0.0);
+    if (E.empty()) return NSFV1_OK;
+    std::vector<double> xyz((size_t)h->nv * dim);
+    if (hipMemcpy(xyz.data(), h->xyz.p, sizeof(double) * xyz.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(NSFV1_ERR_DEVICE, "mesh download");
+    // the groups' connectivities, downloaded when first needed
+    std::vector<std::vector<int32_t>> cn(1 + h->groups.size());
+    for (size_t b = 0; b < E.size(); ++b) {
+        int64_t e = E[b];
+        size_t g = 0;
+        int et = h->et;
+        int64_t ne = h->ne;
+        const DevBuf* cb = &h->conn;
+        while (e >= ne) {
+            e -= ne;
+            const ElemGroup& G = *h->groups[g++];
+            et = G.et;
+            ne = G.ne;
+            cb = &G.conn;
+        }
+        const int nsh = nsh_count(et);
+        if (cn[g].empty()) {
+            cn[g].resize((size_t)ne * nsh);
+            if (hipMemcpy(cn[g].data(), cb->p, sizeof(int32_t) * cn[g].size(), hipMemcpyDeviceToHost) != hipSuccess)
+                return fail(NSFV1_ERR_DEVICE, "connectivity download");
+        }
+        int k = 0, c[4];
+        switch (et) {
+        case 0: side_corners<0>(S[b], &k, c); break;
+        case 1: side_corners<1>(S[b], &k, c); break;
+        case 2: side_corners<2>(S[b], &k, c); break;
+        case 3: side_corners<3>(S[b], &k, c); break;
+        default: side_corners<4>(S[b], &k, c); break;
+        }
+        auto X = [&](int corner, int i) { return xyz[(size_t)cn[g][(size_t)e * nsh + corner] * dim + i]; };
+        double fc[3] = {0, 0, 0};
+        for (int q = 0; q < k; ++q)
+            for (int i = 0; i < dim; ++i) fc[i] += X(c[q], i) / k;
+        for (int q = 0; q < k; ++q) {
+            const int cc = c[q], nx = c[(q + 1) % k], pv = c[(q + k - 1) % k];
+            double* o = out + ((int64_t)b * 4 + q) * dim;
+            for (int i = 0; i < dim; ++i) {
+                double a = 0.0;
+                if (dim == 2) {
+                    const int other = q == 0 ? c[1] : c[0];
+                    a += X(cc, i);
+                    a += 0.5 * (X(cc, i) + X(other, i));
+                    o[i] = a / 2;
+                } else {
+                    a += X(cc, i);
+                    a += 0.5 * (X(cc, i) + X(nx, i));
+                    a += fc[i];
+                    a += 0.5 * (X(pv, i) + X(cc, i));
+                    o[i] = a / 4;
+                }
+            }
+        }
+    }
+    return NSFV1_OK;
+}
+
+// The viscosity at the boundary-face ips of list `kind` (a function-valued
+// viscosity evaluated at nsfv1_bnd_ip_positions): nu[b][j], 4 slots per
+// pair; NULL clears (the constant viscosity then applies).
+int nsfv1_set_boundary_viscosity(nsfv1_handle* h, int kind, const double* nu, int64_t n)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (kind < 0 || kind >= BND_KINDS) return fail(NSFV1_ERR_ARGUMENT, "kind: 0 outflow, 1 symmetry, 2 sliding wall");
+    std::vector<BndList*> lists{&h->bnd[kind]};
+    for (auto& g : h->groups) lists.push_back(&g->bnd[kind]);
+    if (!nu) {
+        for (BndList* B : lists) B->nu.release();
+        return NSFV1_OK;
+    }
+    if (n != (int64_t)h->bnd_elem_in[kind].size() * 4)
+        return fail(NSFV1_ERR_ARGUMENT, "expected 4 values per boundary side: " +
+                                            std::to_string(h->bnd_elem_in[kind].size() * 4));
+    for (BndList* B : lists) {
+        if (B->nbn == 0) continue;
+        std::vector<double> v(B->src.size());
+        for (size_t i = 0; i < v.size(); ++i) v[i] = nu[B->src[i]];
+        int rc = upload(B->nu, v.data(), (int64_t)v.size());
+        if (rc) return rc;
+    }
     return NSFV1_OK;
 }
 

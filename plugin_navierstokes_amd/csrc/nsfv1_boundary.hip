@@ -99,7 +99,7 @@ outflow_rows(OutflowArgs A, const double* __restrict__ u, JT* __restrict__ J, do
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= A.nbn) return;
     const int64_t v = A.bn_node[t];
-    const double nu = A.nu, rho = A.rho;
+    const double rho = A.rho;
     int64_t rows[NC];
 #pragma unroll
     for (int a = 0; a < NC; ++a) rows[a] = A.rowptr[v * NC + a];
@@ -108,6 +108,7 @@ outflow_rows(OutflowArgs A, const double* __restrict__ u, JT* __restrict__ J, do
     for (int a = 0; a < NC; ++a) dsum[a] = 0.0;
 
     for (int it = A.bn_ptr[t]; it < A.bn_ptr[t + 1]; ++it) {
+        const double nu = A.nu_item ? A.nu_item[it] : A.nu;  // at this face's ip
         const int q = A.item_inc[it];
         const int s = A.item_side[it];
         const int e = A.inc_elem[q];

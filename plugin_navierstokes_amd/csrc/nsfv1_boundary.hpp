@@ -24,6 +24,7 @@ struct OutflowArgs {
     int laplace, stokes;
     int kind;  // BND_OUTFLOW, BND_SYMMETRY or BND_SLIDING
     double slide_k, slide_lim;  // sliding wall: sliding factor and sliding limit
+    const double* nu_item;      // [items] viscosity at each face's ip (a function-valued viscosity), or null: nu
 };
 
 enum { BND_OUTFLOW = 0, BND_SYMMETRY = 1, BND_SLIDING = 2, BND_KINDS = 3 };

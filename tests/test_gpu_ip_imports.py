@@ -149,13 +149,16 @@ def test_refusals_and_clearing(cuda):
     disc.assemble(u, jprec=F64)
     assert disc.assembly_path_used() == 4
     disc.set_upwind("full")
-    # boundary discs: refused
+    # boundary discs take the function at their faces (the facade evaluates
+    # it there); without those values the assembly refuses
     belem, bside = M.boundary_sides(mesh, mesh.boundary["lid"])
     assert len(belem) > 0
     disc.set_outflow(belem, bside)
+    disc.assemble(u, jprec=F64)
+    disc.set_boundary_viscosity(0, None)
     with pytest.raises(NSFV1Error) as e:
         disc.assemble(u, jprec=F64)
-    assert e.value.name == "NOT_IMPLEMENTED"
+    assert e.value.name == "ARGUMENT"
     disc.set_outflow([], [])
     disc.assemble(u, jprec=F64)
     # constants clear the imports: the default path runs again
@@ -225,3 +228,62 @@ def test_multigrid_with_function_imports(cuda):
     disc.mg_setup(F64, u)
     disc.set_kinematic_viscosity(0.02)
     disc.mg_setup(F64, u)
+
+
+@pytest.mark.parametrize("jprec", [F64, F32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("kind", ["outflow", "symmetry"])
+@pytest.mark.parametrize("et", ETS + [M.PRISM], ids=lambda e: NAMES.get(e, "prism"))
+def test_function_viscosity_on_boundary_discs(cuda, et, kind, jprec):
+    """The viscosity function at the outflow / symmetry faces
+    (no_normal_stress_outflow_fv1.cpp:188-426, symmetric_boundary_fv1.cpp:
+    224-386 evaluate the master's viscosity there): the face ips against
+    the oracle's boundary faces, J and d against the oracle given the same
+    values at the element ips and at the faces."""
+    import torch
+
+    print(f"case et={et} boundary={kind} jprec={jprec}", flush=True)
+    mesh = small_mesh(et)
+    dim = mesh.dim
+    cfg = {"upwind": "full"}
+    disc = _disc(mesh, cfg)
+    disc.set_kinematic_viscosity(NU)
+    x = mesh.xyz[:, 0]
+    belem, bside = M.boundary_sides(mesh, np.nonzero(np.isclose(x, x.max()))[0])
+    (disc.set_outflow if kind == "outflow" else disc.set_symmetry)(belem, bside)
+    pos = disc.bnd_ip_positions(0 if kind == "outflow" else 1)
+    nu_bf = np.zeros((len(belem), 4))
+    for b, (e, sd) in enumerate(zip(belem, bside)):
+        ipg = O.bnd_faces(et, mesh.xyz[mesh.conn[e]], int(sd))["ipg"]
+        assert np.max(np.abs(pos[b, : len(ipg)] - ipg)) <= 1e-15
+        nu_bf[b, : len(ipg)] = NU(ipg)
+    u = M.random_state(mesh, 23)
+    J, d = disc.assemble(torch.from_numpy(u).to(cuda), s_a=0.8, jprec=jprec)
+    J, d = J.cpu().numpy().astype(np.float64), d.cpu().numpy()
+    rp, ci = disc.pattern()
+    p = oracle_params(dim, cfg)
+    xo = _oracle_ips(et, mesh)
+    p.nu_ip = NU(xo.reshape(-1, dim)).reshape(mesh.ne, -1)
+    Jr, dr = O.assemble(p, et, mesh.xyz, mesh.conn, u, s_a=0.8, rowptr=rp, colidx=ci)
+    Ja, da = O.assemble_abs(p, et, mesh.xyz, mesh.conn, u, s_a=0.8, rowptr=rp, colidx=ci)
+    p.nu_bf = nu_bf
+    Jb, db = O.outflow(p, et, mesh.xyz, mesh.conn, belem, bside, u, s_a=0.8, rowptr=rp, colidx=ci, kind=kind)
+    # the constant viscosity gives another answer, far above rounding (the
+    # test is not vacuous; the symmetry disc's pressure part does not depend
+    # on it)
+    q = oracle_params(dim, cfg)
+    Jc, _ = O.outflow(q, et, mesh.xyz, mesh.conn, belem, bside, u, s_a=0.8, rowptr=rp, colidx=ci, kind=kind)
+    assert np.max(np.abs(Jc - Jb)) > 1e-6 * np.max(np.abs(Jb))
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    rmax = np.zeros(len(rp) - 1)
+    np.maximum.at(rmax, rows, np.abs(Jb))
+    floor = 1e-15 if et == M.PRISM else 0.0
+    Jw, dw = Jr + Jb, dr + db
+    SJ = Ja + rmax[rows]
+    Sd = da + np.abs(db)
+    if jprec == F64:
+        tolJ = 1e-12 * np.abs(Jw) + 1e-13 * SJ + floor * np.abs(Jw).max()
+    else:
+        tolJ = 2.0 ** -17 * SJ + 2.0 ** -20 * np.abs(Jw) + floor * np.abs(Jw).max()
+    told = 1e-12 * np.abs(dw) + 1e-13 * Sd + floor * np.abs(dw).max()
+    assert np.max(np.abs(J - Jw) / (tolJ + 1e-300)) <= 1.0
+    assert np.max(np.abs(d - dw) / (told + 1e-300)) <= 1.0

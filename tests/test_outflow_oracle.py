@@ -278,3 +278,43 @@ def test_sliding_convective_term_is_rounding_level(et):
     J, d = O.outflow_elem(p, et, co, 0, u, "sliding", 0.0, 0.0)
     assert np.abs(d).max() < 1e-14 * np.abs(u).max() ** 2 * 4
     assert np.abs(J).max() < 1e-14 * np.abs(u).max() * 4
+
+
+@pytest.mark.parametrize("kind", ["outflow", "symmetry"])
+@pytest.mark.parametrize("et", ETS)
+def test_boundary_viscosity_import(et, kind):
+    """The viscosity imported at the boundary-face ips (a function-valued
+    viscosity of the master disc, evaluated there): a constant array equals
+    the scalar path bitwise; the side's defect and Jacobian are affine in
+    each face's value, and a face's indicator changes only the momentum
+    rows of its own node."""
+    co = _elem(et, 31)
+    nsh, nc, dim = O.NSH[et], O.DIM[et] + 1, O.DIM[et]
+    u = np.random.default_rng(4).normal(size=nsh * nc)
+    s = 1
+    nbf = len(M.SIDES[et][s])
+    base = dict(dim=dim, upwind="full", nu=0.03, rho=1.2)
+    J0, d0 = O.outflow_elem(O.Params(**base), et, co, s, u, kind)
+    J1, d1 = O.outflow_elem(O.Params(**base, nu_bf=np.full(4, 0.03)), et, co, s, u, kind)
+    assert np.array_equal(J0, J1) and np.array_equal(d0, d1)
+
+    def A(v):
+        return O.outflow_elem(O.Params(**dict(base, nu=1.0), nu_bf=v), et, co, s, u, kind)
+
+    Jz, dz = A(np.zeros(4))
+    node = O.bnd_faces(et, co, s)["node"]
+    dJ, dd = [], []
+    for f in range(nbf):
+        e = np.zeros(4)
+        e[f] = 1.0
+        J, d = A(e)
+        dJ.append(J - Jz)
+        dd.append(d - dz)
+        rows = np.zeros((nsh, nc), dtype=bool)
+        rows[node[f], :dim] = True
+        assert np.all(dd[-1][~rows.ravel()] == 0) and np.all(dJ[-1][~rows.ravel()] == 0), f
+        assert np.any(dd[-1][rows.ravel()] != 0), f
+    v = np.random.default_rng(5).uniform(0.01, 0.1, 4)
+    J, d = A(v)
+    np.testing.assert_allclose(d, dz + sum(v[f] * dd[f] for f in range(nbf)), rtol=1e-12, atol=1e-13 * np.abs(d).max())
+    np.testing.assert_allclose(J, Jz + sum(v[f] * dJ[f] for f in range(nbf)), rtol=1e-12, atol=1e-13 * np.abs(J).max())
