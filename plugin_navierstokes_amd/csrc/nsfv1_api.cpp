@@ -429,7 +429,12 @@ int nsfv1_set_kinematic_viscosity_ip(nsfv1_handle* h, const double* nu, int64_t 
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
     if (!nu) {
+        // the function's values at the boundary faces go with it
         h->nu_ip.release();
+        for (int kind = 0; kind < BND_KINDS; ++kind) {
+            h->bnd[kind].nu.release();
+            for (auto& g : h->groups) g->bnd[kind].nu.release();
+        }
         return NSFV1_OK;
     }
     if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
