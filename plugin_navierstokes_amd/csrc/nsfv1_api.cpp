@@ -937,10 +937,21 @@ static int make_plan(nsfv1_handle* h, const IncView& g, int jbytes, Plan& P)
     // 64-lane workgroup
     const bool rowlane = h->path == 2 && jbytes == 4 && h->max_threads == 64;
     const int64_t inc_budget = rowlane ? 64 / nc : h->max_threads;
+    // element groups of a mixed mesh: a run of nodes without incidences of
+    // the group is one empty range (its workgroup returns at once), so the
+    // other ranges hold only the group's nodes
+    const bool skip_empty = h->mixed();
+    auto empty = [&](int64_t u) { return g.node_inc[u + 1] == g.node_inc[u]; };
     while (v < h->nv) {
         const int64_t v0 = v;
         int rounds = 0;
-        while (v < h->nv) {
+        if (skip_empty && empty(v)) {
+            while (v < h->nv && empty(v)) ++v;
+            wn.push_back((int32_t)v);
+            wr.push_back(0);
+            continue;
+        }
+        while (v < h->nv && !(skip_empty && empty(v))) {
             const int64_t incs = g.node_inc[v + 1] - g.node_inc[v0];
             const int64_t vals = h->h_rowptr[(v + 1) * nc] - h->h_rowptr[v0 * nc];
             const size_t lds = ((size_t)vals * jbytes + (size_t)(v + 1 - v0) * 16 + 15) / 16 * 16 + (size_t)(v + 1 - v0) * nc * 8;
@@ -1213,7 +1224,14 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
                        : launch_assemble(et, jprec, mm, kk, u, u_old, jvals, defect, h->errflag.as<int>(), PP.nwg, PP.lds,
                                          (hipStream_t)stream);
         };
-        e = launch(h->et, m, k, P, h->ne);
+        // J and d start from zero; the first group writes the rows of its
+        // nodes, the others add to them
+        e = hipSuccess;
+        if (jvals) e = launch_zero_words(jvals, h->nnz * (jprec == NSFV1_F64 ? 2 : 1), (hipStream_t)stream);
+        if (e == hipSuccess && defect) e = launch_zero_words(defect, h->ndof * 2, (hipStream_t)stream);
+        KParams k0 = k;
+        k0.accum = 2;
+        if (e == hipSuccess) e = launch(h->et, m, k0, P, h->ne);
         // imports at the integration points run through the groups in order
         int64_t off_ip = h->ne * nip_of(h->et), off_sh = h->ne * nsh_count(h->et);
         for (size_t g = 0; e == hipSuccess && g < h->groups.size(); ++g) {

@@ -961,11 +961,12 @@ coupled_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const JT* __r
     double* dbuf = reinterpret_cast<double*>(smem + (((jlen + (v1 - v0) * PADE) * (int)sizeof(JT) + 15) & ~15));
     const bool wantJ = J != nullptr;
     const int i0 = M.node_inc[v0], i1 = M.node_inc[v1];
-    // a further element group of a mixed mesh adds to the rows the groups
-    // before it wrote (same stream, earlier launch); a workgroup none of
-    // whose nodes touch the group leaves its rows as they are
+    // element groups of a mixed mesh (J and d zeroed before the first): a
+    // workgroup none of whose nodes touch the group leaves its rows as they
+    // are; the first group writes its rows (accum 2), each further group
+    // adds to the rows the groups before it wrote (accum 1, same stream)
     if (P.accum && i0 == i1) return;
-    if (P.accum) {
+    if (P.accum == 1) {
         if (wantJ)
             for (int j = 0; j < v1 - v0; ++j) {
                 const int off = (int)(M.rowptr[(int64_t)(v0 + j) * NC] - jbase);

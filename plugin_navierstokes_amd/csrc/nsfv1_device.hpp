@@ -24,7 +24,8 @@ struct KParams {
     int stokes, laplace, peclet, has_source, has_old;
     int parts;
     int upw_mode;  // skewed-upwind decisions: 0 fresh, 1 fresh + record, 2 replay the record
-    int accum;     // fused rows: 0 assembled from zero, 1 added to what J and d hold (a further element group)
+    int accum;     // row gathers: 0 every row of the plan from zero; element groups of a mixed mesh
+                   // (empty node ranges skipped): 2 written from zero, 1 added to what J and d hold
     double exact, nu, rho, dt, inv_rho, inv_dt;
     double f[3];
     double s_a, s_m, s_f;

@@ -214,6 +214,14 @@ __global__ void fill_value(double* __restrict__ x, double v, int64_t n)
     if (t < n) x[t] = v;
 }
 
+// zeroes n 4-byte words (a kernel, not a memset node, so that a captured
+// assembly replays it faithfully)
+__global__ void zero_words(uint32_t* __restrict__ x, int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) x[t] = 0u;
+}
+
 // beta = sqrt(||r||^2); the squared norm is also copied to res2 if given
 __global__ void gm_beta(const double* __restrict__ n2, double* __restrict__ beta, double* __restrict__ res2)
 {
@@ -736,6 +744,13 @@ hipError_t launch_multi_add(const double* V, int64_t ld, int k, const double* c,
 hipError_t launch_axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t s)
 {
     axpby<<<blocks_for(n, 256), 256, 0, s>>>(a, x, b, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_words(void* x, int64_t nwords, hipStream_t s)
+{
+    if (nwords <= 0) return hipSuccess;
+    zero_words<<<blocks_for(nwords, 256), 256, 0, s>>>(static_cast<uint32_t*>(x), nwords);
     return hipGetLastError();
 }
 

@@ -31,6 +31,7 @@ hipError_t launch_axpby(double a, const double* x, double b, double* y, int64_t 
 // Hessenberg column j (m <= hs <= ...); y = x / *h (0 if *h = 0); Givens
 // least squares of the (m+1) x m Hessenberg H with right-hand side beta e1
 hipError_t launch_fill(double* x, double v, int64_t n, hipStream_t s);
+hipError_t launch_zero_words(void* x, int64_t nwords, hipStream_t s);
 hipError_t launch_gm_beta(const double* n2, double* beta, double* res2, hipStream_t s);
 hipError_t launch_gm_hcol(const double* hd, int hs, double* H, int m, int j, hipStream_t s);
 hipError_t launch_gm_scale(const double* x, const double* h, double* y, int64_t n, hipStream_t s);
