@@ -127,6 +127,21 @@ int nsfv1_set_exact_jacobian(nsfv1_handle* h, double factor);
    level no longer has. */
 int nsfv1_set_kinematic_viscosity_ip(nsfv1_handle* h, const double* nu, int64_t n);
 int nsfv1_set_source_ip(nsfv1_handle* h, const double* f_scv, int64_t n_scv, const double* f_scvf, int64_t n_scvf);
+/* Function-valued density: replaces set_density(SmartPtr<CplUserData>)
+   (navier_stokes_fv1.cpp:118-121), whose two imports the reference
+   evaluates at the SCVF ips (m_imDensitySCVF: fluxes, stabilisation,
+   navier_stokes_fv1.cpp:336-589, stabilization.cpp:229, 321) and at the SCV
+   ips (m_imDensitySCV: mass part and source, navier_stokes_fv1.cpp:805,
+   835, 866):
+     rho_scvf [ne][nip]  at the SCVF ips (ip = the element's edge number);
+     rho_scv  [ne][nsh]  at the SCV ips (the corners).
+   Both or neither (NULL, NULL clears; so does nsfv1_set_density and a new
+   mesh).  An SCVF value of zero or NaN is refused with NSFV1_ERR_ARGUMENT.
+   It runs where the other imports run (generic and coupled paths, mixed
+   meshes, multigrid with the coarse levels' own values, the stabilised
+   velocity export); boundary discs keep a constant density, so an assembly
+   with boundary discs and this import returns NSFV1_ERR_NOT_IMPLEMENTED. */
+int nsfv1_set_density_ip(nsfv1_handle* h, const double* rho_scvf, int64_t n_scvf, const double* rho_scv, int64_t n_scv);
 /* Global positions of the integration points (host array): kind 0 = SCVF
    ips [ne][nip][dim] (FV1Geometry: the mean of edge midpoint, element
    centre and, in 3D, the centres of the edge's two faces), kind 1 = SCV ips

@@ -46,7 +46,7 @@ class _Params(C.Structure):
         ("exact", C.c_double), ("nu", C.c_double), ("rho", C.c_double),
         ("has_source", C.c_int), ("f", C.c_double * 3), ("force_matrix", C.c_int),
         ("nu_ip", C.POINTER(C.c_double)), ("f_scv", C.POINTER(C.c_double)), ("f_scvf", C.POINTER(C.c_double)),
-        ("nu_bf", C.POINTER(C.c_double)),
+        ("nu_bf", C.POINTER(C.c_double)), ("rho_ip", C.POINTER(C.c_double)), ("rho_scv", C.POINTER(C.c_double)),
     ]
 
 
@@ -95,6 +95,10 @@ class Params:
     # viscosity at the boundary-face ips of boundary calls: [nb, 4] for a
     # list of sides (face j = the side's j-th corner), [4] for one side
     nu_bf: object = None
+    # density at the integration points: rho_ip [ne, nip] (SCVF ips),
+    # rho_scv [ne, nsh] (SCV ips, the corners)
+    rho_ip: object = None
+    rho_scv: object = None
 
     def c(self) -> _Params:
         p = _Params()
@@ -113,7 +117,7 @@ class Params:
             for i, v in enumerate(self.source):
                 p.f[i] = float(v)
         keep = []
-        for name in ("nu_ip", "f_scv", "f_scvf", "nu_bf"):
+        for name in ("nu_ip", "f_scv", "f_scvf", "nu_bf", "rho_ip", "rho_scv"):
             a = getattr(self, name)
             if a is not None:
                 a = np.ascontiguousarray(a, dtype=np.float64)

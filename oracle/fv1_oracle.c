@@ -40,6 +40,8 @@ const char* or_last_error(void) { return g_err; }
    source at the SCVF ips, the source at the SCV ips
    (navier_stokes_fv1.cpp:188-247; used at :296, 336, 351, 390, 855-868) */
 static double nu_at(const or_params* p, int ip) { return p->nu_ip ? p->nu_ip[ip] : p->nu; }
+static double rho_at(const or_params* p, int ip) { return p->rho_ip ? p->rho_ip[ip] : p->rho; }
+static double rho_sh(const or_params* p, int sh) { return p->rho_scv ? p->rho_scv[sh] : p->rho; }
 static double src_scvf(const or_params* p, int ip, int d, int dim) { return p->f_scvf ? p->f_scvf[ip * dim + d] : p->f[d]; }
 static double src_scv(const or_params* p, int sh, int d, int dim) { return p->f_scv ? p->f_scv[sh * dim + d] : p->f[d]; }
 
@@ -866,7 +868,7 @@ static int stab_update(const or_params* p, const or_geo* G, const double* uloc, 
                             rhs += sumVel2 * UL(d2, k);
                             S->Svel[ip][d][d2][k] = sumVel2 / diag;
                         }
-                    const double sumP = -1.0 * G->g[ip][k][d] / p->rho;
+                    const double sumP = -1.0 * G->g[ip][k][d] / rho_at(p, ip);
                     rhs += sumP * UL(dim, k);
                     S->Sp[ip][d][k] = sumP / diag;
                 }
@@ -896,7 +898,7 @@ static int stab_update(const or_params* p, const or_geo* G, const double* uloc, 
         for (int ip = 0; ip < nip; ++ip) {
             const double* sv = stdvel[ip];
             for (int k = 0; k < nsh; ++k) {
-                contP[k][ip] = -1.0 * G->g[ip][k][d] / p->rho;
+                contP[k][ip] = -1.0 * G->g[ip][k][d] / rho_at(p, ip);
                 contVel[d][k][ip] = vvisc[ip] * G->phi[ip][k];
                 contVel[d][k][ip] += vconv[ip] * up->U[ip][k];
                 if (flow) {
@@ -1048,10 +1050,9 @@ static void conv_upwind_vel(const or_geo* G, const double* uloc, const prolog* W
 static int jac_A(const or_params* p, const or_geo* G, const double* uloc, const prolog* W, double* Jl)
 {
     const int dim = G->dim, nip = G->nip, nsh = G->nsh, nc = dim + 1, P = dim, nl = nsh * nc;
-    const double rho = p->rho;
 #define JL(a, i, b, j) Jl[((i) * nc + (a)) * nl + (j) * nc + (b)]
     for (int ip = 0; ip < nip; ++ip) {
-        const double nu = nu_at(p, ip);
+        const double nu = nu_at(p, ip), rho = rho_at(p, ip);
         const int fr = G->from[ip], to = G->to[ip];
         const double* n = G->n[ip];
         for (int sh = 0; sh < nsh; ++sh) {
@@ -1163,10 +1164,9 @@ static int jac_A(const or_params* p, const or_geo* G, const double* uloc, const 
 static int def_A(const or_params* p, const or_geo* G, const double* uloc, const prolog* W, double* dl)
 {
     const int dim = G->dim, nip = G->nip, nsh = G->nsh, nc = dim + 1, P = dim;
-    const double rho = p->rho;
 #define DL(a, i) dl[(i) * nc + (a)]
     for (int ip = 0; ip < nip; ++ip) {
-        const double nu = nu_at(p, ip);
+        const double nu = nu_at(p, ip), rho = rho_at(p, ip);
         const int fr = G->from[ip], to = G->to[ip];
         const double* n = G->n[ip];
         double gv[3][3];
@@ -1218,13 +1218,12 @@ static void elem_A_abs(const or_params* p, const or_geo* G, const double* uloc, 
                        const prolog* W, double* Ja, double* da)
 {
     const int dim = G->dim, nip = G->nip, nsh = G->nsh, nc = dim + 1, P = dim, nl = nsh * nc;
-    const double rho = fabs(p->rho);
 #define JA(a, i, b, j) Ja[((i) * nc + (a)) * nl + (j) * nc + (b)]
 #define DA(a, i) da[(i) * nc + (a)]
 #define ADDJ(a, b, j, X) do { const double _x = fabs(X); if (Ja) { JA(a, fr, b, j) += _x; JA(a, to, b, j) += _x; } } while (0)
 #define ADDD(a, X) do { const double _x = fabs(X); if (da) { DA(a, fr) += _x; DA(a, to) += _x; } } while (0)
     for (int ip = 0; ip < nip; ++ip) {
-        const double nu = fabs(nu_at(p, ip));
+        const double nu = fabs(nu_at(p, ip)), rho = fabs(rho_at(p, ip));
         const int fr = G->from[ip], to = G->to[ip];
         double n[3] = {0, 0, 0};
         for (int i = 0; i < dim; ++i) n[i] = fabs(G->n[ip][i]);
@@ -1266,7 +1265,7 @@ static void elem_A_abs(const or_params* p, const or_geo* G, const double* uloc, 
                 }
                 for (int k = 0; k < nsh; ++k) {
                     a += fabs(W->S.Svel[ip][d][d][k] * diag * UL(d, k));
-                    a += fabs(G->g[ip][k][d] / p->rho * UL(P, k));
+                    a += fabs(G->g[ip][k][d] / rho_at(p, ip) * UL(P, k));
                 }
                 sta[d] = a / diag;
             }
@@ -1404,8 +1403,8 @@ int or_elem_M(const or_params* p, int et, const double* corners, const double* u
     const int dim = G.dim, nsh = G.nsh, nc = dim + 1, nl = nsh * nc;
     for (int sh = 0; sh < nsh; ++sh)
         for (int d1 = 0; d1 < dim; ++d1) {
-            if (Jl) Jl[(sh * nc + d1) * nl + sh * nc + d1] += G.vol[sh] * p->rho;
-            if (dl) dl[sh * nc + d1] += UL(d1, sh) * G.vol[sh] * p->rho;
+            if (Jl) Jl[(sh * nc + d1) * nl + sh * nc + d1] += G.vol[sh] * rho_sh(p, sh);
+            if (dl) dl[sh * nc + d1] += UL(d1, sh) * G.vol[sh] * rho_sh(p, sh);
         }
     return 0;
 }
@@ -1419,7 +1418,7 @@ int or_elem_rhs(const or_params* p, int et, const double* corners, double* fl)
     if (rc) return rc;
     const int dim = G.dim, nsh = G.nsh, nc = dim + 1;
     for (int sh = 0; sh < nsh; ++sh)
-        for (int d1 = 0; d1 < dim; ++d1) fl[sh * nc + d1] += src_scv(p, sh, d1, dim) * G.vol[sh] * p->rho;
+        for (int d1 = 0; d1 < dim; ++d1) fl[sh * nc + d1] += src_scv(p, sh, d1, dim) * G.vol[sh] * rho_sh(p, sh);
     return 0;
 }
 
@@ -1451,6 +1450,8 @@ static int assemble_elem(const or_params* p, int et, const double* xyz, const in
     /* this element's rows of the integration-point imports */
     or_params q = *p;
     if (q.nu_ip) q.nu_ip += e * nip;
+    if (q.rho_ip) q.rho_ip += e * nip;
+    if (q.rho_scv) q.rho_scv += e * nsh;
     if (q.f_scv) q.f_scv += e * nsh * dim;
     if (q.f_scvf) q.f_scvf += e * nip * dim;
     p = &q;
@@ -2169,7 +2170,15 @@ int or_export(const or_params* p, int et, const double* xyz, const int32_t* conn
                 if (uold) uo[k * nc + a] = uold[(int64_t)c[k] * nc + a];
             }
         }
-        int rc = or_export_elem(p, et, co, ul, uold ? uo : NULL, dt, what, at, lips, nip, out + e * npt * ncomp);
+        /* this element's rows of the integration-point imports */
+        or_params q = *p;
+        const int64_t sip = (int64_t)REF[et].nip;
+        if (q.nu_ip) q.nu_ip += e * sip;
+        if (q.rho_ip) q.rho_ip += e * sip;
+        if (q.rho_scv) q.rho_scv += e * nsh;
+        if (q.f_scv) q.f_scv += e * nsh * dim;
+        if (q.f_scvf) q.f_scvf += e * sip * dim;
+        int rc = or_export_elem(&q, et, co, ul, uold ? uo : NULL, dt, what, at, lips, nip, out + e * npt * ncomp);
         if (rc) return rc;
     }
     return 0;

@@ -60,6 +60,11 @@ typedef struct {
        calls on a list of sides read [nb][4] (face j = the side's j-th
        corner), element calls on one side read [4] */
     const double* nu_bf;
+    /* density imported at the integration points (NULL: rho), indexed like
+       nu_ip: rho_ip [ne][nip] at the SCVF ips, rho_scv [ne][nsh] at the SCV
+       ips (the corners) */
+    const double* rho_ip;
+    const double* rho_scv;
 } or_params;
 
 /* FV1 geometry of one element (restated ugcore FV1Geometry). */

@@ -36,7 +36,7 @@ ERRORS = {
 EXPORTS = [
     "nsfv1_last_error", "nsfv1_create", "nsfv1_destroy", "nsfv1_prepare_setting", "nsfv1_disc_type",
     "nsfv1_set_kinematic_viscosity", "nsfv1_set_source", "nsfv1_set_exact_jacobian", "nsfv1_set_density",
-    "nsfv1_set_kinematic_viscosity_ip", "nsfv1_set_source_ip", "nsfv1_ip_positions",
+    "nsfv1_set_kinematic_viscosity_ip", "nsfv1_set_source_ip", "nsfv1_set_density_ip", "nsfv1_ip_positions",
     "nsfv1_set_stokes", "nsfv1_set_laplace", "nsfv1_set_peclet_blend", "nsfv1_set_grad_div",
     "nsfv1_set_upwind", "nsfv1_set_stabilization", "nsfv1_set_stabilization_without",
     "nsfv1_set_stabilization_upwind", "nsfv1_set_pac_upwind", "nsfv1_set_mesh", "nsfv1_set_mesh_mixed", "nsfv1_bnd_ip_positions",
@@ -91,6 +91,7 @@ def load_library(path: str | None = None):
     L.nsfv1_set_source.argtypes = [H, C.POINTER(dp), C.c_int]
     L.nsfv1_set_kinematic_viscosity_ip.argtypes = [H, C.POINTER(dp), C.c_int64]
     L.nsfv1_set_source_ip.argtypes = [H, C.POINTER(dp), C.c_int64, C.POINTER(dp), C.c_int64]
+    L.nsfv1_set_density_ip.argtypes = [H, C.POINTER(dp), C.c_int64, C.POINTER(dp), C.c_int64]
     L.nsfv1_ip_positions.argtypes = [H, C.c_int, C.POINTER(dp), C.c_int64]
     L.nsfv1_set_upwind.argtypes = [H, C.c_char_p]
     L.nsfv1_set_stabilization.argtypes = [H, C.c_char_p, C.c_char_p]
@@ -264,8 +265,33 @@ class NavierStokesFV1:
         return buf.value.decode()
 
     # --- IncompressibleNavierStokesBase ---
-    def set_density(self, rho: float):
+    def set_density(self, rho):
+        """A number (constant user data), a function of position x [npts, dim]
+        -> [npts] evaluated at the SCVF and SCV integration points
+        (navier_stokes_fv1.cpp:118-121, 193-194, 244-245), or the pair of
+        values there ([ne, nip], [ne, nsh])."""
+        self._rho_fn = None
+        if callable(rho) or isinstance(rho, tuple):
+            self._rho_fn = rho
+            self._apply_rho_fn()
+            return
         _check(_lib.nsfv1_set_density(self._h, float(rho)))
+
+    def _apply_rho_fn(self):
+        fn = getattr(self, "_rho_fn", None)
+        if fn is None:
+            return
+        if isinstance(fn, tuple):
+            a, b = (np.ascontiguousarray(v, dtype=np.float64) for v in fn)
+        else:
+            xf, xs = self.ip_positions("scvf"), self.ip_positions("scv")
+            a = np.ascontiguousarray(np.asarray(fn(xf.reshape(-1, self.dim)), dtype=np.float64).reshape(xf.shape[:-1]))
+            b = np.ascontiguousarray(np.asarray(fn(xs.reshape(-1, self.dim)), dtype=np.float64).reshape(xs.shape[:-1]))
+        dp = C.POINTER(C.c_double)
+        _check(_lib.nsfv1_set_density_ip(self._h, a.ctypes.data_as(dp), a.size, b.ctypes.data_as(dp), b.size))
+        if callable(fn):
+            for c in getattr(self, "_mg_levels", []):
+                c.set_density(fn)
 
     def set_stokes(self, on: bool):
         _check(_lib.nsfv1_set_stokes(self._h, int(bool(on))))
@@ -327,6 +353,8 @@ class NavierStokesFV1:
             self._apply_nu_fn()
         if callable(getattr(self, "_f_fn", None)):
             self._apply_f_fn()
+        if callable(getattr(self, "_rho_fn", None)):
+            self._apply_rho_fn()
 
     def _set_mesh_mixed(self, mesh):
         """A mesh of several element types (meshes.MixedMesh): groups of
@@ -351,6 +379,8 @@ class NavierStokesFV1:
             self._apply_nu_fn()
         if callable(getattr(self, "_f_fn", None)):
             self._apply_f_fn()
+        if callable(getattr(self, "_rho_fn", None)):
+            self._apply_rho_fn()
 
     def num_dofs(self) -> int:
         n = C.c_int64()
@@ -611,6 +641,9 @@ class NavierStokesFV1:
             self.set_kinematic_viscosity(nu_fn)
         if callable(f_fn):
             self.set_source(f_fn)
+        rho_fn = getattr(fine, "_rho_fn", None)
+        if callable(rho_fn):
+            self.set_density(rho_fn)
 
     def mg_setup(self, jprec, u, u_old=None, dt=0.0, s_a=1.0, s_m=0.0, parts=PART_A, stream=None):
         _check(_lib.nsfv1_mg_setup(self._h, int(jprec), _ptr(u), _ptr(u_old), float(dt), float(s_a), float(s_m),

@@ -197,8 +197,8 @@ struct nsfv1_handle {
     DevBuf pred_part;  // learned predictor: partial sums of the per-step scale
     // imports at the integration points (function-valued viscosity and
     // source, evaluated by the caller): [ne][nip], [ne][nsh][dim], [ne][nip][dim]
-    DevBuf nu_ip, f_scv, f_scvf;
-    bool ip_imports() const { return nu_ip.p != nullptr || f_scv.p != nullptr; }
+    DevBuf nu_ip, f_scv, f_scvf, rho_ip, rho_scv;
+    bool ip_imports() const { return nu_ip.p != nullptr || f_scv.p != nullptr || rho_ip.p != nullptr; }
     int nt_m = 0;
     hipGraph_t nt_graph = nullptr;
     hipGraphExec_t nt_exec = nullptr;
@@ -448,6 +448,28 @@ int nsfv1_set_kinematic_viscosity_ip(nsfv1_handle* h, const double* nu, int64_t 
     return NSFV1_OK;
 }
 
+int nsfv1_set_density_ip(nsfv1_handle* h, const double* rho_scvf, int64_t n_scvf, const double* rho_scv, int64_t n_scv)
+{
+    if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    if (!rho_scvf && !rho_scv) {
+        h->rho_ip.release();
+        h->rho_scv.release();
+        return NSFV1_OK;
+    }
+    if (!rho_scvf || !rho_scv)
+        return fail(NSFV1_ERR_ARGUMENT, "the density import needs its values at both the SCVF and the SCV ips");
+    if (h->et < 0) return fail(NSFV1_ERR_NO_MESH, "no mesh set");
+    const int64_t a = group_points(h, 0), b = group_points(h, 1);
+    if (n_scvf != a || n_scv != b)
+        return fail(NSFV1_ERR_ARGUMENT, "density at the ips: expected ne * nip = " + std::to_string(a) +
+                                            " and ne * nsh = " + std::to_string(b) + " values");
+    for (int64_t i = 0; i < a; ++i)
+        if (!(rho_scvf[i] != 0.0)) return fail(NSFV1_ERR_ARGUMENT, "density at an SCVF ip is zero or NaN");
+    int rc = upload(h->rho_ip, rho_scvf, n_scvf);
+    if (!rc) rc = upload(h->rho_scv, rho_scv, n_scv);
+    return rc;
+}
+
 int nsfv1_set_source_ip(nsfv1_handle* h, const double* f_scv, int64_t n_scv, const double* f_scvf, int64_t n_scvf)
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
@@ -513,6 +535,8 @@ int nsfv1_set_exact_jacobian(nsfv1_handle* h, double factor)
 int nsfv1_set_density(nsfv1_handle* h, double rho)
 {
     if (!h) return fail(NSFV1_ERR_ARGUMENT, "NULL handle");
+    h->rho_ip.release();  // the constant replaces a function's values
+    h->rho_scv.release();
     h->rho = rho;
     return NSFV1_OK;
 }
@@ -844,6 +868,8 @@ static int set_mesh_groups(nsfv1_handle* h, const double* xyz, int64_t nv, const
     h->upw_recorded = false;
     h->nu_elem.release();
     h->nu_ip.release();
+    h->rho_ip.release();
+    h->rho_scv.release();
     if (h->f_scv.p) {
         h->f_scv.release();
         h->f_scvf.release();
@@ -1107,6 +1133,9 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
         if (bnd && h->nu_elem.p)
             return fail(NSFV1_ERR_NOT_IMPLEMENTED,
                         "boundary discs with per-instance viscosity (batched instances) are not provided");
+        if (bnd && h->rho_ip.p)
+            return fail(NSFV1_ERR_NOT_IMPLEMENTED,
+                        "boundary discs with a function-valued density are not provided (their density is a constant)");
         // a function-valued viscosity reaches the outflow and symmetry discs
         // through its values at their boundary faces
         if (h->nu_ip.p)
@@ -1183,7 +1212,8 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
                 h->inc_info.as<int32_t>(), P.wg_node.as<int32_t>(), P.wg_rounds.as<int32_t>(), h->l2inv.as<double>(),
                 h->vol.as<double>(), h->upw_dec.as<uint8_t>(), h->affine.as<uint8_t>(), h->inc_pos.as<uint64_t>(),
                 P.perm.as<int32_t>(), h->nu_elem.as<double>(), h->nu_ip.as<double>(), h->f_scv.as<double>(),
-                h->f_scvf.as<double>(), P.block, P.lane_a.as<int4>(), P.lane_pos.as<uint64_t>(), P.wg_a.as<int4>()};
+                h->f_scvf.as<double>(), h->rho_ip.as<double>(), h->rho_scv.as<double>(), P.block, P.lane_a.as<int4>(),
+                P.lane_pos.as<uint64_t>(), P.wg_a.as<int4>()};
     hipError_t e;
     int mfast = -1;
     if (h->path == 3 && h->march_ok && jprec == NSFV1_F32 && (parts & NSFV1_PART_A) && h->difflen == DL_RAW &&
@@ -1258,6 +1288,8 @@ static int assemble_impl(nsfv1_handle* h, const double* u, const double* u_old, 
             mg.nu_ip = h->nu_ip.p ? h->nu_ip.as<double>() + off_ip : nullptr;
             mg.f_scv = h->f_scv.p ? h->f_scv.as<double>() + off_sh * h->dim : nullptr;
             mg.f_scvf = h->f_scvf.p ? h->f_scvf.as<double>() + off_ip * h->dim : nullptr;
+            mg.rho_ip = h->rho_ip.p ? h->rho_ip.as<double>() + off_ip : nullptr;
+            mg.rho_scv = h->rho_scv.p ? h->rho_scv.as<double>() + off_sh : nullptr;
             off_ip += G.ne * nip_of(G.et);
             off_sh += G.ne * nsh_count(G.et);
             KParams kg = k;
@@ -1700,6 +1732,7 @@ int nsfv1_export(nsfv1_handle* h, int what, int at, const double* local_ips, int
         m.nu_ip = h->nu_ip.as<double>();
         m.f_scv = h->f_scv.as<double>();
         m.f_scvf = h->f_scvf.as<double>();
+        m.rho_ip = h->rho_ip.as<double>();
         if (coupled_config(h)) {
             k.conv_upwind = h->conv_upwind;
             k.pac = h->pac;
@@ -1723,6 +1756,7 @@ int nsfv1_export(nsfv1_handle* h, int what, int at, const double* local_ips, int
             mg.nu_ip = h->nu_ip.p ? h->nu_ip.as<double>() + off_ip : nullptr;
             mg.f_scv = h->f_scv.p ? h->f_scv.as<double>() + off_sh * dim : nullptr;
             mg.f_scvf = h->f_scvf.p ? h->f_scvf.as<double>() + off_ip * dim : nullptr;
+            mg.rho_ip = h->rho_ip.p ? h->rho_ip.as<double>() + off_ip : nullptr;
             e = launch(G.et, mg, G.ne, out + off_ip * dim);
             off_ip += G.ne * nip_of(G.et);
             off_sh += G.ne * nsh_count(G.et);
@@ -1883,6 +1917,10 @@ static void mg_sync_settings(nsfv1_handle* h)
         // imports at the integration points live per level (each level has
         // its own elements); a fine level without them clears the coarse ones
         if (!h->nu_ip.p) c->nu_ip.release();
+        if (!h->rho_ip.p) {
+            c->rho_ip.release();
+            c->rho_scv.release();
+        }
         if (!h->f_scv.p) {
             c->f_scv.release();
             c->f_scvf.release();
@@ -1985,10 +2023,11 @@ int nsfv1_mg_setup(nsfv1_handle* h, int jprec, const double* u, const double* u_
     mg_sync_settings(h);
     for (auto& Lp : h->mg) {
         const nsfv1_handle* c = Lp->h;
-        if ((h->nu_ip.p && !c->nu_ip.p) || (h->f_scv.p && !c->f_scv.p))
+        if ((h->nu_ip.p && !c->nu_ip.p) || (h->f_scv.p && !c->f_scv.p) || (h->rho_ip.p && !c->rho_ip.p))
             return fail(NSFV1_ERR_ARGUMENT, "multigrid with imports at the integration points: every coarse level "
                                             "needs its own imports of the same quantities (nsfv1_set_kinematic_"
-                                            "viscosity_ip / nsfv1_set_source_ip on the coarse handles)");
+                                            "viscosity_ip / nsfv1_set_source_ip / nsfv1_set_density_ip on the "
+                                            "coarse handles)");
     }
     if (!u) return fail(NSFV1_ERR_ARGUMENT, "u is NULL");
     if (jprec != NSFV1_F32 && jprec != NSFV1_F64) return fail(NSFV1_ERR_ARGUMENT, "jprec must be 32 or 64");

@@ -213,6 +213,7 @@ struct Cfg {
     // source at the SCVF ips, indexed by the original SCVF (= edge) number
     const double* nu_ip = nullptr;
     const double* f_scvf = nullptr;
+    const double* rho_ip = nullptr;  // [nip] density at the SCVF ips, or null
     __device__ bool stokes() const { return FAST >= 0 ? false : P.stokes != 0; }
     __device__ bool pac() const { return FAST >= 0 ? false : P.pac != 0; }
     __device__ bool peclet() const { return FAST >= 0 ? false : P.peclet != 0; }
@@ -426,12 +427,16 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
     constexpr int OT = T::edge[IP][0] == 0 ? T::edge[IP][1] : T::edge[IP][0];
     const KParams& P = C.P;
     double nu = C.nu;
-    const double rho = P.rho;
+    double rho = P.rho, inv_rho = P.inv_rho;
     const double* fip = nullptr;  // source at this SCVF's ip (import), or the constant
-    if (C.nu_ip || C.f_scvf) {
+    if (C.nu_ip || C.f_scvf || C.rho_ip) {
         const int oip = edge_between<ET>(pk[0], pk[OT]);
         if (C.nu_ip) nu = C.nu_ip[oip];
         if (C.f_scvf) fip = C.f_scvf + oip * D;
+        if (C.rho_ip) {
+            rho = C.rho_ip[oip];
+            inv_rho = 1.0 / rho;
+        }
     }
 
     double ipg[D], n[D], me[D];
@@ -546,7 +551,7 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
                 for (int sh = 0; sh < N; ++sh) o += TabH<ET>::t.phi[IP][sh] * uold[(int64_t)ccn[sh] * (D + 1) + d];
                 rhs += o * P.inv_dt;
             }
-            rhs += vv * sv[d] + vc * upS[d] - gv[D][d] * P.inv_rho;
+            rhs += vv * sv[d] + vc * upS[d] - gv[D][d] * inv_rho;
             svel[d] = rhs * idg;
         }
     } else {
@@ -603,7 +608,7 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
     R.pw1 = JT(prod * (1.0 - wpe));
     R.vv = JT(vv * idg);
     R.vc = JT(vc * idg);
-    R.sp = JT(-P.inv_rho * idg);
+    R.sp = JT(-inv_rho * idg);
     R.w = JT(wpe);
 }
 
@@ -611,14 +616,17 @@ __device__ inline void scvf_canon(const Cfg<FAST>& C, const double (&co)[Topo<ET
 // navier_stokes_fv1.cpp:317-593
 template <int ET, int FAST, int Q, typename JT>
 __device__ inline void scvf_jac(const Cfg<FAST>& C, const Rec<ET, JT>& R, Lane<ET, JT>& L,
-                                const JT (*G)[Topo<ET>::dim] = nullptr)
+                                const JT (*G)[Topo<ET>::dim] = nullptr, int kv = 0)
 {
     using T = Topo<ET>;
     constexpr int D = T::dim, N = T::nsh;
     constexpr int IP = corner_ip<ET>(0, Q);
     const KParams& P = C.P;
     const bool fields = C.fields();
-    const JT s_ = JT(P.s_a), rh = JT(P.rho), nr = R.nr, ex = JT(P.exact * P.rho);
+    // density at the SCVF ip: the import, read again here so that the
+    // records carry no extra register, or the constant
+    const double rd = C.rho_ip ? C.rho_ip[CanonH<ET>::c.oip[kv][Q]] : P.rho;
+    const JT s_ = JT(P.s_a), rh = JT(rd), nr = R.nr, ex = JT(P.exact * rd);
 #pragma unroll
     for (int sh = 0; sh < N; ++sh) {
         JT g[D];
@@ -723,11 +731,11 @@ __device__ inline void scvf_all(const Cfg<FAST>& C, const double (&co)[Topo<ET>:
 }
 
 template <int ET, int FAST, int Q, typename JT>
-__device__ inline void jac_all(const Cfg<FAST>& C, const Rec<ET, JT>* R, Lane<ET, JT>& L)
+__device__ inline void jac_all(const Cfg<FAST>& C, const Rec<ET, JT>* R, Lane<ET, JT>& L, int kv)
 {
     if constexpr (Q < Topo<ET>::cip) {
-        scvf_jac<ET, FAST, Q, JT>(C, R[Q], L);
-        jac_all<ET, FAST, Q + 1, JT>(C, R, L);
+        scvf_jac<ET, FAST, Q, JT>(C, R[Q], L, nullptr, kv);
+        jac_all<ET, FAST, Q + 1, JT>(C, R, L, kv);
     }
 }
 
@@ -824,17 +832,19 @@ assemble_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const double
             elem_centre<ET>(co, c);
             Rec<ET, JT> R[T::cip];
             const Cfg<FAST> C{P, M.nu_elem ? M.nu_elem[e] : P.nu, M.nu_ip ? M.nu_ip + (int64_t)e * NIP : nullptr,
-                              M.f_scvf ? M.f_scvf + (int64_t)e * NIP * D : nullptr};
+                              M.f_scvf ? M.f_scvf + (int64_t)e * NIP * D : nullptr,
+                              M.rho_ip ? M.rho_ip + (int64_t)e * NIP : nullptr};
             scvf_all<ET, FAST, 0, JT>(C, co, M.xyz, M.conn + (int64_t)e * N, ul, pk, ccn, c, uold, M.l2inv + (int64_t)e * NIP,
                                       (P.upw_mode != 0 && M.upw_dec) ? M.upw_dec + (int64_t)e * NIP : nullptr, kv, wantJ, L, R,
                                       okg, oku, (ET == 0 || ET == 2) || (M.affine != nullptr && M.affine[e] != 0));
-            if (wantJ) jac_all<ET, FAST, 0, JT>(C, R, L);
+            if (wantJ) jac_all<ET, FAST, 0, JT>(C, R, L, kv);
         }
         // mass part and source: navier_stokes_fv1.cpp:781-869
         if (P.parts & 6) {
             const double volk = M.vol[(int64_t)e * N + kv];
+            const double rk = M.rho_scv ? M.rho_scv[(int64_t)e * N + kv] : P.rho;  // density at the SCV ip
             if (P.parts & 2) {
-                const double m = volk * P.rho;
+                const double m = volk * rk;
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
                     if (wantJ) L.jacc[d][0][d] += JT(P.s_m * m);
@@ -845,7 +855,7 @@ assemble_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const double
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
                     const double fk = M.f_scv ? M.f_scv[((int64_t)e * N + kv) * D + d] : P.f[d];
-                    L.dacc[d] += P.s_f * (fk * volk * P.rho);
+                    L.dacc[d] += P.s_f * (fk * volk * rk);
                 }
         }
         if (!oku) kernel_error(err, KERR_CUT_SIDE);
@@ -1738,7 +1748,8 @@ export_stab_vel(KMeshPtrs M, KParams P, const double* __restrict__ u, const doub
     elem_centre<ET>(co, c);
     bool okg = true, oku = true;
     const Cfg<-1> C{P, M.nu_elem ? M.nu_elem[e] : P.nu, M.nu_ip ? M.nu_ip + e * NIP : nullptr,
-                    M.f_scvf ? M.f_scvf + e * NIP * D : nullptr};
+                    M.f_scvf ? M.f_scvf + e * NIP * D : nullptr,
+                        M.rho_ip ? M.rho_ip + e * NIP : nullptr};
     stab_vel_all<ET, 0>(C, co, M.xyz, M.conn + e * N, ul, pk, ccn, c, uold, M.l2inv + e * NIP, kv,
                         out + e * NIP * D, okg, oku, (ET == 0 || ET == 2) || (M.affine != nullptr && M.affine[e] != 0));
     if (!oku) kernel_error(err, KERR_CUT_SIDE);

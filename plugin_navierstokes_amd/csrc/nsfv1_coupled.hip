@@ -536,7 +536,7 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
     // ---- stabilised velocity of the lane's ip.  Its shapes (the derivatives
     //      by each corner's unknowns) are formed corner by corner in the
     //      Jacobian loop below, from the same expressions.
-    const double rho = P.rho;
+    const double rho = M.rho_ip ? M.rho_ip[e * NIP + ip] : P.rho;  // density at the lane's ip
     double svel[D], diag = 1.0;
     if (P.stab_type == STAB_WITHOUT) {  // stabilization.cpp:805-850
 #pragma unroll
@@ -615,7 +615,7 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
                 if (flow) cd0 += S.vd[j] * (S.Ud[j][k] - S.Us[j][k]);
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
-                    const double cp = -1.0 * gj[d] / rho;
+                    const double cp = -1.0 * gj[d] / (M.rho_ip ? M.rho_ip[e * NIP + j] : rho);
                     double cd = cd0;
                     if (flow) {
 #pragma unroll
@@ -1028,8 +1028,9 @@ coupled_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const JT* __r
         // mass part and source: navier_stokes_fv1.cpp:781-869
         if (P.parts & 6) {
             const double volk = M.vol[e * N + kv];
+            const double rk = M.rho_scv ? M.rho_scv[e * N + kv] : P.rho;  // density at the SCV ip
             if (P.parts & 2) {
-                const double m = volk * P.rho;
+                const double m = volk * rk;
                 const double* uv = u + (int64_t)v * NC;
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
@@ -1043,7 +1044,7 @@ coupled_rows(KMeshPtrs M, KParams P, const double* __restrict__ u, const JT* __r
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
                     const double fk = M.f_scv ? M.f_scv[((int64_t)e * N + kv) * D + d] : P.f[d];
-                    dacc[d] += P.s_f * (fk * volk * P.rho);
+                    dacc[d] += P.s_f * (fk * volk * rk);
                 }
         }
         const uint64_t pw = M.inc_pos[inc];

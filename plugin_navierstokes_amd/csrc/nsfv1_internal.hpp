@@ -31,6 +31,8 @@ struct KMeshPtrs {
     const double* nu_ip;      // [ne][nip] kinematic viscosity at the SCVF ips (function-valued import), or null
     const double* f_scv;      // [ne][nsh][dim] source at the SCV ips (the corners), or null
     const double* f_scvf;     // [ne][nip][dim] source at the SCVF ips (stabilisation), or null
+    const double* rho_ip;     // [ne][nip] density at the SCVF ips (function-valued import), or null
+    const double* rho_scv;    // [ne][nsh] density at the SCV ips (the corners), or null
     int block;                // threads per workgroup of the launch (64..256)
     const int4* lane_a;       // [ninc] in lane order: {element, node, corner | slot << 8 | row length << 16, LDS row base}
     const uint64_t* lane_pos; // [ninc] in lane order: inc_pos of the lane's incidence
