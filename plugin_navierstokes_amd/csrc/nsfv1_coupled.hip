@@ -804,7 +804,7 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
                 if (flow) cd0 += S.vd[j] * (S.Ud[j][sh] - S.Us[j][sh]);
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
-                    const double cp = -1.0 * gj[d] / rho;
+                    const double cp = -1.0 * gj[d] / (M.rho_ip ? M.rho_ip[e * NIP + j] : rho);
                     double cd = cd0;
                     double cvo[D];
                     if (flow) {
