@@ -304,7 +304,9 @@ __device__ inline bool upwind_lane(int type, double sign, int ip, const double (
     return true;
 }
 
-template <int ET, typename JT>
+// RHOIP: the density is imported at the ips (M.rho_ip); a separate
+// instance, so that the constant-density kernel keeps its registers
+template <int ET, typename JT, bool RHOIP>
 __global__ void __launch_bounds__(64)
 coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double* __restrict__ uold, int64_t ne,
              JT* __restrict__ recJ, double* __restrict__ recD, double* __restrict__ vel_out, int* __restrict__ err)
@@ -536,7 +538,7 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
     // ---- stabilised velocity of the lane's ip.  Its shapes (the derivatives
     //      by each corner's unknowns) are formed corner by corner in the
     //      Jacobian loop below, from the same expressions.
-    const double rho = M.rho_ip ? M.rho_ip[e * NIP + ip] : P.rho;  // density at the lane's ip
+    const double rho = RHOIP ? M.rho_ip[e * NIP + ip] : P.rho;  // density at the lane's ip
     double svel[D], diag = 1.0;
     if (P.stab_type == STAB_WITHOUT) {  // stabilization.cpp:805-850
 #pragma unroll
@@ -615,7 +617,7 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
                 if (flow) cd0 += S.vd[j] * (S.Ud[j][k] - S.Us[j][k]);
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
-                    const double cp = -1.0 * gj[d] / (M.rho_ip ? M.rho_ip[e * NIP + j] : rho);
+                    const double cp = -1.0 * gj[d] / (RHOIP ? M.rho_ip[e * NIP + j] : rho);
                     double cd = cd0;
                     if (flow) {
 #pragma unroll
@@ -804,7 +806,7 @@ coupled_scvf(KMeshPtrs M, KParams P, const double* __restrict__ u, const double*
                 if (flow) cd0 += S.vd[j] * (S.Ud[j][sh] - S.Us[j][sh]);
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
-                    const double cp = -1.0 * gj[d] / (M.rho_ip ? M.rho_ip[e * NIP + j] : rho);
+                    const double cp = -1.0 * gj[d] / (RHOIP ? M.rho_ip[e * NIP + j] : rho);
                     double cd = cd0;
                     double cvo[D];
                     if (flow) {
@@ -1113,7 +1115,12 @@ static hipError_t launch_coupled_et(const KMeshPtrs& m, const KParams& p, const 
         const int64_t jb = ne * NIP * N * NC * NC * (int64_t)sizeof(JT);
         recD = reinterpret_cast<double*>(static_cast<char*>(recs) + ((jb + 255) / 256) * 256);
     }
-    if (blocks > 0) coupled_scvf<ET, JT><<<blocks, 64, 0, s>>>(m, p, u, uold, ne, recJ, recD, vel_out, err);
+    if (blocks > 0) {
+        if (m.rho_ip)
+            coupled_scvf<ET, JT, true><<<blocks, 64, 0, s>>>(m, p, u, uold, ne, recJ, recD, vel_out, err);
+        else
+            coupled_scvf<ET, JT, false><<<blocks, 64, 0, s>>>(m, p, u, uold, ne, recJ, recD, vel_out, err);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || vel_out) return e;
     coupled_rows<ET, JT><<<nwg, m.block, lds, s>>>(m, p, u, recJ, recD, J, d);
